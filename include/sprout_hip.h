@@ -1,0 +1,149 @@
+/*
+ * sprout_hip.h — C ABI of the MI355X-native Sprout fluid step (libsprout_hip.so).
+ *
+ * This is the drop-in boundary for the path that Sprout's Rust host drives through
+ * `FluidSim` (src/sim/mod.rs:862-3492): a handle owns one tank's device state, the caller
+ * hands over particles, solids and parameters, records substeps, and downloads results.
+ * Every entry point returns an int status (0 = ok, < 0 = SP_E_* code); the text of the last
+ * error of a handle is available through sp_last_error(). No C++ exception or abort crosses
+ * this boundary. Calls on one handle are not thread-safe (the reference takes `&mut self`
+ * everywhere); different handles may be used from different threads.
+ *
+ * All types here are plain C: the layouts of sp_particle and the grid fields follow the
+ * reference byte for byte (static_asserts in sprout_types.h).
+ */
+#ifndef SPROUT_HIP_H
+#define SPROUT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sprout_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define SP_OK 0
+#define SP_E_INVALID (-1)      /* bad argument (null pointer, size, NaN parameter ...) */
+#define SP_E_DEVICE (-2)       /* a HIP runtime call failed */
+#define SP_E_NOMEM (-3)        /* device or host allocation failed */
+#define SP_E_UNSUPPORTED (-4)  /* a feature of the reference this build does not carry */
+#define SP_E_CAPACITY (-5)     /* more particles / materials than the handle holds */
+
+typedef struct sp_sim sp_sim;
+
+/* Creation parameters: the grid and the particle capacity of `FluidSim::new(gpu, grid,
+ * capacity)` (src/sim/mod.rs:1034), plus the HIP device ordinal. */
+typedef struct sp_config {
+    int32_t grid_w, grid_h;  /* cells; at least 8 x 8 */
+    uint32_t capacity;       /* particle slots (the reference sizes W*H*PPC, src/world.rs:155) */
+    int32_t device;          /* HIP device ordinal */
+} sp_config;
+
+/* Replaces `FluidSim::new` (src/sim/mod.rs:1034). The handle owns all device memory. */
+int sp_create(const sp_config* cfg, sp_sim** out);
+/* Frees the handle and its device memory (Drop of FluidSim). NULL is accepted. */
+void sp_destroy(sp_sim* sim);
+/* Text of the last error on this handle ("" if none). Valid until the next call. */
+const char* sp_last_error(const sp_sim* sim);
+/* Text of the last error of sp_create (no handle exists then). */
+const char* sp_create_error(void);
+
+/* Replaces assigning `FluidSim::params` (src/sim/mod.rs:862-1031, field `params`): SI
+ * parameters and the material table. The host side derives the grid-unit tables from
+ * them at the next record (gpu_params / gpu_materials, src/sim/mod.rs:2179-2330).
+ * NaN values are refused (src/sim/params.rs:986-997). */
+int sp_set_params(sp_sim* sim, const sp_sim_params* params);
+
+/* Replaces `FluidSim::set_solids(SolidField)` (src/sim/mod.rs:1977, src/scene.rs:110-122):
+ * sdf[(W+1)*(H+1)] node distances, solid[4*W*H] (open u face, open v face, smoothed solid
+ * occupancy, open volume), region[W*H] (region number | SP_REGION_VENT). temp may be NULL
+ * (insulating solids); a non-NULL temp is refused (heat is not carried by this build). */
+int sp_set_solids(sp_sim* sim, const float* sdf, const float* solid, const uint32_t* region,
+                  const float* temp);
+
+/* Replaces `FluidSim::reset(Vec<Particle>)` (src/sim/mod.rs:1982-2001): ids are assigned
+ * 0..n in order, the spawn fields are settled, state that warm-starts solves is cleared. */
+int sp_reset(sp_sim* sim, const sp_particle* particles, uint32_t n);
+
+/* Replaces `FluidSim::spawn` (src/sim/mod.rs:2027-2048): queues particles for the next
+ * record; *accepted receives how many were taken. */
+int sp_spawn(sp_sim* sim, const sp_particle* particles, uint32_t n, uint32_t* accepted);
+
+/* Replaces `FluidSim::record(builder, substeps, target, tool)` (src/sim/mod.rs:2400-2563):
+ * enqueues `substeps` substeps meant to cover `target_s` seconds on the handle's stream and
+ * returns without waiting. tool may be NULL (idle); only mode 0 is carried. */
+int sp_record(sp_sim* sim, uint32_t substeps, float target_s, const sp_tool_state* tool);
+/* Waits for everything recorded on the handle (submit_and_wait, src/sim/mod.rs:3483-3491). */
+int sp_wait(sp_sim* sim);
+/* Replaces `FluidSim::poll_readbacks` + `solver` (src/sim/mod.rs:3033-3116, :334-363):
+ * copies the solver statistics of the batches finished so far. Never blocks. */
+int sp_poll(sp_sim* sim, sp_solver_stats* out);
+
+/* `FluidSim::advance_blocking(seconds)` (src/sim/mod.rs:3284-3312). */
+int sp_advance_blocking(sp_sim* sim, double seconds);
+/* `FluidSim::step_blocking(n)` (src/sim/mod.rs:3316-3338): n ticks of 1/240 s. */
+int sp_step_blocking(sp_sim* sim, uint32_t n);
+/* `FluidSim::sim_time` (src/sim/mod.rs:1956). */
+int sp_sim_time(sp_sim* sim, double* out);
+
+/* `FluidSim::download_particles` (src/sim/mod.rs:3421-3432): the live particles (dead
+ * slots dropped), in slot order. out may be NULL to query the count; otherwise at most
+ * cap are written and *n receives the number written (or available if out is NULL). */
+int sp_download_particles(sp_sim* sim, sp_particle* out, uint32_t cap, uint32_t* n);
+/* `FluidSim::download_grid` (src/sim/mod.rs:3436-3450) without the rigid labels:
+ * vel[4*W*H] (u, v, state u, state v) and cell[4*W*H] (occupancy, density, viscosity,
+ * level). Either pointer may be NULL. */
+int sp_download_grid(sp_sim* sim, float* vel, float* cell);
+/* `FluidSim::download_pressure` (src/sim/mod.rs:3464-3475): p*dt per cell, and dt. */
+int sp_download_pressure(sp_sim* sim, float* pres, float* dt);
+/* `FluidSim::cell_counts(lo, hi)` (src/sim/mod.rs:3354-3386, count.comp): live particles
+ * per cell in [x0, x0+w) x [y0, y0+h), row by row, into out[w*h]. */
+int sp_cell_counts(sp_sim* sim, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out);
+/* `FluidSim::request_stats/poll_stats` (src/sim/mod.rs:3246-3274, stats.comp), blocking:
+ * live particles, and per table row the count and the concentration sum. */
+int sp_stats(sp_sim* sim, sp_stats_out* out);
+
+/* The periodic counting sort of the particles by cell (record_sort, src/sim/mod.rs:
+ * 3223-3242, sort.comp). Runs on the handle's stream; the order within a cell is stable
+ * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
+int sp_sort(sp_sim* sim);
+
+/* ---- host-side derivations (no device needed) ---------------------------------------- */
+
+/* Default SI parameters and the built-in materials this build carries (water, oil, honey,
+ * mercury, syrup) — SimParams::default (src/sim/params.rs:386-460). */
+void sp_default_params(sp_sim_params* out);
+
+/* Tank::layout (src/sim/mod.rs:271-331) + gpu_params / gpu_materials (src/sim/mod.rs:
+ * 2179-2330) for an isothermal tank holding the table rows `rows[0..n_rows)` (ascending):
+ * fills the grid-unit parameter block and the material table (SP_MAX_MATERIALS rows). */
+int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h,
+                          const int32_t* rows, int32_t n_rows, sp_gparams* out,
+                          sp_gmat* mats);
+
+/* Levels::new (src/sim/mod.rs:727-753): the multigrid hierarchy of a W x H grid. */
+int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out);
+
+/* ---- stage-level access for parity tests and diagnostics ----------------------------- */
+
+/* Copy an internal device field to / from host memory (SP_FIELD_* ids, sizes in bytes).
+ * These synchronise the handle's stream. */
+int sp_debug_download(sp_sim* sim, int32_t field, void* out, size_t bytes);
+int sp_debug_upload(sp_sim* sim, int32_t field, const void* data, size_t bytes);
+/* Runs one stage (SP_STAGE_*) of the substep on the current device state, synchronously. */
+int sp_debug_stage(sp_sim* sim, int32_t stage, int32_t arg);
+/* The grid-unit tables the handle currently uses. */
+int sp_debug_gparams(sp_sim* sim, sp_gparams* out, sp_gmat* mats);
+/* Device time of the last recorded batch's phases (ms): [0] total, [1] particle pass,
+ * [2] grid passes before the solve, [3] pressure solve, [4] projection; and the pressure
+ * iterations of the last substep in [5]. Valid after sp_wait. */
+int sp_timings(sp_sim* sim, float* out6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,290 @@
+/*
+ * sprout_types.h — plain-data types of the Sprout fluid step shared by the C ABI, the HIP
+ * kernels and the CPU oracle.
+ *
+ * Units follow the reference (shaders/common.glsl:3-9): lengths in grid cells, time in
+ * seconds, densities relative to water. Grid layout is the MAC layout of the reference:
+ * u on the left face (i, j+0.5), v on the bottom face (i+0.5, j), scalars at the centre,
+ * row-major cell index c = y*W + x.
+ */
+#ifndef SPROUT_TYPES_H
+#define SPROUT_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_GROUPS 4          /* material.rs: GROUPS */
+#define SP_MAX_PRESENT 12    /* material.rs: MAX_PRESENT */
+#define SP_MAX_MATERIALS 96  /* material.rs: MAX_MATERIALS */
+#define SP_PPC 4             /* params.rs:11 PPC */
+#define SP_ACC_MIN 8         /* mod.rs:68 ACC_MIN */
+#define SP_ACC_MAX 32        /* mod.rs:69 ACC_MAX */
+#define SP_ACC_FIRST 5       /* mod.rs:71 ACC_FIRST */
+#define SP_REGION_MASK 31u   /* common.glsl:220 */
+#define SP_REGION_VENT 256u  /* common.glsl:221 */
+#define SP_SPAWN_MAX 32768   /* mod.rs:63 SPAWN_MAX */
+
+/* Particle kinds (params.rs:21-31). */
+#define SP_KIND_DEAD 0u
+#define SP_KIND_FRESH 0x10000u
+#define SP_KIND_REAL_T 0x20000u
+#define SP_KIND_PUFF 0x40000u
+#define SP_KIND_SRC 0xff00u
+
+/* Material rows carried by this build, at the reference's table rows
+ * (src/sim/material.rs:35-96). */
+#define SP_ROW_WATER 0
+#define SP_ROW_HONEY 1
+#define SP_ROW_OIL 2
+#define SP_ROW_MERCURY 8
+#define SP_BUILTIN_COUNT 59  /* presets.rs:1130-1135: REAL_COUNT + SIMPLE_COUNT + 3 */
+
+/* `struct Particle` (common.glsl:11-19, params.rs:37-54): 48 bytes, std430. */
+typedef struct sp_particle {
+    float pos[2];   /* cells */
+    float vel[2];   /* cells / s */
+    float c[4];     /* APIC affine matrix du/dx, du/dy, dv/dx, dv/dy (1/s) */
+    float s;        /* concentration */
+    uint32_t kind;  /* 0 dead, else table row + 1 in the low byte, flags above */
+    uint32_t id;    /* permanent identity */
+    float t;        /* enthalpy temperature, deg C */
+} sp_particle;
+
+/* A material row in SI units: the fields of `MatProps` (src/sim/material.rs:159-340) that
+ * the isothermal step reads. */
+typedef struct sp_material {
+    int32_t group;         /* phase group 0 aqueous, 1 oily, 2 metal, 3 silicate */
+    int32_t diluent;       /* row of the diluent, -1 none */
+    float density;         /* kg/m^3 */
+    float density_solid;   /* kg/m^3 when frozen, 0 = as liquid */
+    float viscosity;       /* kinematic, m^2/s, at temp_ref */
+    int32_t temp_model;    /* 0 none, 1 Arrhenius, 2 VFT (material.rs TEMP_*) */
+    float temp_b, temp_c;  /* temperature law (K) */
+    float temp_ref;        /* deg C */
+    float tension;         /* against air, N/m */
+    float contact_angle;   /* degrees */
+    float mix_rate;        /* 1/s */
+    float expansion;       /* 1/K */
+    float melt_lo, melt_hi;/* deg C (-1e9: never freezes) */
+    float visc_exponent;   /* viscosity blend exponent against the diluent (visc_exponent) */
+    float load;            /* emitted concentration of a concentrate */
+    float spawn_temp;      /* deg C */
+    float latent;          /* J/kg */
+    float heat_capacity;   /* J/(kg K) */
+} sp_material;
+
+/* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
+typedef struct sp_sim_params {
+    float cell_size;            /* m */
+    float gravity;              /* m/s^2 */
+    float ambient_temp;         /* deg C */
+    float rigid_viscosity;      /* m^2/s */
+    float pair_tension[SP_GROUPS][SP_GROUPS];  /* N/m */
+    float droplet_cells;
+    float cfl;
+    uint32_t advection_order;
+    float dt_max;
+    float fixed_hz;
+    float pressure_tol;
+    uint32_t pressure_max_iters;
+    float correction_tol;
+    float viscosity_tol;
+    uint32_t viscosity_max_iters;
+    float implicit_viscosity_min;
+    float spray_cfl;
+    float speed_cap;
+    float correction_time;
+    float correction_max;
+    float correction_deadband;
+    float correction_step_max;
+    float crowd_jitter;
+    float tool_crowd_max;
+    float apic_clamp;
+    float curvature_max;
+    uint32_t slip_cap;
+    uint32_t sealed_vent;
+    float min_substep_share;
+    int32_t mat_count;
+    sp_material mat[SP_MAX_MATERIALS];
+} sp_sim_params;
+
+/* `ToolState` (params.rs:1141-1150). */
+typedef struct sp_tool_state {
+    float pos[2];
+    float vel[2];
+    float radius;
+    int32_t mode;
+    float strength;
+} sp_tool_state;
+
+/* `SolverStats` (src/sim/mod.rs:334-353). */
+typedef struct sp_solver_stats {
+    uint64_t substeps;
+    uint64_t pressure_unconverged;
+    uint64_t viscosity_unconverged;
+    uint32_t pressure_iters_max;
+    uint32_t viscosity_iters_max;
+    float pressure_residual_max;
+    float viscosity_residual_max;
+    float dt, dt_cfl, umax, umax_grid, umax_part;
+    uint32_t pressure_iters_last;   /* iterations of the latest solve (this build) */
+} sp_solver_stats;
+
+/* `Stats` (src/sim/mod.rs:128-143). */
+typedef struct sp_stats_out {
+    uint32_t live;
+    uint32_t count[SP_MAX_MATERIALS];
+    float conc[SP_MAX_MATERIALS];
+} sp_stats_out;
+
+/* One material row in grid units (`GpuMat`, material.rs:508-547, filled at mod.rs:
+ * 2179-2236), the fields the isothermal step reads. */
+typedef struct sp_gmat {
+    int32_t group, diluent, chan_w, chan_s;   /* ia */
+    int32_t fam, rheo, temp_model, residue;   /* ib */
+    float rho, ln_nu, sigma, mix_rate;        /* a: relative density, ln nu (cells^2/s), tension/rho_w, 1/s */
+    float cos_c, sin_c, grain, rho_solid;     /* b */
+    float visc_exp, expansion, load, pad;     /* misc */
+} sp_gmat;
+
+/* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
+ * 2238-2330): the fields the isothermal step reads, plus the solve set-up. */
+typedef struct sp_gparams {
+    int32_t W, H;
+    uint32_t n_slots, spawn_count;
+    float dt_fixed, gravity, inv_ppc, apic_clamp;
+    float pair[SP_GROUPS][SP_GROUPS];
+    float kappa_max, sealed_vent, cell_size, rigid_friction;   /* misc */
+    int32_t n_present, acc_stride, n_fam;
+    uint32_t group_mask;
+    uint32_t caps;
+    int32_t primary;
+    uint32_t group_heat;
+    int32_t visc_mode;             /* 0 none, 1 explicit, 2 implicit (viscosity_mode) */
+    int32_t present[SP_MAX_PRESENT];
+    int32_t fams[SP_MAX_PRESENT][2];
+    float sep_tau, cfl, tool_crowd_max, density_k;
+    float dt_max, crowd_jitter, ext_speed, wall_margin;
+    float tol[4];    /* pressure, correction, viscosity, correction time */
+    float ext[4];    /* speed cap (cells/s), advection order, spray cfl, currents */
+    float corr[4];   /* correction step max, dead band, slip cap, implicit viscosity min */
+    int32_t p_max_iters, v_max_iters;
+    float heat_x;    /* ambient temperature (P.heat.x) */
+    float ln_rigid;  /* ln rigid viscosity, cells^2/s (P.heat2.x) */
+} sp_gparams;
+
+/* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
+typedef struct sp_stepbuf {
+    float frame_left;
+    float advanced;
+    uint32_t batch_steps;
+    uint32_t batch_done;
+    float dt;
+    float dt_prev;
+    uint32_t umax_grid;  /* float bits */
+    uint32_t umax_part;  /* float bits */
+    float dt_cfl;
+    uint32_t substeps;
+    float umax_seen;
+    uint32_t pad0;       /* float bits: Kahan compensation of `advanced` */
+} sp_stepbuf;
+
+/* Krylov state of the pressure solve (`SolveBuf`, common.glsl:138-149), with the scalar
+ * reductions in FP64. */
+typedef struct sp_solve {
+    double rz[2], alpha[2], beta[2], bb[2], rr[2];
+    uint32_t done[2], iters[2];
+    uint32_t it_max, unconv, solves, arrive;
+    float res_max;
+    uint32_t iters_last;
+    uint32_t pad[2];
+} sp_solve;
+
+#define SP_MAX_LEVELS 16
+/* The multigrid hierarchy (`Levels`, src/sim/mod.rs:715-784). */
+typedef struct sp_levels_info {
+    int32_t n;                   /* levels */
+    int32_t tail;                /* first level of the single-dispatch tail */
+    int32_t total;               /* cells of all levels */
+    int32_t tiles;               /* tile flags of levels 0..tail */
+    int32_t dims[SP_MAX_LEVELS][2];
+    int32_t offs[SP_MAX_LEVELS];
+    int32_t tile_offs[SP_MAX_LEVELS];
+} sp_levels_info;
+
+/* Constants of the solver (common.glsl:455-466, mod.rs:100-119). */
+#define SP_MASS_SCALE 1048576.0f
+#define SP_MOM_SCALE 4194304.0f
+#define SP_MIN_FACE_MASS 64
+#define SP_THETA_MIN 0.1f
+#define SP_REST_PHI 1.0f
+#define SP_LIQUID_THR 0.5f
+#define SP_FACE_EMPTY 1e30f
+#define SP_FACE_CLOSED (-1e30f)
+#define SP_MG_OMEGA 1.8f
+#define SP_COARSEST_CELLS 4
+#define SP_TAIL_CELLS 1500
+#define SP_TAIL_SHARED 3328
+#define SP_TAIL_SHARED_COARSE 1024
+#define SP_TAIL_HALF_SWEEPS 9
+#define SP_TAIL_WARP_CELLS 128
+#define SP_SORT_INTERVAL 128
+#define SP_SORT_MIN_PARTICLES 16384
+
+/* Internal fields (sp_debug_download / sp_debug_upload). Sizes: cells N = W*H. */
+enum {
+    SP_FIELD_PARTICLES = 0,  /* sp_particle[n_slots] */
+    SP_FIELD_ACC = 1,        /* int32[N*stride] */
+    SP_FIELD_VEL = 2,        /* float4[N] */
+    SP_FIELD_CELL = 3,       /* float4[N] */
+    SP_FIELD_COMP = 4,       /* float4[N] */
+    SP_FIELD_COEF = 5,       /* float4[N] */
+    SP_FIELD_SURF = 6,       /* float4[N] */
+    SP_FIELD_PJUMP = 7,      /* float2[N] */
+    SP_FIELD_MGA = 8,        /* float4[total] */
+    SP_FIELD_MGD = 9,        /* float[total] */
+    SP_FIELD_PRHS = 10,      /* float4[N] */
+    SP_FIELD_PRES = 11,      /* float2[N] */
+    SP_FIELD_STEP = 12,      /* sp_stepbuf */
+    SP_FIELD_SOLVE = 13,     /* sp_solve */
+    SP_FIELD_SPROP = 14,     /* float4[N] */
+    SP_FIELD_AUX = 15,       /* float4[N] */
+    SP_FIELD_GRHO = 16,      /* float4[N] */
+    SP_FIELD_VEL0 = 17,      /* float2[N] */
+    SP_FIELD_MGZ = 18,       /* float2[total] */
+    SP_FIELD_MGR = 19,       /* float2[total] */
+    SP_FIELD_PTILE = 20,     /* uint32[tiles] */
+    SP_FIELD_REGION_AIR = 21,/* uint32 */
+    SP_FIELD_SURF2 = 22,     /* float4[N] */
+    SP_FIELD_VW = 23,        /* float4[N] */
+    SP_FIELD_VS = 24,        /* float4[N] */
+    SP_FIELD_GBASE = 25,     /* float4[N] */
+    SP_FIELD_COUNT
+};
+
+/* Stages of one substep (record_substep, src/sim/mod.rs:2615-2708), for sp_debug_stage. */
+enum {
+    SP_STAGE_STEP = 0,       /* step.comp */
+    SP_STAGE_CLEAR_ACC = 1,  /* fill acc */
+    SP_STAGE_PARTICLES = 2,  /* particles.comp; arg = gather flag */
+    SP_STAGE_GRID_PREP = 3,
+    SP_STAGE_COEF = 4,
+    SP_STAGE_SURF = 5,       /* surf_smooth + surf_curv + surf_apply */
+    SP_STAGE_VISC = 6,       /* explicit viscosity (visc_setup + visc_apply mode 2) */
+    SP_STAGE_RHS = 7,        /* fill ptile + rhs.comp */
+    SP_STAGE_PRESSURE = 8,   /* record_pressure (first solve) */
+    SP_STAGE_PROJECT = 9,
+    SP_STAGE_SUBSTEP = 10,   /* the whole substep; arg = gather flag */
+    SP_STAGE_MG_BUILD = 11,  /* the Galerkin hierarchy only */
+    SP_STAGE_VCYCLE = 12,    /* one V-cycle: mgr level 0 -> mgz level 0 */
+    SP_STAGE_COUNT
+};
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

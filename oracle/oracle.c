@@ -1,0 +1,1873 @@
+/*
+ * oracle.c — CPU restatement of the Sprout fluid step (the reference's GLSL compute
+ * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
+ * times the CPU baseline; nothing in the product calls it.
+ *
+ * Scope: the isothermal substep without the CAP-gated passes (rigid, rheology, magnets,
+ * elasticity, settling, steam, reactions), tools, currents or film spatter:
+ *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
+ *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
+ *   visc_apply.comp mode 2 (explicit viscosity), rhs.comp, mg_build/down/up/tail.comp,
+ *   cg_scale/init/apply/update.comp with the reduce_tail.glsl scalar steps, project.comp,
+ *   sort.comp, count.comp, stats.comp.
+ * Every float operation follows the shader's order (compile with -ffp-contract=off), so
+ * that the integer scatter sums and cell indices can be compared bit for bit. The CG
+ * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
+ * fixed tree (reduce_tail.glsl:28-41).
+ *
+ * Built by oracle/Makefile into oracle/_ref/liboracle.so (gcc, no GPU).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NO_REDUCE 0xffffffffu
+
+typedef struct { float x, y; } v2;
+typedef struct { float x, y, z, w; } v4;
+
+struct or_sim {
+    int W, H, N;
+    uint32_t cap, n_slots;
+    sp_gparams P;
+    sp_gmat mats[SP_MAX_MATERIALS];
+    sp_particle* part;
+    int32_t* acc;
+    v4 *vel, *cell, *comp, *coef, *surf, *surf2, *sprop, *aux, *grho, *gbase, *solid, *prhs;
+    v4 *vw, *vs;
+    v2 *vel0, *pjump, *pres;
+    float *sdf, *mixb, *p2keep;
+    uint32_t* region;
+    uint32_t region_air;
+    sp_levels_info lv;
+    v4* mga;
+    float* mgd;
+    v2 *mgx, *mgz, *mgr, *cgp, *cgq;
+    uint32_t* ptile;
+    sp_stepbuf S;
+    sp_solve K;
+};
+
+/* ---- small helpers with the shader's float semantics ---------------------------------- */
+
+static float fmaxf_(float a, float b) { return a > b ? a : b; }
+static float fminf_(float a, float b) { return a < b ? a : b; }
+static float clampf(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }
+static int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+/* GLSL mix(x, y, a) = x (1 - a) + y a. */
+static float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+static float len2(float x, float y) { return sqrtf(x * x + y * y); }
+static float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+static float smoothstepf(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static int face_live(float s) { return fabsf(s) < 1e29f; }
+static int face_closed(float s) { return s < -1e29f; }
+static float v4c(v4 a, int k) { return k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w; }
+
+/* hash (air.glsl:4-11) */
+static uint32_t hash_u(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+/* bspline (common.glsl:638-643) */
+static void bspline(float f, float w[3]) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+}
+
+static int cidx(const or_sim* s, int x, int y) { return y * s->W + x; }
+
+/* row_density (common.glsl:475-479) */
+static float row_density(const or_sim* s, int row, float sc) {
+    int d = s->mats[row].diluent;
+    float rho = s->mats[row].rho;
+    return d >= 0 ? mixf(s->mats[d].rho, rho, sc) : rho;
+}
+
+/* terminal (common.glsl:370-374) */
+static float terminal(const or_sim* s, float drho, float rho_c, float nu_c, float d) {
+    float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * s->P.gravity * d;
+    return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
+}
+
+/* group_sorts (common.glsl:590-602) */
+static int group_sorts(const or_sim* s, int g) {
+    if (g <= 0) return 0;
+    int n = 0;
+    for (int j = 0; j < s->P.n_present; j++) {
+        int r = s->P.present[j];
+        const sp_gmat* m = &s->mats[r];
+        if (m->diluent >= 0 && s->mats[m->diluent].group == g) return 0;
+        if (m->group != g) continue;
+        if (m->diluent >= 0) return 0;
+        n++;
+    }
+    return n >= 2;
+}
+
+/* sample_sdf (air.glsl:13-22) */
+static float sample_sdf(const or_sim* s, float px, float py, float* gx, float* gy) {
+    float x = clampf(px, 0.0f, (float)s->W - 0.001f), y = clampf(py, 0.0f, (float)s->H - 0.001f);
+    int ix = (int)x, iy = (int)y;
+    float fx = x - (float)ix, fy = y - (float)iy;
+    int w1 = s->W + 1;
+    int k = iy * w1 + ix;
+    float d00 = s->sdf[k], d10 = s->sdf[k + 1], d01 = s->sdf[k + w1], d11 = s->sdf[k + w1 + 1];
+    *gx = mixf(d10 - d00, d11 - d01, fy);
+    *gy = mixf(d01 - d00, d11 - d10, fx);
+    return mixf(mixf(d00, d10, fx), mixf(d01, d11, fx), fy);
+}
+
+/* sample_vel (air.glsl:41-78) */
+static v2 sample_vel(const or_sim* s, float px, float py, v2 fb) {
+    v2 o = fb;
+    {
+        float gx = px, gy = py - 0.5f;
+        int bx = (int)floorf(gx), by = (int)floorf(gy);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        float sw = 0.0f, su = 0.0f;
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+                v4 q = s->vel[cidx(s, bx + a, by + b)];
+                if (face_live(q.z)) {
+                    float wt = (a == 0 ? 1.0f - fx : fx) * (b == 0 ? 1.0f - fy : fy);
+                    sw += wt;
+                    su += wt * q.x;
+                }
+            }
+        if (sw > 1e-3f) o.x = su / sw;
+    }
+    {
+        float gx = px - 0.5f, gy = py;
+        int bx = (int)floorf(gx), by = (int)floorf(gy);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        float sw = 0.0f, su = 0.0f;
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+                v4 q = s->vel[cidx(s, bx + a, by + b)];
+                if (face_live(q.w)) {
+                    float wt = (a == 0 ? 1.0f - fx : fx) * (b == 0 ? 1.0f - fy : fy);
+                    sw += wt;
+                    su += wt * q.y;
+                }
+            }
+        if (sw > 1e-3f) o.y = su / sw;
+    }
+    return o;
+}
+
+/* collide (particles.comp:51-65) */
+static int collide(const or_sim* s, float* x, float* y, float* vx, float* vy, int beyond) {
+    float nx, ny;
+    float d = sample_sdf(s, *x, *y, &nx, &ny);
+    if (d < s->P.wall_margin) {
+        float l = len2(nx, ny);
+        if (l > 1e-5f) {
+            nx /= l;
+            ny /= l;
+            float k = s->P.wall_margin - d + (beyond ? fmaxf_(-d, 0.0f) : 0.0f);
+            *x += nx * k;
+            *y += ny * k;
+            float vn = *vx * nx + *vy * ny;
+            if (vn < 0.0f) {
+                *vx -= vn * nx;
+                *vy -= vn * ny;
+            }
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* ---- step.comp (step.comp:22-50) ------------------------------------------------------ */
+
+static void stage_step(or_sim* s) {
+    sp_stepbuf* S = &s->S;
+    const sp_gparams* P = &s->P;
+    float u = fmaxf_(fminf_(u2f(S->umax_grid), P->ext[0]), u2f(S->umax_part));
+    u = fmaxf_(u, P->ext_speed);
+    float dt_cfl = P->cfl / (u + sqrtf(5.0f * fmaxf_(P->gravity, 0.0f)) + 1e-6f);
+    float dt;
+    if (P->dt_fixed > 0.0f) {
+        dt = P->dt_fixed;
+    } else {
+        uint32_t left = S->batch_steps > S->batch_done ? S->batch_steps - S->batch_done : 1u;
+        dt = fminf_(fminf_(dt_cfl, P->dt_max), fmaxf_(S->frame_left, 0.0f) / (float)left);
+    }
+    dt = fmaxf_(dt, 1e-6f);
+    S->dt_prev = S->dt > 0.0f ? S->dt : dt;
+    S->dt = dt;
+    S->dt_cfl = fminf_(dt_cfl, P->dt_max);
+    S->umax_seen = u;
+    S->frame_left -= dt;
+    float lo = u2f(S->pad0);
+    float y = dt - lo;
+    float t = S->advanced + y;
+    S->pad0 = f2u((t - S->advanced) - y);
+    S->advanced = t;
+    S->batch_done += 1u;
+    S->substeps += 1u;
+    S->umax_grid = 0u;
+    S->umax_part = 0u;
+    s->region_air = 0u;
+}
+
+/* ---- particles.comp ------------------------------------------------------------------- */
+
+typedef struct {
+    int live;
+    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc;
+    int cw, cs;
+} scatter_in;
+
+static void acc_add(or_sim* s, int x, int y, int ch, int v) {
+    s->acc[(size_t)cidx(s, x, y) * s->P.acc_stride + ch] += v;
+}
+
+/* deposit (particles.comp:1006-1062) */
+static void deposit(or_sim* s, const scatter_in* g) {
+    float w[2][3];
+    {
+        float gx = g->px, gy = g->py - 0.5f;
+        int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline(fx, w[0]);
+        bspline(fy, w[1]);
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                float m = w[0][a] * w[1][b] * g->rho;
+                float dx = (float)a - fx, dy = (float)b - fy;
+                acc_add(s, bx + a, by + b, 0, (int)floorf(m * SP_MASS_SCALE + 0.5f));
+                acc_add(s, bx + a, by + b, 1, (int)floorf(m * (g->vsx + (g->Csx * dx + g->Csy * dy)) * SP_MOM_SCALE + 0.5f));
+            }
+    }
+    {
+        float gx = g->px - 0.5f, gy = g->py;
+        int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline(fx, w[0]);
+        bspline(fy, w[1]);
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                float m = w[0][a] * w[1][b] * g->rho;
+                float dx = (float)a - fx, dy = (float)b - fy;
+                acc_add(s, bx + a, by + b, 2, (int)floorf(m * SP_MASS_SCALE + 0.5f));
+                acc_add(s, bx + a, by + b, 3, (int)floorf(m * (g->vsy + (g->Csz * dx + g->Csw * dy)) * SP_MOM_SCALE + 0.5f));
+            }
+    }
+    {
+        float gx = g->px - 0.5f, gy = g->py - 0.5f;
+        int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline(fx, w[0]);
+        bspline(fy, w[1]);
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                int wi = (int)floorf(w[0][a] * w[1][b] * SP_MASS_SCALE + 0.5f);
+                acc_add(s, bx + a, by + b, 4, wi);
+                if (g->cw >= 0) acc_add(s, bx + a, by + b, g->cw, wi);
+                if (g->cs >= 0 && g->sc > 0.0f)
+                    acc_add(s, bx + a, by + b, g->cs, (int)floorf(w[0][a] * w[1][b] * g->sc * SP_MASS_SCALE + 0.5f));
+            }
+    }
+}
+
+/* G2P of one face component (particles.comp:165-222). comp = 0: u faces, 1: v faces. */
+static int gather_face(const or_sim* s, float px, float py, int comp, float* u, float* cr, float* Ca, float* Cb) {
+    float gx = comp == 0 ? px : px - 0.5f, gy = comp == 0 ? py - 0.5f : py;
+    int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+    float fx = gx - (float)bx, fy = gy - (float)by;
+    float wx[3], wy[3];
+    bspline(fx, wx);
+    bspline(fy, wy);
+    float sw = 0.0f, su = 0.0f, sc = 0.0f, sdx = 0.0f, sdy = 0.0f, sudx = 0.0f, sudy = 0.0f;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+            v4 q = s->vel[cidx(s, bx + a, by + b)];
+            float st = comp == 0 ? q.z : q.w, val = comp == 0 ? q.x : q.y;
+            if (face_live(st)) {
+                float wt = wx[a] * wy[b];
+                float dx = (float)a - fx, dy = (float)b - fy;
+                sw += wt;
+                su += wt * val;
+                sc += wt * st;
+                sdx += wt * dx;
+                sdy += wt * dy;
+                float wu = wt * val;
+                sudx += wu * dx;
+                sudy += wu * dy;
+            }
+        }
+    if (sw > 1e-4f) {
+        float uu = su / sw;
+        *u = uu;
+        *cr = sc / sw;
+        *Ca = 4.0f * (sudx - uu * sdx) / sw;
+        *Cb = 4.0f * (sudy - uu * sdy) / sw;
+        return 1;
+    }
+    return 0;
+}
+
+/* advance (particles.comp:98-985), the in-scope branches. Returns the scatter inputs. */
+static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* umax_part) {
+    const sp_gparams* P = &s->P;
+    g->live = 0;
+    if (id >= s->n_slots) return;
+    sp_particle p = s->part[id];
+    if (p.kind == SP_KIND_DEAD) return;
+    if ((p.kind & 255u) == 0u) return;  /* a puff: not carried (no steam without heat) */
+    float dt = s->S.dt;
+    int row = (int)(p.kind & 255u) - 1;
+    const sp_gmat* M = &s->mats[row];
+    int Mgroup = M->group, Mdil = M->diluent, Mcw = M->chan_w, Mcs = M->chan_s;
+    int fam = M->fam;
+    if (gather) {
+        float vx = p.vel[0], vy = p.vel[1];
+        float C[4] = {p.c[0], p.c[1], p.c[2], p.c[3]};
+        float corrx = 0.0f, corry = 0.0f, slip = 0.0f, crowd = 0.0f;
+        gather_face(s, p.pos[0], p.pos[1], 0, &vx, &corrx, &C[0], &C[1]);
+        gather_face(s, p.pos[0], p.pos[1], 1, &vy, &corry, &C[2], &C[3]);
+        int held = 0;
+        /* Stabilisation (particles.comp:281-285). */
+        if (P->apic_clamp > 0.0f) {
+            float lim = P->apic_clamp / dt;
+            for (int k = 0; k < 4; k++) C[k] = clampf(C[k], -lim, lim);
+        }
+        if (P->corr[0] > 0.0f) {
+            float cl = len2(corrx, corry), cmax = P->corr[0] / dt;
+            if (cl > cmax) {
+                float k = cmax / cl;
+                corrx *= k;
+                corry *= k;
+            }
+        }
+        /* Composition gather (particles.comp:288-327). */
+        {
+            float gx = p.pos[0] - 0.5f, gy = p.pos[1] - 0.5f;
+            int bx = (int)floorf(gx - 0.5f), by = (int)floorf(gy - 0.5f);
+            float fx = gx - (float)bx, fy = gy - (float)by;
+            float wx[3], wy[3];
+            bspline(fx, wx);
+            bspline(fy, wy);
+            float sw = 0.0f, sfw = 0.0f, sfs = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f;
+            int more = (P->group_mask & 12u) != 0u;
+            int groups = (P->group_mask & (P->group_mask - 1u)) != 0u;
+            int sorts = group_sorts(s, Mgroup);
+            float sbg = 0.0f;
+            int nmix = P->n_fam - 1;
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    int ci = cidx(s, bx + a, by + b);
+                    v4 q = s->comp[ci];
+                    float wt = wx[a] * wy[b];
+                    crowd += wt * q.w;
+                    if (q.x > 0.0f) {
+                        sw += wt;
+                        sgx += wt * q.y;
+                        if (more) {
+                            sgy += wt * s->aux[ci].z;
+                            sgz += wt * s->aux[ci].w;
+                        }
+                        if (sorts) sbg += wt * v4c(s->gbase[ci], Mgroup - 1);
+                        float sb = fam > 0 ? s->mixb[(size_t)ci * nmix + fam - 1] : q.z;
+                        if (sb >= 0.0f) {
+                            sfw += wt;
+                            sfs += wt * sb;
+                        }
+                    }
+                }
+            if (sw > 1e-4f) {
+                /* Drift between phase groups (particles.comp:374-477). */
+                sgx /= sw;
+                sgy /= sw;
+                sgz /= sw;
+                float sgv[3] = {sgx, sgy, sgz};
+                int gg = Mgroup;
+                float fg = gg == 0 ? 1.0f - sgx - sgy - sgz : sgv[gg - 1];
+                float others = 1.0f - fg;
+                if (others > 1e-4f && groups && !held) {
+                    float sgr[4] = {0, 0, 0, 0}, snu[4] = {0, 0, 0, 0}, snw[4] = {0, 0, 0, 0};
+                    for (int b = 0; b < 3; b++)
+                        for (int a = 0; a < 3; a++) {
+                            int ci = cidx(s, bx + a, by + b);
+                            v4 q = s->comp[ci];
+                            if (q.x <= 0.0f) continue;
+                            float wt = wx[a] * wy[b];
+                            v4 gr = s->grho[ci];
+                            sgr[0] += wt * gr.x;
+                            sgr[1] += wt * gr.y;
+                            sgr[2] += wt * gr.z;
+                            sgr[3] += wt * gr.w;
+                            v4 ax = s->aux[ci];
+                            float fc[4] = {1.0f - q.y - ax.z - ax.w, q.y, ax.z, ax.w};
+                            float top = fmaxf_(fmaxf_(fc[0], fc[1]), fmaxf_(fc[2], fc[3]));
+                            float oh[4];
+                            for (int k = 0; k < 4; k++) oh[k] = fc[k] == top ? 1.0f : 0.0f;
+                            float ohs = fmaxf_(oh[0] + oh[1] + oh[2] + oh[3], 1.0f);
+                            for (int k = 0; k < 4; k++) oh[k] /= ohs;
+                            float wa = wt * ax.x;
+                            for (int k = 0; k < 4; k++) {
+                                snu[k] += wa * oh[k];
+                                snw[k] += wt * oh[k];
+                            }
+                        }
+                    for (int k = 0; k < 4; k++) sgr[k] /= sw;
+                    float rho_g = fg > 1e-4f ? sgr[gg] / fg : row_density(s, row, p.s);
+                    float rho_o = (sgr[0] + sgr[1] + sgr[2] + sgr[3] - sgr[gg]) / others;
+                    float drho = rho_g - rho_o;
+                    int pcx = clampi((int)p.pos[0], 1, s->W - 2), pcy = clampi((int)p.pos[1], 1, s->H - 2);
+                    int k = cidx(s, pcx, pcy);
+                    float fsv[4] = {1.0f - sgx - sgy - sgz, sgx, sgy, sgz};
+                    int G = fsv[0] >= fmaxf_(fsv[1], fmaxf_(fsv[2], fsv[3])) ? 0 : fsv[1] >= fmaxf_(fsv[2], fsv[3]) ? 1 : fsv[2] >= fsv[3] ? 2 : 3;
+                    float rho_c = sgr[G] / fmaxf_(fsv[G], 1e-4f);
+                    float nu_c = snw[G] > 1e-6f ? snu[G] / snw[G] : s->aux[k].x;
+                    float vt = terminal(s, drho, rho_c, nu_c, P->sep_tau);
+                    float sep_max = P->corr[2] > 0.0f ? P->cfl / dt : 1e30f;
+                    slip = -fminf_(vt, sep_max) * signf(drho) * others;
+                    float fs[4] = {1.0f - sgx - sgy - sgz, sgx, sgy, sgz};
+                    fs[gg] = -1.0f;
+                    int h = fs[0] >= fmaxf_(fs[1], fmaxf_(fs[2], fs[3])) ? 0 : fs[1] >= fmaxf_(fs[2], fs[3]) ? 1 : fs[2] >= fs[3] ? 2 : 3;
+                    float bond = fabsf(drho) * P->gravity / fmaxf_(P->pair[gg][h], 1e-9f);
+                    float hold = clampf(1.0f - bond, 0.0f, 1.0f);
+                    if (slip != 0.0f && hold > 0.0f) {
+                        int hi = gg > h ? gg : h;
+                        int W = s->W;
+                        float grx, gry;
+                        if (hi == 1) {
+                            grx = 0.5f * (s->surf[k + 1].y - s->surf[k - 1].y);
+                            gry = 0.5f * (s->surf[k + W].y - s->surf[k - W].y);
+                        } else if (hi == 2) {
+                            grx = 0.5f * (s->surf2[k + 1].x - s->surf2[k - 1].x);
+                            gry = 0.5f * (s->surf2[k + W].x - s->surf2[k - W].x);
+                        } else {
+                            grx = 0.5f * (s->surf2[k + 1].y - s->surf2[k - 1].y);
+                            gry = 0.5f * (s->surf2[k + W].y - s->surf2[k - W].y);
+                        }
+                        float resolved = 1.0f - clampf((0.12f - len2(grx, gry)) / (0.12f - 0.04f), 0.0f, 1.0f);
+                        slip *= 1.0f - hold * resolved;
+                    }
+                }
+                /* Drift within a phase group (particles.comp:490-502). */
+                if (sorts && !held) {
+                    float fgi = groups ? sgv[Mgroup - 1] : 1.0f;
+                    if (fgi > 1e-4f) {
+                        float rmean = sbg / sw / fgi;
+                        float drho = M->rho - rmean;
+                        int pcx = clampi((int)p.pos[0], 0, s->W - 1), pcy = clampi((int)p.pos[1], 0, s->H - 1);
+                        v4 cq = s->cell[cidx(s, pcx, pcy)];
+                        float nu_c = cq.y > 0.0f ? cq.z / cq.y : s->aux[cidx(s, pcx, pcy)].x;
+                        float vt = terminal(s, drho, rmean, nu_c, P->sep_tau);
+                        float sep_max = P->corr[2] > 0.0f ? P->cfl / dt : 1e30f;
+                        slip -= fminf_(vt, sep_max) * signf(drho);
+                    }
+                }
+                /* Miscible exchange within a family (particles.comp:527-543). */
+                int diluent0 = fam < 0 && P->primary >= 0 && P->fams[0][1] == row;
+                if (fam >= 0 || diluent0) {
+                    if (sfw > 1e-4f) {
+                        float s0 = diluent0 ? 0.0f : p.s;
+                        int rr = diluent0 ? P->primary : row;
+                        float s1 = clampf(s0 + (sfs / sfw - s0) * (1.0f - expf(-s->mats[rr].mix_rate * dt)), 0.0f, 1.0f);
+                        if (!diluent0) {
+                            p.s = s1;
+                        } else if (s1 > 0.0f) {
+                            row = P->primary;
+                            M = &s->mats[row];
+                            Mgroup = M->group;
+                            Mdil = M->diluent;
+                            Mcw = M->chan_w;
+                            Mcs = M->chan_s;
+                            p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                            p.s = s1;
+                        }
+                    }
+                }
+                /* A pure diluent joins a family at concentration 0 (particles.comp:544-569). */
+                if (fam < 0 && (P->caps & 1u) && (int)(p.kind & 255u) - 1 == row && s->mats[row].fam < 0) {
+                    for (int j = P->primary >= 0 ? 1 : 0; j < P->n_fam; j++) {
+                        if (P->fams[j][1] != row) continue;
+                        float found = 0.0f;
+                        if (j == 0) {
+                            found = sfw;
+                        } else {
+                            for (int b = 0; b < 3; b++)
+                                for (int a = 0; a < 3; a++)
+                                    if (s->mixb[(size_t)cidx(s, bx + a, by + b) * nmix + j - 1] >= 0.0f) found += wx[a] * wy[b];
+                        }
+                        if (found > 1e-4f) {
+                            row = P->fams[j][0];
+                            M = &s->mats[row];
+                            Mgroup = M->group;
+                            Mdil = M->diluent;
+                            Mcw = M->chan_w;
+                            Mcs = M->chan_s;
+                            p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                            p.s = 0.0f;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        (void)Mdil;
+        /* Speed cap (particles.comp:831-835). */
+        float vcap = P->ext[0];
+        {
+            float sp = len2(vx, vy);
+            if (sp > vcap) {
+                float k = vcap / sp;
+                vx *= k;
+                vy *= k;
+            }
+        }
+        /* Midpoint advection (particles.comp:847-865). */
+        float vax = vx, vay = vy;
+        if (P->ext[1] >= 2.0f && !held) {
+            float h = 0.5f * dt;
+            float ex = (vx + corrx) + 0.0f, ey = (vy + corry) + slip;
+            float xmx = clampf(p.pos[0] + h * ex, 2.01f, (float)s->W - 2.01f);
+            float xmy = clampf(p.pos[1] + h * ey, 2.01f, (float)s->H - 2.01f);
+            v2 vv = {vx, vy};
+            v2 here = sample_vel(s, p.pos[0], p.pos[1], vv);
+            v2 mid = sample_vel(s, xmx, xmy, here);
+            vax = vx + mid.x - here.x;
+            vay = vy + mid.y - here.y;
+            float sp = len2(vax, vay);
+            if (sp > vcap) {
+                float k = vcap / sp;
+                vax *= k;
+                vay *= k;
+            }
+            if (crowd < SP_LIQUID_THR && (vax * vax + vay * vay) < 0.01f * (vx * vx + vy * vy)) {
+                vx = vax;
+                vy = vay;
+            }
+        }
+        float dispx = (vax + corrx) + 0.0f, dispy = (vay + corry) + slip;
+        {
+            float mine = len2(dispx, dispy);
+            if (crowd < SP_LIQUID_THR) mine /= P->ext[2];
+            if (mine > *umax_part) *umax_part = mine;
+        }
+        float sdx = dt * dispx, sdy = dt * dispy;
+        int pieces = (int)ceilf(len2(sdx, sdy));
+        pieces = clampi(pieces, 1, 32);
+        sdx /= (float)pieces;
+        sdy /= (float)pieces;
+        float x = p.pos[0], y = p.pos[1];
+        for (int k = 1; k < pieces; k++) {
+            x += sdx;
+            y += sdy;
+            collide(s, &x, &y, &vx, &vy, 1);
+        }
+        x += sdx;
+        y += sdy;
+        float excess = clampf(crowd - 1.5f, 0.0f, 1.0f);
+        if (excess > 0.0f && P->crowd_jitter > 0.0f) {
+            uint32_t hh = hash_u(p.id ^ hash_u(f2u(p.pos[0]) ^ s->S.substeps));
+            x += ((float)(hh & 0xffffu) / 65535.0f - 0.5f) * P->crowd_jitter * excess;
+            y += ((float)(hh >> 16) / 65535.0f - 0.5f) * P->crowd_jitter * excess;
+        }
+        collide(s, &x, &y, &vx, &vy, 1);
+        int again = collide(s, &x, &y, &vx, &vy, 1);
+        if (again) {
+            float gx_, gy_;
+            if (sample_sdf(s, x, y, &gx_, &gy_) < -1.0f) collide(s, &x, &y, &vx, &vy, 0);
+        }
+        x = clampf(x, 2.01f, (float)s->W - 2.01f);
+        y = clampf(y, 2.01f, (float)s->H - 2.01f);
+        p.pos[0] = x;
+        p.pos[1] = y;
+        p.vel[0] = vx;
+        p.vel[1] = vy;
+        for (int k = 0; k < 4; k++) p.c[k] = C[k];
+        p.kind &= ~SP_KIND_FRESH;
+        s->part[id] = p;
+    }
+    float rho_p = row_density(s, row, p.s);
+    g->live = 1;
+    g->px = p.pos[0];
+    g->py = p.pos[1];
+    g->vsx = p.vel[0] * dt;
+    g->vsy = p.vel[1] * dt;
+    g->Csx = p.c[0] * dt;
+    g->Csy = p.c[1] * dt;
+    g->Csz = p.c[2] * dt;
+    g->Csw = p.c[3] * dt;
+    g->rho = rho_p;
+    g->sc = p.s;
+    g->cw = Mcw;
+    g->cs = Mcs;
+}
+
+static void stage_particles(or_sim* s, int gather) {
+    float umax = u2f(s->S.umax_part);
+    for (uint32_t id = 0; id < s->n_slots; id++) {
+        scatter_in g;
+        advance(s, id, gather, &g, &umax);
+        if (g.live) deposit(s, &g);
+    }
+    /* atomicMax on float bits of non-negative floats (particles.comp:872-877). */
+    if (umax > u2f(s->S.umax_part)) s->S.umax_part = f2u(umax);
+}
+
+static void stage_clear_acc(or_sim* s) {
+    memset(s->acc, 0, sizeof(int32_t) * (size_t)s->N * s->P.acc_stride);
+}
+
+/* ---- grid_prep.comp (grid_prep.comp:13-263) ------------------------------------------- */
+
+static int packed(const or_sim* s, int i) {
+    v4 sol = s->solid[i];
+    float phi_f = (float)s->acc[(size_t)i * s->P.acc_stride + 4] / SP_MASS_SCALE * s->P.inv_ppc;
+    return sol.w > 0.02f && phi_f > 0.04f && phi_f + sol.z > SP_LIQUID_THR;
+}
+
+static void stage_grid_prep(or_sim* s) {
+    const sp_gparams* P = &s->P;
+    int W = s->W, H = s->H;
+    int nmix = P->n_fam - 1;
+    int any_sorts = group_sorts(s, 1) || group_sorts(s, 2) || group_sorts(s, 3);
+    int multi = (P->group_mask & (P->group_mask - 1u)) != 0u;
+    float dt = s->S.dt;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            size_t k = (size_t)i * P->acc_stride;
+            int um = s->acc[k], umom = s->acc[k + 1], vm = s->acc[k + 2], vmom = s->acc[k + 3];
+            int cw = s->acc[k + 4];
+            v4 sol = s->solid[i];
+            v4 o = {0.0f, 0.0f, SP_FACE_EMPTY, SP_FACE_EMPTY};
+            if (sol.x <= 0.0f) {
+                o.z = SP_FACE_CLOSED;
+            } else if (um >= SP_MIN_FACE_MASS) {
+                o.x = ((float)umom / SP_MOM_SCALE) / ((float)um / SP_MASS_SCALE) / dt;
+                o.z = 0.0f;
+            }
+            if (sol.y <= 0.0f) {
+                o.w = SP_FACE_CLOSED;
+            } else if (vm >= SP_MIN_FACE_MASS) {
+                o.y = ((float)vmom / SP_MOM_SCALE) / ((float)vm / SP_MASS_SCALE) / dt - P->gravity * dt;
+                o.w = 0.0f;
+            }
+            s->vel[i] = o;
+            s->vel0[i].x = o.x;
+            s->vel0[i].y = o.y;
+            float phi_f = (float)cw / SP_MASS_SCALE * P->inv_ppc;
+            float f[4] = {0, 0, 0, 0}, rg[4] = {0, 0, 0, 0}, lg[4] = {0, 0, 0, 0};
+            float bg[3] = {0, 0, 0};
+            float sigma = 0.0f, cx_ = 0.0f, cy_ = 0.0f, fam0 = -1.0f;
+            float rho = 0.0f, mu = 0.0f, rho_c = 0.0f, nu_c = 0.0f;
+            int major = 0, top_row = -1;
+            float top_share = 0.0f;
+            for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
+            if (cw > 0) {
+                int others = 0;
+                for (int j = 0; j < P->n_present; j++) {
+                    int ch = s->mats[P->present[j]].chan_w;
+                    if (ch >= 0) others += s->acc[k + ch];
+                }
+                float sh[SP_MAX_PRESENT], scn[SP_MAX_PRESENT];
+                for (int j = 0; j < P->n_present; j++) {
+                    sh[j] = 0.0f;
+                    scn[j] = 1.0f;
+                    int r = P->present[j];
+                    const sp_gmat* m = &s->mats[r];
+                    if (m->chan_w < 0 && P->primary >= 0 && r != P->primary) continue;
+                    int w = m->chan_w >= 0 ? s->acc[k + m->chan_w] : (cw - others > 0 ? cw - others : 0);
+                    if (w <= 0) continue;
+                    sh[j] = w >= cw ? 1.0f : clampf((float)w / (float)cw, 0.0f, 1.0f);
+                    if (m->diluent >= 0 && m->chan_s >= 0) scn[j] = clampf((float)s->acc[k + m->chan_s] / (float)w, 0.0f, 1.0f);
+                }
+                for (int j = 0; j < P->n_present; j++) {
+                    float share = sh[j];
+                    if (share <= 0.0f) continue;
+                    int r = P->present[j];
+                    const sp_gmat* m = &s->mats[r];
+                    float sc = scn[j];
+                    float rho_m = m->rho, lnu = m->ln_nu, sig = m->sigma;
+                    float xp = 1.0f;
+                    float csx = m->cos_c, csy = m->sin_c;
+                    if (m->diluent >= 0) {
+                        const sp_gmat* d = &s->mats[m->diluent];
+                        rho_m = mixf(d->rho, rho_m, sc);
+                        float bs = m->visc_exp == 1.0f ? sc : powf(sc, m->visc_exp);
+                        lnu = mixf(d->ln_nu, lnu, bs);
+                        sig = mixf(d->sigma, sig, sc);
+                        csx = mixf(d->cos_c, csx, sc);
+                        csy = mixf(d->sin_c, csy, sc);
+                        int fm = m->fam;
+                        if (fm == 0) fam0 = sc;
+                        else if (fm > 0) s->mixb[(size_t)i * nmix + fm - 1] = sc;
+                    }
+                    rho_m *= xp;
+                    if (share > top_share) {
+                        top_share = share;
+                        top_row = r;
+                    }
+                    int g = m->group;
+                    f[g] += share;
+                    rg[g] += share * rho_m;
+                    if (g > 0) bg[g - 1] += share * m->rho;
+                    lg[g] += share * lnu;
+                    sigma += share * sig;
+                    cx_ += share * csx;
+                    cy_ += share * csy;
+                }
+                for (int g = 0; g < 4; g++) {
+                    if (f[g] <= 0.0f) continue;
+                    rho += rg[g];
+                    mu += rg[g] * expf(lg[g] / f[g]);
+                    if (f[g] > f[major]) major = g;
+                }
+                if (f[major] > 0.0f) {
+                    rho_c = rg[major] / f[major];
+                    nu_c = expf(lg[major] / f[major]);
+                }
+            }
+            float phi_t = phi_f + sol.z;
+            int has = phi_f > 0.04f && sol.w > 0.02f;
+            float level = has ? phi_t - SP_LIQUID_THR : -SP_LIQUID_THR;
+            if (has && level <= 0.0f && cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
+                if ((packed(s, i - 1) && packed(s, i + 1)) || (packed(s, i - W) && packed(s, i + W))) level = 0.05f;
+                else if (P->gravity > 0.0f && phi_t > 0.25f && packed(s, i - W)) level = 0.05f;
+            }
+            if (sol.w > 0.0f && level <= 0.0f) s->region_air |= 1u << (s->region[i] & SP_REGION_MASK);
+            s->cell[i] = (v4){phi_f, rho, mu, level};
+            s->comp[i] = (v4){rho, f[1], fam0, phi_t};
+            if (any_sorts) s->gbase[i] = (v4){bg[0], bg[1], bg[2], 0.0f};
+            if (multi) {
+                s->aux[i] = (v4){nu_c, rho_c, f[2], f[3]};
+                s->grho[i] = (v4){rg[0], rg[1], rg[2], rg[3]};
+            }
+            s->sprop[i] = (v4){sigma, cx_, cy_, (float)major + (float)(top_row + 1) / 128.0f};
+            if (level <= 0.0f) s->pres[i] = (v2){0.0f, 0.0f};
+        }
+}
+
+/* ---- grid_coef.comp (grid_coef.comp:12-38) -------------------------------------------- */
+
+static void face_coef(v4 a, v4 b, float open, float* g, float* rho) {
+    float wsum = a.x + b.x;
+    *rho = wsum > 1e-6f ? (a.x * a.y + b.x * b.y) / wsum : 1.0f;
+    *g = 0.0f;
+    int fa = a.w > 0.0f, fb = b.w > 0.0f;
+    if (open <= 0.0f || !(fa || fb)) return;
+    float theta = 1.0f;
+    if (fa != fb) {
+        float lf = fa ? a.w : b.w;
+        float la = fa ? b.w : a.w;
+        theta = clampf(lf / (lf - la), SP_THETA_MIN, 1.0f);
+    }
+    *g = 1.0f / (fmaxf_(*rho, 0.05f) * theta);
+}
+
+static void stage_coef(or_sim* s) {
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < s->W; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 o = {0.0f, 0.0f, 1.0f, 1.0f};
+            if (cx > 0 && cy > 0) {
+                v4 me = s->cell[i];
+                v4 sol = s->solid[i];
+                face_coef(me, s->cell[i - 1], sol.x, &o.x, &o.z);
+                face_coef(me, s->cell[i - s->W], sol.y, &o.y, &o.w);
+            }
+            s->coef[i] = o;
+        }
+}
+
+/* ---- surf_smooth / surf_curv / surf_apply --------------------------------------------- */
+
+static void stage_surf_smooth(or_sim* s) {
+    static const float B5[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
+    int more = (s->P.group_mask & 12u) != 0u;
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < s->W; cx++) {
+            float sx = 0, sy = 0, s2x = 0, s2y = 0, wsum = 0;
+            for (int b = -2; b <= 2; b++)
+                for (int a = -2; a <= 2; a++) {
+                    int nx = cx + a, ny = cy + b;
+                    if (nx < 0 || ny < 0 || nx >= s->W || ny >= s->H) continue;
+                    int j = cidx(s, nx, ny);
+                    float open = s->solid[j].w;
+                    if (open <= 0.02f) continue;
+                    float phi = clampf(s->cell[j].x / (SP_REST_PHI * open), 0.0f, 1.0f);
+                    float w = B5[a + 2] * B5[b + 2];
+                    sx += w * phi;
+                    sy += w * (phi * s->comp[j].y);
+                    if (more) {
+                        float wp = w * phi;
+                        s2x += wp * s->aux[j].z;
+                        s2y += wp * s->aux[j].w;
+                    }
+                    wsum += w;
+                }
+            int i = cidx(s, cx, cy);
+            s->surf[i].x = wsum > 0.0f ? sx / wsum : 0.0f;
+            s->surf[i].y = wsum > 0.0f ? sy / wsum : 0.0f;
+            if (more) {
+                s->surf2[i].x = wsum > 0.0f ? s2x / wsum : 0.0f;
+                s->surf2[i].y = wsum > 0.0f ? s2y / wsum : 0.0f;
+            }
+        }
+}
+
+static float sfield(const or_sim* s, int x, int y, int k) {
+    x = clampi(x, 0, s->W - 1);
+    y = clampi(y, 0, s->H - 1);
+    int i = cidx(s, x, y);
+    if (k < 2) return k == 0 ? s->surf[i].x : s->surf[i].y;
+    return k == 2 ? s->surf2[i].x : s->surf2[i].y;
+}
+
+static float node_sdf(const or_sim* s, int x, int y) {
+    x = clampi(x, 0, s->W);
+    y = clampi(y, 0, s->H);
+    return s->sdf[y * (s->W + 1) + x];
+}
+
+static v2 contact_at(const or_sim* s, int nx, int ny) {
+    v4 sp = s->sprop[cidx(s, clampi(nx, 0, s->W - 1), clampi(ny, 0, s->H - 1))];
+    float cx = sp.y, cy = sp.z;
+    if (cx * cx + cy * cy > 1e-12f) {
+        float l = len2(cx, cy);
+        return (v2){cx / l, cy / l};
+    }
+    for (int b = -3; b <= 2; b++)
+        for (int a = -3; a <= 2; a++) {
+            int j = cidx(s, clampi(nx + a, 0, s->W - 1), clampi(ny + b, 0, s->H - 1));
+            float occ = s->cell[j].x;
+            cx += s->sprop[j].y * occ;
+            cy += s->sprop[j].z * occ;
+        }
+    if (cx * cx + cy * cy > 1e-12f) {
+        float l = len2(cx, cy);
+        return (v2){cx / l, cy / l};
+    }
+    return (v2){0.0f, 1.0f};
+}
+
+static v2 normal_node(const or_sim* s, int nx, int ny, int k, float* mag) {
+    const float G_MIN = 0.05f;
+    float a = sfield(s, nx, ny, k), b = sfield(s, nx - 1, ny, k), d = sfield(s, nx, ny - 1, k), e = sfield(s, nx - 1, ny - 1, k);
+    float gx = 0.5f * (a + d - b - e), gy = 0.5f * (a + b - d - e);
+    *mag = len2(gx, gy);
+    if (*mag < G_MIN) return (v2){0.0f, 0.0f};
+    v2 nn = {-gx / *mag, -gy / *mag};
+    if (k == 0) {
+        float dd = node_sdf(s, nx, ny);
+        if (dd < 0.5f) {
+            float gwx = node_sdf(s, nx + 1, ny) - node_sdf(s, nx - 1, ny);
+            float gwy = node_sdf(s, nx, ny + 1) - node_sdf(s, nx, ny - 1);
+            float lw = len2(gwx, gwy);
+            if (lw > 1e-4f) {
+                float nwx = gwx / lw, nwy = gwy / lw;
+                float gdn = gx * nwx + gy * nwy;
+                float gtx = gx - gdn * nwx, gty = gy - gdn * nwy;
+                float along = len2(gtx, gty);
+                if (along > 0.5f * G_MIN) {
+                    float tx = -gtx / along, ty = -gty / along;
+                    v2 cs = contact_at(s, nx, ny);
+                    float ncx = cs.x * nwx + cs.y * tx, ncy = cs.x * nwy + cs.y * ty;
+                    float sm = smoothstepf(0.5f * G_MIN, G_MIN, along);
+                    float mx = mixf(nn.x, ncx, sm), my = mixf(nn.y, ncy, sm);
+                    float lm = len2(mx, my);
+                    if (lm > 1e-4f) {
+                        nn.x = mx / lm;
+                        nn.y = my / lm;
+                    } else {
+                        nn.x = ncx;
+                        nn.y = ncy;
+                    }
+                }
+            }
+        }
+    }
+    return nn;
+}
+
+static float curvature(const or_sim* s, int cx, int cy, int k, float kmax) {
+    float m00, m10, m01, m11;
+    v2 n00 = normal_node(s, cx, cy, k, &m00);
+    v2 n10 = normal_node(s, cx + 1, cy, k, &m10);
+    v2 n01 = normal_node(s, cx, cy + 1, k, &m01);
+    v2 n11 = normal_node(s, cx + 1, cy + 1, k, &m11);
+    if (0.25f * (m00 + m10 + m01 + m11) < 0.05f) return 0.0f;
+    float kap = 0.5f * (n10.x + n11.x - n00.x - n01.x) + 0.5f * (n01.y + n11.y - n00.y - n10.y);
+    return kmax > 0.0f ? clampf(kap, -kmax, kmax) : kap;
+}
+
+static void stage_surf_curv(or_sim* s) {
+    float kmax = s->P.kappa_max;
+    uint32_t gm = s->P.group_mask;
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < s->W; cx++) {
+            int i = cidx(s, cx, cy);
+            float k0 = curvature(s, cx, cy, 0, kmax);
+            float k1 = curvature(s, cx, cy, 1, kmax);
+            s->surf[i].z = k0;
+            s->surf[i].w = k1;
+            if (gm & 12u) {
+                s->surf2[i].z = (gm & 4u) ? curvature(s, cx, cy, 2, kmax) : 0.0f;
+                s->surf2[i].w = (gm & 8u) ? curvature(s, cx, cy, 3, kmax) : 0.0f;
+            }
+        }
+}
+
+static float air_jump(const or_sim* s, int a, int b) {
+    int la = s->cell[a].w > 0.0f;
+    int liq = la ? a : b, air = la ? b : a;
+    float lf = s->cell[liq].w, lgv = s->cell[air].w;
+    float theta = clampf(lf / (lf - lgv), SP_THETA_MIN, 1.0f);
+    float kl = s->surf[liq].z, kg = s->surf[air].z;
+    float kap = kg != 0.0f ? mixf(kl, kg, theta) : kl;
+    return s->S.dt * s->sprop[liq].x * kap;
+}
+
+static float share_of(const or_sim* s, int i, int g) {
+    if (g == 1) return s->comp[i].y;
+    if (g == 2) return s->aux[i].z;
+    if (g == 3) return s->aux[i].w;
+    return 1.0f - s->comp[i].y - s->aux[i].z - s->aux[i].w;
+}
+
+static float curv_of(const or_sim* s, int i, int g) {
+    return g == 1 ? s->surf[i].w : g == 2 ? s->surf2[i].z : s->surf2[i].w;
+}
+
+static float pair_jump(const or_sim* s, int c, int n) {
+    int gc = (int)s->sprop[c].w, gn = (int)s->sprop[n].w;
+    if (gc == gn) return 0.0f;
+    int hi = gc > gn ? gc : gn, lo = gc < gn ? gc : gn;
+    float dc = share_of(s, c, hi) - share_of(s, c, lo), dn = share_of(s, n, hi) - share_of(s, n, lo);
+    if (dc * dn >= 0.0f) return 0.0f;
+    float theta = -dn / (dc - dn);
+    float kc = curv_of(s, c, hi), kn = curv_of(s, n, hi);
+    float kap = kc == 0.0f ? kn : kn == 0.0f ? kc : mixf(kn, kc, theta);
+    float j = s->S.dt * s->P.pair[hi][lo] * kap;
+    return gc == hi ? j : -j;
+}
+
+static float face_jump(const or_sim* s, int c, int n, float open) {
+    if (open <= 0.0f) return 0.0f;
+    int lc = s->cell[c].w > 0.0f, ln = s->cell[n].w > 0.0f;
+    if (lc != ln) return air_jump(s, c, n);
+    if (lc && ln) return pair_jump(s, c, n);
+    return 0.0f;
+}
+
+static void stage_surf_apply(or_sim* s) {
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < s->W; cx++) {
+            int i = cidx(s, cx, cy);
+            v2 pj = {0.0f, 0.0f};
+            if (cx > 0 && cy > 0) {
+                v4 sol = s->solid[i];
+                pj.x = face_jump(s, i, i - 1, sol.x);
+                pj.y = face_jump(s, i, i - s->W, sol.y);
+            }
+            s->pjump[i] = pj;
+        }
+}
+
+/* ---- explicit viscosity: visc_setup.comp + visc_apply.comp mode 2 ---------------------- */
+
+static int live_u(const or_sim* s, int i) { return face_live(s->vel[i].z); }
+static int live_v(const or_sim* s, int i) { return face_live(s->vel[i].w); }
+static int closed_u(const or_sim* s, int i) { return face_closed(s->vel[i].z); }
+static int closed_v(const or_sim* s, int i) { return face_closed(s->vel[i].w); }
+static float vel_u(const or_sim* s, int i) { return live_u(s, i) ? s->vel[i].x : 0.0f; }
+static float vel_v(const or_sim* s, int i) { return live_v(s, i) ? s->vel[i].y : 0.0f; }
+
+static v4 strain_of_vel(const or_sim* s, int i, float wx, float wy) {
+    int W = s->W;
+    v4 o = {0, 0, 0, 0};
+    if (wx > 0.0f) {
+        o.x = wx * (vel_u(s, i + 1) - vel_u(s, i));
+        o.y = wx * (vel_v(s, i + W) - vel_v(s, i));
+    }
+    if (wy > 0.0f) {
+        float ua = vel_u(s, i), ub = vel_u(s, i - W), va = vel_v(s, i), vb = vel_v(s, i - 1);
+        if (closed_u(s, i) && live_u(s, i - W)) ua = -ub;
+        if (closed_u(s, i - W) && live_u(s, i)) ub = -ua;
+        if (closed_v(s, i) && live_v(s, i - 1)) va = -vb;
+        if (closed_v(s, i - 1) && live_v(s, i)) vb = -va;
+        o.z = wy * 0.5f * ((ua - ub) + (va - vb));
+    }
+    return o;
+}
+
+static void stage_visc_explicit(or_sim* s) {
+    int W = s->W, H = s->H;
+    float dt = s->S.dt;
+    /* visc_setup.comp:77-121 (the tile flags only matter to the implicit solve). */
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            float wc = 0.0f, wn = 0.0f;
+            if (cx >= 1 && cy >= 1 && cx < W - 1 && cy < H - 1) {
+                v4 me = s->cell[i];
+#define OK_U(j) (live_u(s, j) || closed_u(s, j))
+#define OK_V(j) (live_v(s, j) || closed_v(s, j))
+                if (me.w > 0.0f && OK_U(i) && OK_U(i + 1) && OK_V(i) && OK_V(i + W)) wc = 2.0f * dt * me.z;
+                int cs4[4] = {i - W - 1, i - W, i - 1, i};
+                int nliq = 0, any_air = 0;
+                float mu = 0.0f;
+                for (int k = 0; k < 4; k++) {
+                    v4 q = s->cell[cs4[k]];
+                    if (q.w > 0.0f) {
+                        nliq++;
+                        mu += q.z;
+                    } else if (s->solid[cs4[k]].w > 0.02f) {
+                        any_air = 1;
+                    }
+                }
+                if (nliq > 0 && !any_air && OK_U(i) && OK_U(i - W) && OK_V(i) && OK_V(i - 1))
+                    wn = 4.0f * dt * (mu / (float)nliq) * ((float)nliq / 4.0f);
+#undef OK_U
+#undef OK_V
+            }
+            s->vw[i].x = wc;
+            s->vw[i].y = wn;
+        }
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            s->vs[i] = strain_of_vel(s, i, s->vw[i].x, s->vw[i].y);
+        }
+    /* visc_apply.comp mode 2 (visc_apply.comp:137-170). */
+    for (int cy = 1; cy < H - 1; cy++)
+        for (int cx = 1; cx < W - 1; cx++) {
+            int i = cidx(s, cx, cy);
+            float gx = 0.0f, gy = 0.0f;
+            if (live_u(s, i)) {
+                float c1 = closed_u(s, i - W) ? 1.0f : 0.5f, c2 = closed_u(s, i + W) ? -1.0f : -0.5f;
+                gx = s->vs[i - 1].x - s->vs[i].x + s->vs[i].z * c1 + s->vs[i + W].z * c2;
+            }
+            if (live_v(s, i)) {
+                float d1 = closed_v(s, i - 1) ? 1.0f : 0.5f, d2 = closed_v(s, i + 1) ? -1.0f : -0.5f;
+                gy = s->vs[i - W].y - s->vs[i].y + s->vs[i].z * d1 + s->vs[i + 1].z * d2;
+            }
+            v4 cf = s->coef[i];
+            if (live_u(s, i)) s->vel[i].x -= gx / cf.z;
+            if (live_v(s, i)) s->vel[i].y -= gy / cf.w;
+        }
+}
+
+/* ---- rhs.comp (rhs.comp:58-133) ------------------------------------------------------- */
+
+static void stage_rhs(or_sim* s) {
+    const sp_gparams* P = &s->P;
+    int W = s->W, H = s->H;
+    int tx = (W + 15) >> 4;
+    memset(s->ptile, 0, sizeof(uint32_t) * (size_t)tx * ((H + 15) >> 4));
+    float dt = s->S.dt;
+    float corr_gain = fminf_(1.0f - expf(-dt / P->tol[3]), P->density_k);
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 me = s->cell[i];
+            if (me.w <= 0.0f || cx == 0 || cy == 0 || cx == W - 1 || cy == H - 1) {
+                s->mga[i] = (v4){0, 0, 0, 0};
+                s->mgd[i] = 0.0f;
+                s->prhs[i] = (v4){0, 0, 0, 0};
+                continue;
+            }
+            v4 sol = s->solid[i];
+            v4 cf = s->coef[i];
+            float aL = sol.x, aB = sol.y, aR = s->solid[i + 1].x, aT = s->solid[i + W].y;
+            v4 w = {aL * cf.x, aR * s->coef[i + 1].x, aB * cf.y, aT * s->coef[i + W].y};
+            float diag = w.x + w.y + w.z + w.w;
+            uint32_t rg = s->region[i];
+            int sealed = (s->region_air & (1u << (rg & SP_REGION_MASK))) == 0u;
+            if ((rg & SP_REGION_VENT) != 0u && sealed && P->sealed_vent > 0.0f) diag += 1.0f / fmaxf_(me.y, 0.05f);
+            v4 vme = s->vel[i];
+            float div = aR * s->vel[i + 1].x - aL * vme.x + aT * s->vel[i + W].y - aB * vme.y;
+            int lR = s->cell[i + 1].w > 0.0f, lT = s->cell[i + W].w > 0.0f;
+            float jump = w.x * s->pjump[i].x + w.y * (lR ? -s->pjump[i + 1].x : s->pjump[i + 1].x)
+                + w.z * s->pjump[i].y + w.w * (lT ? -s->pjump[i + W].y : s->pjump[i + W].y);
+            float err = clampf(s->comp[i].w - SP_REST_PHI, -0.5f, 0.5f);
+            {
+                int interior = (aL <= 0.0f || s->cell[i - 1].w > 0.0f) && (aR <= 0.0f || s->cell[i + 1].w > 0.0f)
+                    && (aB <= 0.0f || s->cell[i - W].w > 0.0f) && (aT <= 0.0f || s->cell[i + W].w > 0.0f);
+                int pressed = s->pres[i].x * (dt / s->S.dt_prev) > 0.5f * me.y * P->gravity * dt;
+                if (interior && !pressed) err = fmaxf_(err - P->corr[1], 0.0f);
+                else if (!interior && err < 0.0f) err = 0.0f;
+            }
+            float sv = corr_gain * err;
+            if (sealed && P->sealed_vent > 0.0f) sv = 0.0f;
+            if (diag > 1e-9f) s->ptile[(cy >> 4) * tx + (cx >> 4)] = 1u;
+            s->mga[i] = diag > 1e-9f ? w : (v4){0, 0, 0, 0};
+            s->mgd[i] = diag > 1e-9f ? diag : 0.0f;
+            s->prhs[i] = (v4){-div + jump, sv / dt, 0.0f, 0.0f};
+        }
+}
+
+/* ---- pressure: multigrid-preconditioned CG (mod.rs:2929-3010) ------------------------- */
+
+static const int NBX[4] = {-1, 1, 0, 0}, NBY[4] = {0, 0, -1, 1};
+
+void or_levels(int32_t W, int32_t H, sp_levels_info* L) {
+    memset(L, 0, sizeof(*L));
+    int n = 0;
+    L->dims[0][0] = W;
+    L->dims[0][1] = H;
+    n = 1;
+    while (L->dims[n - 1][0] * L->dims[n - 1][1] > SP_COARSEST_CELLS && n < SP_MAX_LEVELS) {
+        L->dims[n][0] = (L->dims[n - 1][0] + 1) / 2;
+        L->dims[n][1] = (L->dims[n - 1][1] + 1) / 2;
+        n++;
+    }
+    L->n = n;
+    int total = 0;
+    for (int l = 0; l < n; l++) {
+        L->offs[l] = total;
+        total += L->dims[l][0] * L->dims[l][1];
+    }
+    L->total = total;
+    int tail = n - 1;
+    for (int l = 1; l < n; l++)
+        if (L->dims[l][0] * L->dims[l][1] <= SP_TAIL_CELLS) {
+            tail = l;
+            break;
+        }
+    if (tail < 1) tail = 1;
+    if (tail > n - 1) tail = n - 1;
+    L->tail = tail;
+    int tiles = 0;
+    for (int l = 0; l <= tail; l++) {
+        L->tile_offs[l] = tiles;
+        tiles += ((L->dims[l][0] + 15) / 16) * ((L->dims[l][1] + 15) / 16);
+    }
+    L->tiles = tiles;
+}
+
+typedef struct { int off, w, h; } lvl;
+static lvl level_of(const or_sim* s, int l) {
+    lvl r = {s->lv.offs[l], s->lv.dims[l][0], s->lv.dims[l][1]};
+    return r;
+}
+static int inside(lvl L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y < L.h; }
+static int lat(lvl L, int x, int y) { return L.off + y * L.w + x; }
+static int unknown(const or_sim* s, lvl L, int x, int y) { return inside(L, x, y) && s->mgd[lat(L, x, y)] > 0.0f; }
+
+/* Galerkin coarse matrix of coarse cell (cx, cy) (mg_build.comp:80-102, mg_tail.comp:44-70). */
+static void build_cell(or_sim* s, lvl F, lvl C, int cx, int cy) {
+    float D = 0.0f;
+    v4 A = {0, 0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+        int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
+        if (!unknown(s, F, x, y)) continue;
+        int i = lat(F, x, y);
+        D += s->mgd[i];
+        v4 w = s->mga[i];
+        for (int d = 0; d < 4; d++) {
+            int nx = x + NBX[d], ny = y + NBY[d];
+            if (!unknown(s, F, nx, ny)) continue;
+            float wd = v4c(w, d);
+            if ((nx >> 1) == cx && (ny >> 1) == cy) D -= wd;
+            else if (d == 0) A.x += wd;
+            else if (d == 1) A.y += wd;
+            else if (d == 2) A.z += wd;
+            else A.w += wd;
+        }
+    }
+    int o = lat(C, cx, cy);
+    s->mgd[o] = D > 1e-7f ? D : 0.0f;
+    s->mga[o] = D > 1e-7f ? A : (v4){0, 0, 0, 0};
+}
+
+static void stage_mg_build(or_sim* s) {
+    const sp_levels_info* L = &s->lv;
+    for (int l = 1; l <= L->tail; l++) {
+        lvl F = level_of(s, l - 1), C = level_of(s, l);
+        /* Tile flags: a coarse tile is active when one of its four fine tiles is. */
+        int tfx = (F.w + 15) >> 4, tfy = (F.h + 15) >> 4;
+        int tcx = (C.w + 15) >> 4, tcy = (C.h + 15) >> 4;
+        for (int ty = 0; ty < tcy; ty++)
+            for (int tx = 0; tx < tcx; tx++) {
+                uint32_t on = 0;
+                for (int k = 0; k < 4; k++) {
+                    int fx = 2 * tx + (k & 1), fy = 2 * ty + (k >> 1);
+                    if (fx < tfx && fy < tfy) on |= s->ptile[L->tile_offs[l - 1] + fy * tfx + fx];
+                }
+                s->ptile[L->tile_offs[l] + ty * tcx + tx] = on ? 1u : 0u;
+            }
+        for (int cy = 0; cy < C.h; cy++)
+            for (int cx = 0; cx < C.w; cx++) build_cell(s, F, C, cx, cy);
+    }
+    for (int l = L->tail; l < L->n - 1; l++) {
+        lvl F = level_of(s, l), C = level_of(s, l + 1);
+        for (int cy = 0; cy < C.h; cy++)
+            for (int cx = 0; cx < C.w; cx++) build_cell(s, F, C, cx, cy);
+    }
+}
+
+/* black cell after the first red half-sweep from zero (mg_down.comp:126-137) */
+static v2 down_black(const or_sim* s, lvl F, int x, int y) {
+    int i = lat(F, x, y);
+    v4 w = s->mga[i];
+    v2 sum = s->mgr[i];
+    for (int d = 0; d < 4; d++) {
+        int mx = x + NBX[d], my = y + NBY[d];
+        if (!unknown(s, F, mx, my)) continue;
+        int j = lat(F, mx, my);
+        float wd = v4c(w, d);
+        sum.x += wd * s->mgr[j].x / s->mgd[j];
+        sum.y += wd * s->mgr[j].y / s->mgd[j];
+    }
+    float D = s->mgd[i];
+    return (v2){sum.x / D, sum.y / D};
+}
+
+static v2 down_z(const or_sim* s, lvl F, int x, int y) {
+    if (!unknown(s, F, x, y)) return (v2){0, 0};
+    int i = lat(F, x, y);
+    if (((x + y) & 1) == 0) return (v2){s->mgr[i].x / s->mgd[i], s->mgr[i].y / s->mgd[i]};
+    return down_black(s, F, x, y);
+}
+
+/* mg_down.comp:139-172 */
+static void mg_down(or_sim* s, int l) {
+    lvl F = level_of(s, l), C = level_of(s, l + 1);
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) s->mgx[lat(F, x, y)] = down_z(s, F, x, y);
+    for (int cy = 0; cy < C.h; cy++)
+        for (int cx = 0; cx < C.w; cx++) {
+            v2 sum = {0, 0};
+            for (int k = 0; k < 2; k++) {
+                int qx = 2 * cx + k, qy = 2 * cy + k;
+                if (!unknown(s, F, qx, qy)) continue;
+                v4 w = s->mga[lat(F, qx, qy)];
+                for (int d = 0; d < 4; d++) {
+                    int nx = qx + NBX[d], ny = qy + NBY[d];
+                    if (!unknown(s, F, nx, ny)) continue;
+                    v2 zn = s->mgx[lat(F, nx, ny)];
+                    float wd = v4c(w, d);
+                    sum.x += wd * zn.x;
+                    sum.y += wd * zn.y;
+                }
+            }
+            s->mgr[lat(C, cx, cy)] = sum;
+        }
+}
+
+/* black cell of the way up (mg_up.comp:198-208) */
+static v2 up_black(const or_sim* s, lvl F, lvl C, int x, int y) {
+    int i = lat(F, x, y);
+    v4 w = s->mga[i];
+    v2 sum = s->mgr[i];
+    for (int d = 0; d < 4; d++) {
+        int mx = x + NBX[d], my = y + NBY[d];
+        if (!unknown(s, F, mx, my)) continue;
+        float wd = v4c(w, d);
+        v2 xm = s->mgx[lat(F, mx, my)];
+        v2 zc = s->mgz[lat(C, mx >> 1, my >> 1)];
+        sum.x += wd * (xm.x + SP_MG_OMEGA * zc.x);
+        sum.y += wd * (xm.y + SP_MG_OMEGA * zc.y);
+    }
+    float D = s->mgd[i];
+    return (v2){sum.x / D, sum.y / D};
+}
+
+/* mg_up.comp:210-247; on level 0 returns the r.z and r.r sums (FP64). */
+static void mg_up(or_sim* s, int l, double sums[4]) {
+    lvl F = level_of(s, l), C = level_of(s, l + 1);
+    /* the black cells first (their values only feed red neighbours and the output) */
+    v2* nb = (v2*)calloc((size_t)F.w * F.h, sizeof(v2));
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++)
+            if (unknown(s, F, x, y) && ((x + y) & 1)) nb[y * F.w + x] = up_black(s, F, C, x, y);
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            int i = lat(F, x, y);
+            v2 z = {0, 0};
+            if (unknown(s, F, x, y)) {
+                v2 r = s->mgr[i];
+                if (((x + y) & 1) == 0) {
+                    v4 w = s->mga[i];
+                    v2 sum = r;
+                    for (int d = 0; d < 4; d++) {
+                        int nx = x + NBX[d], ny = y + NBY[d];
+                        if (!unknown(s, F, nx, ny)) continue;
+                        float wd = v4c(w, d);
+                        v2 zn = nb[ny * F.w + nx];
+                        sum.x += wd * zn.x;
+                        sum.y += wd * zn.y;
+                    }
+                    float D = s->mgd[i];
+                    z.x = sum.x / D;
+                    z.y = sum.y / D;
+                } else {
+                    z = nb[y * F.w + x];
+                }
+                if (sums) {
+                    sums[0] += (double)r.x * (double)z.x;
+                    sums[1] += (double)r.y * (double)z.y;
+                    sums[2] += (double)r.x * (double)r.x;
+                    sums[3] += (double)r.y * (double)r.y;
+                }
+            }
+            s->mgz[i] = z;
+        }
+    free(nb);
+}
+
+/* mg_tail.comp mode 0 (mg_tail.comp:159-225): levels first..last in shared arrays. */
+static void mg_tail(or_sim* s) {
+    const sp_levels_info* L = &s->lv;
+    int first = L->tail, last = L->n - 1;
+    int BASE = L->offs[first];
+    int total = L->offs[last] + L->dims[last][0] * L->dims[last][1] - BASE;
+    int NF = L->dims[first][0] * L->dims[first][1];
+    v2* sx = (v2*)calloc((size_t)total, sizeof(v2));
+    float* sd = (float*)calloc((size_t)total, sizeof(float));
+    v2* sr = (v2*)calloc((size_t)(total > NF ? total - NF : 1), sizeof(v2));
+    for (int t = 0; t < total; t++) sd[t] = s->mgd[BASE + t];
+#define RHS_AT(sl) ((sl) < NF ? s->mgr[BASE + (sl)] : sr[(sl) - NF])
+#define SAT(Lv, x, y) ((Lv).off - BASE + (y) * (Lv).w + (x))
+#define SUNK(Lv, x, y) (inside(Lv, x, y) && sd[SAT(Lv, x, y)] > 0.0f)
+    /* one half-sweep of one colour; colours are independent within a half-sweep */
+#define SWEEP(Lv, colour, zero)                                                               \
+    do {                                                                                      \
+        for (int t = 0; t < (Lv).w * (Lv).h; t++) {                                           \
+            int x = t % (Lv).w, y = t / (Lv).w;                                               \
+            int sl = (Lv).off - BASE + t;                                                     \
+            float D = sd[sl];                                                                 \
+            if (D <= 0.0f) {                                                                  \
+                if (zero) sx[sl] = (v2){0, 0};                                                \
+                continue;                                                                     \
+            }                                                                                 \
+            if (((x + y) & 1) != (colour)) {                                                  \
+                if (zero) sx[sl] = (v2){0, 0};                                                \
+                continue;                                                                     \
+            }                                                                                 \
+            v2 sum = RHS_AT(sl);                                                              \
+            if (!(zero)) {                                                                    \
+                v4 w = s->mga[BASE + sl];                                                     \
+                for (int d = 0; d < 4; d++) {                                                 \
+                    int nx = x + NBX[d], ny = y + NBY[d];                                     \
+                    if (SUNK(Lv, nx, ny)) {                                                   \
+                        float wd = v4c(w, d);                                                 \
+                        v2 xn = sx[SAT(Lv, nx, ny)];                                          \
+                        sum.x += wd * xn.x;                                                   \
+                        sum.y += wd * xn.y;                                                   \
+                    }                                                                         \
+                }                                                                             \
+            }                                                                                 \
+            sx[sl] = (v2){sum.x / D, sum.y / D};                                              \
+        }                                                                                     \
+    } while (0)
+    for (int l = first; l < last; l++) {
+        lvl F = level_of(s, l), C = level_of(s, l + 1);
+        SWEEP(F, 0, 1);
+        SWEEP(F, 1, 0);
+        for (int t = 0; t < C.w * C.h; t++) {
+            int cx = t % C.w, cy = t / C.w;
+            v2 sum = {0, 0};
+            for (int k = 0; k < 4; k++) {
+                int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
+                if (!SUNK(F, x, y)) continue;
+                int sl = SAT(F, x, y);
+                v4 w = s->mga[BASE + sl];
+                v2 az = {sd[sl] * sx[sl].x, sd[sl] * sx[sl].y};
+                for (int d = 0; d < 4; d++) {
+                    int nx = x + NBX[d], ny = y + NBY[d];
+                    if (SUNK(F, nx, ny)) {
+                        float wd = v4c(w, d);
+                        v2 xn = sx[SAT(F, nx, ny)];
+                        az.x -= wd * xn.x;
+                        az.y -= wd * xn.y;
+                    }
+                }
+                v2 r = RHS_AT(sl);
+                sum.x += r.x - az.x;
+                sum.y += r.y - az.y;
+            }
+            sr[C.off - BASE + t - NF] = sum;
+        }
+    }
+    {
+        lvl E = level_of(s, last);
+        SWEEP(E, 0, 1);
+        for (int h = 1; h < SP_TAIL_HALF_SWEEPS; h++) SWEEP(E, h & 1, 0);
+    }
+    for (int l = last - 1; l >= first; l--) {
+        lvl F = level_of(s, l), C = level_of(s, l + 1);
+        /* coarse-grid correction fused with the black half-sweep (mg_tail.comp:141-157) */
+        for (int t = 0; t < F.w * F.h; t++) {
+            int x = t % F.w, y = t / F.w;
+            if (((x + y) & 1) == 0) continue;
+            int sl = SAT(F, x, y);
+            float D = sd[sl];
+            if (D <= 0.0f) continue;
+            v4 w = s->mga[BASE + sl];
+            v2 sum = RHS_AT(sl);
+            for (int d = 0; d < 4; d++) {
+                int nx = x + NBX[d], ny = y + NBY[d];
+                if (!SUNK(F, nx, ny)) continue;
+                float wd = v4c(w, d);
+                v2 xn = sx[SAT(F, nx, ny)];
+                v2 zc = sx[SAT(C, nx >> 1, ny >> 1)];
+                sum.x += wd * (xn.x + SP_MG_OMEGA * zc.x);
+                sum.y += wd * (xn.y + SP_MG_OMEGA * zc.y);
+            }
+            sx[sl] = (v2){sum.x / D, sum.y / D};
+        }
+        SWEEP(F, 0, 0);
+    }
+    for (int t = 0; t < NF; t++) s->mgz[BASE + t] = sx[t];
+#undef RHS_AT
+#undef SAT
+#undef SUNK
+#undef SWEEP
+    free(sx);
+    free(sd);
+    free(sr);
+}
+
+static void vcycle(or_sim* s, double sums[4]) {
+    for (int l = 0; l < s->lv.tail; l++) mg_down(s, l);
+    mg_tail(s);
+    for (int l = s->lv.tail - 1; l >= 0; l--) mg_up(s, l, l == 0 ? sums : NULL);
+}
+
+/* cg_scale.comp:14-27 (mode 0) */
+static void cg_scale(or_sim* s) {
+    lvl F = level_of(s, 0);
+    int tx = (F.w + 15) >> 4;
+    float ratio = s->S.dt / s->S.dt_prev;
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            if (s->ptile[(y >> 4) * tx + (x >> 4)] == 0u) continue;
+            int i = lat(F, x, y);
+            if (s->mgd[i] > 0.0f) {
+                float p = s->pres[i].x;
+                s->p2keep[i] = 0.0f;
+                s->pres[i].x = p * ratio;
+            } else {
+                s->pres[i] = (v2){0, 0};
+            }
+        }
+}
+
+/* cg_init.comp:38-64: r = b - A x, and b.b */
+static void cg_init(or_sim* s, double bb[2]) {
+    lvl F = level_of(s, 0);
+    bb[0] = bb[1] = 0.0;
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            int i = lat(F, x, y);
+            float D = s->mgd[i];
+            if (D > 0.0f) {
+                v2 xv = s->pres[i];
+                v4 w = s->mga[i];
+                v2 ax = {D * xv.x, D * xv.y};
+                for (int d = 0; d < 4; d++) {
+                    int nx = x + NBX[d], ny = y + NBY[d];
+                    if (unknown(s, F, nx, ny)) {
+                        float wd = v4c(w, d);
+                        v2 pn = s->pres[lat(F, nx, ny)];
+                        ax.x -= wd * pn.x;
+                        ax.y -= wd * pn.y;
+                    }
+                }
+                v4 b = s->prhs[i];
+                s->mgr[i] = (v2){b.x - ax.x, b.y - ax.y};
+                bb[0] += (double)b.x * (double)b.x;
+                bb[1] += (double)b.y * (double)b.y;
+            } else {
+                s->mgr[i] = (v2){0, 0};
+            }
+        }
+}
+
+static v2 direction(const or_sim* s, int i, float bx, float by) {
+    v2 z = s->mgz[i];
+    if (bx == 0.0f && by == 0.0f) return z;
+    v2 po = s->cgp[i];
+    return (v2){bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y};
+}
+
+static void stage_pressure(or_sim* s) {
+    const sp_gparams* P = &s->P;
+    sp_solve* K = &s->K;
+    lvl F = level_of(s, 0);
+    int N = F.w * F.h;
+    stage_mg_build(s);
+    cg_scale(s);
+    double bb[2];
+    cg_init(s, bb);
+    double a[4] = {0, 0, 0, 0};
+    vcycle(s, a);
+    /* cg_reduce mode 0 (reduce_tail.glsl:61-78) */
+    double tol2x = (double)P->tol[0] * (double)P->tol[0], tol2y = (double)P->tol[1] * (double)P->tol[1];
+    K->bb[0] = bb[0];
+    K->bb[1] = bb[1];
+    K->rz[0] = a[0];
+    K->rz[1] = a[1];
+    K->rr[0] = a[2];
+    K->rr[1] = a[3];
+    K->beta[0] = K->beta[1] = 0.0;
+    K->alpha[0] = K->alpha[1] = 0.0;
+    K->iters[0] = K->iters[1] = 0;
+    K->done[0] = a[2] <= tol2x * bb[0];
+    K->done[1] = a[3] <= tol2y * bb[1];
+    K->solves += 1;
+    v2* np = (v2*)malloc(sizeof(v2) * (size_t)N);
+    uint32_t k = 0;
+    for (; k < (uint32_t)P->p_max_iters && !(K->done[0] && K->done[1]); k++) {
+        /* cg_apply.comp:87-115, alpha (reduce_tail.glsl:79-87) */
+        float bx = (float)K->beta[0], by = (float)K->beta[1];
+        double pq[2] = {0, 0};
+        for (int y = 0; y < F.h; y++)
+            for (int x = 0; x < F.w; x++) {
+                int i = lat(F, x, y);
+                float D = s->mgd[i];
+                if (D > 0.0f) {
+                    v2 p = direction(s, i, bx, by);
+                    v4 w = s->mga[i];
+                    v2 q = {D * p.x, D * p.y};
+                    for (int d = 0; d < 4; d++) {
+                        int nx = x + NBX[d], ny = y + NBY[d];
+                        if (unknown(s, F, nx, ny)) {
+                            float wd = v4c(w, d);
+                            v2 pn = direction(s, lat(F, nx, ny), bx, by);
+                            q.x -= wd * pn.x;
+                            q.y -= wd * pn.y;
+                        }
+                    }
+                    np[i] = p;
+                    s->cgq[i] = q;
+                    pq[0] += (double)p.x * (double)q.x;
+                    pq[1] += (double)p.y * (double)q.y;
+                } else {
+                    np[i] = (v2){0, 0};
+                }
+            }
+        memcpy(s->cgp, np, sizeof(v2) * (size_t)N);
+        K->alpha[0] = (K->done[0] == 0 && pq[0] > 0.0) ? K->rz[0] / pq[0] : 0.0;
+        K->alpha[1] = (K->done[1] == 0 && pq[1] > 0.0) ? K->rz[1] / pq[1] : 0.0;
+        /* cg_update.comp:126-135 */
+        float ax = (float)K->alpha[0], ay = (float)K->alpha[1];
+        for (int i = 0; i < N; i++) {
+            if (s->mgd[i] <= 0.0f) continue;
+            s->pres[i].x += ax * s->cgp[i].x;
+            s->pres[i].y += ay * s->cgp[i].y;
+            s->mgr[i].x -= ax * s->cgq[i].x;
+            s->mgr[i].y -= ay * s->cgq[i].y;
+        }
+        /* V-cycle, then cg_reduce mode 2 (reduce_tail.glsl:88-104) */
+        double r[4] = {0, 0, 0, 0};
+        vcycle(s, r);
+        uint32_t was0 = K->done[0], was1 = K->done[1];
+        K->iters[0] += 1u - was0;
+        K->iters[1] += 1u - was1;
+        uint32_t now0 = r[2] <= tol2x * K->bb[0], now1 = r[3] <= tol2y * K->bb[1];
+        uint32_t d0 = was0 | now0, d1 = was1 | now1;
+        K->beta[0] = (d0 == 0 && K->rz[0] > 0.0) ? r[0] / K->rz[0] : 0.0;
+        K->beta[1] = (d1 == 0 && K->rz[1] > 0.0) ? r[1] / K->rz[1] : 0.0;
+        K->rz[0] = r[0];
+        K->rz[1] = r[1];
+        if (!was0) K->rr[0] = r[2];
+        if (!was1) K->rr[1] = r[3];
+        K->done[0] = d0;
+        K->done[1] = d1;
+    }
+    free(np);
+    /* cg_reduce mode 3 (reduce_tail.glsl:105-116) */
+    uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];
+    if (it > K->it_max) K->it_max = it;
+    if (K->done[0] == 0) K->unconv += 1;
+    float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
+    if (rel > K->res_max) K->res_max = rel;
+    K->iters_last = k;
+}
+
+/* ---- project.comp (project.comp:26-58) ------------------------------------------------ */
+
+static float difference(int lc, int ln, float pc, float pn, float j) {
+    if (lc && ln) return pc - pn - j;
+    return (lc ? pc : j) - (ln ? pn : j);
+}
+
+static void stage_project(or_sim* s) {
+    float m = u2f(s->S.umax_grid);
+    for (int cy = 1; cy < s->H; cy++)
+        for (int cx = 1; cx < s->W; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 cf = s->coef[i];
+            v4 me = s->vel[i];
+            if (s->mgd[i] > 0.0f) s->prhs[i].z = s->pres[i].x;
+            if (cf.x != 0.0f || cf.y != 0.0f) {
+                v2 p = s->pres[i];
+                int liq = s->cell[i].w > 0.0f;
+                if (face_live(me.z) && cf.x != 0.0f) {
+                    int n = i - 1;
+                    me.x -= cf.x * difference(liq, s->cell[n].w > 0.0f, p.x, s->pres[n].x, s->pjump[i].x);
+                    me.z = -cf.x * (p.y - s->pres[n].y);
+                }
+                if (face_live(me.w) && cf.y != 0.0f) {
+                    int n = i - s->W;
+                    me.y -= cf.y * difference(liq, s->cell[n].w > 0.0f, p.x, s->pres[n].x, s->pjump[i].y);
+                    me.w = -cf.y * (p.y - s->pres[n].y);
+                }
+                s->vel[i] = me;
+            }
+            float mm = 0.0f;
+            if (face_live(me.z) && cf.x != 0.0f) mm = fabsf(me.x);
+            if (face_live(me.w) && cf.y != 0.0f) mm = fmaxf_(mm, fabsf(me.y));
+            if (mm > m) m = mm;
+        }
+    s->S.umax_grid = f2u(m);
+}
+
+/* ---- public entry points -------------------------------------------------------------- */
+
+void or_stage(or_sim* s, int stage, int arg) {
+    switch (stage) {
+    case SP_STAGE_STEP: stage_step(s); break;
+    case SP_STAGE_CLEAR_ACC: stage_clear_acc(s); break;
+    case SP_STAGE_PARTICLES: stage_particles(s, arg); break;
+    case SP_STAGE_GRID_PREP: stage_grid_prep(s); break;
+    case SP_STAGE_COEF: stage_coef(s); break;
+    case SP_STAGE_SURF:
+        stage_surf_smooth(s);
+        stage_surf_curv(s);
+        stage_surf_apply(s);
+        break;
+    case SP_STAGE_VISC:
+        if (s->P.visc_mode == 1) stage_visc_explicit(s);
+        break;
+    case SP_STAGE_RHS: stage_rhs(s); break;
+    case SP_STAGE_PRESSURE: stage_pressure(s); break;
+    case SP_STAGE_PROJECT: stage_project(s); break;
+    case SP_STAGE_SUBSTEP: or_substep(s, arg); break;
+    case SP_STAGE_MG_BUILD: stage_mg_build(s); break;
+    case SP_STAGE_VCYCLE: {
+        double a[4] = {0, 0, 0, 0};
+        vcycle(s, a);
+        break;
+    }
+    default: break;
+    }
+}
+
+void or_substep(or_sim* s, int gather) {
+    stage_step(s);
+    stage_clear_acc(s);
+    stage_particles(s, gather);
+    stage_grid_prep(s);
+    stage_coef(s);
+    stage_surf_smooth(s);
+    stage_surf_curv(s);
+    stage_surf_apply(s);
+    if (s->P.visc_mode == 1) stage_visc_explicit(s);
+    stage_rhs(s);
+    stage_pressure(s);
+    stage_project(s);
+}
+
+or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
+    or_sim* s = (or_sim*)calloc(1, sizeof(or_sim));
+    s->W = W;
+    s->H = H;
+    s->N = W * H;
+    s->cap = capacity;
+    size_t N = (size_t)s->N;
+    s->part = (sp_particle*)calloc(capacity ? capacity : 1, sizeof(sp_particle));
+    s->acc = (int32_t*)calloc(N * SP_ACC_MAX, sizeof(int32_t));
+    v4** f4[] = {&s->vel, &s->cell, &s->comp, &s->coef, &s->surf, &s->surf2, &s->sprop, &s->aux, &s->grho, &s->gbase, &s->solid, &s->prhs, &s->vw, &s->vs};
+    for (size_t k = 0; k < sizeof(f4) / sizeof(f4[0]); k++) *f4[k] = (v4*)calloc(N, sizeof(v4));
+    s->vel0 = (v2*)calloc(N, sizeof(v2));
+    s->pjump = (v2*)calloc(N, sizeof(v2));
+    s->pres = (v2*)calloc(N, sizeof(v2));
+    s->sdf = (float*)calloc((size_t)(W + 1) * (H + 1), sizeof(float));
+    s->mixb = (float*)calloc(N * SP_MAX_PRESENT, sizeof(float));
+    s->p2keep = (float*)calloc(N, sizeof(float));
+    s->region = (uint32_t*)calloc(N, sizeof(uint32_t));
+    or_levels(W, H, &s->lv);
+    size_t T = (size_t)s->lv.total;
+    s->mga = (v4*)calloc(T, sizeof(v4));
+    s->mgd = (float*)calloc(T, sizeof(float));
+    s->mgx = (v2*)calloc(T, sizeof(v2));
+    s->mgz = (v2*)calloc(T, sizeof(v2));
+    s->mgr = (v2*)calloc(T, sizeof(v2));
+    s->cgp = (v2*)calloc(N, sizeof(v2));
+    s->cgq = (v2*)calloc(N, sizeof(v2));
+    s->ptile = (uint32_t*)calloc((size_t)s->lv.tiles, sizeof(uint32_t));
+    return s;
+}
+
+void or_destroy(or_sim* s) {
+    if (!s) return;
+    free(s->part);
+    free(s->acc);
+    v4* f4[] = {s->vel, s->cell, s->comp, s->coef, s->surf, s->surf2, s->sprop, s->aux, s->grho, s->gbase, s->solid, s->prhs, s->vw, s->vs};
+    for (size_t k = 0; k < sizeof(f4) / sizeof(f4[0]); k++) free(f4[k]);
+    free(s->vel0);
+    free(s->pjump);
+    free(s->pres);
+    free(s->sdf);
+    free(s->mixb);
+    free(s->p2keep);
+    free(s->region);
+    free(s->mga);
+    free(s->mgd);
+    free(s->mgx);
+    free(s->mgz);
+    free(s->mgr);
+    free(s->cgp);
+    free(s->cgq);
+    free(s->ptile);
+    free(s);
+}
+
+void or_set_gparams(or_sim* s, const sp_gparams* P, const sp_gmat* mats) {
+    s->P = *P;
+    memcpy(s->mats, mats, sizeof(sp_gmat) * SP_MAX_MATERIALS);
+}
+
+void or_set_solids(or_sim* s, const float* sdf, const float* solid, const uint32_t* region) {
+    memcpy(s->sdf, sdf, sizeof(float) * (size_t)(s->W + 1) * (s->H + 1));
+    memcpy(s->solid, solid, sizeof(v4) * (size_t)s->N);
+    memcpy(s->region, region, sizeof(uint32_t) * (size_t)s->N);
+}
+
+void or_reset(or_sim* s, const sp_particle* p, uint32_t n) {
+    if (n > s->cap) n = s->cap;
+    memcpy(s->part, p, sizeof(sp_particle) * n);
+    s->n_slots = n;
+    s->P.n_slots = n;
+    memset(s->pres, 0, sizeof(v2) * (size_t)s->N);
+    memset(s->p2keep, 0, sizeof(float) * (size_t)s->N);
+    memset(&s->S, 0, sizeof(s->S));
+}
+
+void or_begin_batch(or_sim* s, float target, uint32_t substeps) {
+    s->S.frame_left = target > 0.0f ? target : 0.0f;
+    s->S.batch_steps = substeps;
+    s->S.batch_done = 0;
+}
+
+uint32_t or_n_slots(or_sim* s) { return s->n_slots; }
+uint32_t or_last_iters(or_sim* s) { return s->K.iters_last; }
+
+static void* field_ptr(or_sim* s, int field, size_t* bytes) {
+    size_t N = (size_t)s->N, T = (size_t)s->lv.total;
+    switch (field) {
+    case SP_FIELD_PARTICLES: *bytes = sizeof(sp_particle) * s->n_slots; return s->part;
+    case SP_FIELD_ACC: *bytes = sizeof(int32_t) * N * s->P.acc_stride; return s->acc;
+    case SP_FIELD_VEL: *bytes = sizeof(v4) * N; return s->vel;
+    case SP_FIELD_CELL: *bytes = sizeof(v4) * N; return s->cell;
+    case SP_FIELD_COMP: *bytes = sizeof(v4) * N; return s->comp;
+    case SP_FIELD_COEF: *bytes = sizeof(v4) * N; return s->coef;
+    case SP_FIELD_SURF: *bytes = sizeof(v4) * N; return s->surf;
+    case SP_FIELD_PJUMP: *bytes = sizeof(v2) * N; return s->pjump;
+    case SP_FIELD_MGA: *bytes = sizeof(v4) * T; return s->mga;
+    case SP_FIELD_MGD: *bytes = sizeof(float) * T; return s->mgd;
+    case SP_FIELD_PRHS: *bytes = sizeof(v4) * N; return s->prhs;
+    case SP_FIELD_PRES: *bytes = sizeof(v2) * N; return s->pres;
+    case SP_FIELD_STEP: *bytes = sizeof(sp_stepbuf); return &s->S;
+    case SP_FIELD_SOLVE: *bytes = sizeof(sp_solve); return &s->K;
+    case SP_FIELD_SPROP: *bytes = sizeof(v4) * N; return s->sprop;
+    case SP_FIELD_AUX: *bytes = sizeof(v4) * N; return s->aux;
+    case SP_FIELD_GRHO: *bytes = sizeof(v4) * N; return s->grho;
+    case SP_FIELD_VEL0: *bytes = sizeof(v2) * N; return s->vel0;
+    case SP_FIELD_MGZ: *bytes = sizeof(v2) * T; return s->mgz;
+    case SP_FIELD_MGR: *bytes = sizeof(v2) * T; return s->mgr;
+    case SP_FIELD_PTILE: *bytes = sizeof(uint32_t) * (size_t)s->lv.tiles; return s->ptile;
+    case SP_FIELD_REGION_AIR: *bytes = sizeof(uint32_t); return &s->region_air;
+    case SP_FIELD_SURF2: *bytes = sizeof(v4) * N; return s->surf2;
+    case SP_FIELD_VW: *bytes = sizeof(v4) * N; return s->vw;
+    case SP_FIELD_VS: *bytes = sizeof(v4) * N; return s->vs;
+    case SP_FIELD_GBASE: *bytes = sizeof(v4) * N; return s->gbase;
+    default: *bytes = 0; return NULL;
+    }
+}
+
+long or_get(or_sim* s, int field, void* out, size_t bytes) {
+    size_t n;
+    void* p = field_ptr(s, field, &n);
+    if (!p || bytes < n) return -1;
+    memcpy(out, p, n);
+    return (long)n;
+}
+
+long or_put(or_sim* s, int field, const void* in, size_t bytes) {
+    size_t n;
+    void* p = field_ptr(s, field, &n);
+    if (field == SP_FIELD_PARTICLES) {
+        n = bytes < sizeof(sp_particle) * s->cap ? bytes : sizeof(sp_particle) * s->cap;
+        s->n_slots = (uint32_t)(n / sizeof(sp_particle));
+        s->P.n_slots = s->n_slots;
+    }
+    if (!p || bytes < n) return -1;
+    memcpy(p, in, n);
+    return (long)n;
+}
+
+/* sort.comp with a defined order: by cell_of(pos), stable by slot; dead slots dropped to
+ * the end (sort.comp:28-68). */
+void or_sort(or_sim* s) {
+    size_t N = (size_t)s->N;
+    uint32_t* count = (uint32_t*)calloc(N + 1, sizeof(uint32_t));
+    int* cellof = (int*)malloc(sizeof(int) * (s->n_slots ? s->n_slots : 1));
+    for (uint32_t id = 0; id < s->n_slots; id++) {
+        cellof[id] = -1;
+        if (s->part[id].kind == SP_KIND_DEAD) continue;
+        int x = clampi((int)s->part[id].pos[0], 0, s->W - 1), y = clampi((int)s->part[id].pos[1], 0, s->H - 1);
+        cellof[id] = cidx(s, x, y);
+        count[cellof[id] + 1]++;
+    }
+    for (size_t c = 0; c < N; c++) count[c + 1] += count[c];
+    sp_particle* out = (sp_particle*)calloc(s->cap ? s->cap : 1, sizeof(sp_particle));
+    uint32_t live = count[N];
+    for (uint32_t id = 0; id < s->n_slots; id++)
+        if (cellof[id] >= 0) out[count[cellof[id]]++] = s->part[id];
+    /* dead slots keep their (dead) contents at the end */
+    uint32_t k = live;
+    for (uint32_t id = 0; id < s->n_slots; id++)
+        if (cellof[id] < 0) out[k++] = s->part[id];
+    memcpy(s->part, out, sizeof(sp_particle) * s->n_slots);
+    free(out);
+    free(cellof);
+    free(count);
+}
+
+void or_cell_counts(or_sim* s, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out) {
+    memset(out, 0, sizeof(uint32_t) * (size_t)(w > 0 && h > 0 ? w * h : 0));
+    for (uint32_t id = 0; id < s->n_slots; id++) {
+        if ((s->part[id].kind & 255u) == 0u) continue;
+        int x = (int)floorf(s->part[id].pos[0]) - x0, y = (int)floorf(s->part[id].pos[1]) - y0;
+        if (x < 0 || y < 0 || x >= w || y >= h) continue;
+        out[y * w + x]++;
+    }
+}
+
+void or_stats(or_sim* s, uint32_t* live, uint32_t* count, uint32_t* conc256) {
+    *live = 0;
+    memset(count, 0, sizeof(uint32_t) * SP_MAX_MATERIALS);
+    memset(conc256, 0, sizeof(uint32_t) * SP_MAX_MATERIALS);
+    for (uint32_t id = 0; id < s->n_slots; id++) {
+        uint32_t kind = s->part[id].kind & 255u;
+        if (kind == 0u) continue;
+        (*live)++;
+        count[kind - 1]++;
+        conc256[kind - 1] += (uint32_t)(clampf(s->part[id].s, 0.0f, 1.0f) * 256.0f + 0.5f);
+    }
+}

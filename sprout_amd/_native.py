@@ -1,0 +1,219 @@
+"""ctypes view of the C ABI in include/sprout_hip.h (libsprout_hip.so).
+
+The library is built in-tree (sprout_amd/libsprout_hip.so) by sprout_amd.build; loading it
+fails loudly when it is missing: there is no CPU fallback for the fluid step.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsprout_hip.so")
+
+GROUPS = 4
+MAX_PRESENT = 12
+MAX_MATERIALS = 96
+MAX_LEVELS = 16
+PPC = 4
+SPAWN_MAX = 32768
+
+KIND_DEAD = 0
+KIND_FRESH = 0x10000
+KIND_REAL_T = 0x20000
+SPAWN_TEMP = -1.0e4
+SPAWN_S = -1.0
+
+ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
+
+# Status codes
+OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY = 0, -1, -2, -3, -4, -5
+
+# Field ids (SP_FIELD_*)
+FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7, MGA=8, MGD=9,
+             PRHS=10, PRES=11, STEP=12, SOLVE=13, SPROP=14, AUX=15, GRHO=16, VEL0=17, MGZ=18,
+             MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25)
+# Stage ids (SP_STAGE_*)
+STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
+             PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12)
+
+# `struct Particle` (common.glsl:11-19): 48 bytes.
+PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
+                           ("s", "<f4"), ("kind", "<u4"), ("id", "<u4"), ("t", "<f4")])
+assert PARTICLE_DTYPE.itemsize == 48
+
+
+class Material(C.Structure):
+    _fields_ = [("group", C.c_int32), ("diluent", C.c_int32), ("density", C.c_float),
+                ("density_solid", C.c_float), ("viscosity", C.c_float), ("temp_model", C.c_int32),
+                ("temp_b", C.c_float), ("temp_c", C.c_float), ("temp_ref", C.c_float),
+                ("tension", C.c_float), ("contact_angle", C.c_float), ("mix_rate", C.c_float),
+                ("expansion", C.c_float), ("melt_lo", C.c_float), ("melt_hi", C.c_float),
+                ("visc_exponent", C.c_float), ("load", C.c_float), ("spawn_temp", C.c_float),
+                ("latent", C.c_float), ("heat_capacity", C.c_float)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("cell_size", C.c_float), ("gravity", C.c_float), ("ambient_temp", C.c_float),
+                ("rigid_viscosity", C.c_float), ("pair_tension", (C.c_float * GROUPS) * GROUPS),
+                ("droplet_cells", C.c_float), ("cfl", C.c_float), ("advection_order", C.c_uint32),
+                ("dt_max", C.c_float), ("fixed_hz", C.c_float), ("pressure_tol", C.c_float),
+                ("pressure_max_iters", C.c_uint32), ("correction_tol", C.c_float),
+                ("viscosity_tol", C.c_float), ("viscosity_max_iters", C.c_uint32),
+                ("implicit_viscosity_min", C.c_float), ("spray_cfl", C.c_float),
+                ("speed_cap", C.c_float), ("correction_time", C.c_float),
+                ("correction_max", C.c_float), ("correction_deadband", C.c_float),
+                ("correction_step_max", C.c_float), ("crowd_jitter", C.c_float),
+                ("tool_crowd_max", C.c_float), ("apic_clamp", C.c_float),
+                ("curvature_max", C.c_float), ("slip_cap", C.c_uint32), ("sealed_vent", C.c_uint32),
+                ("min_substep_share", C.c_float), ("mat_count", C.c_int32),
+                ("mat", Material * MAX_MATERIALS)]
+
+
+class ToolState(C.Structure):
+    _fields_ = [("pos", C.c_float * 2), ("vel", C.c_float * 2), ("radius", C.c_float),
+                ("mode", C.c_int32), ("strength", C.c_float)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("substeps", C.c_uint64), ("pressure_unconverged", C.c_uint64),
+                ("viscosity_unconverged", C.c_uint64), ("pressure_iters_max", C.c_uint32),
+                ("viscosity_iters_max", C.c_uint32), ("pressure_residual_max", C.c_float),
+                ("viscosity_residual_max", C.c_float), ("dt", C.c_float), ("dt_cfl", C.c_float),
+                ("umax", C.c_float), ("umax_grid", C.c_float), ("umax_part", C.c_float),
+                ("pressure_iters_last", C.c_uint32)]
+
+
+class StatsOut(C.Structure):
+    _fields_ = [("live", C.c_uint32), ("count", C.c_uint32 * MAX_MATERIALS),
+                ("conc", C.c_float * MAX_MATERIALS)]
+
+
+class GMat(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("group", "diluent", "chan_w", "chan_s", "fam", "rheo",
+                                          "temp_model", "residue")] + \
+               [(n, C.c_float) for n in ("rho", "ln_nu", "sigma", "mix_rate", "cos_c", "sin_c",
+                                          "grain", "rho_solid", "visc_exp", "expansion", "load",
+                                          "pad")]
+
+
+class GParams(C.Structure):
+    _fields_ = [("W", C.c_int32), ("H", C.c_int32), ("n_slots", C.c_uint32),
+                ("spawn_count", C.c_uint32), ("dt_fixed", C.c_float), ("gravity", C.c_float),
+                ("inv_ppc", C.c_float), ("apic_clamp", C.c_float),
+                ("pair", (C.c_float * GROUPS) * GROUPS), ("kappa_max", C.c_float),
+                ("sealed_vent", C.c_float), ("cell_size", C.c_float), ("rigid_friction", C.c_float),
+                ("n_present", C.c_int32), ("acc_stride", C.c_int32), ("n_fam", C.c_int32),
+                ("group_mask", C.c_uint32), ("caps", C.c_uint32), ("primary", C.c_int32),
+                ("group_heat", C.c_uint32), ("visc_mode", C.c_int32),
+                ("present", C.c_int32 * MAX_PRESENT), ("fams", (C.c_int32 * 2) * MAX_PRESENT),
+                ("sep_tau", C.c_float), ("cfl", C.c_float), ("tool_crowd_max", C.c_float),
+                ("density_k", C.c_float), ("dt_max", C.c_float), ("crowd_jitter", C.c_float),
+                ("ext_speed", C.c_float), ("wall_margin", C.c_float), ("tol", C.c_float * 4),
+                ("ext", C.c_float * 4), ("corr", C.c_float * 4), ("p_max_iters", C.c_int32),
+                ("v_max_iters", C.c_int32), ("heat_x", C.c_float), ("ln_rigid", C.c_float)]
+
+
+class LevelsInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("tail", C.c_int32), ("total", C.c_int32), ("tiles", C.c_int32),
+                ("dims", (C.c_int32 * 2) * MAX_LEVELS), ("offs", C.c_int32 * MAX_LEVELS),
+                ("tile_offs", C.c_int32 * MAX_LEVELS)]
+
+
+class Config(C.Structure):
+    _fields_ = [("grid_w", C.c_int32), ("grid_h", C.c_int32), ("capacity", C.c_uint32),
+                ("device", C.c_int32)]
+
+
+STEP_DTYPE = np.dtype([("frame_left", "<f4"), ("advanced", "<f4"), ("batch_steps", "<u4"),
+                       ("batch_done", "<u4"), ("dt", "<f4"), ("dt_prev", "<f4"),
+                       ("umax_grid", "<u4"), ("umax_part", "<u4"), ("dt_cfl", "<f4"),
+                       ("substeps", "<u4"), ("umax_seen", "<f4"), ("pad0", "<u4")])
+SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<f8", (2,)),
+                        ("bb", "<f8", (2,)), ("rr", "<f8", (2,)), ("done", "<u4", (2,)),
+                        ("iters", "<u4", (2,)), ("it_max", "<u4"), ("unconv", "<u4"),
+                        ("solves", "<u4"), ("arrive", "<u4"), ("res_max", "<f4"),
+                        ("iters_last", "<u4"), ("pad", "<u4", (2,))])
+
+# Exported symbols of include/sprout_hip.h, with their argument types.
+_P = C.c_void_p
+SIGNATURES = {
+    "sp_create": ([C.POINTER(Config), C.POINTER(C.c_void_p)], C.c_int),
+    "sp_destroy": ([_P], None),
+    "sp_last_error": ([_P], C.c_char_p),
+    "sp_create_error": ([], C.c_char_p),
+    "sp_set_params": ([_P, C.POINTER(SimParams)], C.c_int),
+    "sp_set_solids": ([_P, _P, _P, _P, _P], C.c_int),
+    "sp_reset": ([_P, _P, C.c_uint32], C.c_int),
+    "sp_spawn": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_record": ([_P, C.c_uint32, C.c_float, C.POINTER(ToolState)], C.c_int),
+    "sp_wait": ([_P], C.c_int),
+    "sp_poll": ([_P, C.POINTER(SolverStats)], C.c_int),
+    "sp_advance_blocking": ([_P, C.c_double], C.c_int),
+    "sp_step_blocking": ([_P, C.c_uint32], C.c_int),
+    "sp_sim_time": ([_P, C.POINTER(C.c_double)], C.c_int),
+    "sp_download_particles": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_download_grid": ([_P, _P, _P], C.c_int),
+    "sp_download_pressure": ([_P, _P, C.POINTER(C.c_float)], C.c_int),
+    "sp_cell_counts": ([_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P], C.c_int),
+    "sp_stats": ([_P, C.POINTER(StatsOut)], C.c_int),
+    "sp_sort": ([_P], C.c_int),
+    "sp_default_params": ([C.POINTER(SimParams)], None),
+    "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32,
+                               C.POINTER(GParams), _P], C.c_int),
+    "sp_levels": ([C.c_int32, C.c_int32, C.POINTER(LevelsInfo)], C.c_int),
+    "sp_debug_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "sp_debug_upload": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "sp_debug_stage": ([_P, C.c_int32, C.c_int32], C.c_int),
+    "sp_debug_gparams": ([_P, C.POINTER(GParams), _P], C.c_int),
+    "sp_timings": ([_P, _P], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libsprout_hip.so. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run sprout_amd.build.build() (hipcc, gfx950) first; "
+                               "the fluid step has no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+class SproutError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sprout_hip error {code}: {msg}")
+        self.code = code
+
+
+def default_params():
+    p = SimParams()
+    lib().sp_default_params(C.byref(p))
+    return p
+
+
+def derive_grid_params(params, W, H, rows):
+    """sp_derive_grid_params: the grid-unit block and material table for a tank of `rows`."""
+    g = GParams()
+    mats = (GMat * MAX_MATERIALS)()
+    arr = (C.c_int32 * max(len(rows), 1))(*rows)
+    rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), C.byref(g), mats)
+    if rc != OK:
+        raise SproutError(rc, "derive_grid_params")
+    return g, mats
+
+
+def levels(W, H):
+    L = LevelsInfo()
+    rc = lib().sp_levels(W, H, C.byref(L))
+    if rc != OK:
+        raise SproutError(rc, "levels")
+    return L

@@ -1,0 +1,115 @@
+// dev.h — device-side helpers of the MI355X fluid step (gfx950, wave64).
+//
+// Float arithmetic follows the reference shaders' operation order and is compiled with
+// -ffp-contract=off, so that quantities the reference makes order-independent (the
+// fixed-point scatter sums, cell indices) come out bit-identical to a CPU evaluation of the
+// same inputs. GLSL built-ins are spelled out with the semantics used throughout this
+// build: mix(x, y, a) = x (1 - a) + y a, length = sqrt(x^2 + y^2), normalize = v / length.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sprout_types.h"
+
+#define SP_DEV static __device__ __forceinline__
+
+// Device pointers of one tank, passed by value to every kernel.
+struct Ctx {
+    int W, H, N;
+    const sp_gparams* P;
+    const sp_gmat* mats;
+    sp_stepbuf* S;
+    sp_solve* K;
+    sp_particle* part;
+    int32_t* acc;
+    float4 *vel, *cell, *comp, *coef, *surf, *surf2, *sprop, *aux, *grho, *gbase, *prhs, *vw, *vs;
+    const float4* solid;
+    float2 *vel0, *pjump, *pres;
+    const float* sdf;
+    float *mixb, *p2keep;
+    const uint32_t* region;
+    uint32_t* region_air;
+    float4* mga;
+    float* mgd;
+    float2 *mgx, *mgz, *mgr, *cgp, *cgp2, *cgq;
+    uint32_t* ptile;
+    double4* part_sums;   // per-block partial sums of a reduction
+    const int4* lev;      // per level: offset, width, height, tile offset
+    int n_levels, tail;
+};
+
+SP_DEV float fmaxf_(float a, float b) { return a > b ? a : b; }
+SP_DEV float fminf_(float a, float b) { return a < b ? a : b; }
+SP_DEV float clampf_(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }
+SP_DEV int clampi_(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+SP_DEV float mixf_(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+SP_DEV float len2_(float x, float y) { return __fsqrt_rn(x * x + y * y); }
+SP_DEV float signf_(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+SP_DEV float div_(float a, float b) { return __fdiv_rn(a, b); }
+SP_DEV bool face_live(float s) { return fabsf(s) < 1e29f; }
+SP_DEV bool face_closed(float s) { return s < -1e29f; }
+SP_DEV float f4c(const float4& a, int k) { return k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w; }
+SP_DEV float smoothstep_(float e0, float e1, float x) {
+    float t = clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// Integer hash of the per-particle random numbers (air.glsl:4-11).
+SP_DEV uint32_t hash_u(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Quadratic B-spline weights at fractional offset f in [0.5, 1.5) (common.glsl:638-643).
+SP_DEV void bspline3(float f, float w[3]) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+}
+
+SP_DEV int floori(float x) { return (int)floorf(x); }
+
+// Wave64 reductions.
+SP_DEV float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf_(v, __shfl_xor(v, o, 64));
+    return v;
+}
+SP_DEV int wave_min_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+SP_DEV int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// atomicMax on the bits of a non-negative float.
+SP_DEV void atomic_max_pos_float(uint32_t* addr, float v) {
+    if (v > 0.0f) atomicMax(addr, __float_as_uint(v));
+}
+
+// Whether phase group g sorts itself by density below the grid scale (common.glsl:590-602).
+SP_DEV bool group_sorts(const sp_gparams& P, const sp_gmat* mats, int g) {
+    if (g <= 0) return false;
+    int n = 0;
+    for (int j = 0; j < P.n_present; j++) {
+        int r = P.present[j];
+        const sp_gmat& m = mats[r];
+        if (m.diluent >= 0 && mats[m.diluent].group == g) return false;
+        if (m.group != g) continue;
+        if (m.diluent >= 0) return false;
+        n++;
+    }
+    return n >= 2;
+}

@@ -1,0 +1,32 @@
+// host.h — host-side declarations of the MI355X fluid step library.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../../include/sprout_hip.h"
+
+namespace sp {
+
+// What the tank holds and the accumulator channel of each row (Tank, mod.rs:225-245).
+struct Tank {
+    std::vector<int> rows;
+    std::vector<std::array<int, 3>> chan;  // weight channel, concentration channel, family
+    int stride = SP_ACC_MIN;
+    int families = 0;
+    uint32_t group_mask = 0;
+    int primary = -1;
+};
+
+void default_params(sp_sim_params* p);
+std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present);
+bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t);
+int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
+           std::string* why);
+void levels(int W, int H, sp_levels_info& L);
+int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows);
+float dt_limit(const sp_sim_params& p, const std::vector<int>& rows);
+void settle_spawn(const sp_sim_params& p, sp_particle& q);
+
+}  // namespace sp

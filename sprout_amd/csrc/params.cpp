@@ -1,0 +1,441 @@
+// params.cpp — host-side derivations of the fluid step: default parameters and the
+// built-in materials this build carries, the tank's accumulator layout, and the grid-unit
+// parameter block and material table the kernels read. No device code; these run on the
+// CPU in the C-ABI calls that change them.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "host.h"
+
+namespace sp {
+
+static const float WATER_DENSITY = 998.0f;  // params.rs:808
+static const float T_REF = 20.0f;           // material.rs:27
+
+// MatProps::default (material.rs:344-420), the fields this build reads.
+static sp_material default_material() {
+    sp_material m;
+    memset(&m, 0, sizeof(m));
+    m.group = 0;
+    m.diluent = -1;
+    m.density = 998.0f;
+    m.density_solid = 0.0f;
+    m.viscosity = 1.0e-6f;
+    m.temp_model = 0;
+    m.temp_b = 0.0f;
+    m.temp_c = 0.0f;
+    m.temp_ref = T_REF;
+    m.tension = 0.072f;
+    m.contact_angle = 90.0f;
+    m.mix_rate = 0.0f;
+    m.expansion = 4.0e-4f;
+    m.melt_lo = -1.0e9f;
+    m.melt_hi = -1.0e9f;
+    m.visc_exponent = 1.0f;
+    m.load = 1.0f;
+    m.spawn_temp = T_REF;
+    m.latent = 0.0f;
+    m.heat_capacity = 4182.0f;
+    return m;
+}
+
+void default_params(sp_sim_params* p) {
+    memset(p, 0, sizeof(*p));
+    // SimParams::default (params.rs:386-460).
+    p->cell_size = 0.01f;
+    p->gravity = 9.81f;
+    p->ambient_temp = T_REF;
+    p->rigid_viscosity = 1.0f;
+    // default_pair_tension (material.rs:489-505).
+    float t[4][4] = {{0.0f, 0.025f, 0.375f, 0.0f}, {0.025f, 0.0f, 0.36f, 0.0f}, {0.375f, 0.36f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+    memcpy(p->pair_tension, t, sizeof(t));
+    p->droplet_cells = 1.0f;
+    p->cfl = 4.0f;
+    p->advection_order = 2;
+    p->dt_max = 1.0f / 80.0f;
+    p->fixed_hz = 0.0f;
+    p->pressure_tol = 1e-4f;
+    p->pressure_max_iters = 60;
+    p->correction_tol = 1e-2f;
+    p->viscosity_tol = 1e-3f;
+    p->viscosity_max_iters = 200;
+    p->implicit_viscosity_min = 1e-3f;
+    p->spray_cfl = 4.0f;
+    p->speed_cap = 1.0f;
+    p->correction_time = (1.0f / 240.0f) / 0.69314718f;
+    p->correction_max = 0.5f;
+    p->correction_deadband = 0.25f;
+    p->correction_step_max = 0.5f;
+    p->crowd_jitter = 0.5f;
+    p->tool_crowd_max = 2.0f;
+    p->apic_clamp = 1.0f;
+    p->curvature_max = 0.5f;
+    p->slip_cap = 1;
+    p->sealed_vent = 1;
+    p->min_substep_share = 0.02f;
+    p->mat_count = SP_BUILTIN_COUNT;
+    for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
+    // presets.rs:37-57 water
+    {
+        sp_material& m = p->mat[SP_ROW_WATER];
+        m.temp_model = 2;
+        m.temp_b = 578.9f;
+        m.temp_c = 137.5f;
+        m.melt_lo = -0.5f;
+        m.melt_hi = 0.0f;
+        m.latent = 334.0e3f;
+        m.density_solid = 917.0f;
+    }
+    // presets.rs:61-89 honey
+    {
+        sp_material& m = p->mat[SP_ROW_HONEY];
+        m.diluent = SP_ROW_WATER;
+        m.density = 1.4f * 998.0f;
+        m.viscosity = 0.06f;
+        m.temp_model = 1;
+        m.temp_b = 10500.0f;
+        m.tension = 0.076f;
+        m.mix_rate = 1.5f;
+        m.heat_capacity = 2300.0f;
+        m.expansion = 5.0e-4f;
+    }
+    // presets.rs:92-111 olive oil
+    {
+        sp_material& m = p->mat[SP_ROW_OIL];
+        m.group = 1;
+        m.density = 0.9f * 998.0f;
+        m.viscosity = 6.0e-5f;
+        m.temp_model = 1;
+        m.temp_b = 3900.0f;
+        m.tension = 0.032f;
+        m.heat_capacity = 1970.0f;
+        m.expansion = 7.0e-4f;
+        m.melt_lo = -8.0f;
+        m.melt_hi = -4.0f;
+        m.latent = 150.0e3f;
+        m.density_solid = 925.0f;
+    }
+    // presets.rs:298-327 mercury
+    {
+        sp_material& m = p->mat[SP_ROW_MERCURY];
+        m.group = 2;
+        m.density = 13534.0f;
+        m.density_solid = 14184.0f;
+        m.viscosity = 1.13e-7f;
+        m.tension = 0.485f;
+        m.contact_angle = 140.0f;
+        m.heat_capacity = 139.5f;
+        m.expansion = 1.8e-4f;
+        m.melt_lo = -39.0f;
+        m.melt_hi = -38.8f;
+        m.latent = 11.4e3f;
+    }
+}
+
+static bool freezes(const sp_material& m) { return m.melt_hi > -273.0f; }
+
+// MatProps::temp_factor_ln (material.rs:432-442).
+static float temp_factor_ln(const sp_material& m, float t) {
+    float tk = std::max(t + 273.15f, 1.0f);
+    float tref = m.temp_ref + 273.15f;
+    float x = 0.0f;
+    if (m.temp_model == 1) x = m.temp_b * (1.0f / tk - 1.0f / tref);
+    else if (m.temp_model == 2) x = m.temp_b / std::max(tk - m.temp_c, 5.0f) - m.temp_b / std::max(tref - m.temp_c, 5.0f);
+    return std::min(std::max(x, -30.0f), 30.0f);
+}
+
+static float viscosity_at(const sp_material& m, float t) { return m.viscosity * expf(temp_factor_ln(m, t)); }
+
+static float tension_cells(const sp_sim_params& p, float sigma) {
+    float dx = p.cell_size;
+    return sigma / (WATER_DENSITY * dx * dx * dx);
+}
+
+// SimParams::dt_capillary (params.rs:962-984).
+static float dt_capillary(const sp_sim_params& p, const std::vector<int>& rows) {
+    float worst = 0.0f;
+    for (int i : rows) {
+        const sp_material& m = p.mat[i];
+        worst = std::max(worst, tension_cells(p, m.tension) / (m.density / WATER_DENSITY));
+        for (int j : rows) {
+            const sp_material& n = p.mat[j];
+            if (n.group != m.group) {
+                float sigma = tension_cells(p, p.pair_tension[m.group][n.group]);
+                worst = std::max(worst, sigma / (std::min(m.density, n.density) / WATER_DENSITY));
+            }
+        }
+    }
+    if (worst <= 0.0f) return INFINITY;
+    return 0.5f * sqrtf(1.0f / (2.0f * 3.14159265f * worst));
+}
+
+float dt_limit(const sp_sim_params& p, const std::vector<int>& rows) { return std::min(p.dt_max, dt_capillary(p, rows)); }
+
+// FluidSim::viscosity_mode (mod.rs:2768-2792), isothermal.
+int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows) {
+    float dt = dt_limit(p, rows);
+    float d = 0.0f;
+    for (int r : rows) {
+        float cells = 1.0f / (p.cell_size * p.cell_size);
+        float nu = viscosity_at(p.mat[r], p.ambient_temp) * cells;
+        d = std::max(d, nu * dt);
+    }
+    if (d <= 0.0f) return 0;
+    if (d <= 0.05f) return 1;
+    return 2;
+}
+
+// The rows a tank holding `present` needs: those and their diluents (SimParams::closure,
+// params.rs:927-950, without residues and reaction products, which this build does not carry).
+std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present) {
+    std::vector<int> rows = present;
+    for (size_t k = 0; k < rows.size(); k++) {
+        int d = p.mat[rows[k]].diluent;
+        if (d >= 0 && std::find(rows.begin(), rows.end(), d) == rows.end()) rows.push_back(d);
+    }
+    std::sort(rows.begin(), rows.end());
+    return rows;
+}
+
+// Tank::layout (mod.rs:271-331), isothermal and without elastic channels.
+bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t) {
+    if ((int)rows.size() > SP_MAX_PRESENT) return false;
+    t = Tank();
+    t.rows = rows;
+    int next = SP_ACC_FIRST;
+    uint32_t group_mask = 0;
+    int primary = -1;
+    for (size_t k = 0; k < rows.size(); k++)
+        if (p.mat[rows[k]].diluent == rows[0]) {
+            primary = (int)k;
+            break;
+        }
+    int families = primary >= 0 ? 1 : 0;
+    for (size_t k = 0; k < rows.size(); k++) {
+        const sp_material& m = p.mat[rows[k]];
+        group_mask |= 1u << std::min(m.group, 3);
+        int c[3] = {-1, -1, -1};
+        if (k > 0 && (int)k != primary) c[0] = next++;
+        if (m.diluent >= 0) {
+            c[1] = next++;
+            if ((int)k == primary) c[2] = 0;
+            else c[2] = families++;
+        }
+        t.chan.push_back({c[0], c[1], c[2]});
+    }
+    if (next > SP_ACC_MAX) return false;
+    int stride = std::max(next, SP_ACC_MIN);
+    stride = (stride + 3) / 4 * 4;
+    t.stride = stride;
+    t.families = families;
+    t.group_mask = group_mask;
+    t.primary = primary >= 0 ? rows[primary] : -1;
+    return true;
+}
+
+// FluidSim::caps (mod.rs:2717-2766) for an isothermal tank; returns false for a tank that
+// needs a CAP-gated pass this build does not carry.
+static bool tank_caps(const sp_sim_params& p, const Tank& t, uint32_t& caps) {
+    caps = 0;
+    if (t.families > 0) caps |= 1u;  // CAP_FAMILIES
+    bool rigid = false;
+    for (int r : t.rows) {
+        const sp_material& m = p.mat[r];
+        rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (freezes(m) && p.ambient_temp < m.melt_hi);
+    }
+    if (rigid) caps |= 4u;  // CAP_RIGID
+    return !rigid;
+}
+
+int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why) {
+    memset(&g, 0, sizeof(g));
+    memset(mats, 0, sizeof(sp_gmat) * SP_MAX_MATERIALS);
+    const float dx = p.cell_size;
+    // gpu_materials (mod.rs:2179-2236), isothermal.
+    for (int r = 0; r < SP_MAX_MATERIALS; r++) {
+        const sp_material& m = p.mat[r];
+        int chan[3] = {-1, -1, -1};
+        for (size_t k = 0; k < t.rows.size(); k++)
+            if (t.rows[k] == r) {
+                chan[0] = t.chan[k][0];
+                chan[1] = t.chan[k][1];
+                chan[2] = t.chan[k][2];
+            }
+        float ang = m.contact_angle * 0.017453292f;
+        float sn = sinf(ang), cs = cosf(ang);
+        float solid = m.density_solid > 0.0f ? m.density_solid : m.density;
+        float ln_nu = logf(std::max(m.viscosity / (dx * dx), 1e-8f));
+        float rho = m.density;
+        ln_nu += temp_factor_ln(m, p.ambient_temp);
+        rho *= std::min(std::max(1.0f - m.expansion * (p.ambient_temp - m.temp_ref), 0.7f), 1.3f);
+        if (freezes(m) && m.diluent < 0 && p.ambient_temp < m.melt_hi) {
+            float fs = std::min(std::max((m.melt_hi - p.ambient_temp) / std::max(m.melt_hi - m.melt_lo, 1e-3f), 0.0f), 1.0f);
+            float frozen = std::max(logf(p.rigid_viscosity / (dx * dx)), ln_nu);
+            ln_nu += (frozen - ln_nu) * fs;
+            rho += (solid - rho) * fs;
+        }
+        sp_gmat& o = mats[r];
+        o.group = std::min(m.group, 3);
+        o.diluent = m.diluent;
+        o.chan_w = chan[0];
+        o.chan_s = chan[1];
+        o.fam = chan[2];
+        o.rheo = 0;
+        o.temp_model = m.temp_model;
+        o.residue = -1;
+        o.rho = rho / WATER_DENSITY;
+        o.ln_nu = ln_nu;
+        o.sigma = tension_cells(p, m.tension);
+        o.mix_rate = m.mix_rate;
+        o.cos_c = cs;
+        o.sin_c = sn;
+        o.grain = 0.0f;
+        o.rho_solid = solid / WATER_DENSITY;
+        o.visc_exp = m.visc_exponent;
+        o.expansion = m.expansion;
+        o.load = m.load;
+    }
+    uint32_t caps = 0;
+    if (!tank_caps(p, t, caps)) {
+        if (why) *why = "the tank needs rigid bodies (a liquid that freezes or exceeds rigid_viscosity), which this build does not carry";
+        return SP_E_UNSUPPORTED;
+    }
+    // gpu_params (mod.rs:2238-2330).
+    g.W = W;
+    g.H = H;
+    g.n_slots = n_slots;
+    g.spawn_count = 0;
+    g.dt_fixed = p.fixed_hz > 0.0f ? 1.0f / p.fixed_hz : 0.0f;
+    g.gravity = p.gravity / p.cell_size;
+    g.inv_ppc = 1.0f / (float)SP_PPC;
+    g.apic_clamp = p.apic_clamp;
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) g.pair[a][b] = tension_cells(p, p.pair_tension[a][b]);
+    g.kappa_max = p.curvature_max;
+    g.sealed_vent = (float)p.sealed_vent;
+    g.cell_size = p.cell_size;
+    g.rigid_friction = -1.0f;
+    g.n_present = (int)t.rows.size();
+    g.acc_stride = std::max(t.stride, SP_ACC_MIN);
+    g.n_fam = t.families;
+    g.group_mask = t.group_mask;
+    g.caps = caps;
+    g.primary = t.rows.empty() ? -1 : t.primary;
+    g.group_heat = 0;
+    g.visc_mode = viscosity_mode(p, t.rows);
+    for (int k = 0; k < SP_MAX_PRESENT; k++) {
+        g.present[k] = 0;
+        g.fams[k][0] = -1;
+        g.fams[k][1] = -1;
+    }
+    for (size_t k = 0; k < t.rows.size(); k++) {
+        g.present[k] = t.rows[k];
+        if (t.chan[k][2] >= 0) {
+            g.fams[t.chan[k][2]][0] = t.rows[k];
+            g.fams[t.chan[k][2]][1] = p.mat[t.rows[k]].diluent;
+        }
+    }
+    g.sep_tau = p.droplet_cells;
+    g.cfl = p.cfl;
+    g.tool_crowd_max = p.tool_crowd_max;
+    g.density_k = p.correction_max;
+    g.dt_max = dt_limit(p, t.rows);
+    g.crowd_jitter = p.crowd_jitter;
+    g.ext_speed = 0.0f;
+    g.wall_margin = 0.1f;
+    g.tol[0] = p.pressure_tol;
+    g.tol[1] = p.correction_tol;
+    g.tol[2] = p.viscosity_tol;
+    g.tol[3] = p.correction_time;
+    {
+        float gc = p.gravity / p.cell_size;
+        g.ext[0] = (p.speed_cap > 0.0f && gc > 0.0f) ? p.speed_cap * sqrtf(2.0f * gc * (float)H) : 1e30f;
+    }
+    g.ext[1] = (float)p.advection_order;
+    g.ext[2] = p.spray_cfl;
+    g.ext[3] = 0.0f;
+    g.corr[0] = p.correction_step_max;
+    g.corr[1] = p.correction_deadband;
+    g.corr[2] = (float)p.slip_cap;
+    g.corr[3] = p.implicit_viscosity_min;
+    g.p_max_iters = (int)p.pressure_max_iters;
+    g.v_max_iters = (int)p.viscosity_max_iters;
+    g.heat_x = p.ambient_temp;
+    g.ln_rigid = logf(p.rigid_viscosity / (dx * dx));
+    return SP_OK;
+}
+
+// Levels::new (mod.rs:727-753).
+void levels(int W, int H, sp_levels_info& L) {
+    memset(&L, 0, sizeof(L));
+    L.dims[0][0] = W;
+    L.dims[0][1] = H;
+    int n = 1;
+    while (L.dims[n - 1][0] * L.dims[n - 1][1] > SP_COARSEST_CELLS && n < SP_MAX_LEVELS) {
+        L.dims[n][0] = (L.dims[n - 1][0] + 1) / 2;
+        L.dims[n][1] = (L.dims[n - 1][1] + 1) / 2;
+        n++;
+    }
+    L.n = n;
+    int total = 0;
+    for (int l = 0; l < n; l++) {
+        L.offs[l] = total;
+        total += L.dims[l][0] * L.dims[l][1];
+    }
+    L.total = total;
+    int tail = n - 1;
+    for (int l = 1; l < n; l++)
+        if (L.dims[l][0] * L.dims[l][1] <= SP_TAIL_CELLS) {
+            tail = l;
+            break;
+        }
+    tail = std::max(1, std::min(tail, n - 1));
+    L.tail = tail;
+    int tiles = 0;
+    for (int l = 0; l <= tail; l++) {
+        L.tile_offs[l] = tiles;
+        tiles += ((L.dims[l][0] + 15) / 16) * ((L.dims[l][1] + 15) / 16);
+    }
+    L.tiles = tiles;
+}
+
+// Particle::settle_spawn (params.rs:70-88), isothermal tanks.
+void settle_spawn(const sp_sim_params& p, sp_particle& q) {
+    uint32_t low = q.kind & 255u;
+    if (q.kind == SP_KIND_DEAD || low == 0u) return;
+    const sp_material& def = p.mat[low - 1];
+    if (q.t == -1.0e4f) {
+        q.t = def.spawn_temp;
+        q.kind |= SP_KIND_REAL_T;
+    }
+    if (q.s == -1.0f) q.s = def.diluent >= 0 ? std::min(std::max(def.load, 0.0f), 1.0f) : 1.0f;
+    if (q.kind & SP_KIND_REAL_T) {
+        q.kind &= ~SP_KIND_REAL_T;
+        // SimParams::enthalpy_temp (params.rs:844-855): the temperature itself above the
+        // melting range, which is all an isothermal tank carries.
+        const sp_material& own = def;
+        float lo, hi, l;
+        if (own.diluent < 0) {
+            lo = own.melt_lo;
+            hi = std::max(own.melt_hi, own.melt_lo + 0.25f);
+            l = freezes(own) ? own.latent / own.heat_capacity : 0.0f;
+        } else {
+            const sp_material& d = p.mat[own.diluent];
+            lo = d.melt_lo;
+            hi = std::max(d.melt_hi, d.melt_lo + 0.25f);
+            l = 0.0f;
+        }
+        float t = q.t;
+        if (lo < -273.0f || t >= hi) q.t = t;
+        else if (t <= lo) q.t = t - l;
+        else {
+            float fr = (t - lo) / std::max(hi - lo, 1e-3f);
+            q.t = (lo - l) + fr * (hi - lo + l);
+        }
+    }
+}
+
+}  // namespace sp

@@ -1,0 +1,629 @@
+// particles.hip — the per-particle pass of a substep on gfx950, and the particle
+// bookkeeping kernels (sort, count, stats, claim).
+//
+// One work-item per particle slot, 256 per work-group (four wave64s). Each work-item runs
+// the gather (grid -> particle, APIC), the advection with collision against the solid SDF,
+// and then the scatter (particle -> grid) of mass, momentum and composition as 32-bit fixed
+// point (particles.comp:98-1101). The scatter goes to an LDS box that spans the cells the
+// group's particles touch when it fits (particles are kept sorted by cell, so it usually
+// does), and is flushed to HBM with one atomic per non-zero word; otherwise each node is
+// added with a global atomic. Integer sums are order-independent, so the result is the same
+// either way and bit-identical to a serial evaluation.
+#include "dev.h"
+
+namespace {
+
+constexpr int PT = 256;           // work-items per group
+constexpr int LDS_INTS = 12288;   // scatter box budget: 48 KiB
+
+struct ScatterIn {
+    bool live;
+    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc;
+    int cw, cs;
+};
+
+SP_DEV int cidx(const Ctx& c, int x, int y) { return y * c.W + x; }
+
+// Relative density of a row at concentration s (common.glsl:475-479).
+SP_DEV float row_density(const Ctx& c, int row, float s) {
+    int d = c.mats[row].diluent;
+    float rho = c.mats[row].rho;
+    return d >= 0 ? mixf_(c.mats[d].rho, rho, s) : rho;
+}
+
+// Terminal drift speed (common.glsl:370-374).
+SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d) {
+    float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * c.P->gravity * d;
+    return qc > 0.0f ? 2.0f * qc / (qb + __fsqrt_rn(qb * qb + 4.0f * qa * qc)) : 0.0f;
+}
+
+// Bilinear SDF and its gradient (air.glsl:13-22).
+SP_DEV float sample_sdf(const Ctx& c, float px, float py, float& gx, float& gy) {
+    float x = clampf_(px, 0.0f, (float)c.W - 0.001f), y = clampf_(py, 0.0f, (float)c.H - 0.001f);
+    int ix = (int)x, iy = (int)y;
+    float fx = x - (float)ix, fy = y - (float)iy;
+    int w1 = c.W + 1;
+    int k = iy * w1 + ix;
+    float d00 = c.sdf[k], d10 = c.sdf[k + 1], d01 = c.sdf[k + w1], d11 = c.sdf[k + w1 + 1];
+    gx = mixf_(d10 - d00, d11 - d01, fy);
+    gy = mixf_(d01 - d00, d11 - d10, fx);
+    return mixf_(mixf_(d00, d10, fx), mixf_(d01, d11, fx), fy);
+}
+
+// Bilinear velocity over live faces, `fb` where no face is live (air.glsl:41-78).
+SP_DEV float2 sample_vel(const Ctx& c, float px, float py, float2 fb) {
+    float2 o = fb;
+    {
+        float gx = px, gy = py - 0.5f;
+        int bx = floori(gx), by = floori(gy);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        float sw = 0.0f, su = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                float4 q = c.vel[cidx(c, bx + a, by + b)];
+                if (face_live(q.z)) {
+                    float wt = (a == 0 ? 1.0f - fx : fx) * (b == 0 ? 1.0f - fy : fy);
+                    sw += wt;
+                    su += wt * q.x;
+                }
+            }
+        if (sw > 1e-3f) o.x = su / sw;
+    }
+    {
+        float gx = px - 0.5f, gy = py;
+        int bx = floori(gx), by = floori(gy);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        float sw = 0.0f, su = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                float4 q = c.vel[cidx(c, bx + a, by + b)];
+                if (face_live(q.w)) {
+                    float wt = (a == 0 ? 1.0f - fx : fx) * (b == 0 ? 1.0f - fy : fy);
+                    sw += wt;
+                    su += wt * q.y;
+                }
+            }
+        if (sw > 1e-3f) o.y = su / sw;
+    }
+    return o;
+}
+
+// One push out of the solids (particles.comp:51-65).
+SP_DEV bool collide(const Ctx& c, float& x, float& y, float& vx, float& vy, bool beyond) {
+    float nx, ny;
+    float d = sample_sdf(c, x, y, nx, ny);
+    float wm = c.P->wall_margin;
+    if (d < wm) {
+        float l = len2_(nx, ny);
+        if (l > 1e-5f) {
+            nx = nx / l;
+            ny = ny / l;
+            float k = wm - d + (beyond ? fmaxf_(-d, 0.0f) : 0.0f);
+            x += nx * k;
+            y += ny * k;
+            float vn = vx * nx + vy * ny;
+            if (vn < 0.0f) {
+                vx -= vn * nx;
+                vy -= vn * ny;
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// APIC gather of one face component over its 3 x 3 live faces (particles.comp:165-222).
+SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, float& cr, float& Ca, float& Cb) {
+    float gx = comp == 0 ? px : px - 0.5f, gy = comp == 0 ? py - 0.5f : py;
+    int bx = floori(gx - 0.5f), by = floori(gy - 0.5f);
+    float fx = gx - (float)bx, fy = gy - (float)by;
+    float wx[3], wy[3];
+    bspline3(fx, wx);
+    bspline3(fy, wy);
+    float sw = 0.0f, su = 0.0f, sc = 0.0f, sdx = 0.0f, sdy = 0.0f, sudx = 0.0f, sudy = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float4 q = c.vel[cidx(c, bx + a, by + b)];
+            float st = comp == 0 ? q.z : q.w, val = comp == 0 ? q.x : q.y;
+            if (face_live(st)) {
+                float wt = wx[a] * wy[b];
+                float dx = (float)a - fx, dy = (float)b - fy;
+                sw += wt;
+                su += wt * val;
+                sc += wt * st;
+                sdx += wt * dx;
+                sdy += wt * dy;
+                float wu = wt * val;
+                sudx += wu * dx;
+                sudy += wu * dy;
+            }
+        }
+    if (sw > 1e-4f) {
+        float uu = su / sw;
+        u = uu;
+        cr = sc / sw;
+        Ca = 4.0f * (sudx - uu * sdx) / sw;
+        Cb = 4.0f * (sudy - uu * sdy) / sw;
+    }
+}
+
+// Gather, drift, exchange, advect and collide one particle (particles.comp:98-985, the
+// branches this build carries); fills the scatter inputs.
+SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& umax) {
+    const sp_gparams& P = *c.P;
+    g.live = false;
+    if (id >= P.n_slots) return;
+    sp_particle p = c.part[id];
+    if (p.kind == SP_KIND_DEAD) return;
+    if ((p.kind & 255u) == 0u) return;
+    float dt = c.S->dt;
+    int row = (int)(p.kind & 255u) - 1;
+    int Mgroup = c.mats[row].group, Mcw = c.mats[row].chan_w, Mcs = c.mats[row].chan_s;
+    int fam = c.mats[row].fam;
+    if (gather) {
+        float vx = p.vel[0], vy = p.vel[1];
+        float C0 = p.c[0], C1 = p.c[1], C2 = p.c[2], C3 = p.c[3];
+        float corrx = 0.0f, corry = 0.0f, slip = 0.0f, crowd = 0.0f;
+        gather_face(c, p.pos[0], p.pos[1], 0, vx, corrx, C0, C1);
+        gather_face(c, p.pos[0], p.pos[1], 1, vy, corry, C2, C3);
+        if (P.apic_clamp > 0.0f) {
+            float lim = P.apic_clamp / dt;
+            C0 = clampf_(C0, -lim, lim);
+            C1 = clampf_(C1, -lim, lim);
+            C2 = clampf_(C2, -lim, lim);
+            C3 = clampf_(C3, -lim, lim);
+        }
+        if (P.corr[0] > 0.0f) {
+            float cl = len2_(corrx, corry), cmax = P.corr[0] / dt;
+            if (cl > cmax) {
+                float k = cmax / cl;
+                corrx *= k;
+                corry *= k;
+            }
+        }
+        {
+            float gx = p.pos[0] - 0.5f, gy = p.pos[1] - 0.5f;
+            int bx = floori(gx - 0.5f), by = floori(gy - 0.5f);
+            float fx = gx - (float)bx, fy = gy - (float)by;
+            float wx[3], wy[3];
+            bspline3(fx, wx);
+            bspline3(fy, wy);
+            float sw = 0.0f, sfw = 0.0f, sfs = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f;
+            bool more = (P.group_mask & 12u) != 0u;
+            bool groups = (P.group_mask & (P.group_mask - 1u)) != 0u;
+            bool sorts = group_sorts(P, c.mats, Mgroup);
+            float sbg = 0.0f;
+            int nmix = P.n_fam - 1;
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    int ci = cidx(c, bx + a, by + b);
+                    float4 q = c.comp[ci];
+                    float wt = wx[a] * wy[b];
+                    crowd += wt * q.w;
+                    if (q.x > 0.0f) {
+                        sw += wt;
+                        sgx += wt * q.y;
+                        if (more) {
+                            float4 ax = c.aux[ci];
+                            sgy += wt * ax.z;
+                            sgz += wt * ax.w;
+                        }
+                        if (sorts) sbg += wt * f4c(c.gbase[ci], Mgroup - 1);
+                        float sb = fam > 0 ? c.mixb[(size_t)ci * nmix + fam - 1] : q.z;
+                        if (sb >= 0.0f) {
+                            sfw += wt;
+                            sfs += wt * sb;
+                        }
+                    }
+                }
+            if (sw > 1e-4f) {
+                sgx /= sw;
+                sgy /= sw;
+                sgz /= sw;
+                float sgv[3] = {sgx, sgy, sgz};
+                int gg = Mgroup;
+                float fg = gg == 0 ? 1.0f - sgx - sgy - sgz : sgv[gg - 1];
+                float others = 1.0f - fg;
+                if (others > 1e-4f && groups) {
+                    float sgr[4] = {0, 0, 0, 0}, snu[4] = {0, 0, 0, 0}, snw[4] = {0, 0, 0, 0};
+                    for (int b = 0; b < 3; b++)
+                        for (int a = 0; a < 3; a++) {
+                            int ci = cidx(c, bx + a, by + b);
+                            float4 q = c.comp[ci];
+                            if (q.x <= 0.0f) continue;
+                            float wt = wx[a] * wy[b];
+                            float4 gr = c.grho[ci];
+                            sgr[0] += wt * gr.x;
+                            sgr[1] += wt * gr.y;
+                            sgr[2] += wt * gr.z;
+                            sgr[3] += wt * gr.w;
+                            float4 ax = c.aux[ci];
+                            float fc[4] = {1.0f - q.y - ax.z - ax.w, q.y, ax.z, ax.w};
+                            float top = fmaxf_(fmaxf_(fc[0], fc[1]), fmaxf_(fc[2], fc[3]));
+                            float oh[4];
+                            for (int k = 0; k < 4; k++) oh[k] = fc[k] == top ? 1.0f : 0.0f;
+                            float ohs = fmaxf_(oh[0] + oh[1] + oh[2] + oh[3], 1.0f);
+                            for (int k = 0; k < 4; k++) oh[k] /= ohs;
+                            float wa = wt * ax.x;
+                            for (int k = 0; k < 4; k++) {
+                                snu[k] += wa * oh[k];
+                                snw[k] += wt * oh[k];
+                            }
+                        }
+                    for (int k = 0; k < 4; k++) sgr[k] /= sw;
+                    float rho_g = fg > 1e-4f ? sgr[gg] / fg : row_density(c, row, p.s);
+                    float rho_o = (sgr[0] + sgr[1] + sgr[2] + sgr[3] - sgr[gg]) / others;
+                    float drho = rho_g - rho_o;
+                    int pcx = clampi_((int)p.pos[0], 1, c.W - 2), pcy = clampi_((int)p.pos[1], 1, c.H - 2);
+                    int k = cidx(c, pcx, pcy);
+                    float fsv[4] = {1.0f - sgx - sgy - sgz, sgx, sgy, sgz};
+                    int G = fsv[0] >= fmaxf_(fsv[1], fmaxf_(fsv[2], fsv[3])) ? 0 : fsv[1] >= fmaxf_(fsv[2], fsv[3]) ? 1 : fsv[2] >= fsv[3] ? 2 : 3;
+                    float rho_c = sgr[G] / fmaxf_(fsv[G], 1e-4f);
+                    float nu_c = snw[G] > 1e-6f ? snu[G] / snw[G] : c.aux[k].x;
+                    float vt = terminal(c, drho, rho_c, nu_c, P.sep_tau);
+                    float sep_max = P.corr[2] > 0.0f ? P.cfl / dt : 1e30f;
+                    slip = -fminf_(vt, sep_max) * signf_(drho) * others;
+                    float fs[4] = {1.0f - sgx - sgy - sgz, sgx, sgy, sgz};
+                    fs[gg] = -1.0f;
+                    int h = fs[0] >= fmaxf_(fs[1], fmaxf_(fs[2], fs[3])) ? 0 : fs[1] >= fmaxf_(fs[2], fs[3]) ? 1 : fs[2] >= fs[3] ? 2 : 3;
+                    float bond = fabsf(drho) * P.gravity / fmaxf_(P.pair[gg][h], 1e-9f);
+                    float hold = clampf_(1.0f - bond, 0.0f, 1.0f);
+                    if (slip != 0.0f && hold > 0.0f) {
+                        int hi = gg > h ? gg : h;
+                        int W = c.W;
+                        float grx, gry;
+                        if (hi == 1) {
+                            grx = 0.5f * (c.surf[k + 1].y - c.surf[k - 1].y);
+                            gry = 0.5f * (c.surf[k + W].y - c.surf[k - W].y);
+                        } else if (hi == 2) {
+                            grx = 0.5f * (c.surf2[k + 1].x - c.surf2[k - 1].x);
+                            gry = 0.5f * (c.surf2[k + W].x - c.surf2[k - W].x);
+                        } else {
+                            grx = 0.5f * (c.surf2[k + 1].y - c.surf2[k - 1].y);
+                            gry = 0.5f * (c.surf2[k + W].y - c.surf2[k - W].y);
+                        }
+                        float resolved = 1.0f - clampf_((0.12f - len2_(grx, gry)) / (0.12f - 0.04f), 0.0f, 1.0f);
+                        slip *= 1.0f - hold * resolved;
+                    }
+                }
+                if (sorts) {
+                    float fgi = groups ? sgv[Mgroup - 1] : 1.0f;
+                    if (fgi > 1e-4f) {
+                        float rmean = sbg / sw / fgi;
+                        float drho = c.mats[row].rho - rmean;
+                        int pcx = clampi_((int)p.pos[0], 0, c.W - 1), pcy = clampi_((int)p.pos[1], 0, c.H - 1);
+                        float4 cq = c.cell[cidx(c, pcx, pcy)];
+                        float nu_c = cq.y > 0.0f ? cq.z / cq.y : c.aux[cidx(c, pcx, pcy)].x;
+                        float vt = terminal(c, drho, rmean, nu_c, P.sep_tau);
+                        float sep_max = P.corr[2] > 0.0f ? P.cfl / dt : 1e30f;
+                        slip -= fminf_(vt, sep_max) * signf_(drho);
+                    }
+                }
+                bool diluent0 = fam < 0 && P.primary >= 0 && P.fams[0][1] == row;
+                if (fam >= 0 || diluent0) {
+                    if (sfw > 1e-4f) {
+                        float s0 = diluent0 ? 0.0f : p.s;
+                        int rr = diluent0 ? P.primary : row;
+                        float s1 = clampf_(s0 + (sfs / sfw - s0) * (1.0f - expf(-c.mats[rr].mix_rate * dt)), 0.0f, 1.0f);
+                        if (!diluent0) {
+                            p.s = s1;
+                        } else if (s1 > 0.0f) {
+                            row = P.primary;
+                            Mgroup = c.mats[row].group;
+                            Mcw = c.mats[row].chan_w;
+                            Mcs = c.mats[row].chan_s;
+                            p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                            p.s = s1;
+                        }
+                    }
+                }
+                if (fam < 0 && (P.caps & 1u) && (int)(p.kind & 255u) - 1 == row && c.mats[row].fam < 0) {
+                    for (int j = P.primary >= 0 ? 1 : 0; j < P.n_fam; j++) {
+                        if (P.fams[j][1] != row) continue;
+                        float found = 0.0f;
+                        if (j == 0) {
+                            found = sfw;
+                        } else {
+                            for (int b = 0; b < 3; b++)
+                                for (int a = 0; a < 3; a++)
+                                    if (c.mixb[(size_t)cidx(c, bx + a, by + b) * nmix + j - 1] >= 0.0f) found += wx[a] * wy[b];
+                        }
+                        if (found > 1e-4f) {
+                            row = P.fams[j][0];
+                            Mgroup = c.mats[row].group;
+                            Mcw = c.mats[row].chan_w;
+                            Mcs = c.mats[row].chan_s;
+                            p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                            p.s = 0.0f;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        float vcap = P.ext[0];
+        {
+            float sp = len2_(vx, vy);
+            if (sp > vcap) {
+                float k = vcap / sp;
+                vx *= k;
+                vy *= k;
+            }
+        }
+        float vax = vx, vay = vy;
+        if (P.ext[1] >= 2.0f) {
+            float h = 0.5f * dt;
+            float ex = (vx + corrx) + 0.0f, ey = (vy + corry) + slip;
+            float xmx = clampf_(p.pos[0] + h * ex, 2.01f, (float)c.W - 2.01f);
+            float xmy = clampf_(p.pos[1] + h * ey, 2.01f, (float)c.H - 2.01f);
+            float2 here = sample_vel(c, p.pos[0], p.pos[1], make_float2(vx, vy));
+            float2 mid = sample_vel(c, xmx, xmy, here);
+            vax = vx + mid.x - here.x;
+            vay = vy + mid.y - here.y;
+            float sp = len2_(vax, vay);
+            if (sp > vcap) {
+                float k = vcap / sp;
+                vax *= k;
+                vay *= k;
+            }
+            if (crowd < SP_LIQUID_THR && (vax * vax + vay * vay) < 0.01f * (vx * vx + vy * vy)) {
+                vx = vax;
+                vy = vay;
+            }
+        }
+        float dispx = (vax + corrx) + 0.0f, dispy = (vay + corry) + slip;
+        {
+            float mine = len2_(dispx, dispy);
+            if (crowd < SP_LIQUID_THR) mine /= P.ext[2];
+            umax = fmaxf_(umax, mine);
+        }
+        float sdx = dt * dispx, sdy = dt * dispy;
+        int pieces = clampi_((int)ceilf(len2_(sdx, sdy)), 1, 32);
+        sdx /= (float)pieces;
+        sdy /= (float)pieces;
+        float x = p.pos[0], y = p.pos[1];
+        for (int k = 1; k < pieces; k++) {
+            x += sdx;
+            y += sdy;
+            collide(c, x, y, vx, vy, true);
+        }
+        x += sdx;
+        y += sdy;
+        float excess = clampf_(crowd - 1.5f, 0.0f, 1.0f);
+        if (excess > 0.0f && P.crowd_jitter > 0.0f) {
+            uint32_t hh = hash_u(p.id ^ hash_u(__float_as_uint(p.pos[0]) ^ c.S->substeps));
+            x += ((float)(hh & 0xffffu) / 65535.0f - 0.5f) * P.crowd_jitter * excess;
+            y += ((float)(hh >> 16) / 65535.0f - 0.5f) * P.crowd_jitter * excess;
+        }
+        collide(c, x, y, vx, vy, true);
+        bool again = collide(c, x, y, vx, vy, true);
+        if (again) {
+            float gx_, gy_;
+            if (sample_sdf(c, x, y, gx_, gy_) < -1.0f) collide(c, x, y, vx, vy, false);
+        }
+        x = clampf_(x, 2.01f, (float)c.W - 2.01f);
+        y = clampf_(y, 2.01f, (float)c.H - 2.01f);
+        p.pos[0] = x;
+        p.pos[1] = y;
+        p.vel[0] = vx;
+        p.vel[1] = vy;
+        p.c[0] = C0;
+        p.c[1] = C1;
+        p.c[2] = C2;
+        p.c[3] = C3;
+        p.kind &= ~SP_KIND_FRESH;
+        c.part[id] = p;
+    }
+    g.live = true;
+    g.px = p.pos[0];
+    g.py = p.pos[1];
+    g.vsx = p.vel[0] * dt;
+    g.vsy = p.vel[1] * dt;
+    g.Csx = p.c[0] * dt;
+    g.Csy = p.c[1] * dt;
+    g.Csz = p.c[2] * dt;
+    g.Csw = p.c[3] * dt;
+    g.rho = row_density(c, row, p.s);
+    g.sc = p.s;
+    g.cw = Mcw;
+    g.cs = Mcs;
+}
+
+// The scatter of one particle (particles.comp:997-1062) through `add(node x, node y,
+// channel, value)`.
+template <class Add>
+SP_DEV void deposit(const ScatterIn& g, Add add) {
+    float wx[3], wy[3];
+    {
+        float gx = g.px, gy = g.py - 0.5f;
+        int bx = floori(gx - 0.5f), by = floori(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline3(fx, wx);
+        bspline3(fy, wy);
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float m = wx[a] * wy[b] * g.rho;
+                float dx = (float)a - fx, dy = (float)b - fy;
+                add(bx + a, by + b, 0, (int)floorf(m * SP_MASS_SCALE + 0.5f));
+                add(bx + a, by + b, 1, (int)floorf(m * (g.vsx + (g.Csx * dx + g.Csy * dy)) * SP_MOM_SCALE + 0.5f));
+            }
+    }
+    {
+        float gx = g.px - 0.5f, gy = g.py;
+        int bx = floori(gx - 0.5f), by = floori(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline3(fx, wx);
+        bspline3(fy, wy);
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float m = wx[a] * wy[b] * g.rho;
+                float dx = (float)a - fx, dy = (float)b - fy;
+                add(bx + a, by + b, 2, (int)floorf(m * SP_MASS_SCALE + 0.5f));
+                add(bx + a, by + b, 3, (int)floorf(m * (g.vsy + (g.Csz * dx + g.Csw * dy)) * SP_MOM_SCALE + 0.5f));
+            }
+    }
+    {
+        float gx = g.px - 0.5f, gy = g.py - 0.5f;
+        int bx = floori(gx - 0.5f), by = floori(gy - 0.5f);
+        float fx = gx - (float)bx, fy = gy - (float)by;
+        bspline3(fx, wx);
+        bspline3(fy, wy);
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int wi = (int)floorf(wx[a] * wy[b] * SP_MASS_SCALE + 0.5f);
+                add(bx + a, by + b, 4, wi);
+                if (g.cw >= 0) add(bx + a, by + b, g.cw, wi);
+                if (g.cs >= 0 && g.sc > 0.0f) add(bx + a, by + b, g.cs, (int)floorf(wx[a] * wy[b] * g.sc * SP_MASS_SCALE + 0.5f));
+            }
+    }
+}
+
+}  // namespace
+
+// The particle pass (particles.comp:1064-1101).
+__global__ __launch_bounds__(PT) void k_particles(Ctx c, int gather) {
+    __shared__ int sacc[LDS_INTS];
+    __shared__ int sbox[4];
+    const int t = threadIdx.x;
+    const uint32_t id = blockIdx.x * PT + t;
+    ScatterIn g;
+    float umax = 0.0f;
+    advance(c, id, gather, g, umax);
+    umax = wave_max(umax);
+    if ((t & 63) == 0) atomic_max_pos_float(&c.S->umax_part, umax);
+
+    if (t == 0) {
+        sbox[0] = 0x7fffffff;
+        sbox[1] = 0x7fffffff;
+        sbox[2] = -0x7fffffff;
+        sbox[3] = -0x7fffffff;
+    }
+    __syncthreads();
+    int fx = g.live ? floori(g.px) : 0x7fffffff, fy = g.live ? floori(g.py) : 0x7fffffff;
+    int lox = wave_min_i(fx), loy = wave_min_i(fy);
+    int hix = wave_max_i(g.live ? fx : -0x7fffffff), hiy = wave_max_i(g.live ? fy : -0x7fffffff);
+    if ((t & 63) == 0 && lox != 0x7fffffff) {
+        atomicMin(&sbox[0], lox);
+        atomicMin(&sbox[1], loy);
+        atomicMax(&sbox[2], hix);
+        atomicMax(&sbox[3], hiy);
+    }
+    __syncthreads();
+    if (sbox[0] == 0x7fffffff) return;  // no live particle in the group
+    const int stride = c.P->acc_stride;
+    const int b0x = sbox[0] - 1, b0y = sbox[1] - 1;
+    const int bw = sbox[2] + 2 - b0x + 1;
+    const int bh = sbox[3] + 2 - b0y + 1;
+    const int n = bw * bh * stride;
+    const bool shared = n <= LDS_INTS;
+    if (shared) {
+        for (int k = t; k < n; k += PT) sacc[k] = 0;
+        __syncthreads();
+        if (g.live)
+            deposit(g, [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * stride + ch], v); });
+        __syncthreads();
+        for (int k = t; k < n; k += PT) {
+            int v = sacc[k];
+            if (v == 0) continue;
+            int cell = k / stride;
+            int x = b0x + cell % bw, y = b0y + cell / bw;
+            atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + (k - cell * stride)], v);
+        }
+    } else if (g.live) {
+        deposit(g, [&](int x, int y, int ch, int v) { atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + ch], v); });
+    }
+}
+
+// ---- counting sort by cell (sort.comp), stable by slot ---------------------------------
+//
+// The key of a slot is its cell (cell_of, sort.comp:28-30), or N for a dead slot, so dead
+// slots collect at the end. The (key, slot) pairs go through a stable LSD radix sort, and
+// the particles are gathered into the new order: within a cell the earlier slot stays
+// first, which makes the permutation a defined one (the reference's is atomic-ordered,
+// sort.comp:64).
+
+__global__ __launch_bounds__(256) void k_sort_keys(Ctx c, uint32_t* keys, uint32_t* vals) {
+    uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= c.P->n_slots) return;
+    sp_particle p = c.part[id];
+    uint32_t key = (uint32_t)c.N;
+    if (p.kind != SP_KIND_DEAD) {
+        int x = clampi_((int)p.pos[0], 0, c.W - 1), y = clampi_((int)p.pos[1], 0, c.H - 1);
+        key = (uint32_t)(y * c.W + x);
+    }
+    keys[id] = key;
+    vals[id] = id;
+}
+
+__global__ __launch_bounds__(256) void k_sort_gather(Ctx c, const uint32_t* vals, sp_particle* out, uint32_t n) {
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    out[j] = c.part[vals[j]];
+}
+
+// Live particles per cell in a window (count.comp:86-92).
+__global__ __launch_bounds__(256) void k_count(Ctx c, int x0, int y0, int w, int h, uint32_t* out) {
+    uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= c.P->n_slots) return;
+    sp_particle p = c.part[id];
+    if ((p.kind & 255u) == 0u) return;
+    int x = floori(p.pos[0]) - x0, y = floori(p.pos[1]) - y0;
+    if (x < 0 || y < 0 || x >= w || y >= h) return;
+    atomicAdd(&out[y * w + x], 1u);
+}
+
+// Live count and per-row counts / concentration sums (stats.comp:123-151).
+__global__ __launch_bounds__(64) void k_stats(Ctx c, uint32_t* counter) {
+    const uint32_t RUN = 64u;
+    uint32_t first = (blockIdx.x * 64 + threadIdx.x) * RUN;
+    uint32_t n_slots = c.P->n_slots;
+    if (first >= n_slots) return;
+    uint32_t last = first + RUN < n_slots ? first + RUN : n_slots;
+    uint32_t live = 0, cur = 0, n = 0, conc = 0;
+    for (uint32_t id = first; id < last; id++) {
+        uint32_t kind = c.part[id].kind & 255u;
+        if (kind == 0u || kind > (uint32_t)SP_MAX_MATERIALS) continue;
+        live++;
+        if (kind != cur) {
+            if (n > 0u) {
+                atomicAdd(&counter[8u + 2u * (cur - 1u)], n);
+                atomicAdd(&counter[9u + 2u * (cur - 1u)], conc);
+            }
+            cur = kind;
+            n = 0u;
+            conc = 0u;
+        }
+        n++;
+        conc += (uint32_t)(clampf_(c.part[id].s, 0.0f, 1.0f) * 256.0f + 0.5f);
+    }
+    if (n > 0u) {
+        atomicAdd(&counter[8u + 2u * (cur - 1u)], n);
+        atomicAdd(&counter[9u + 2u * (cur - 1u)], conc);
+    }
+    if (live > 0u) atomicAdd(&counter[4], live);
+}
+
+// Dead slots claim pending spawns (claim.comp:10-17).
+__global__ __launch_bounds__(256) void k_claim(Ctx c, const sp_particle* spawn, uint32_t spawn_count, uint32_t* counter) {
+    uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= c.P->n_slots) return;
+    if (c.part[id].kind != SP_KIND_DEAD) return;
+    if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= spawn_count) return;
+    uint32_t k = atomicAdd(counter, 1u);
+    if (k < spawn_count) c.part[id] = spawn[k];
+}
+

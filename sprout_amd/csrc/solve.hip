@@ -1,0 +1,656 @@
+// solve.hip — the pressure solve on gfx950: conjugate gradients on two right-hand sides
+// at once (pressure * dt and the density-correction potential), preconditioned by an
+// aggregation multigrid V-cycle (mod.rs:2929-3010; mg_*.comp, cg_*.comp).
+//
+// Layout: every level of the hierarchy back to back in mga (float4 couplings L, R, B, T),
+// mgd (diagonal), mgx / mgz / mgr (float2 per cell: pre-smoothed iterate, finished
+// iterate, right-hand side), as the reference lays them out (mg.glsl:1-37). Level kernels
+// run one work-item per cell on 16 x 16 tiles and skip tiles without unknowns (ptile).
+//
+// Precision: cell arithmetic is FP32 in the reference's operation order; the scalar
+// reductions (r.z, r.r, p.q, b.b) are FP64 — the product of two floats is exact in double
+// and the sums are formed per work-group and then in a fixed tree by the last work-group to
+// finish, so the scalars do not depend on scheduling.
+//
+// Convergence without host round trips: the scalar state lives in device memory (sp_solve);
+// every kernel of an iteration reads the two `done` flags at entry and returns at once when
+// both potentials have converged, so a batch can carry a fixed number of recorded iterations.
+#include "dev.h"
+
+namespace {
+
+constexpr int NB_X[4] = {-1, 1, 0, 0};
+constexpr int NB_Y[4] = {0, 0, -1, 1};
+
+struct Lv {
+    int off, w, h, toff;
+};
+
+SP_DEV Lv level(const Ctx& c, int l) {
+    int4 L = c.lev[l];
+    return Lv{L.x, L.y, L.z, L.w};
+}
+SP_DEV bool inside(const Lv& L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y < L.h; }
+SP_DEV int at(const Lv& L, int x, int y) { return L.off + y * L.w + x; }
+SP_DEV bool unknown(const Ctx& c, const Lv& L, int x, int y) { return inside(L, x, y) && c.mgd[at(L, x, y)] > 0.0f; }
+SP_DEV bool solve_done(const Ctx& c) { return c.K->done[0] != 0u && c.K->done[1] != 0u; }
+
+SP_DEV int tid2() { return threadIdx.y * 16 + threadIdx.x; }
+SP_DEV int tile_slot() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// FP64 sum of one double4 per work-item over a 256 work-item group (fixed tree).
+SP_DEV double4 group_sum4(double4 v, double4* lds) {
+    const int t = tid2();
+    lds[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            double4 a = lds[t], b = lds[t + s];
+            lds[t] = make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+        }
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+SP_DEV void store_part(double4* p, double4 v) {
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->z, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->w, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+SP_DEV double4 load_part(const double4* p) {
+    double4 v;
+    v.x = __hip_atomic_load(&p->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.y = __hip_atomic_load(&p->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.z = __hip_atomic_load(&p->z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.w = __hip_atomic_load(&p->w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
+// Sum of parts[base .. base + n) by one 256 work-item group in a fixed order.
+SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) {
+    const int t = tid2();
+    double4 s = make_double4(0, 0, 0, 0);
+    for (int k = t; k < n; k += 256) {
+        double4 v = load_part(&c.part_sums[base + k]);
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    return group_sum4(s, lds);
+}
+
+enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2 };
+
+// The scalar step of the pressure CG (reduce_tail.glsl:59-116 modes 0, 1, 2), FP64.
+// parts [0, n): r.z (xy), r.r (zw) of the last V-cycle; [n, 2n): b.b (xy) or p.q (xy).
+SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
+    sp_solve* K = c.K;
+    const sp_gparams& P = *c.P;
+    const double tx = (double)P.tol[0] * (double)P.tol[0], ty = (double)P.tol[1] * (double)P.tol[1];
+    if (mode == RED_START) {
+        double4 a = total(c, 0, n, lds);
+        double4 b = total(c, n, n, lds);
+        if (tid2() == 0) {
+            K->bb[0] = b.x;
+            K->bb[1] = b.y;
+            K->rz[0] = a.x;
+            K->rz[1] = a.y;
+            K->rr[0] = a.z;
+            K->rr[1] = a.w;
+            K->beta[0] = K->beta[1] = 0.0;
+            K->alpha[0] = K->alpha[1] = 0.0;
+            K->iters[0] = K->iters[1] = 0u;
+            K->done[0] = a.z <= tx * b.x ? 1u : 0u;
+            K->done[1] = a.w <= ty * b.y ? 1u : 0u;
+            K->solves += 1u;
+        }
+    } else if (mode == RED_ALPHA) {
+        double4 b = total(c, n, n, lds);
+        if (tid2() == 0) {
+            K->alpha[0] = (K->done[0] == 0u && b.x > 0.0) ? K->rz[0] / b.x : 0.0;
+            K->alpha[1] = (K->done[1] == 0u && b.y > 0.0) ? K->rz[1] / b.y : 0.0;
+        }
+    } else {
+        double4 a = total(c, 0, n, lds);
+        if (tid2() == 0) {
+            uint32_t was0 = K->done[0], was1 = K->done[1];
+            K->iters[0] += 1u - was0;
+            K->iters[1] += 1u - was1;
+            uint32_t d0 = was0 | (a.z <= tx * K->bb[0] ? 1u : 0u);
+            uint32_t d1 = was1 | (a.w <= ty * K->bb[1] ? 1u : 0u);
+            K->beta[0] = (d0 == 0u && K->rz[0] > 0.0) ? a.x / K->rz[0] : 0.0;
+            K->beta[1] = (d1 == 0u && K->rz[1] > 0.0) ? a.y / K->rz[1] : 0.0;
+            K->rz[0] = a.x;
+            K->rz[1] = a.y;
+            if (!was0) K->rr[0] = a.z;
+            if (!was1) K->rr[1] = a.w;
+            K->done[0] = d0;
+            K->done[1] = d1;
+        }
+    }
+}
+
+// Called by every work-item of a 256 work-item group after the group's partial is in
+// part_sums: the last group of the launch to arrive runs the scalar step (reduce_tail.glsl:
+// 209-223).
+SP_DEV void finish_group(const Ctx& c, int mode, int nblocks, double4* lds) {
+    __shared__ int s_last;
+    if (tid2() == 0) {
+        __threadfence();
+        uint32_t prev = atomicAdd(&c.K->arrive, 1u);
+        s_last = prev + 1u == (uint32_t)nblocks ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    reduce_scalars(c, mode, nblocks, lds);
+    if (tid2() == 0) c.K->arrive = 0u;
+}
+
+}  // namespace
+
+// ---- hierarchy set-up -------------------------------------------------------------------
+
+// Galerkin coarse matrix of coarse cell (cx, cy) from fine level F (mg_build.comp:80-102).
+SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
+    float D = 0.0f;
+    float4 A = make_float4(0, 0, 0, 0);
+    for (int k = 0; k < 4; k++) {
+        int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
+        if (!unknown(c, F, x, y)) continue;
+        int i = at(F, x, y);
+        D += c.mgd[i];
+        float4 w = c.mga[i];
+        for (int d = 0; d < 4; d++) {
+            int nx = x + NB_X[d], ny = y + NB_Y[d];
+            if (!unknown(c, F, nx, ny)) continue;
+            float wd = f4c(w, d);
+            if ((nx >> 1) == cx && (ny >> 1) == cy) D -= wd;
+            else if (d == 0) A.x += wd;
+            else if (d == 1) A.y += wd;
+            else if (d == 2) A.z += wd;
+            else A.w += wd;
+        }
+    }
+    int o = at(C, cx, cy);
+    c.mgd[o] = D > 1e-7f ? D : 0.0f;
+    c.mga[o] = D > 1e-7f ? A : make_float4(0, 0, 0, 0);
+}
+
+// mg_build.comp:53-103: coarse level l from level l - 1, over coarse 16 x 16 tiles.
+__global__ __launch_bounds__(256) void k_mg_build(Ctx c, int l) {
+    __shared__ uint32_t s_state;
+    const Lv F = level(c, l - 1), C = level(c, l);
+    if (tid2() == 0) {
+        int tfx = (F.w + 15) >> 4, tfy = (F.h + 15) >> 4;
+        uint32_t on = 0;
+        for (int k = 0; k < 4; k++) {
+            int tx = 2 * (int)blockIdx.x + (k & 1), ty = 2 * (int)blockIdx.y + (k >> 1);
+            if (tx < tfx && ty < tfy) on |= c.ptile[F.toff + ty * tfx + tx];
+        }
+        uint32_t slot = C.toff + tile_slot();
+        uint32_t was = c.ptile[slot];
+        c.ptile[slot] = on ? 1u : 0u;
+        s_state = (on ? 1u : 0u) | (was ? 2u : 0u);
+    }
+    __syncthreads();
+    uint32_t state = s_state;
+    if (state == 0u) return;
+    int cx = blockIdx.x * 16 + threadIdx.x, cy = blockIdx.y * 16 + threadIdx.y;
+    if (!inside(C, cx, cy)) return;
+    if ((state & 1u) == 0u) {
+        c.mgd[at(C, cx, cy)] = 0.0f;
+        c.mga[at(C, cx, cy)] = make_float4(0, 0, 0, 0);
+        return;
+    }
+    build_cell(c, F, C, cx, cy);
+}
+
+// mg_tail.comp mode 1: the coarse matrices of every level below the tail, one group.
+__global__ __launch_bounds__(1024) void k_mg_tail_build(Ctx c) {
+    for (int l = c.tail; l < c.n_levels - 1; l++) {
+        const Lv F = level(c, l), C = level(c, l + 1);
+        for (int t = threadIdx.x; t < C.w * C.h; t += 1024) build_cell(c, F, C, t % C.w, t / C.w);
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ---- Krylov kernels on level 0 ---------------------------------------------------------
+
+// cg_scale.comp mode 0: rescale the warm start to this substep's length.
+__global__ __launch_bounds__(256) void k_cg_scale(Ctx c) {
+    if (c.ptile[tile_slot()] == 0u) return;
+    const Lv F = level(c, 0);
+    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (!inside(F, x, y)) return;
+    int i = at(F, x, y);
+    if (c.mgd[i] > 0.0f) {
+        float p = c.pres[i].x;
+        c.p2keep[i] = 0.0f;
+        c.pres[i].x = p * (c.S->dt / c.S->dt_prev);
+    } else {
+        c.pres[i] = make_float2(0.0f, 0.0f);
+    }
+}
+
+// cg_init.comp: r = b - A x and the partial sums of b.b (into the second half of parts).
+__global__ __launch_bounds__(256) void k_cg_init(Ctx c, int nblocks) {
+    __shared__ double4 lds[256];
+    const int slot = tile_slot();
+    if (c.ptile[slot] == 0u) {
+        if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
+        return;
+    }
+    const Lv F = level(c, 0);
+    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    double4 acc = make_double4(0, 0, 0, 0);
+    if (inside(F, x, y)) {
+        int i = at(F, x, y);
+        float D = c.mgd[i];
+        if (D > 0.0f) {
+            float2 xv = c.pres[i];
+            float4 w = c.mga[i];
+            float2 ax = make_float2(D * xv.x, D * xv.y);
+            for (int d = 0; d < 4; d++) {
+                int nx = x + NB_X[d], ny = y + NB_Y[d];
+                if (unknown(c, F, nx, ny)) {
+                    float wd = f4c(w, d);
+                    float2 pn = c.pres[at(F, nx, ny)];
+                    ax.x -= wd * pn.x;
+                    ax.y -= wd * pn.y;
+                }
+            }
+            float4 b = c.prhs[i];
+            c.mgr[i] = make_float2(b.x - ax.x, b.y - ax.y);
+            acc.x = (double)b.x * (double)b.x;
+            acc.y = (double)b.y * (double)b.y;
+        } else {
+            c.mgr[i] = make_float2(0.0f, 0.0f);
+        }
+    }
+    double4 s = group_sum4(acc, lds);
+    if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], s);
+}
+
+namespace {
+SP_DEV float2 direction(const Ctx& c, int i, float bx, float by, const float2* cgp_old) {
+    float2 z = c.mgz[i];
+    if (bx == 0.0f && by == 0.0f) return z;
+    float2 po = cgp_old[i];
+    return make_float2(bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y);
+}
+}  // namespace
+
+// cg_apply.comp: p = z + beta p_old, q = A p, partial p.q; the last group forms alpha.
+__global__ __launch_bounds__(256) void k_cg_apply(Ctx c, int nblocks, const float2* __restrict__ p_old, float2* __restrict__ p_new) {
+    __shared__ double4 lds[256];
+    if (solve_done(c)) return;
+    const int slot = tile_slot();
+    if (c.ptile[slot] == 0u) {
+        if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
+        finish_group(c, RED_ALPHA, nblocks, lds);
+        return;
+    }
+    const Lv F = level(c, 0);
+    const float bx = (float)c.K->beta[0], by = (float)c.K->beta[1];
+    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    double4 acc = make_double4(0, 0, 0, 0);
+    if (inside(F, x, y)) {
+        int i = at(F, x, y);
+        float D = c.mgd[i];
+        if (D > 0.0f) {
+            float2 p = direction(c, i, bx, by, p_old);
+            float4 w = c.mga[i];
+            float2 q = make_float2(D * p.x, D * p.y);
+            for (int d = 0; d < 4; d++) {
+                int nx = x + NB_X[d], ny = y + NB_Y[d];
+                if (unknown(c, F, nx, ny)) {
+                    float wd = f4c(w, d);
+                    float2 pn = direction(c, at(F, nx, ny), bx, by, p_old);
+                    q.x -= wd * pn.x;
+                    q.y -= wd * pn.y;
+                }
+            }
+            p_new[i] = p;
+            c.cgq[i] = q;
+            acc.x = (double)p.x * (double)q.x;
+            acc.y = (double)p.y * (double)q.y;
+        } else {
+            p_new[i] = make_float2(0.0f, 0.0f);
+        }
+    }
+    double4 s = group_sum4(acc, lds);
+    if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], s);
+    finish_group(c, RED_ALPHA, nblocks, lds);
+}
+
+// cg_update.comp: x += alpha p, r -= alpha q.
+__global__ __launch_bounds__(256) void k_cg_update(Ctx c, const float2* __restrict__ p_new) {
+    if (solve_done(c)) return;
+    if (c.ptile[tile_slot()] == 0u) return;
+    const Lv F = level(c, 0);
+    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (!inside(F, x, y)) return;
+    int i = at(F, x, y);
+    if (c.mgd[i] <= 0.0f) return;
+    const float ax = (float)c.K->alpha[0], ay = (float)c.K->alpha[1];
+    float2 p = p_new[i], q = c.cgq[i];
+    float2 x0 = c.pres[i], r = c.mgr[i];
+    x0.x += ax * p.x;
+    x0.y += ay * p.y;
+    r.x -= ax * q.x;
+    r.y -= ay * q.y;
+    c.pres[i] = x0;
+    c.mgr[i] = r;
+}
+
+// ---- the V-cycle ------------------------------------------------------------------------
+
+namespace {
+// Value of black cell (x, y) after the red half-sweep from zero (mg_down.comp:126-137).
+SP_DEV float2 down_black(const Ctx& c, const Lv& F, int x, int y) {
+    int i = at(F, x, y);
+    float4 w = c.mga[i];
+    float2 sum = c.mgr[i];
+    for (int d = 0; d < 4; d++) {
+        int mx = x + NB_X[d], my = y + NB_Y[d];
+        if (!unknown(c, F, mx, my)) continue;
+        int j = at(F, mx, my);
+        float wd = f4c(w, d);
+        float dj = c.mgd[j];
+        float2 rj = c.mgr[j];
+        sum.x += wd * rj.x / dj;
+        sum.y += wd * rj.y / dj;
+    }
+    float D = c.mgd[i];
+    return make_float2(sum.x / D, sum.y / D);
+}
+}  // namespace
+
+// mg_down.comp:139-172: pre-smoothing from zero on level l and restriction of the residual
+// to level l + 1.
+__global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated) {
+    __shared__ float2 sz[256];
+    if (gated && solve_done(c)) return;
+    const Lv F = level(c, l), C = level(c, l + 1);
+    if (c.ptile[F.toff + tile_slot()] == 0u) return;
+    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const bool red = ((x + y) & 1) == 0;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (inside(F, x, y)) {
+        int i = at(F, x, y);
+        if (c.mgd[i] > 0.0f) {
+            if (red) {
+                float D = c.mgd[i];
+                float2 r = c.mgr[i];
+                z = make_float2(r.x / D, r.y / D);
+            } else {
+                z = down_black(c, F, x, y);
+            }
+        }
+        c.mgx[i] = z;
+    }
+    sz[tid2()] = z;
+    __syncthreads();
+    if (((lx | ly) & 1) != 0) return;
+    const int cx = x >> 1, cy = y >> 1;
+    if (!inside(C, cx, cy)) return;
+    float2 sum = make_float2(0.0f, 0.0f);
+    for (int k = 0; k < 2; k++) {
+        int qx = x + k, qy = y + k;
+        if (!unknown(c, F, qx, qy)) continue;
+        float4 w = c.mga[at(F, qx, qy)];
+        for (int d = 0; d < 4; d++) {
+            int nx = qx + NB_X[d], ny = qy + NB_Y[d];
+            if (!unknown(c, F, nx, ny)) continue;
+            int lnx = lx + k + NB_X[d], lny = ly + k + NB_Y[d];
+            bool here = lnx >= 0 && lny >= 0 && lnx < 16 && lny < 16;
+            float2 zn = here ? sz[lny * 16 + lnx] : down_black(c, F, nx, ny);
+            float wd = f4c(w, d);
+            sum.x += wd * zn.x;
+            sum.y += wd * zn.y;
+        }
+    }
+    c.mgr[at(C, cx, cy)] = sum;
+}
+
+namespace {
+// New value of black cell (x, y) on the way up (mg_up.comp:198-208).
+SP_DEV float2 up_black(const Ctx& c, const Lv& F, const Lv& C, int x, int y) {
+    int i = at(F, x, y);
+    float4 w = c.mga[i];
+    float2 sum = c.mgr[i];
+    for (int d = 0; d < 4; d++) {
+        int mx = x + NB_X[d], my = y + NB_Y[d];
+        if (!unknown(c, F, mx, my)) continue;
+        float wd = f4c(w, d);
+        float2 xm = c.mgx[at(F, mx, my)];
+        float2 zc = c.mgz[at(C, mx >> 1, my >> 1)];
+        sum.x += wd * (xm.x + SP_MG_OMEGA * zc.x);
+        sum.y += wd * (xm.y + SP_MG_OMEGA * zc.y);
+    }
+    float D = c.mgd[i];
+    return make_float2(sum.x / D, sum.y / D);
+}
+}  // namespace
+
+// mg_up.comp:210-247: coarse-grid correction and the black-then-red post-smoothing on
+// level l. On level 0 (reduce >= 0) also the partial sums r.z, r.r and, in the last group,
+// the scalar step `reduce`.
+__global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int reduce, int nblocks) {
+    __shared__ float2 sz[256];
+    __shared__ double4 lds[256];
+    if (gated && solve_done(c)) return;
+    const Lv F = level(c, l), C = level(c, l + 1);
+    const int slot = tile_slot();
+    if (c.ptile[F.toff + slot] == 0u) {
+        if (reduce >= 0) {
+            if (tid2() == 0) store_part(&c.part_sums[slot], make_double4(0, 0, 0, 0));
+            finish_group(c, reduce, nblocks, lds);
+        }
+        return;
+    }
+    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const bool red = ((x + y) & 1) == 0;
+    const bool unk = unknown(c, F, x, y);
+    float2 z = make_float2(0.0f, 0.0f);
+    if (unk && !red) z = up_black(c, F, C, x, y);
+    sz[tid2()] = z;
+    __syncthreads();
+    double4 acc = make_double4(0, 0, 0, 0);
+    if (inside(F, x, y)) {
+        int i = at(F, x, y);
+        if (unk) {
+            float2 r = c.mgr[i];
+            if (red) {
+                float4 w = c.mga[i];
+                float2 sum = r;
+                for (int d = 0; d < 4; d++) {
+                    int nx = x + NB_X[d], ny = y + NB_Y[d];
+                    if (!unknown(c, F, nx, ny)) continue;
+                    int lnx = lx + NB_X[d], lny = ly + NB_Y[d];
+                    bool here = lnx >= 0 && lny >= 0 && lnx < 16 && lny < 16;
+                    float2 zn = here ? sz[lny * 16 + lnx] : up_black(c, F, C, nx, ny);
+                    float wd = f4c(w, d);
+                    sum.x += wd * zn.x;
+                    sum.y += wd * zn.y;
+                }
+                float D = c.mgd[i];
+                z = make_float2(sum.x / D, sum.y / D);
+            }
+            acc = make_double4((double)r.x * (double)z.x, (double)r.y * (double)z.y, (double)r.x * (double)r.x,
+                               (double)r.y * (double)r.y);
+        }
+        c.mgz[i] = z;
+    }
+    if (reduce >= 0) {
+        double4 s = group_sum4(acc, lds);
+        if (tid2() == 0) store_part(&c.part_sums[slot], s);
+        finish_group(c, reduce, nblocks, lds);
+    }
+}
+
+// mg_tail.comp mode 0 (mg_tail.comp:159-225): every level from the tail down in one
+// 1024 work-item group, with the iterate, diagonal and coarse right-hand sides in LDS.
+// Levels of more than 128 cells are swept by the whole group, smaller ones by its first
+// wave64 alone (the arithmetic and its order per cell are the same either way).
+#define WSYNC()                                                  \
+    do {                                                         \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   \
+        __builtin_amdgcn_wave_barrier();                         \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");   \
+    } while (0)
+
+__global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
+    __shared__ float2 sx[SP_TAIL_SHARED];
+    __shared__ float sd[SP_TAIL_SHARED];
+    __shared__ float2 sr[SP_TAIL_SHARED_COARSE];
+    __shared__ int4 sl[SP_MAX_LEVELS];
+    if (gated && solve_done(c)) return;
+    const int first = c.tail, last = c.n_levels - 1;
+    const int4 Lf = c.lev[first], Le = c.lev[last];
+    const int BASE = Lf.x;
+    const int total = Le.x + Le.y * Le.z - BASE;
+    const int NF = Lf.y * Lf.z;
+    const int bt = threadIdx.x;
+    for (int t = bt; t < total; t += 1024) sd[t] = c.mgd[BASE + t];
+    if (bt <= last) sl[bt] = c.lev[bt];
+    __syncthreads();
+
+    auto rhs_at = [&](int s) -> float2 { return s < NF ? c.mgr[BASE + s] : sr[s - NF]; };
+    auto sat = [&](int4 L, int x, int y) { return L.x - BASE + y * L.y + x; };
+    auto sunk = [&](int4 L, int x, int y) { return x >= 0 && y >= 0 && x < L.y && y < L.z && sd[sat(L, x, y)] > 0.0f; };
+    auto sweep = [&](int4 L, int colour, bool zero, int t0, int step) {
+        for (int t = t0; t < L.y * L.z; t += step) {
+            int x = t % L.y, y = t / L.y;
+            int s = L.x - BASE + t;
+            float D = sd[s];
+            if (D <= 0.0f || ((x + y) & 1) != colour) {
+                if (zero) sx[s] = make_float2(0.0f, 0.0f);
+                continue;
+            }
+            float2 sum = rhs_at(s);
+            if (!zero) {
+                float4 w = c.mga[BASE + s];
+                for (int d = 0; d < 4; d++) {
+                    int nx = x + NB_X[d], ny = y + NB_Y[d];
+                    if (sunk(L, nx, ny)) {
+                        float wd = f4c(w, d);
+                        float2 xn = sx[sat(L, nx, ny)];
+                        sum.x += wd * xn.x;
+                        sum.y += wd * xn.y;
+                    }
+                }
+            }
+            sx[s] = make_float2(sum.x / D, sum.y / D);
+        }
+    };
+    auto restrict_to = [&](int4 F, int4 C, int t0, int step) {
+        for (int t = t0; t < C.y * C.z; t += step) {
+            int cx = t % C.y, cy = t / C.y;
+            float2 sum = make_float2(0.0f, 0.0f);
+            for (int k = 0; k < 4; k++) {
+                int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
+                if (!sunk(F, x, y)) continue;
+                int s = sat(F, x, y);
+                float4 w = c.mga[BASE + s];
+                float2 az = make_float2(sd[s] * sx[s].x, sd[s] * sx[s].y);
+                for (int d = 0; d < 4; d++) {
+                    int nx = x + NB_X[d], ny = y + NB_Y[d];
+                    if (sunk(F, nx, ny)) {
+                        float wd = f4c(w, d);
+                        float2 xn = sx[sat(F, nx, ny)];
+                        az.x -= wd * xn.x;
+                        az.y -= wd * xn.y;
+                    }
+                }
+                float2 r = rhs_at(s);
+                sum.x += r.x - az.x;
+                sum.y += r.y - az.y;
+            }
+            sr[C.x - BASE + t - NF] = sum;
+        }
+    };
+    auto post_black = [&](int4 F, int4 C, int t0, int step) {
+        for (int t = t0; t < F.y * F.z; t += step) {
+            int x = t % F.y, y = t / F.y;
+            if (((x + y) & 1) == 0) continue;
+            int s = F.x - BASE + t;
+            float D = sd[s];
+            if (D <= 0.0f) continue;
+            float4 w = c.mga[BASE + s];
+            float2 sum = rhs_at(s);
+            for (int d = 0; d < 4; d++) {
+                int nx = x + NB_X[d], ny = y + NB_Y[d];
+                if (!sunk(F, nx, ny)) continue;
+                float wd = f4c(w, d);
+                float2 xn = sx[sat(F, nx, ny)];
+                float2 zc = sx[sat(C, nx >> 1, ny >> 1)];
+                sum.x += wd * (xn.x + SP_MG_OMEGA * zc.x);
+                sum.y += wd * (xn.y + SP_MG_OMEGA * zc.y);
+            }
+            sx[s] = make_float2(sum.x / D, sum.y / D);
+        }
+    };
+
+    int ls = first;
+    while (ls < last && sl[ls].y * sl[ls].z > SP_TAIL_WARP_CELLS) ls++;
+    for (int l = first; l < ls; l++) {
+        sweep(sl[l], 0, true, bt, 1024);
+        __syncthreads();
+        sweep(sl[l], 1, false, bt, 1024);
+        __syncthreads();
+        restrict_to(sl[l], sl[l + 1], bt, 1024);
+        __syncthreads();
+    }
+    if (bt < 64) {
+        const int wt = bt, ws = 64;
+        for (int l = ls; l < last; l++) {
+            sweep(sl[l], 0, true, wt, ws);
+            WSYNC();
+            sweep(sl[l], 1, false, wt, ws);
+            WSYNC();
+            restrict_to(sl[l], sl[l + 1], wt, ws);
+            WSYNC();
+        }
+        sweep(sl[last], 0, true, wt, ws);
+        WSYNC();
+        for (int h = 1; h < SP_TAIL_HALF_SWEEPS; h++) {
+            sweep(sl[last], h & 1, false, wt, ws);
+            WSYNC();
+        }
+        for (int l = last - 1; l >= ls; l--) {
+            post_black(sl[l], sl[l + 1], wt, ws);
+            WSYNC();
+            sweep(sl[l], 0, false, wt, ws);
+            WSYNC();
+        }
+    }
+    __syncthreads();
+    for (int l = ls - 1; l >= first; l--) {
+        post_black(sl[l], sl[l + 1], bt, 1024);
+        __syncthreads();
+        sweep(sl[l], 0, false, bt, 1024);
+        __syncthreads();
+    }
+    for (int t = bt; t < NF; t += 1024) c.mgz[BASE + t] = sx[t];
+}
+
+// Solve statistics at the end of a solve (reduce_tail.glsl:105-116, mode 3).
+__global__ void k_cg_end(Ctx c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sp_solve* K = c.K;
+    uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];
+    if (it > K->it_max) K->it_max = it;
+    if (K->done[0] == 0u) K->unconv += 1u;
+    float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
+    if (rel > K->res_max) K->res_max = rel;
+    K->iters_last = it;
+}

@@ -1,0 +1,181 @@
+"""FluidSim — the reference's solver-host interface (src/sim/mod.rs:862-3492) over the C ABI
+of libsprout_hip.so.
+
+Method names, argument meaning and error behaviour follow the reference's `FluidSim`:
+`new`, `reset`, `set_solids`, `spawn`, `record`, `step_blocking`, `advance_blocking`,
+`sim_time`, `download_particles`, `download_grid`, `download_pressure`, `cell_counts`,
+`request_stats` / `stats`, `solver` (the SolverStats), `params`. Host buffers are copied in
+and out; device memory stays owned by the handle.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class FluidSim:
+    def __init__(self, grid, capacity, device=0, params=None):
+        """FluidSim::new(gpu, grid, capacity) (mod.rs:1034)."""
+        self.grid = (int(grid[0]), int(grid[1]))
+        self.capacity = int(capacity)
+        L = N.lib()
+        cfg = N.Config(self.grid[0], self.grid[1], self.capacity, device)
+        h = C.c_void_p()
+        rc = L.sp_create(C.byref(cfg), C.byref(h))
+        if rc != N.OK:
+            raise N.SproutError(rc, L.sp_create_error().decode())
+        self._h = h
+        self._lib = L
+        self.params = params if params is not None else N.default_params()
+        self.set_params(self.params)
+
+    # -- lifetime ----------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            raise N.SproutError(rc, f"{what}: {self._lib.sp_last_error(self._h).decode()}")
+
+    # -- inputs ------------------------------------------------------------------------
+    def set_params(self, params):
+        """Assigning FluidSim::params (the grid-unit tables follow at the next record)."""
+        self.params = params
+        self._check(self._lib.sp_set_params(self._h, C.byref(params)), "set_params")
+
+    def set_solids(self, field):
+        """FluidSim::set_solids(SolidField) (mod.rs:1977)."""
+        W, H = self.grid
+        sdf = np.ascontiguousarray(field.sdf, np.float32)
+        solid = np.ascontiguousarray(field.solid, np.float32)
+        region = np.ascontiguousarray(field.region, np.uint32)
+        assert sdf.size == (W + 1) * (H + 1) and solid.size == 4 * W * H and region.size == W * H
+        self._check(self._lib.sp_set_solids(self._h, sdf.ctypes.data, solid.ctypes.data, region.ctypes.data, None),
+                    "set_solids")
+
+    def reset(self, particles):
+        """FluidSim::reset(Vec<Particle>) (mod.rs:1982-2001)."""
+        p = np.ascontiguousarray(particles, N.PARTICLE_DTYPE)
+        self._check(self._lib.sp_reset(self._h, p.ctypes.data if len(p) else None, len(p)), "reset")
+
+    def spawn(self, particles):
+        """FluidSim::spawn (mod.rs:2027-2048): returns how many were accepted."""
+        p = np.ascontiguousarray(particles, N.PARTICLE_DTYPE)
+        n = C.c_uint32()
+        self._check(self._lib.sp_spawn(self._h, p.ctypes.data if len(p) else None, len(p), C.byref(n)), "spawn")
+        return n.value
+
+    # -- stepping ----------------------------------------------------------------------
+    def record(self, substeps, target, tool=None):
+        """FluidSim::record (mod.rs:2400-2563): enqueue only."""
+        t = C.byref(tool) if tool is not None else None
+        self._check(self._lib.sp_record(self._h, substeps, float(target), t), "record")
+
+    def wait(self):
+        self._check(self._lib.sp_wait(self._h), "wait")
+
+    def step_blocking(self, substeps):
+        """FluidSim::step_blocking (mod.rs:3316-3338): ticks of 1/240 s."""
+        self._check(self._lib.sp_step_blocking(self._h, substeps), "step_blocking")
+
+    def advance_blocking(self, seconds):
+        """FluidSim::advance_blocking (mod.rs:3284-3312)."""
+        self._check(self._lib.sp_advance_blocking(self._h, float(seconds)), "advance_blocking")
+
+    def sim_time(self):
+        t = C.c_double()
+        self._check(self._lib.sp_sim_time(self._h, C.byref(t)), "sim_time")
+        return t.value
+
+    @property
+    def solver(self):
+        """FluidSim::solver (SolverStats, mod.rs:334-353)."""
+        s = N.SolverStats()
+        self._check(self._lib.sp_poll(self._h, C.byref(s)), "poll")
+        return s
+
+    def poll_readbacks(self):
+        return self.solver
+
+    def timings(self):
+        out = np.zeros(6, np.float32)
+        self._check(self._lib.sp_timings(self._h, out.ctypes.data), "timings")
+        return out
+
+    def sort(self):
+        self._check(self._lib.sp_sort(self._h), "sort")
+
+    # -- outputs -----------------------------------------------------------------------
+    def download_particles(self):
+        """FluidSim::download_particles (mod.rs:3421-3432): live particles in slot order."""
+        n = C.c_uint32()
+        self._check(self._lib.sp_download_particles(self._h, None, 0, C.byref(n)), "download_particles")
+        out = np.zeros(n.value, N.PARTICLE_DTYPE)
+        if n.value:
+            self._check(self._lib.sp_download_particles(self._h, out.ctypes.data, n.value, C.byref(n)),
+                        "download_particles")
+        return out[: n.value]
+
+    def download_grid(self):
+        """FluidSim::download_grid (mod.rs:3436-3450): (vel[H,W,4], cell[H,W,4])."""
+        W, H = self.grid
+        vel = np.zeros((H, W, 4), np.float32)
+        cell = np.zeros((H, W, 4), np.float32)
+        self._check(self._lib.sp_download_grid(self._h, vel.ctypes.data, cell.ctypes.data), "download_grid")
+        return vel, cell
+
+    def download_pressure(self):
+        """FluidSim::download_pressure (mod.rs:3464-3475): (p*dt per cell [H,W], dt)."""
+        W, H = self.grid
+        p = np.zeros((H, W), np.float32)
+        dt = C.c_float()
+        self._check(self._lib.sp_download_pressure(self._h, p.ctypes.data, C.byref(dt)), "download_pressure")
+        return p, dt.value
+
+    def cell_counts(self, lo, hi):
+        """FluidSim::cell_counts(lo, hi) (mod.rs:3354-3386)."""
+        w, h = hi[0] - lo[0], hi[1] - lo[1]
+        out = np.zeros(max(w, 0) * max(h, 0), np.uint32)
+        self._check(self._lib.sp_cell_counts(self._h, lo[0], lo[1], w, h, out.ctypes.data if out.size else None),
+                    "cell_counts")
+        return out
+
+    def stats(self):
+        """Stats (mod.rs:128-143, stats.comp): live particles, per-row counts and concentrations."""
+        s = N.StatsOut()
+        self._check(self._lib.sp_stats(self._h, C.byref(s)), "stats")
+        return s
+
+    # -- stage-level access (parity tests, diagnostics) ---------------------------------
+    def field(self, name, dtype, shape):
+        out = np.zeros(shape, dtype)
+        self._check(self._lib.sp_debug_download(self._h, N.FIELD[name], out.ctypes.data, out.nbytes), f"field {name}")
+        return out
+
+    def put_field(self, name, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(self._lib.sp_debug_upload(self._h, N.FIELD[name], arr.ctypes.data, arr.nbytes), f"put {name}")
+
+    def stage(self, name, arg=0):
+        self._check(self._lib.sp_debug_stage(self._h, N.STAGE[name], arg), f"stage {name}")
+
+    def grid_params(self):
+        g = N.GParams()
+        mats = (N.GMat * N.MAX_MATERIALS)()
+        self._check(self._lib.sp_debug_gparams(self._h, C.byref(g), mats), "grid_params")
+        return g, mats
