@@ -137,10 +137,15 @@ int sp_debug_upload(sp_sim* sim, int32_t field, const void* data, size_t bytes);
 int sp_debug_stage(sp_sim* sim, int32_t stage, int32_t arg);
 /* The grid-unit tables the handle currently uses. */
 int sp_debug_gparams(sp_sim* sim, sp_gparams* out, sp_gmat* mats);
-/* Device time of the last recorded batch's phases (ms): [0] total, [1] particle pass,
- * [2] grid passes before the solve, [3] pressure solve, [4] projection; and the pressure
- * iterations of the last substep in [5]. Valid after sp_wait. */
+/* Device time per substep of the last recorded batch's phases (ms, averaged over its
+ * substeps): [0] total, [1] step + accumulator clear + particle pass, [2] grid passes
+ * before the solve, [3] pressure solve, [4] projection; the pressure iterations of the last
+ * substep in [5]. Valid after sp_wait. */
 int sp_timings(sp_sim* sim, float* out6);
+/* Average device time (ms) of one launch of a hot kernel over the substeps of the last
+ * batch (kernel 0: the particle pass, k_particles), timed with HIP events on the handle's
+ * stream, and the number of launches averaged. */
+int sp_kernel_time(sp_sim* sim, int32_t kernel, float* avg_ms, uint32_t* launches);
 
 #ifdef __cplusplus
 }

@@ -167,6 +167,7 @@ SIGNATURES = {
     "sp_debug_stage": ([_P, C.c_int32, C.c_int32], C.c_int),
     "sp_debug_gparams": ([_P, C.POINTER(GParams), _P], C.c_int),
     "sp_timings": ([_P, _P], C.c_int),
+    "sp_kernel_time": ([_P, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)], C.c_int),
 }
 
 _lib = None

@@ -5,6 +5,8 @@
 // fixed-point scatter sums, cell indices) come out bit-identical to a CPU evaluation of the
 // same inputs. GLSL built-ins are spelled out with the semantics used throughout this
 // build: mix(x, y, a) = x (1 - a) + y a, length = sqrt(x^2 + y^2), normalize = v / length.
+// Square roots use sqrtf, which hipcc lowers to the correctly rounded llvm.sqrt.f32
+// (__fsqrt_rn carries !fpmath 3.0, a 3-ulp approximation); '/' is correctly rounded.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,9 +46,8 @@ SP_DEV float fminf_(float a, float b) { return a < b ? a : b; }
 SP_DEV float clampf_(float x, float lo, float hi) { return fminf_(fmaxf_(x, lo), hi); }
 SP_DEV int clampi_(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 SP_DEV float mixf_(float x, float y, float a) { return x * (1.0f - a) + y * a; }
-SP_DEV float len2_(float x, float y) { return __fsqrt_rn(x * x + y * y); }
+SP_DEV float len2_(float x, float y) { return sqrtf(x * x + y * y); }
 SP_DEV float signf_(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
-SP_DEV float div_(float a, float b) { return __fdiv_rn(a, b); }
 SP_DEV bool face_live(float s) { return fabsf(s) < 1e29f; }
 SP_DEV bool face_closed(float s) { return s < -1e29f; }
 SP_DEV float f4c(const float4& a, int k) { return k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w; }

@@ -25,7 +25,7 @@ __global__ void k_step(Ctx c) {
     const sp_gparams& P = *c.P;
     float u = fmaxf_(fminf_(__uint_as_float(S->umax_grid), P.ext[0]), __uint_as_float(S->umax_part));
     u = fmaxf_(u, P.ext_speed);
-    float dt_cfl = P.cfl / (u + __fsqrt_rn(5.0f * fmaxf_(P.gravity, 0.0f)) + 1e-6f);
+    float dt_cfl = P.cfl / (u + sqrtf(5.0f * fmaxf_(P.gravity, 0.0f)) + 1e-6f);
     float dt;
     if (P.dt_fixed > 0.0f) {
         dt = P.dt_fixed;

@@ -34,7 +34,7 @@ SP_DEV float row_density(const Ctx& c, int row, float s) {
 // Terminal drift speed (common.glsl:370-374).
 SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d) {
     float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * c.P->gravity * d;
-    return qc > 0.0f ? 2.0f * qc / (qb + __fsqrt_rn(qb * qb + 4.0f * qa * qc)) : 0.0f;
+    return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
 }
 
 // Bilinear SDF and its gradient (air.glsl:13-22).

@@ -94,10 +94,16 @@ struct sp_sim {
     uint64_t substeps_base = 0;
     uint32_t last_unconv = 0;
 
-    // timing
+    // timing: per substep of a batch (up to TIMED_SUBSTEPS), events at the phase
+    // boundaries and tightly around the particle kernel
+    static constexpr int TIMED_SUBSTEPS = 64;
+    static constexpr int EV_PER = 7;
     bool timing = false;
-    hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> ev;
+    uint32_t timed = 0;  // substeps of the current batch with events recorded
+    uint32_t timed_last = 0;
     float timings[6] = {};
+    float particle_ms = 0.0f;
 
     Ctx ctx() const { return c; }
 };
@@ -195,17 +201,21 @@ void launch_sort(sp_sim* s) {
 
 void launch_substep(sp_sim* s, int gather, uint32_t p_iters) {
     const Ctx& c = s->c;
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->stream));
+    hipEvent_t* e = nullptr;
+    if (s->timing && s->timed < (uint32_t)sp_sim::TIMED_SUBSTEPS) e = &s->ev[(size_t)s->timed++ * sp_sim::EV_PER];
+    if (e) HIPCHK(hipEventRecord(e[0], s->stream));
     hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c);
     launch_clear_acc(s);
+    if (e) HIPCHK(hipEventRecord(e[5], s->stream));
     launch_particles(s, gather);
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[1], s->stream));
+    if (e) HIPCHK(hipEventRecord(e[6], s->stream));
+    if (e) HIPCHK(hipEventRecord(e[1], s->stream));
     launch_grid_passes(s);
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[2], s->stream));
+    if (e) HIPCHK(hipEventRecord(e[2], s->stream));
     launch_pressure(s, p_iters);
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[3], s->stream));
+    if (e) HIPCHK(hipEventRecord(e[3], s->stream));
     hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, c);
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[4], s->stream));
+    if (e) HIPCHK(hipEventRecord(e[4], s->stream));
     HIPCHK(hipGetLastError());
 }
 
@@ -319,14 +329,30 @@ int record(sp_sim* s, uint32_t substeps, float target) {
 
 int wait(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (s->timing) {
-        float t[5];
-        for (int k = 0; k < 4; k++) HIPCHK(hipEventElapsedTime(&t[k], s->ev[k], s->ev[k + 1]));
-        s->timings[1] = t[0];
-        s->timings[2] = t[1];
-        s->timings[3] = t[2];
-        s->timings[4] = t[3];
-        s->timings[0] = t[0] + t[1] + t[2] + t[3];
+    if (s->timing && s->timed > 0) {
+        // averages per substep: [0] total, [1] step + clear + particle pass, [2] grid passes,
+        // [3] pressure solve, [4] projection; [5] below is the iteration count
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (uint32_t k = 0; k < s->timed; k++) {
+            hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
+            float t;
+            for (int q = 0; q < 4; q++) {
+                HIPCHK(hipEventElapsedTime(&t, e[q], e[q + 1]));
+                acc[q + 1] += t;
+                acc[0] += t;
+            }
+        }
+        for (int q = 0; q < 5; q++) s->timings[q] = (float)(acc[q] / s->timed);
+        double kp = 0;
+        for (uint32_t k = 0; k < s->timed; k++) {
+            hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, e[5], e[6]));
+            kp += t;
+        }
+        s->particle_ms = (float)(kp / s->timed);
+        s->timed_last = s->timed;
+        s->timed = 0;
     }
     readback(s);
     s->timings[5] = (float)s->stats.pressure_iters_last;
@@ -421,6 +447,7 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         s->device = cfg->device;
         HIPCHK(hipSetDevice(s->device));
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->ev.assign((size_t)sp_sim::TIMED_SUBSTEPS * sp_sim::EV_PER, nullptr);
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         sp::levels(s->W, s->H, s->lv);
         s->nb0 = ((s->W + 15) / 16) * ((s->H + 15) / 16);
@@ -795,6 +822,15 @@ int sp_timings(sp_sim* s, float* out6) {
     return guarded(s, [&]() -> int {
         if (!out6) return SP_E_INVALID;
         for (int k = 0; k < 6; k++) out6[k] = s->timings[k];
+        return SP_OK;
+    });
+}
+
+int sp_kernel_time(sp_sim* s, int32_t kernel, float* avg_ms, uint32_t* launches) {
+    return guarded(s, [&]() -> int {
+        if (!avg_ms || !launches || kernel != 0) return SP_E_INVALID;
+        *avg_ms = s->particle_ms;
+        *launches = s->timed_last;
         return SP_OK;
     });
 }

@@ -117,6 +117,12 @@ class FluidSim:
         self._check(self._lib.sp_timings(self._h, out.ctypes.data), "timings")
         return out
 
+    def kernel_time(self, kernel=0):
+        """(average ms per launch, launches) of a hot kernel in the last batch."""
+        ms, n = C.c_float(), C.c_uint32()
+        self._check(self._lib.sp_kernel_time(self._h, kernel, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
     def sort(self):
         self._check(self._lib.sp_sort(self._h), "sort")
 

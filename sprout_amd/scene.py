@@ -293,6 +293,19 @@ class Rng:
         return np.float32(x >> 8) / np.float32(1 << 24)
 
 
+def _rng_stream(seed, n):
+    """n successive Rng(seed).next() values: (x >> 8) / 2^24 is exact in float32."""
+    x = seed & 0xFFFFFFFF
+    out = [0] * n
+    M = 0xFFFFFFFF
+    for k in range(n):
+        x ^= (x << 13) & M
+        x ^= x >> 17
+        x ^= (x << 5) & M
+        out[k] = x >> 8
+    return (np.asarray(out, np.float64) * (1.0 / (1 << 24))).astype(np.float32)
+
+
 def new_particles(row, pos):
     """Particle::new(material, pos, 0) for every position (params.rs:60-62)."""
     out = np.zeros(len(pos), PARTICLE_DTYPE)
@@ -315,12 +328,8 @@ def fill_rect(scene, rect, row, n_before=0):
     x1 = int(_rnd(float(np.float32(min(np.float32(rect[2]), np.float32(w - W))) / step)))
     y0 = int(_rnd(float(np.float32(max(np.float32(rect[1]), W)) / step)))
     y1 = int(_rnd(float(np.float32(min(np.float32(rect[3]), np.float32(h - W))) / step)))
-    rng = Rng(0x9E3779B9 ^ ((n_before * 747796405 + 1) & 0xFFFFFFFF))
     nx, ny = max(x1 - x0, 0), max(y1 - y0, 0)
-    r = np.empty(2 * nx * ny, np.float32)
-    for k in range(2 * nx * ny):
-        r[k] = rng.next()
-    r = r.reshape(ny, nx, 2)
+    r = _rng_stream(0x9E3779B9 ^ ((n_before * 747796405 + 1) & 0xFFFFFFFF), 2 * nx * ny).reshape(ny, nx, 2)
     I, J = np.meshgrid(np.arange(x0, x1, dtype=np.float32), np.arange(y0, y1, dtype=np.float32))
     half, q = np.float32(0.5), np.float32(0.5)
     px = ((I + half + (r[..., 0] - half) * q) * step).astype(np.float32).reshape(-1)

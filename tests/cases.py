@@ -23,13 +23,15 @@ def settle(parts, params):
     """Particle::settle_spawn (params.rs:70-88) for ids 0..n, isothermal rows."""
     q = parts.copy()
     q["id"] = np.arange(len(q), dtype=np.uint32)
-    for k in range(len(q)):
-        row = int(q["kind"][k] & 255) - 1
-        m = params.mat[row]
-        if q["t"][k] == N.SPAWN_TEMP:
-            q["t"][k] = m.spawn_temp
-        if q["s"][k] == N.SPAWN_S:
-            q["s"][k] = min(max(m.load, 0.0), 1.0) if m.diluent >= 0 else 1.0
+    rows = (q["kind"] & 255).astype(np.int64) - 1
+    for row in np.unique(rows):
+        m = params.mat[int(row)]
+        sel = rows == row
+        t = q["t"][sel]
+        q["t"][sel] = np.where(t == N.SPAWN_TEMP, np.float32(m.spawn_temp), t)
+        sv = q["s"][sel]
+        load = np.float32(min(max(m.load, 0.0), 1.0)) if m.diluent >= 0 else np.float32(1.0)
+        q["s"][sel] = np.where(sv == N.SPAWN_S, load, sv)
     return q
 
 

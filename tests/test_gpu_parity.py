@@ -173,8 +173,8 @@ def test_pressure_solve_matches_oracle(small):
     assert rel_l2(a[:, 0], b[:, 0]) <= 1e-5
     assert rel_l2(a[:, 1], b[:, 1]) <= 1e-5
     ka, kb = p.f("SOLVE", N.SOLVE_DTYPE, 1)
-    assert ka["done"].tolist() == [1, 1]
-    assert abs(int(ka["iters"].max()) - int(kb["iters"].max())) <= 1
+    assert ka[0]["done"].tolist() == [1, 1]
+    assert abs(int(ka[0]["iters"].max()) - int(kb[0]["iters"].max())) <= 1
 
 
 def test_vcycle_bitexact(small):

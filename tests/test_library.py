@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every entry point include/sprout_hip.h declares (no
+compute: these run without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from sprout_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:int|void|const char\*)\s+(sp_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(N.LIB_PATH)
+    names = declared()
+    assert len(names) >= 25
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_header():
+    assert declared() == set(N.SIGNATURES)
+
+
+def test_struct_layouts():
+    assert N.PARTICLE_DTYPE.itemsize == 48  # params.rs:1319
+    assert C.sizeof(N.GMat) == 80
+    assert N.STEP_DTYPE.itemsize == 48      # 12 words (mod.rs:123 STEP_WORDS)
+
+
+def test_errors_cross_the_abi_as_codes():
+    L = N.lib()
+    h = C.c_void_p()
+    cfg = N.Config(4, 4, 16, 0)
+    assert L.sp_create(C.byref(cfg), C.byref(h)) == N.E_INVALID
+    assert b"grid" in L.sp_create_error()
+    assert L.sp_create(None, C.byref(h)) == N.E_INVALID
+    p = N.default_params()
+    g = N.GParams()
+    mats = (N.GMat * N.MAX_MATERIALS)()
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, -1, C.byref(g), mats) == N.E_INVALID
+    # thirteen materials do not fit a tank (MAX_PRESENT 12, mod.rs:2051-2081)
+    rows = (C.c_int32 * 13)(*range(13))
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, rows, 13, C.byref(g), mats) == N.E_CAPACITY
+
+
+def test_viscosity_modes_follow_the_diffusion_number():
+    """viscosity_mode (mod.rs:2768-2792): none for an inviscid tank, explicit for water,
+    implicit for honey."""
+    p = N.default_params()
+    p.mat[0].viscosity = 0.0
+    g, _ = N.derive_grid_params(p, 64, 64, [0])
+    assert g.visc_mode == 0
+    p = N.default_params()
+    g, _ = N.derive_grid_params(p, 64, 64, [0])
+    assert g.visc_mode == 1
+    g, _ = N.derive_grid_params(p, 64, 64, [1])
+    assert g.visc_mode == 2
