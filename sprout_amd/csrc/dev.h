@@ -100,6 +100,24 @@ SP_DEV void atomic_max_pos_float(uint32_t* addr, float v) {
     if (v > 0.0f) atomicMax(addr, __float_as_uint(v));
 }
 
+// Work-group maximum of a non-negative float folded into *addr (float bits) with one
+// atomic per group, and only when the group's maximum exceeds what *addr already holds:
+// one atomic per wave on a single word serialises thousands of groups. `lds` holds one
+// float per wave64 of the group; every work-item of the group must call it.
+template <int NT>
+SP_DEV void group_max_to(uint32_t* addr, float v, float* lds) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    v = wave_max(v);
+    if ((t & 63) == 0) lds[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        float m = lds[0];
+        for (int k = 1; k < NT / 64; k++) m = fmaxf_(m, lds[k]);
+        if (m > 0.0f && m > __uint_as_float(__hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            atomicMax(addr, __float_as_uint(m));
+    }
+}
+
 // Whether phase group g sorts itself by density below the grid scale (common.glsl:590-602).
 SP_DEV bool group_sorts(const sp_gparams& P, const sp_gmat* mats, int g) {
     if (g <= 0) return false;

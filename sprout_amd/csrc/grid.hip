@@ -589,6 +589,6 @@ __global__ __launch_bounds__(256) void k_project(Ctx c) {
         if (face_live(me.z) && cf.x != 0.0f) m = fabsf(me.x);
         if (face_live(me.w) && cf.y != 0.0f) m = fmaxf_(m, fabsf(me.y));
     }
-    m = wave_max(m);
-    if (((threadIdx.y * 16 + threadIdx.x) & 63) == 0) atomic_max_pos_float(&c.S->umax_grid, m);
+    __shared__ float s_max[4];
+    group_max_to<256>(&c.S->umax_grid, m, s_max);
 }

@@ -502,9 +502,9 @@ __global__ __launch_bounds__(PT) void k_particles(Ctx c, int gather) {
     const uint32_t id = blockIdx.x * PT + t;
     ScatterIn g;
     float umax = 0.0f;
+    __shared__ float s_max[PT / 64];
     advance(c, id, gather, g, umax);
-    umax = wave_max(umax);
-    if ((t & 63) == 0) atomic_max_pos_float(&c.S->umax_part, umax);
+    group_max_to<PT>(&c.S->umax_part, umax, s_max);
 
     if (t == 0) {
         sbox[0] = 0x7fffffff;
@@ -556,14 +556,19 @@ __global__ __launch_bounds__(PT) void k_particles(Ctx c, int gather) {
 // first, which makes the permutation a defined one (the reference's is atomic-ordered,
 // sort.comp:64).
 
-__global__ __launch_bounds__(256) void k_sort_keys(Ctx c, uint32_t* keys, uint32_t* vals) {
+// tiled = 0: key = cell (the reference's order); tiled = 1: key = 8 x 8 tile, then cell within
+// the tile. The substep loop keeps the particles in tiled order (neighbours in memory are
+// neighbours in both x and y, so a work-group's scatter box stays near 11 x 11 cells);
+// sp_sort gives the reference's order. Either order leaves every result unchanged.
+__global__ __launch_bounds__(256) void k_sort_keys(Ctx c, uint32_t* keys, uint32_t* vals, int tiled, uint32_t dead) {
     uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= c.P->n_slots) return;
     sp_particle p = c.part[id];
-    uint32_t key = (uint32_t)c.N;
+    uint32_t key = dead;
     if (p.kind != SP_KIND_DEAD) {
         int x = clampi_((int)p.pos[0], 0, c.W - 1), y = clampi_((int)p.pos[1], 0, c.H - 1);
-        key = (uint32_t)(y * c.W + x);
+        if (tiled) key = (uint32_t)((((y >> 3) * ((c.W + 7) >> 3) + (x >> 3)) << 6) | ((y & 7) << 3) | (x & 7));
+        else key = (uint32_t)(y * c.W + x);
     }
     keys[id] = key;
     vals[id] = id;

@@ -53,28 +53,14 @@ SP_DEV double4 group_sum4(double4 v, double4* lds) {
     return lds[0];
 }
 
-SP_DEV void store_part(double4* p, double4 v) {
-    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&p->z, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&p->w, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-SP_DEV double4 load_part(const double4* p) {
-    double4 v;
-    v.x = __hip_atomic_load(&p->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v.y = __hip_atomic_load(&p->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v.z = __hip_atomic_load(&p->z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v.w = __hip_atomic_load(&p->w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return v;
-}
+SP_DEV void store_part(double4* p, double4 v) { *p = v; }
 
 // Sum of parts[base .. base + n) by one 256 work-item group in a fixed order.
 SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) {
     const int t = tid2();
     double4 s = make_double4(0, 0, 0, 0);
     for (int k = t; k < n; k += 256) {
-        double4 v = load_part(&c.part_sums[base + k]);
+        double4 v = c.part_sums[base + k];
         s.x += v.x;
         s.y += v.y;
         s.z += v.z;
@@ -132,23 +118,6 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
             K->done[1] = d1;
         }
     }
-}
-
-// Called by every work-item of a 256 work-item group after the group's partial is in
-// part_sums: the last group of the launch to arrive runs the scalar step (reduce_tail.glsl:
-// 209-223).
-SP_DEV void finish_group(const Ctx& c, int mode, int nblocks, double4* lds) {
-    __shared__ int s_last;
-    if (tid2() == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&c.K->arrive, 1u);
-        s_last = prev + 1u == (uint32_t)nblocks ? 1 : 0;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    reduce_scalars(c, mode, nblocks, lds);
-    if (tid2() == 0) c.K->arrive = 0u;
 }
 
 }  // namespace
@@ -277,41 +246,77 @@ __global__ __launch_bounds__(256) void k_cg_init(Ctx c, int nblocks) {
     if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], s);
 }
 
+// ---- LDS-staged level kernels ----------------------------------------------------------
+//
+// Each work-group owns a 16 x 16 tile of its level. It first loads what its stencils reach
+// (the tile plus a ring of one or two cells: diagonals, couplings, right-hand sides,
+// iterates) into LDS with every work-item issuing its loads at once, then computes from
+// LDS. The arithmetic per cell, and its order, is that of the reference's kernels (which
+// read neighbours from global memory and recompute values they do not hold); only where
+// the operands come from changes, so results are bit-identical.
+
 namespace {
-SP_DEV float2 direction(const Ctx& c, int i, float bx, float by, const float2* cgp_old) {
-    float2 z = c.mgz[i];
+
+constexpr int R1 = 18;  // tile + 1-cell ring
+constexpr int R2 = 20;  // tile + 2-cell ring
+
+// Index of global cell (x, y) in a ring-r staging array of the tile at (x0, y0).
+SP_DEV int sidx(int x, int y, int x0, int y0, int r) { return (y - y0 + r) * (16 + 2 * r) + (x - x0 + r); }
+
+SP_DEV float2 direction_of(float2 z, float2 po, float bx, float by) {
     if (bx == 0.0f && by == 0.0f) return z;
-    float2 po = cgp_old[i];
     return make_float2(bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y);
 }
+
 }  // namespace
 
-// cg_apply.comp: p = z + beta p_old, q = A p, partial p.q; the last group forms alpha.
+// cg_apply.comp: p = z + beta p_old, q = A p, and the partial sums of p.q (second half of
+// the partial buffer).
 __global__ __launch_bounds__(256) void k_cg_apply(Ctx c, int nblocks, const float2* __restrict__ p_old, float2* __restrict__ p_new) {
     __shared__ double4 lds[256];
+    __shared__ float sd[R1 * R1];
+    __shared__ float2 sp[R1 * R1];
     if (solve_done(c)) return;
     const int slot = tile_slot();
     if (c.ptile[slot] == 0u) {
         if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
-        finish_group(c, RED_ALPHA, nblocks, lds);
         return;
     }
     const Lv F = level(c, 0);
     const float bx = (float)c.K->beta[0], by = (float)c.K->beta[1];
-    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+    const int t = tid2();
+    for (int k = t; k < R1 * R1; k += 256) {
+        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+        float d = 0.0f;
+        float2 pv = make_float2(0.0f, 0.0f);
+        if (inside(F, gx, gy)) {
+            int i = at(F, gx, gy);
+            d = c.mgd[i];
+            if (d > 0.0f) {
+                float2 po = (bx == 0.0f && by == 0.0f) ? make_float2(0.0f, 0.0f) : p_old[i];
+                pv = direction_of(c.mgz[i], po, bx, by);
+            }
+        }
+        sd[k] = d;
+        sp[k] = pv;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     double4 acc = make_double4(0, 0, 0, 0);
     if (inside(F, x, y)) {
         int i = at(F, x, y);
-        float D = c.mgd[i];
+        int si = sidx(x, y, x0, y0, 1);
+        float D = sd[si];
         if (D > 0.0f) {
-            float2 p = direction(c, i, bx, by, p_old);
+            float2 p = sp[si];
             float4 w = c.mga[i];
             float2 q = make_float2(D * p.x, D * p.y);
             for (int d = 0; d < 4; d++) {
-                int nx = x + NB_X[d], ny = y + NB_Y[d];
-                if (unknown(c, F, nx, ny)) {
+                int sn = sidx(x + NB_X[d], y + NB_Y[d], x0, y0, 1);
+                if (sd[sn] > 0.0f) {
                     float wd = f4c(w, d);
-                    float2 pn = direction(c, at(F, nx, ny), bx, by, p_old);
+                    float2 pn = sp[sn];
                     q.x -= wd * pn.x;
                     q.y -= wd * pn.y;
                 }
@@ -325,93 +330,66 @@ __global__ __launch_bounds__(256) void k_cg_apply(Ctx c, int nblocks, const floa
         }
     }
     double4 s = group_sum4(acc, lds);
-    if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], s);
-    finish_group(c, RED_ALPHA, nblocks, lds);
+    if (t == 0) store_part(&c.part_sums[nblocks + slot], s);
 }
-
-// cg_update.comp: x += alpha p, r -= alpha q.
-__global__ __launch_bounds__(256) void k_cg_update(Ctx c, const float2* __restrict__ p_new) {
-    if (solve_done(c)) return;
-    if (c.ptile[tile_slot()] == 0u) return;
-    const Lv F = level(c, 0);
-    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-    if (!inside(F, x, y)) return;
-    int i = at(F, x, y);
-    if (c.mgd[i] <= 0.0f) return;
-    const float ax = (float)c.K->alpha[0], ay = (float)c.K->alpha[1];
-    float2 p = p_new[i], q = c.cgq[i];
-    float2 x0 = c.pres[i], r = c.mgr[i];
-    x0.x += ax * p.x;
-    x0.y += ay * p.y;
-    r.x -= ax * q.x;
-    r.y -= ay * q.y;
-    c.pres[i] = x0;
-    c.mgr[i] = r;
-}
-
-// ---- the V-cycle ------------------------------------------------------------------------
 
 namespace {
-// Value of black cell (x, y) after the red half-sweep from zero (mg_down.comp:126-137).
-SP_DEV float2 down_black(const Ctx& c, const Lv& F, int x, int y) {
-    int i = at(F, x, y);
-    float4 w = c.mga[i];
-    float2 sum = c.mgr[i];
-    for (int d = 0; d < 4; d++) {
-        int mx = x + NB_X[d], my = y + NB_Y[d];
-        if (!unknown(c, F, mx, my)) continue;
-        int j = at(F, mx, my);
-        float wd = f4c(w, d);
-        float dj = c.mgd[j];
-        float2 rj = c.mgr[j];
-        sum.x += wd * rj.x / dj;
-        sum.y += wd * rj.y / dj;
-    }
-    float D = c.mgd[i];
-    return make_float2(sum.x / D, sum.y / D);
-}
-}  // namespace
 
-// mg_down.comp:139-172: pre-smoothing from zero on level l and restriction of the residual
-// to level l + 1.
-__global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated) {
-    __shared__ float2 sz[256];
-    if (gated && solve_done(c)) return;
-    const Lv F = level(c, l), C = level(c, l + 1);
-    if (c.ptile[F.toff + tile_slot()] == 0u) return;
-    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const bool red = ((x + y) & 1) == 0;
-    float2 z = make_float2(0.0f, 0.0f);
-    if (inside(F, x, y)) {
-        int i = at(F, x, y);
-        if (c.mgd[i] > 0.0f) {
-            if (red) {
-                float D = c.mgd[i];
-                float2 r = c.mgr[i];
+// The way down on one level (mg_down.comp:139-172) over a tile whose residual for the tile
+// plus two rings is in sr (ring 2) and diagonals in sd (ring 2): z of the tile and its first
+// ring, then the restriction of the red cells' residual to the coarse level.
+struct DownStage {
+    float sd[R2 * R2];
+    float2 sr[R2 * R2];
+    float4 sa[R1 * R1];
+    float2 sz[R1 * R1];
+};
+
+SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0, DownStage& S, float2* mgx_out) {
+    const int t = tid2();
+    // z after the red half-sweep from zero: red z = r / D; black from its red neighbours.
+    for (int k = t; k < R1 * R1; k += 256) {
+        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+        int s2 = sidx(gx, gy, x0, y0, 2);
+        float D = S.sd[s2];
+        float2 z = make_float2(0.0f, 0.0f);
+        if (D > 0.0f) {
+            float2 r = S.sr[s2];
+            if (((gx + gy) & 1) == 0) {
                 z = make_float2(r.x / D, r.y / D);
             } else {
-                z = down_black(c, F, x, y);
+                float4 w = S.sa[k];
+                float2 sum = r;
+                for (int d = 0; d < 4; d++) {
+                    int sn = sidx(gx + NB_X[d], gy + NB_Y[d], x0, y0, 2);
+                    float dj = S.sd[sn];
+                    if (!(dj > 0.0f)) continue;
+                    float wd = f4c(w, d);
+                    float2 rj = S.sr[sn];
+                    sum.x += wd * rj.x / dj;
+                    sum.y += wd * rj.y / dj;
+                }
+                z = make_float2(sum.x / D, sum.y / D);
             }
         }
-        c.mgx[i] = z;
+        S.sz[k] = z;
     }
-    sz[tid2()] = z;
     __syncthreads();
-    if (((lx | ly) & 1) != 0) return;
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (inside(F, x, y)) mgx_out[at(F, x, y)] = S.sz[sidx(x, y, x0, y0, 1)];
+    if (((threadIdx.x | threadIdx.y) & 1) != 0) return;
     const int cx = x >> 1, cy = y >> 1;
     if (!inside(C, cx, cy)) return;
     float2 sum = make_float2(0.0f, 0.0f);
     for (int k = 0; k < 2; k++) {
         int qx = x + k, qy = y + k;
-        if (!unknown(c, F, qx, qy)) continue;
-        float4 w = c.mga[at(F, qx, qy)];
+        int sq = sidx(qx, qy, x0, y0, 1);
+        if (!(S.sd[sidx(qx, qy, x0, y0, 2)] > 0.0f)) continue;
+        float4 w = S.sa[sq];
         for (int d = 0; d < 4; d++) {
             int nx = qx + NB_X[d], ny = qy + NB_Y[d];
-            if (!unknown(c, F, nx, ny)) continue;
-            int lnx = lx + k + NB_X[d], lny = ly + k + NB_Y[d];
-            bool here = lnx >= 0 && lny >= 0 && lnx < 16 && lny < 16;
-            float2 zn = here ? sz[lny * 16 + lnx] : down_black(c, F, nx, ny);
+            if (!(S.sd[sidx(nx, ny, x0, y0, 2)] > 0.0f)) continue;
+            float2 zn = S.sz[sidx(nx, ny, x0, y0, 1)];
             float wd = f4c(w, d);
             sum.x += wd * zn.x;
             sum.y += wd * zn.y;
@@ -420,80 +398,180 @@ __global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated) {
     c.mgr[at(C, cx, cy)] = sum;
 }
 
-namespace {
-// New value of black cell (x, y) on the way up (mg_up.comp:198-208).
-SP_DEV float2 up_black(const Ctx& c, const Lv& F, const Lv& C, int x, int y) {
-    int i = at(F, x, y);
-    float4 w = c.mga[i];
-    float2 sum = c.mgr[i];
-    for (int d = 0; d < 4; d++) {
-        int mx = x + NB_X[d], my = y + NB_Y[d];
-        if (!unknown(c, F, mx, my)) continue;
-        float wd = f4c(w, d);
-        float2 xm = c.mgx[at(F, mx, my)];
-        float2 zc = c.mgz[at(C, mx >> 1, my >> 1)];
-        sum.x += wd * (xm.x + SP_MG_OMEGA * zc.x);
-        sum.y += wd * (xm.y + SP_MG_OMEGA * zc.y);
+// Loads the diagonals (ring 2) and couplings (ring 1) of the tile.
+SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int x0, int y0, DownStage& S) {
+    const int t = tid2();
+    for (int k = t; k < R2 * R2; k += 256) {
+        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
+        S.sd[k] = inside(F, gx, gy) ? c.mgd[at(F, gx, gy)] : 0.0f;
     }
-    float D = c.mgd[i];
-    return make_float2(sum.x / D, sum.y / D);
+    for (int k = t; k < R1 * R1; k += 256) {
+        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+        S.sa[k] = inside(F, gx, gy) ? c.mga[at(F, gx, gy)] : make_float4(0, 0, 0, 0);
+    }
 }
+
 }  // namespace
 
-// mg_up.comp:210-247: coarse-grid correction and the black-then-red post-smoothing on
-// level l. On level 0 (reduce >= 0) also the partial sums r.z, r.r and, in the last group,
-// the scalar step `reduce`.
-__global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int reduce, int nblocks) {
-    __shared__ float2 sz[256];
+// mg_down.comp:139-172 on level l: pre-smoothing from zero and restriction of the residual.
+// r_in: the level's right-hand side (level 0: the CG residual buffer of this iteration).
+__global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated, const float2* __restrict__ r_in) {
+    __shared__ DownStage S;
+    if (gated && solve_done(c)) return;
+    const Lv F = level(c, l), C = level(c, l + 1);
+    if (c.ptile[F.toff + tile_slot()] == 0u) return;
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+    const float2* r = l == 0 ? r_in : c.mgr;
+    stage_matrix(c, F, x0, y0, S);
+    for (int k = tid2(); k < R2 * R2; k += 256) {
+        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
+        S.sr[k] = inside(F, gx, gy) ? r[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    down_compute(c, F, C, x0, y0, S, c.mgx);
+}
+
+// cg_update.comp fused with the way down on level 0: x += alpha p, r_out = r_in - alpha q
+// (residual buffers alternate between iterations, so a tile may read its neighbours' old
+// residual while they write their new one), then mg_down level 0 on r_out.
+__global__ __launch_bounds__(256) void k_update_down0(Ctx c, const float2* __restrict__ r_in, float2* __restrict__ r_out,
+                                                      const float2* __restrict__ p_new) {
+    __shared__ DownStage S;
+    if (solve_done(c)) return;
+    const Lv F = level(c, 0), C = level(c, 1);
+    if (c.ptile[tile_slot()] == 0u) return;
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+    const float ax = (float)c.K->alpha[0], ay = (float)c.K->alpha[1];
+    stage_matrix(c, F, x0, y0, S);
+    for (int k = tid2(); k < R2 * R2; k += 256) {
+        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
+        float2 rv = make_float2(0.0f, 0.0f);
+        if (inside(F, gx, gy)) {
+            int i = at(F, gx, gy);
+            rv = r_in[i];
+            if (c.mgd[i] > 0.0f) {
+                float2 q = c.cgq[i];
+                rv.x -= ax * q.x;
+                rv.y -= ay * q.y;
+            }
+        }
+        S.sr[k] = rv;
+    }
+    {
+        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+        if (inside(F, x, y)) {
+            int i = at(F, x, y);
+            if (c.mgd[i] > 0.0f) {
+                float2 p = p_new[i];
+                float2 x1 = c.pres[i];
+                x1.x += ax * p.x;
+                x1.y += ay * p.y;
+                c.pres[i] = x1;
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+        if (inside(F, x, y)) r_out[at(F, x, y)] = S.sr[sidx(x, y, x0, y0, 2)];
+    }
+    down_compute(c, F, C, x0, y0, S, c.mgx);
+}
+
+// mg_up.comp:210-247 on level l: coarse-grid correction and the black-then-red
+// post-smoothing; on level 0 (reduce >= 0) also the partial sums r.z, r.r.
+__global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int reduce, int nblocks, const float2* __restrict__ r_in) {
     __shared__ double4 lds[256];
+    __shared__ float sd[R2 * R2];
+    __shared__ float2 sx[R2 * R2];
+    __shared__ float4 sa[R1 * R1];
+    __shared__ float2 sr[R1 * R1];
+    __shared__ float2 szc[10 * 10];
+    __shared__ float2 sb[R1 * R1];
     if (gated && solve_done(c)) return;
     const Lv F = level(c, l), C = level(c, l + 1);
     const int slot = tile_slot();
     if (c.ptile[F.toff + slot] == 0u) {
-        if (reduce >= 0) {
-            if (tid2() == 0) store_part(&c.part_sums[slot], make_double4(0, 0, 0, 0));
-            finish_group(c, reduce, nblocks, lds);
-        }
+        if (reduce >= 0 && tid2() == 0) store_part(&c.part_sums[slot], make_double4(0, 0, 0, 0));
         return;
     }
-    const int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const bool red = ((x + y) & 1) == 0;
-    const bool unk = unknown(c, F, x, y);
-    float2 z = make_float2(0.0f, 0.0f);
-    if (unk && !red) z = up_black(c, F, C, x, y);
-    sz[tid2()] = z;
+    const int t = tid2();
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+    const float2* r = l == 0 ? r_in : c.mgr;
+    for (int k = t; k < R2 * R2; k += 256) {
+        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
+        bool in = inside(F, gx, gy);
+        sd[k] = in ? c.mgd[at(F, gx, gy)] : 0.0f;
+        sx[k] = in ? c.mgx[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < R1 * R1; k += 256) {
+        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+        bool in = inside(F, gx, gy);
+        sa[k] = in ? c.mga[at(F, gx, gy)] : make_float4(0, 0, 0, 0);
+        sr[k] = in ? r[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+    }
+    // parents of the ring-2 region: coarse cells x0/2 - 1 .. x0/2 + 8
+    const int cx0 = (x0 >> 1) - 1, cy0 = (y0 >> 1) - 1;
+    for (int k = t; k < 100; k += 256) {
+        int gx = cx0 + k % 10, gy = cy0 + k / 10;
+        szc[k] = inside(C, gx, gy) ? c.mgz[at(C, gx, gy)] : make_float2(0.0f, 0.0f);
+    }
     __syncthreads();
+    // new values of the black cells of the tile and its first ring (mg_up.comp:198-208)
+    for (int k = t; k < R1 * R1; k += 256) {
+        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+        float2 z = make_float2(0.0f, 0.0f);
+        float D = sd[sidx(gx, gy, x0, y0, 2)];
+        if (D > 0.0f && ((gx + gy) & 1) != 0) {
+            float4 w = sa[k];
+            float2 sum = sr[k];
+            for (int d = 0; d < 4; d++) {
+                int mx = gx + NB_X[d], my = gy + NB_Y[d];
+                int sm = sidx(mx, my, x0, y0, 2);
+                if (!(sd[sm] > 0.0f)) continue;
+                float wd = f4c(w, d);
+                float2 xm = sx[sm];
+                float2 zc = szc[((my >> 1) - cy0) * 10 + ((mx >> 1) - cx0)];
+                sum.x += wd * (xm.x + SP_MG_OMEGA * zc.x);
+                sum.y += wd * (xm.y + SP_MG_OMEGA * zc.y);
+            }
+            z = make_float2(sum.x / D, sum.y / D);
+        }
+        sb[k] = z;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     double4 acc = make_double4(0, 0, 0, 0);
     if (inside(F, x, y)) {
         int i = at(F, x, y);
-        if (unk) {
-            float2 r = c.mgr[i];
-            if (red) {
-                float4 w = c.mga[i];
-                float2 sum = r;
+        int s1 = sidx(x, y, x0, y0, 1);
+        float D = sd[sidx(x, y, x0, y0, 2)];
+        float2 z = make_float2(0.0f, 0.0f);
+        if (D > 0.0f) {
+            float2 rv = sr[s1];
+            if (((x + y) & 1) == 0) {
+                float4 w = sa[s1];
+                float2 sum = rv;
                 for (int d = 0; d < 4; d++) {
                     int nx = x + NB_X[d], ny = y + NB_Y[d];
-                    if (!unknown(c, F, nx, ny)) continue;
-                    int lnx = lx + NB_X[d], lny = ly + NB_Y[d];
-                    bool here = lnx >= 0 && lny >= 0 && lnx < 16 && lny < 16;
-                    float2 zn = here ? sz[lny * 16 + lnx] : up_black(c, F, C, nx, ny);
+                    if (!(sd[sidx(nx, ny, x0, y0, 2)] > 0.0f)) continue;
                     float wd = f4c(w, d);
+                    float2 zn = sb[sidx(nx, ny, x0, y0, 1)];
                     sum.x += wd * zn.x;
                     sum.y += wd * zn.y;
                 }
-                float D = c.mgd[i];
                 z = make_float2(sum.x / D, sum.y / D);
+            } else {
+                z = sb[s1];
             }
-            acc = make_double4((double)r.x * (double)z.x, (double)r.y * (double)z.y, (double)r.x * (double)r.x,
-                               (double)r.y * (double)r.y);
+            acc = make_double4((double)rv.x * (double)z.x, (double)rv.y * (double)z.y, (double)rv.x * (double)rv.x,
+                               (double)rv.y * (double)rv.y);
         }
         c.mgz[i] = z;
     }
     if (reduce >= 0) {
-        double4 s = group_sum4(acc, lds);
-        if (tid2() == 0) store_part(&c.part_sums[slot], s);
-        finish_group(c, reduce, nblocks, lds);
+        double4 s2 = group_sum4(acc, lds);
+        if (t == 0) store_part(&c.part_sums[slot], s2);
     }
 }
 
@@ -641,6 +719,17 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
         __syncthreads();
     }
     for (int t = bt; t < NF; t += 1024) c.mgz[BASE + t] = sx[t];
+}
+
+// The scalar step of the CG (cg_reduce.comp / reduce_tail.glsl modes 0, 1, 2) over the
+// partial sums the previous kernel wrote: one 256 work-item group, fixed summation order.
+// The reference folds this into the last work-group of the producing kernel; here the
+// kernel boundary orders the partials instead, which on a multi-XCD part is far cheaper
+// than a device-scope release fence in every producing work-group.
+__global__ __launch_bounds__(256) void k_cg_reduce(Ctx c, int mode, int nblocks, int gated) {
+    __shared__ double4 lds[256];
+    if (gated && solve_done(c)) return;
+    reduce_scalars(c, mode, nblocks, lds);
 }
 
 // Solve statistics at the end of a solve (reduce_tail.glsl:105-116, mode 3).

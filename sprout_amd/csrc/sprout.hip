@@ -76,6 +76,7 @@ struct sp_sim {
     uint32_t* d_vals = nullptr;
     uint32_t* d_keys2 = nullptr;
     uint32_t* d_vals2 = nullptr;
+    float2* d_r2 = nullptr;  // second level-0 residual buffer of the CG
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
 
@@ -84,7 +85,7 @@ struct sp_sim {
     uint32_t next_id = 0;
     bool grid_valid = false;
     uint32_t since_sort = 0;
-    uint32_t sort_interval = 16;
+    uint32_t sort_interval = 8;
     std::vector<sp_particle> pending_spawn;
     double time_base = 0.0;
     double sim_time = 0.0;
@@ -124,15 +125,18 @@ const dim3 TILE(16, 16, 1);
 
 // ---- the substep schedule (record_substep, mod.rs:2615-2708) ----------------------------
 
-void launch_vcycle(sp_sim* s, int gated, int reduce) {
+// One V-cycle on the level-0 residual r0 (record_vcycle, mod.rs:2930-2954); with
+// `skip_down0` the way down on level 0 has already run (fused into the CG update).
+void launch_vcycle(sp_sim* s, int gated, int reduce, const float2* r0, bool skip_down0 = false) {
     const Ctx& c = s->c;
     const sp_levels_info& L = s->lv;
-    for (int l = 0; l < L.tail; l++)
-        hipLaunchKernelGGL(k_mg_down, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated);
+    for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
+        hipLaunchKernelGGL(k_mg_down, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated, r0);
     hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, s->stream, c, gated);
     for (int l = L.tail - 1; l >= 0; l--)
         hipLaunchKernelGGL(k_mg_up, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated,
-                           l == 0 ? reduce : -1, s->nb0);
+                           l == 0 ? reduce : -1, s->nb0, r0);
+    if (reduce >= 0) hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(256), 0, s->stream, c, reduce, s->nb0, gated);
 }
 
 void launch_mg_build(sp_sim* s) {
@@ -143,19 +147,25 @@ void launch_mg_build(sp_sim* s) {
     if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build, dim3(1), dim3(1024), 0, s->stream, c);
 }
 
-// record_pressure + record_cg (mod.rs:2956-3010), the first solve.
+// record_pressure + record_cg (mod.rs:2956-3010), the first solve. The level-0 residual
+// alternates between mgr and r2 from one iteration to the next (k_update_down0).
 void launch_pressure(sp_sim* s, uint32_t iters) {
     const Ctx& c = s->c;
     dim3 g0 = grid_groups(s->W, s->H);
     launch_mg_build(s);
     hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, s->stream, c);
     hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, s->stream, c, s->nb0);
-    launch_vcycle(s, 0, 0);
+    float2* rb[2] = {c.mgr, s->d_r2};
+    launch_vcycle(s, 0, 0, rb[0]);
     float2* bufs[2] = {c.cgp, c.cgp2};
     for (uint32_t k = 0; k < iters; k++) {
-        hipLaunchKernelGGL(k_cg_apply, g0, TILE, 0, s->stream, c, s->nb0, (const float2*)bufs[k & 1], bufs[(k + 1) & 1]);
-        hipLaunchKernelGGL(k_cg_update, g0, TILE, 0, s->stream, c, (const float2*)bufs[(k + 1) & 1]);
-        launch_vcycle(s, 1, 2);
+        const float2* p_old = bufs[k & 1];
+        float2* p_new = bufs[(k + 1) & 1];
+        hipLaunchKernelGGL(k_cg_apply, g0, TILE, 0, s->stream, c, s->nb0, p_old, p_new);
+        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(256), 0, s->stream, c, 1, s->nb0, 1);
+        hipLaunchKernelGGL(k_update_down0, g0, TILE, 0, s->stream, c, (const float2*)rb[k & 1], rb[(k + 1) & 1],
+                           (const float2*)p_new);
+        launch_vcycle(s, 1, 2, rb[(k + 1) & 1], true);
     }
     hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s->stream, c);
 }
@@ -185,12 +195,15 @@ void launch_particles(sp_sim* s, int gather) {
     hipLaunchKernelGGL(k_particles, dim3(blocks), dim3(256), 0, s->stream, s->c, gather);
 }
 
-void launch_sort(sp_sim* s) {
+void launch_sort(sp_sim* s, int tiled) {
     if (s->n_slots == 0) return;
     uint32_t blocks = (s->n_slots + 255) / 256;
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, s->stream, s->c, s->d_keys, s->d_vals);
+    // keys below 2^end_bit; dead slots take the largest, so they collect at the end
+    uint32_t live_max = tiled ? (uint32_t)(((s->W + 7) / 8) * ((s->H + 7) / 8) * 64) : (uint32_t)s->N;
     int end_bit = 1;
-    while ((1u << end_bit) <= (uint32_t)s->N && end_bit < 32) end_bit++;
+    while (end_bit < 32 && (1u << end_bit) <= live_max) end_bit++;
+    uint32_t dead = end_bit >= 32 ? 0xffffffffu : (1u << end_bit) - 1u;
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, s->stream, s->c, s->d_keys, s->d_vals, tiled, dead);
     size_t bytes = s->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(s->d_sort_tmp, bytes, s->d_keys, s->d_keys2, s->d_vals, s->d_vals2,
                                               (int)s->n_slots, 0, end_bit, s->stream));
@@ -317,7 +330,7 @@ int record(sp_sim* s, uint32_t substeps, float target) {
     }
     for (uint32_t k = 0; k < substeps; k++) {
         if (s->since_sort >= s->sort_interval) {
-            launch_sort(s);
+            launch_sort(s, 1);
             s->since_sort = 0;
         }
         s->since_sort++;
@@ -498,6 +511,7 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         c.cgp = dalloc<float2>(N, o);
         c.cgp2 = dalloc<float2>(N, o);
         c.cgq = dalloc<float2>(N, o);
+        s->d_r2 = dalloc<float2>(N, o);
         c.ptile = dalloc<uint32_t>((size_t)s->lv.tiles, o);
         c.part_sums = dalloc<double4>((size_t)2 * s->nb0, o);
         int4* lev = dalloc<int4>(SP_MAX_LEVELS, o);
@@ -812,7 +826,7 @@ int sp_sort(sp_sim* s) {
     return guarded(s, [&]() -> int {
         int rc = upload_tables(s);
         if (rc != SP_OK) return rc;
-        launch_sort(s);
+        launch_sort(s, 0);
         HIPCHK(hipStreamSynchronize(s->stream));
         return SP_OK;
     });
@@ -948,7 +962,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
             s->grid_valid = true;
             break;
         case SP_STAGE_MG_BUILD: launch_mg_build(s); break;
-        case SP_STAGE_VCYCLE: launch_vcycle(s, 0, -1); break;
+        case SP_STAGE_VCYCLE: launch_vcycle(s, 0, -1, s->c.mgr); break;
         default: set_err(s, "unknown stage %d", stage); return SP_E_INVALID;
         }
         HIPCHK(hipGetLastError());
