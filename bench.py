@@ -32,6 +32,19 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_particles_dam_break_1024x576_pmc.md")
+
+
+def pmc_traffic(W, H, n_p):
+    """HBM bytes per k_particles launch from the committed FETCH_SIZE + WRITE_SIZE summary
+    (each counter scaled by what it reads for a plain copy of known size). It is quoted only
+    for the workload it was measured on; any other grid or particle count gets null."""
+    if (W, H, n_p) != (1024, 576, 1494912) or not os.path.exists(PMC_SUMMARY):
+        return None
+    for line in open(PMC_SUMMARY):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def workload(grid=(1024, 576), frac=(0.8, 0.8)):
@@ -78,7 +91,7 @@ def main():
     ap.add_argument("--grid", default="1024x576")
     ap.add_argument("--frac", default="0.8x0.8")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=4)
+    ap.add_argument("--cpu-substeps", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -162,7 +175,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pmc_traffic(W, H, n_p),
             "launch_ms": k_ms,
             "alg_bytes_per_launch": alg_bytes,
         },

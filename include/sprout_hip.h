@@ -9,8 +9,8 @@
  * this boundary. Calls on one handle are not thread-safe (the reference takes `&mut self`
  * everywhere); different handles may be used from different threads.
  *
- * All types here are plain C: the layouts of sp_particle and the grid fields follow the
- * reference byte for byte (static_asserts in sprout_types.h).
+ * All types here are plain C: the layouts of sp_particle and the step state follow the
+ * reference byte for byte (checked by static assertions in sprout_types.h).
  */
 #ifndef SPROUT_HIP_H
 #define SPROUT_HIP_H
@@ -113,8 +113,8 @@ int sp_sort(sp_sim* sim);
 
 /* ---- host-side derivations (no device needed) ---------------------------------------- */
 
-/* Default SI parameters and the built-in materials this build carries (water, oil, honey,
- * mercury, syrup) — SimParams::default (src/sim/params.rs:386-460). */
+/* Default SI parameters and the built-in materials this build carries (water, honey, oil,
+ * mercury at table rows 0, 1, 2, 8) — SimParams::default (src/sim/params.rs:386-460). */
 void sp_default_params(sp_sim_params* out);
 
 /* Tank::layout (src/sim/mod.rs:271-331) + gpu_params / gpu_materials (src/sim/mod.rs:

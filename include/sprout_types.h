@@ -287,4 +287,15 @@ enum {
 }
 #endif
 
+/* Byte layouts shared with the reference's std430 buffers and with the device code. */
+#ifdef __cplusplus
+#define SP_STATIC_ASSERT static_assert
+#else
+#define SP_STATIC_ASSERT _Static_assert
+#endif
+SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
+SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 80, "one grid-unit material row is 20 words");
+SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
+
 #endif

@@ -586,10 +586,10 @@ __global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int redu
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");   \
     } while (0)
 
+// LDS (dynamic, sized at launch to the hierarchy): per tail cell its couplings (float4),
+// iterate (float2) and diagonal; the right-hand sides of all but the first tail level.
 __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
-    __shared__ float2 sx[SP_TAIL_SHARED];
-    __shared__ float sd[SP_TAIL_SHARED];
-    __shared__ float2 sr[SP_TAIL_SHARED_COARSE];
+    extern __shared__ float4 tail_lds[];
     __shared__ int4 sl[SP_MAX_LEVELS];
     if (gated && solve_done(c)) return;
     const int first = c.tail, last = c.n_levels - 1;
@@ -597,8 +597,15 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
     const int BASE = Lf.x;
     const int total = Le.x + Le.y * Le.z - BASE;
     const int NF = Lf.y * Lf.z;
+    float4* sa = tail_lds;
+    float2* sx = reinterpret_cast<float2*>(sa + total);
+    float2* sr = sx + total;
+    float* sd = reinterpret_cast<float*>(sr + (total - NF));
     const int bt = threadIdx.x;
-    for (int t = bt; t < total; t += 1024) sd[t] = c.mgd[BASE + t];
+    for (int t = bt; t < total; t += 1024) {
+        sd[t] = c.mgd[BASE + t];
+        sa[t] = c.mga[BASE + t];
+    }
     if (bt <= last) sl[bt] = c.lev[bt];
     __syncthreads();
 
@@ -616,7 +623,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
             }
             float2 sum = rhs_at(s);
             if (!zero) {
-                float4 w = c.mga[BASE + s];
+                float4 w = sa[s];
                 for (int d = 0; d < 4; d++) {
                     int nx = x + NB_X[d], ny = y + NB_Y[d];
                     if (sunk(L, nx, ny)) {
@@ -638,7 +645,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
                 int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
                 if (!sunk(F, x, y)) continue;
                 int s = sat(F, x, y);
-                float4 w = c.mga[BASE + s];
+                float4 w = sa[s];
                 float2 az = make_float2(sd[s] * sx[s].x, sd[s] * sx[s].y);
                 for (int d = 0; d < 4; d++) {
                     int nx = x + NB_X[d], ny = y + NB_Y[d];
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
             int s = F.x - BASE + t;
             float D = sd[s];
             if (D <= 0.0f) continue;
-            float4 w = c.mga[BASE + s];
+            float4 w = sa[s];
             float2 sum = rhs_at(s);
             for (int d = 0; d < 4; d++) {
                 int nx = x + NB_X[d], ny = y + NB_Y[d];

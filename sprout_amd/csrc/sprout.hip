@@ -7,6 +7,7 @@
 #include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -47,6 +48,14 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 
 }  // namespace
 
+// Instantiated graphs of one (recorded iterations, viscosity mode) key: grid passes,
+// pressure solve, projection.
+struct Graphs {
+    uint32_t iters = 0;
+    int visc = -1;
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+};
+
 struct sp_sim {
     int W = 0, H = 0, N = 0;
     uint32_t cap = 0;
@@ -63,6 +72,7 @@ struct sp_sim {
     bool tables_dirty = true;
     sp_levels_info lv{};
     int nb0 = 0;  // level-0 work-groups (16 x 16 tiles)
+    size_t tail_lds_bytes = 0;  // dynamic LDS of k_mg_tail
 
     // device
     Ctx c{};
@@ -105,6 +115,8 @@ struct sp_sim {
     uint32_t timed_last = 0;
     float timings[6] = {};
     float particle_ms = 0.0f;
+    bool use_graphs = true;
+    std::vector<Graphs> graphs;
 
     Ctx ctx() const { return c; }
 };
@@ -132,7 +144,7 @@ void launch_vcycle(sp_sim* s, int gated, int reduce, const float2* r0, bool skip
     const sp_levels_info& L = s->lv;
     for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
         hipLaunchKernelGGL(k_mg_down, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated, r0);
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, s->stream, c, gated);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), s->tail_lds_bytes, s->stream, c, gated);
     for (int l = L.tail - 1; l >= 0; l--)
         hipLaunchKernelGGL(k_mg_up, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated,
                            l == 0 ? reduce : -1, s->nb0, r0);
@@ -212,10 +224,54 @@ void launch_sort(sp_sim* s, int tiled) {
     HIPCHK(hipMemcpyAsync(s->c.part, s->d_part_tmp, sizeof(sp_particle) * s->n_slots, hipMemcpyDeviceToDevice, s->stream));
 }
 
+// The grid passes, the pressure solve with `p_iters` recorded iterations, and the
+// projection as three HIP graphs, captured once per (iterations, viscosity mode) and
+// replayed: a substep is some two hundred dependent launches, which the host could not
+// issue as fast as the device retires them. The kernels read everything that changes
+// between substeps (dt, tables, solve state) from device memory, so a replay needs no
+// new arguments. Events between the graphs time the phases.
+void destroy_graphs(sp_sim* s) {
+    for (auto& g : s->graphs)
+        for (auto& e : g.exec)
+            if (e) {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
+            }
+    s->graphs.clear();
+}
+
+hipGraphExec_t capture(sp_sim* s, int part, uint32_t p_iters) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    if (part == 0) launch_grid_passes(s);
+    else if (part == 1) launch_pressure(s, p_iters);
+    else hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, s->c);
+    hipError_t le = hipGetLastError();
+    HIPCHK(hipStreamEndCapture(s->stream, &g));
+    HIPCHK(le);
+    hipGraphExec_t e = nullptr;
+    HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return e;
+}
+
+Graphs& graphs_for(sp_sim* s, uint32_t p_iters) {
+    for (auto& g : s->graphs)
+        if (g.iters == p_iters && g.visc == s->gp.visc_mode) return g;
+    if (s->graphs.size() >= 12) destroy_graphs(s);
+    s->graphs.emplace_back();
+    Graphs& g = s->graphs.back();
+    g.iters = p_iters;
+    g.visc = s->gp.visc_mode;
+    for (int k = 0; k < 3; k++) g.exec[k] = capture(s, k, p_iters);
+    return g;
+}
+
 void launch_substep(sp_sim* s, int gather, uint32_t p_iters) {
     const Ctx& c = s->c;
     hipEvent_t* e = nullptr;
     if (s->timing && s->timed < (uint32_t)sp_sim::TIMED_SUBSTEPS) e = &s->ev[(size_t)s->timed++ * sp_sim::EV_PER];
+    Graphs* g = s->use_graphs ? &graphs_for(s, p_iters) : nullptr;
     if (e) HIPCHK(hipEventRecord(e[0], s->stream));
     hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c);
     launch_clear_acc(s);
@@ -223,11 +279,14 @@ void launch_substep(sp_sim* s, int gather, uint32_t p_iters) {
     launch_particles(s, gather);
     if (e) HIPCHK(hipEventRecord(e[6], s->stream));
     if (e) HIPCHK(hipEventRecord(e[1], s->stream));
-    launch_grid_passes(s);
+    if (g) HIPCHK(hipGraphLaunch(g->exec[0], s->stream));
+    else launch_grid_passes(s);
     if (e) HIPCHK(hipEventRecord(e[2], s->stream));
-    launch_pressure(s, p_iters);
+    if (g) HIPCHK(hipGraphLaunch(g->exec[1], s->stream));
+    else launch_pressure(s, p_iters);
     if (e) HIPCHK(hipEventRecord(e[3], s->stream));
-    hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, c);
+    if (g) HIPCHK(hipGraphLaunch(g->exec[2], s->stream));
+    else hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, c);
     if (e) HIPCHK(hipEventRecord(e[4], s->stream));
     HIPCHK(hipGetLastError());
 }
@@ -464,6 +523,17 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         sp::levels(s->W, s->H, s->lv);
         s->nb0 = ((s->W + 15) / 16) * ((s->H + 15) / 16);
+        {
+            const sp_levels_info& L = s->lv;
+            size_t total = (size_t)(L.offs[L.n - 1] + L.dims[L.n - 1][0] * L.dims[L.n - 1][1] - L.offs[L.tail]);
+            size_t nf = (size_t)L.dims[L.tail][0] * L.dims[L.tail][1];
+            s->tail_lds_bytes = total * (16 + 8 + 4) + (total - nf) * 8;
+            if (total > SP_TAIL_SHARED || total - nf > SP_TAIL_SHARED_COARSE) {
+                sp_destroy(s);
+                g_create_error = "multigrid tail does not fit its shared-memory budget";
+                return SP_E_INVALID;
+            }
+        }
         const size_t N = (size_t)s->N, T = (size_t)s->lv.total;
         auto& o = s->owned;
         Ctx& c = s->c;
@@ -535,6 +605,10 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         HIPCHK(hipDeviceSynchronize());
         sp::default_params(&s->params);
         s->timing = true;
+        // SPROUT_GRAPHS=0 launches the substep's kernels one by one instead of replaying
+        // captured graphs (profilers that cannot follow graph launches need it)
+        const char* gv = getenv("SPROUT_GRAPHS");
+        s->use_graphs = !(gv && gv[0] == '0');
     } catch (const HipError& e) {
         char buf[256];
         snprintf(buf, sizeof(buf), "%s failed: %s", e.what, hipGetErrorString(e.e));
@@ -550,6 +624,7 @@ void sp_destroy(sp_sim* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    destroy_graphs(s);
     for (void* p : s->owned) (void)hipFree(p);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
