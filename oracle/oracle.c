@@ -1014,7 +1014,7 @@ static v4 strain_of_vel(const or_sim* s, int i, float wx, float wy) {
 static void stage_visc_explicit(or_sim* s) {
     int W = s->W, H = s->H;
     float dt = s->S.dt;
-    /* visc_setup.comp:77-121 (the tile flags only matter to the implicit solve). */
+    /* visc_setup.comp:14-58 (the tile flags only matter to the implicit solve). */
     for (int cy = 0; cy < H; cy++)
         for (int cx = 0; cx < W; cx++) {
             int i = cidx(s, cx, cy);
@@ -1049,7 +1049,7 @@ static void stage_visc_explicit(or_sim* s) {
             int i = cidx(s, cx, cy);
             s->vs[i] = strain_of_vel(s, i, s->vw[i].x, s->vw[i].y);
         }
-    /* visc_apply.comp mode 2 (visc_apply.comp:137-170). */
+    /* visc_apply.comp mode 2 (visc_apply.comp:16-48). */
     for (int cy = 1; cy < H - 1; cy++)
         for (int cx = 1; cx < W - 1; cx++) {
             int i = cidx(s, cx, cy);
@@ -1068,7 +1068,7 @@ static void stage_visc_explicit(or_sim* s) {
         }
 }
 
-/* ---- rhs.comp (rhs.comp:58-133) ------------------------------------------------------- */
+/* ---- rhs.comp (rhs.comp:19-93) ------------------------------------------------------- */
 
 static void stage_rhs(or_sim* s) {
     const sp_gparams* P = &s->P;
@@ -1165,7 +1165,7 @@ static int inside(lvl L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y
 static int lat(lvl L, int x, int y) { return L.off + y * L.w + x; }
 static int unknown(const or_sim* s, lvl L, int x, int y) { return inside(L, x, y) && s->mgd[lat(L, x, y)] > 0.0f; }
 
-/* Galerkin coarse matrix of coarse cell (cx, cy) (mg_build.comp:80-102, mg_tail.comp:44-70). */
+/* Galerkin coarse matrix of coarse cell (cx, cy) (mg_build.comp:43-65, mg_tail.comp:44-70). */
 static void build_cell(or_sim* s, lvl F, lvl C, int cx, int cy) {
     float D = 0.0f;
     v4 A = {0, 0, 0, 0};
@@ -1217,7 +1217,7 @@ static void stage_mg_build(or_sim* s) {
     }
 }
 
-/* black cell after the first red half-sweep from zero (mg_down.comp:126-137) */
+/* black cell after the first red half-sweep from zero (mg_down.comp:23-34) */
 static v2 down_black(const or_sim* s, lvl F, int x, int y) {
     int i = lat(F, x, y);
     v4 w = s->mga[i];
@@ -1241,7 +1241,7 @@ static v2 down_z(const or_sim* s, lvl F, int x, int y) {
     return down_black(s, F, x, y);
 }
 
-/* mg_down.comp:139-172 */
+/* mg_down.comp:36-69 */
 static void mg_down(or_sim* s, int l) {
     lvl F = level_of(s, l), C = level_of(s, l + 1);
     for (int y = 0; y < F.h; y++)
@@ -1266,7 +1266,7 @@ static void mg_down(or_sim* s, int l) {
         }
 }
 
-/* black cell of the way up (mg_up.comp:198-208) */
+/* black cell of the way up (mg_up.comp:26-36) */
 static v2 up_black(const or_sim* s, lvl F, lvl C, int x, int y) {
     int i = lat(F, x, y);
     v4 w = s->mga[i];
@@ -1284,7 +1284,7 @@ static v2 up_black(const or_sim* s, lvl F, lvl C, int x, int y) {
     return (v2){sum.x / D, sum.y / D};
 }
 
-/* mg_up.comp:210-247; on level 0 returns the r.z and r.r sums (FP64). */
+/* mg_up.comp:38-75; on level 0 returns the r.z and r.r sums (FP64). */
 static void mg_up(or_sim* s, int l, double sums[4]) {
     lvl F = level_of(s, l), C = level_of(s, l + 1);
     /* the black cells first (their values only feed red neighbours and the output) */
@@ -1465,7 +1465,7 @@ static void cg_scale(or_sim* s) {
         }
 }
 
-/* cg_init.comp:38-64: r = b - A x, and b.b */
+/* cg_init.comp:11-37: r = b - A x, and b.b */
 static void cg_init(or_sim* s, double bb[2]) {
     lvl F = level_of(s, 0);
     bb[0] = bb[1] = 0.0;
@@ -1531,7 +1531,7 @@ static void stage_pressure(or_sim* s) {
     v2* np = (v2*)malloc(sizeof(v2) * (size_t)N);
     uint32_t k = 0;
     for (; k < (uint32_t)P->p_max_iters && !(K->done[0] && K->done[1]); k++) {
-        /* cg_apply.comp:87-115, alpha (reduce_tail.glsl:79-87) */
+        /* cg_apply.comp:15-51, alpha (reduce_tail.glsl:79-87) */
         float bx = (float)K->beta[0], by = (float)K->beta[1];
         double pq[2] = {0, 0};
         for (int y = 0; y < F.h; y++)
@@ -1562,7 +1562,7 @@ static void stage_pressure(or_sim* s) {
         memcpy(s->cgp, np, sizeof(v2) * (size_t)N);
         K->alpha[0] = (K->done[0] == 0 && pq[0] > 0.0) ? K->rz[0] / pq[0] : 0.0;
         K->alpha[1] = (K->done[1] == 0 && pq[1] > 0.0) ? K->rz[1] / pq[1] : 0.0;
-        /* cg_update.comp:126-135 */
+        /* cg_update.comp:11-20 */
         float ax = (float)K->alpha[0], ay = (float)K->alpha[1];
         for (int i = 0; i < N; i++) {
             if (s->mgd[i] <= 0.0f) continue;

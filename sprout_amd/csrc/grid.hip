@@ -209,7 +209,7 @@ SP_DEV void face_coef(float4 a, float4 b, float open, float& g, float& rho) {
 }
 }  // namespace
 
-// grid_coef.comp:27-38 fused with surf_smooth.comp:74-102 (both read only cell, comp,
+// grid_coef.comp:27-38 fused with surf_smooth.comp:16-44 (both read only cell, comp,
 // aux and solid, which grid_prep has finished).
 __global__ __launch_bounds__(256) void k_coef_smooth(Ctx c) {
     int cx, cy, i;
@@ -409,7 +409,7 @@ SP_DEV float face_jump(const Ctx& c, int ci, int n, float open) {
 }
 }  // namespace
 
-// surf_apply.comp:192-203.
+// surf_apply.comp:71-82.
 __global__ __launch_bounds__(256) void k_surf_apply(Ctx c) {
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
@@ -431,7 +431,7 @@ SP_DEV float vel_u(const Ctx& c, int i) { return live_u(c, i) ? c.vel[i].x : 0.0
 SP_DEV float vel_v(const Ctx& c, int i) { return live_v(c, i) ? c.vel[i].y : 0.0f; }
 }  // namespace
 
-// visc_setup.comp:77-121: strain weights of the cell and its lower-left node, and the
+// visc_setup.comp:14-58: strain weights of the cell and its lower-left node, and the
 // strain of the current velocities with them (visc.glsl:47-63).
 __global__ __launch_bounds__(256) void k_visc_setup(Ctx c) {
     int cx, cy, i;
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) void k_visc_setup(Ctx c) {
     c.vs[i] = o;
 }
 
-// visc_apply.comp mode 2 (visc_apply.comp:137-170): the explicit viscous step.
+// visc_apply.comp mode 2 (visc_apply.comp:16-48): the explicit viscous step.
 __global__ __launch_bounds__(256) void k_visc_explicit(Ctx c) {
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(256) void k_visc_explicit(Ctx c) {
     c.vel[i] = v;
 }
 
-// rhs.comp:58-133: the level-0 matrix, both right-hand sides, and the tile flags.
+// rhs.comp:19-93: the level-0 matrix, both right-hand sides, and the tile flags.
 __global__ __launch_bounds__(256) void k_rhs(Ctx c) {
     int cx, cy, i;
     const bool in = cell_of_thread(c, cx, cy, i);

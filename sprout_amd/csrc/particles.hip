@@ -580,7 +580,7 @@ __global__ __launch_bounds__(256) void k_sort_gather(Ctx c, const uint32_t* vals
     out[j] = c.part[vals[j]];
 }
 
-// Live particles per cell in a window (count.comp:86-92).
+// Live particles per cell in a window (count.comp:17-23).
 __global__ __launch_bounds__(256) void k_count(Ctx c, int x0, int y0, int w, int h, uint32_t* out) {
     uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= c.P->n_slots) return;
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void k_count(Ctx c, int x0, int y0, int w, int
     atomicAdd(&out[y * w + x], 1u);
 }
 
-// Live count and per-row counts / concentration sums (stats.comp:123-151).
+// Live count and per-row counts / concentration sums (stats.comp:14-42).
 __global__ __launch_bounds__(64) void k_stats(Ctx c, uint32_t* counter) {
     const uint32_t RUN = 64u;
     uint32_t first = (blockIdx.x * 64 + threadIdx.x) * RUN;

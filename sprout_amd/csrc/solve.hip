@@ -124,7 +124,7 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
 
 // ---- hierarchy set-up -------------------------------------------------------------------
 
-// Galerkin coarse matrix of coarse cell (cx, cy) from fine level F (mg_build.comp:80-102).
+// Galerkin coarse matrix of coarse cell (cx, cy) from fine level F (mg_build.comp:43-65).
 SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
     float D = 0.0f;
     float4 A = make_float4(0, 0, 0, 0);
@@ -150,7 +150,7 @@ SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
     c.mga[o] = D > 1e-7f ? A : make_float4(0, 0, 0, 0);
 }
 
-// mg_build.comp:53-103: coarse level l from level l - 1, over coarse 16 x 16 tiles.
+// mg_build.comp:16-65: coarse level l from level l - 1, over coarse 16 x 16 tiles.
 __global__ __launch_bounds__(256) void k_mg_build(Ctx c, int l) {
     __shared__ uint32_t s_state;
     const Lv F = level(c, l - 1), C = level(c, l);
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_cg_apply(Ctx c, int nblocks, const floa
 
 namespace {
 
-// The way down on one level (mg_down.comp:139-172) over a tile whose residual for the tile
+// The way down on one level (mg_down.comp:36-69) over a tile whose residual for the tile
 // plus two rings is in sr (ring 2) and diagonals in sd (ring 2): z of the tile and its first
 // ring, then the restriction of the red cells' residual to the coarse level.
 struct DownStage {
@@ -413,7 +413,7 @@ SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int x0, int y0, DownStage& S
 
 }  // namespace
 
-// mg_down.comp:139-172 on level l: pre-smoothing from zero and restriction of the residual.
+// mg_down.comp:36-69 on level l: pre-smoothing from zero and restriction of the residual.
 // r_in: the level's right-hand side (level 0: the CG residual buffer of this iteration).
 __global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated, const float2* __restrict__ r_in) {
     __shared__ DownStage S;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void k_update_down0(Ctx c, const float2* __res
     down_compute(c, F, C, x0, y0, S, c.mgx);
 }
 
-// mg_up.comp:210-247 on level l: coarse-grid correction and the black-then-red
+// mg_up.comp:38-75 on level l: coarse-grid correction and the black-then-red
 // post-smoothing; on level 0 (reduce >= 0) also the partial sums r.z, r.r.
 __global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int reduce, int nblocks, const float2* __restrict__ r_in) {
     __shared__ double4 lds[256];
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int redu
         szc[k] = inside(C, gx, gy) ? c.mgz[at(C, gx, gy)] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    // new values of the black cells of the tile and its first ring (mg_up.comp:198-208)
+    // new values of the black cells of the tile and its first ring (mg_up.comp:26-36)
     for (int k = t; k < R1 * R1; k += 256) {
         int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
         float2 z = make_float2(0.0f, 0.0f);

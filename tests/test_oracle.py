@@ -274,7 +274,7 @@ def test_mgpcg_solves_the_system():
 
 
 def discrete_divergence(vel, solid, W, H):
-    """rhs.comp:87-88: aR u(i+1) - aL u(i) + aT v(i+W) - aB v(i), per cell."""
+    """rhs.comp:49: aR u(i+1) - aL u(i) + aT v(i+W) - aB v(i), per cell."""
     u = vel[..., 0].astype(np.float64)
     v = vel[..., 1].astype(np.float64)
     aL, aB = solid[..., 0].astype(np.float64), solid[..., 1].astype(np.float64)
@@ -285,7 +285,7 @@ def discrete_divergence(vel, solid, W, H):
 
 
 def test_divergence_after_projection():
-    """After project.comp the discrete divergence of every pressure unknown (rhs.comp:88)
+    """After project.comp the discrete divergence of every pressure unknown (rhs.comp:49)
     is at the solver's tolerance relative to the divergence it started from."""
     W, H = 64, 36
     o, g, mats, parts, field = make((W, H))
