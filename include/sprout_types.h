@@ -107,6 +107,12 @@ typedef struct sp_sim_params {
     uint32_t slip_cap;
     uint32_t sealed_vent;
     float min_substep_share;
+    uint32_t viscous_projection;   /* 1: a second pressure solve after the implicit viscosity */
+    uint32_t viscous_warm;         /* 1: each pressure solve starts from its own part of the last */
+    uint32_t viscosity_warm;       /* 1: the viscosity solve starts from the last substep's change */
+    uint32_t viscosity_precond;    /* 1: multigrid V-cycle, 0: Jacobi */
+    uint32_t viscosity_mg_levels;  /* levels below the grid in the viscosity V-cycle */
+    uint32_t viscosity_mg_sweeps;  /* Gauss-Seidel sweeps per visit of a level (1 or 2) */
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -174,6 +180,13 @@ typedef struct sp_gparams {
     int32_t p_max_iters, v_max_iters;
     float heat_x;    /* ambient temperature (P.heat.x) */
     float ln_rigid;  /* ln rigid viscosity, cells^2/s (P.heat2.x) */
+    /* The implicit viscosity solve (visc_mode 2), from SimParams: */
+    int32_t v_twice;      /* viscous_projection: pressure, viscosity, pressure again */
+    int32_t v_p2warm;     /* viscous_warm: P1 and P2 warm-start their own solves */
+    int32_t v_warm;       /* viscosity_warm */
+    int32_t v_precond;    /* viscosity_precond: 1 multigrid, 0 Jacobi */
+    int32_t v_mg_levels;  /* levels below the grid in its V-cycle (1 .. tail) */
+    int32_t v_mg_sweeps;  /* 1 or 2 */
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -200,7 +213,12 @@ typedef struct sp_solve {
     uint32_t it_max, unconv, solves, arrive;
     float res_max;
     uint32_t iters_last;
-    uint32_t pad[2];
+    uint32_t p2_it_max, p2_unconv;   /* the second pressure solve (after the viscosity) */
+    /* the viscosity solve: one right-hand side (u and v faces together) */
+    double v_rz, v_alpha, v_beta, v_r0, v_rr, v_pq;
+    uint32_t v_done, v_iters, v_it_max, v_unconv;
+    float v_res_max;
+    uint32_t v_iters_last, p2_iters_last, pad;
 } sp_solve;
 
 #define SP_MAX_LEVELS 16
@@ -262,6 +280,23 @@ enum {
     SP_FIELD_VW = 23,        /* float4[N] */
     SP_FIELD_VS = 24,        /* float4[N] */
     SP_FIELD_GBASE = 25,     /* float4[N] */
+    SP_FIELD_VR = 26,        /* float2[N]  viscosity residual */
+    SP_FIELD_VP = 27,        /* float2[N]  search direction */
+    SP_FIELD_VP2 = 28,       /* float2[N]  the other direction buffer (Jacobi) */
+    SP_FIELD_VQ = 29,        /* float2[N]  A p */
+    SP_FIELD_VD = 30,        /* float2[N]  the change x - u* of the last solve */
+    SP_FIELD_VR2 = 31,       /* float2[N]  second residual buffer (multigrid) */
+    SP_FIELD_VQ2 = 32,       /* float2[N]  second A p buffer (multigrid) */
+    SP_FIELD_VAZ = 33,       /* float2[N]  A z */
+    SP_FIELD_VTILES = 34,    /* uint32[tiles of level 0] active viscosity tiles */
+    SP_FIELD_VTILE = 35,     /* uint32[tiles] per-level tile flags of the viscosity multigrid */
+    SP_FIELD_VMA = 36,       /* float4[total] */
+    SP_FIELD_VMB = 37,       /* float4[total] */
+    SP_FIELD_VMD = 38,       /* float2[total] */
+    SP_FIELD_VMR = 39,       /* float2[total] */
+    SP_FIELD_VMX = 40,       /* float2[total] */
+    SP_FIELD_VMZ = 41,       /* float2[total] */
+    SP_FIELD_P2KEEP = 42,    /* float[N] */
     SP_FIELD_COUNT
 };
 
@@ -273,13 +308,19 @@ enum {
     SP_STAGE_GRID_PREP = 3,
     SP_STAGE_COEF = 4,
     SP_STAGE_SURF = 5,       /* surf_smooth + surf_curv + surf_apply */
-    SP_STAGE_VISC = 6,       /* explicit viscosity (visc_setup + visc_apply mode 2) */
+    SP_STAGE_VISC = 6,       /* viscosity before the first solve: explicit, or the implicit split (+ solve) */
     SP_STAGE_RHS = 7,        /* fill ptile + rhs.comp */
     SP_STAGE_PRESSURE = 8,   /* record_pressure (first solve) */
     SP_STAGE_PROJECT = 9,
     SP_STAGE_SUBSTEP = 10,   /* the whole substep; arg = gather flag */
     SP_STAGE_MG_BUILD = 11,  /* the Galerkin hierarchy only */
     SP_STAGE_VCYCLE = 12,    /* one V-cycle: mgr level 0 -> mgz level 0 */
+    SP_STAGE_VISC_SPLIT = 13,   /* record_viscosity_split: weights, tiles, explicit step, matrices */
+    SP_STAGE_VISC_SOLVE = 14,   /* record_viscosity_solve; arg = restrain */
+    SP_STAGE_RHS_VISC = 15,     /* rhs_visc.comp */
+    SP_STAGE_PRESSURE2 = 16,    /* the second pressure solve */
+    SP_STAGE_PROJECT_VISC = 17, /* project_visc.comp */
+    SP_STAGE_VISC_VCYCLE = 18,  /* one viscosity V-cycle: vr -> vmz level 0 (and vaz) */
     SP_STAGE_COUNT
 };
 

@@ -7,15 +7,17 @@
  * elasticity, settling, steam, reactions), tools, currents or film spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
- *   visc_apply.comp mode 2 (explicit viscosity), rhs.comp, mg_build/down/up/tail.comp,
- *   cg_scale/init/apply/update.comp with the reduce_tail.glsl scalar steps, project.comp,
- *   sort.comp, count.comp, stats.comp.
+ *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
+ *   (visc_setup/split/diag/strain/apply/iter/update.comp, vmg_build/down/up.comp,
+ *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
+ *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
+ *   pressure solve + project_visc.comp, sort.comp, count.comp, stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
  * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
  * fixed tree (reduce_tail.glsl:28-41).
  *
- * Built by oracle/Makefile into oracle/_ref/liboracle.so (gcc, no GPU).
+ * Built by oracle/Makefile into oracle/liboracle.so (gcc, no GPU).
  */
 #include "oracle.h"
 
@@ -46,6 +48,12 @@ struct or_sim {
     float* mgd;
     v2 *mgx, *mgz, *mgr, *cgp, *cgq;
     uint32_t* ptile;
+    /* implicit viscosity */
+    v2 *vr, *vp, *vp2, *vq, *vd, *vr2, *vq2, *vaz;
+    uint32_t* vtiles;  /* active viscosity tiles (16 x 16) of the grid */
+    uint32_t* vtile;   /* per-level tile flags of the viscosity multigrid */
+    v4 *vma, *vmb;
+    v2 *vmd, *vmr, *vmx, *vmz;
     sp_stepbuf S;
     sp_solve K;
 };
@@ -1011,44 +1019,11 @@ static v4 strain_of_vel(const or_sim* s, int i, float wx, float wy) {
     return o;
 }
 
+static void visc_setup(or_sim* s, int mark);
+
 static void stage_visc_explicit(or_sim* s) {
     int W = s->W, H = s->H;
-    float dt = s->S.dt;
-    /* visc_setup.comp:14-58 (the tile flags only matter to the implicit solve). */
-    for (int cy = 0; cy < H; cy++)
-        for (int cx = 0; cx < W; cx++) {
-            int i = cidx(s, cx, cy);
-            float wc = 0.0f, wn = 0.0f;
-            if (cx >= 1 && cy >= 1 && cx < W - 1 && cy < H - 1) {
-                v4 me = s->cell[i];
-#define OK_U(j) (live_u(s, j) || closed_u(s, j))
-#define OK_V(j) (live_v(s, j) || closed_v(s, j))
-                if (me.w > 0.0f && OK_U(i) && OK_U(i + 1) && OK_V(i) && OK_V(i + W)) wc = 2.0f * dt * me.z;
-                int cs4[4] = {i - W - 1, i - W, i - 1, i};
-                int nliq = 0, any_air = 0;
-                float mu = 0.0f;
-                for (int k = 0; k < 4; k++) {
-                    v4 q = s->cell[cs4[k]];
-                    if (q.w > 0.0f) {
-                        nliq++;
-                        mu += q.z;
-                    } else if (s->solid[cs4[k]].w > 0.02f) {
-                        any_air = 1;
-                    }
-                }
-                if (nliq > 0 && !any_air && OK_U(i) && OK_U(i - W) && OK_V(i) && OK_V(i - 1))
-                    wn = 4.0f * dt * (mu / (float)nliq) * ((float)nliq / 4.0f);
-#undef OK_U
-#undef OK_V
-            }
-            s->vw[i].x = wc;
-            s->vw[i].y = wn;
-        }
-    for (int cy = 0; cy < H; cy++)
-        for (int cx = 0; cx < W; cx++) {
-            int i = cidx(s, cx, cy);
-            s->vs[i] = strain_of_vel(s, i, s->vw[i].x, s->vw[i].y);
-        }
+    visc_setup(s, 0);
     /* visc_apply.comp mode 2 (visc_apply.comp:16-48). */
     for (int cy = 1; cy < H - 1; cy++)
         for (int cx = 1; cx < W - 1; cx++) {
@@ -1446,7 +1421,8 @@ static void vcycle(or_sim* s, double sums[4]) {
     for (int l = s->lv.tail - 1; l >= 0; l--) mg_up(s, l, l == 0 ? sums : NULL);
 }
 
-/* cg_scale.comp:14-27 (mode 0) */
+/* cg_scale.comp:14-27. Mode 1 (a second solve follows the viscosity, with its own warm
+ * start): the first solve starts from last substep's P1 = P - P2. */
 static void cg_scale(or_sim* s) {
     lvl F = level_of(s, 0);
     int tx = (F.w + 15) >> 4;
@@ -1457,7 +1433,8 @@ static void cg_scale(or_sim* s) {
             int i = lat(F, x, y);
             if (s->mgd[i] > 0.0f) {
                 float p = s->pres[i].x;
-                s->p2keep[i] = 0.0f;
+                if (s->P.v_p2warm) p -= s->p2keep[i];
+                else s->p2keep[i] = 0.0f;
                 s->pres[i].x = p * ratio;
             } else {
                 s->pres[i] = (v2){0, 0};
@@ -1503,13 +1480,14 @@ static v2 direction(const or_sim* s, int i, float bx, float by) {
     return (v2){bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y};
 }
 
-static void stage_pressure(or_sim* s) {
+/* The CG iterations of a pressure solve from pres (record_cg, mod.rs:3012-3031). second:
+ * the solve after the viscosity (cg_reduce modes 8 and 9: the density correction counts as
+ * solved). */
+static void pressure_iterate(or_sim* s, int second) {
     const sp_gparams* P = &s->P;
     sp_solve* K = &s->K;
     lvl F = level_of(s, 0);
     int N = F.w * F.h;
-    stage_mg_build(s);
-    cg_scale(s);
     double bb[2];
     cg_init(s, bb);
     double a[4] = {0, 0, 0, 0};
@@ -1526,7 +1504,7 @@ static void stage_pressure(or_sim* s) {
     K->alpha[0] = K->alpha[1] = 0.0;
     K->iters[0] = K->iters[1] = 0;
     K->done[0] = a[2] <= tol2x * bb[0];
-    K->done[1] = a[3] <= tol2y * bb[1];
+    K->done[1] = second ? 1u : (a[3] <= tol2y * bb[1]);
     K->solves += 1;
     v2* np = (v2*)malloc(sizeof(v2) * (size_t)N);
     uint32_t k = 0;
@@ -1589,13 +1567,26 @@ static void stage_pressure(or_sim* s) {
         K->done[1] = d1;
     }
     free(np);
-    /* cg_reduce mode 3 (reduce_tail.glsl:105-116) */
+    /* cg_reduce mode 3 / 9 (reduce_tail.glsl:105-116) */
     uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];
-    if (it > K->it_max) K->it_max = it;
-    if (K->done[0] == 0) K->unconv += 1;
+    if (second) {
+        if (K->iters[0] > K->p2_it_max) K->p2_it_max = K->iters[0];
+        if (K->done[0] == 0) K->p2_unconv += 1;
+        K->p2_iters_last = K->iters[0];
+    } else {
+        if (it > K->it_max) K->it_max = it;
+        if (K->done[0] == 0) K->unconv += 1;
+        K->iters_last = k;
+    }
     float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
     if (rel > K->res_max) K->res_max = rel;
-    K->iters_last = k;
+}
+
+/* record_pressure (mod.rs:2956-2990), the first solve. */
+static void stage_pressure(or_sim* s) {
+    stage_mg_build(s);
+    cg_scale(s);
+    pressure_iterate(s, 0);
 }
 
 /* ---- project.comp (project.comp:26-58) ------------------------------------------------ */
@@ -1636,6 +1627,770 @@ static void stage_project(or_sim* s) {
     s->S.umax_grid = f2u(m);
 }
 
+/* ---- implicit viscosity (visc.glsl, visc_setup/split/diag/strain/apply/iter/update.comp,
+ *      vmg.glsl, vmg_build/down/up.comp, visc_mgu/mgz.comp, reduce_tail.glsl modes 4-7,
+ *      10-13) ------------------------------------------------------------------------------
+ *
+ * The reference runs the viscosity multigrid tile by tile, each 16 x 16 tile smoothing in
+ * shared memory a region of the tile plus 2 NU (+1) cells, every half-sweep on a ring one
+ * cell narrower than the last (vmg.glsl:113-134). That is the same arithmetic as sweeping the
+ * whole level (the cells each ring needs were updated exactly by the half-sweep before), so
+ * the restatement sweeps whole levels, reading zero outside the level, and writes back only
+ * the cells of active tiles, as the tiles do. */
+
+static int vtile_of(const or_sim* s, int x, int y) { return (y >> 4) * ((s->W + 15) >> 4) + (x >> 4); }
+static int v_on(const or_sim* s, int x, int y) { return s->vtiles[vtile_of(s, x, y)] != 0u; }
+static int v_interior(const or_sim* s, int x, int y) { return x >= 1 && y >= 1 && x < s->W - 1 && y < s->H - 1; }
+/* GLSL max(x, y): y if x < y, otherwise x. */
+static float gmax(float x, float y) { return x < y ? y : x; }
+
+/* Per-cell FP64 dot products of two float2 (the build's precision for the solve scalars). */
+static double dot2(v2 a, v2 b) { return (double)a.x * (double)b.x + (double)a.y * (double)b.y; }
+
+/* visc_setup.comp:14-58 with the tile marks (mark != 0): weights, strain of the velocity. */
+static void visc_setup(or_sim* s, int mark) {
+    int W = s->W, H = s->H;
+    float dt = s->S.dt;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            float wc = 0.0f, wn = 0.0f;
+            if (v_interior(s, cx, cy)) {
+                v4 me = s->cell[i];
+#define OK_U(j) (live_u(s, j) || closed_u(s, j))
+#define OK_V(j) (live_v(s, j) || closed_v(s, j))
+                if (me.w > 0.0f && OK_U(i) && OK_U(i + 1) && OK_V(i) && OK_V(i + W)) wc = 2.0f * dt * me.z;
+                int cs4[4] = {i - W - 1, i - W, i - 1, i};
+                int nliq = 0, any_air = 0;
+                float mu = 0.0f;
+                for (int k = 0; k < 4; k++) {
+                    v4 q = s->cell[cs4[k]];
+                    if (q.w > 0.0f) {
+                        nliq++;
+                        mu += q.z;
+                    } else if (s->solid[cs4[k]].w > 0.02f) {
+                        any_air = 1;
+                    }
+                }
+                if (nliq > 0 && !any_air && OK_U(i) && OK_U(i - W) && OK_V(i) && OK_V(i - 1))
+                    wn = 4.0f * dt * (mu / (float)nliq) * ((float)nliq / 4.0f);
+#undef OK_U
+#undef OK_V
+            }
+            s->vw[i].x = wc;
+            s->vw[i].y = wn;
+        }
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            s->vs[i] = strain_of_vel(s, i, s->vw[i].x, s->vw[i].y);
+            if (!mark) continue;
+            v4 me = s->cell[i];
+            if (me.w > 0.0f && dt * me.z / fmaxf_(me.y, 0.05f) >= s->P.corr[3]) {
+                for (int b = -1; b <= 1; b++)
+                    for (int a = -1; a <= 1; a++)
+                        s->vtiles[vtile_of(s, clampi(cx + a, 0, W - 1), clampi(cy + b, 0, H - 1))] = 1u;
+            }
+        }
+}
+
+/* G^T of a weighted strain field st at the faces of cell i (visc_apply.comp:16-28). */
+static v2 gts_of(const or_sim* s, const v4* st, int i) {
+    int W = s->W;
+    v2 g = {0.0f, 0.0f};
+    if (live_u(s, i)) {
+        float c1 = closed_u(s, i - W) ? 1.0f : 0.5f, c2 = closed_u(s, i + W) ? -1.0f : -0.5f;
+        g.x = st[i - 1].x - st[i].x + st[i].z * c1 + st[i + W].z * c2;
+    }
+    if (live_v(s, i)) {
+        float d1 = closed_v(s, i - 1) ? 1.0f : 0.5f, d2 = closed_v(s, i + 1) ? -1.0f : -0.5f;
+        g.y = st[i - W].y - st[i].y + st[i].z * d1 + st[i + 1].z * d2;
+    }
+    return g;
+}
+
+/* visc_split.comp:24-38: the explicit step in the inactive tiles. */
+static void visc_split(or_sim* s) {
+    for (int cy = 1; cy < s->H - 1; cy++)
+        for (int cx = 1; cx < s->W - 1; cx++) {
+            if (v_on(s, cx, cy)) continue;
+            int i = cidx(s, cx, cy);
+            v2 g = gts_of(s, s->vs, i);
+            v4 cf = s->coef[i];
+            if (live_u(s, i)) s->vel[i].x -= g.x / cf.z;
+            if (live_v(s, i)) s->vel[i].y -= g.y / cf.w;
+        }
+}
+
+/* Strain weights as the implicit system sees them (visc_diag.comp:75-81). */
+static v2 v_wts(const or_sim* s, int x, int y) {
+    if (!v_on(s, x, y)) return (v2){0.0f, 0.0f};
+    v4 w = s->vw[cidx(s, x, y)];
+    return (v2){w.x, w.y};
+}
+static int unk_u(const or_sim* s, int x, int y) { return v_interior(s, x, y) && v_on(s, x, y) && live_u(s, cidx(s, x, y)); }
+static int unk_v(const or_sim* s, int x, int y) { return v_interior(s, x, y) && v_on(s, x, y) && live_v(s, cidx(s, x, y)); }
+
+/* The couplings of A at cell (x, y) (visc_diag.comp:83-117). */
+static void couplings(const or_sim* s, int x, int y, v4* a, v4* b) {
+    int W = s->W, i = cidx(s, x, y);
+    int u = unk_u(s, x, y), v = unk_v(s, x, y);
+    *a = (v4){0, 0, 0, 0};
+    *b = (v4){0, 0, 0, 0};
+    if (u && unk_u(s, x - 1, y)) a->x = -v_wts(s, x - 1, y).x;
+    if (v && unk_v(s, x, y - 1)) a->w = -v_wts(s, x, y - 1).x;
+    float wn = v_wts(s, x, y).y;
+    if (wn > 0.0f) {
+        float ua = u ? (closed_u(s, i - W) ? 1.0f : 0.5f) : 0.0f;
+        float ub = unk_u(s, x, y - 1) ? -(closed_u(s, i) ? 1.0f : 0.5f) : 0.0f;
+        float va = v ? (closed_v(s, i - 1) ? 1.0f : 0.5f) : 0.0f;
+        float vb = unk_v(s, x - 1, y) ? -(closed_v(s, i) ? 1.0f : 0.5f) : 0.0f;
+        a->y = wn * ua * ub;
+        a->z = wn * va * vb;
+        b->x = wn * ua * va;
+        b->y = wn * ua * vb;
+    }
+    float wu = v_wts(s, x, y + 1).y;
+    if (wu > 0.0f && u) {
+        float ub = -(closed_u(s, i + W) ? 1.0f : 0.5f);
+        float va = unk_v(s, x, y + 1) ? (closed_v(s, i + W - 1) ? 1.0f : 0.5f) : 0.0f;
+        float vb = unk_v(s, x - 1, y + 1) ? -(closed_v(s, i + W) ? 1.0f : 0.5f) : 0.0f;
+        b->z = wu * ub * va;
+        b->w = wu * ub * vb;
+    }
+}
+
+/* visc_diag.comp:121-184: the Jacobi diagonal, the warm start, the clearing of inactive
+ * tiles and (multigrid) the matrix of level 0. */
+static void visc_diag(or_sim* s) {
+    int W = s->W, H = s->H;
+    int mg = s->P.v_precond != 0, warm = s->P.v_warm != 0;
+    int ntx = (W + 15) >> 4, nt = ntx * ((H + 15) >> 4);
+    uint8_t* was = (uint8_t*)calloc((size_t)nt, 1);
+    if (mg)
+        for (int t = 0; t < nt; t++) {
+            was[t] = s->vtile[t] != 0u;
+            s->vtile[t] = s->vtiles[t];
+        }
+    float ratio = s->S.dt / s->S.dt_prev;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            int on = v_on(s, cx, cy);
+            v2 d = {1.0f, 1.0f};
+            int lu = live_u(s, i), lv = live_v(s, i);
+            if (v_interior(s, cx, cy)) {
+                v4 cf = s->coef[i];
+                v2 me = v_wts(s, cx, cy);
+                if (lu) {
+                    float c1 = closed_u(s, i - W) ? 1.0f : 0.5f, c2 = closed_u(s, i + W) ? 1.0f : 0.5f;
+                    d.x = cf.z + v_wts(s, cx - 1, cy).x + me.x + me.y * c1 * c1 + v_wts(s, cx, cy + 1).y * c2 * c2;
+                }
+                if (lv) {
+                    float d1 = closed_v(s, i - 1) ? 1.0f : 0.5f, d2 = closed_v(s, i + 1) ? 1.0f : 0.5f;
+                    d.y = cf.w + v_wts(s, cx, cy - 1).x + me.x + me.y * d1 * d1 + v_wts(s, cx + 1, cy).y * d2 * d2;
+                }
+            }
+            v4 o = s->vw[i];
+            o.z = d.x;
+            o.w = d.y;
+            v2 dv = {0.0f, 0.0f};
+            if (warm && on && v_interior(s, cx, cy)) {
+                dv.x = s->vd[i].x * ratio * (lu ? 1.0f : 0.0f);
+                dv.y = s->vd[i].y * ratio * (lv ? 1.0f : 0.0f);
+            }
+            s->vd[i] = dv;
+            if (!on) {
+                o.x = o.y = 0.0f;
+                s->vs[i] = (v4){0, 0, 0, 0};
+                s->vr[i] = s->vp[i] = s->vp2[i] = s->vq[i] = (v2){0, 0};
+            }
+            if (mg && (on || was[vtile_of(s, cx, cy)])) {
+                v4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+                v2 dm = {0.0f, 0.0f};
+                if (on && v_interior(s, cx, cy)) {
+                    couplings(s, cx, cy, &a, &b);
+                    dm.x = d.x * (lu ? 1.0f : 0.0f);
+                    dm.y = d.y * (lv ? 1.0f : 0.0f);
+                }
+                if (!on) s->vmz[i] = s->vaz[i] = s->vr2[i] = s->vq2[i] = (v2){0, 0};
+                s->vma[i] = a;
+                s->vmb[i] = b;
+                s->vmd[i] = dm;
+            }
+            /* written after the neighbours' weights are read: o.xy of an active tile does
+             * not change and an inactive tile's weights read as zero either way */
+            s->vw[i] = o;
+        }
+    free(was);
+}
+
+static int vflag(const or_sim* s, int l, int slot) {
+    return l == 0 ? s->vtiles[slot] != 0u : s->vtile[s->lv.tile_offs[l] + slot] != 0u;
+}
+
+/* vmg_build.comp:187-240: level l of the viscosity multigrid from level l - 1. A tile that
+ * is not active reads as zero (the reference clears it once when it goes inactive). */
+static void vmg_build(or_sim* s, int l) {
+    lvl F = level_of(s, l - 1), C = level_of(s, l);
+    static const int UVX[4] = {0, -1, 0, -1}, UVY[4] = {0, 0, 1, 1};
+    int tfx = (F.w + 15) >> 4, tfy = (F.h + 15) >> 4;
+    int tcx = (C.w + 15) >> 4, tcy = (C.h + 15) >> 4;
+    for (int ty = 0; ty < tcy; ty++)
+        for (int tx = 0; tx < tcx; tx++) {
+            uint32_t on = 0;
+            for (int k = 0; k < 4; k++) {
+                int fx = 2 * tx + (k & 1), fy = 2 * ty + (k >> 1);
+                if (fx < tfx && fy < tfy) on |= (uint32_t)vflag(s, l - 1, fy * tfx + fx);
+            }
+            s->vtile[s->lv.tile_offs[l] + ty * tcx + tx] = on ? 1u : 0u;
+        }
+    for (int cy = 0; cy < C.h; cy++)
+        for (int cx = 0; cx < C.w; cx++) {
+            int o = lat(C, cx, cy);
+            if (!vflag(s, l, (cy >> 4) * tcx + (cx >> 4))) {
+                s->vma[o] = s->vmb[o] = (v4){0, 0, 0, 0};
+                s->vmd[o] = (v2){0, 0};
+                continue;
+            }
+            v2 D = {0.0f, 0.0f};
+            v4 A = {0, 0, 0, 0}, B = {0, 0, 0, 0};
+            for (int k = 0; k < 4; k++) {
+                int ax = k & 1, ay = k >> 1;
+                int x = 2 * cx + ax, y = 2 * cy + ay;
+                if (!inside(F, x, y)) continue;
+                int i = lat(F, x, y);
+                v4 a = s->vma[i], b = s->vmb[i];
+                D.x += s->vmd[i].x;
+                D.y += s->vmd[i].y;
+                if (ax == 1) {
+                    D.x += 2.0f * a.x;
+                    D.y += 2.0f * a.z;
+                } else {
+                    A.x += a.x;
+                    A.z += a.z;
+                }
+                if (ay == 1) {
+                    D.x += 2.0f * a.y;
+                    D.y += 2.0f * a.w;
+                } else {
+                    A.y += a.y;
+                    A.w += a.w;
+                }
+                for (int m = 0; m < 4; m++) {
+                    int rx = ((x + UVX[m]) >> 1) - cx, ry = ((y + UVY[m]) >> 1) - cy;
+                    int sl = (rx == 0 ? 0 : 1) + (ry == 0 ? 0 : 2);
+                    float bm = v4c(b, m);
+                    if (sl == 0) B.x += bm;
+                    else if (sl == 1) B.y += bm;
+                    else if (sl == 2) B.z += bm;
+                    else B.w += bm;
+                }
+            }
+            s->vma[o] = A;
+            s->vmb[o] = B;
+            s->vmd[o] = (v2){gmax(D.x, 0.0f), gmax(D.y, 0.0f)};
+        }
+}
+
+/* One level of the viscosity multigrid in whole-level arrays (vmg.glsl:64-134). */
+typedef struct {
+    lvl L;
+    v4 *a, *b;
+    v2 *d, *r, *x;
+} vlev;
+
+static v4 va_(const vlev* V, int x, int y) { return inside(V->L, x, y) ? V->a[lat(V->L, x, y) - V->L.off] : (v4){0, 0, 0, 0}; }
+static v4 vb_(const vlev* V, int x, int y) { return inside(V->L, x, y) ? V->b[lat(V->L, x, y) - V->L.off] : (v4){0, 0, 0, 0}; }
+static v2 vx_(const vlev* V, const v2* xs, int x, int y) { return inside(V->L, x, y) ? xs[lat(V->L, x, y) - V->L.off] : (v2){0, 0}; }
+
+/* Off-diagonal part of A x at (x, y) (vmg.glsl:69-78). */
+static v2 v_offd(const vlev* V, const v2* xs, int x, int y) {
+    v4 a = va_(V, x, y), b = vb_(V, x, y);
+    v2 xl = vx_(V, xs, x - 1, y), xr = vx_(V, xs, x + 1, y), xd = vx_(V, xs, x, y - 1), xu = vx_(V, xs, x, y + 1);
+    v2 xc = vx_(V, xs, x, y), xlu = vx_(V, xs, x - 1, y + 1), xrd = vx_(V, xs, x + 1, y - 1);
+    v4 ar = va_(V, x + 1, y), au = va_(V, x, y + 1), br = vb_(V, x + 1, y), bd = vb_(V, x, y - 1), brd = vb_(V, x + 1, y - 1);
+    float u = a.x * xl.x + ar.x * xr.x + a.y * xd.x + au.y * xu.x + b.x * xc.y + b.y * xl.y + b.z * xu.y + b.w * xlu.y;
+    float v = a.z * xl.y + ar.z * xr.y + a.w * xd.y + au.w * xu.y + b.x * xc.x + br.y * xr.x + bd.z * xd.x + brd.w * xrd.x;
+    return (v2){u, v};
+}
+
+/* A half-sweep of one colour over the level: every update reads the values from before it
+ * (vmg.glsl:119-134). */
+static void v_half_sweep(vlev* V, int red, v2* tmp) {
+    int n = V->L.w * V->L.h;
+    memcpy(tmp, V->x, sizeof(v2) * (size_t)n);
+    for (int y = 0; y < V->L.h; y++)
+        for (int x = 0; x < V->L.w; x++) {
+            if ((((x + y) & 1) == 0) != red) continue;
+            int k = y * V->L.w + x;
+            v2 d = V->d[k];
+            v2 o = v_offd(V, tmp, x, y);
+            v2 sres = {V->r[k].x - o.x, V->r[k].y - o.y};
+            V->x[k] = (v2){d.x > 0.0f ? sres.x / d.x : 0.0f, d.y > 0.0f ? sres.y / d.y : 0.0f};
+        }
+}
+
+static vlev vlevel(or_sim* s, int l, v2* rhs, v2* xs) {
+    vlev V;
+    V.L = level_of(s, l);
+    V.a = s->vma + V.L.off;
+    V.b = s->vmb + V.L.off;
+    V.d = s->vmd + V.L.off;
+    V.r = rhs;
+    V.x = xs;
+    return V;
+}
+
+/* vmg.glsl:140-168 (way_down) on level l with right-hand side rhs (level-local array):
+ * pre-smoothing from zero, vmx of the active tiles, and the restricted residual into vmr of
+ * level l + 1 unless l is the coarsest level of the cycle. */
+static void vmg_down(or_sim* s, int l, int coarsest, v2* rhs) {
+    lvl F = level_of(s, l);
+    int n = F.w * F.h, halo = 2 * s->P.v_mg_sweeps;
+    v2* xs = (v2*)calloc((size_t)n, sizeof(v2));
+    v2* tmp = (v2*)malloc(sizeof(v2) * (size_t)n);
+    vlev V = vlevel(s, l, rhs, xs);
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            int k = y * F.w + x;
+            v2 d = V.d[k], r = rhs[k];
+            v2 z = {d.x > 0.0f ? r.x / d.x : 0.0f, d.y > 0.0f ? r.y / d.y : 0.0f};
+            xs[k] = ((x + y) & 1) == 0 ? z : (v2){0.0f, 0.0f};
+        }
+    for (int h = 1; h < halo; h++) v_half_sweep(&V, (h & 1) == 0, tmp);
+    int tx = (F.w + 15) >> 4;
+    v2* res = tmp;
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            int k = y * F.w + x;
+            v2 d = V.d[k], o = v_offd(&V, xs, x, y);
+            res[k].x = (rhs[k].x - d.x * xs[k].x - o.x) * (d.x > 0.0f ? 1.0f : 0.0f);
+            res[k].y = (rhs[k].y - d.y * xs[k].y - o.y) * (d.y > 0.0f ? 1.0f : 0.0f);
+            if (vflag(s, l, (y >> 4) * tx + (x >> 4))) s->vmx[F.off + k] = xs[k];
+        }
+    if (!coarsest) {
+        lvl C = level_of(s, l + 1);
+        for (int cy = 0; cy < C.h; cy++)
+            for (int cx = 0; cx < C.w; cx++) {
+                int x = 2 * cx, y = 2 * cy;
+                if (!vflag(s, l, (y >> 4) * tx + (x >> 4))) continue;
+                v2 r00 = vx_(&V, res, x, y), r10 = vx_(&V, res, x + 1, y), r01 = vx_(&V, res, x, y + 1), r11 = vx_(&V, res, x + 1, y + 1);
+                s->vmr[lat(C, cx, cy)] = (v2){r00.x + r10.x + r01.x + r11.x, r00.y + r10.y + r01.y + r11.y};
+            }
+    }
+    free(xs);
+    free(tmp);
+}
+
+/* The start of a way up: x = vmx (+ the coarse correction), masked to the unknowns, then
+ * NU sweeps black then red (vmg_up.comp:269-290, visc_mgz.comp:94-105). Returns x. */
+static v2* vmg_up_x(or_sim* s, int l, int coarsest, v2* rhs, vlev* V) {
+    lvl F = level_of(s, l);
+    int n = F.w * F.h, halo = 2 * s->P.v_mg_sweeps;
+    v2* xs = (v2*)calloc((size_t)n, sizeof(v2));
+    v2* tmp = (v2*)malloc(sizeof(v2) * (size_t)n);
+    *V = vlevel(s, l, rhs, xs);
+    lvl C = level_of(s, l + 1 < s->lv.n ? l + 1 : s->lv.n - 1);
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            int k = y * F.w + x;
+            v2 xv = s->vmx[F.off + k];
+            if (!coarsest) {
+                v2 zc = s->vmz[lat(C, x >> 1, y >> 1)];
+                xv.x += zc.x;
+                xv.y += zc.y;
+            }
+            v2 d = V->d[k];
+            xs[k] = (v2){xv.x * (d.x > 0.0f ? 1.0f : 0.0f), xv.y * (d.y > 0.0f ? 1.0f : 0.0f)};
+        }
+    for (int h = 1; h <= halo; h++) v_half_sweep(V, (h & 1) == 0, tmp);
+    free(tmp);
+    return xs;
+}
+
+/* vmg_up.comp:269-290 on a coarse level l. */
+static void vmg_up(or_sim* s, int l, int coarsest) {
+    lvl F = level_of(s, l);
+    vlev V;
+    v2* xs = vmg_up_x(s, l, coarsest, s->vmr + F.off, &V);
+    int tx = (F.w + 15) >> 4;
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++)
+            if (vflag(s, l, (y >> 4) * tx + (x >> 4))) s->vmz[F.off + y * F.w + x] = xs[y * F.w + x];
+    free(xs);
+}
+
+/* The coarse levels of a V-cycle, down and up (record_visc_coarse, mod.rs:2838-2853). */
+static void visc_coarse(or_sim* s) {
+    int n = s->P.v_mg_levels;
+    for (int l = 1; l <= n; l++) vmg_down(s, l, l == n, s->vmr + s->lv.offs[l]);
+    for (int l = n; l >= 1; l--) vmg_up(s, l, l == n);
+}
+
+/* visc_mgz.comp:85-120: the way up on level 0 from residual r; z, A z and the sums
+ * r.z, z.Az, z.q (q: this iteration's A p). */
+static void visc_mgz(or_sim* s, const v2* r, const v2* q, double sums[3]) {
+    lvl F = level_of(s, 0);
+    vlev V;
+    v2* rl = (v2*)malloc(sizeof(v2) * (size_t)s->N);
+    memcpy(rl, r, sizeof(v2) * (size_t)s->N);
+    v2* xs = vmg_up_x(s, 0, 0, rl, &V);
+    sums[0] = sums[1] = sums[2] = 0.0;
+    for (int y = 0; y < F.h; y++)
+        for (int x = 0; x < F.w; x++) {
+            if (!v_on(s, x, y)) continue;
+            int i = y * F.w + x;
+            v2 z = xs[i], d = V.d[i], o = v_offd(&V, xs, x, y);
+            v2 az = {d.x * z.x + o.x, d.y * z.y + o.y};
+            s->vmz[i] = z;
+            s->vaz[i] = az;
+            sums[0] += dot2(rl[i], z);
+            sums[1] += dot2(z, az);
+            sums[2] += dot2(z, q[i]);
+        }
+    free(xs);
+    free(rl);
+}
+
+/* visc_mgu.comp:28-65: the update of the preconditioned iteration, then the way down on
+ * level 0 from the new residual. r_in / q_in: last iteration's; r_out / q_out: this one's. */
+static void visc_mgu(or_sim* s, const v2* r_in, const v2* q_in, v2* r_out, v2* q_out, double sums[2]) {
+    int W = s->W, H = s->H, N = s->N;
+    sp_solve* K = &s->K;
+    float a = (float)K->v_alpha, b = (float)K->v_beta;
+    v2* rn = (v2*)malloc(sizeof(v2) * (size_t)N);
+    for (int i = 0; i < N; i++) {
+        v2 qn = b == 0.0f ? s->vaz[i] : (v2){s->vaz[i].x + b * q_in[i].x, s->vaz[i].y + b * q_in[i].y};
+        rn[i] = (v2){r_in[i].x - a * qn.x, r_in[i].y - a * qn.y};
+    }
+    sums[0] = sums[1] = 0.0;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            if (!v_on(s, cx, cy)) continue;
+            int i = cidx(s, cx, cy);
+            v2 z = s->vmz[i];
+            v2 p = b == 0.0f ? z : (v2){z.x + b * s->vp[i].x, z.y + b * s->vp[i].y};
+            v2 q = b == 0.0f ? s->vaz[i] : (v2){s->vaz[i].x + b * q_in[i].x, s->vaz[i].y + b * q_in[i].y};
+            v2 r = {r_in[i].x - a * q.x, r_in[i].y - a * q.y};
+            if (v_interior(s, cx, cy)) {
+                if (face_live(s->vel[i].z)) s->vel[i].x += a * p.x;
+                if (face_live(s->vel[i].w)) s->vel[i].y += a * p.y;
+                s->vd[i].x += a * p.x;
+                s->vd[i].y += a * p.y;
+            }
+            s->vp[i] = p;
+            q_out[i] = q;
+            r_out[i] = r;
+            sums[0] += dot2(r, r);
+            sums[1] += dot2(p, q);
+        }
+    vmg_down(s, 0, 0, rn);
+    free(rn);
+}
+
+/* The strain of a search direction p (cell values, zero outside the interior) at every cell
+ * (visc_iter.comp:55-75). */
+static void strain_of_dir(const or_sim* s, const v2* p, v4* out) {
+    int W = s->W;
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 w = s->vw[i], o = {0, 0, 0, 0};
+#define PU(j) (live_u(s, j) ? p[j].x : 0.0f)
+#define PV(j) (live_v(s, j) ? p[j].y : 0.0f)
+            if (w.x > 0.0f) {
+                o.x = w.x * (PU(i + 1) - PU(i));
+                o.y = w.x * (PV(i + W) - PV(i));
+            }
+            if (w.y > 0.0f) {
+                float ua = PU(i), ub = PU(i - W), va = PV(i), vb = PV(i - 1);
+                if (closed_u(s, i) && live_u(s, i - W)) ua = -ub;
+                if (closed_u(s, i - W) && live_u(s, i)) ub = -ua;
+                if (closed_v(s, i) && live_v(s, i - 1)) va = -vb;
+                if (closed_v(s, i - 1) && live_v(s, i)) vb = -va;
+                o.z = w.y * 0.5f * ((ua - ub) + (va - vb));
+            }
+#undef PU
+#undef PV
+            out[i] = o;
+        }
+}
+
+/* visc_iter.comp: with warm != 0 the warm start (x = u* + d, r -= A d; sums r.z, r.r);
+ * otherwise p = z + beta p_old into p_new, q = A p (sums p.q). */
+static void visc_iter(or_sim* s, int warm, const v2* p_old, v2* p_new, double sums[2]) {
+    int W = s->W, H = s->H, N = s->N;
+    float beta = (float)s->K.v_beta;
+    v2* pd = (v2*)calloc((size_t)N, sizeof(v2));
+    v4* st = (v4*)malloc(sizeof(v4) * (size_t)N);
+    for (int cy = 1; cy < H - 1; cy++)
+        for (int cx = 1; cx < W - 1; cx++) {
+            int i = cidx(s, cx, cy);
+            if (warm) {
+                pd[i] = s->vd[i];
+            } else {
+                v4 vw = s->vw[i];
+                v2 z = {s->vr[i].x / vw.z, s->vr[i].y / vw.w};
+                pd[i] = beta == 0.0f ? z : (v2){z.x + beta * p_old[i].x, z.y + beta * p_old[i].y};
+            }
+        }
+    strain_of_dir(s, pd, st);
+    sums[0] = sums[1] = 0.0;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            if (!v_on(s, cx, cy)) continue;
+            int i = cidx(s, cx, cy);
+            if (!warm) p_new[i] = pd[i];
+            if (!v_interior(s, cx, cy)) {
+                if (!warm) s->vq[i] = (v2){0, 0};
+                continue;
+            }
+            v2 g = gts_of(s, st, i);
+            float lx = live_u(s, i) ? 1.0f : 0.0f, ly = live_v(s, i) ? 1.0f : 0.0f;
+            v4 cf = s->coef[i];
+            v2 p = {pd[i].x * lx, pd[i].y * ly};
+            v2 q = {(cf.z * p.x + g.x) * lx, (cf.w * p.y + g.y) * ly};
+            if (warm) {
+                s->vel[i].x += p.x;
+                s->vel[i].y += p.y;
+                v2 r = {s->vr[i].x - q.x, s->vr[i].y - q.y};
+                s->vr[i] = r;
+                v2 z = {r.x / s->vw[i].z, r.y / s->vw[i].w};
+                sums[0] += dot2(r, z);
+                sums[1] += dot2(r, r);
+            } else {
+                s->vq[i] = q;
+                sums[0] += dot2(p, q);
+            }
+        }
+    free(pd);
+    free(st);
+}
+
+/* visc_update.comp:11-36 with this iteration's direction p. */
+static void visc_update(or_sim* s, const v2* p, double sums[2]) {
+    float a = (float)s->K.v_alpha;
+    sums[0] = sums[1] = 0.0;
+    for (int cy = 1; cy < s->H - 1; cy++)
+        for (int cx = 1; cx < s->W - 1; cx++) {
+            if (!v_on(s, cx, cy)) continue;
+            int i = cidx(s, cx, cy);
+            int lu = live_u(s, i), lv = live_v(s, i);
+            if (lu) s->vel[i].x += a * p[i].x;
+            if (lv) s->vel[i].y += a * p[i].y;
+            s->vd[i].x += a * (lu ? p[i].x : 0.0f);
+            s->vd[i].y += a * (lv ? p[i].y : 0.0f);
+            v2 r = {s->vr[i].x - a * s->vq[i].x, s->vr[i].y - a * s->vq[i].y};
+            s->vr[i] = r;
+            v2 z = {r.x / s->vw[i].z, r.y / s->vw[i].w};
+            sums[0] += dot2(r, z);
+            sums[1] += dot2(r, r);
+        }
+}
+
+/* visc_apply.comp:30-65 mode 0: r = -G^T W G u*, p = z = r / diag; sums r.z, r.r. */
+static void visc_start(or_sim* s, double sums[2]) {
+    sums[0] = sums[1] = 0.0;
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < s->W; cx++) {
+            if (!v_on(s, cx, cy)) continue;
+            int i = cidx(s, cx, cy);
+            if (!v_interior(s, cx, cy)) {
+                s->vr[i] = s->vp[i] = (v2){0, 0};
+                continue;
+            }
+            v2 g = gts_of(s, s->vs, i);
+            float lx = live_u(s, i) ? 1.0f : 0.0f, ly = live_v(s, i) ? 1.0f : 0.0f;
+            v2 r = {-g.x * lx, -g.y * ly};
+            v2 z = {r.x / s->vw[i].z, r.y / s->vw[i].w};
+            s->vr[i] = r;
+            s->vp[i] = z;
+            sums[0] += dot2(r, z);
+            sums[1] += dot2(r, r);
+        }
+}
+
+/* record_viscosity_split (mod.rs:2812-2833). */
+static void stage_visc_split(or_sim* s) {
+    int nt = ((s->W + 15) >> 4) * ((s->H + 15) >> 4);
+    memset(s->vtiles, 0, sizeof(uint32_t) * (size_t)nt);
+    visc_setup(s, 1);
+    visc_split(s);
+    visc_diag(s);
+    if (s->P.v_precond)
+        for (int l = 1; l <= s->P.v_mg_levels; l++) vmg_build(s, l);
+}
+
+/* record_viscosity_solve (mod.rs:2867-2927) with the scalar steps of reduce_tail.glsl
+ * (modes 4, 10, 11, 12, 13 / 5, 6, and 7 at the end) in FP64. */
+static void stage_visc_solve(or_sim* s, int restrain) {
+    sp_solve* K = &s->K;
+    const double tol2 = (double)s->P.tol[2] * (double)s->P.tol[2];
+    if (restrain)
+        for (int cy = 0; cy < s->H; cy++)
+            for (int cx = 0; cx < s->W; cx++)
+                if (v_on(s, cx, cy)) {
+                    int i = cidx(s, cx, cy);
+                    s->vs[i] = strain_of_vel(s, i, s->vw[i].x, s->vw[i].y);
+                }
+    double a2[2];
+    visc_start(s, a2);
+    /* mode 4 */
+    K->v_rz = a2[0];
+    K->v_r0 = a2[1];
+    K->v_rr = a2[1];
+    K->v_beta = 0.0;
+    K->v_alpha = 0.0;
+    K->v_iters = 0;
+    K->v_done = (a2[1] <= 1e-30 || a2[0] <= 0.0) ? 1u : 0u;
+    if (s->P.v_warm && !K->v_done) {
+        visc_iter(s, 1, NULL, NULL, a2);
+        /* mode 10 */
+        K->v_rz = a2[0];
+        K->v_rr = a2[1];
+        K->v_beta = 0.0;
+        K->v_done = (a2[1] <= tol2 * K->v_r0 || a2[0] <= 0.0) ? 1u : 0u;
+    }
+    uint32_t k = 0;
+    uint32_t cap = (uint32_t)s->P.v_max_iters;
+    if (s->P.v_precond) {
+        /* The residual ping-pongs between vr and vr2, A p between vq and vq2. */
+        v2 *rb[2] = {s->vr, s->vr2}, *qb[2] = {s->vq, s->vq2};
+        if (!K->v_done) {
+            vmg_down(s, 0, 0, s->vr);
+            visc_coarse(s);
+            double a3[3];
+            visc_mgz(s, s->vr, s->vq, a3);
+            /* mode 11 */
+            K->v_rz = a3[0];
+            K->v_beta = 0.0;
+            K->v_pq = a3[1];
+            K->v_alpha = a3[1] > 0.0 ? a3[0] / a3[1] : 0.0;
+            if (a3[0] <= 0.0 || a3[1] <= 0.0) K->v_done = 1u;
+        }
+        for (; k < cap && !K->v_done; k++) {
+            int c = k & 1;
+            double m[2];
+            visc_mgu(s, rb[c], qb[c], rb[c ^ 1], qb[c ^ 1], m);
+            /* mode 12 */
+            uint32_t was = K->v_done;
+            K->v_iters += 1u - was;
+            uint32_t now = m[0] <= tol2 * K->v_r0;
+            if (!was) K->v_rr = m[0];
+            K->v_pq = m[1];
+            K->v_done = was | now;
+            if (K->v_done) {
+                k++;
+                break;
+            }
+            visc_coarse(s);
+            double a3[3];
+            visc_mgz(s, rb[c ^ 1], qb[c ^ 1], a3);
+            /* mode 13 */
+            double beta = (K->v_done == 0u && K->v_rz > 0.0) ? a3[0] / K->v_rz : 0.0;
+            double pq = a3[1] + 2.0 * beta * a3[2] + beta * beta * K->v_pq;
+            K->v_beta = beta;
+            K->v_alpha = (K->v_done == 0u && pq > 0.0) ? a3[0] / pq : 0.0;
+            K->v_rz = a3[0];
+        }
+    } else {
+        v2 *pb[2] = {s->vp, s->vp2};
+        for (; k < cap && !K->v_done; k++) {
+            int c = k & 1;
+            double m[2];
+            visc_iter(s, 0, pb[c], pb[c ^ 1], m);
+            /* mode 5 */
+            K->v_alpha = (K->v_done == 0u && m[0] > 0.0) ? K->v_rz / m[0] : 0.0;
+            visc_update(s, pb[c ^ 1], m);
+            /* mode 6 */
+            uint32_t was = K->v_done;
+            K->v_iters += 1u - was;
+            uint32_t now = m[1] <= tol2 * K->v_r0;
+            uint32_t done = was | now;
+            K->v_beta = (done == 0u && K->v_rz > 0.0) ? m[0] / K->v_rz : 0.0;
+            K->v_rz = m[0];
+            if (!was) K->v_rr = m[1];
+            K->v_done = done;
+        }
+    }
+    /* mode 7 */
+    if (K->v_iters > K->v_it_max) K->v_it_max = K->v_iters;
+    if (K->v_done == 0u) K->v_unconv += 1u;
+    float rel = K->v_r0 > 0.0 ? (float)sqrt(K->v_rr / K->v_r0) : 0.0f;
+    if (rel > K->v_res_max) K->v_res_max = rel;
+    K->v_iters_last = K->v_iters;
+}
+
+/* rhs_visc.comp:143-168: b = -div(u) + A P1 for the second solve, and its start. */
+static void stage_rhs_visc(or_sim* s) {
+    int W = s->W;
+    float ratio = s->S.dt / s->S.dt_prev;
+    /* P1 is read from prhs.z of the neighbours, which this pass also rewrites (with the
+     * same value): read it from a copy. */
+    float* p1s = (float*)malloc(sizeof(float) * (size_t)s->N);
+    for (int i = 0; i < s->N; i++) p1s[i] = s->prhs[i].z;
+    for (int cy = 0; cy < s->H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            float D = s->mgd[i];
+            if (D <= 0.0f) {
+                s->prhs[i].z = 0.0f;
+                continue;
+            }
+            v4 w = s->mga[i];
+            float p1 = p1s[i];
+            float ap = D * p1;
+            if (s->mgd[i - 1] > 0.0f) ap -= w.x * p1s[i - 1];
+            if (s->mgd[i + 1] > 0.0f) ap -= w.y * p1s[i + 1];
+            if (s->mgd[i - W] > 0.0f) ap -= w.z * p1s[i - W];
+            if (s->mgd[i + W] > 0.0f) ap -= w.w * p1s[i + W];
+            float aL = s->solid[i].x, aB = s->solid[i].y, aR = s->solid[i + 1].x, aT = s->solid[i + W].y;
+            v4 vme = s->vel[i];
+            float div = aR * s->vel[i + 1].x - aL * vme.x + aT * s->vel[i + W].y - aB * vme.y;
+            v4 r = s->prhs[i];
+            s->prhs[i] = (v4){ap - div, r.y, p1, 0.0f};
+            if (s->P.v_p2warm) s->pres[i].x = p1 + s->p2keep[i] * ratio;
+        }
+    free(p1s);
+}
+
+/* project_visc.comp:187-210: subtract the gradient of P2 = P - P1, keep P2. */
+static void stage_project_visc(or_sim* s) {
+    float m = u2f(s->S.umax_grid);
+#define P2(j) (s->mgd[j] > 0.0f ? s->pres[j].x - s->prhs[j].z : 0.0f)
+    for (int cy = 1; cy < s->H; cy++)
+        for (int cx = 1; cx < s->W; cx++) {
+            int i = cidx(s, cx, cy);
+            s->p2keep[i] = P2(i);
+            v4 cf = s->coef[i];
+            v4 me = s->vel[i];
+            int lu = face_live(me.z) && cf.x != 0.0f, lv = face_live(me.w) && cf.y != 0.0f;
+            if (lu || lv) {
+                float pc = P2(i);
+                if (lu) me.x -= cf.x * (pc - P2(i - 1));
+                if (lv) me.y -= cf.y * (pc - P2(i - s->W));
+                s->vel[i] = me;
+            }
+            float mm = 0.0f;
+            if (lu) mm = fabsf(me.x);
+            if (lv) mm = fmaxf_(mm, fabsf(me.y));
+            if (mm > m) m = mm;
+        }
+#undef P2
+    s->S.umax_grid = f2u(m);
+}
+
+/* The viscosity before the first pressure solve (record_substep, mod.rs:2681-2686). */
+static void stage_visc_before(or_sim* s) {
+    if (s->P.visc_mode == 1) {
+        stage_visc_explicit(s);
+    } else if (s->P.visc_mode == 2) {
+        stage_visc_split(s);
+        if (!s->P.v_twice) stage_visc_solve(s, 0);
+    }
+}
+
 /* ---- public entry points -------------------------------------------------------------- */
 
 void or_stage(or_sim* s, int stage, int arg) {
@@ -1650,9 +2405,19 @@ void or_stage(or_sim* s, int stage, int arg) {
         stage_surf_curv(s);
         stage_surf_apply(s);
         break;
-    case SP_STAGE_VISC:
-        if (s->P.visc_mode == 1) stage_visc_explicit(s);
+    case SP_STAGE_VISC: stage_visc_before(s); break;
+    case SP_STAGE_VISC_SPLIT: stage_visc_split(s); break;
+    case SP_STAGE_VISC_SOLVE: stage_visc_solve(s, arg); break;
+    case SP_STAGE_RHS_VISC: stage_rhs_visc(s); break;
+    case SP_STAGE_PRESSURE2: pressure_iterate(s, 1); break;
+    case SP_STAGE_PROJECT_VISC: stage_project_visc(s); break;
+    case SP_STAGE_VISC_VCYCLE: {
+        double a3[3];
+        vmg_down(s, 0, 0, s->vr);
+        visc_coarse(s);
+        visc_mgz(s, s->vr, s->vq, a3);
         break;
+    }
     case SP_STAGE_RHS: stage_rhs(s); break;
     case SP_STAGE_PRESSURE: stage_pressure(s); break;
     case SP_STAGE_PROJECT: stage_project(s); break;
@@ -1676,10 +2441,16 @@ void or_substep(or_sim* s, int gather) {
     stage_surf_smooth(s);
     stage_surf_curv(s);
     stage_surf_apply(s);
-    if (s->P.visc_mode == 1) stage_visc_explicit(s);
+    stage_visc_before(s);
     stage_rhs(s);
     stage_pressure(s);
     stage_project(s);
+    if (s->P.visc_mode == 2 && s->P.v_twice) {
+        stage_visc_solve(s, 1);
+        stage_rhs_visc(s);
+        pressure_iterate(s, 1);
+        stage_project_visc(s);
+    }
 }
 
 or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
@@ -1710,6 +2481,14 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->cgp = (v2*)calloc(N, sizeof(v2));
     s->cgq = (v2*)calloc(N, sizeof(v2));
     s->ptile = (uint32_t*)calloc((size_t)s->lv.tiles, sizeof(uint32_t));
+    v2** f2[] = {&s->vr, &s->vp, &s->vp2, &s->vq, &s->vd, &s->vr2, &s->vq2, &s->vaz};
+    for (size_t k = 0; k < sizeof(f2) / sizeof(f2[0]); k++) *f2[k] = (v2*)calloc(N, sizeof(v2));
+    s->vtiles = (uint32_t*)calloc((size_t)(((W + 15) / 16) * ((H + 15) / 16)), sizeof(uint32_t));
+    s->vtile = (uint32_t*)calloc((size_t)s->lv.tiles, sizeof(uint32_t));
+    s->vma = (v4*)calloc(T, sizeof(v4));
+    s->vmb = (v4*)calloc(T, sizeof(v4));
+    v2** t2[] = {&s->vmd, &s->vmr, &s->vmx, &s->vmz};
+    for (size_t k = 0; k < sizeof(t2) / sizeof(t2[0]); k++) *t2[k] = (v2*)calloc(T, sizeof(v2));
     return s;
 }
 
@@ -1734,6 +2513,12 @@ void or_destroy(or_sim* s) {
     free(s->cgp);
     free(s->cgq);
     free(s->ptile);
+    v2* f2[] = {s->vr, s->vp, s->vp2, s->vq, s->vd, s->vr2, s->vq2, s->vaz, s->vmd, s->vmr, s->vmx, s->vmz};
+    for (size_t k = 0; k < sizeof(f2) / sizeof(f2[0]); k++) free(f2[k]);
+    free(s->vtiles);
+    free(s->vtile);
+    free(s->vma);
+    free(s->vmb);
     free(s);
 }
 
@@ -1755,6 +2540,7 @@ void or_reset(or_sim* s, const sp_particle* p, uint32_t n) {
     s->P.n_slots = n;
     memset(s->pres, 0, sizeof(v2) * (size_t)s->N);
     memset(s->p2keep, 0, sizeof(float) * (size_t)s->N);
+    memset(s->vd, 0, sizeof(v2) * (size_t)s->N);
     memset(&s->S, 0, sizeof(s->S));
 }
 
@@ -1796,6 +2582,23 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_VW: *bytes = sizeof(v4) * N; return s->vw;
     case SP_FIELD_VS: *bytes = sizeof(v4) * N; return s->vs;
     case SP_FIELD_GBASE: *bytes = sizeof(v4) * N; return s->gbase;
+    case SP_FIELD_VR: *bytes = sizeof(v2) * N; return s->vr;
+    case SP_FIELD_VP: *bytes = sizeof(v2) * N; return s->vp;
+    case SP_FIELD_VP2: *bytes = sizeof(v2) * N; return s->vp2;
+    case SP_FIELD_VQ: *bytes = sizeof(v2) * N; return s->vq;
+    case SP_FIELD_VD: *bytes = sizeof(v2) * N; return s->vd;
+    case SP_FIELD_VR2: *bytes = sizeof(v2) * N; return s->vr2;
+    case SP_FIELD_VQ2: *bytes = sizeof(v2) * N; return s->vq2;
+    case SP_FIELD_VAZ: *bytes = sizeof(v2) * N; return s->vaz;
+    case SP_FIELD_VTILES: *bytes = sizeof(uint32_t) * (size_t)(((s->W + 15) / 16) * ((s->H + 15) / 16)); return s->vtiles;
+    case SP_FIELD_VTILE: *bytes = sizeof(uint32_t) * (size_t)s->lv.tiles; return s->vtile;
+    case SP_FIELD_VMA: *bytes = sizeof(v4) * T; return s->vma;
+    case SP_FIELD_VMB: *bytes = sizeof(v4) * T; return s->vmb;
+    case SP_FIELD_VMD: *bytes = sizeof(v2) * T; return s->vmd;
+    case SP_FIELD_VMR: *bytes = sizeof(v2) * T; return s->vmr;
+    case SP_FIELD_VMX: *bytes = sizeof(v2) * T; return s->vmx;
+    case SP_FIELD_VMZ: *bytes = sizeof(v2) * T; return s->vmz;
+    case SP_FIELD_P2KEEP: *bytes = sizeof(float) * N; return s->p2keep;
     default: *bytes = 0; return NULL;
     }
 }

@@ -32,10 +32,14 @@ OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY = 0, -1, -2, -3, -4,
 # Field ids (SP_FIELD_*)
 FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7, MGA=8, MGD=9,
              PRHS=10, PRES=11, STEP=12, SOLVE=13, SPROP=14, AUX=15, GRHO=16, VEL0=17, MGZ=18,
-             MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25)
+             MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
+             VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
+             VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
-             PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12)
+             PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
+             VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
+             VISC_VCYCLE=18)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -66,7 +70,10 @@ class SimParams(C.Structure):
                 ("correction_step_max", C.c_float), ("crowd_jitter", C.c_float),
                 ("tool_crowd_max", C.c_float), ("apic_clamp", C.c_float),
                 ("curvature_max", C.c_float), ("slip_cap", C.c_uint32), ("sealed_vent", C.c_uint32),
-                ("min_substep_share", C.c_float), ("mat_count", C.c_int32),
+                ("min_substep_share", C.c_float), ("viscous_projection", C.c_uint32),
+                ("viscous_warm", C.c_uint32), ("viscosity_warm", C.c_uint32),
+                ("viscosity_precond", C.c_uint32), ("viscosity_mg_levels", C.c_uint32),
+                ("viscosity_mg_sweeps", C.c_uint32), ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
 
@@ -111,7 +118,9 @@ class GParams(C.Structure):
                 ("density_k", C.c_float), ("dt_max", C.c_float), ("crowd_jitter", C.c_float),
                 ("ext_speed", C.c_float), ("wall_margin", C.c_float), ("tol", C.c_float * 4),
                 ("ext", C.c_float * 4), ("corr", C.c_float * 4), ("p_max_iters", C.c_int32),
-                ("v_max_iters", C.c_int32), ("heat_x", C.c_float), ("ln_rigid", C.c_float)]
+                ("v_max_iters", C.c_int32), ("heat_x", C.c_float), ("ln_rigid", C.c_float),
+                ("v_twice", C.c_int32), ("v_p2warm", C.c_int32), ("v_warm", C.c_int32),
+                ("v_precond", C.c_int32), ("v_mg_levels", C.c_int32), ("v_mg_sweeps", C.c_int32)]
 
 
 class LevelsInfo(C.Structure):
@@ -133,7 +142,11 @@ SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<
                         ("bb", "<f8", (2,)), ("rr", "<f8", (2,)), ("done", "<u4", (2,)),
                         ("iters", "<u4", (2,)), ("it_max", "<u4"), ("unconv", "<u4"),
                         ("solves", "<u4"), ("arrive", "<u4"), ("res_max", "<f4"),
-                        ("iters_last", "<u4"), ("pad", "<u4", (2,))])
+                        ("iters_last", "<u4"), ("p2_it_max", "<u4"), ("p2_unconv", "<u4"),
+                        ("v_rz", "<f8"), ("v_alpha", "<f8"), ("v_beta", "<f8"), ("v_r0", "<f8"),
+                        ("v_rr", "<f8"), ("v_pq", "<f8"), ("v_done", "<u4"), ("v_iters", "<u4"),
+                        ("v_it_max", "<u4"), ("v_unconv", "<u4"), ("v_res_max", "<f4"),
+                        ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("pad", "<u4")])
 
 # Exported symbols of include/sprout_hip.h, with their argument types.
 _P = C.c_void_p

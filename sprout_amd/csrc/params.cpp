@@ -76,6 +76,13 @@ void default_params(sp_sim_params* p) {
     p->slip_cap = 1;
     p->sealed_vent = 1;
     p->min_substep_share = 0.02f;
+    // params.rs:433-438
+    p->viscous_projection = 1;
+    p->viscous_warm = 1;
+    p->viscosity_warm = 1;
+    p->viscosity_precond = 1;
+    p->viscosity_mg_levels = 1;
+    p->viscosity_mg_sweeps = 2;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -365,6 +372,20 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.v_max_iters = (int)p.viscosity_max_iters;
     g.heat_x = p.ambient_temp;
     g.ln_rigid = logf(p.rigid_viscosity / (dx * dx));
+    // The implicit viscosity's options: twice_projected (mod.rs:2712-2715), the cg_scale warm
+    // mode (mod.rs:2978-2979), visc_mg_levels / visc_mg_sweeps (mod.rs:2353-2366).
+    const bool implicit = g.visc_mode == 2;
+    g.v_twice = implicit && p.viscous_projection != 0 ? 1 : 0;
+    g.v_p2warm = g.v_twice && p.viscous_warm != 0 ? 1 : 0;
+    g.v_warm = p.viscosity_warm != 0 ? 1 : 0;
+    g.v_precond = p.viscosity_precond != 0 ? 1 : 0;
+    {
+        sp_levels_info L;
+        levels(W, H, L);
+        int want = std::max((int)p.viscosity_mg_levels, 1);
+        g.v_mg_levels = std::min(want, L.tail);
+    }
+    g.v_mg_sweeps = (int)std::min(std::max(p.viscosity_mg_sweeps, 1u), 2u);
     return SP_OK;
 }
 
