@@ -39,6 +39,13 @@ struct Ctx {
     double4* part_sums;   // per-block partial sums of a reduction
     const int4* lev;      // per level: offset, width, height, tile offset
     int n_levels, tail;
+    // implicit viscosity (per cell of the grid; the multigrid per cell of every level)
+    float2 *vr, *vp, *vp2, *vq, *vd, *vr2, *vq2, *vaz;
+    uint32_t* vtiles;     // active viscosity tiles of the grid
+    uint32_t* vtile;      // tile flags of the viscosity multigrid, per level
+    float4 *vma, *vmb;
+    float2 *vmd, *vmr, *vmx, *vmz;
+    double4* vpart;       // per-tile partial sums of the viscosity solve
 };
 
 SP_DEV float fmaxf_(float a, float b) { return a > b ? a : b; }

@@ -1,6 +1,6 @@
 // grid.hip — the per-cell passes of a substep on gfx950: time step, accumulator clear,
-// grid preparation, face coefficients, surface tension, explicit viscosity, the pressure
-// right-hand side and the projection.
+// grid preparation, face coefficients, surface tension, the pressure right-hand side and
+// the projection (the viscosity passes are in visc.hip).
 //
 // Every pass is one work-item per cell in 16 x 16 work-groups (four wave64s), the tile the
 // pressure hierarchy skips by. They are stencil work with no contraction: HBM-bound, each
@@ -422,6 +422,8 @@ __global__ __launch_bounds__(256) void k_surf_apply(Ctx c) {
     c.pjump[i] = pj;
 }
 
+// Face states of the cell's u and v faces (vel.zw: live, or the FACE_EMPTY / FACE_CLOSED
+// sentinels), used by the viscosity kernels.
 namespace {
 SP_DEV bool live_u(const Ctx& c, int i) { return face_live(c.vel[i].z); }
 SP_DEV bool live_v(const Ctx& c, int i) { return face_live(c.vel[i].w); }
@@ -430,78 +432,6 @@ SP_DEV bool closed_v(const Ctx& c, int i) { return face_closed(c.vel[i].w); }
 SP_DEV float vel_u(const Ctx& c, int i) { return live_u(c, i) ? c.vel[i].x : 0.0f; }
 SP_DEV float vel_v(const Ctx& c, int i) { return live_v(c, i) ? c.vel[i].y : 0.0f; }
 }  // namespace
-
-// visc_setup.comp:14-58: strain weights of the cell and its lower-left node, and the
-// strain of the current velocities with them (visc.glsl:47-63).
-__global__ __launch_bounds__(256) void k_visc_setup(Ctx c) {
-    int cx, cy, i;
-    if (!cell_of_thread(c, cx, cy, i)) return;
-    const int W = c.W, H = c.H;
-    const float dt = c.S->dt;
-    float wc = 0.0f, wn = 0.0f;
-    if (cx >= 1 && cy >= 1 && cx < W - 1 && cy < H - 1) {
-        float4 me = c.cell[i];
-        auto ok_u = [&](int j) { return live_u(c, j) || closed_u(c, j); };
-        auto ok_v = [&](int j) { return live_v(c, j) || closed_v(c, j); };
-        if (me.w > 0.0f && ok_u(i) && ok_u(i + 1) && ok_v(i) && ok_v(i + W)) wc = 2.0f * dt * me.z;
-        int cs4[4] = {i - W - 1, i - W, i - 1, i};
-        int nliq = 0;
-        bool any_air = false;
-        float mu = 0.0f;
-        for (int k = 0; k < 4; k++) {
-            float4 q = c.cell[cs4[k]];
-            if (q.w > 0.0f) {
-                nliq++;
-                mu += q.z;
-            } else if (c.solid[cs4[k]].w > 0.02f) {
-                any_air = true;
-            }
-        }
-        if (nliq > 0 && !any_air && ok_u(i) && ok_u(i - W) && ok_v(i) && ok_v(i - 1))
-            wn = 4.0f * dt * (mu / (float)nliq) * ((float)nliq / 4.0f);
-    }
-    float4 vw = c.vw[i];
-    vw.x = wc;
-    vw.y = wn;
-    c.vw[i] = vw;
-    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (wc > 0.0f) {
-        o.x = wc * (vel_u(c, i + 1) - vel_u(c, i));
-        o.y = wc * (vel_v(c, i + W) - vel_v(c, i));
-    }
-    if (wn > 0.0f) {
-        float ua = vel_u(c, i), ub = vel_u(c, i - W), va = vel_v(c, i), vb = vel_v(c, i - 1);
-        if (closed_u(c, i) && live_u(c, i - W)) ua = -ub;
-        if (closed_u(c, i - W) && live_u(c, i)) ub = -ua;
-        if (closed_v(c, i) && live_v(c, i - 1)) va = -vb;
-        if (closed_v(c, i - 1) && live_v(c, i)) vb = -va;
-        o.z = wn * 0.5f * ((ua - ub) + (va - vb));
-    }
-    c.vs[i] = o;
-}
-
-// visc_apply.comp mode 2 (visc_apply.comp:16-48): the explicit viscous step.
-__global__ __launch_bounds__(256) void k_visc_explicit(Ctx c) {
-    int cx, cy, i;
-    if (!cell_of_thread(c, cx, cy, i)) return;
-    const int W = c.W;
-    if (!(cx >= 1 && cy >= 1 && cx < W - 1 && cy < c.H - 1)) return;
-    float4 v = c.vel[i];
-    bool lu = face_live(v.z), lv = face_live(v.w);
-    float gx = 0.0f, gy = 0.0f;
-    if (lu) {
-        float c1 = closed_u(c, i - W) ? 1.0f : 0.5f, c2 = closed_u(c, i + W) ? -1.0f : -0.5f;
-        gx = c.vs[i - 1].x - c.vs[i].x + c.vs[i].z * c1 + c.vs[i + W].z * c2;
-    }
-    if (lv) {
-        float d1 = closed_v(c, i - 1) ? 1.0f : 0.5f, d2 = closed_v(c, i + 1) ? -1.0f : -0.5f;
-        gy = c.vs[i - W].y - c.vs[i].y + c.vs[i].z * d1 + c.vs[i + 1].z * d2;
-    }
-    float4 cf = c.coef[i];
-    if (lu) v.x -= gx / cf.z;
-    if (lv) v.y -= gy / cf.w;
-    c.vel[i] = v;
-}
 
 // rhs.comp:19-93: the level-0 matrix, both right-hand sides, and the tile flags.
 __global__ __launch_bounds__(256) void k_rhs(Ctx c) {

@@ -56,11 +56,11 @@ SP_DEV double4 group_sum4(double4 v, double4* lds) {
 SP_DEV void store_part(double4* p, double4 v) { *p = v; }
 
 // Sum of parts[base .. base + n) by one 256 work-item group in a fixed order.
-SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) {
+SP_DEV double4 total_of(const double4* parts, int base, int n, double4* lds) {
     const int t = tid2();
     double4 s = make_double4(0, 0, 0, 0);
     for (int k = t; k < n; k += 256) {
-        double4 v = c.part_sums[base + k];
+        double4 v = parts[base + k];
         s.x += v.x;
         s.y += v.y;
         s.z += v.z;
@@ -68,8 +68,11 @@ SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) {
     }
     return group_sum4(s, lds);
 }
+SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) { return total_of(c.part_sums, base, n, lds); }
 
-enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2 };
+// Modes 0, 1, 2 of reduce_tail.glsl, and 8: the start of the second solve (after the
+// viscosity), where the density-correction potential counts as solved.
+enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2, RED_START2 = 8 };
 
 // The scalar step of the pressure CG (reduce_tail.glsl:59-116 modes 0, 1, 2), FP64.
 // parts [0, n): r.z (xy), r.r (zw) of the last V-cycle; [n, 2n): b.b (xy) or p.q (xy).
@@ -77,7 +80,7 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
     sp_solve* K = c.K;
     const sp_gparams& P = *c.P;
     const double tx = (double)P.tol[0] * (double)P.tol[0], ty = (double)P.tol[1] * (double)P.tol[1];
-    if (mode == RED_START) {
+    if (mode == RED_START || mode == RED_START2) {
         double4 a = total(c, 0, n, lds);
         double4 b = total(c, n, n, lds);
         if (tid2() == 0) {
@@ -91,7 +94,7 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
             K->alpha[0] = K->alpha[1] = 0.0;
             K->iters[0] = K->iters[1] = 0u;
             K->done[0] = a.z <= tx * b.x ? 1u : 0u;
-            K->done[1] = a.w <= ty * b.y ? 1u : 0u;
+            K->done[1] = (mode == RED_START2 || a.w <= ty * b.y) ? 1u : 0u;
             K->solves += 1u;
         }
     } else if (mode == RED_ALPHA) {
@@ -191,7 +194,9 @@ __global__ __launch_bounds__(1024) void k_mg_tail_build(Ctx c) {
 
 // ---- Krylov kernels on level 0 ---------------------------------------------------------
 
-// cg_scale.comp mode 0: rescale the warm start to this substep's length.
+// cg_scale.comp:14-27: rescale the warm start to this substep's length. With a second solve
+// after the viscosity that warm-starts from its own part (mode 1), the first starts from
+// P1 = P - P2 and P2 is kept; otherwise P2 is forgotten.
 __global__ __launch_bounds__(256) void k_cg_scale(Ctx c) {
     if (c.ptile[tile_slot()] == 0u) return;
     const Lv F = level(c, 0);
@@ -200,7 +205,8 @@ __global__ __launch_bounds__(256) void k_cg_scale(Ctx c) {
     int i = at(F, x, y);
     if (c.mgd[i] > 0.0f) {
         float p = c.pres[i].x;
-        c.p2keep[i] = 0.0f;
+        if (c.P->v_p2warm) p -= c.p2keep[i];
+        else c.p2keep[i] = 0.0f;
         c.pres[i].x = p * (c.S->dt / c.S->dt_prev);
     } else {
         c.pres[i] = make_float2(0.0f, 0.0f);
@@ -739,14 +745,21 @@ __global__ __launch_bounds__(256) void k_cg_reduce(Ctx c, int mode, int nblocks,
     reduce_scalars(c, mode, nblocks, lds);
 }
 
-// Solve statistics at the end of a solve (reduce_tail.glsl:105-116, mode 3).
-__global__ void k_cg_end(Ctx c) {
+// Solve statistics at the end of a solve (reduce_tail.glsl:105-116, modes 3 and 9: the
+// second solve keeps its own iteration count).
+__global__ void k_cg_end(Ctx c, int second) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sp_solve* K = c.K;
     uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];
-    if (it > K->it_max) K->it_max = it;
-    if (K->done[0] == 0u) K->unconv += 1u;
+    if (second) {
+        if (K->iters[0] > K->p2_it_max) K->p2_it_max = K->iters[0];
+        if (K->done[0] == 0u) K->p2_unconv += 1u;
+        K->p2_iters_last = K->iters[0];
+    } else {
+        if (it > K->it_max) K->it_max = it;
+        if (K->done[0] == 0u) K->unconv += 1u;
+        K->iters_last = it;
+    }
     float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
     if (rel > K->res_max) K->res_max = rel;
-    K->iters_last = it;
 }

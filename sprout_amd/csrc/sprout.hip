@@ -20,6 +20,7 @@
 #include "host.h"
 #include "particles.hip"
 #include "solve.hip"
+#include "visc.hip"
 
 namespace {
 
@@ -48,12 +49,20 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 
 }  // namespace
 
-// Instantiated graphs of one (recorded iterations, viscosity mode) key: grid passes,
-// pressure solve, projection.
+// Recorded iterations of one substep: first pressure solve, second pressure solve (after
+// the viscosity), viscosity solve (record_substep, mod.rs:2615-2623).
+struct Recs {
+    uint32_t p = 0, p2 = 0, v = 0;
+    bool operator==(const Recs& o) const { return p == o.p && p2 == o.p2 && v == o.v; }
+};
+
+// Instantiated graphs of one (recorded iterations, schedule) key: grid passes (with the
+// viscosity before the first solve), pressure solve, projection, and the viscosity solve
+// with the second projection.
 struct Graphs {
-    uint32_t iters = 0;
-    int visc = -1;
-    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    Recs recs;
+    uint32_t sched = 0xffffffffu;
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 struct sp_sim {
@@ -100,20 +109,20 @@ struct sp_sim {
     double time_base = 0.0;
     double sim_time = 0.0;
     float dt_estimate = 1.0f / 240.0f;
-    uint32_t p_rec = 60;
+    Recs rec{60, 60, 200};
     sp_solver_stats stats{};
     uint64_t substeps_base = 0;
-    uint32_t last_unconv = 0;
+    uint32_t last_unconv = 0, last_p2_unconv = 0, last_v_unconv = 0;
 
     // timing: per substep of a batch (up to TIMED_SUBSTEPS), events at the phase
     // boundaries and tightly around the particle kernel
     static constexpr int TIMED_SUBSTEPS = 64;
-    static constexpr int EV_PER = 7;
+    static constexpr int EV_PER = 8;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     uint32_t timed = 0;  // substeps of the current batch with events recorded
     uint32_t timed_last = 0;
-    float timings[6] = {};
+    float timings[8] = {};
     float particle_ms = 0.0f;
     bool use_graphs = true;
     std::vector<Graphs> graphs;
@@ -159,16 +168,20 @@ void launch_mg_build(sp_sim* s) {
     if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build, dim3(1), dim3(1024), 0, s->stream, c);
 }
 
-// record_pressure + record_cg (mod.rs:2956-3010), the first solve. The level-0 residual
-// alternates between mgr and r2 from one iteration to the next (k_update_down0).
-void launch_pressure(sp_sim* s, uint32_t iters) {
+// record_pressure + record_cg (mod.rs:2956-3010). The first solve builds the hierarchy and
+// rescales the warm start; the second (after the viscosity, `second`) reuses both and
+// starts from what rhs_visc.comp left. The level-0 residual alternates between mgr and r2
+// from one iteration to the next (k_update_down0).
+void launch_pressure(sp_sim* s, uint32_t iters, bool second = false) {
     const Ctx& c = s->c;
     dim3 g0 = grid_groups(s->W, s->H);
-    launch_mg_build(s);
-    hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, s->stream, c);
+    if (!second) {
+        launch_mg_build(s);
+        hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, s->stream, c);
+    }
     hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, s->stream, c, s->nb0);
     float2* rb[2] = {c.mgr, s->d_r2};
-    launch_vcycle(s, 0, 0, rb[0]);
+    launch_vcycle(s, 0, second ? (int)RED_START2 : (int)RED_START, rb[0]);
     float2* bufs[2] = {c.cgp, c.cgp2};
     for (uint32_t k = 0; k < iters; k++) {
         const float2* p_old = bufs[k & 1];
@@ -179,20 +192,127 @@ void launch_pressure(sp_sim* s, uint32_t iters) {
                            (const float2*)p_new);
         launch_vcycle(s, 1, 2, rb[(k + 1) & 1], true);
     }
-    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s->stream, c);
+    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s->stream, c, second ? 1 : 0);
 }
 
-void launch_grid_passes(sp_sim* s) {
+// ---- the implicit viscosity (record_viscosity_split / _solve, mod.rs:2812-2927) ----------
+
+// Weights and active tiles, the explicit step outside them, the Jacobi diagonal, the warm
+// start, and the multigrid's matrices.
+void launch_visc_split(sp_sim* s) {
+    const Ctx& c = s->c;
+    dim3 g0 = grid_groups(s->W, s->H);
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::max((s->nb0 / 4 + 255) / 256, 1)), dim3(256), 0, s->stream, c.vtiles,
+                       (size_t)s->nb0, 0u);
+    hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c, 1);
+    hipLaunchKernelGGL(k_visc_split, g0, TILE, 0, s->stream, c);
+    hipLaunchKernelGGL(k_visc_diag, g0, TILE, 0, s->stream, c);
+    if (s->gp.v_precond)
+        for (int l = 1; l <= s->gp.v_mg_levels; l++)
+            hipLaunchKernelGGL(k_vmg_build, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l);
+}
+
+void launch_visc_reduce(sp_sim* s, int mode) {
+    hipLaunchKernelGGL(k_visc_reduce, dim3(1), dim3(256), 0, s->stream, s->c, mode, s->nb0);
+}
+
+// The coarse levels of a viscosity V-cycle, down and up (record_visc_coarse, mod.rs:2838-2853).
+template <int NU>
+void launch_visc_coarse(sp_sim* s) {
+    const Ctx& c = s->c;
+    const int n = s->gp.v_mg_levels;
+    for (int l = 1; l <= n; l++)
+        hipLaunchKernelGGL(k_vmg_down<NU>, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l,
+                           l == n ? 1 : 0);
+    for (int l = n; l >= 1; l--)
+        hipLaunchKernelGGL(k_vmg_up<NU>, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l,
+                           l == n ? 1 : 0);
+}
+
+// Multigrid-preconditioned CG: z of the starting residual and the first step length, then
+// per iteration the update with the way down on level 0, the coarse levels and the way up
+// with A z and both step lengths. Residual and A p alternate between two buffers.
+template <int NU>
+void launch_visc_mg(sp_sim* s, uint32_t iters) {
+    const Ctx& c = s->c;
+    dim3 g0 = grid_groups(s->W, s->H);
+    hipLaunchKernelGGL(k_vmg_down<NU>, g0, TILE, 0, s->stream, c, 0, 0);
+    launch_visc_coarse<NU>(s);
+    hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
+    launch_visc_reduce(s, VRED_MG_START);
+    float2* rb[2] = {c.vr, c.vr2};
+    float2* qb[2] = {c.vq, c.vq2};
+    for (uint32_t k = 0; k < iters; k++) {
+        const int a = k & 1, b = a ^ 1;
+        hipLaunchKernelGGL(k_visc_mgu<NU>, g0, TILE, 0, s->stream, c, (const float2*)rb[a], (const float2*)qb[a], rb[b], qb[b]);
+        launch_visc_reduce(s, VRED_MG_UPDATE);
+        launch_visc_coarse<NU>(s);
+        hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, s->stream, c, (const float2*)rb[b], (const float2*)qb[b]);
+        launch_visc_reduce(s, VRED_MG_ITER);
+    }
+}
+
+// The implicit solve on the active tiles. `restrain`: the first pressure solve changed the
+// velocity since the split, so the strain of u* is taken again.
+void launch_visc_solve(sp_sim* s, uint32_t iters, bool restrain) {
+    const Ctx& c = s->c;
+    dim3 g0 = grid_groups(s->W, s->H);
+    if (restrain) hipLaunchKernelGGL(k_visc_strain, g0, TILE, 0, s->stream, c);
+    hipLaunchKernelGGL(k_visc_start, g0, TILE, 0, s->stream, c);
+    launch_visc_reduce(s, VRED_START);
+    if (s->gp.v_warm) {
+        hipLaunchKernelGGL(k_visc_warm, g0, TILE, 0, s->stream, c);
+        launch_visc_reduce(s, VRED_WARM);
+    }
+    if (s->gp.v_precond) {
+        if (s->gp.v_mg_sweeps == 1) launch_visc_mg<1>(s, iters);
+        else launch_visc_mg<2>(s, iters);
+    } else {
+        float2* pb[2] = {c.vp, c.vp2};
+        for (uint32_t k = 0; k < iters; k++) {
+            const int a = k & 1, b = a ^ 1;
+            hipLaunchKernelGGL(k_visc_iter, g0, TILE, 0, s->stream, c, (const float2*)pb[a], pb[b]);
+            launch_visc_reduce(s, VRED_ALPHA);
+            hipLaunchKernelGGL(k_visc_update, g0, TILE, 0, s->stream, c, (const float2*)pb[b]);
+            launch_visc_reduce(s, VRED_BETA);
+        }
+    }
+    hipLaunchKernelGGL(k_visc_end, dim3(1), dim3(64), 0, s->stream, c);
+}
+
+// The viscosity before the first pressure solve (record_substep, mod.rs:2681-2686).
+void launch_visc_before(sp_sim* s, const Recs& r) {
+    const Ctx& c = s->c;
+    dim3 g0 = grid_groups(s->W, s->H);
+    if (s->gp.visc_mode == 1) {
+        hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c, 0);
+        hipLaunchKernelGGL(k_visc_explicit, g0, TILE, 0, s->stream, c);
+    } else if (s->gp.visc_mode == 2) {
+        launch_visc_split(s);
+        if (!s->gp.v_twice) launch_visc_solve(s, r.v, false);
+    }
+}
+
+// After the first projection: the viscosity solve and the second projection
+// (mod.rs:2690-2698).
+void launch_visc_after(sp_sim* s, const Recs& r) {
+    if (!(s->gp.visc_mode == 2 && s->gp.v_twice)) return;
+    const Ctx& c = s->c;
+    dim3 g0 = grid_groups(s->W, s->H);
+    launch_visc_solve(s, r.v, true);
+    hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, c);
+    launch_pressure(s, r.p2, true);
+    hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, c);
+}
+
+void launch_grid_passes(sp_sim* s, const Recs& r) {
     const Ctx& c = s->c;
     dim3 g0 = grid_groups(s->W, s->H);
     hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, s->stream, c);
     hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, c);
     hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, s->stream, c);
     hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, s->stream, c);
-    if (s->gp.visc_mode == 1) {
-        hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c);
-        hipLaunchKernelGGL(k_visc_explicit, g0, TILE, 0, s->stream, c);
-    }
+    launch_visc_before(s, r);
     hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, c);
 }
 
@@ -240,12 +360,13 @@ void destroy_graphs(sp_sim* s) {
     s->graphs.clear();
 }
 
-hipGraphExec_t capture(sp_sim* s, int part, uint32_t p_iters) {
+hipGraphExec_t capture(sp_sim* s, int part, const Recs& r) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    if (part == 0) launch_grid_passes(s);
-    else if (part == 1) launch_pressure(s, p_iters);
-    else hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, s->c);
+    if (part == 0) launch_grid_passes(s, r);
+    else if (part == 1) launch_pressure(s, r.p);
+    else if (part == 2) hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, s->c);
+    else launch_visc_after(s, r);
     hipError_t le = hipGetLastError();
     HIPCHK(hipStreamEndCapture(s->stream, &g));
     HIPCHK(le);
@@ -255,23 +376,32 @@ hipGraphExec_t capture(sp_sim* s, int part, uint32_t p_iters) {
     return e;
 }
 
-Graphs& graphs_for(sp_sim* s, uint32_t p_iters) {
+// What the host decides about a substep's launches from the tables (the parameters the
+// kernels read themselves stay out of the key).
+uint32_t schedule_key(const sp_sim* s) {
+    const sp_gparams& g = s->gp;
+    return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
+           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10;
+}
+
+Graphs& graphs_for(sp_sim* s, const Recs& r) {
+    const uint32_t key = schedule_key(s);
     for (auto& g : s->graphs)
-        if (g.iters == p_iters && g.visc == s->gp.visc_mode) return g;
+        if (g.recs == r && g.sched == key) return g;
     if (s->graphs.size() >= 12) destroy_graphs(s);
     s->graphs.emplace_back();
     Graphs& g = s->graphs.back();
-    g.iters = p_iters;
-    g.visc = s->gp.visc_mode;
-    for (int k = 0; k < 3; k++) g.exec[k] = capture(s, k, p_iters);
+    g.recs = r;
+    g.sched = key;
+    for (int k = 0; k < 4; k++) g.exec[k] = capture(s, k, r);
     return g;
 }
 
-void launch_substep(sp_sim* s, int gather, uint32_t p_iters) {
+void launch_substep(sp_sim* s, int gather, const Recs& r) {
     const Ctx& c = s->c;
     hipEvent_t* e = nullptr;
     if (s->timing && s->timed < (uint32_t)sp_sim::TIMED_SUBSTEPS) e = &s->ev[(size_t)s->timed++ * sp_sim::EV_PER];
-    Graphs* g = s->use_graphs ? &graphs_for(s, p_iters) : nullptr;
+    Graphs* g = s->use_graphs ? &graphs_for(s, r) : nullptr;
     if (e) HIPCHK(hipEventRecord(e[0], s->stream));
     hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c);
     launch_clear_acc(s);
@@ -280,14 +410,17 @@ void launch_substep(sp_sim* s, int gather, uint32_t p_iters) {
     if (e) HIPCHK(hipEventRecord(e[6], s->stream));
     if (e) HIPCHK(hipEventRecord(e[1], s->stream));
     if (g) HIPCHK(hipGraphLaunch(g->exec[0], s->stream));
-    else launch_grid_passes(s);
+    else launch_grid_passes(s, r);
     if (e) HIPCHK(hipEventRecord(e[2], s->stream));
     if (g) HIPCHK(hipGraphLaunch(g->exec[1], s->stream));
-    else launch_pressure(s, p_iters);
+    else launch_pressure(s, r.p);
     if (e) HIPCHK(hipEventRecord(e[3], s->stream));
     if (g) HIPCHK(hipGraphLaunch(g->exec[2], s->stream));
     else hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, c);
     if (e) HIPCHK(hipEventRecord(e[4], s->stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[3], s->stream));
+    else launch_visc_after(s, r);
+    if (e) HIPCHK(hipEventRecord(e[7], s->stream));
     HIPCHK(hipGetLastError());
 }
 
@@ -299,10 +432,6 @@ int upload_tables(sp_sim* s) {
     if (rc != SP_OK) {
         set_err(s, "%s", why.c_str());
         return rc;
-    }
-    if (s->gp.visc_mode == 2) {
-        set_err(s, "the tank needs the implicit viscosity solve (nu dt / dx^2 > 0.05), which this build does not carry yet");
-        return SP_E_UNSUPPORTED;
     }
     HIPCHK(hipMemcpyAsync(s->d_P, &s->gp, sizeof(sp_gparams), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemcpyAsync(s->d_mats, s->gm, sizeof(s->gm), hipMemcpyHostToDevice, s->stream));
@@ -328,21 +457,34 @@ void readback(sp_sim* s) {
     st.umax = S.umax_seen;
     st.umax_grid = __builtin_bit_cast(float, S.umax_grid);
     st.umax_part = __builtin_bit_cast(float, S.umax_part);
-    st.pressure_iters_max = std::max(st.pressure_iters_max, K.it_max);
-    uint32_t new_unconv = K.unconv - s->last_unconv;
+    st.pressure_iters_max = std::max(st.pressure_iters_max, std::max(K.it_max, K.p2_it_max));
+    st.viscosity_iters_max = std::max(st.viscosity_iters_max, K.v_it_max);
+    const uint32_t p_un = K.unconv - s->last_unconv, p2_un = K.p2_unconv - s->last_p2_unconv,
+                   v_un = K.v_unconv - s->last_v_unconv;
     s->last_unconv = K.unconv;
-    st.pressure_unconverged += new_unconv;
+    s->last_p2_unconv = K.p2_unconv;
+    s->last_v_unconv = K.v_unconv;
+    st.pressure_unconverged += p_un + p2_un;
+    st.viscosity_unconverged += v_un;
     st.pressure_residual_max = std::max(st.pressure_residual_max, K.res_max);
+    st.viscosity_residual_max = std::max(st.viscosity_residual_max, K.v_res_max);
     st.pressure_iters_last = K.iters_last;
     if (st.dt_cfl > 0.0f) s->dt_estimate = st.dt_cfl;
-    // Adapt the recorded iterations (mod.rs:3100-3115).
-    uint32_t maxi = s->params.pressure_max_iters;
-    uint32_t used = K.it_max;
-    if (new_unconv > 0) s->p_rec = std::min(s->p_rec + s->p_rec / 2 + 2, maxi);
-    else s->p_rec = std::min(std::max(used + (uint32_t)(used * 0.25f) + 2, 2u), maxi);
-    // the per-batch maximum starts again
+    // Adapt the recorded iterations (mod.rs:3100-3115): a little more than the batch needed,
+    // half as much again when a solve ran out.
+    auto adapt = [](uint32_t& rec, uint32_t used, uint32_t unconverged, uint32_t maxi, float margin) {
+        if (unconverged > 0) rec = std::min(rec + rec / 2 + 2, maxi);
+        else rec = std::min(std::max(used + (uint32_t)((float)used * margin) + 2, 2u), maxi);
+    };
+    adapt(s->rec.p, K.it_max, p_un, s->params.pressure_max_iters, 0.25f);
+    if (s->gp.visc_mode == 2 && s->gp.v_twice) adapt(s->rec.p2, K.p2_it_max, p2_un, s->params.pressure_max_iters, 0.25f);
+    else s->rec.p2 = s->rec.p;
+    if (s->gp.visc_mode == 2) adapt(s->rec.v, K.v_it_max, v_un, s->params.viscosity_max_iters, 0.5f);
+    // the per-batch maxima start again
     uint32_t zero = 0;
     HIPCHK(hipMemcpyAsync(&s->c.K->it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(&s->c.K->p2_it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(&s->c.K->v_it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
 }
 
@@ -393,7 +535,7 @@ int record(sp_sim* s, uint32_t substeps, float target) {
             s->since_sort = 0;
         }
         s->since_sort++;
-        launch_substep(s, s->grid_valid ? 1 : 0, s->p_rec);
+        launch_substep(s, s->grid_valid ? 1 : 0, s->rec);
         s->grid_valid = true;
     }
     return SP_OK;
@@ -402,19 +544,21 @@ int record(sp_sim* s, uint32_t substeps, float target) {
 int wait(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->timing && s->timed > 0) {
-        // averages per substep: [0] total, [1] step + clear + particle pass, [2] grid passes,
-        // [3] pressure solve, [4] projection; [5] below is the iteration count
-        double acc[5] = {0, 0, 0, 0, 0};
+        // averages per substep: [0] total, [1] step + clear + particle pass, [2] grid passes
+        // (with the viscosity before the first solve), [3] pressure solve, [4] projection,
+        // [5] viscosity solve and second projection; [6], [7] below are iteration counts
+        double acc[6] = {0, 0, 0, 0, 0, 0};
         for (uint32_t k = 0; k < s->timed; k++) {
             hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
+            const int from[5] = {0, 1, 2, 3, 4}, to[5] = {1, 2, 3, 4, 7};
             float t;
-            for (int q = 0; q < 4; q++) {
-                HIPCHK(hipEventElapsedTime(&t, e[q], e[q + 1]));
+            for (int q = 0; q < 5; q++) {
+                HIPCHK(hipEventElapsedTime(&t, e[from[q]], e[to[q]]));
                 acc[q + 1] += t;
                 acc[0] += t;
             }
         }
-        for (int q = 0; q < 5; q++) s->timings[q] = (float)(acc[q] / s->timed);
+        for (int q = 0; q < 6; q++) s->timings[q] = (float)(acc[q] / s->timed);
         double kp = 0;
         for (uint32_t k = 0; k < s->timed; k++) {
             hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
@@ -427,7 +571,8 @@ int wait(sp_sim* s) {
         s->timed = 0;
     }
     readback(s);
-    s->timings[5] = (float)s->stats.pressure_iters_last;
+    s->timings[6] = (float)s->stats.pressure_iters_last;
+    s->timings[7] = (float)s->stats.viscosity_iters_max;
     return SP_OK;
 }
 
@@ -584,6 +729,23 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         s->d_r2 = dalloc<float2>(N, o);
         c.ptile = dalloc<uint32_t>((size_t)s->lv.tiles, o);
         c.part_sums = dalloc<double4>((size_t)2 * s->nb0, o);
+        c.vr = dalloc<float2>(N, o);
+        c.vp = dalloc<float2>(N, o);
+        c.vp2 = dalloc<float2>(N, o);
+        c.vq = dalloc<float2>(N, o);
+        c.vd = dalloc<float2>(N, o);
+        c.vr2 = dalloc<float2>(N, o);
+        c.vq2 = dalloc<float2>(N, o);
+        c.vaz = dalloc<float2>(N, o);
+        c.vtiles = dalloc<uint32_t>((size_t)s->nb0, o);
+        c.vtile = dalloc<uint32_t>((size_t)s->lv.tiles, o);
+        c.vma = dalloc<float4>(T, o);
+        c.vmb = dalloc<float4>(T, o);
+        c.vmd = dalloc<float2>(T, o);
+        c.vmr = dalloc<float2>(T, o);
+        c.vmx = dalloc<float2>(T, o);
+        c.vmz = dalloc<float2>(T, o);
+        c.vpart = dalloc<double4>((size_t)s->nb0, o);
         int4* lev = dalloc<int4>(SP_MAX_LEVELS, o);
         int4 h_lev[SP_MAX_LEVELS] = {};
         for (int l = 0; l < s->lv.n; l++) h_lev[l] = make_int4(s->lv.offs[l], s->lv.dims[l][0], s->lv.dims[l][1], s->lv.tile_offs[l]);
@@ -651,7 +813,7 @@ int sp_set_params(sp_sim* s, const sp_sim_params* p) {
         s->params = *p;
         s->params_set = true;
         s->tables_dirty = true;
-        s->p_rec = p->pressure_max_iters;
+        s->rec = Recs{p->pressure_max_iters, p->pressure_max_iters, std::max(p->viscosity_max_iters, 1u)};
         // the tank's layout follows the table (diluents may have changed)
         std::vector<int> rows = sp::closure(s->params, s->tank.rows);
         sp::Tank t;
@@ -700,6 +862,7 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
             HIPCHK(hipMemcpyAsync(s->c.part, ps.data(), sizeof(sp_particle) * ps.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemsetAsync(s->c.pres, 0, sizeof(float2) * (size_t)s->N, s->stream));
         HIPCHK(hipMemsetAsync(s->c.p2keep, 0, sizeof(float) * (size_t)s->N, s->stream));
+        HIPCHK(hipMemsetAsync(s->c.vd, 0, sizeof(float2) * (size_t)s->N, s->stream));
         HIPCHK(hipMemsetAsync(s->c.S, 0, sizeof(sp_stepbuf), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->pending_spawn.clear();
@@ -707,7 +870,9 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         s->substeps_base = s->stats.substeps;
         s->grid_valid = false;
         s->dt_estimate = 1.0f / 240.0f;
-        s->p_rec = s->params.pressure_max_iters;
+        // a new scene starts from no pressure and no idea how many iterations its solves
+        // need: record the most until read-backs say otherwise (mod.rs:2478-2483)
+        s->rec = Recs{s->params.pressure_max_iters, s->params.pressure_max_iters, std::max(s->params.viscosity_max_iters, 1u)};
         // a sort before the first substep puts neighbours next to each other
         s->since_sort = s->sort_interval;
         return SP_OK;
@@ -907,10 +1072,10 @@ int sp_sort(sp_sim* s) {
     });
 }
 
-int sp_timings(sp_sim* s, float* out6) {
+int sp_timings(sp_sim* s, float* out8) {
     return guarded(s, [&]() -> int {
-        if (!out6) return SP_E_INVALID;
-        for (int k = 0; k < 6; k++) out6[k] = s->timings[k];
+        if (!out8) return SP_E_INVALID;
+        for (int k = 0; k < 8; k++) out8[k] = s->timings[k];
         return SP_OK;
     });
 }
@@ -954,6 +1119,23 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_VW: *bytes = 16 * N; return c.vw;
     case SP_FIELD_VS: *bytes = 16 * N; return c.vs;
     case SP_FIELD_GBASE: *bytes = 16 * N; return c.gbase;
+    case SP_FIELD_VR: *bytes = 8 * N; return c.vr;
+    case SP_FIELD_VP: *bytes = 8 * N; return c.vp;
+    case SP_FIELD_VP2: *bytes = 8 * N; return c.vp2;
+    case SP_FIELD_VQ: *bytes = 8 * N; return c.vq;
+    case SP_FIELD_VD: *bytes = 8 * N; return c.vd;
+    case SP_FIELD_VR2: *bytes = 8 * N; return c.vr2;
+    case SP_FIELD_VQ2: *bytes = 8 * N; return c.vq2;
+    case SP_FIELD_VAZ: *bytes = 8 * N; return c.vaz;
+    case SP_FIELD_VTILES: *bytes = 4 * (size_t)s->nb0; return c.vtiles;
+    case SP_FIELD_VTILE: *bytes = 4 * (size_t)s->lv.tiles; return c.vtile;
+    case SP_FIELD_VMA: *bytes = 16 * T; return c.vma;
+    case SP_FIELD_VMB: *bytes = 16 * T; return c.vmb;
+    case SP_FIELD_VMD: *bytes = 8 * T; return c.vmd;
+    case SP_FIELD_VMR: *bytes = 8 * T; return c.vmr;
+    case SP_FIELD_VMX: *bytes = 8 * T; return c.vmx;
+    case SP_FIELD_VMZ: *bytes = 8 * T; return c.vmz;
+    case SP_FIELD_P2KEEP: *bytes = 4 * N; return c.p2keep;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1012,6 +1194,8 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         if (rc != SP_OK) return rc;
         const Ctx& c = s->c;
         dim3 g0 = grid_groups(s->W, s->H);
+        // stages run with the most iterations the parameters allow
+        const Recs full{s->params.pressure_max_iters, s->params.pressure_max_iters, std::max(s->params.viscosity_max_iters, 1u)};
         switch (stage) {
         case SP_STAGE_STEP: hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c); break;
         case SP_STAGE_CLEAR_ACC: launch_clear_acc(s); break;
@@ -1023,17 +1207,36 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
             hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, s->stream, c);
             hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, s->stream, c);
             break;
-        case SP_STAGE_VISC:
-            if (s->gp.visc_mode == 1) {
-                hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c);
-                hipLaunchKernelGGL(k_visc_explicit, g0, TILE, 0, s->stream, c);
+        case SP_STAGE_VISC: launch_visc_before(s, full); break;
+        case SP_STAGE_VISC_SPLIT: launch_visc_split(s); break;
+        case SP_STAGE_VISC_SOLVE: launch_visc_solve(s, full.v, arg != 0); break;
+        case SP_STAGE_RHS_VISC: hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, c); break;
+        case SP_STAGE_PRESSURE2: launch_pressure(s, full.p2, true); break;
+        case SP_STAGE_PROJECT_VISC: hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, c); break;
+        case SP_STAGE_VISC_VCYCLE: {
+            // One V-cycle from vr, whatever the state of the last solve: the level kernels
+            // return at once on a converged solve, so the flag is cleared around it.
+            uint32_t done = 0, zero = 0;
+            HIPCHK(hipMemcpy(&done, &c.K->v_done, 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&c.K->v_done, &zero, 4, hipMemcpyHostToDevice));
+            if (s->gp.v_mg_sweeps == 1) {
+                hipLaunchKernelGGL(k_vmg_down<1>, g0, TILE, 0, s->stream, c, 0, 0);
+                launch_visc_coarse<1>(s);
+                hipLaunchKernelGGL(k_visc_mgz<1>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
+            } else {
+                hipLaunchKernelGGL(k_vmg_down<2>, g0, TILE, 0, s->stream, c, 0, 0);
+                launch_visc_coarse<2>(s);
+                hipLaunchKernelGGL(k_visc_mgz<2>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
             }
+            HIPCHK(hipStreamSynchronize(s->stream));
+            HIPCHK(hipMemcpy(&c.K->v_done, &done, 4, hipMemcpyHostToDevice));
             break;
+        }
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, c); break;
-        case SP_STAGE_PRESSURE: launch_pressure(s, s->params.pressure_max_iters); break;
+        case SP_STAGE_PRESSURE: launch_pressure(s, full.p); break;
         case SP_STAGE_PROJECT: hipLaunchKernelGGL(k_project, g0, TILE, 0, s->stream, c); break;
         case SP_STAGE_SUBSTEP:
-            launch_substep(s, arg, s->params.pressure_max_iters);
+            launch_substep(s, arg, full);
             s->grid_valid = true;
             break;
         case SP_STAGE_MG_BUILD: launch_mg_build(s); break;

@@ -113,7 +113,9 @@ class FluidSim:
         return self.solver
 
     def timings(self):
-        out = np.zeros(6, np.float32)
+        """sp_timings: ms per substep of [total, particles, grid, pressure, projection,
+        viscosity + second projection], then the pressure and viscosity iterations."""
+        out = np.zeros(8, np.float32)
         self._check(self._lib.sp_timings(self._h, out.ctypes.data), "timings")
         return out
 

@@ -29,10 +29,12 @@ def rel_l2(a, b):
 class Pair:
     """The same tank on the device and in the oracle."""
 
-    def __init__(self, grid, frac=(0.4, 0.6), **kw):
+    def __init__(self, grid, frac=(0.4, 0.6), opts=None, **kw):
         self.grid = grid
         W, H = grid
         self.params, self.field, self.parts, rows = cases.dam_break(grid, frac=frac, **kw)
+        for k, v in (opts or {}).items():
+            setattr(self.params, k, v)
         cap = W * H * 4
         self.gpu = FluidSim(grid, cap, params=self.params)
         self.gpu.set_solids(self.field)
@@ -59,7 +61,15 @@ class Pair:
                   "SURF": (self.N, 4), "PJUMP": (self.N, 2), "PRHS": (self.N, 4), "PRES": (self.N, 2),
                   "MGA": (self.L.total, 4), "MGD": (self.L.total,), "SPROP": (self.N, 4),
                   "VEL0": (self.N, 2), "PTILE": (self.L.tiles,), "AUX": (self.N, 4), "GRHO": (self.N, 4),
-                  "SURF2": (self.N, 4), "GBASE": (self.N, 4)}
+                  "SURF2": (self.N, 4), "GBASE": (self.N, 4), "VW": (self.N, 4), "VS": (self.N, 4),
+                  "P2KEEP": (self.N,)}
+        for v in ("VR", "VP", "VP2", "VQ", "VD", "VR2", "VQ2", "VAZ"):
+            shapes[v] = (self.N, 2)
+        for v in ("VMA", "VMB"):
+            shapes[v] = (self.L.total, 4)
+        for v in ("VMD", "VMR", "VMX", "VMZ"):
+            shapes[v] = (self.L.total, 2)
+        n0 = ((self.grid[0] + 15) // 16) * ((self.grid[1] + 15) // 16)
         for n in names:
             if n == "STEP":
                 self.gpu.put_field("STEP", self.ora.field("STEP", N.STEP_DTYPE, 1))
@@ -69,8 +79,10 @@ class Pair:
                 self.gpu.put_field("ACC", self.ora.field("ACC", np.int32, self.N * self.stride))
             elif n == "PARTICLES":
                 self.gpu.put_field("PARTICLES", self.ora.field("PARTICLES", N.PARTICLE_DTYPE, self.ora.n_slots()))
-            elif n == "PTILE":
-                self.gpu.put_field("PTILE", self.ora.field("PTILE", np.uint32, self.L.tiles))
+            elif n in ("PTILE", "VTILE"):
+                self.gpu.put_field(n, self.ora.field(n, np.uint32, self.L.tiles))
+            elif n == "VTILES":
+                self.gpu.put_field(n, self.ora.field(n, np.uint32, n0))
             else:
                 dt = np.float32
                 self.gpu.put_field(n, self.ora.field(n, dt, shapes[n]))
