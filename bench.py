@@ -1,23 +1,27 @@
-"""Benchmark of the MI355X fluid step: particle-steps/s and pressure-solve ms at 1024 x 576.
+"""Benchmark of the MI355X fluid step at 1024 x 576: particle-steps/s and pressure-solve ms.
 
-One "step" is one substep of the fluid step (step, particle pass, grid passes, MGPCG
-pressure solve, projection) over the whole tank, recorded in batches of 16 substeps as the
-reference's blocking helpers record them (advance_blocking, mod.rs:3284-3312). Inputs are
-resident in device memory when the timed region starts.
+One "step" is one substep of the fluid step (step, particle pass, grid passes, viscosity,
+the MGPCG pressure solves, projections) over the whole tank, recorded in batches of 16
+substeps as the reference's blocking helpers record them (advance_blocking,
+mod.rs:3284-3312). Inputs are resident in device memory when the timed region starts.
 
-Workload (named in config.workload): a 1024 x 576 dam break in FP32 — the grid of the
-metric, with the plain single-liquid physics of the reference's CPU-runnable dam-break case
-(water, cell 5 mm, fixed dt 1/240 s, pressure tolerance 1e-6, 3-cell solid frame) and a
-block of 80 % x 80 % of the tank, about 1.49 M particles at 4 per cell (the particle count
-of the four-liquid 1024 x 576 case, whose viscous liquids need the implicit viscosity solve
-this build does not carry yet).
+Workloads (config.workload):
+* four_liquids_1024x576_fp32 (default): oil, water, honey and mercury in four seeded
+  rectangles tiling 80 % x 80 % of a 1024 x 576 tank (about 1.49 M particles at 4 per cell),
+  densities 800 / 1000 / 1300 / 13500 kg/m^3, viscosities log-uniform over 1e-6 .. 1e-1 m^2/s
+  and tensions uniform over 0.02 .. 0.07 N/m from seed 0; cell 5 mm, fixed dt 1/240 s;
+  pressure tolerance 1e-6 and viscosity tolerance 1e-5 (the tightest the reference
+  accepts); the implicit viscosity with the second pressure solve; all FP32 with FP64
+  reductions.
+* dam_break_1024x576_fp32 (--workload dam): one block of water of the same size, explicit
+  viscosity, one pressure solve.
 
-Prints one JSON line. `roofline` is for the particle pass (k_particles), the kernel with the
-largest share of the substep: algorithmic bytes per launch = 96 B per live particle (its
-48-byte state read and written) + 100 B per grid cell (face velocities 16, composition 16,
-SDF 4, accumulator read-modify-write 64), over its HIP-event-timed average launch.
-`cpu_baseline` times the CPU oracle (a single-threaded C restatement of the same step) on a
-bounded sample of the same workload.
+Prints one JSON line. `roofline` is for the particle pass (k_particles), the largest single
+kernel of a substep: algorithmic bytes per launch = 96 B per live particle (its 48-byte
+state read and written) + 100 B per grid cell (face velocities 16, composition 16, SDF 4,
+accumulator read-modify-write 64), over its HIP-event-timed average launch. `cpu_baseline`
+times the CPU oracle (a single-threaded C restatement of the same step) on a bounded sample
+of the same workload.
 """
 import argparse
 import json
@@ -32,29 +36,33 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_particles_dam_break_1024x576_pmc.md")
+PMC_SUMMARY = {"dam": os.path.join(ROOT, "profiles", "k_particles_dam_break_1024x576_pmc.md"),
+               "four": os.path.join(ROOT, "profiles", "k_particles_four_liquids_1024x576_pmc.md")}
 
 
-def pmc_traffic(W, H, n_p):
+def pmc_traffic(kind, W, H):
     """HBM bytes per k_particles launch from the committed FETCH_SIZE + WRITE_SIZE summary
-    (each counter scaled by what it reads for a plain copy of known size). It is quoted only
-    for the workload it was measured on; any other grid or particle count gets null."""
-    if (W, H, n_p) != (1024, 576, 1494912) or not os.path.exists(PMC_SUMMARY):
+    of this workload (each counter scaled by what it reads for a plain copy of known size),
+    or None where none was measured."""
+    path = PMC_SUMMARY[kind]
+    if (W, H) != (1024, 576) or not os.path.exists(path):
         return None
-    for line in open(PMC_SUMMARY):
+    for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":")[1])
     return None
 
 
-def workload(grid=(1024, 576), frac=(0.8, 0.8)):
+def workload(kind, grid):
     import cases
-    return cases.dam_break(grid, frac=frac)
+    if kind == "four":
+        return cases.four_liquids(grid, seed=0, viscosity_tol=1e-5)
+    return cases.dam_break(grid, frac=(0.8, 0.8))
 
 
-def cpu_baseline(grid, frac, params, field, parts, rows, substeps=4):
-    """The oracle on the host's cores: `substeps` substeps of the same tank from the same
-    state (the first one without a gather, as after a reset)."""
+def cpu_baseline(grid, params, field, parts, rows, substeps):
+    """The oracle on one host core: `substeps` substeps of the same tank from the same state,
+    after one untimed substep (the first after a reset, without a gather)."""
     from sprout_amd import _native as N
     from oracle.binding import OracleSim
 
@@ -89,20 +97,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--grid", default="1024x576")
-    ap.add_argument("--frac", default="0.8x0.8")
+    ap.add_argument("--workload", choices=["four", "dam"], default="four")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=24)
+    ap.add_argument("--cpu-substeps", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how a tank would shard", file=sys.stderr)
+              "a tank shards by scene across GPUs, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
     from sprout_amd.fluid_sim import FluidSim
 
     W, H = (int(x) for x in args.grid.split("x"))
-    frac = tuple(float(x) for x in args.frac.split("x"))
-    params, field, parts, rows = workload((W, H), frac)
+    params, field, parts, rows = workload(args.workload, (W, H))
     n_p = len(parts)
     sim = FluidSim((W, H), W * H * 4, params=params)
     sim.set_solids(field)
@@ -138,6 +145,22 @@ def main():
     alg_bytes = 96.0 * n_p + 100.0 * W * H
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     live = sim.stats().live
+    name = {"four": "four_liquids", "dam": "dam_break"}[args.workload]
+    config = {
+        "workload": f"{name}_{W}x{H}_fp32",
+        "grid": [W, H],
+        "particles": n_p,
+        "live_particles_end": int(live),
+        "cell_size_m": 0.005,
+        "dt": "1/240 s fixed",
+        "pressure_tol": float(params.pressure_tol),
+        "substeps_per_batch": args.batch,
+    }
+    if args.workload == "four":
+        config["seed"] = 0
+        config["liquids"] = [{"row": r, "density": params.mat[r].density, "viscosity": params.mat[r].viscosity,
+                              "tension": params.mat[r].tension} for r in rows]
+        config["viscosity_tol"] = float(params.viscosity_tol)
     out = {
         "metric": "particle-steps/s & pressure-solve ms at 1024x576; p50/p99 step latency, batched",
         "value": value,
@@ -151,23 +174,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"dam_break_{W}x{H}_fp32",
-            "grid": [W, H],
-            "particles": n_p,
-            "live_particles_end": int(live),
-            "block": list(frac),
-            "cell_size_m": 0.005,
-            "dt": "1/240 s fixed",
-            "pressure_tol": 1e-6,
-            "substeps_per_batch": args.batch,
-        },
+        "config": config,
         "pressure_solve_ms": float(ph[3]),
+        "pressure2_solve_ms": float(ph[6]),
+        "viscosity_solve_ms": float(ph[5]),
         "pressure_iters_max": int(st.pressure_iters_max),
+        "viscosity_iters_max": int(st.viscosity_iters_max),
         "pressure_unconverged": int(st.pressure_unconverged),
+        "viscosity_unconverged": int(st.viscosity_unconverged),
         "pressure_residual_max": float(st.pressure_residual_max),
+        "viscosity_residual_max": float(st.viscosity_residual_max),
         "phase_ms_per_substep": {"particles": float(ph[1]), "grid": float(ph[2]), "pressure": float(ph[3]),
-                                  "project": float(ph[4]), "device_total": float(ph[0])},
+                                  "project": float(ph[4]), "viscosity": float(ph[5]),
+                                  "pressure2_project": float(ph[6]), "device_total": float(ph[0])},
         "roofline": {
             "kernel": "k_particles",
             "bound": "hbm",
@@ -175,7 +194,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": pmc_traffic(W, H, n_p),
+            "traffic": pmc_traffic(args.workload, W, H),
             "launch_ms": k_ms,
             "alg_bytes_per_launch": alg_bytes,
         },
@@ -183,7 +202,7 @@ def main():
     }
     sim.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline((W, H), frac, params, field, parts, rows, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline((W, H), params, field, parts, rows, args.cpu_substeps)
     print(json.dumps(out))
 
 

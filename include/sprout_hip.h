@@ -140,10 +140,10 @@ int sp_debug_gparams(sp_sim* sim, sp_gparams* out, sp_gmat* mats);
 /* Device time per substep of the last recorded batch's phases (ms, averaged over its
  * substeps): [0] total, [1] step + accumulator clear + particle pass, [2] grid passes
  * before the solve (with the viscosity that precedes it), [3] pressure solve, [4]
- * projection, [5] implicit viscosity solve + second pressure solve + second projection (0
- * unless the tank takes them); [6] the iterations of the last first pressure solve, [7] the
- * most viscosity iterations of the batch. Valid after sp_wait. */
-int sp_timings(sp_sim* sim, float* out8);
+ * projection, [5] implicit viscosity solve after the projection, [6] second pressure solve
+ * and projection ([5] and [6] are 0 unless the tank takes them); then the iterations of the
+ * latest solves: [7] pressure, [8] viscosity, [9] second pressure. Valid after sp_wait. */
+int sp_timings(sp_sim* sim, float* out10);
 /* Average device time (ms) of one launch of a hot kernel over the substeps of the last
  * batch (kernel 0: the particle pass, k_particles), timed with HIP events on the handle's
  * stream, and the number of launches averaged. */

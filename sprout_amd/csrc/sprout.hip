@@ -57,12 +57,13 @@ struct Recs {
 };
 
 // Instantiated graphs of one (recorded iterations, schedule) key: grid passes (with the
-// viscosity before the first solve), pressure solve, projection, and the viscosity solve
-// with the second projection.
+// viscosity before the first solve), pressure solve, projection, the viscosity solve after
+// it, and the second pressure solve with its projection.
+constexpr int GRAPH_PARTS = 5;
 struct Graphs {
     Recs recs;
     uint32_t sched = 0xffffffffu;
-    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[GRAPH_PARTS] = {};
 };
 
 struct sp_sim {
@@ -113,16 +114,17 @@ struct sp_sim {
     sp_solver_stats stats{};
     uint64_t substeps_base = 0;
     uint32_t last_unconv = 0, last_p2_unconv = 0, last_v_unconv = 0;
+    uint32_t last_v_iters = 0, last_p2_iters = 0;  // of the latest solves
 
     // timing: per substep of a batch (up to TIMED_SUBSTEPS), events at the phase
     // boundaries and tightly around the particle kernel
     static constexpr int TIMED_SUBSTEPS = 64;
-    static constexpr int EV_PER = 8;
+    static constexpr int EV_PER = 9;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     uint32_t timed = 0;  // substeps of the current batch with events recorded
     uint32_t timed_last = 0;
-    float timings[8] = {};
+    float timings[10] = {};
     float particle_ms = 0.0f;
     bool use_graphs = true;
     std::vector<Graphs> graphs;
@@ -293,13 +295,18 @@ void launch_visc_before(sp_sim* s, const Recs& r) {
     }
 }
 
-// After the first projection: the viscosity solve and the second projection
-// (mod.rs:2690-2698).
+bool twice_projected(const sp_sim* s) { return s->gp.visc_mode == 2 && s->gp.v_twice; }
+
+// After the first projection: the viscosity solve (mod.rs:2690-2691) ...
 void launch_visc_after(sp_sim* s, const Recs& r) {
-    if (!(s->gp.visc_mode == 2 && s->gp.v_twice)) return;
+    if (twice_projected(s)) launch_visc_solve(s, r.v, true);
+}
+
+// ... and the second pressure solve with its projection (mod.rs:2692-2697).
+void launch_second_projection(sp_sim* s, const Recs& r) {
+    if (!twice_projected(s)) return;
     const Ctx& c = s->c;
     dim3 g0 = grid_groups(s->W, s->H);
-    launch_visc_solve(s, r.v, true);
     hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, c);
     launch_pressure(s, r.p2, true);
     hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, c);
@@ -366,7 +373,8 @@ hipGraphExec_t capture(sp_sim* s, int part, const Recs& r) {
     if (part == 0) launch_grid_passes(s, r);
     else if (part == 1) launch_pressure(s, r.p);
     else if (part == 2) hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, s->c);
-    else launch_visc_after(s, r);
+    else if (part == 3) launch_visc_after(s, r);
+    else launch_second_projection(s, r);
     hipError_t le = hipGetLastError();
     HIPCHK(hipStreamEndCapture(s->stream, &g));
     HIPCHK(le);
@@ -393,7 +401,7 @@ Graphs& graphs_for(sp_sim* s, const Recs& r) {
     Graphs& g = s->graphs.back();
     g.recs = r;
     g.sched = key;
-    for (int k = 0; k < 4; k++) g.exec[k] = capture(s, k, r);
+    for (int k = 0; k < GRAPH_PARTS; k++) g.exec[k] = capture(s, k, r);
     return g;
 }
 
@@ -421,6 +429,9 @@ void launch_substep(sp_sim* s, int gather, const Recs& r) {
     if (g) HIPCHK(hipGraphLaunch(g->exec[3], s->stream));
     else launch_visc_after(s, r);
     if (e) HIPCHK(hipEventRecord(e[7], s->stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[4], s->stream));
+    else launch_second_projection(s, r);
+    if (e) HIPCHK(hipEventRecord(e[8], s->stream));
     HIPCHK(hipGetLastError());
 }
 
@@ -469,6 +480,8 @@ void readback(sp_sim* s) {
     st.pressure_residual_max = std::max(st.pressure_residual_max, K.res_max);
     st.viscosity_residual_max = std::max(st.viscosity_residual_max, K.v_res_max);
     st.pressure_iters_last = K.iters_last;
+    s->last_v_iters = K.v_iters_last;
+    s->last_p2_iters = K.p2_iters_last;
     if (st.dt_cfl > 0.0f) s->dt_estimate = st.dt_cfl;
     // Adapt the recorded iterations (mod.rs:3100-3115): a little more than the batch needed,
     // half as much again when a solve ran out.
@@ -546,19 +559,20 @@ int wait(sp_sim* s) {
     if (s->timing && s->timed > 0) {
         // averages per substep: [0] total, [1] step + clear + particle pass, [2] grid passes
         // (with the viscosity before the first solve), [3] pressure solve, [4] projection,
-        // [5] viscosity solve and second projection; [6], [7] below are iteration counts
-        double acc[6] = {0, 0, 0, 0, 0, 0};
+        // [5] viscosity solve after it, [6] second pressure solve and projection; [7]-[9]
+        // below are iteration counts
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
         for (uint32_t k = 0; k < s->timed; k++) {
             hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
-            const int from[5] = {0, 1, 2, 3, 4}, to[5] = {1, 2, 3, 4, 7};
+            const int from[6] = {0, 1, 2, 3, 4, 7}, to[6] = {1, 2, 3, 4, 7, 8};
             float t;
-            for (int q = 0; q < 5; q++) {
+            for (int q = 0; q < 6; q++) {
                 HIPCHK(hipEventElapsedTime(&t, e[from[q]], e[to[q]]));
                 acc[q + 1] += t;
                 acc[0] += t;
             }
         }
-        for (int q = 0; q < 6; q++) s->timings[q] = (float)(acc[q] / s->timed);
+        for (int q = 0; q < 7; q++) s->timings[q] = (float)(acc[q] / s->timed);
         double kp = 0;
         for (uint32_t k = 0; k < s->timed; k++) {
             hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
@@ -571,8 +585,9 @@ int wait(sp_sim* s) {
         s->timed = 0;
     }
     readback(s);
-    s->timings[6] = (float)s->stats.pressure_iters_last;
-    s->timings[7] = (float)s->stats.viscosity_iters_max;
+    s->timings[7] = (float)s->stats.pressure_iters_last;
+    s->timings[8] = (float)s->last_v_iters;
+    s->timings[9] = (float)s->last_p2_iters;
     return SP_OK;
 }
 
@@ -1072,10 +1087,10 @@ int sp_sort(sp_sim* s) {
     });
 }
 
-int sp_timings(sp_sim* s, float* out8) {
+int sp_timings(sp_sim* s, float* out10) {
     return guarded(s, [&]() -> int {
-        if (!out8) return SP_E_INVALID;
-        for (int k = 0; k < 8; k++) out8[k] = s->timings[k];
+        if (!out10) return SP_E_INVALID;
+        for (int k = 0; k < 10; k++) out10[k] = s->timings[k];
         return SP_OK;
     });
 }

@@ -114,8 +114,9 @@ class FluidSim:
 
     def timings(self):
         """sp_timings: ms per substep of [total, particles, grid, pressure, projection,
-        viscosity + second projection], then the pressure and viscosity iterations."""
-        out = np.zeros(8, np.float32)
+        viscosity solve, second pressure solve + projection], then the iterations of the
+        latest pressure, viscosity and second pressure solves."""
+        out = np.zeros(10, np.float32)
         self._check(self._lib.sp_timings(self._h, out.ctypes.data), "timings")
         return out
 

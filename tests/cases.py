@@ -1,8 +1,13 @@
-"""Seeded inputs of the parity tests and benchmarks: the dam-break tanks of the fluid step.
+"""Seeded inputs of the parity tests and benchmarks.
 
 Dam break (the reference's headless dam-break workload): one block of water, x < 0.4 W,
 y < 0.6 H, on a 2 x 2 jittered lattice per cell (fill_rect, maps.rs:468-490), in the solid
 tank frame (scene.rs:8, WALL = 3 cells), zero initial velocity, fixed dt 1/240 s.
+
+Four liquids: oil, water, honey (miscible with water) and mercury in four seeded rectangles
+tiling a block of 80 % x 80 % of the tank, densities 800 / 1000 / 1300 / 13500 kg/m^3,
+kinematic viscosities log-uniform over 1e-6 .. 1e-1 m^2/s and surface tensions uniform over
+0.02 .. 0.07 N/m, both drawn from the seed; fixed dt 1/240 s, pressure tolerance 1e-6.
 """
 import numpy as np
 
@@ -42,3 +47,34 @@ def dam_break(grid, frac=(0.4, 0.6), **kw):
     field = scene.rasterize()
     parts = settle(parts, params)
     return params, field, parts, [N.ROW_WATER]
+
+
+FOUR_ROWS = (N.ROW_OIL, N.ROW_WATER, N.ROW_HONEY, N.ROW_MERCURY)
+FOUR_DENSITY = {N.ROW_OIL: 800.0, N.ROW_WATER: 1000.0, N.ROW_HONEY: 1300.0, N.ROW_MERCURY: 13500.0}
+
+
+def four_liquids(grid, seed=0, fill=(0.8, 0.8), cell_size=0.005, pressure_tol=1e-6, viscosity_tol=1e-3):
+    """(params, solid field, settled particles, rows) of the four-liquid tank."""
+    rng = np.random.default_rng(seed)
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.viscosity_tol = viscosity_tol
+    for r in FOUR_ROWS:
+        m = params.mat[r]
+        m.density = FOUR_DENSITY[r]
+        m.viscosity = float(10.0 ** rng.uniform(-6.0, -1.0))
+        m.tension = float(rng.uniform(0.02, 0.07))
+    W, H = grid
+    scene = sc.Scene(grid)
+    X, Y = fill[0] * W, fill[1] * H
+    xs = float(rng.uniform(0.35, 0.65)) * X
+    y0, y1 = (float(rng.uniform(0.35, 0.65)) * Y for _ in range(2))
+    rects = [(0.0, 0.0, xs, y0), (0.0, y0, xs, Y), (xs, 0.0, X, y1), (xs, y1, X, Y)]
+    order = [FOUR_ROWS[k] for k in rng.permutation(4)]
+    parts = []
+    n = 0
+    for rect, row in zip(rects, order):
+        q = sc.fill_rect(scene, rect, row, n_before=n)
+        n += len(q)
+        parts.append(q)
+    parts = settle(np.concatenate(parts), params)
+    return params, scene.rasterize(), parts, sorted(FOUR_ROWS)

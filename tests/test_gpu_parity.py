@@ -29,10 +29,13 @@ def rel_l2(a, b):
 class Pair:
     """The same tank on the device and in the oracle."""
 
-    def __init__(self, grid, frac=(0.4, 0.6), opts=None, **kw):
+    def __init__(self, grid, frac=(0.4, 0.6), opts=None, case=None, **kw):
         self.grid = grid
         W, H = grid
-        self.params, self.field, self.parts, rows = cases.dam_break(grid, frac=frac, **kw)
+        if case is None:
+            self.params, self.field, self.parts, rows = cases.dam_break(grid, frac=frac, **kw)
+        else:
+            self.params, self.field, self.parts, rows = case(grid, **kw)
         for k, v in (opts or {}).items():
             setattr(self.params, k, v)
         cap = W * H * 4
