@@ -16,7 +16,7 @@
 
 #define SP_DEV static __device__ __forceinline__
 
-// Device pointers of one tank, passed by value to every kernel.
+// Device pointers of one tank, rebuilt at every kernel's entry from its TankRef.
 struct Ctx {
     int W, H, N;
     const sp_gparams* P;
@@ -35,6 +35,7 @@ struct Ctx {
     float4* mga;
     float* mgd;
     float2 *mgx, *mgz, *mgr, *cgp, *cgp2, *cgq;
+    float2* r2;           // second level-0 residual buffer of the CG (alternates with mgr)
     uint32_t* ptile;
     double4* part_sums;   // per-block partial sums of a reduction
     const int4* lev;      // per level: offset, width, height, tile offset
@@ -47,6 +48,76 @@ struct Ctx {
     float2 *vmd, *vmr, *vmx, *vmz;
     double4* vpart;       // per-tile partial sums of the viscosity solve
 };
+
+// A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
+// and the sizes the buffers' offsets follow from. Kernels take the tanks of their launch by
+// value in the kernel-argument segment and index them by blockIdx.z; pointers derived from
+// a kernel argument are known to be global memory (global and scalar loads), which pointers
+// read from ordinary memory are not.
+struct TankRef {
+    char* base;
+    int W, H;
+    uint32_t cap;        // particle slots
+    int total;           // cells of every multigrid level
+    int tiles;           // 16 x 16 tile flags of levels 0 .. tail
+    int n_levels, tail;
+};
+
+#define SP_LAUNCH_TANKS 32  // most tanks of one launch (32 x 40 B of kernel arguments)
+struct TankBatch {
+    TankRef t[SP_LAUNCH_TANKS];
+};
+
+// The arena's layout: every buffer at a 256-byte aligned offset, in a fixed order. With
+// base = nullptr, the returned `bytes` is the arena's size.
+__host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes = nullptr) {
+    Ctx c;
+    size_t o = 0;
+    auto take = [&](size_t n) {
+        char* p = r.base + o;
+        o += (n + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t N = (size_t)r.W * r.H, T = (size_t)r.total;
+    const size_t nb0 = (size_t)((r.W + 15) / 16) * ((r.H + 15) / 16);
+    c.W = r.W;
+    c.H = r.H;
+    c.N = (int)N;
+    c.n_levels = r.n_levels;
+    c.tail = r.tail;
+    c.P = (const sp_gparams*)take(sizeof(sp_gparams));
+    c.mats = (const sp_gmat*)take(sizeof(sp_gmat) * SP_MAX_MATERIALS);
+    c.S = (sp_stepbuf*)take(sizeof(sp_stepbuf));
+    c.K = (sp_solve*)take(sizeof(sp_solve));
+    c.lev = (const int4*)take(sizeof(int4) * SP_MAX_LEVELS);
+    c.region_air = (uint32_t*)take(sizeof(uint32_t));
+    c.part = (sp_particle*)take(sizeof(sp_particle) * (r.cap ? r.cap : 1));
+    c.acc = (int32_t*)take(sizeof(int32_t) * N * SP_ACC_MAX);
+    float4** f4[] = {&c.vel, &c.cell, &c.comp, &c.coef, &c.surf, &c.surf2, &c.sprop, &c.aux, &c.grho,
+                     &c.gbase, &c.prhs, &c.vw, &c.vs};
+    for (float4** f : f4) *f = (float4*)take(sizeof(float4) * N);
+    c.solid = (const float4*)take(sizeof(float4) * N);
+    float2** f2[] = {&c.vel0, &c.pjump, &c.pres, &c.cgp, &c.cgp2, &c.cgq, &c.r2, &c.vr, &c.vp, &c.vp2, &c.vq,
+                     &c.vd, &c.vr2, &c.vq2, &c.vaz};
+    for (float2** f : f2) *f = (float2*)take(sizeof(float2) * N);
+    c.sdf = (const float*)take(sizeof(float) * (size_t)(r.W + 1) * (r.H + 1));
+    c.mixb = (float*)take(sizeof(float) * N * SP_MAX_PRESENT);
+    c.p2keep = (float*)take(sizeof(float) * N);
+    c.region = (const uint32_t*)take(sizeof(uint32_t) * N);
+    c.mga = (float4*)take(sizeof(float4) * T);
+    c.mgd = (float*)take(sizeof(float) * T);
+    float2** t2[] = {&c.mgx, &c.mgz, &c.mgr, &c.vmd, &c.vmr, &c.vmx, &c.vmz};
+    for (float2** f : t2) *f = (float2*)take(sizeof(float2) * T);
+    c.vma = (float4*)take(sizeof(float4) * T);
+    c.vmb = (float4*)take(sizeof(float4) * T);
+    c.ptile = (uint32_t*)take(sizeof(uint32_t) * (size_t)r.tiles);
+    c.vtile = (uint32_t*)take(sizeof(uint32_t) * (size_t)r.tiles);
+    c.vtiles = (uint32_t*)take(sizeof(uint32_t) * nb0);
+    c.part_sums = (double4*)take(sizeof(double4) * 2 * nb0);
+    c.vpart = (double4*)take(sizeof(double4) * nb0);
+    if (bytes) *bytes = o;
+    return c;
+}
 
 SP_DEV float fmaxf_(float a, float b) { return a > b ? a : b; }
 SP_DEV float fminf_(float a, float b) { return a < b ? a : b; }

@@ -19,7 +19,8 @@ SP_DEV bool cell_of_thread(const Ctx& c, int& x, int& y, int& i) {
 }  // namespace
 
 // step.comp:22-50: the substep length and the Kahan-summed simulated time.
-__global__ void k_step(Ctx c) {
+__global__ void k_step(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sp_stepbuf* S = c.S;
     const sp_gparams& P = *c.P;
@@ -61,6 +62,24 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_
     if (blockIdx.x == 0 && k < n) p[k] = v;
 }
 
+// Each tank's accumulator cleared (record_substep, mod.rs:2628): W H stride words in a
+// grid-stride loop of 16-byte stores, the last few singly.
+__global__ __launch_bounds__(256) void k_clear_acc(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const size_t n = (size_t)c.N * (size_t)c.P->acc_stride, n4 = n / 4;
+    uint4* p4 = reinterpret_cast<uint4*>(c.acc);
+    for (size_t k = blockIdx.x * (size_t)256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256) p4[k] = make_uint4(0, 0, 0, 0);
+    const size_t k = n4 * 4 + blockIdx.x * (size_t)256 + threadIdx.x;
+    if (blockIdx.x == 0 && k < n) c.acc[k] = 0;
+}
+
+// Each tank's active viscosity tiles cleared before k_visc_setup marks them.
+__global__ __launch_bounds__(256) void k_clear_vtiles(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const int n = ((c.W + 15) >> 4) * ((c.H + 15) >> 4);
+    for (int k = threadIdx.x; k < n; k += 256) c.vtiles[k] = 0u;
+}
+
 namespace {
 SP_DEV bool packed(const Ctx& c, int i) {
     float4 sol = c.solid[i];
@@ -70,7 +89,8 @@ SP_DEV bool packed(const Ctx& c, int i) {
 }  // namespace
 
 // grid_prep.comp:20-263, isothermal, without the CAP-gated fields.
-__global__ __launch_bounds__(256) void k_grid_prep(Ctx c) {
+__global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     const sp_gparams& P = *c.P;
@@ -211,7 +231,8 @@ SP_DEV void face_coef(float4 a, float4 b, float open, float& g, float& rho) {
 
 // grid_coef.comp:27-38 fused with surf_smooth.comp:16-44 (both read only cell, comp,
 // aux and solid, which grid_prep has finished).
-__global__ __launch_bounds__(256) void k_coef_smooth(Ctx c) {
+__global__ __launch_bounds__(256) void k_coef_smooth(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     float4 o = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
@@ -348,7 +369,8 @@ SP_DEV float curvature(const Ctx& c, int cx, int cy, int k, float kmax) {
 }  // namespace
 
 // surf_curv.comp:104-121.
-__global__ __launch_bounds__(256) void k_surf_curv(Ctx c) {
+__global__ __launch_bounds__(256) void k_surf_curv(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     float kmax = c.P->kappa_max;
@@ -410,7 +432,8 @@ SP_DEV float face_jump(const Ctx& c, int ci, int n, float open) {
 }  // namespace
 
 // surf_apply.comp:71-82.
-__global__ __launch_bounds__(256) void k_surf_apply(Ctx c) {
+__global__ __launch_bounds__(256) void k_surf_apply(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     float2 pj = make_float2(0.0f, 0.0f);
@@ -434,7 +457,8 @@ SP_DEV float vel_v(const Ctx& c, int i) { return live_v(c, i) ? c.vel[i].y : 0.0
 }  // namespace
 
 // rhs.comp:19-93: the level-0 matrix, both right-hand sides, and the tile flags.
-__global__ __launch_bounds__(256) void k_rhs(Ctx c) {
+__global__ __launch_bounds__(256) void k_rhs(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     const bool in = cell_of_thread(c, cx, cy, i);
     __shared__ int s_any;
@@ -493,7 +517,8 @@ SP_DEV float difference(bool lc, bool ln, float pc, float pn, float j) {
 }  // namespace
 
 // project.comp:26-58.
-__global__ __launch_bounds__(256) void k_project(Ctx c) {
+__global__ __launch_bounds__(256) void k_project(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     const bool in = cell_of_thread(c, cx, cy, i);
     float m = 0.0f;

@@ -495,7 +495,8 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
 }  // namespace
 
 // The particle pass (particles.comp:1064-1101).
-__global__ __launch_bounds__(PT) void k_particles(Ctx c, int gather) {
+__global__ __launch_bounds__(PT) void k_particles(const TankBatch tb, int gather) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ int sacc[LDS_INTS];
     __shared__ int sbox[4];
     const int t = threadIdx.x;
@@ -560,7 +561,8 @@ __global__ __launch_bounds__(PT) void k_particles(Ctx c, int gather) {
 // the tile. The substep loop keeps the particles in tiled order (neighbours in memory are
 // neighbours in both x and y, so a work-group's scatter box stays near 11 x 11 cells);
 // sp_sort gives the reference's order. Either order leaves every result unchanged.
-__global__ __launch_bounds__(256) void k_sort_keys(Ctx c, uint32_t* keys, uint32_t* vals, int tiled, uint32_t dead) {
+__global__ __launch_bounds__(256) void k_sort_keys(const TankBatch tb, uint32_t* keys, uint32_t* vals, int tiled, uint32_t dead) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= c.P->n_slots) return;
     sp_particle p = c.part[id];
@@ -574,14 +576,16 @@ __global__ __launch_bounds__(256) void k_sort_keys(Ctx c, uint32_t* keys, uint32
     vals[id] = id;
 }
 
-__global__ __launch_bounds__(256) void k_sort_gather(Ctx c, const uint32_t* vals, sp_particle* out, uint32_t n) {
+__global__ __launch_bounds__(256) void k_sort_gather(const TankBatch tb, const uint32_t* vals, sp_particle* out, uint32_t n) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     uint32_t j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     out[j] = c.part[vals[j]];
 }
 
 // Live particles per cell in a window (count.comp:17-23).
-__global__ __launch_bounds__(256) void k_count(Ctx c, int x0, int y0, int w, int h, uint32_t* out) {
+__global__ __launch_bounds__(256) void k_count(const TankBatch tb, int x0, int y0, int w, int h, uint32_t* out) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= c.P->n_slots) return;
     sp_particle p = c.part[id];
@@ -592,7 +596,8 @@ __global__ __launch_bounds__(256) void k_count(Ctx c, int x0, int y0, int w, int
 }
 
 // Live count and per-row counts / concentration sums (stats.comp:14-42).
-__global__ __launch_bounds__(64) void k_stats(Ctx c, uint32_t* counter) {
+__global__ __launch_bounds__(64) void k_stats(const TankBatch tb, uint32_t* counter) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     const uint32_t RUN = 64u;
     uint32_t first = (blockIdx.x * 64 + threadIdx.x) * RUN;
     uint32_t n_slots = c.P->n_slots;
@@ -623,7 +628,8 @@ __global__ __launch_bounds__(64) void k_stats(Ctx c, uint32_t* counter) {
 }
 
 // Dead slots claim pending spawns (claim.comp:10-17).
-__global__ __launch_bounds__(256) void k_claim(Ctx c, const sp_particle* spawn, uint32_t spawn_count, uint32_t* counter) {
+__global__ __launch_bounds__(256) void k_claim(const TankBatch tb, const sp_particle* spawn, uint32_t spawn_count, uint32_t* counter) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= c.P->n_slots) return;
     if (c.part[id].kind != SP_KIND_DEAD) return;

@@ -1,0 +1,382 @@
+// schedule.hip — the substep schedule on a HIP stream (record_substep, mod.rs:2615-2708):
+// which kernels run, in which order, with which recorded iteration counts, for one launch
+// group of T tanks of the same grid and solver schedule (a Run). Every launch covers all T
+// tanks (blockIdx.z = tank), each tank reading its own device state through its context, so
+// a group's substep costs the same number of launches as one tank's. A single handle is the
+// group of one.
+//
+// Parts of a substep are captured once per (iteration counts, schedule) as HIP graphs and
+// replayed: a substep is several hundred dependent launches. The kernels read everything
+// that changes between substeps (dt, tables, solve state) from device memory, so a replay
+// needs no new arguments.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHK(x)                                     \
+    do {                                              \
+        hipError_t e_ = (x);                          \
+        if (e_ != hipSuccess) throw HipError{e_, #x}; \
+    } while (0)
+
+// One launch group.
+struct Run {
+    TankBatch tb{};  // the T tanks (kernel argument of every launch)
+    int T = 1;
+    int W = 0, H = 0, nb0 = 0;  // grid, and its 16 x 16 tiles
+    sp_levels_info lv{};
+    sp_gparams gp{};    // host copy of the tables: the schedule fields are equal across the group
+    hipStream_t stream = nullptr;
+    size_t tail_lds = 0;  // dynamic LDS of k_mg_tail
+    uint32_t slots = 0;   // most particle slots of any tank of the group
+};
+
+// Recorded iterations of one substep: first pressure solve, second pressure solve (after
+// the viscosity), viscosity solve (record_substep, mod.rs:2615-2623).
+struct Recs {
+    uint32_t p = 0, p2 = 0, v = 0;
+    bool operator==(const Recs& o) const { return p == o.p && p2 == o.p2 && v == o.v; }
+};
+
+namespace {
+
+const dim3 TILE(16, 16, 1);
+dim3 groups_of(const Run& r, int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16, r.T); }
+dim3 per_tank(const Run& r) { return dim3(1, 1, r.T); }
+dim3 grid0(const Run& r) { return groups_of(r, r.W, r.H); }
+
+// ---- pressure (record_pressure / record_cg / record_vcycle, mod.rs:2929-3031) -------------
+
+// One V-cycle on the level-0 residual (mgr, or r2 with rsel); with `skip_down0` the way down
+// on level 0 has already run (fused into the CG update).
+void launch_vcycle(const Run& r, int gated, int reduce, int rsel, bool skip_down0 = false) {
+    const sp_levels_info& L = r.lv;
+    for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
+        hipLaunchKernelGGL(k_mg_down, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated, rsel);
+    hipLaunchKernelGGL(k_mg_tail, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, gated);
+    for (int l = L.tail - 1; l >= 0; l--)
+        hipLaunchKernelGGL(k_mg_up, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated,
+                           l == 0 ? reduce : -1, r.nb0, rsel);
+    if (reduce >= 0) hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, reduce, r.nb0, gated);
+}
+
+void launch_mg_build(const Run& r) {
+    const sp_levels_info& L = r.lv;
+    for (int l = 1; l <= L.tail; l++)
+        hipLaunchKernelGGL(k_mg_build, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l);
+    if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build, per_tank(r), dim3(1024), 0, r.stream, r.tb);
+}
+
+// The first solve builds the hierarchy and rescales the warm start; the second (after the
+// viscosity, `second`) reuses both and starts from what rhs_visc left. The level-0 residual
+// alternates between mgr and r2, the direction between cgp and cgp2, with the iteration's
+// parity (k_update_down0, k_cg_apply).
+void launch_pressure(const Run& r, uint32_t iters, bool second = false) {
+    const dim3 g0 = grid0(r);
+    if (!second) {
+        launch_mg_build(r);
+        hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, r.stream, r.tb);
+    }
+    hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, r.stream, r.tb, r.nb0);
+    launch_vcycle(r, 0, second ? (int)RED_START2 : (int)RED_START, 0);
+    for (uint32_t k = 0; k < iters; k++) {
+        const int par = (int)(k & 1);
+        hipLaunchKernelGGL(k_cg_apply, g0, TILE, 0, r.stream, r.tb, r.nb0, par);
+        hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, (int)RED_ALPHA, r.nb0, 1);
+        hipLaunchKernelGGL(k_update_down0, g0, TILE, 0, r.stream, r.tb, par);
+        launch_vcycle(r, 1, RED_BETA, par ^ 1, true);
+    }
+    hipLaunchKernelGGL(k_cg_end, per_tank(r), dim3(64), 0, r.stream, r.tb, second ? 1 : 0);
+}
+
+// ---- the implicit viscosity (record_viscosity_split / _solve, mod.rs:2812-2927) ----------
+
+// Weights and active tiles, the explicit step outside them, the Jacobi diagonal, the warm
+// start, and the multigrid's matrices.
+void launch_visc_split(const Run& r) {
+    const dim3 g0 = grid0(r);
+    hipLaunchKernelGGL(k_clear_vtiles, per_tank(r), dim3(256), 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, r.stream, r.tb, 1);
+    hipLaunchKernelGGL(k_visc_split, g0, TILE, 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_visc_diag, g0, TILE, 0, r.stream, r.tb);
+    if (r.gp.v_precond)
+        for (int l = 1; l <= r.gp.v_mg_levels; l++)
+            hipLaunchKernelGGL(k_vmg_build, groups_of(r, r.lv.dims[l][0], r.lv.dims[l][1]), TILE, 0, r.stream, r.tb, l);
+}
+
+void launch_visc_reduce(const Run& r, int mode) {
+    hipLaunchKernelGGL(k_visc_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, mode, r.nb0);
+}
+
+// The coarse levels of a viscosity V-cycle, down and up (record_visc_coarse, mod.rs:2838-2853).
+template <int NU>
+void launch_visc_coarse(const Run& r) {
+    const int n = r.gp.v_mg_levels;
+    for (int l = 1; l <= n; l++)
+        hipLaunchKernelGGL(k_vmg_down<NU>, groups_of(r, r.lv.dims[l][0], r.lv.dims[l][1]), TILE, 0, r.stream, r.tb, l,
+                           l == n ? 1 : 0);
+    for (int l = n; l >= 1; l--)
+        hipLaunchKernelGGL(k_vmg_up<NU>, groups_of(r, r.lv.dims[l][0], r.lv.dims[l][1]), TILE, 0, r.stream, r.tb, l,
+                           l == n ? 1 : 0);
+}
+
+// One V-cycle of the viscosity multigrid from vr: z, A z in vmz, vaz (the debug stage).
+template <int NU>
+void launch_visc_vcycle(const Run& r) {
+    const dim3 g0 = grid0(r);
+    hipLaunchKernelGGL(k_vmg_down<NU>, g0, TILE, 0, r.stream, r.tb, 0, 0);
+    launch_visc_coarse<NU>(r);
+    hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, r.stream, r.tb, 0);
+}
+
+// Multigrid-preconditioned CG: z of the starting residual and the first step length, then
+// per iteration the update with the way down on level 0, the coarse levels and the way up
+// with A z and both step lengths. Residual and A p alternate between (vr, vq) and
+// (vr2, vq2) with the iteration's parity.
+template <int NU>
+void launch_visc_mg(const Run& r, uint32_t iters) {
+    const dim3 g0 = grid0(r);
+    launch_visc_vcycle<NU>(r);
+    launch_visc_reduce(r, VRED_MG_START);
+    for (uint32_t k = 0; k < iters; k++) {
+        const int par = (int)(k & 1);
+        hipLaunchKernelGGL(k_visc_mgu<NU>, g0, TILE, 0, r.stream, r.tb, par);
+        launch_visc_reduce(r, VRED_MG_UPDATE);
+        launch_visc_coarse<NU>(r);
+        hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, r.stream, r.tb, par ^ 1);
+        launch_visc_reduce(r, VRED_MG_ITER);
+    }
+}
+
+// The implicit solve on the active tiles. `restrain`: the first pressure solve changed the
+// velocity since the split, so the strain of u* is taken again.
+void launch_visc_solve(const Run& r, uint32_t iters, bool restrain) {
+    const dim3 g0 = grid0(r);
+    if (restrain) hipLaunchKernelGGL(k_visc_strain, g0, TILE, 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_visc_start, g0, TILE, 0, r.stream, r.tb);
+    launch_visc_reduce(r, VRED_START);
+    if (r.gp.v_warm) {
+        hipLaunchKernelGGL(k_visc_warm, g0, TILE, 0, r.stream, r.tb);
+        launch_visc_reduce(r, VRED_WARM);
+    }
+    if (r.gp.v_precond) {
+        if (r.gp.v_mg_sweeps == 1) launch_visc_mg<1>(r, iters);
+        else launch_visc_mg<2>(r, iters);
+    } else {
+        for (uint32_t k = 0; k < iters; k++) {
+            const int par = (int)(k & 1);
+            hipLaunchKernelGGL(k_visc_iter, g0, TILE, 0, r.stream, r.tb, par);
+            launch_visc_reduce(r, VRED_ALPHA);
+            hipLaunchKernelGGL(k_visc_update, g0, TILE, 0, r.stream, r.tb, par);
+            launch_visc_reduce(r, VRED_BETA);
+        }
+    }
+    hipLaunchKernelGGL(k_visc_end, per_tank(r), dim3(64), 0, r.stream, r.tb);
+}
+
+bool twice_projected(const sp_gparams& gp) { return gp.visc_mode == 2 && gp.v_twice; }
+
+// The viscosity before the first pressure solve (record_substep, mod.rs:2681-2686).
+void launch_visc_before(const Run& r, const Recs& recs) {
+    const dim3 g0 = grid0(r);
+    if (r.gp.visc_mode == 1) {
+        hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, r.stream, r.tb, 0);
+        hipLaunchKernelGGL(k_visc_explicit, g0, TILE, 0, r.stream, r.tb);
+    } else if (r.gp.visc_mode == 2) {
+        launch_visc_split(r);
+        if (!r.gp.v_twice) launch_visc_solve(r, recs.v, false);
+    }
+}
+
+// After the first projection: the viscosity solve (mod.rs:2690-2691) ...
+void launch_visc_after(const Run& r, const Recs& recs) {
+    if (twice_projected(r.gp)) launch_visc_solve(r, recs.v, true);
+}
+
+// ... and the second pressure solve with its projection (mod.rs:2692-2697).
+void launch_second_projection(const Run& r, const Recs& recs) {
+    if (!twice_projected(r.gp)) return;
+    const dim3 g0 = grid0(r);
+    hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, r.stream, r.tb);
+    launch_pressure(r, recs.p2, true);
+    hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, r.stream, r.tb);
+}
+
+// ---- the other passes ------------------------------------------------------------------
+
+void launch_grid_passes(const Run& r, const Recs& recs) {
+    const dim3 g0 = grid0(r);
+    hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, r.stream, r.tb);
+    hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, r.stream, r.tb);
+    launch_visc_before(r, recs);
+    hipLaunchKernelGGL(k_rhs, g0, TILE, 0, r.stream, r.tb);
+}
+
+void launch_project(const Run& r) { hipLaunchKernelGGL(k_project, grid0(r), TILE, 0, r.stream, r.tb); }
+
+void launch_clear_acc(const Run& r) {
+    // a grid-stride fill of each tank's W H stride words; enough groups for the widest stride
+    const size_t words = (size_t)r.W * r.H * SP_ACC_MAX;
+    const int blocks = (int)std::min<size_t>((words / 4 + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_clear_acc, dim3(std::max(blocks, 1), 1, r.T), dim3(256), 0, r.stream, r.tb);
+}
+
+void launch_particles(const Run& r, int gather) {
+    const uint32_t blocks = std::max<uint32_t>((r.slots + 255) / 256, 1);
+    hipLaunchKernelGGL(k_particles, dim3(blocks, 1, r.T), dim3(256), 0, r.stream, r.tb, gather);
+}
+
+// What the host decides about a substep's launches from the tables (the parameters the
+// kernels read themselves stay out of the key).
+uint32_t schedule_key(const sp_gparams& g) {
+    return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
+           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10;
+}
+
+}  // namespace
+
+// ---- graphs and timing ------------------------------------------------------------------
+
+// Instantiated graphs of one (recorded iterations, schedule) key: grid passes (with the
+// viscosity before the first solve), pressure solve, projection, the viscosity solve after
+// it, and the second pressure solve with its projection.
+constexpr int GRAPH_PARTS = 5;
+struct Graphs {
+    Recs recs;
+    uint32_t sched = 0xffffffffu;
+    hipGraphExec_t exec[GRAPH_PARTS] = {};
+};
+
+struct GraphSet {
+    std::vector<Graphs> graphs;
+    bool on = true;
+
+    void clear() {
+        for (auto& g : graphs)
+            for (auto& e : g.exec)
+                if (e) {
+                    (void)hipGraphExecDestroy(e);
+                    e = nullptr;
+                }
+        graphs.clear();
+    }
+    ~GraphSet() { clear(); }
+};
+
+// Device time per substep of the phases of a substep, from events at their boundaries:
+// [0] total, [1] step + accumulator clear + particle pass, [2] grid passes (with the
+// viscosity before the first solve), [3] pressure solve, [4] projection, [5] viscosity
+// solve after it, [6] second pressure solve and projection; and the particle kernel alone.
+struct Timing {
+    static constexpr int SUBSTEPS = 64;
+    static constexpr int EV_PER = 9;
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    uint32_t timed = 0, timed_last = 0;
+    float phase[7] = {};
+    float particle_ms = 0.0f;
+
+    void create() {
+        ev.assign((size_t)SUBSTEPS * EV_PER, nullptr);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        on = true;
+    }
+    ~Timing() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    hipEvent_t* next() { return on && timed < (uint32_t)SUBSTEPS ? &ev[(size_t)timed++ * EV_PER] : nullptr; }
+    // Averages over the substeps recorded since the last call (after the stream finished).
+    void collect() {
+        if (!on || timed == 0) return;
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0}, kp = 0;
+        const int from[6] = {0, 1, 2, 3, 4, 7}, to[6] = {1, 2, 3, 4, 7, 8};
+        for (uint32_t k = 0; k < timed; k++) {
+            hipEvent_t* e = &ev[(size_t)k * EV_PER];
+            float t;
+            for (int q = 0; q < 6; q++) {
+                HIPCHK(hipEventElapsedTime(&t, e[from[q]], e[to[q]]));
+                acc[q + 1] += t;
+                acc[0] += t;
+            }
+            HIPCHK(hipEventElapsedTime(&t, e[5], e[6]));
+            kp += t;
+        }
+        for (int q = 0; q < 7; q++) phase[q] = (float)(acc[q] / timed);
+        particle_ms = (float)(kp / timed);
+        timed_last = timed;
+        timed = 0;
+    }
+};
+
+namespace {
+
+hipGraphExec_t capture(const Run& r, int part, const Recs& recs) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
+    if (part == 0) launch_grid_passes(r, recs);
+    else if (part == 1) launch_pressure(r, recs.p);
+    else if (part == 2) launch_project(r);
+    else if (part == 3) launch_visc_after(r, recs);
+    else launch_second_projection(r, recs);
+    hipError_t le = hipGetLastError();
+    HIPCHK(hipStreamEndCapture(r.stream, &g));
+    HIPCHK(le);
+    hipGraphExec_t e = nullptr;
+    HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return e;
+}
+
+Graphs& graphs_for(GraphSet& gs, const Run& r, const Recs& recs) {
+    const uint32_t key = schedule_key(r.gp);
+    for (auto& g : gs.graphs)
+        if (g.recs == recs && g.sched == key) return g;
+    if (gs.graphs.size() >= 12) gs.clear();
+    gs.graphs.emplace_back();
+    Graphs& g = gs.graphs.back();
+    g.recs = recs;
+    g.sched = key;
+    for (int k = 0; k < GRAPH_PARTS; k++) g.exec[k] = capture(r, k, recs);
+    return g;
+}
+
+// One whole substep of the group (record_substep, mod.rs:2615-2708). gather = 0 on the
+// first substep after a reset (no grid velocities to gather yet).
+void launch_substep(const Run& r, GraphSet* gs, Timing* tm, int gather, const Recs& recs) {
+    hipEvent_t* e = tm ? tm->next() : nullptr;
+    Graphs* g = gs && gs->on ? &graphs_for(*gs, r, recs) : nullptr;
+    if (e) HIPCHK(hipEventRecord(e[0], r.stream));
+    hipLaunchKernelGGL(k_step, per_tank(r), dim3(64), 0, r.stream, r.tb);
+    launch_clear_acc(r);
+    if (e) HIPCHK(hipEventRecord(e[5], r.stream));
+    launch_particles(r, gather);
+    if (e) HIPCHK(hipEventRecord(e[6], r.stream));
+    if (e) HIPCHK(hipEventRecord(e[1], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[0], r.stream));
+    else launch_grid_passes(r, recs);
+    if (e) HIPCHK(hipEventRecord(e[2], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[1], r.stream));
+    else launch_pressure(r, recs.p);
+    if (e) HIPCHK(hipEventRecord(e[3], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[2], r.stream));
+    else launch_project(r);
+    if (e) HIPCHK(hipEventRecord(e[4], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[3], r.stream));
+    else launch_visc_after(r, recs);
+    if (e) HIPCHK(hipEventRecord(e[7], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(g->exec[4], r.stream));
+    else launch_second_projection(r, recs);
+    if (e) HIPCHK(hipEventRecord(e[8], r.stream));
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace

@@ -38,37 +38,58 @@ SP_DEV bool solve_done(const Ctx& c) { return c.K->done[0] != 0u && c.K->done[1]
 SP_DEV int tid2() { return threadIdx.y * 16 + threadIdx.x; }
 SP_DEV int tile_slot() { return blockIdx.y * gridDim.x + blockIdx.x; }
 
-// FP64 sum of one double4 per work-item over a 256 work-item group (fixed tree).
+SP_DEV void add4(double4& a, const double4& b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    a.w += b.w;
+}
+
+// FP64 sum of one double4 per work-item over a group of NT work-items in a fixed order: a
+// butterfly across each wave64 (x + y = y + x exactly, so every lane holds the same sum),
+// then the waves in order. Every work-item gets the total; `lds` (NT / 64 entries) may be
+// reused once the call returns. Every work-item of the group must call it.
+template <int NT = 256>
 SP_DEV double4 group_sum4(double4 v, double4* lds) {
-    const int t = tid2();
-    lds[t] = v;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) {
-            double4 a = lds[t], b = lds[t + s];
-            lds[t] = make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-        }
-        __syncthreads();
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o);
+        v.y += __shfl_xor(v.y, o);
+        v.z += __shfl_xor(v.z, o);
+        v.w += __shfl_xor(v.w, o);
     }
-    return lds[0];
+    if ((t & 63) == 0) lds[t >> 6] = v;
+    __syncthreads();
+    double4 s = lds[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) add4(s, lds[w]);
+    __syncthreads();
+    return s;
 }
 
 SP_DEV void store_part(double4* p, double4 v) { *p = v; }
 
-// Sum of parts[base .. base + n) by one 256 work-item group in a fixed order.
+// Sum of parts[base .. base + n) by one group of NT work-items in a fixed order; each
+// work-item issues its loads four at a time before adding, so their latencies overlap.
+template <int NT>
 SP_DEV double4 total_of(const double4* parts, int base, int n, double4* lds) {
-    const int t = tid2();
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
     double4 s = make_double4(0, 0, 0, 0);
-    for (int k = t; k < n; k += 256) {
-        double4 v = parts[base + k];
-        s.x += v.x;
-        s.y += v.y;
-        s.z += v.z;
-        s.w += v.w;
+    for (int k0 = 0; k0 < n; k0 += 4 * NT) {
+        double4 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + j * NT + t;
+            v[j] = k < n ? parts[base + k] : make_double4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) add4(s, v[j]);
     }
-    return group_sum4(s, lds);
+    return group_sum4<NT>(s, lds);
 }
-SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) { return total_of(c.part_sums, base, n, lds); }
+constexpr int REDUCE_NT = 256;  // work-items of the one-group reduction kernels
+SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) { return total_of<REDUCE_NT>(c.part_sums, base, n, lds); }
 
 // Modes 0, 1, 2 of reduce_tail.glsl, and 8: the start of the second solve (after the
 // viscosity), where the density-correction potential counts as solved.
@@ -154,7 +175,8 @@ SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
 }
 
 // mg_build.comp:16-65: coarse level l from level l - 1, over coarse 16 x 16 tiles.
-__global__ __launch_bounds__(256) void k_mg_build(Ctx c, int l) {
+__global__ __launch_bounds__(256) void k_mg_build(const TankBatch tb, int l) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ uint32_t s_state;
     const Lv F = level(c, l - 1), C = level(c, l);
     if (tid2() == 0) {
@@ -183,7 +205,8 @@ __global__ __launch_bounds__(256) void k_mg_build(Ctx c, int l) {
 }
 
 // mg_tail.comp mode 1: the coarse matrices of every level below the tail, one group.
-__global__ __launch_bounds__(1024) void k_mg_tail_build(Ctx c) {
+__global__ __launch_bounds__(1024) void k_mg_tail_build(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     for (int l = c.tail; l < c.n_levels - 1; l++) {
         const Lv F = level(c, l), C = level(c, l + 1);
         for (int t = threadIdx.x; t < C.w * C.h; t += 1024) build_cell(c, F, C, t % C.w, t / C.w);
@@ -197,7 +220,8 @@ __global__ __launch_bounds__(1024) void k_mg_tail_build(Ctx c) {
 // cg_scale.comp:14-27: rescale the warm start to this substep's length. With a second solve
 // after the viscosity that warm-starts from its own part (mode 1), the first starts from
 // P1 = P - P2 and P2 is kept; otherwise P2 is forgotten.
-__global__ __launch_bounds__(256) void k_cg_scale(Ctx c) {
+__global__ __launch_bounds__(256) void k_cg_scale(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (c.ptile[tile_slot()] == 0u) return;
     const Lv F = level(c, 0);
     int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
@@ -214,8 +238,9 @@ __global__ __launch_bounds__(256) void k_cg_scale(Ctx c) {
 }
 
 // cg_init.comp: r = b - A x and the partial sums of b.b (into the second half of parts).
-__global__ __launch_bounds__(256) void k_cg_init(Ctx c, int nblocks) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(256) void k_cg_init(const TankBatch tb, int nblocks) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    __shared__ double4 lds[4];
     const int slot = tile_slot();
     if (c.ptile[slot] == 0u) {
         if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
@@ -278,8 +303,12 @@ SP_DEV float2 direction_of(float2 z, float2 po, float bx, float by) {
 
 // cg_apply.comp: p = z + beta p_old, q = A p, and the partial sums of p.q (second half of
 // the partial buffer).
-__global__ __launch_bounds__(256) void k_cg_apply(Ctx c, int nblocks, const float2* __restrict__ p_old, float2* __restrict__ p_new) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblocks, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    // the search direction alternates between cgp and cgp2 with the iteration's parity
+    const float2* __restrict__ p_old = par ? c.cgp2 : c.cgp;
+    float2* __restrict__ p_new = par ? c.cgp : c.cgp2;
+    __shared__ double4 lds[4];
     __shared__ float sd[R1 * R1];
     __shared__ float2 sp[R1 * R1];
     if (solve_done(c)) return;
@@ -421,7 +450,9 @@ SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int x0, int y0, DownStage& S
 
 // mg_down.comp:36-69 on level l: pre-smoothing from zero and restriction of the residual.
 // r_in: the level's right-hand side (level 0: the CG residual buffer of this iteration).
-__global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated, const float2* __restrict__ r_in) {
+__global__ __launch_bounds__(256) void k_mg_down(const TankBatch tb, int l, int gated, int rsel) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
     __shared__ DownStage S;
     if (gated && solve_done(c)) return;
     const Lv F = level(c, l), C = level(c, l + 1);
@@ -440,8 +471,12 @@ __global__ __launch_bounds__(256) void k_mg_down(Ctx c, int l, int gated, const 
 // cg_update.comp fused with the way down on level 0: x += alpha p, r_out = r_in - alpha q
 // (residual buffers alternate between iterations, so a tile may read its neighbours' old
 // residual while they write their new one), then mg_down level 0 on r_out.
-__global__ __launch_bounds__(256) void k_update_down0(Ctx c, const float2* __restrict__ r_in, float2* __restrict__ r_out,
-                                                      const float2* __restrict__ p_new) {
+__global__ __launch_bounds__(256) void k_update_down0(const TankBatch tb, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    // residual (mgr / r2) and direction (cgp / cgp2) buffers of iteration parity `par`
+    const float2* __restrict__ r_in = par ? c.r2 : c.mgr;
+    float2* __restrict__ r_out = par ? c.mgr : c.r2;
+    const float2* __restrict__ p_new = par ? c.cgp : c.cgp2;
     __shared__ DownStage S;
     if (solve_done(c)) return;
     const Lv F = level(c, 0), C = level(c, 1);
@@ -486,8 +521,10 @@ __global__ __launch_bounds__(256) void k_update_down0(Ctx c, const float2* __res
 
 // mg_up.comp:38-75 on level l: coarse-grid correction and the black-then-red
 // post-smoothing; on level 0 (reduce >= 0) also the partial sums r.z, r.r.
-__global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int reduce, int nblocks, const float2* __restrict__ r_in) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int gated, int reduce, int nblocks, int rsel) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
+    __shared__ double4 lds[4];
     __shared__ float sd[R2 * R2];
     __shared__ float2 sx[R2 * R2];
     __shared__ float4 sa[R1 * R1];
@@ -593,8 +630,10 @@ __global__ __launch_bounds__(256) void k_mg_up(Ctx c, int l, int gated, int redu
     } while (0)
 
 // LDS (dynamic, sized at launch to the hierarchy): per tail cell its couplings (float4),
-// iterate (float2) and diagonal; the right-hand sides of all but the first tail level.
-__global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
+// iterate and right-hand side (float2) and diagonal; the first level's right-hand side is
+// loaded once with the matrix, so that the sweeps never wait on global memory.
+__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     extern __shared__ float4 tail_lds[];
     __shared__ int4 sl[SP_MAX_LEVELS];
     if (gated && solve_done(c)) return;
@@ -606,16 +645,17 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
     float4* sa = tail_lds;
     float2* sx = reinterpret_cast<float2*>(sa + total);
     float2* sr = sx + total;
-    float* sd = reinterpret_cast<float*>(sr + (total - NF));
+    float* sd = reinterpret_cast<float*>(sr + total);
     const int bt = threadIdx.x;
     for (int t = bt; t < total; t += 1024) {
         sd[t] = c.mgd[BASE + t];
         sa[t] = c.mga[BASE + t];
+        if (t < NF) sr[t] = c.mgr[BASE + t];
     }
     if (bt <= last) sl[bt] = c.lev[bt];
     __syncthreads();
 
-    auto rhs_at = [&](int s) -> float2 { return s < NF ? c.mgr[BASE + s] : sr[s - NF]; };
+    auto rhs_at = [&](int s) -> float2 { return sr[s]; };
     auto sat = [&](int4 L, int x, int y) { return L.x - BASE + y * L.y + x; };
     auto sunk = [&](int4 L, int x, int y) { return x >= 0 && y >= 0 && x < L.y && y < L.z && sd[sat(L, x, y)] > 0.0f; };
     auto sweep = [&](int4 L, int colour, bool zero, int t0, int step) {
@@ -666,7 +706,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
                 sum.x += r.x - az.x;
                 sum.y += r.y - az.y;
             }
-            sr[C.x - BASE + t - NF] = sum;
+            sr[C.x - BASE + t] = sum;
         }
     };
     auto post_black = [&](int4 F, int4 C, int t0, int step) {
@@ -739,15 +779,17 @@ __global__ __launch_bounds__(1024) void k_mg_tail(Ctx c, int gated) {
 // The reference folds this into the last work-group of the producing kernel; here the
 // kernel boundary orders the partials instead, which on a multi-XCD part is far cheaper
 // than a device-scope release fence in every producing work-group.
-__global__ __launch_bounds__(256) void k_cg_reduce(Ctx c, int mode, int nblocks, int gated) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(REDUCE_NT) void k_cg_reduce(const TankBatch tb, int mode, int nblocks, int gated) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    __shared__ double4 lds[REDUCE_NT / 64];
     if (gated && solve_done(c)) return;
     reduce_scalars(c, mode, nblocks, lds);
 }
 
 // Solve statistics at the end of a solve (reduce_tail.glsl:105-116, modes 3 and 9: the
 // second solve keeps its own iteration count).
-__global__ void k_cg_end(Ctx c, int second) {
+__global__ void k_cg_end(const TankBatch tb, int second) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sp_solve* K = c.K;
     uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];

@@ -1,6 +1,7 @@
-// sprout.hip — libsprout_hip.so: the device state of one tank, the substep schedule on a
-// HIP stream, and the C ABI of sprout_hip.h (the boundary that replaces FluidSim's
-// Vulkano dispatch, src/sim/mod.rs:862-3492).
+// sprout.hip — libsprout_hip.so: the device state of one tank and the C ABI of
+// sprout_hip.h (the boundary that replaces FluidSim's Vulkano dispatch,
+// src/sim/mod.rs:862-3492). The substep schedule is in schedule.hip, batches of tanks in
+// batch.hip.
 //
 // One translation unit: the kernel files are included here so that the kernels and the
 // launch code see each other's inline helpers.
@@ -21,21 +22,11 @@
 #include "particles.hip"
 #include "solve.hip"
 #include "visc.hip"
+#include "schedule.hip"
 
 namespace {
 
 thread_local std::string g_create_error;
-
-struct HipError {
-    hipError_t e;
-    const char* what;
-};
-
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) throw HipError{e_, #x}; \
-    } while (0)
 
 template <class T>
 T* dalloc(size_t n, std::vector<void*>& owned) {
@@ -48,23 +39,6 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 }
 
 }  // namespace
-
-// Recorded iterations of one substep: first pressure solve, second pressure solve (after
-// the viscosity), viscosity solve (record_substep, mod.rs:2615-2623).
-struct Recs {
-    uint32_t p = 0, p2 = 0, v = 0;
-    bool operator==(const Recs& o) const { return p == o.p && p2 == o.p2 && v == o.v; }
-};
-
-// Instantiated graphs of one (recorded iterations, schedule) key: grid passes (with the
-// viscosity before the first solve), pressure solve, projection, the viscosity solve after
-// it, and the second pressure solve with its projection.
-constexpr int GRAPH_PARTS = 5;
-struct Graphs {
-    Recs recs;
-    uint32_t sched = 0xffffffffu;
-    hipGraphExec_t exec[GRAPH_PARTS] = {};
-};
 
 struct sp_sim {
     int W = 0, H = 0, N = 0;
@@ -85,9 +59,9 @@ struct sp_sim {
     size_t tail_lds_bytes = 0;  // dynamic LDS of k_mg_tail
 
     // device
-    Ctx c{};
-    sp_gparams* d_P = nullptr;
-    sp_gmat* d_mats = nullptr;
+    TankRef ref{};            // the tank's arena and sizes
+    Ctx c{};                  // its buffers (host copy of what every kernel rebuilds)
+    Run run;                  // the launch group of this tank alone
     sp_particle* d_part_tmp = nullptr;
     sp_particle* d_spawn = nullptr;
     uint32_t* d_counter = nullptr;  // stats / claim counters
@@ -96,7 +70,6 @@ struct sp_sim {
     uint32_t* d_vals = nullptr;
     uint32_t* d_keys2 = nullptr;
     uint32_t* d_vals2 = nullptr;
-    float2* d_r2 = nullptr;  // second level-0 residual buffer of the CG
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
 
@@ -116,20 +89,9 @@ struct sp_sim {
     uint32_t last_unconv = 0, last_p2_unconv = 0, last_v_unconv = 0;
     uint32_t last_v_iters = 0, last_p2_iters = 0;  // of the latest solves
 
-    // timing: per substep of a batch (up to TIMED_SUBSTEPS), events at the phase
-    // boundaries and tightly around the particle kernel
-    static constexpr int TIMED_SUBSTEPS = 64;
-    static constexpr int EV_PER = 9;
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    uint32_t timed = 0;  // substeps of the current batch with events recorded
-    uint32_t timed_last = 0;
-    float timings[10] = {};
-    float particle_ms = 0.0f;
-    bool use_graphs = true;
-    std::vector<Graphs> graphs;
-
-    Ctx ctx() const { return c; }
+    GraphSet gs;
+    Timing tm;
+    bool in_batch = false;  // a batch drives the tank (sp_batch_*)
 };
 
 namespace {
@@ -143,198 +105,10 @@ void set_err(sp_sim* s, const char* fmt, ...) {
     s->err = buf;
 }
 
-dim3 grid_groups(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16, 1); }
-const dim3 TILE(16, 16, 1);
-
-// ---- the substep schedule (record_substep, mod.rs:2615-2708) ----------------------------
-
-// One V-cycle on the level-0 residual r0 (record_vcycle, mod.rs:2930-2954); with
-// `skip_down0` the way down on level 0 has already run (fused into the CG update).
-void launch_vcycle(sp_sim* s, int gated, int reduce, const float2* r0, bool skip_down0 = false) {
-    const Ctx& c = s->c;
-    const sp_levels_info& L = s->lv;
-    for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
-        hipLaunchKernelGGL(k_mg_down, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated, r0);
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), s->tail_lds_bytes, s->stream, c, gated);
-    for (int l = L.tail - 1; l >= 0; l--)
-        hipLaunchKernelGGL(k_mg_up, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l, gated,
-                           l == 0 ? reduce : -1, s->nb0, r0);
-    if (reduce >= 0) hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(256), 0, s->stream, c, reduce, s->nb0, gated);
-}
-
-void launch_mg_build(sp_sim* s) {
-    const Ctx& c = s->c;
-    const sp_levels_info& L = s->lv;
-    for (int l = 1; l <= L.tail; l++)
-        hipLaunchKernelGGL(k_mg_build, grid_groups(L.dims[l][0], L.dims[l][1]), TILE, 0, s->stream, c, l);
-    if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build, dim3(1), dim3(1024), 0, s->stream, c);
-}
-
-// record_pressure + record_cg (mod.rs:2956-3010). The first solve builds the hierarchy and
-// rescales the warm start; the second (after the viscosity, `second`) reuses both and
-// starts from what rhs_visc.comp left. The level-0 residual alternates between mgr and r2
-// from one iteration to the next (k_update_down0).
-void launch_pressure(sp_sim* s, uint32_t iters, bool second = false) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    if (!second) {
-        launch_mg_build(s);
-        hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, s->stream, c);
-    }
-    hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, s->stream, c, s->nb0);
-    float2* rb[2] = {c.mgr, s->d_r2};
-    launch_vcycle(s, 0, second ? (int)RED_START2 : (int)RED_START, rb[0]);
-    float2* bufs[2] = {c.cgp, c.cgp2};
-    for (uint32_t k = 0; k < iters; k++) {
-        const float2* p_old = bufs[k & 1];
-        float2* p_new = bufs[(k + 1) & 1];
-        hipLaunchKernelGGL(k_cg_apply, g0, TILE, 0, s->stream, c, s->nb0, p_old, p_new);
-        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(256), 0, s->stream, c, 1, s->nb0, 1);
-        hipLaunchKernelGGL(k_update_down0, g0, TILE, 0, s->stream, c, (const float2*)rb[k & 1], rb[(k + 1) & 1],
-                           (const float2*)p_new);
-        launch_vcycle(s, 1, 2, rb[(k + 1) & 1], true);
-    }
-    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s->stream, c, second ? 1 : 0);
-}
-
-// ---- the implicit viscosity (record_viscosity_split / _solve, mod.rs:2812-2927) ----------
-
-// Weights and active tiles, the explicit step outside them, the Jacobi diagonal, the warm
-// start, and the multigrid's matrices.
-void launch_visc_split(sp_sim* s) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::max((s->nb0 / 4 + 255) / 256, 1)), dim3(256), 0, s->stream, c.vtiles,
-                       (size_t)s->nb0, 0u);
-    hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c, 1);
-    hipLaunchKernelGGL(k_visc_split, g0, TILE, 0, s->stream, c);
-    hipLaunchKernelGGL(k_visc_diag, g0, TILE, 0, s->stream, c);
-    if (s->gp.v_precond)
-        for (int l = 1; l <= s->gp.v_mg_levels; l++)
-            hipLaunchKernelGGL(k_vmg_build, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l);
-}
-
-void launch_visc_reduce(sp_sim* s, int mode) {
-    hipLaunchKernelGGL(k_visc_reduce, dim3(1), dim3(256), 0, s->stream, s->c, mode, s->nb0);
-}
-
-// The coarse levels of a viscosity V-cycle, down and up (record_visc_coarse, mod.rs:2838-2853).
-template <int NU>
-void launch_visc_coarse(sp_sim* s) {
-    const Ctx& c = s->c;
-    const int n = s->gp.v_mg_levels;
-    for (int l = 1; l <= n; l++)
-        hipLaunchKernelGGL(k_vmg_down<NU>, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l,
-                           l == n ? 1 : 0);
-    for (int l = n; l >= 1; l--)
-        hipLaunchKernelGGL(k_vmg_up<NU>, grid_groups(s->lv.dims[l][0], s->lv.dims[l][1]), TILE, 0, s->stream, c, l,
-                           l == n ? 1 : 0);
-}
-
-// Multigrid-preconditioned CG: z of the starting residual and the first step length, then
-// per iteration the update with the way down on level 0, the coarse levels and the way up
-// with A z and both step lengths. Residual and A p alternate between two buffers.
-template <int NU>
-void launch_visc_mg(sp_sim* s, uint32_t iters) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    hipLaunchKernelGGL(k_vmg_down<NU>, g0, TILE, 0, s->stream, c, 0, 0);
-    launch_visc_coarse<NU>(s);
-    hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
-    launch_visc_reduce(s, VRED_MG_START);
-    float2* rb[2] = {c.vr, c.vr2};
-    float2* qb[2] = {c.vq, c.vq2};
-    for (uint32_t k = 0; k < iters; k++) {
-        const int a = k & 1, b = a ^ 1;
-        hipLaunchKernelGGL(k_visc_mgu<NU>, g0, TILE, 0, s->stream, c, (const float2*)rb[a], (const float2*)qb[a], rb[b], qb[b]);
-        launch_visc_reduce(s, VRED_MG_UPDATE);
-        launch_visc_coarse<NU>(s);
-        hipLaunchKernelGGL(k_visc_mgz<NU>, g0, TILE, 0, s->stream, c, (const float2*)rb[b], (const float2*)qb[b]);
-        launch_visc_reduce(s, VRED_MG_ITER);
-    }
-}
-
-// The implicit solve on the active tiles. `restrain`: the first pressure solve changed the
-// velocity since the split, so the strain of u* is taken again.
-void launch_visc_solve(sp_sim* s, uint32_t iters, bool restrain) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    if (restrain) hipLaunchKernelGGL(k_visc_strain, g0, TILE, 0, s->stream, c);
-    hipLaunchKernelGGL(k_visc_start, g0, TILE, 0, s->stream, c);
-    launch_visc_reduce(s, VRED_START);
-    if (s->gp.v_warm) {
-        hipLaunchKernelGGL(k_visc_warm, g0, TILE, 0, s->stream, c);
-        launch_visc_reduce(s, VRED_WARM);
-    }
-    if (s->gp.v_precond) {
-        if (s->gp.v_mg_sweeps == 1) launch_visc_mg<1>(s, iters);
-        else launch_visc_mg<2>(s, iters);
-    } else {
-        float2* pb[2] = {c.vp, c.vp2};
-        for (uint32_t k = 0; k < iters; k++) {
-            const int a = k & 1, b = a ^ 1;
-            hipLaunchKernelGGL(k_visc_iter, g0, TILE, 0, s->stream, c, (const float2*)pb[a], pb[b]);
-            launch_visc_reduce(s, VRED_ALPHA);
-            hipLaunchKernelGGL(k_visc_update, g0, TILE, 0, s->stream, c, (const float2*)pb[b]);
-            launch_visc_reduce(s, VRED_BETA);
-        }
-    }
-    hipLaunchKernelGGL(k_visc_end, dim3(1), dim3(64), 0, s->stream, c);
-}
-
-// The viscosity before the first pressure solve (record_substep, mod.rs:2681-2686).
-void launch_visc_before(sp_sim* s, const Recs& r) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    if (s->gp.visc_mode == 1) {
-        hipLaunchKernelGGL(k_visc_setup, g0, TILE, 0, s->stream, c, 0);
-        hipLaunchKernelGGL(k_visc_explicit, g0, TILE, 0, s->stream, c);
-    } else if (s->gp.visc_mode == 2) {
-        launch_visc_split(s);
-        if (!s->gp.v_twice) launch_visc_solve(s, r.v, false);
-    }
-}
-
-bool twice_projected(const sp_sim* s) { return s->gp.visc_mode == 2 && s->gp.v_twice; }
-
-// After the first projection: the viscosity solve (mod.rs:2690-2691) ...
-void launch_visc_after(sp_sim* s, const Recs& r) {
-    if (twice_projected(s)) launch_visc_solve(s, r.v, true);
-}
-
-// ... and the second pressure solve with its projection (mod.rs:2692-2697).
-void launch_second_projection(sp_sim* s, const Recs& r) {
-    if (!twice_projected(s)) return;
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, c);
-    launch_pressure(s, r.p2, true);
-    hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, c);
-}
-
-void launch_grid_passes(sp_sim* s, const Recs& r) {
-    const Ctx& c = s->c;
-    dim3 g0 = grid_groups(s->W, s->H);
-    hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, s->stream, c);
-    hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, c);
-    hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, s->stream, c);
-    hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, s->stream, c);
-    launch_visc_before(s, r);
-    hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, c);
-}
-
-void launch_clear_acc(sp_sim* s) {
-    size_t n = (size_t)s->N * s->gp.acc_stride;
-    int blocks = (int)std::min<size_t>((n / 4 + 255) / 256 + 1, 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s->stream, (uint32_t*)s->c.acc, n, 0u);
-}
-
-void launch_particles(sp_sim* s, int gather) {
-    uint32_t blocks = std::max<uint32_t>((s->n_slots + 255) / 256, 1);
-    hipLaunchKernelGGL(k_particles, dim3(blocks), dim3(256), 0, s->stream, s->c, gather);
-}
-
-void launch_sort(sp_sim* s, int tiled) {
+// The counting sort of the particles by cell (record_sort, mod.rs:3223-3242), on `stream`:
+// keys are the cell (or the 8 x 8 tile, then the cell, with `tiled`), dead slots last;
+// stable by the previous slot.
+void launch_sort(sp_sim* s, int tiled, hipStream_t stream) {
     if (s->n_slots == 0) return;
     uint32_t blocks = (s->n_slots + 255) / 256;
     // keys below 2^end_bit; dead slots take the largest, so they collect at the end
@@ -342,97 +116,13 @@ void launch_sort(sp_sim* s, int tiled) {
     int end_bit = 1;
     while (end_bit < 32 && (1u << end_bit) <= live_max) end_bit++;
     uint32_t dead = end_bit >= 32 ? 0xffffffffu : (1u << end_bit) - 1u;
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, s->stream, s->c, s->d_keys, s->d_vals, tiled, dead);
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, stream, s->run.tb, s->d_keys, s->d_vals, tiled, dead);
     size_t bytes = s->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(s->d_sort_tmp, bytes, s->d_keys, s->d_keys2, s->d_vals, s->d_vals2,
-                                              (int)s->n_slots, 0, end_bit, s->stream));
-    hipLaunchKernelGGL(k_sort_gather, dim3(blocks), dim3(256), 0, s->stream, s->c, (const uint32_t*)s->d_vals2,
+                                              (int)s->n_slots, 0, end_bit, stream));
+    hipLaunchKernelGGL(k_sort_gather, dim3(blocks), dim3(256), 0, stream, s->run.tb, (const uint32_t*)s->d_vals2,
                        s->d_part_tmp, s->n_slots);
-    HIPCHK(hipMemcpyAsync(s->c.part, s->d_part_tmp, sizeof(sp_particle) * s->n_slots, hipMemcpyDeviceToDevice, s->stream));
-}
-
-// The grid passes, the pressure solve with `p_iters` recorded iterations, and the
-// projection as three HIP graphs, captured once per (iterations, viscosity mode) and
-// replayed: a substep is some two hundred dependent launches, which the host could not
-// issue as fast as the device retires them. The kernels read everything that changes
-// between substeps (dt, tables, solve state) from device memory, so a replay needs no
-// new arguments. Events between the graphs time the phases.
-void destroy_graphs(sp_sim* s) {
-    for (auto& g : s->graphs)
-        for (auto& e : g.exec)
-            if (e) {
-                (void)hipGraphExecDestroy(e);
-                e = nullptr;
-            }
-    s->graphs.clear();
-}
-
-hipGraphExec_t capture(sp_sim* s, int part, const Recs& r) {
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    if (part == 0) launch_grid_passes(s, r);
-    else if (part == 1) launch_pressure(s, r.p);
-    else if (part == 2) hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, s->c);
-    else if (part == 3) launch_visc_after(s, r);
-    else launch_second_projection(s, r);
-    hipError_t le = hipGetLastError();
-    HIPCHK(hipStreamEndCapture(s->stream, &g));
-    HIPCHK(le);
-    hipGraphExec_t e = nullptr;
-    HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(g));
-    return e;
-}
-
-// What the host decides about a substep's launches from the tables (the parameters the
-// kernels read themselves stay out of the key).
-uint32_t schedule_key(const sp_sim* s) {
-    const sp_gparams& g = s->gp;
-    return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
-           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10;
-}
-
-Graphs& graphs_for(sp_sim* s, const Recs& r) {
-    const uint32_t key = schedule_key(s);
-    for (auto& g : s->graphs)
-        if (g.recs == r && g.sched == key) return g;
-    if (s->graphs.size() >= 12) destroy_graphs(s);
-    s->graphs.emplace_back();
-    Graphs& g = s->graphs.back();
-    g.recs = r;
-    g.sched = key;
-    for (int k = 0; k < GRAPH_PARTS; k++) g.exec[k] = capture(s, k, r);
-    return g;
-}
-
-void launch_substep(sp_sim* s, int gather, const Recs& r) {
-    const Ctx& c = s->c;
-    hipEvent_t* e = nullptr;
-    if (s->timing && s->timed < (uint32_t)sp_sim::TIMED_SUBSTEPS) e = &s->ev[(size_t)s->timed++ * sp_sim::EV_PER];
-    Graphs* g = s->use_graphs ? &graphs_for(s, r) : nullptr;
-    if (e) HIPCHK(hipEventRecord(e[0], s->stream));
-    hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c);
-    launch_clear_acc(s);
-    if (e) HIPCHK(hipEventRecord(e[5], s->stream));
-    launch_particles(s, gather);
-    if (e) HIPCHK(hipEventRecord(e[6], s->stream));
-    if (e) HIPCHK(hipEventRecord(e[1], s->stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[0], s->stream));
-    else launch_grid_passes(s, r);
-    if (e) HIPCHK(hipEventRecord(e[2], s->stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[1], s->stream));
-    else launch_pressure(s, r.p);
-    if (e) HIPCHK(hipEventRecord(e[3], s->stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[2], s->stream));
-    else hipLaunchKernelGGL(k_project, grid_groups(s->W, s->H), TILE, 0, s->stream, c);
-    if (e) HIPCHK(hipEventRecord(e[4], s->stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[3], s->stream));
-    else launch_visc_after(s, r);
-    if (e) HIPCHK(hipEventRecord(e[7], s->stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[4], s->stream));
-    else launch_second_projection(s, r);
-    if (e) HIPCHK(hipEventRecord(e[8], s->stream));
-    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->c.part, s->d_part_tmp, sizeof(sp_particle) * s->n_slots, hipMemcpyDeviceToDevice, stream));
 }
 
 // Uploads the grid-unit tables if the parameters or the tank changed.
@@ -444,10 +134,12 @@ int upload_tables(sp_sim* s) {
         set_err(s, "%s", why.c_str());
         return rc;
     }
-    HIPCHK(hipMemcpyAsync(s->d_P, &s->gp, sizeof(sp_gparams), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->d_mats, s->gm, sizeof(s->gm), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync((void*)s->c.P, &s->gp, sizeof(sp_gparams), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync((void*)s->c.mats, s->gm, sizeof(s->gm), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->tables_dirty = false;
+    s->run.gp = s->gp;
+    s->run.slots = s->n_slots;
     return SP_OK;
 }
 
@@ -529,8 +221,8 @@ int record(sp_sim* s, uint32_t substeps, float target) {
     if (rc != SP_OK) return rc;
     if (claim > 0) {
         HIPCHK(hipMemsetAsync(s->d_counter, 0, sizeof(uint32_t) * 4, s->stream));
-        hipLaunchKernelGGL(k_claim, dim3((s->n_slots + 255) / 256), dim3(256), 0, s->stream, s->c, (const sp_particle*)s->d_spawn,
-                           claim, s->d_counter);
+        hipLaunchKernelGGL(k_claim, dim3((s->n_slots + 255) / 256), dim3(256), 0, s->stream, s->run.tb,
+                           (const sp_particle*)s->d_spawn, claim, s->d_counter);
     }
     if (substeps > 0) {
         // StepBuf words 0, 2, 3: target time, recorded substeps, done (record, mod.rs:2521-2526)
@@ -544,11 +236,11 @@ int record(sp_sim* s, uint32_t substeps, float target) {
     }
     for (uint32_t k = 0; k < substeps; k++) {
         if (s->since_sort >= s->sort_interval) {
-            launch_sort(s, 1);
+            launch_sort(s, 1, s->stream);
             s->since_sort = 0;
         }
         s->since_sort++;
-        launch_substep(s, s->grid_valid ? 1 : 0, s->rec);
+        launch_substep(s->run, &s->gs, &s->tm, s->grid_valid ? 1 : 0, s->rec);
         s->grid_valid = true;
     }
     return SP_OK;
@@ -556,38 +248,8 @@ int record(sp_sim* s, uint32_t substeps, float target) {
 
 int wait(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (s->timing && s->timed > 0) {
-        // averages per substep: [0] total, [1] step + clear + particle pass, [2] grid passes
-        // (with the viscosity before the first solve), [3] pressure solve, [4] projection,
-        // [5] viscosity solve after it, [6] second pressure solve and projection; [7]-[9]
-        // below are iteration counts
-        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t k = 0; k < s->timed; k++) {
-            hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
-            const int from[6] = {0, 1, 2, 3, 4, 7}, to[6] = {1, 2, 3, 4, 7, 8};
-            float t;
-            for (int q = 0; q < 6; q++) {
-                HIPCHK(hipEventElapsedTime(&t, e[from[q]], e[to[q]]));
-                acc[q + 1] += t;
-                acc[0] += t;
-            }
-        }
-        for (int q = 0; q < 7; q++) s->timings[q] = (float)(acc[q] / s->timed);
-        double kp = 0;
-        for (uint32_t k = 0; k < s->timed; k++) {
-            hipEvent_t* e = &s->ev[(size_t)k * sp_sim::EV_PER];
-            float t;
-            HIPCHK(hipEventElapsedTime(&t, e[5], e[6]));
-            kp += t;
-        }
-        s->particle_ms = (float)(kp / s->timed);
-        s->timed_last = s->timed;
-        s->timed = 0;
-    }
+    s->tm.collect();
     readback(s);
-    s->timings[7] = (float)s->stats.pressure_iters_last;
-    s->timings[8] = (float)s->last_v_iters;
-    s->timings[9] = (float)s->last_p2_iters;
     return SP_OK;
 }
 
@@ -679,95 +341,36 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         s->device = cfg->device;
         HIPCHK(hipSetDevice(s->device));
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        s->ev.assign((size_t)sp_sim::TIMED_SUBSTEPS * sp_sim::EV_PER, nullptr);
-        for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        s->tm.create();
         sp::levels(s->W, s->H, s->lv);
         s->nb0 = ((s->W + 15) / 16) * ((s->H + 15) / 16);
         {
             const sp_levels_info& L = s->lv;
             size_t total = (size_t)(L.offs[L.n - 1] + L.dims[L.n - 1][0] * L.dims[L.n - 1][1] - L.offs[L.tail]);
             size_t nf = (size_t)L.dims[L.tail][0] * L.dims[L.tail][1];
-            s->tail_lds_bytes = total * (16 + 8 + 4) + (total - nf) * 8;
+            s->tail_lds_bytes = total * (16 + 8 + 8 + 4);
             if (total > SP_TAIL_SHARED || total - nf > SP_TAIL_SHARED_COARSE) {
                 sp_destroy(s);
                 g_create_error = "multigrid tail does not fit its shared-memory budget";
                 return SP_E_INVALID;
             }
         }
-        const size_t N = (size_t)s->N, T = (size_t)s->lv.total;
+        const size_t N = (size_t)s->N;
         auto& o = s->owned;
+        // one arena for every buffer the kernels reach through the tank's context
+        TankRef& ref = s->ref;
+        ref = TankRef{nullptr, s->W, s->H, s->cap, s->lv.total, s->lv.tiles, s->lv.n, s->lv.tail};
+        size_t arena_bytes = 0;
+        (void)ctx_of(ref, &arena_bytes);
+        ref.base = dalloc<char>(arena_bytes, o);
+        s->c = ctx_of(ref);
         Ctx& c = s->c;
-        c.W = s->W;
-        c.H = s->H;
-        c.N = s->N;
-        s->d_P = dalloc<sp_gparams>(1, o);
-        s->d_mats = dalloc<sp_gmat>(SP_MAX_MATERIALS, o);
-        c.P = s->d_P;
-        c.mats = s->d_mats;
-        c.S = dalloc<sp_stepbuf>(1, o);
-        c.K = dalloc<sp_solve>(1, o);
-        c.part = dalloc<sp_particle>(s->cap, o);
         s->d_part_tmp = dalloc<sp_particle>(s->cap, o);
         s->d_spawn = dalloc<sp_particle>(SP_SPAWN_MAX, o);
         s->d_counter = dalloc<uint32_t>(8 + 2 * SP_MAX_MATERIALS, o);
-        c.acc = dalloc<int32_t>(N * SP_ACC_MAX, o);
-        c.vel = dalloc<float4>(N, o);
-        c.cell = dalloc<float4>(N, o);
-        c.comp = dalloc<float4>(N, o);
-        c.coef = dalloc<float4>(N, o);
-        c.surf = dalloc<float4>(N, o);
-        c.surf2 = dalloc<float4>(N, o);
-        c.sprop = dalloc<float4>(N, o);
-        c.aux = dalloc<float4>(N, o);
-        c.grho = dalloc<float4>(N, o);
-        c.gbase = dalloc<float4>(N, o);
-        c.prhs = dalloc<float4>(N, o);
-        c.vw = dalloc<float4>(N, o);
-        c.vs = dalloc<float4>(N, o);
-        c.solid = dalloc<float4>(N, o);
-        c.vel0 = dalloc<float2>(N, o);
-        c.pjump = dalloc<float2>(N, o);
-        c.pres = dalloc<float2>(N, o);
-        c.sdf = dalloc<float>((size_t)(s->W + 1) * (s->H + 1), o);
-        c.mixb = dalloc<float>(N * SP_MAX_PRESENT, o);
-        c.p2keep = dalloc<float>(N, o);
-        c.region = dalloc<uint32_t>(N, o);
-        c.region_air = dalloc<uint32_t>(1, o);
-        c.mga = dalloc<float4>(T, o);
-        c.mgd = dalloc<float>(T, o);
-        c.mgx = dalloc<float2>(T, o);
-        c.mgz = dalloc<float2>(T, o);
-        c.mgr = dalloc<float2>(T, o);
-        c.cgp = dalloc<float2>(N, o);
-        c.cgp2 = dalloc<float2>(N, o);
-        c.cgq = dalloc<float2>(N, o);
-        s->d_r2 = dalloc<float2>(N, o);
-        c.ptile = dalloc<uint32_t>((size_t)s->lv.tiles, o);
-        c.part_sums = dalloc<double4>((size_t)2 * s->nb0, o);
-        c.vr = dalloc<float2>(N, o);
-        c.vp = dalloc<float2>(N, o);
-        c.vp2 = dalloc<float2>(N, o);
-        c.vq = dalloc<float2>(N, o);
-        c.vd = dalloc<float2>(N, o);
-        c.vr2 = dalloc<float2>(N, o);
-        c.vq2 = dalloc<float2>(N, o);
-        c.vaz = dalloc<float2>(N, o);
-        c.vtiles = dalloc<uint32_t>((size_t)s->nb0, o);
-        c.vtile = dalloc<uint32_t>((size_t)s->lv.tiles, o);
-        c.vma = dalloc<float4>(T, o);
-        c.vmb = dalloc<float4>(T, o);
-        c.vmd = dalloc<float2>(T, o);
-        c.vmr = dalloc<float2>(T, o);
-        c.vmx = dalloc<float2>(T, o);
-        c.vmz = dalloc<float2>(T, o);
-        c.vpart = dalloc<double4>((size_t)s->nb0, o);
-        int4* lev = dalloc<int4>(SP_MAX_LEVELS, o);
         int4 h_lev[SP_MAX_LEVELS] = {};
         for (int l = 0; l < s->lv.n; l++) h_lev[l] = make_int4(s->lv.offs[l], s->lv.dims[l][0], s->lv.dims[l][1], s->lv.tile_offs[l]);
-        HIPCHK(hipMemcpy(lev, h_lev, sizeof(h_lev), hipMemcpyHostToDevice));
-        c.lev = lev;
-        c.n_levels = s->lv.n;
-        c.tail = s->lv.tail;
+        HIPCHK(hipMemcpy((void*)c.lev, h_lev, sizeof(h_lev), hipMemcpyHostToDevice));
         s->d_keys = dalloc<uint32_t>(s->cap, o);
         s->d_vals = dalloc<uint32_t>(s->cap, o);
         s->d_keys2 = dalloc<uint32_t>(s->cap, o);
@@ -781,11 +384,19 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         // solid field: closed everywhere until set_solids
         HIPCHK(hipDeviceSynchronize());
         sp::default_params(&s->params);
-        s->timing = true;
+        Run& r = s->run;
+        r.tb.t[0] = ref;
+        r.T = 1;
+        r.W = s->W;
+        r.H = s->H;
+        r.nb0 = s->nb0;
+        r.lv = s->lv;
+        r.stream = s->stream;
+        r.tail_lds = s->tail_lds_bytes;
         // SPROUT_GRAPHS=0 launches the substep's kernels one by one instead of replaying
         // captured graphs (profilers that cannot follow graph launches need it)
         const char* gv = getenv("SPROUT_GRAPHS");
-        s->use_graphs = !(gv && gv[0] == '0');
+        s->gs.on = !(gv && gv[0] == '0');
     } catch (const HipError& e) {
         char buf[256];
         snprintf(buf, sizeof(buf), "%s failed: %s", e.what, hipGetErrorString(e.e));
@@ -801,12 +412,11 @@ void sp_destroy(sp_sim* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    destroy_graphs(s);
+    s->gs.clear();
     for (void* p : s->owned) (void)hipFree(p);
-    for (auto& e : s->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
-    delete s;
+    hipStream_t st = s->stream;
+    delete s;  // graphs and events go with it
+    if (st) (void)hipStreamDestroy(st);
 }
 
 int sp_set_params(sp_sim* s, const sp_sim_params* p) {
@@ -1049,7 +659,8 @@ int sp_cell_counts(sp_sim* s, int32_t x0, int32_t y0, int32_t w, int32_t h, uint
         if (w == 0 || h == 0) return SP_OK;
         HIPCHK(hipMemsetAsync(s->d_count_out, 0, sizeof(uint32_t) * (size_t)w * h, s->stream));
         if (s->n_slots)
-            hipLaunchKernelGGL(k_count, dim3((s->n_slots + 255) / 256), dim3(256), 0, s->stream, s->c, x0, y0, w, h, s->d_count_out);
+            hipLaunchKernelGGL(k_count, dim3((s->n_slots + 255) / 256), dim3(256), 0, s->stream, s->run.tb, x0, y0, w, h,
+                               s->d_count_out);
         HIPCHK(hipMemcpyAsync(out, s->d_count_out, sizeof(uint32_t) * (size_t)w * h, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         return SP_OK;
@@ -1064,7 +675,7 @@ int sp_stats(sp_sim* s, sp_stats_out* out) {
         const int words = 8 + 2 * SP_MAX_MATERIALS;
         HIPCHK(hipMemsetAsync(s->d_counter, 0, sizeof(uint32_t) * words, s->stream));
         uint32_t runs = (s->n_slots + 63) / 64;
-        if (runs) hipLaunchKernelGGL(k_stats, dim3((runs + 63) / 64), dim3(64), 0, s->stream, s->c, s->d_counter);
+        if (runs) hipLaunchKernelGGL(k_stats, dim3((runs + 63) / 64), dim3(64), 0, s->stream, s->run.tb, s->d_counter);
         std::vector<uint32_t> h(words);
         HIPCHK(hipMemcpyAsync(h.data(), s->d_counter, sizeof(uint32_t) * words, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
@@ -1081,7 +692,7 @@ int sp_sort(sp_sim* s) {
     return guarded(s, [&]() -> int {
         int rc = upload_tables(s);
         if (rc != SP_OK) return rc;
-        launch_sort(s, 0);
+        launch_sort(s, 0, s->stream);
         HIPCHK(hipStreamSynchronize(s->stream));
         return SP_OK;
     });
@@ -1090,7 +701,10 @@ int sp_sort(sp_sim* s) {
 int sp_timings(sp_sim* s, float* out10) {
     return guarded(s, [&]() -> int {
         if (!out10) return SP_E_INVALID;
-        for (int k = 0; k < 10; k++) out10[k] = s->timings[k];
+        for (int k = 0; k < 7; k++) out10[k] = s->tm.phase[k];
+        out10[7] = (float)s->stats.pressure_iters_last;
+        out10[8] = (float)s->last_v_iters;
+        out10[9] = (float)s->last_p2_iters;
         return SP_OK;
     });
 }
@@ -1098,8 +712,8 @@ int sp_timings(sp_sim* s, float* out10) {
 int sp_kernel_time(sp_sim* s, int32_t kernel, float* avg_ms, uint32_t* launches) {
     return guarded(s, [&]() -> int {
         if (!avg_ms || !launches || kernel != 0) return SP_E_INVALID;
-        *avg_ms = s->particle_ms;
-        *launches = s->timed_last;
+        *avg_ms = s->tm.particle_ms;
+        *launches = s->tm.timed_last;
         return SP_OK;
     });
 }
@@ -1208,54 +822,49 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         int rc = upload_tables(s);
         if (rc != SP_OK) return rc;
         const Ctx& c = s->c;
-        dim3 g0 = grid_groups(s->W, s->H);
+        const Run& r = s->run;
+        const dim3 g0 = grid0(r);
+        const TankBatch& cs = r.tb;
         // stages run with the most iterations the parameters allow
         const Recs full{s->params.pressure_max_iters, s->params.pressure_max_iters, std::max(s->params.viscosity_max_iters, 1u)};
         switch (stage) {
-        case SP_STAGE_STEP: hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, s->stream, c); break;
-        case SP_STAGE_CLEAR_ACC: launch_clear_acc(s); break;
-        case SP_STAGE_PARTICLES: launch_particles(s, arg); break;
-        case SP_STAGE_GRID_PREP: hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, s->stream, c); break;
-        case SP_STAGE_COEF: hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, c); break;
+        case SP_STAGE_STEP: hipLaunchKernelGGL(k_step, per_tank(r), dim3(64), 0, s->stream, cs); break;
+        case SP_STAGE_CLEAR_ACC: launch_clear_acc(r); break;
+        case SP_STAGE_PARTICLES: launch_particles(r, arg); break;
+        case SP_STAGE_GRID_PREP: hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_COEF: hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_SURF:
-            hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, c);
-            hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, s->stream, c);
-            hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, s->stream, c);
+            hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, s->stream, cs);
+            hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, s->stream, cs);
+            hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, s->stream, cs);
             break;
-        case SP_STAGE_VISC: launch_visc_before(s, full); break;
-        case SP_STAGE_VISC_SPLIT: launch_visc_split(s); break;
-        case SP_STAGE_VISC_SOLVE: launch_visc_solve(s, full.v, arg != 0); break;
-        case SP_STAGE_RHS_VISC: hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, c); break;
-        case SP_STAGE_PRESSURE2: launch_pressure(s, full.p2, true); break;
-        case SP_STAGE_PROJECT_VISC: hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, c); break;
+        case SP_STAGE_VISC: launch_visc_before(r, full); break;
+        case SP_STAGE_VISC_SPLIT: launch_visc_split(r); break;
+        case SP_STAGE_VISC_SOLVE: launch_visc_solve(r, full.v, arg != 0); break;
+        case SP_STAGE_RHS_VISC: hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_PRESSURE2: launch_pressure(r, full.p2, true); break;
+        case SP_STAGE_PROJECT_VISC: hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_VISC_VCYCLE: {
             // One V-cycle from vr, whatever the state of the last solve: the level kernels
             // return at once on a converged solve, so the flag is cleared around it.
             uint32_t done = 0, zero = 0;
             HIPCHK(hipMemcpy(&done, &c.K->v_done, 4, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(&c.K->v_done, &zero, 4, hipMemcpyHostToDevice));
-            if (s->gp.v_mg_sweeps == 1) {
-                hipLaunchKernelGGL(k_vmg_down<1>, g0, TILE, 0, s->stream, c, 0, 0);
-                launch_visc_coarse<1>(s);
-                hipLaunchKernelGGL(k_visc_mgz<1>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
-            } else {
-                hipLaunchKernelGGL(k_vmg_down<2>, g0, TILE, 0, s->stream, c, 0, 0);
-                launch_visc_coarse<2>(s);
-                hipLaunchKernelGGL(k_visc_mgz<2>, g0, TILE, 0, s->stream, c, (const float2*)c.vr, (const float2*)c.vq);
-            }
+            if (s->gp.v_mg_sweeps == 1) launch_visc_vcycle<1>(r);
+            else launch_visc_vcycle<2>(r);
             HIPCHK(hipStreamSynchronize(s->stream));
             HIPCHK(hipMemcpy(&c.K->v_done, &done, 4, hipMemcpyHostToDevice));
             break;
         }
-        case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, c); break;
-        case SP_STAGE_PRESSURE: launch_pressure(s, full.p); break;
-        case SP_STAGE_PROJECT: hipLaunchKernelGGL(k_project, g0, TILE, 0, s->stream, c); break;
+        case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_PRESSURE: launch_pressure(r, full.p); break;
+        case SP_STAGE_PROJECT: launch_project(r); break;
         case SP_STAGE_SUBSTEP:
-            launch_substep(s, arg, full);
+            launch_substep(r, &s->gs, nullptr, arg, full);
             s->grid_valid = true;
             break;
-        case SP_STAGE_MG_BUILD: launch_mg_build(s); break;
-        case SP_STAGE_VCYCLE: launch_vcycle(s, 0, -1, s->c.mgr); break;
+        case SP_STAGE_MG_BUILD: launch_mg_build(r); break;
+        case SP_STAGE_VCYCLE: launch_vcycle(r, 0, -1, 0); break;
         default: set_err(s, "unknown stage %d", stage); return SP_E_INVALID;
         }
         HIPCHK(hipGetLastError());

@@ -75,7 +75,8 @@ SP_DEV float2 gts_of(const Ctx& c, const float4* __restrict__ st, int i) {
 // within one cell of a cell whose diffusion number reaches implicit_viscosity_min (the tile
 // flags are cleared before; each group collects the marks for its tile and the eight around
 // it in LDS and stores them once).
-__global__ __launch_bounds__(256) void k_visc_setup(Ctx c, int mark) {
+__global__ __launch_bounds__(256) void k_visc_setup(const TankBatch tb, int mark) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ uint32_t s_mark[9];
     const int t = tid2();
     if (mark && t < 9) s_mark[t] = 0u;
@@ -129,7 +130,8 @@ __global__ __launch_bounds__(256) void k_visc_setup(Ctx c, int mark) {
 }
 
 // visc_apply.comp mode 2 (visc_apply.comp:16-48): the explicit viscous step everywhere.
-__global__ __launch_bounds__(256) void k_visc_explicit(Ctx c) {
+__global__ __launch_bounds__(256) void k_visc_explicit(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     if (!v_interior(c, cx, cy)) return;
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(256) void k_visc_explicit(Ctx c) {
 }
 
 // visc_split.comp:24-38: the explicit step in the tiles the implicit solve leaves out.
-__global__ __launch_bounds__(256) void k_visc_split(Ctx c) {
+__global__ __launch_bounds__(256) void k_visc_split(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (c.vtiles[tile_slot()] != 0u) return;
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
@@ -201,7 +204,8 @@ SP_DEV void couplings(const Ctx& c, int x, int y, float4& a, float4& b) {
 // the warm start (vd), the inactive tiles taken out of the system, and (multigrid) the matrix
 // of level 0 in the form the V-cycle keeps it. A tile that has just gone inactive has its
 // matrix and multigrid vectors cleared once.
-__global__ __launch_bounds__(256) void k_visc_diag(Ctx c) {
+__global__ __launch_bounds__(256) void k_visc_diag(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ uint32_t s_was;
     const int slot = tile_slot();
     const bool mg = c.P->v_precond != 0;
@@ -282,7 +286,8 @@ SP_DEV bool vflag(const Ctx& c, int l, int slot) {
 // vmg_build.comp:187-240: level l of the viscosity multigrid from level l - 1 (P^T A P with
 // 2 x 2 aggregates per component), over coarse 16 x 16 tiles, with the tile flags of
 // mg_build.comp.
-__global__ __launch_bounds__(256) void k_vmg_build(Ctx c, int l) {
+__global__ __launch_bounds__(256) void k_vmg_build(const TankBatch tb, int l) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ uint32_t s_state;
     const Lv F = level(c, l - 1), C = level(c, l);
     if (tid2() == 0) {
@@ -352,7 +357,8 @@ __global__ __launch_bounds__(256) void k_vmg_build(Ctx c, int l) {
 
 // visc_strain.comp:11-17: the strain of the current velocities in the active tiles (the
 // first pressure solve changed them since the split).
-__global__ __launch_bounds__(256) void k_visc_strain(Ctx c) {
+__global__ __launch_bounds__(256) void k_visc_strain(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (c.vtiles[tile_slot()] == 0u) return;
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
@@ -362,8 +368,9 @@ __global__ __launch_bounds__(256) void k_visc_strain(Ctx c) {
 
 // visc_apply.comp mode 0: r = b - A u* = -G^T W G u*, p = z = r / diag, and the partial
 // sums of r.z and r.r.
-__global__ __launch_bounds__(256) void k_visc_start(Ctx c) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(256) void k_visc_start(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    __shared__ double4 lds[4];
     const int slot = tile_slot();
     if (c.vtiles[slot] == 0u) {
         if (tid2() == 0) c.vpart[slot] = make_double4(0, 0, 0, 0);
@@ -463,9 +470,10 @@ SP_DEV float2 gts_staged(const Ctx& c, const DirStage& S, int x0, int y0, int x,
 // visc_iter.comp with WARM (viscosity_warm): the warm start of the solve. It steps from
 // x0 = u* by d = vd, last substep's change, rescaled and masked by k_visc_diag: x = u* + d,
 // r = r0 - A d, with the partial sums of r.z and r.r.
-__global__ __launch_bounds__(256) void k_visc_warm(Ctx c) {
+__global__ __launch_bounds__(256) void k_visc_warm(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ DirStage S;
-    __shared__ double4 lds[256];
+    __shared__ double4 lds[4];
     if (v_done(c)) return;
     const int slot = tile_slot();
     if (c.vtiles[slot] == 0u) {
@@ -501,9 +509,13 @@ __global__ __launch_bounds__(256) void k_visc_warm(Ctx c) {
 
 // visc_iter.comp: one Jacobi-preconditioned iteration up to its step length: p = z + beta
 // p_old (z = r / diag), s = W G p, q = M p + G^T s, and the partial sums of p.q.
-__global__ __launch_bounds__(256) void k_visc_iter(Ctx c, const float2* __restrict__ p_old, float2* __restrict__ p_new) {
+__global__ __launch_bounds__(256) void k_visc_iter(const TankBatch tb, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    // the direction alternates between vp and vp2 with the iteration's parity
+    const float2* __restrict__ p_old = par ? c.vp2 : c.vp;
+    float2* __restrict__ p_new = par ? c.vp : c.vp2;
     __shared__ DirStage S;
-    __shared__ double4 lds[256];
+    __shared__ double4 lds[4];
     if (v_done(c)) return;
     const int slot = tile_slot();
     if (c.vtiles[slot] == 0u) {
@@ -543,8 +555,10 @@ __global__ __launch_bounds__(256) void k_visc_iter(Ctx c, const float2* __restri
 
 // visc_update.comp:11-36: u += alpha p, vd += alpha p, r -= alpha q, and the partial sums of
 // r.z and r.r.
-__global__ __launch_bounds__(256) void k_visc_update(Ctx c, const float2* __restrict__ p_dir) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(256) void k_visc_update(const TankBatch tb, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ p_dir = par ? c.vp : c.vp2;
+    __shared__ double4 lds[4];
     if (v_done(c)) return;
     const int slot = tile_slot();
     if (c.vtiles[slot] == 0u) {
@@ -725,7 +739,8 @@ struct VRegion {
 // vmg_down.comp: the way down of a viscosity V-cycle on level l (level 0 only at the start of
 // a solve, from the residual vr; the iterations take theirs inside k_visc_mgu).
 template <int NU>
-__global__ __launch_bounds__(256) void k_vmg_down(Ctx c, int l, int coarsest) {
+__global__ __launch_bounds__(256) void k_vmg_down(const TankBatch tb, int l, int coarsest) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ VRegion<NU, 0> R;
     __shared__ float2 sres[256];
     if (v_done(c)) return;
@@ -740,7 +755,8 @@ __global__ __launch_bounds__(256) void k_vmg_down(Ctx c, int l, int coarsest) {
 
 // vmg_up.comp: the way up on a coarse level l, the finished values to vmz.
 template <int NU>
-__global__ __launch_bounds__(256) void k_vmg_up(Ctx c, int l, int coarsest) {
+__global__ __launch_bounds__(256) void k_vmg_up(const TankBatch tb, int l, int coarsest) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ VRegion<NU, 0> R;
     if (v_done(c)) return;
     if (!vflag(c, l, tile_slot())) return;
@@ -758,10 +774,15 @@ __global__ __launch_bounds__(256) void k_vmg_up(Ctx c, int l, int coarsest) {
 // partial sums of r.r and p.q, then the way down on level 0 from the new residual (which the
 // group recomputes for its rings from r_in and q_in, so it need not wait for its neighbours).
 template <int NU>
-__global__ __launch_bounds__(256) void k_visc_mgu(Ctx c, const float2* __restrict__ r_in, const float2* __restrict__ q_in,
-                                                  float2* __restrict__ r_out, float2* __restrict__ q_out) {
+__global__ __launch_bounds__(256) void k_visc_mgu(const TankBatch tb, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    // residual and A p alternate between (vr, vq) and (vr2, vq2) with the iteration's parity
+    const float2* __restrict__ r_in = par ? c.vr2 : c.vr;
+    const float2* __restrict__ q_in = par ? c.vq2 : c.vq;
+    float2* __restrict__ r_out = par ? c.vr : c.vr2;
+    float2* __restrict__ q_out = par ? c.vq : c.vq2;
     __shared__ VRegion<NU, 0> R;
-    __shared__ double4 lds[256];
+    __shared__ double4 lds[4];
     __shared__ float2 sres[256];
     if (v_done(c)) return;
     const int slot = tile_slot();
@@ -821,9 +842,12 @@ __global__ __launch_bounds__(256) void k_visc_mgu(Ctx c, const float2* __restric
 // the tile follows in LDS; writes z (vmz) and A z (vaz) with the partial sums of r.z, z.A z
 // and z.q (q = A p of this iteration).
 template <int NU>
-__global__ __launch_bounds__(256) void k_visc_mgz(Ctx c, const float2* __restrict__ r, const float2* __restrict__ q) {
+__global__ __launch_bounds__(256) void k_visc_mgz(const TankBatch tb, int sel) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ r = sel ? c.vr2 : c.vr;
+    const float2* __restrict__ q = sel ? c.vq2 : c.vq;
     __shared__ VRegion<NU, 1> R;
-    __shared__ double4 lds[256];
+    __shared__ double4 lds[4];
     if (v_done(c)) return;
     const int slot = tile_slot();
     if (c.vtiles[slot] == 0u) {
@@ -858,12 +882,13 @@ __global__ __launch_bounds__(256) void k_visc_mgz(Ctx c, const float2* __restric
 enum { VRED_START = 4, VRED_ALPHA = 5, VRED_BETA = 6, VRED_WARM = 10, VRED_MG_START = 11, VRED_MG_UPDATE = 12,
        VRED_MG_ITER = 13 };
 
-__global__ __launch_bounds__(256) void k_visc_reduce(Ctx c, int mode, int nblocks) {
-    __shared__ double4 lds[256];
+__global__ __launch_bounds__(REDUCE_NT) void k_visc_reduce(const TankBatch tb, int mode, int nblocks) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    __shared__ double4 lds[REDUCE_NT / 64];
     sp_solve* K = c.K;
     if (mode != VRED_START && v_done(c)) return;
     const double tol2 = (double)c.P->tol[2] * (double)c.P->tol[2];
-    const double4 a = total_of(c.vpart, 0, nblocks, lds);
+    const double4 a = total_of<REDUCE_NT>(c.vpart, 0, nblocks, lds);
     if (tid2() != 0) return;
     switch (mode) {
     case VRED_START:
@@ -922,7 +947,8 @@ __global__ __launch_bounds__(256) void k_visc_reduce(Ctx c, int mode, int nblock
 }
 
 // Statistics at the end of a viscosity solve (reduce_tail.glsl:186-192, mode 7).
-__global__ void k_visc_end(Ctx c) {
+__global__ void k_visc_end(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     sp_solve* K = c.K;
     if (K->v_iters > K->v_it_max) K->v_it_max = K->v_iters;
@@ -937,7 +963,8 @@ __global__ void k_visc_end(Ctx c) {
 // rhs_visc.comp:143-168: b = -div(u) + A P1 (P1 in prhs.z, kept by the first projection),
 // and the start P1 + last substep's P2 rescaled (viscous_warm). Neighbours rewrite prhs with
 // the same P1 in z, so reading their z while they write is safe.
-__global__ __launch_bounds__(256) void k_rhs_visc(Ctx c) {
+__global__ __launch_bounds__(256) void k_rhs_visc(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     const float D = c.mgd[i];
@@ -963,7 +990,8 @@ __global__ __launch_bounds__(256) void k_rhs_visc(Ctx c) {
 
 // project_visc.comp:187-210: subtract the gradient of P2 = P - P1 (zero in air), keep P2 for
 // the next substep, and report the fastest liquid face again.
-__global__ __launch_bounds__(256) void k_project_visc(Ctx c) {
+__global__ __launch_bounds__(256) void k_project_visc(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
     const bool in = cell_of_thread(c, cx, cy, i);
     float m = 0.0f;
