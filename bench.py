@@ -53,10 +53,10 @@ def pmc_traffic(kind, W, H):
     return None
 
 
-def workload(kind, grid):
+def workload(kind, grid, seed=0):
     import cases
     if kind == "four":
-        return cases.four_liquids(grid, seed=0, viscosity_tol=1e-5)
+        return cases.four_liquids(grid, seed=seed, viscosity_tol=1e-5)
     return cases.dam_break(grid, frac=(0.8, 0.8))
 
 
@@ -100,27 +100,41 @@ def main():
     ap.add_argument("--workload", choices=["four", "dam"], default="four")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=8)
+    ap.add_argument("--tanks", type=int, default=1,
+                    help="tanks advanced together in one batch (seeds 0 .. tanks-1; value is their aggregate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "a tank shards by scene across GPUs, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
-    from sprout_amd.fluid_sim import FluidSim
+    from sprout_amd.fluid_sim import FluidBatch, FluidSim
 
     W, H = (int(x) for x in args.grid.split("x"))
-    params, field, parts, rows = workload(args.workload, (W, H))
-    n_p = len(parts)
-    sim = FluidSim((W, H), W * H * 4, params=params)
-    sim.set_solids(field)
-    sim.reset(parts)
+    tanks = []
+    for seed in range(max(args.tanks, 1)):
+        params, field, parts, rows = workload(args.workload, (W, H), seed)
+        sim = FluidSim((W, H), W * H * 4, params=params)
+        sim.set_solids(field)
+        sim.reset(parts)
+        tanks.append((sim, params, field, parts, rows))
+    sim, params, field, parts, rows = tanks[0]
+    n_p = sum(len(t[3]) for t in tanks)
+    batch = FluidBatch([t[0] for t in tanks]) if len(tanks) > 1 else None
+
+    def step(k):
+        if batch is not None:
+            batch.record(k, k / 240.0)
+            batch.wait()
+        else:
+            sim.record(k, k / 240.0)
+            sim.wait()
 
     def run(n):
         done = 0
         while done < n:
             k = min(args.batch, n - done)
-            sim.record(k, k / 240.0)
-            sim.wait()
+            step(k)
             done += k
 
     run(args.warmup)
@@ -130,10 +144,14 @@ def main():
     done = 0
     while done < args.steps:
         k = min(args.batch, args.steps - done)
-        sim.record(k, k / 240.0)
-        sim.wait()
-        ker.append(sim.kernel_time(0))
-        phases.append(sim.timings().copy())
+        step(k)
+        if batch is None:
+            ker.append(sim.kernel_time(0))
+            phases.append(sim.timings().copy()[:7])
+        else:
+            bt = batch.timings(0)
+            ker.append((float(bt[7]), k))
+            phases.append(bt[:7].copy())
         done += k
     wall = time.perf_counter() - t0
     st = sim.solver
@@ -142,12 +160,15 @@ def main():
     launches = sum(n for _, n in ker)
     k_ms = sum(ms * n for ms, n in ker) / max(launches, 1)
     ph = np.mean(np.array(phases), axis=0)
-    alg_bytes = 96.0 * n_p + 100.0 * W * H
+    # one particle-pass launch covers every tank of the (first) launch group
+    group_tanks = batch.groups()[0] if batch is not None else 1
+    alg_bytes = group_tanks * (96.0 * len(parts) + 100.0 * W * H)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    live = sim.stats().live
+    live = sum(t[0].stats().live for t in tanks)
     name = {"four": "four_liquids", "dam": "dam_break"}[args.workload]
     config = {
-        "workload": f"{name}_{W}x{H}_fp32",
+        "workload": f"{name}_{W}x{H}_fp32" + (f"_batch{len(tanks)}" if len(tanks) > 1 else ""),
+        "tanks": len(tanks),
         "grid": [W, H],
         "particles": n_p,
         "live_particles_end": int(live),
@@ -200,7 +221,10 @@ def main():
         },
         "cpu_baseline": None,
     }
-    sim.close()
+    if batch is not None:
+        batch.close()
+    for t in tanks:
+        t[0].close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline((W, H), params, field, parts, rows, args.cpu_substeps)
     print(json.dumps(out))

@@ -111,6 +111,29 @@ int sp_stats(sp_sim* sim, sp_stats_out* out);
  * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
 int sp_sort(sp_sim* sim);
 
+/* ---- batches of tanks ------------------------------------------------------------------ */
+
+/* No counterpart in the reference, which steps one tank per FluidSim: the batched workload.
+ * A batch advances many handles together. Tanks of the same grid size and solver schedule
+ * share every kernel launch of a substep (up to 32 tanks per launch), each group of them on
+ * a stream of its own so that groups overlap on the device. Each tank's results are those it
+ * gets stepped alone. The handles stay owned by the caller, must outlive the batch, be on one
+ * device, and must not be used on their own while a batch call on them runs. */
+typedef struct sp_batch sp_batch;
+int sp_batch_create(sp_sim* const* sims, uint32_t n, sp_batch** out);
+void sp_batch_destroy(sp_batch* batch);
+const char* sp_batch_last_error(const sp_batch* batch);
+/* sp_record for every tank of the batch: `substeps` substeps covering `target_s` seconds,
+ * enqueued on the groups' streams (spawns, tables and step words of each tank first). */
+int sp_batch_record(sp_batch* batch, uint32_t substeps, float target_s);
+/* Waits for the batch and folds each tank's solver statistics in (sp_poll then reads them). */
+int sp_batch_wait(sp_batch* batch);
+/* The launch groups of the last record: their number, and the tanks of each (up to cap). */
+int sp_batch_groups(const sp_batch* batch, uint32_t* n_groups, uint32_t* tanks_per_group, uint32_t cap);
+/* Device ms per substep of a group's phases ([0]-[6] as sp_timings) and of its particle
+ * kernel ([7]), averaged over the substeps of the last record. */
+int sp_batch_timings(const sp_batch* batch, uint32_t group, float* out8);
+
 /* ---- host-side derivations (no device needed) ---------------------------------------- */
 
 /* Default SI parameters and the built-in materials this build carries (water, honey, oil,

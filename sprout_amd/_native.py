@@ -181,6 +181,13 @@ SIGNATURES = {
     "sp_debug_gparams": ([_P, C.POINTER(GParams), _P], C.c_int),
     "sp_timings": ([_P, _P], C.c_int),
     "sp_kernel_time": ([_P, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)], C.c_int),
+    "sp_batch_create": ([_P, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
+    "sp_batch_destroy": ([_P], None),
+    "sp_batch_last_error": ([_P], C.c_char_p),
+    "sp_batch_record": ([_P, C.c_uint32, C.c_float], C.c_int),
+    "sp_batch_wait": ([_P], C.c_int),
+    "sp_batch_groups": ([_P, C.POINTER(C.c_uint32), _P, C.c_uint32], C.c_int),
+    "sp_batch_timings": ([_P, C.c_uint32, _P], C.c_int),
 }
 
 _lib = None

@@ -495,8 +495,11 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
 }  // namespace
 
 // The particle pass (particles.comp:1064-1101).
+// gather < 0: each tank decides from its own step state (the first substep after a reset,
+// which k_step has just counted, has no grid velocities to gather yet).
 __global__ __launch_bounds__(PT) void k_particles(const TankBatch tb, int gather) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    if (gather < 0) gather = c.S->substeps > 1u ? 1 : 0;
     __shared__ int sacc[LDS_INTS];
     __shared__ int sbox[4];
     const int t = threadIdx.x;

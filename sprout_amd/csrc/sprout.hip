@@ -193,7 +193,10 @@ void readback(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
 }
 
-int record(sp_sim* s, uint32_t substeps, float target) {
+// The per-tank part of FluidSim::record (mod.rs:2400-2563) before the substeps are
+// recorded: pending spawns into fresh or dead slots, the tables, and the batch words of the
+// step state. On the tank's own stream.
+int record_prepare(sp_sim* s, uint32_t substeps, float target) {
     if (!s->params_set) {
         set_err(s, "sp_set_params has not been called");
         return SP_E_INVALID;
@@ -234,12 +237,23 @@ int record(sp_sim* s, uint32_t substeps, float target) {
         HIPCHK(hipMemcpyAsync(&s->c.S->frame_left, &words[0], 4, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(&s->c.S->batch_steps, &words[2], 8, hipMemcpyHostToDevice, s->stream));
     }
+    return SP_OK;
+}
+
+// The periodic sort before a substep (record_sort, mod.rs:3223-3242), on `stream`.
+void maybe_sort(sp_sim* s, hipStream_t stream) {
+    if (s->since_sort >= s->sort_interval) {
+        launch_sort(s, 1, stream);
+        s->since_sort = 0;
+    }
+    s->since_sort++;
+}
+
+int record(sp_sim* s, uint32_t substeps, float target) {
+    int rc = record_prepare(s, substeps, target);
+    if (rc != SP_OK) return rc;
     for (uint32_t k = 0; k < substeps; k++) {
-        if (s->since_sort >= s->sort_interval) {
-            launch_sort(s, 1, s->stream);
-            s->since_sort = 0;
-        }
-        s->since_sort++;
+        maybe_sort(s, s->stream);
         launch_substep(s->run, &s->gs, &s->tm, s->grid_valid ? 1 : 0, s->rec);
         s->grid_valid = true;
     }
@@ -874,3 +888,5 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
 }
 
 }  // extern "C"
+
+#include "batch.hip"

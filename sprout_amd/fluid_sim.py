@@ -188,3 +188,52 @@ class FluidSim:
         mats = (N.GMat * N.MAX_MATERIALS)()
         self._check(self._lib.sp_debug_gparams(self._h, C.byref(g), mats), "grid_params")
         return g, mats
+
+
+class FluidBatch:
+    """Many FluidSim handles advanced together (sp_batch_*): tanks of the same grid and
+    solver schedule share each kernel launch of a substep, groups overlap on the device, and
+    each tank's results are those it gets stepped alone."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        L = N.lib()
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        rc = L.sp_batch_create(arr, len(self.sims), C.byref(h))
+        if rc != N.OK:
+            raise N.SproutError(rc, "sp_batch_create")
+        self._h = h
+        self._lib = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            raise N.SproutError(rc, f"{what}: {self._lib.sp_batch_last_error(self._h).decode()}")
+
+    def record(self, substeps, target):
+        self._check(self._lib.sp_batch_record(self._h, substeps, float(target)), "batch record")
+
+    def wait(self):
+        self._check(self._lib.sp_batch_wait(self._h), "batch wait")
+
+    def groups(self):
+        n = C.c_uint32()
+        sizes = np.zeros(len(self.sims), np.uint32)
+        self._check(self._lib.sp_batch_groups(self._h, C.byref(n), sizes.ctypes.data, len(sizes)), "batch groups")
+        return sizes[: n.value].tolist()
+
+    def timings(self, group):
+        out = np.zeros(8, np.float32)
+        self._check(self._lib.sp_batch_timings(self._h, group, out.ctypes.data), "batch timings")
+        return out
