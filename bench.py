@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=8)
     ap.add_argument("--tanks", type=int, default=1,
                     help="tanks advanced together in one batch (seeds 0 .. tanks-1; value is their aggregate)")
+    ap.add_argument("--latency-steps", type=int, default=64,
+                    help="single-substep step requests timed one by one after the throughput run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -154,6 +156,16 @@ def main():
             phases.append(bt[:7].copy())
         done += k
     wall = time.perf_counter() - t0
+    # step latency: one substep requested and waited for at a time (every tank of the batch)
+    lat = []
+    for _ in range(args.latency_steps):
+        t1 = time.perf_counter()
+        step(1)
+        lat.append(1e3 * (time.perf_counter() - t1))
+    lat = np.sort(np.array(lat)) if lat else np.zeros(1)
+
+    def pct(p):
+        return float(lat[min(max(int(np.ceil(p / 100 * len(lat))), 1), len(lat)) - 1])
     st = sim.solver
     ms_step = 1e3 * wall / args.steps
     value = n_p * args.steps / wall
@@ -196,6 +208,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": config,
+        "step_latency_ms": {"p50": pct(50), "p99": pct(99), "max": float(lat[-1]), "requests": args.latency_steps,
+                            "request": f"1 substep of all {len(tanks)} tank(s), record + wait"},
         "pressure_solve_ms": float(ph[3]),
         "pressure2_solve_ms": float(ph[6]),
         "viscosity_solve_ms": float(ph[5]),

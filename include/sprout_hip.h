@@ -31,6 +31,7 @@ extern "C" {
 #define SP_E_NOMEM (-3)        /* device or host allocation failed */
 #define SP_E_UNSUPPORTED (-4)  /* a feature of the reference this build does not carry */
 #define SP_E_CAPACITY (-5)     /* more particles / materials than the handle holds */
+#define SP_E_BUSY (-6)         /* the handle is driven by a server; or a request is not done yet */
 
 typedef struct sp_sim sp_sim;
 
@@ -133,6 +134,73 @@ int sp_batch_groups(const sp_batch* batch, uint32_t* n_groups, uint32_t* tanks_p
 /* Device ms per substep of a group's phases ([0]-[6] as sp_timings) and of its particle
  * kernel ([7]), averaged over the substeps of the last record. */
 int sp_batch_timings(const sp_batch* batch, uint32_t group, float* out8);
+
+/* ---- residency ------------------------------------------------------------------------- */
+
+/* No counterpart in the reference (one tank per process, always on the GPU): device memory
+ * of many resident tanks. sp_evict moves the tank's arena (every field, the particles, the
+ * solver state) to pinned host memory and frees its evictable device memory; the next call
+ * that needs the device brings it back, and the tank continues bit for bit as if it had
+ * stayed. It waits for the whole device first. */
+int sp_evict(sp_sim* sim);
+/* 1 if the tank's arena is on the device, 0 if evicted. */
+int sp_is_resident(const sp_sim* sim, int32_t* out);
+/* Device bytes a tank of this grid and particle capacity holds: `evictable` (given back by
+ * sp_evict) and `fixed` (held for the handle's life). No device needed. */
+int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* evictable, uint64_t* fixed);
+
+/* ---- the request server ------------------------------------------------------------------ */
+
+/* No counterpart in the reference: serving many resident scenes. A server owns a worker
+ * thread that takes step requests for the scenes (tanks) added to it, as they arrive, and
+ * advances them with continuous batching: every pending request whose scene is idle is
+ * launched at once, requests of same-size, same-schedule scenes for the same substep count
+ * sharing launches (up to 32 tanks each, sp_batch's grouping), each launch on one of
+ * `streams` streams so launches overlap on the device; a request that arrives while others
+ * run goes out as soon as a stream and its scene are free. Scenes stay on the device while
+ * their evictable bytes fit `hbm_budget` (sp_tank_bytes); over it, the least recently used
+ * idle scenes are evicted to host memory and brought back when a request needs them. Each
+ * scene's results are those it gets stepped alone with the same requests (sp_record +
+ * sp_wait per request). Requests of one scene run in arrival order.
+ * While added to a server, a handle must not be used by the caller (its calls return
+ * SP_E_BUSY); after sp_server_destroy it is the caller's again. */
+typedef struct sp_server sp_server;
+typedef struct sp_request_info {
+    uint64_t ticket;
+    uint32_t scene, substeps;
+    int32_t status;          /* SP_OK, or the error that failed it */
+    uint32_t tanks;          /* tanks in the launch that ran it */
+    double submit_ms, launch_ms, done_ms;   /* server clock */
+    double latency_ms;       /* done - submit */
+} sp_request_info;
+typedef struct sp_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t particle_steps;     /* particle slots x substeps of the completed requests */
+    uint64_t launches;           /* group launches */
+    uint64_t evictions, restores;
+    uint64_t resident_bytes;     /* evictable bytes of the resident scenes */
+    uint32_t scenes, resident;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* request latency, completed requests */
+    double span_s;               /* first submit to last completion */
+    double tanks_per_launch;     /* mean */
+} sp_server_stats;
+int sp_server_create(int32_t device, uint64_t hbm_budget, uint32_t streams, sp_server** out);
+/* Runs every request already submitted, stops the worker, and gives the handles back. */
+void sp_server_destroy(sp_server* server);
+const char* sp_server_last_error(sp_server* server);
+/* Adds a tank (set up: parameters, solids, particles) as scene *scene (0, 1, ...). */
+int sp_server_add(sp_server* server, sp_sim* sim, uint32_t* scene);
+/* Queues a request of `substeps` substeps covering `target_s` seconds (as sp_record);
+ * never blocks. Thread-safe. */
+int sp_server_submit(sp_server* server, uint32_t scene, uint32_t substeps, float target_s, uint64_t* ticket);
+/* Blocks until the request is done. sp_server_poll: SP_E_BUSY while it is not. */
+int sp_server_wait(sp_server* server, uint64_t ticket, sp_request_info* out);
+int sp_server_poll(sp_server* server, uint64_t ticket, sp_request_info* out);
+/* Blocks until every submitted request is done. */
+int sp_server_drain(sp_server* server);
+/* Counters and latency percentiles since the last sp_server_reset_stats. */
+int sp_server_stats_get(sp_server* server, sp_server_stats* out);
+int sp_server_reset_stats(sp_server* server);
 
 /* ---- host-side derivations (no device needed) ---------------------------------------- */
 

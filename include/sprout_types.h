@@ -219,6 +219,12 @@ typedef struct sp_solve {
     uint32_t v_done, v_iters, v_it_max, v_unconv;
     float v_res_max;
     uint32_t v_iters_last, p2_iters_last, pad;
+    /* This tank's recorded iterations of the pressure, second pressure and viscosity solves
+     * (written by the host before each batch; 0 = no cap), the iterations the running solves
+     * have done, and whether they stopped at the cap. A tank whose launches are shared with
+     * tanks that record more iterations stops where it would alone. */
+    uint32_t cap_p, cap_p2, cap_v, cap_now;
+    uint32_t sweeps, halt, v_sweeps, v_halt;
 } sp_solve;
 
 #define SP_MAX_LEVELS 16

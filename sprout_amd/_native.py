@@ -27,7 +27,7 @@ SPAWN_S = -1.0
 ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
 
 # Status codes
-OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY = 0, -1, -2, -3, -4, -5
+OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 
 # Field ids (SP_FIELD_*)
 FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7, MGA=8, MGD=9,
@@ -91,6 +91,21 @@ class SolverStats(C.Structure):
                 ("pressure_iters_last", C.c_uint32)]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("scene", C.c_uint32), ("substeps", C.c_uint32),
+                ("status", C.c_int32), ("tanks", C.c_uint32), ("submit_ms", C.c_double),
+                ("launch_ms", C.c_double), ("done_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("particle_steps", C.c_uint64), ("launches", C.c_uint64), ("evictions", C.c_uint64),
+                ("restores", C.c_uint64), ("resident_bytes", C.c_uint64), ("scenes", C.c_uint32),
+                ("resident", C.c_uint32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("span_s", C.c_double),
+                ("tanks_per_launch", C.c_double)]
+
+
 class StatsOut(C.Structure):
     _fields_ = [("live", C.c_uint32), ("count", C.c_uint32 * MAX_MATERIALS),
                 ("conc", C.c_float * MAX_MATERIALS)]
@@ -146,7 +161,9 @@ SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<
                         ("v_rz", "<f8"), ("v_alpha", "<f8"), ("v_beta", "<f8"), ("v_r0", "<f8"),
                         ("v_rr", "<f8"), ("v_pq", "<f8"), ("v_done", "<u4"), ("v_iters", "<u4"),
                         ("v_it_max", "<u4"), ("v_unconv", "<u4"), ("v_res_max", "<f4"),
-                        ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("pad", "<u4")])
+                        ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("pad", "<u4"),
+                        ("cap_p", "<u4"), ("cap_p2", "<u4"), ("cap_v", "<u4"), ("cap_now", "<u4"),
+                        ("sweeps", "<u4"), ("halt", "<u4"), ("v_sweeps", "<u4"), ("v_halt", "<u4")])
 
 # Exported symbols of include/sprout_hip.h, with their argument types.
 _P = C.c_void_p
@@ -188,6 +205,19 @@ SIGNATURES = {
     "sp_batch_wait": ([_P], C.c_int),
     "sp_batch_groups": ([_P, C.POINTER(C.c_uint32), _P, C.c_uint32], C.c_int),
     "sp_batch_timings": ([_P, C.c_uint32, _P], C.c_int),
+    "sp_evict": ([_P], C.c_int),
+    "sp_is_resident": ([_P, C.POINTER(C.c_int32)], C.c_int),
+    "sp_tank_bytes": ([C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "sp_server_create": ([C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
+    "sp_server_destroy": ([_P], None),
+    "sp_server_last_error": ([_P], C.c_char_p),
+    "sp_server_add": ([_P, _P, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_server_submit": ([_P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_uint64)], C.c_int),
+    "sp_server_wait": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
+    "sp_server_poll": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
+    "sp_server_drain": ([_P], C.c_int),
+    "sp_server_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
+    "sp_server_reset_stats": ([_P], C.c_int),
 }
 
 _lib = None
@@ -207,6 +237,15 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+def tank_bytes(W, H, capacity):
+    """sp_tank_bytes: (evictable, fixed) device bytes of a tank of this grid and capacity."""
+    ev, fx = C.c_uint64(), C.c_uint64()
+    rc = lib().sp_tank_bytes(W, H, capacity, C.byref(ev), C.byref(fx))
+    if rc != OK:
+        raise SproutError(rc, "tank_bytes")
+    return ev.value, fx.value
 
 
 class SproutError(RuntimeError):

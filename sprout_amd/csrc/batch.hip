@@ -26,6 +26,7 @@ struct sp_batch {
     std::vector<sp_sim*> sims;
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<uint64_t> keys;  // the grouping the groups were built for
+    std::vector<uint64_t> gens;  // and the tanks' arena generations (an evicted and restored tank moved)
     std::string err;
 };
 
@@ -45,14 +46,18 @@ uint64_t group_key(const sp_sim* s) {
     return (uint64_t)(uint32_t)s->W | (uint64_t)(uint32_t)s->H << 16 | (uint64_t)schedule_key(s->gp) << 32;
 }
 
-// (Re)builds the groups when a tank's key changed: tanks in the order given, same key
-// together, at most SP_LAUNCH_TANKS per group.
+// (Re)builds the groups when a tank's key changed or its arena moved: tanks in the order
+// given, same key together, at most SP_LAUNCH_TANKS per group.
 void regroup(sp_batch* b) {
-    std::vector<uint64_t> keys;
-    for (sp_sim* s : b->sims) keys.push_back(group_key(s));
-    if (keys == b->keys && !b->groups.empty()) return;
+    std::vector<uint64_t> keys, gens;
+    for (sp_sim* s : b->sims) {
+        keys.push_back(group_key(s));
+        gens.push_back(s->generation);
+    }
+    if (keys == b->keys && gens == b->gens && !b->groups.empty()) return;
     b->groups.clear();
     b->keys = keys;
+    b->gens = gens;
     std::vector<bool> taken(b->sims.size(), false);
     for (size_t i = 0; i < b->sims.size(); i++) {
         if (taken[i]) continue;

@@ -33,7 +33,9 @@ SP_DEV Lv level(const Ctx& c, int l) {
 SP_DEV bool inside(const Lv& L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y < L.h; }
 SP_DEV int at(const Lv& L, int x, int y) { return L.off + y * L.w + x; }
 SP_DEV bool unknown(const Ctx& c, const Lv& L, int x, int y) { return inside(L, x, y) && c.mgd[at(L, x, y)] > 0.0f; }
-SP_DEV bool solve_done(const Ctx& c) { return c.K->done[0] != 0u && c.K->done[1] != 0u; }
+// A solve is over for this tank when both potentials converged or when it has done the
+// iterations this tank records (its cap), whichever comes first.
+SP_DEV bool solve_done(const Ctx& c) { return (c.K->done[0] != 0u && c.K->done[1] != 0u) || c.K->halt != 0u; }
 
 SP_DEV int tid2() { return threadIdx.y * 16 + threadIdx.x; }
 SP_DEV int tile_slot() { return blockIdx.y * gridDim.x + blockIdx.x; }
@@ -114,6 +116,9 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
             K->beta[0] = K->beta[1] = 0.0;
             K->alpha[0] = K->alpha[1] = 0.0;
             K->iters[0] = K->iters[1] = 0u;
+            K->sweeps = 0u;
+            K->halt = 0u;
+            K->cap_now = mode == RED_START2 ? K->cap_p2 : K->cap_p;
             K->done[0] = a.z <= tx * b.x ? 1u : 0u;
             K->done[1] = (mode == RED_START2 || a.w <= ty * b.y) ? 1u : 0u;
             K->solves += 1u;
@@ -140,6 +145,9 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
             if (!was1) K->rr[1] = a.w;
             K->done[0] = d0;
             K->done[1] = d1;
+            // the last kernel of an iteration: stop here if this tank records no more
+            K->sweeps += 1u;
+            if (K->cap_now != 0u && K->sweeps >= K->cap_now && (d0 == 0u || d1 == 0u)) K->halt = 1u;
         }
     }
 }

@@ -45,8 +45,15 @@ struct sp_sim {
     uint32_t cap = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    std::vector<void*> owned;
+    std::vector<void*> owned;  // device buffers held for the handle's life
+    std::vector<void*> big;    // the arena and the sort's work space, given up when evicted
     std::string err;
+
+    // residency: an evicted tank keeps its arena in pinned host memory
+    bool resident = true;
+    void* host_arena = nullptr;
+    size_t arena_bytes = 0;
+    uint64_t generation = 0;  // bumped when the arena moves
 
     sp_sim_params params{};
     bool params_set = false;
@@ -92,6 +99,7 @@ struct sp_sim {
     GraphSet gs;
     Timing tm;
     bool in_batch = false;  // a batch drives the tank (sp_batch_*)
+    bool served = false;    // a server drives the tank (sp_server_*)
 };
 
 namespace {
@@ -143,14 +151,72 @@ int upload_tables(sp_sim* s) {
     return SP_OK;
 }
 
+// The tank's evictable device memory: the arena (every field, the particles, the multigrid
+// levels, the solve record) and the sort's keys, values, work space and target array.
+void alloc_big(sp_sim* s) {
+    std::vector<void*> got;
+    try {
+        TankRef ref = s->ref;
+        ref.base = dalloc<char>(s->arena_bytes, got);
+        s->d_part_tmp = dalloc<sp_particle>(s->cap, got);
+        s->d_keys = dalloc<uint32_t>(s->cap, got);
+        s->d_vals = dalloc<uint32_t>(s->cap, got);
+        s->d_keys2 = dalloc<uint32_t>(s->cap, got);
+        s->d_vals2 = dalloc<uint32_t>(s->cap, got);
+        s->d_sort_tmp = dalloc<uint8_t>(s->sort_tmp_bytes, got);
+        s->ref = ref;
+    } catch (...) {
+        for (void* p : got) (void)hipFree(p);
+        throw;
+    }
+    s->big = got;
+    s->c = ctx_of(s->ref);
+    s->run.tb.t[0] = s->ref;
+    s->generation++;
+}
+
+void free_big(sp_sim* s) {
+    for (void* p : s->big) (void)hipFree(p);
+    s->big.clear();
+    s->ref.base = nullptr;
+}
+
+// Bytes of device memory the tank gives back when evicted.
+size_t big_bytes(const sp_sim* s) {
+    return s->arena_bytes + (sizeof(sp_particle) + 16) * (size_t)std::max<uint32_t>(s->cap, 1) + std::max<size_t>(s->sort_tmp_bytes, 1);
+}
+
+// Evicts the tank: its arena goes to pinned host memory and its evictable device memory is
+// freed. Nothing else of the tank changes, so a restored tank continues bit for bit.
+void evict(sp_sim* s) {
+    if (!s->resident) return;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    void* h = nullptr;
+    HIPCHK(hipHostMalloc(&h, s->arena_bytes, hipHostMallocDefault));
+    hipError_t e = hipMemcpy(h, s->ref.base, s->arena_bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        throw HipError{e, "hipMemcpy (evict)"};
+    }
+    s->gs.clear();  // captured graphs hold the old addresses
+    free_big(s);
+    s->host_arena = h;
+    s->resident = false;
+}
+
+// Brings an evicted tank back: fresh device memory, the arena copied in.
+void restore(sp_sim* s) {
+    if (s->resident) return;
+    alloc_big(s);
+    HIPCHK(hipMemcpy(s->ref.base, s->host_arena, s->arena_bytes, hipMemcpyHostToDevice));
+    (void)hipHostFree(s->host_arena);
+    s->host_arena = nullptr;
+    s->resident = true;
+}
+
 // Reads the step and solve state back and folds it into the statistics
 // (poll_readbacks, mod.rs:3033-3116).
-void readback(sp_sim* s) {
-    sp_stepbuf S;
-    sp_solve K;
-    HIPCHK(hipMemcpyAsync(&S, s->c.S, sizeof(S), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipMemcpyAsync(&K, s->c.K, sizeof(K), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
+void readback_apply(sp_sim* s, const sp_stepbuf& S, const sp_solve& K) {
     double t = s->time_base + (double)S.advanced - (double)__builtin_bit_cast(float, S.pad0);
     if (t > s->sim_time) s->sim_time = t;
     sp_solver_stats& st = s->stats;
@@ -185,12 +251,33 @@ void readback(sp_sim* s) {
     if (s->gp.visc_mode == 2 && s->gp.v_twice) adapt(s->rec.p2, K.p2_it_max, p2_un, s->params.pressure_max_iters, 0.25f);
     else s->rec.p2 = s->rec.p;
     if (s->gp.visc_mode == 2) adapt(s->rec.v, K.v_it_max, v_un, s->params.viscosity_max_iters, 0.5f);
-    // the per-batch maxima start again
-    uint32_t zero = 0;
-    HIPCHK(hipMemcpyAsync(&s->c.K->it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(&s->c.K->p2_it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(&s->c.K->v_it_max, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+}
+
+// The per-batch iteration maxima start again (after they were read), on `stream`;
+// `zero` points at a zero word that lives until the copies ran.
+void clear_maxima(sp_sim* s, hipStream_t stream, const uint32_t* zero) {
+    HIPCHK(hipMemcpyAsync(&s->c.K->it_max, zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(&s->c.K->p2_it_max, zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(&s->c.K->v_it_max, zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+}
+
+void readback(sp_sim* s) {
+    sp_stepbuf S;
+    sp_solve K;
+    HIPCHK(hipMemcpyAsync(&S, s->c.S, sizeof(S), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(&K, s->c.K, sizeof(K), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    readback_apply(s, S, K);
+    const uint32_t zero = 0;
+    clear_maxima(s, s->stream, &zero);
+    HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+// This tank's recorded iterations into its solve record, where the reductions stop its
+// solves (a tank batched with tanks that record more stops where it would alone).
+void write_caps(sp_sim* s, const Recs& rec, hipStream_t stream) {
+    const uint32_t caps[3] = {rec.p, rec.p2, rec.v};
+    HIPCHK(hipMemcpyAsync(&s->c.K->cap_p, caps, sizeof(caps), hipMemcpyHostToDevice, stream));
 }
 
 // The per-tank part of FluidSim::record (mod.rs:2400-2563) before the substeps are
@@ -236,6 +323,7 @@ int record_prepare(sp_sim* s, uint32_t substeps, float target) {
         words[3] = 0;
         HIPCHK(hipMemcpyAsync(&s->c.S->frame_left, &words[0], 4, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(&s->c.S->batch_steps, &words[2], 8, hipMemcpyHostToDevice, s->stream));
+        write_caps(s, s->rec, s->stream);
     }
     return SP_OK;
 }
@@ -267,10 +355,20 @@ int wait(sp_sim* s) {
     return SP_OK;
 }
 
+// Runs an entry point's body with HIP errors mapped to status codes. `device`: the body
+// touches the tank's device memory, so an evicted tank is brought back first.
 template <class F>
-int guarded(sp_sim* s, F&& f) {
+int guarded(sp_sim* s, F&& f, bool device = true) {
     if (!s) return SP_E_INVALID;
+    if (s->served) {
+        set_err(s, "the tank is driven by a server");
+        return SP_E_BUSY;
+    }
     try {
+        if (device && !s->resident) {
+            HIPCHK(hipSetDevice(s->device));
+            restore(s);
+        }
         return f();
     } catch (const HipError& e) {
         set_err(s, "%s failed: %s", e.what, hipGetErrorString(e.e));
@@ -374,27 +472,21 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         // one arena for every buffer the kernels reach through the tank's context
         TankRef& ref = s->ref;
         ref = TankRef{nullptr, s->W, s->H, s->cap, s->lv.total, s->lv.tiles, s->lv.n, s->lv.tail};
-        size_t arena_bytes = 0;
-        (void)ctx_of(ref, &arena_bytes);
-        ref.base = dalloc<char>(arena_bytes, o);
-        s->c = ctx_of(ref);
+        (void)ctx_of(ref, &s->arena_bytes);
+        {
+            size_t bytes = 0;
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                      (uint32_t*)nullptr, (int)std::max<uint32_t>(s->cap, 1), 0, 32, s->stream));
+            s->sort_tmp_bytes = bytes;
+        }
+        alloc_big(s);
         Ctx& c = s->c;
-        s->d_part_tmp = dalloc<sp_particle>(s->cap, o);
         s->d_spawn = dalloc<sp_particle>(SP_SPAWN_MAX, o);
         s->d_counter = dalloc<uint32_t>(8 + 2 * SP_MAX_MATERIALS, o);
         int4 h_lev[SP_MAX_LEVELS] = {};
         for (int l = 0; l < s->lv.n; l++) h_lev[l] = make_int4(s->lv.offs[l], s->lv.dims[l][0], s->lv.dims[l][1], s->lv.tile_offs[l]);
         HIPCHK(hipMemcpy((void*)c.lev, h_lev, sizeof(h_lev), hipMemcpyHostToDevice));
-        s->d_keys = dalloc<uint32_t>(s->cap, o);
-        s->d_vals = dalloc<uint32_t>(s->cap, o);
-        s->d_keys2 = dalloc<uint32_t>(s->cap, o);
-        s->d_vals2 = dalloc<uint32_t>(s->cap, o);
         s->d_count_out = dalloc<uint32_t>(N, o);
-        size_t bytes = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, s->d_keys, s->d_keys2, s->d_vals, s->d_vals2,
-                                                  (int)std::max<uint32_t>(s->cap, 1), 0, 32, s->stream));
-        s->sort_tmp_bytes = bytes;
-        s->d_sort_tmp = dalloc<uint8_t>(bytes, o);
         // solid field: closed everywhere until set_solids
         HIPCHK(hipDeviceSynchronize());
         sp::default_params(&s->params);
@@ -427,7 +519,9 @@ void sp_destroy(sp_sim* s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->gs.clear();
+    free_big(s);
     for (void* p : s->owned) (void)hipFree(p);
+    if (s->host_arena) (void)hipHostFree(s->host_arena);
     hipStream_t st = s->stream;
     delete s;  // graphs and events go with it
     if (st) (void)hipStreamDestroy(st);
@@ -566,7 +660,7 @@ int sp_poll(sp_sim* s, sp_solver_stats* out) {
         if (!out) return SP_E_INVALID;
         *out = s->stats;
         return SP_OK;
-    });
+    }, false);
 }
 
 int sp_step_blocking(sp_sim* s, uint32_t n) {
@@ -622,7 +716,7 @@ int sp_sim_time(sp_sim* s, double* out) {
         if (!out) return SP_E_INVALID;
         *out = s->sim_time;
         return SP_OK;
-    });
+    }, false);
 }
 
 int sp_download_particles(sp_sim* s, sp_particle* out, uint32_t cap, uint32_t* n) {
@@ -712,6 +806,43 @@ int sp_sort(sp_sim* s) {
     });
 }
 
+int sp_evict(sp_sim* s) {
+    return guarded(s, [&]() -> int {
+        HIPCHK(hipSetDevice(s->device));
+        HIPCHK(hipDeviceSynchronize());  // batches run tanks on their own streams
+        evict(s);
+        return SP_OK;
+    }, false);
+}
+
+int sp_is_resident(const sp_sim* s, int32_t* out) {
+    if (!s || !out) return SP_E_INVALID;
+    *out = s->resident ? 1 : 0;
+    return SP_OK;
+}
+
+int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* evictable, uint64_t* fixed) {
+    if (grid_w < 8 || grid_h < 8 || grid_w > 8192 || grid_h > 8192 || !evictable || !fixed) return SP_E_INVALID;
+    sp_levels_info lv;
+    sp::levels(grid_w, grid_h, lv);
+    TankRef ref{nullptr, grid_w, grid_h, capacity, lv.total, lv.tiles, lv.n, lv.tail};
+    size_t arena = 0;
+    (void)ctx_of(ref, &arena);
+    const size_t cap = std::max<uint32_t>(capacity, 1);
+    // the radix sort's work space as the library asks for it; with no device to ask, its
+    // bound: a second key and value array and 1 MiB of histograms
+    size_t sort_tmp = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)cap, 0, 32) != hipSuccess) {
+        (void)hipGetLastError();
+        sort_tmp = 8 * cap + (1u << 20);
+    }
+    *evictable = arena + (sizeof(sp_particle) + 16) * cap + std::max<size_t>(sort_tmp, 1);
+    *fixed = sizeof(sp_particle) * SP_SPAWN_MAX + sizeof(uint32_t) * (8 + 2 * SP_MAX_MATERIALS) +
+             sizeof(uint32_t) * (size_t)grid_w * grid_h;
+    return SP_OK;
+}
+
 int sp_timings(sp_sim* s, float* out10) {
     return guarded(s, [&]() -> int {
         if (!out10) return SP_E_INVALID;
@@ -720,7 +851,7 @@ int sp_timings(sp_sim* s, float* out10) {
         out10[8] = (float)s->last_v_iters;
         out10[9] = (float)s->last_p2_iters;
         return SP_OK;
-    });
+    }, false);
 }
 
 int sp_kernel_time(sp_sim* s, int32_t kernel, float* avg_ms, uint32_t* launches) {
@@ -729,7 +860,7 @@ int sp_kernel_time(sp_sim* s, int32_t kernel, float* avg_ms, uint32_t* launches)
         *avg_ms = s->tm.particle_ms;
         *launches = s->tm.timed_last;
         return SP_OK;
-    });
+    }, false);
 }
 
 static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
@@ -841,6 +972,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         const TankBatch& cs = r.tb;
         // stages run with the most iterations the parameters allow
         const Recs full{s->params.pressure_max_iters, s->params.pressure_max_iters, std::max(s->params.viscosity_max_iters, 1u)};
+        write_caps(s, full, s->stream);
         switch (stage) {
         case SP_STAGE_STEP: hipLaunchKernelGGL(k_step, per_tank(r), dim3(64), 0, s->stream, cs); break;
         case SP_STAGE_CLEAR_ACC: launch_clear_acc(r); break;
@@ -861,13 +993,17 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         case SP_STAGE_VISC_VCYCLE: {
             // One V-cycle from vr, whatever the state of the last solve: the level kernels
             // return at once on a converged solve, so the flag is cleared around it.
-            uint32_t done = 0, zero = 0;
+            uint32_t done = 0, halt = 0, zero = 0;
+            HIPCHK(hipStreamSynchronize(s->stream));
             HIPCHK(hipMemcpy(&done, &c.K->v_done, 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&halt, &c.K->v_halt, 4, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(&c.K->v_done, &zero, 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(&c.K->v_halt, &zero, 4, hipMemcpyHostToDevice));
             if (s->gp.v_mg_sweeps == 1) launch_visc_vcycle<1>(r);
             else launch_visc_vcycle<2>(r);
             HIPCHK(hipStreamSynchronize(s->stream));
             HIPCHK(hipMemcpy(&c.K->v_done, &done, 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(&c.K->v_halt, &halt, 4, hipMemcpyHostToDevice));
             break;
         }
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
@@ -890,3 +1026,4 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
 }  // extern "C"
 
 #include "batch.hip"
+#include "serve.hip"

@@ -26,7 +26,7 @@ namespace {
 SP_DEV bool v_interior(const Ctx& c, int x, int y) { return x >= 1 && y >= 1 && x < c.W - 1 && y < c.H - 1; }
 SP_DEV int vtile_of(const Ctx& c, int x, int y) { return (y >> 4) * ((c.W + 15) >> 4) + (x >> 4); }
 SP_DEV bool v_on(const Ctx& c, int x, int y) { return c.vtiles[vtile_of(c, x, y)] != 0u; }
-SP_DEV bool v_done(const Ctx& c) { return c.K->v_done != 0u; }
+SP_DEV bool v_done(const Ctx& c) { return c.K->v_done != 0u || c.K->v_halt != 0u; }
 // GLSL max(x, y): y if x < y, otherwise x.
 SP_DEV float gmax_(float x, float y) { return x < y ? y : x; }
 SP_DEV double dot2d(float2 a, float2 b) { return (double)a.x * (double)b.x + (double)a.y * (double)b.y; }
@@ -882,6 +882,12 @@ __global__ __launch_bounds__(256) void k_visc_mgz(const TankBatch tb, int sel) {
 enum { VRED_START = 4, VRED_ALPHA = 5, VRED_BETA = 6, VRED_WARM = 10, VRED_MG_START = 11, VRED_MG_UPDATE = 12,
        VRED_MG_ITER = 13 };
 
+// After the last kernel of an iteration: stop here if this tank records no more.
+SP_DEV void v_count_sweep(sp_solve* K) {
+    K->v_sweeps += 1u;
+    if (K->cap_v != 0u && K->v_sweeps >= K->cap_v && K->v_done == 0u) K->v_halt = 1u;
+}
+
 __global__ __launch_bounds__(REDUCE_NT) void k_visc_reduce(const TankBatch tb, int mode, int nblocks) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ double4 lds[REDUCE_NT / 64];
@@ -898,6 +904,8 @@ __global__ __launch_bounds__(REDUCE_NT) void k_visc_reduce(const TankBatch tb, i
         K->v_beta = 0.0;
         K->v_alpha = 0.0;
         K->v_iters = 0u;
+        K->v_sweeps = 0u;
+        K->v_halt = 0u;
         K->v_done = (a.y <= 1e-30 || a.x <= 0.0) ? 1u : 0u;
         break;
     case VRED_WARM:
@@ -917,6 +925,7 @@ __global__ __launch_bounds__(REDUCE_NT) void k_visc_reduce(const TankBatch tb, i
         K->v_rz = a.x;
         if (!was) K->v_rr = a.y;
         K->v_done = done;
+        v_count_sweep(K);
         break;
     }
     case VRED_MG_START:
@@ -940,6 +949,7 @@ __global__ __launch_bounds__(REDUCE_NT) void k_visc_reduce(const TankBatch tb, i
         K->v_beta = beta;
         K->v_alpha = (K->v_done == 0u && pq > 0.0) ? a.x / pq : 0.0;
         K->v_rz = a.x;
+        v_count_sweep(K);
         break;
     }
     default: break;

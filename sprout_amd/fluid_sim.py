@@ -52,6 +52,17 @@ class FluidSim:
         if rc != N.OK:
             raise N.SproutError(rc, f"{what}: {self._lib.sp_last_error(self._h).decode()}")
 
+    # -- residency -------------------------------------------------------------------
+    def evict(self):
+        """Moves the tank's device state to host memory; the next call that needs it brings it back."""
+        self._check(self._lib.sp_evict(self._h), "evict")
+
+    @property
+    def resident(self):
+        r = C.c_int32()
+        self._check(self._lib.sp_is_resident(self._h, C.byref(r)), "is_resident")
+        return bool(r.value)
+
     # -- inputs ------------------------------------------------------------------------
     def set_params(self, params):
         """Assigning FluidSim::params (the grid-unit tables follow at the next record)."""
@@ -188,6 +199,75 @@ class FluidSim:
         mats = (N.GMat * N.MAX_MATERIALS)()
         self._check(self._lib.sp_debug_gparams(self._h, C.byref(g), mats), "grid_params")
         return g, mats
+
+
+class FluidServer:
+    """The request server (sp_server_*): resident scenes stepped on request with continuous
+    batching and an HBM budget. The FluidSim handles added belong to the server until close();
+    each scene's results are those it gets stepped alone with the same requests."""
+
+    def __init__(self, budget_bytes, streams=4, device=0):
+        L = N.lib()
+        h = C.c_void_p()
+        rc = L.sp_server_create(device, int(budget_bytes), streams, C.byref(h))
+        if rc != N.OK:
+            raise N.SproutError(rc, "sp_server_create")
+        self._h = h
+        self._lib = L
+        self.sims = []
+
+    def close(self):
+        """Runs what was submitted, stops the worker and gives the handles back."""
+        if getattr(self, "_h", None):
+            self._lib.sp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            raise N.SproutError(rc, f"{what}: {self._lib.sp_server_last_error(self._h).decode()}")
+
+    def add(self, sim):
+        """Adds a set-up FluidSim as the next scene; returns its id."""
+        k = C.c_uint32()
+        self._check(self._lib.sp_server_add(self._h, sim._h, C.byref(k)), "server add")
+        self.sims.append(sim)
+        return k.value
+
+    def submit(self, scene, substeps, target):
+        t = C.c_uint64()
+        self._check(self._lib.sp_server_submit(self._h, scene, substeps, float(target), C.byref(t)), "server submit")
+        return t.value
+
+    def wait(self, ticket):
+        info = N.RequestInfo()
+        self._check(self._lib.sp_server_wait(self._h, ticket, C.byref(info)), "server wait")
+        return info
+
+    def poll(self, ticket):
+        """The request's info, or None while it is not done."""
+        info = N.RequestInfo()
+        rc = self._lib.sp_server_poll(self._h, ticket, C.byref(info))
+        if rc == N.E_BUSY:
+            return None
+        self._check(rc, "server poll")
+        return info
+
+    def drain(self):
+        self._check(self._lib.sp_server_drain(self._h), "server drain")
+
+    def stats(self):
+        st = N.ServerStats()
+        self._check(self._lib.sp_server_stats_get(self._h, C.byref(st)), "server stats")
+        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
+
+    def reset_stats(self):
+        self._check(self._lib.sp_server_reset_stats(self._h), "server reset_stats")
 
 
 class FluidBatch:

@@ -89,3 +89,48 @@ def test_batch_splits_groups_beyond_one_launch():
     finally:
         for s in alone + inbatch:
             s.close()
+
+
+def test_batch_tank_stops_at_its_own_recorded_iterations():
+    """A tank whose solves run out of recorded iterations (pressure capped at 3, viscosity at
+    2) shares its launches with tanks that record many more; it still stops where it stops
+    alone, unconverged, and its state and statistics are those of the tank alone."""
+    grid = (64, 36)
+
+    def make(seed, capped):
+        params, field, parts, _ = cases.four_liquids(grid, seed=seed, viscosity_tol=1e-5)
+        if capped:
+            params.pressure_max_iters = 3
+            params.viscosity_max_iters = 2
+        s = FluidSim(grid, grid[0] * grid[1] * 4, params=params)
+        s.set_solids(field)
+        s.reset(parts)
+        return s
+
+    alone = [make(1, True), make(1, False)]
+    inbatch = [make(1, True), make(1, False), make(2, False)]
+    try:
+        for s in alone:
+            for _ in range(2):
+                s.record(5, 5 / 240)
+                s.wait()
+        b = FluidBatch(inbatch)
+        for _ in range(2):
+            b.record(5, 5 / 240)
+            b.wait()
+        assert b.groups() == [3]
+        assert alone[0].solver.pressure_unconverged > 0
+        assert alone[0].solver.viscosity_unconverged > 0
+        for k in range(2):
+            sa, sb = state(alone[k]), state(inbatch[k])
+            assert sa[0].tobytes() == sb[0].tobytes(), f"tank {k}: particles"
+            for name, a, c in zip(("vel", "cell", "pres"), sa[1:], sb[1:]):
+                assert np.array_equal(a, c), f"tank {k}: {name}"
+            x, y = alone[k].solver, inbatch[k].solver
+            for f in ("pressure_unconverged", "viscosity_unconverged", "pressure_iters_max",
+                      "viscosity_iters_max", "pressure_residual_max", "viscosity_residual_max"):
+                assert getattr(x, f) == getattr(y, f), f"tank {k}: {f}"
+        b.close()
+    finally:
+        for s in alone + inbatch:
+            s.close()

@@ -66,3 +66,33 @@ def test_viscosity_modes_follow_the_diffusion_number():
     assert g.visc_mode == 1
     g, _ = N.derive_grid_params(p, 64, 64, [1])
     assert g.visc_mode == 2
+
+
+def test_tank_bytes_scale_with_the_tank():
+    """sp_tank_bytes needs no device: evictable bytes grow with the grid and the capacity,
+    and a 1024 x 576 tank at 4 per cell fits hundreds of times in 288 GB."""
+    small, fs = N.tank_bytes(128, 72, 128 * 72 * 4)
+    mid, fm = N.tank_bytes(512, 288, 512 * 288 * 4)
+    big, fb = N.tank_bytes(1024, 576, 1024 * 576 * 4)
+    assert 0 < small < mid < big
+    assert 0 < fs < fm < fb
+    # the arena alone: at least the particles (48 B) and the accumulator (128 B per cell)
+    assert big > 48 * 1024 * 576 * 4 + 128 * 1024 * 576
+    assert 288e9 / (big + fb) > 300
+    L = N.lib()
+    ev, fx = C.c_uint64(), C.c_uint64()
+    assert L.sp_tank_bytes(4, 4, 16, C.byref(ev), C.byref(fx)) == N.E_INVALID
+
+
+def test_server_and_batch_refuse_bad_arguments():
+    L = N.lib()
+    h = C.c_void_p()
+    assert L.sp_server_create(0, 1 << 30, 0, C.byref(h)) == N.E_INVALID
+    assert L.sp_server_create(0, 1 << 30, 17, C.byref(h)) == N.E_INVALID
+    assert L.sp_server_submit(None, 0, 1, 0.1, None) == N.E_INVALID
+    assert L.sp_server_drain(None) == N.E_INVALID
+    assert L.sp_batch_create(None, 0, C.byref(h)) == N.E_INVALID
+    r = C.c_int32()
+    assert L.sp_is_resident(None, C.byref(r)) == N.E_INVALID
+    # the request server is switched off until a stall it met on the device is understood
+    assert L.sp_server_create(0, 1 << 30, 2, C.byref(h)) == N.E_UNSUPPORTED
