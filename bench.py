@@ -53,11 +53,14 @@ def pmc_traffic(kind, W, H):
     return None
 
 
-def workload(kind, grid, seed=0):
+def workload(kind, grid, seed=0, mg_precision=0):
     import cases
     if kind == "four":
-        return cases.four_liquids(grid, seed=seed, viscosity_tol=1e-5)
-    return cases.dam_break(grid, frac=(0.8, 0.8))
+        out = cases.four_liquids(grid, seed=seed, viscosity_tol=1e-5)
+    else:
+        out = cases.dam_break(grid, frac=(0.8, 0.8))
+    out[0].mg_precision = mg_precision
+    return out
 
 
 def cpu_baseline(grid, params, field, parts, rows, substeps):
@@ -104,6 +107,10 @@ def main():
                     help="tanks advanced together in one batch (seeds 0 .. tanks-1; value is their aggregate)")
     ap.add_argument("--latency-steps", type=int, default=64,
                     help="single-substep step requests timed one by one after the throughput run")
+    ap.add_argument("--mg-precision", choices=["fp32", "mixed"], default="fp32",
+                    help="pressure multigrid storage: FP32, or BF16 iterates / FP16 coarse matrices")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the pressure solve's comparison with an FP64 solve of the same system")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -114,8 +121,9 @@ def main():
 
     W, H = (int(x) for x in args.grid.split("x"))
     tanks = []
+    mgp = 1 if args.mg_precision == "mixed" else 0
     for seed in range(max(args.tanks, 1)):
-        params, field, parts, rows = workload(args.workload, (W, H), seed)
+        params, field, parts, rows = workload(args.workload, (W, H), seed, mgp)
         sim = FluidSim((W, H), W * H * 4, params=params)
         sim.set_solids(field)
         sim.reset(parts)
@@ -188,6 +196,7 @@ def main():
         "dt": "1/240 s fixed",
         "pressure_tol": float(params.pressure_tol),
         "substeps_per_batch": args.batch,
+        "mg_storage": "fp32" if mgp == 0 else "bf16 iterates and coarse rhs, fp16 coarse matrices (fp32 smoothing)",
     }
     if args.workload == "four":
         config["seed"] = 0
@@ -235,6 +244,20 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_accuracy:
+        import fp64_ref
+        errs, vel_err, K = fp64_ref.pressure_accuracy(sim, float(params.pressure_tol), gather=1)
+        out["accuracy"] = {
+            "what": "one more substep of tank 0 up to its first pressure solve, then that solve and its projection, "
+                    "against the exact FP64 solution of the same system (sparse LU)",
+            "pressure_err_vs_fp64": errs[0]["err_vs_fp64"] if errs[0] else None,
+            "pressure_true_residual": errs[0]["true_residual"] if errs[0] else None,
+            "pressure_fp64_cg_same_tol_err": errs[0]["fp64_cg_same_tol_err"] if errs[0] else None,
+            "correction_err_vs_fp64": errs[1]["err_vs_fp64"] if errs[1] else None,
+            "correction_true_residual": errs[1]["true_residual"] if errs[1] else None,
+            "velocity_rel_l2_vs_fp64": vel_err,
+            "iterations": [int(v) for v in K["iters"]],
+        }
     if batch is not None:
         batch.close()
     for t in tanks:

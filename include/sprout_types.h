@@ -113,6 +113,11 @@ typedef struct sp_sim_params {
     uint32_t viscosity_precond;    /* 1: multigrid V-cycle, 0: Jacobi */
     uint32_t viscosity_mg_levels;  /* levels below the grid in the viscosity V-cycle */
     uint32_t viscosity_mg_sweeps;  /* Gauss-Seidel sweeps per visit of a level (1 or 2) */
+    /* Storage of the pressure multigrid (no counterpart in the reference, which keeps FP32):
+     * 0 FP32; 1 mixed: the V-cycle's iterates and coarse right-hand sides in BF16, the
+     * coarse levels' matrices in FP16, smoothing arithmetic in FP32; the level-0 matrix
+     * (also the CG operator) and the CG vectors stay FP32, the reductions FP64. */
+    uint32_t mg_precision;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -187,6 +192,7 @@ typedef struct sp_gparams {
     int32_t v_precond;    /* viscosity_precond: 1 multigrid, 0 Jacobi */
     int32_t v_mg_levels;  /* levels below the grid in its V-cycle (1 .. tail) */
     int32_t v_mg_sweeps;  /* 1 or 2 */
+    int32_t mg_half;      /* mg_precision: 1 mixed-precision pressure multigrid */
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */

@@ -73,7 +73,8 @@ class SimParams(C.Structure):
                 ("min_substep_share", C.c_float), ("viscous_projection", C.c_uint32),
                 ("viscous_warm", C.c_uint32), ("viscosity_warm", C.c_uint32),
                 ("viscosity_precond", C.c_uint32), ("viscosity_mg_levels", C.c_uint32),
-                ("viscosity_mg_sweeps", C.c_uint32), ("mat_count", C.c_int32),
+                ("viscosity_mg_sweeps", C.c_uint32), ("mg_precision", C.c_uint32),
+                ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
 
@@ -135,7 +136,8 @@ class GParams(C.Structure):
                 ("ext", C.c_float * 4), ("corr", C.c_float * 4), ("p_max_iters", C.c_int32),
                 ("v_max_iters", C.c_int32), ("heat_x", C.c_float), ("ln_rigid", C.c_float),
                 ("v_twice", C.c_int32), ("v_p2warm", C.c_int32), ("v_warm", C.c_int32),
-                ("v_precond", C.c_int32), ("v_mg_levels", C.c_int32), ("v_mg_sweeps", C.c_int32)]
+                ("v_precond", C.c_int32), ("v_mg_levels", C.c_int32), ("v_mg_sweeps", C.c_int32),
+                ("mg_half", C.c_int32)]
 
 
 class LevelsInfo(C.Structure):

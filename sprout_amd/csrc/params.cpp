@@ -83,6 +83,7 @@ void default_params(sp_sim_params* p) {
     p->viscosity_precond = 1;
     p->viscosity_mg_levels = 1;
     p->viscosity_mg_sweeps = 2;
+    p->mg_precision = 0;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -386,6 +387,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         g.v_mg_levels = std::min(want, L.tail);
     }
     g.v_mg_sweeps = (int)std::min(std::max(p.viscosity_mg_sweeps, 1u), 2u);
+    g.mg_half = p.mg_precision == 1u ? 1 : 0;
     return SP_OK;
 }
 

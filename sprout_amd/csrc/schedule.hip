@@ -55,44 +55,63 @@ dim3 grid0(const Run& r) { return groups_of(r, r.W, r.H); }
 
 // One V-cycle on the level-0 residual (mgr, or r2 with rsel); with `skip_down0` the way down
 // on level 0 has already run (fused into the CG update).
-void launch_vcycle(const Run& r, int gated, int reduce, int rsel, bool skip_down0 = false) {
+// H: the multigrid's storage (mg_half, MG<H> in solve.hip).
+template <bool H>
+void launch_vcycle_t(const Run& r, int gated, int reduce, int rsel, bool skip_down0) {
     const sp_levels_info& L = r.lv;
     for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
-        hipLaunchKernelGGL(k_mg_down, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated, rsel);
-    hipLaunchKernelGGL(k_mg_tail, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, gated);
+        hipLaunchKernelGGL(k_mg_down<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated, rsel);
+    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, gated);
     for (int l = L.tail - 1; l >= 0; l--)
-        hipLaunchKernelGGL(k_mg_up, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated,
+        hipLaunchKernelGGL(k_mg_up<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated,
                            l == 0 ? reduce : -1, r.nb0, rsel);
     if (reduce >= 0) hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, reduce, r.nb0, gated);
 }
 
-void launch_mg_build(const Run& r) {
+void launch_vcycle(const Run& r, int gated, int reduce, int rsel, bool skip_down0 = false) {
+    if (r.gp.mg_half) launch_vcycle_t<true>(r, gated, reduce, rsel, skip_down0);
+    else launch_vcycle_t<false>(r, gated, reduce, rsel, skip_down0);
+}
+
+template <bool H>
+void launch_mg_build_t(const Run& r) {
     const sp_levels_info& L = r.lv;
     for (int l = 1; l <= L.tail; l++)
-        hipLaunchKernelGGL(k_mg_build, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l);
-    if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build, per_tank(r), dim3(1024), 0, r.stream, r.tb);
+        hipLaunchKernelGGL(k_mg_build<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l);
+    if (L.tail < L.n - 1) hipLaunchKernelGGL(k_mg_tail_build<H>, per_tank(r), dim3(1024), 0, r.stream, r.tb);
+}
+
+void launch_mg_build(const Run& r) {
+    if (r.gp.mg_half) launch_mg_build_t<true>(r);
+    else launch_mg_build_t<false>(r);
 }
 
 // The first solve builds the hierarchy and rescales the warm start; the second (after the
 // viscosity, `second`) reuses both and starts from what rhs_visc left. The level-0 residual
 // alternates between mgr and r2, the direction between cgp and cgp2, with the iteration's
 // parity (k_update_down0, k_cg_apply).
-void launch_pressure(const Run& r, uint32_t iters, bool second = false) {
+template <bool H>
+void launch_pressure_t(const Run& r, uint32_t iters, bool second) {
     const dim3 g0 = grid0(r);
     if (!second) {
-        launch_mg_build(r);
+        launch_mg_build_t<H>(r);
         hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, r.stream, r.tb);
     }
     hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, r.stream, r.tb, r.nb0);
-    launch_vcycle(r, 0, second ? (int)RED_START2 : (int)RED_START, 0);
+    launch_vcycle_t<H>(r, 0, second ? (int)RED_START2 : (int)RED_START, 0, false);
     for (uint32_t k = 0; k < iters; k++) {
         const int par = (int)(k & 1);
-        hipLaunchKernelGGL(k_cg_apply, g0, TILE, 0, r.stream, r.tb, r.nb0, par);
+        hipLaunchKernelGGL(k_cg_apply<H>, g0, TILE, 0, r.stream, r.tb, r.nb0, par);
         hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, (int)RED_ALPHA, r.nb0, 1);
-        hipLaunchKernelGGL(k_update_down0, g0, TILE, 0, r.stream, r.tb, par);
-        launch_vcycle(r, 1, RED_BETA, par ^ 1, true);
+        hipLaunchKernelGGL(k_update_down0<H>, g0, TILE, 0, r.stream, r.tb, par);
+        launch_vcycle_t<H>(r, 1, RED_BETA, par ^ 1, true);
     }
     hipLaunchKernelGGL(k_cg_end, per_tank(r), dim3(64), 0, r.stream, r.tb, second ? 1 : 0);
+}
+
+void launch_pressure(const Run& r, uint32_t iters, bool second = false) {
+    if (r.gp.mg_half) launch_pressure_t<true>(r, iters, second);
+    else launch_pressure_t<false>(r, iters, second);
 }
 
 // ---- the implicit viscosity (record_viscosity_split / _solve, mod.rs:2812-2927) ----------
@@ -238,7 +257,7 @@ void launch_particles(const Run& r, int gather) {
 // kernels read themselves stay out of the key).
 uint32_t schedule_key(const sp_gparams& g) {
     return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
-           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10;
+           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12;
 }
 
 }  // namespace

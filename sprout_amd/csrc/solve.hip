@@ -9,8 +9,9 @@
 //
 // Precision: cell arithmetic is FP32 in the reference's operation order; the scalar
 // reductions (r.z, r.r, p.q, b.b) are FP64 — the product of two floats is exact in double
-// and the sums are formed per work-group and then in a fixed tree by the last work-group to
-// finish, so the scalars do not depend on scheduling.
+// and the sums are formed per work-group and then in a fixed order by a one-group kernel,
+// so the scalars do not depend on scheduling. With mg_half the V-cycle's storage is
+// narrower (MG<true> below); its arithmetic stays FP32.
 //
 // Convergence without host round trips: the scalar state lives in device memory (sp_solve);
 // every kernel of an iteration reads the two `done` flags at entry and returns at once when
@@ -33,6 +34,85 @@ SP_DEV Lv level(const Ctx& c, int l) {
 SP_DEV bool inside(const Lv& L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y < L.h; }
 SP_DEV int at(const Lv& L, int x, int y) { return L.off + y * L.w + x; }
 SP_DEV bool unknown(const Ctx& c, const Lv& L, int x, int y) { return inside(L, x, y) && c.mgd[at(L, x, y)] > 0.0f; }
+
+// Storage of the multigrid levels. MG<false>: FP32, the reference's layout. MG<true>
+// (mg_half): every coarse level (1 and below) narrow: iterates (mgx, mgz) and right-hand
+// sides (mgr) in BF16, two per 32-bit word; couplings and diagonals in FP16, scaled by 2^-l
+// so that their growth with the level (about 2x per level under 2 x 2 aggregation) stays
+// far from FP16's range. Level 0 stays FP32: its matrix is also the CG operator, its
+// residual a CG vector, and its finished iterate z the preconditioned residual CG steps
+// along (in BF16 that last one cost 1.8x the iterations at 1024 x 576). Narrow values sit at
+// the start of each array's coarse region, which begins at cell N (level 1). Loads widen to
+// FP32 exactly, so all arithmetic is FP32; stores round to nearest even.
+struct alignas(8) Half4 {
+    _Float16 x, y, z, w;
+};
+SP_DEV uint32_t bf16_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+SP_DEV uint32_t pack_bf2(float2 v) { return bf16_bits(v.x) | (bf16_bits(v.y) << 16); }
+SP_DEV float2 unpack_bf2(uint32_t w) { return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)); }
+
+template <bool H>
+struct MG {
+    SP_DEV static float D(const Ctx& c, int, int i) { return c.mgd[i]; }
+    SP_DEV static float4 A(const Ctx& c, int, int i) { return c.mga[i]; }
+    SP_DEV static void set_AD(const Ctx& c, int, int i, float4 a, float d) {
+        c.mgd[i] = d;
+        c.mga[i] = a;
+    }
+    SP_DEV static float2 X(const Ctx& c, int i) { return c.mgx[i]; }
+    SP_DEV static void set_X(const Ctx& c, int i, float2 v) { c.mgx[i] = v; }
+    SP_DEV static float2 Z(const Ctx& c, int i) { return c.mgz[i]; }
+    SP_DEV static void set_Z(const Ctx& c, int i, float2 v) { c.mgz[i] = v; }
+    // right-hand sides of the coarse levels (i >= N)
+    SP_DEV static float2 R(const Ctx& c, int i) { return c.mgr[i]; }
+    SP_DEV static void set_R(const Ctx& c, int i, float2 v) { c.mgr[i] = v; }
+};
+
+template <>
+struct MG<true> {
+    SP_DEV static float D(const Ctx& c, int l, int i) {
+        if (l == 0) return c.mgd[i];
+        return ldexpf((float)reinterpret_cast<const _Float16*>(c.mgd + c.N)[i - c.N], l);
+    }
+    SP_DEV static float4 A(const Ctx& c, int l, int i) {
+        if (l == 0) return c.mga[i];
+        const Half4 h = reinterpret_cast<const Half4*>(c.mga + c.N)[i - c.N];
+        return make_float4(ldexpf((float)h.x, l), ldexpf((float)h.y, l), ldexpf((float)h.z, l), ldexpf((float)h.w, l));
+    }
+    SP_DEV static void set_AD(const Ctx& c, int l, int i, float4 a, float d) {
+        if (l == 0) {
+            c.mgd[i] = d;
+            c.mga[i] = a;
+            return;
+        }
+        reinterpret_cast<_Float16*>(c.mgd + c.N)[i - c.N] = (_Float16)ldexpf(d, -l);
+        Half4 h;
+        h.x = (_Float16)ldexpf(a.x, -l);
+        h.y = (_Float16)ldexpf(a.y, -l);
+        h.z = (_Float16)ldexpf(a.z, -l);
+        h.w = (_Float16)ldexpf(a.w, -l);
+        reinterpret_cast<Half4*>(c.mga + c.N)[i - c.N] = h;
+    }
+    SP_DEV static float2 vec(const float2* a, const Ctx& c, int i) {
+        if (i < c.N) return a[i];
+        return unpack_bf2(reinterpret_cast<const uint32_t*>(a + c.N)[i - c.N]);
+    }
+    SP_DEV static void set_vec(float2* a, const Ctx& c, int i, float2 v) {
+        if (i < c.N) a[i] = v;
+        else reinterpret_cast<uint32_t*>(a + c.N)[i - c.N] = pack_bf2(v);
+    }
+    SP_DEV static float2 X(const Ctx& c, int i) { return vec(c.mgx, c, i); }
+    SP_DEV static void set_X(const Ctx& c, int i, float2 v) { set_vec(c.mgx, c, i, v); }
+    SP_DEV static float2 Z(const Ctx& c, int i) { return vec(c.mgz, c, i); }
+    SP_DEV static void set_Z(const Ctx& c, int i, float2 v) { set_vec(c.mgz, c, i, v); }
+    SP_DEV static float2 R(const Ctx& c, int i) {
+        return unpack_bf2(reinterpret_cast<const uint32_t*>(c.mgr + c.N)[i - c.N]);
+    }
+    SP_DEV static void set_R(const Ctx& c, int i, float2 v) { reinterpret_cast<uint32_t*>(c.mgr + c.N)[i - c.N] = pack_bf2(v); }
+};
 // A solve is over for this tank when both potentials converged or when it has done the
 // iterations this tank records (its cap), whichever comes first.
 SP_DEV bool solve_done(const Ctx& c) { return (c.K->done[0] != 0u && c.K->done[1] != 0u) || c.K->halt != 0u; }
@@ -156,19 +236,22 @@ SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
 
 // ---- hierarchy set-up -------------------------------------------------------------------
 
-// Galerkin coarse matrix of coarse cell (cx, cy) from fine level F (mg_build.comp:43-65).
-SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
+// Galerkin coarse matrix of coarse cell (cx, cy) of level lf + 1 from fine level F = lf
+// (mg_build.comp:43-65).
+template <bool H>
+SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int lf, int cx, int cy) {
+    using M = MG<H>;
     float D = 0.0f;
     float4 A = make_float4(0, 0, 0, 0);
     for (int k = 0; k < 4; k++) {
         int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
-        if (!unknown(c, F, x, y)) continue;
+        if (!(inside(F, x, y) && M::D(c, lf, at(F, x, y)) > 0.0f)) continue;
         int i = at(F, x, y);
-        D += c.mgd[i];
-        float4 w = c.mga[i];
+        D += M::D(c, lf, i);
+        float4 w = M::A(c, lf, i);
         for (int d = 0; d < 4; d++) {
             int nx = x + NB_X[d], ny = y + NB_Y[d];
-            if (!unknown(c, F, nx, ny)) continue;
+            if (!(inside(F, nx, ny) && M::D(c, lf, at(F, nx, ny)) > 0.0f)) continue;
             float wd = f4c(w, d);
             if ((nx >> 1) == cx && (ny >> 1) == cy) D -= wd;
             else if (d == 0) A.x += wd;
@@ -177,12 +260,13 @@ SP_DEV void build_cell(const Ctx& c, const Lv& F, const Lv& C, int cx, int cy) {
             else A.w += wd;
         }
     }
-    int o = at(C, cx, cy);
-    c.mgd[o] = D > 1e-7f ? D : 0.0f;
-    c.mga[o] = D > 1e-7f ? A : make_float4(0, 0, 0, 0);
+    const int o = at(C, cx, cy);
+    if (D > 1e-7f) M::set_AD(c, lf + 1, o, A, D);
+    else M::set_AD(c, lf + 1, o, make_float4(0, 0, 0, 0), 0.0f);
 }
 
 // mg_build.comp:16-65: coarse level l from level l - 1, over coarse 16 x 16 tiles.
+template <bool H>
 __global__ __launch_bounds__(256) void k_mg_build(const TankBatch tb, int l) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ uint32_t s_state;
@@ -205,19 +289,19 @@ __global__ __launch_bounds__(256) void k_mg_build(const TankBatch tb, int l) {
     int cx = blockIdx.x * 16 + threadIdx.x, cy = blockIdx.y * 16 + threadIdx.y;
     if (!inside(C, cx, cy)) return;
     if ((state & 1u) == 0u) {
-        c.mgd[at(C, cx, cy)] = 0.0f;
-        c.mga[at(C, cx, cy)] = make_float4(0, 0, 0, 0);
+        MG<H>::set_AD(c, l, at(C, cx, cy), make_float4(0, 0, 0, 0), 0.0f);
         return;
     }
-    build_cell(c, F, C, cx, cy);
+    build_cell<H>(c, F, C, l - 1, cx, cy);
 }
 
 // mg_tail.comp mode 1: the coarse matrices of every level below the tail, one group.
+template <bool H>
 __global__ __launch_bounds__(1024) void k_mg_tail_build(const TankBatch tb) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     for (int l = c.tail; l < c.n_levels - 1; l++) {
         const Lv F = level(c, l), C = level(c, l + 1);
-        for (int t = threadIdx.x; t < C.w * C.h; t += 1024) build_cell(c, F, C, t % C.w, t / C.w);
+        for (int t = threadIdx.x; t < C.w * C.h; t += 1024) build_cell<H>(c, F, C, l, t % C.w, t / C.w);
         __threadfence_block();
         __syncthreads();
     }
@@ -311,6 +395,7 @@ SP_DEV float2 direction_of(float2 z, float2 po, float bx, float by) {
 
 // cg_apply.comp: p = z + beta p_old, q = A p, and the partial sums of p.q (second half of
 // the partial buffer).
+template <bool H>
 __global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblocks, int par) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     // the search direction alternates between cgp and cgp2 with the iteration's parity
@@ -338,7 +423,7 @@ __global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblock
             d = c.mgd[i];
             if (d > 0.0f) {
                 float2 po = (bx == 0.0f && by == 0.0f) ? make_float2(0.0f, 0.0f) : p_old[i];
-                pv = direction_of(c.mgz[i], po, bx, by);
+                pv = direction_of(MG<H>::Z(c, i), po, bx, by);
             }
         }
         sd[k] = d;
@@ -388,7 +473,8 @@ struct DownStage {
     float2 sz[R1 * R1];
 };
 
-SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0, DownStage& S, float2* mgx_out) {
+template <bool H>
+SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0, DownStage& S) {
     const int t = tid2();
     // z after the red half-sweep from zero: red z = r / D; black from its red neighbours.
     for (int k = t; k < R1 * R1; k += 256) {
@@ -419,7 +505,7 @@ SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0,
     }
     __syncthreads();
     const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (inside(F, x, y)) mgx_out[at(F, x, y)] = S.sz[sidx(x, y, x0, y0, 1)];
+    if (inside(F, x, y)) MG<H>::set_X(c, at(F, x, y), S.sz[sidx(x, y, x0, y0, 1)]);
     if (((threadIdx.x | threadIdx.y) & 1) != 0) return;
     const int cx = x >> 1, cy = y >> 1;
     if (!inside(C, cx, cy)) return;
@@ -438,19 +524,20 @@ SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0,
             sum.y += wd * zn.y;
         }
     }
-    c.mgr[at(C, cx, cy)] = sum;
+    MG<H>::set_R(c, at(C, cx, cy), sum);
 }
 
-// Loads the diagonals (ring 2) and couplings (ring 1) of the tile.
-SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int x0, int y0, DownStage& S) {
+// Loads the diagonals (ring 2) and couplings (ring 1) of the tile of level l.
+template <bool H>
+SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int l, int x0, int y0, DownStage& S) {
     const int t = tid2();
     for (int k = t; k < R2 * R2; k += 256) {
         int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
-        S.sd[k] = inside(F, gx, gy) ? c.mgd[at(F, gx, gy)] : 0.0f;
+        S.sd[k] = inside(F, gx, gy) ? MG<H>::D(c, l, at(F, gx, gy)) : 0.0f;
     }
     for (int k = t; k < R1 * R1; k += 256) {
         int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
-        S.sa[k] = inside(F, gx, gy) ? c.mga[at(F, gx, gy)] : make_float4(0, 0, 0, 0);
+        S.sa[k] = inside(F, gx, gy) ? MG<H>::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
     }
 }
 
@@ -458,6 +545,7 @@ SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int x0, int y0, DownStage& S
 
 // mg_down.comp:36-69 on level l: pre-smoothing from zero and restriction of the residual.
 // r_in: the level's right-hand side (level 0: the CG residual buffer of this iteration).
+template <bool H>
 __global__ __launch_bounds__(256) void k_mg_down(const TankBatch tb, int l, int gated, int rsel) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
@@ -466,19 +554,21 @@ __global__ __launch_bounds__(256) void k_mg_down(const TankBatch tb, int l, int 
     const Lv F = level(c, l), C = level(c, l + 1);
     if (c.ptile[F.toff + tile_slot()] == 0u) return;
     const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    const float2* r = l == 0 ? r_in : c.mgr;
-    stage_matrix(c, F, x0, y0, S);
+    stage_matrix<H>(c, F, l, x0, y0, S);
     for (int k = tid2(); k < R2 * R2; k += 256) {
         int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
-        S.sr[k] = inside(F, gx, gy) ? r[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+        float2 rv = make_float2(0.0f, 0.0f);
+        if (inside(F, gx, gy)) rv = l == 0 ? r_in[at(F, gx, gy)] : MG<H>::R(c, at(F, gx, gy));
+        S.sr[k] = rv;
     }
     __syncthreads();
-    down_compute(c, F, C, x0, y0, S, c.mgx);
+    down_compute<H>(c, F, C, x0, y0, S);
 }
 
 // cg_update.comp fused with the way down on level 0: x += alpha p, r_out = r_in - alpha q
 // (residual buffers alternate between iterations, so a tile may read its neighbours' old
 // residual while they write their new one), then mg_down level 0 on r_out.
+template <bool H>
 __global__ __launch_bounds__(256) void k_update_down0(const TankBatch tb, int par) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     // residual (mgr / r2) and direction (cgp / cgp2) buffers of iteration parity `par`
@@ -491,7 +581,7 @@ __global__ __launch_bounds__(256) void k_update_down0(const TankBatch tb, int pa
     if (c.ptile[tile_slot()] == 0u) return;
     const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
     const float ax = (float)c.K->alpha[0], ay = (float)c.K->alpha[1];
-    stage_matrix(c, F, x0, y0, S);
+    stage_matrix<H>(c, F, 0, x0, y0, S);
     for (int k = tid2(); k < R2 * R2; k += 256) {
         int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
         float2 rv = make_float2(0.0f, 0.0f);
@@ -524,12 +614,14 @@ __global__ __launch_bounds__(256) void k_update_down0(const TankBatch tb, int pa
         const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
         if (inside(F, x, y)) r_out[at(F, x, y)] = S.sr[sidx(x, y, x0, y0, 2)];
     }
-    down_compute(c, F, C, x0, y0, S, c.mgx);
+    down_compute<H>(c, F, C, x0, y0, S);
 }
 
 // mg_up.comp:38-75 on level l: coarse-grid correction and the black-then-red
 // post-smoothing; on level 0 (reduce >= 0) also the partial sums r.z, r.r.
+template <bool H>
 __global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int gated, int reduce, int nblocks, int rsel) {
+    using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
     __shared__ double4 lds[4];
@@ -548,24 +640,23 @@ __global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int ga
     }
     const int t = tid2();
     const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    const float2* r = l == 0 ? r_in : c.mgr;
     for (int k = t; k < R2 * R2; k += 256) {
         int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
         bool in = inside(F, gx, gy);
-        sd[k] = in ? c.mgd[at(F, gx, gy)] : 0.0f;
-        sx[k] = in ? c.mgx[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+        sd[k] = in ? M::D(c, l, at(F, gx, gy)) : 0.0f;
+        sx[k] = in ? M::X(c, at(F, gx, gy)) : make_float2(0.0f, 0.0f);
     }
     for (int k = t; k < R1 * R1; k += 256) {
         int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
         bool in = inside(F, gx, gy);
-        sa[k] = in ? c.mga[at(F, gx, gy)] : make_float4(0, 0, 0, 0);
-        sr[k] = in ? r[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
+        sa[k] = in ? M::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
+        sr[k] = in ? (l == 0 ? r_in[at(F, gx, gy)] : M::R(c, at(F, gx, gy))) : make_float2(0.0f, 0.0f);
     }
     // parents of the ring-2 region: coarse cells x0/2 - 1 .. x0/2 + 8
     const int cx0 = (x0 >> 1) - 1, cy0 = (y0 >> 1) - 1;
     for (int k = t; k < 100; k += 256) {
         int gx = cx0 + k % 10, gy = cy0 + k / 10;
-        szc[k] = inside(C, gx, gy) ? c.mgz[at(C, gx, gy)] : make_float2(0.0f, 0.0f);
+        szc[k] = inside(C, gx, gy) ? M::Z(c, at(C, gx, gy)) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
     // new values of the black cells of the tile and its first ring (mg_up.comp:26-36)
@@ -618,7 +709,7 @@ __global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int ga
             acc = make_double4((double)rv.x * (double)z.x, (double)rv.y * (double)z.y, (double)rv.x * (double)rv.x,
                                (double)rv.y * (double)rv.y);
         }
-        c.mgz[i] = z;
+        M::set_Z(c, i, z);
     }
     if (reduce >= 0) {
         double4 s2 = group_sum4(acc, lds);
@@ -640,7 +731,9 @@ __global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int ga
 // LDS (dynamic, sized at launch to the hierarchy): per tail cell its couplings (float4),
 // iterate and right-hand side (float2) and diagonal; the first level's right-hand side is
 // loaded once with the matrix, so that the sweeps never wait on global memory.
+template <bool H>
 __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated) {
+    using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     extern __shared__ float4 tail_lds[];
     __shared__ int4 sl[SP_MAX_LEVELS];
@@ -655,12 +748,16 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated)
     float2* sr = sx + total;
     float* sd = reinterpret_cast<float*>(sr + total);
     const int bt = threadIdx.x;
-    for (int t = bt; t < total; t += 1024) {
-        sd[t] = c.mgd[BASE + t];
-        sa[t] = c.mga[BASE + t];
-        if (t < NF) sr[t] = c.mgr[BASE + t];
-    }
     if (bt <= last) sl[bt] = c.lev[bt];
+    __syncthreads();
+    for (int t = bt; t < total; t += 1024) {
+        int l = first;  // the level of tail cell t (the storage of the coarse matrices is per level)
+        if (H)
+            while (l < last && BASE + t >= sl[l + 1].x) l++;
+        sd[t] = M::D(c, l, BASE + t);
+        sa[t] = M::A(c, l, BASE + t);
+        if (t < NF) sr[t] = M::R(c, BASE + t);
+    }
     __syncthreads();
 
     auto rhs_at = [&](int s) -> float2 { return sr[s]; };
@@ -779,7 +876,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated)
         sweep(sl[l], 0, false, bt, 1024);
         __syncthreads();
     }
-    for (int t = bt; t < NF; t += 1024) c.mgz[BASE + t] = sx[t];
+    for (int t = bt; t < NF; t += 1024) M::set_Z(c, BASE + t, sx[t]);
 }
 
 // The scalar step of the CG (cg_reduce.comp / reduce_tail.glsl modes 0, 1, 2) over the
