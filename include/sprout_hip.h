@@ -58,10 +58,12 @@ const char* sp_create_error(void);
  * NaN values are refused (src/sim/params.rs:986-997). */
 int sp_set_params(sp_sim* sim, const sp_sim_params* params);
 
-/* Replaces `FluidSim::set_solids(SolidField)` (src/sim/mod.rs:1977, src/scene.rs:110-122):
- * sdf[(W+1)*(H+1)] node distances, solid[4*W*H] (open u face, open v face, smoothed solid
- * occupancy, open volume), region[W*H] (region number | SP_REGION_VENT). temp may be NULL
- * (insulating solids); a non-NULL temp is refused (heat is not carried by this build). */
+/* Replaces `FluidSim::set_solids(SolidField)` (src/sim/mod.rs:1977, 2437-2448,
+ * src/scene.rs:110-122): sdf[(W+1)*(H+1)] node distances, solid[4*W*H] (open u face, open v
+ * face, smoothed solid occupancy, open volume), region[W*H] (region number |
+ * SP_REGION_VENT), temp[W*H] the temperature (deg C) a solid is held at, SP_NO_TEMP where
+ * solids insulate, or NULL when no solid has a temperature. A non-NULL temp makes the tank
+ * thermal (enable_heat, src/sim/mod.rs:2163-2177). */
 int sp_set_solids(sp_sim* sim, const float* sdf, const float* solid, const uint32_t* region,
                   const float* temp);
 
@@ -100,6 +102,11 @@ int sp_download_particles(sp_sim* sim, sp_particle* out, uint32_t cap, uint32_t*
 int sp_download_grid(sp_sim* sim, float* vel, float* cell);
 /* `FluidSim::download_pressure` (src/sim/mod.rs:3464-3475): p*dt per cell, and dt. */
 int sp_download_pressure(sp_sim* sim, float* pres, float* dt);
+/* `FluidSim::download_ledger` (src/sim/mod.rs:3389-3406): energy that crossed the boundary
+ * since the last reset, J per metre of depth: [0] gained from air, [1] lost to air, [2]
+ * gained from solids, [3] lost to solids, [4]-[7] heat tool, cool tool, boiled off, made
+ * inside (0 here: those passes are not carried). */
+int sp_download_ledger(sp_sim* sim, double* energy8);
 /* `FluidSim::cell_counts(lo, hi)` (src/sim/mod.rs:3354-3386, count.comp): live particles
  * per cell in [x0, x0+w) x [y0, y0+h), row by row, into out[w*h]. */
 int sp_cell_counts(sp_sim* sim, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out);
@@ -209,10 +216,11 @@ int sp_server_reset_stats(sp_server* server);
 void sp_default_params(sp_sim_params* out);
 
 /* Tank::layout (src/sim/mod.rs:271-331) + gpu_params / gpu_materials (src/sim/mod.rs:
- * 2179-2330) for an isothermal tank holding the table rows `rows[0..n_rows)` (ascending):
- * fills the grid-unit parameter block and the material table (SP_MAX_MATERIALS rows). */
+ * 2179-2330) for a tank holding the table rows `rows[0..n_rows)` (ascending), isothermal
+ * or (thermal = 1) carrying temperatures: fills the grid-unit parameter block and the
+ * material table (SP_MAX_MATERIALS rows). */
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h,
-                          const int32_t* rows, int32_t n_rows, sp_gparams* out,
+                          const int32_t* rows, int32_t n_rows, int32_t thermal, sp_gparams* out,
                           sp_gmat* mats);
 
 /* Levels::new (src/sim/mod.rs:727-753): the multigrid hierarchy of a W x H grid. */

@@ -74,6 +74,11 @@ typedef struct sp_material {
     float spawn_temp;      /* deg C */
     float latent;          /* J/kg */
     float heat_capacity;   /* J/(kg K) */
+    float conductivity;    /* W/(m K) */
+    float emissivity;      /* grey body, at the free surface */
+    float boil;            /* deg C (1e9: never boils) */
+    float latent_vap;      /* J/kg */
+    float freeze_shift;    /* K per unit concentration (a concentrate freezes like its diluent, shifted) */
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -118,6 +123,15 @@ typedef struct sp_sim_params {
      * coarse levels' matrices in FP16, smoothing arithmetic in FP32; the level-0 matrix
      * (also the CG operator) and the CG vectors stay FP32, the reductions FP64. */
     uint32_t mg_precision;
+    /* Heat (params.rs:168-193): conduction speed-up over real time (0: no heat pass), the
+     * air's film coefficient (W/(m^2 K)), the heat transfer coefficient between different
+     * liquids (W/(m^2 K)), the Smagorinsky constant of the sub-grid mixing (0 off), and
+     * evaporative cooling of aqueous surfaces (1 on). */
+    float heat_rate;
+    float air_cooling;
+    float contact_heat;
+    float eddy_heat;
+    uint32_t evaporation;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -158,7 +172,10 @@ typedef struct sp_gmat {
     int32_t fam, rheo, temp_model, residue;   /* ib */
     float rho, ln_nu, sigma, mix_rate;        /* a: relative density, ln nu (cells^2/s), tension/rho_w, 1/s */
     float cos_c, sin_c, grain, rho_solid;     /* b */
-    float visc_exp, expansion, load, pad;     /* misc */
+    float visc_exp, expansion, load, freeze_shift;  /* misc */
+    float th[4];     /* volumetric heat capacity / water's; conductivity (cells^2/s as water's); solidus, liquidus (deg C) */
+    float th2[4];    /* latent heat as degrees (L / c); emissivity; boiling point (deg C); latent heat of vaporisation as degrees */
+    float temp[4];   /* temperature law: B, C (K); reference temperature (K); 0 */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -193,6 +210,15 @@ typedef struct sp_gparams {
     int32_t v_mg_levels;  /* levels below the grid in its V-cycle (1 .. tail) */
     int32_t v_mg_sweeps;  /* 1 or 2 */
     int32_t mg_half;      /* mg_precision: 1 mixed-precision pressure multigrid */
+    /* Heat (GpuParams heat, heat2, heat3; mod.rs:2299-2321), used where caps has CAP_THERMAL: */
+    float heat_rate;      /* heat.y: conduction speed-up (0: no heat pass) */
+    float film;           /* heat.z: air film coefficient, cells / s */
+    float radiation;      /* heat.w: Stefan-Boltzmann constant over water's heat per cell, cells / s / K^3 */
+    float contact;        /* heat2.w: heat transfer coefficient between different liquids, cells / s */
+    float eddy;           /* heat3.x: Smagorinsky constant of the sub-grid mixing (0 off) */
+    float evap;           /* heat3.y: L_vap / c_p,air (K; 0: no evaporation) */
+    int32_t chan_t;       /* heat2.z: accumulator channel of the temperature sum (-1: isothermal) */
+    int32_t pad_heat;
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -263,6 +289,15 @@ typedef struct sp_levels_info {
 #define SP_TAIL_WARP_CELLS 128
 #define SP_SORT_INTERVAL 128
 #define SP_SORT_MIN_PARTICLES 16384
+/* Capabilities (common.glsl:341-352) */
+#define SP_CAP_FAMILIES 1u
+#define SP_CAP_THERMAL 2u
+#define SP_CAP_RIGID 4u
+/* Heat (common.glsl:431-436) */
+#define SP_NO_TEMP (-1e9f)
+#define SP_LEDGER_SCALE 65536.0f
+#define SP_T_SCALE 4096.0f
+#define SP_LEDGER_WORDS 16
 
 /* Internal fields (sp_debug_download / sp_debug_upload). Sizes: cells N = W*H. */
 enum {
@@ -309,6 +344,11 @@ enum {
     SP_FIELD_VMX = 40,       /* float2[total] */
     SP_FIELD_VMZ = 41,       /* float2[total] */
     SP_FIELD_P2KEEP = 42,    /* float[N] */
+    SP_FIELD_THERM = 43,     /* float4[N]  temperature, heat capacity, conductivity, frozen share */
+    SP_FIELD_DHEAT = 44,     /* float4[N]  change of enthalpy temperature, exchange share, mean, 0 */
+    SP_FIELD_HCELL = 45,     /* int2[N]    heat capacity and heat sums of the particles in the cell */
+    SP_FIELD_SOLID_T = 46,   /* float[N]   temperature a solid is held at (-1e9: insulating) */
+    SP_FIELD_LEDGER = 47,    /* uint32[16] energy entries 0-7, 64-bit fixed point */
     SP_FIELD_COUNT
 };
 
@@ -333,6 +373,7 @@ enum {
     SP_STAGE_PRESSURE2 = 16,    /* the second pressure solve */
     SP_STAGE_PROJECT_VISC = 17, /* project_visc.comp */
     SP_STAGE_VISC_VCYCLE = 18,  /* one viscosity V-cycle: vr -> vmz level 0 (and vaz) */
+    SP_STAGE_HEAT = 19,         /* heat.comp */
     SP_STAGE_COUNT
 };
 
@@ -348,7 +389,7 @@ enum {
 #endif
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
-SP_STATIC_ASSERT(sizeof(sp_gmat) == 80, "one grid-unit material row is 20 words");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 128, "one grid-unit material row is 32 words");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 
 #endif

@@ -13,10 +13,11 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              PRHS=10, PRES=11, STEP=12, SOLVE=13, SPROP=14, AUX=15, GRHO=16, VEL0=17, MGZ=18,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
-             VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42)
+             VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
+             LEDGER=47)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
-             VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17, VISC_VCYCLE=18)
+             VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17, VISC_VCYCLE=18, HEAT=19)
 
 _lib = None
 
@@ -36,7 +37,7 @@ def lib():
         L.or_create.restype = P
         L.or_destroy.argtypes = [P]
         L.or_set_gparams.argtypes = [P, P, P]
-        L.or_set_solids.argtypes = [P, P, P, P]
+        L.or_set_solids.argtypes = [P, P, P, P, P]
         L.or_reset.argtypes = [P, P, C.c_uint32]
         L.or_begin_batch.argtypes = [P, C.c_float, C.c_uint32]
         L.or_substep.argtypes = [P, C.c_int]
@@ -77,7 +78,10 @@ class OracleSim:
         sdf = np.ascontiguousarray(field.sdf, np.float32)
         solid = np.ascontiguousarray(field.solid, np.float32)
         region = np.ascontiguousarray(field.region, np.uint32)
-        self._lib.or_set_solids(self._h, sdf.ctypes.data, solid.ctypes.data, region.ctypes.data)
+        temp = getattr(field, "temp", None)
+        temp = None if temp is None or len(temp) == 0 else np.ascontiguousarray(temp, np.float32)
+        self._lib.or_set_solids(self._h, sdf.ctypes.data, solid.ctypes.data, region.ctypes.data,
+                                None if temp is None else temp.ctypes.data)
 
     def reset(self, particles):
         p = np.ascontiguousarray(particles)

@@ -56,6 +56,11 @@ struct or_sim {
     v2 *vmd, *vmr, *vmx, *vmz;
     sp_stepbuf S;
     sp_solve K;
+    /* heat */
+    v4 *therm, *dheat;
+    int32_t* hcell;   /* 2 per cell */
+    float* solid_t;
+    uint32_t ledger[SP_LEDGER_WORDS];
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -105,6 +110,81 @@ static float row_density(const or_sim* s, int row, float sc) {
 }
 
 /* terminal (common.glsl:370-374) */
+/* ---- heat helpers (common.glsl:408-453, 485-530, 579-606; particles.comp:67-80) ------- */
+
+static int thermal(const or_sim* s) { return (s->P.caps & SP_CAP_THERMAL) != 0u; }
+
+/* temp_law (common.glsl:410-417): ln of the viscosity factor of row r at T (deg C). */
+static float temp_law(const or_sim* s, int r, float T) {
+    const sp_gmat* m = &s->mats[r];
+    if (m->temp_model == 0) return 0.0f;
+    float Tk = fmaxf_(T + 273.15f, 1.0f);
+    float x = m->temp_model == 1 ? m->temp[0] * (1.0f / Tk - 1.0f / m->temp[2])
+                                 : m->temp[0] / fmaxf_(Tk - m->temp[1], 5.0f) - m->temp[0] / fmaxf_(m->temp[2] - m->temp[1], 5.0f);
+    return clampf(x, -30.0f, 30.0f);
+}
+
+/* expansion (common.glsl:485-494): density factor of row r at concentration sc and T. */
+static float expansion(const or_sim* s, int row, float sc, float T) {
+    const sp_gmat* m = &s->mats[row];
+    float b = m->expansion, tr = m->temp[2] - 273.15f, tb = m->th2[2];
+    if (m->diluent >= 0) {
+        const sp_gmat* d = &s->mats[m->diluent];
+        b = mixf(d->expansion, b, sc);
+        tr = mixf(d->temp[2] - 273.15f, tr, sc);
+        tb = mixf(d->th2[2], tb, sc);
+    }
+    return clampf(1.0f - b * (fminf_(T, tb) - tr), 0.7f, 1.3f);
+}
+
+/* thermal_of (common.glsl:500-509): solidus, liquidus, latent heat (K), heat capacity. */
+static v4 thermal_of(const or_sim* s, int row, float sc) {
+    const sp_gmat* m = &s->mats[row];
+    float ld = m->th2[0];
+    if (m->diluent < 0) return (v4){m->th[2], m->th[3], ld, m->th[0]};
+    const sp_gmat* d = &s->mats[m->diluent];
+    float shift = m->freeze_shift * sc;
+    float C = mixf(d->th[0], m->th[0], sc);
+    return (v4){d->th[2] + shift, d->th[3] + shift, mixf(d->th[0] * d->th2[0], m->th[0] * ld, sc) / C, C};
+}
+
+/* phase_of (common.glsl:515-521): temperature and solid share of enthalpy temperature te. */
+static v2 phase_of(float te, v4 m) {
+    if (te >= m.y) return (v2){te, 0.0f};
+    float lo = m.x - m.z;
+    if (te <= lo) return (v2){te + m.z, 1.0f};
+    float fr = (te - lo) / (m.y - lo);
+    return (v2){m.x + fr * (m.y - m.x), 1.0f - fr};
+}
+
+/* heat_quanta (particles.comp:70-73) */
+static void heat_quanta(float C, float T, int32_t q[2]) {
+    int ci = (int)floorf(C * SP_MASS_SCALE + 0.5f);
+    q[0] = ci;
+    q[1] = (int)floorf((float)ci * (T + 273.15f) / SP_T_SCALE + 0.5f);
+}
+
+/* keep_heat (particles.comp:78-80) */
+static float keep_heat(float t, float c0, float c1) { return c0 == c1 ? t : (t + 273.15f) * c0 / c1 - 273.15f; }
+
+/* The boiling point of row at concentration sc (particles.comp:368, 957). */
+static float boil_of(const or_sim* s, int row, float sc) {
+    const sp_gmat* m = &s->mats[row];
+    return m->diluent >= 0 ? mixf(s->mats[m->diluent].th2[2], m->th2[2], sc) : m->th2[2];
+}
+
+/* group_heat_chan (common.glsl:582-584) */
+static int group_heat_chan(const or_sim* s, int g) { return (int)((s->P.group_heat >> (8u * (uint32_t)g)) & 255u) - 1; }
+
+/* ledger_add (common.glsl:446-451): a 64-bit fixed-point sum in two words. */
+static void ledger_add(or_sim* s, int e, float v) {
+    uint32_t x = (uint32_t)(v * SP_LEDGER_SCALE + 0.5f);
+    if (x == 0u) return;
+    uint32_t old = s->ledger[2 * e];
+    s->ledger[2 * e] = old + x;
+    if (old + x < old) s->ledger[2 * e + 1] += 1u;
+}
+
 static float terminal(const or_sim* s, float drho, float rho_c, float nu_c, float d) {
     float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * s->P.gravity * d;
     return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
@@ -236,8 +316,8 @@ static void stage_step(or_sim* s) {
 
 typedef struct {
     int live;
-    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc;
-    int cw, cs;
+    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc, CT;
+    int cw, cs, ct, cg;
 } scatter_in;
 
 static void acc_add(or_sim* s, int x, int y, int ch, int v) {
@@ -288,6 +368,12 @@ static void deposit(or_sim* s, const scatter_in* g) {
                 if (g->cw >= 0) acc_add(s, bx + a, by + b, g->cw, wi);
                 if (g->cs >= 0 && g->sc > 0.0f)
                     acc_add(s, bx + a, by + b, g->cs, (int)floorf(w[0][a] * w[1][b] * g->sc * SP_MASS_SCALE + 0.5f));
+                /* heat: C (T + 273.15) / T_SCALE into the sum and the group's own (particles.comp:1047-1051) */
+                if (g->ct >= 0) {
+                    int h = (int)floorf(w[0][a] * w[1][b] * g->CT * SP_MASS_SCALE + 0.5f);
+                    acc_add(s, bx + a, by + b, g->ct, h);
+                    if (g->cg >= 0) acc_add(s, bx + a, by + b, g->cg, h);
+                }
             }
     }
 }
@@ -342,6 +428,7 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
     const sp_gmat* M = &s->mats[row];
     int Mgroup = M->group, Mdil = M->diluent, Mcw = M->chan_w, Mcs = M->chan_s;
     int fam = M->fam;
+    const int th_on = thermal(s);
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C[4] = {p.c[0], p.c[1], p.c[2], p.c[3]};
@@ -397,6 +484,24 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                         }
                     }
                 }
+            /* Heat (particles.comp:353-373): the change of enthalpy temperature the heat pass
+             * worked out for the cells this particle scattered to, with the same weights, and
+             * the exchange among the particles of its own cell against its own temperature
+             * as it entered the cell's sums. */
+            if (th_on && (p.kind & SP_KIND_FRESH) == 0u) {
+                float dT = 0.0f;
+                for (int b = 0; b < 3; b++)
+                    for (int a = 0; a < 3; a++) dT += wx[a] * wy[b] * s->dheat[cidx(s, bx + a, by + b)].x;
+                int ox = clampi((int)floorf(p.pos[0]), 0, s->W - 1), oy = clampi((int)floorf(p.pos[1]), 0, s->H - 1);
+                v4 own = s->dheat[cidx(s, ox, oy)];
+                if (own.y > 0.0f) {
+                    v4 th = thermal_of(s, row, p.s);
+                    int32_t q[2];
+                    heat_quanta(th.w, fminf_(phase_of(p.t, th).x, boil_of(s, row, p.s)), q);
+                    dT += own.y * (own.z - (float)q[1] / (float)q[0]) * SP_T_SCALE;
+                }
+                p.t += dT;
+            }
             if (sw > 1e-4f) {
                 /* Drift between phase groups (particles.comp:374-477). */
                 sgx /= sw;
@@ -489,6 +594,7 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                         float s0 = diluent0 ? 0.0f : p.s;
                         int rr = diluent0 ? P->primary : row;
                         float s1 = clampf(s0 + (sfs / sfw - s0) * (1.0f - expf(-s->mats[rr].mix_rate * dt)), 0.0f, 1.0f);
+                        float c0 = th_on ? thermal_of(s, row, p.s).w : 0.0f;
                         if (!diluent0) {
                             p.s = s1;
                         } else if (s1 > 0.0f) {
@@ -501,6 +607,8 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                             p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
                             p.s = s1;
                         }
+                        /* the same heat content at the new heat capacity (particles.comp:541) */
+                        if (th_on) p.t = keep_heat(p.t, c0, thermal_of(s, row, p.s).w);
                     }
                 }
                 /* A pure diluent joins a family at concentration 0 (particles.comp:544-569). */
@@ -606,6 +714,28 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
         s->part[id] = p;
     }
     float rho_p = row_density(s, row, p.s);
+    /* Thermal state for the scatter (particles.comp:951-969): temperature capped at the
+     * boiling point, the expansion and the solid's density where frozen, and the particle's
+     * quanta into its own cell's sums for the exchange inside the cell. */
+    float CT = 0.0f;
+    if (th_on) {
+        const sp_gmat* Mr = &s->mats[row];
+        v4 th = thermal_of(s, row, p.s);
+        v2 ph = phase_of(p.t, th);
+        float tb = boil_of(s, row, p.s);
+        rho_p *= expansion(s, row, p.s, ph.x);
+        if (ph.y > 0.0f) {
+            float rs = Mr->diluent >= 0 ? mixf(s->mats[Mr->diluent].rho_solid, Mr->rho_solid, p.s) : Mr->rho_solid;
+            rho_p = mixf(rho_p, rs, ph.y);
+        }
+        CT = th.w * (fminf_(ph.x, tb) + 273.15f) / SP_T_SCALE;
+        int hx = clampi((int)floorf(p.pos[0]), 0, s->W - 1), hy = clampi((int)floorf(p.pos[1]), 0, s->H - 1);
+        int32_t q[2];
+        heat_quanta(th.w, fminf_(ph.x, tb), q);
+        size_t hc = (size_t)cidx(s, hx, hy);
+        s->hcell[2 * hc] += q[0];
+        s->hcell[2 * hc + 1] += q[1];
+    }
     g->live = 1;
     g->px = p.pos[0];
     g->py = p.pos[1];
@@ -619,6 +749,9 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
     g->sc = p.s;
     g->cw = Mcw;
     g->cs = Mcs;
+    g->CT = CT;
+    g->ct = th_on ? s->P.chan_t : -1;
+    g->cg = th_on ? group_heat_chan(s, s->mats[row].group) : -1;
 }
 
 static void stage_particles(or_sim* s, int gather) {
@@ -634,6 +767,7 @@ static void stage_particles(or_sim* s, int gather) {
 
 static void stage_clear_acc(or_sim* s) {
     memset(s->acc, 0, sizeof(int32_t) * (size_t)s->N * s->P.acc_stride);
+    if (thermal(s)) memset(s->hcell, 0, sizeof(int32_t) * 2 * (size_t)s->N);
 }
 
 /* ---- grid_prep.comp (grid_prep.comp:13-263) ------------------------------------------- */
@@ -650,6 +784,7 @@ static void stage_grid_prep(or_sim* s) {
     int nmix = P->n_fam - 1;
     int any_sorts = group_sorts(s, 1) || group_sorts(s, 2) || group_sorts(s, 3);
     int multi = (P->group_mask & (P->group_mask - 1u)) != 0u;
+    const int th_on = thermal(s);
     float dt = s->S.dt;
     for (int cy = 0; cy < H; cy++)
         for (int cx = 0; cx < W; cx++) {
@@ -681,6 +816,7 @@ static void stage_grid_prep(or_sim* s) {
             float rho = 0.0f, mu = 0.0f, rho_c = 0.0f, nu_c = 0.0f;
             int major = 0, top_row = -1;
             float top_share = 0.0f;
+            float T = P->heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
             for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
             if (cw > 0) {
                 int others = 0;
@@ -689,6 +825,7 @@ static void stage_grid_prep(or_sim* s) {
                     if (ch >= 0) others += s->acc[k + ch];
                 }
                 float sh[SP_MAX_PRESENT], scn[SP_MAX_PRESENT];
+                float Csh = 0.0f, Cg[4] = {0, 0, 0, 0};
                 for (int j = 0; j < P->n_present; j++) {
                     sh[j] = 0.0f;
                     scn[j] = 1.0f;
@@ -699,6 +836,37 @@ static void stage_grid_prep(or_sim* s) {
                     if (w <= 0) continue;
                     sh[j] = w >= cw ? 1.0f : clampf((float)w / (float)cw, 0.0f, 1.0f);
                     if (m->diluent >= 0 && m->chan_s >= 0) scn[j] = clampf((float)s->acc[k + m->chan_s] / (float)w, 0.0f, 1.0f);
+                    /* with heat: the cell's heat capacity, conductivity and emissivity
+                     * (grid_prep.comp:100-108) */
+                    if (th_on) {
+                        float cr = sh[j] * thermal_of(s, r, scn[j]).w;
+                        Csh += cr;
+                        Cg[m->group] += cr;
+                        float kp = m->th[1];
+                        kappa += sh[j] * (m->diluent >= 0 ? mixf(s->mats[m->diluent].th[1], kp, scn[j]) : kp);
+                        float em = m->th2[1];
+                        emis += sh[j] * (m->diluent >= 0 ? mixf(s->mats[m->diluent].th2[1], em, scn[j]) : em);
+                    }
+                }
+                /* The temperature of each phase group from its own sum (group_temperatures,
+                 * common.glsl:598-622); T leaves as the cell's mean. */
+                float Tg[4] = {T, T, T, T};
+                if (th_on && Csh > 0.0f) {
+                    int Htot = s->acc[k + P->chan_t];
+                    T = (float)Htot * SP_T_SCALE / (Csh * (float)cw) - 273.15f;
+                    for (int q = 0; q < 4; q++) Tg[q] = T;
+                    int rest = Htot, first = -1;
+                    for (int q = 0; q < 4; q++) {
+                        int ch = group_heat_chan(s, q);
+                        if (ch >= 0) {
+                            int h = s->acc[k + ch];
+                            rest -= h;
+                            if (Cg[q] > 0.0f) Tg[q] = (float)h * SP_T_SCALE / (Cg[q] * (float)cw) - 273.15f;
+                        } else if (first < 0 && (P->group_mask & (1u << q)) != 0u) {
+                            first = q;
+                        }
+                    }
+                    if (first >= 0 && Cg[first] > 0.0f) Tg[first] = (float)rest * SP_T_SCALE / (Cg[first] * (float)cw) - 273.15f;
                 }
                 for (int j = 0; j < P->n_present; j++) {
                     float share = sh[j];
@@ -707,13 +875,15 @@ static void stage_grid_prep(or_sim* s) {
                     const sp_gmat* m = &s->mats[r];
                     float sc = scn[j];
                     float rho_m = m->rho, lnu = m->ln_nu, sig = m->sigma;
-                    float xp = 1.0f;
+                    float Tm = Tg[m->group];
+                    if (th_on) lnu += temp_law(s, r, Tm);
+                    float xp = th_on ? expansion(s, r, sc, Tm) : 1.0f;
                     float csx = m->cos_c, csy = m->sin_c;
                     if (m->diluent >= 0) {
                         const sp_gmat* d = &s->mats[m->diluent];
                         rho_m = mixf(d->rho, rho_m, sc);
                         float bs = m->visc_exp == 1.0f ? sc : powf(sc, m->visc_exp);
-                        lnu = mixf(d->ln_nu, lnu, bs);
+                        lnu = mixf(th_on ? d->ln_nu + temp_law(s, m->diluent, Tm) : d->ln_nu, lnu, bs);
                         sig = mixf(d->sigma, sig, sc);
                         csx = mixf(d->cos_c, csx, sc);
                         csy = mixf(d->sin_c, csy, sc);
@@ -722,6 +892,18 @@ static void stage_grid_prep(or_sim* s) {
                         else if (fm > 0) s->mixb[(size_t)i * nmix + fm - 1] = sc;
                     }
                     rho_m *= xp;
+                    /* frozen liquid: the frozen viscosity and the solid's density
+                     * (grid_prep.comp:150-159) */
+                    if (th_on) {
+                        v4 th = thermal_of(s, r, sc);
+                        float fs = th.y > -1e8f ? clampf((th.y - Tm) / fmaxf_(th.y - th.x, 1e-3f), 0.0f, 1.0f) : 0.0f;
+                        if (fs > 0.0f) {
+                            lnu = mixf(lnu, fmaxf_(lnu, P->ln_rigid), fs);
+                            float rs = m->rho_solid;
+                            rho_m = mixf(rho_m, m->diluent >= 0 ? mixf(s->mats[m->diluent].rho_solid, rs, sc) : rs, fs);
+                            frozen += share * fs;
+                        }
+                    }
                     if (share > top_share) {
                         top_share = share;
                         top_row = r;
@@ -745,6 +927,7 @@ static void stage_grid_prep(or_sim* s) {
                     rho_c = rg[major] / f[major];
                     nu_c = expf(lg[major] / f[major]);
                 }
+                Craw = Csh * (float)cw / SP_MASS_SCALE;
             }
             float phi_t = phi_f + sol.z;
             int has = phi_f > 0.04f && sol.w > 0.02f;
@@ -762,7 +945,125 @@ static void stage_grid_prep(or_sim* s) {
                 s->grho[i] = (v4){rg[0], rg[1], rg[2], rg[3]};
             }
             s->sprop[i] = (v4){sigma, cx_, cy_, (float)major + (float)(top_row + 1) / 128.0f};
+            if (th_on) {
+                s->therm[i] = (v4){T, Craw, kappa, frozen};
+                s->dheat[i] = (v4){P->heat_rate > 0.0f ? emis : 0.0f, 0.0f, 0.0f, 0.0f};
+            }
             if (level <= 0.0f) s->pres[i] = (v2){0.0f, 0.0f};
+        }
+}
+
+/* ---- heat.comp (heat.comp:57-198) -------------------------------------------------------- */
+
+/* y_sat (heat.comp:66-70): mass fraction of water vapour in saturated air at T. */
+static float y_sat(float T) {
+    T = clampf(T, -40.0f, 99.0f);
+    float p = 0.61078f * expf(17.27f * T / (T + 237.3f));
+    return 0.622f * p / (101.325f - 0.378f * p);
+}
+
+/* centre_vel (heat.comp:73-79) */
+static v2 centre_vel(const or_sim* s, int i) {
+    int last = s->W * s->H - 1;
+    v4 v = s->vel[i], vr = s->vel[i + 1 < last ? i + 1 : last], vt = s->vel[i + s->W < last ? i + s->W : last];
+    float ax = face_live(v.z) ? v.x : 0.0f, ay = face_live(v.w) ? v.y : 0.0f;
+    float bx = face_live(vr.z) ? vr.x : 0.0f, by = face_live(vt.w) ? vt.y : 0.0f;
+    return (v2){0.5f * (ax + bx), 0.5f * (ay + by)};
+}
+
+/* The leading material of cell i, + 1 (0: none), as grid_prep left it in sprop.w. */
+static int lead_of(const or_sim* s, int i) {
+    float w = s->sprop[i].w;
+    return (int)((w - floorf(w)) * 128.0f + 0.5f);
+}
+
+static void stage_heat(or_sim* s) {
+    const sp_gparams* P = &s->P;
+    const float PHI_MIN = 0.04f, PR_T = 0.85f;
+    int W = s->W, H = s->H;
+    float k = s->S.dt * P->heat_rate;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 me = s->therm[i];
+            float emis = s->dheat[i].x;
+            float phi = s->cell[i].x;
+            /* exchange inside the cell (heat.comp:88-106) */
+            int32_t hx = s->hcell[2 * (size_t)i], hy = s->hcell[2 * (size_t)i + 1];
+            float own_x = 0.0f, own_y = 0.0f;
+            if (hx > 0 && me.y > 0.0f && phi > 0.0f) {
+                float rate = 5.78f * me.z * 3.14159265f / (P->inv_ppc * me.y * P->inv_ppc / phi);
+                own_x = 1.0f - expf(-k * rate);
+                own_y = (float)hy / (float)hx;
+            }
+            if (phi <= PHI_MIN || me.y <= 0.0f || cx < 1 || cy < 1 || cx >= W - 1 || cy >= H - 1) {
+                s->dheat[i] = (v4){0.0f, own_x, own_y, 0.0f};
+                continue;
+            }
+            phi = fminf_(phi, 1.0f);
+            int nb[4] = {i - 1, i + 1, i - W, i + W};
+            float open[4] = {s->solid[i].x, s->solid[i + 1].x, s->solid[i].y, s->solid[i + W].y};
+            float Tk = me.x + 273.15f, Ta = P->heat_x + 273.15f;
+            float cond = 0.0f, air = 0.0f, wall = 0.0f, contact = 0.0f;
+            float cpc = me.y * P->inv_ppc;
+            float hc = P->contact;
+            int lead = lead_of(s, i);
+            float eddy_l2 = P->eddy * P->eddy / PR_T;
+            int eddy = eddy_l2 > 0.0f && me.w < 0.5f;
+            v2 u_me = eddy ? centre_vel(s, i) : (v2){0.0f, 0.0f};
+            float eddy_max = 0.2f / fmaxf_(k, 1e-9f);
+            float per_rate = 1.0f / fmaxf_(P->heat_rate, 1e-6f);
+            float evap = 0.0f;
+            for (int q = 0; q < 4; q++) {
+                int n = nb[q];
+                float a = open[q];
+                float pn = s->cell[n].x;
+                v4 o = s->therm[n];
+                if (a > 0.0f) {
+                    float pl = (pn > PHI_MIN && o.y > 0.0f) ? fminf_(pn, 1.0f) : 0.0f;
+                    if (pl > 0.0f) {
+                        float gc = 2.0f * me.z * o.z / fmaxf_(me.z + o.z, 1e-12f);
+                        int ln = lead_of(s, n);
+                        if (eddy && o.w < 0.5f && ln == lead) {
+                            v2 un = centre_vel(s, n);
+                            float strain = 1.4142f * len2(un.x - u_me.x, un.y - u_me.y);
+                            float alpha = fminf_(eddy_l2 * strain * per_rate, eddy_max);
+                            float c1 = cpc / phi, c2 = o.y * P->inv_ppc / pl;
+                            gc += alpha * 2.0f * c1 * c2 / fmaxf_(c1 + c2, 1e-12f);
+                        }
+                        cond += a * fminf_(phi, pl) * gc * (o.x - me.x);
+                        if (hc > 0.0f && ln != lead) {
+                            float g1 = fmaxf_(2.0f * me.z, hc), g2 = fmaxf_(2.0f * o.z, hc);
+                            float gx = fmaxf_(g1 * g2 / (g1 + g2) - gc, 0.0f) * a * fminf_(phi, pl);
+                            float cn = o.y * P->inv_ppc;
+                            float x = gx * k * (1.0f / cpc + 1.0f / cn);
+                            contact += 0.25f * (1.0f - expf(-4.0f * x)) * cpc * cn / (cpc + cn) * (o.x - me.x);
+                        }
+                    }
+                    float exposed = fmaxf_(phi - pl, 0.0f);
+                    if (exposed > 0.0f && s->solid[n].w > 0.02f) {
+                        float h = P->film + emis * P->radiation * (Ta * Ta + Tk * Tk) * (Ta + Tk);
+                        air += a * exposed * h * (P->heat_x - me.x);
+                        if (P->evap > 0.0f && lead > 0 && s->mats[lead - 1].group == 0 && s->mats[lead - 1].th2[2] < 150.0f && me.w < 0.5f)
+                            evap += a * exposed * P->film * P->evap * fmaxf_(y_sat(me.x) - y_sat(P->heat_x), 0.0f);
+                    }
+                }
+                if (a < 1.0f) {
+                    float ts = fmaxf_(s->solid_t[n], s->solid_t[i]);
+                    if (ts > 0.5f * SP_NO_TEMP) wall += (1.0f - a) * phi * 2.0f * me.z * (ts - me.x);
+                }
+            }
+            if (evap > 0.0f) air -= evap;
+            air *= k;
+            wall *= k;
+            float lim = cpc * fabsf(P->heat_x - me.x);
+            air = clampf(air, -lim, lim);
+            float e = k * cond + contact + air + wall;
+            s->dheat[i] = (v4){e / cpc, own_x, own_y, 0.0f};
+            if (air > 0.0f) ledger_add(s, 0, air);
+            else ledger_add(s, 1, -air);
+            if (wall > 0.0f) ledger_add(s, 2, wall);
+            else ledger_add(s, 3, -wall);
         }
 }
 
@@ -2418,6 +2719,7 @@ void or_stage(or_sim* s, int stage, int arg) {
         visc_mgz(s, s->vr, s->vq, a3);
         break;
     }
+    case SP_STAGE_HEAT: stage_heat(s); break;
     case SP_STAGE_RHS: stage_rhs(s); break;
     case SP_STAGE_PRESSURE: stage_pressure(s); break;
     case SP_STAGE_PROJECT: stage_project(s); break;
@@ -2441,6 +2743,7 @@ void or_substep(or_sim* s, int gather) {
     stage_surf_smooth(s);
     stage_surf_curv(s);
     stage_surf_apply(s);
+    if (thermal(s) && s->P.heat_rate > 0.0f) stage_heat(s);
     stage_visc_before(s);
     stage_rhs(s);
     stage_pressure(s);
@@ -2489,6 +2792,11 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->vmb = (v4*)calloc(T, sizeof(v4));
     v2** t2[] = {&s->vmd, &s->vmr, &s->vmx, &s->vmz};
     for (size_t k = 0; k < sizeof(t2) / sizeof(t2[0]); k++) *t2[k] = (v2*)calloc(T, sizeof(v2));
+    s->therm = (v4*)calloc(N, sizeof(v4));
+    s->dheat = (v4*)calloc(N, sizeof(v4));
+    s->hcell = (int32_t*)calloc(2 * N, sizeof(int32_t));
+    s->solid_t = (float*)calloc(N, sizeof(float));
+    for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
 
@@ -2519,6 +2827,10 @@ void or_destroy(or_sim* s) {
     free(s->vtile);
     free(s->vma);
     free(s->vmb);
+    free(s->therm);
+    free(s->dheat);
+    free(s->hcell);
+    free(s->solid_t);
     free(s);
 }
 
@@ -2527,10 +2839,13 @@ void or_set_gparams(or_sim* s, const sp_gparams* P, const sp_gmat* mats) {
     memcpy(s->mats, mats, sizeof(sp_gmat) * SP_MAX_MATERIALS);
 }
 
-void or_set_solids(or_sim* s, const float* sdf, const float* solid, const uint32_t* region) {
+/* temp: the temperature each cell's solid is held at, or NULL (insulating everywhere,
+ * mod.rs:2442-2447). */
+void or_set_solids(or_sim* s, const float* sdf, const float* solid, const uint32_t* region, const float* temp) {
     memcpy(s->sdf, sdf, sizeof(float) * (size_t)(s->W + 1) * (s->H + 1));
     memcpy(s->solid, solid, sizeof(v4) * (size_t)s->N);
     memcpy(s->region, region, sizeof(uint32_t) * (size_t)s->N);
+    for (int i = 0; i < s->N; i++) s->solid_t[i] = temp ? temp[i] : SP_NO_TEMP;
 }
 
 void or_reset(or_sim* s, const sp_particle* p, uint32_t n) {
@@ -2542,6 +2857,8 @@ void or_reset(or_sim* s, const sp_particle* p, uint32_t n) {
     memset(s->p2keep, 0, sizeof(float) * (size_t)s->N);
     memset(s->vd, 0, sizeof(v2) * (size_t)s->N);
     memset(&s->S, 0, sizeof(s->S));
+    memset(s->dheat, 0, sizeof(v4) * (size_t)s->N);
+    memset(s->ledger, 0, sizeof(s->ledger));
 }
 
 void or_begin_batch(or_sim* s, float target, uint32_t substeps) {
@@ -2599,6 +2916,11 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_VMX: *bytes = sizeof(v2) * T; return s->vmx;
     case SP_FIELD_VMZ: *bytes = sizeof(v2) * T; return s->vmz;
     case SP_FIELD_P2KEEP: *bytes = sizeof(float) * N; return s->p2keep;
+    case SP_FIELD_THERM: *bytes = sizeof(v4) * N; return s->therm;
+    case SP_FIELD_DHEAT: *bytes = sizeof(v4) * N; return s->dheat;
+    case SP_FIELD_HCELL: *bytes = sizeof(int32_t) * 2 * N; return s->hcell;
+    case SP_FIELD_SOLID_T: *bytes = sizeof(float) * N; return s->solid_t;
+    case SP_FIELD_LEDGER: *bytes = sizeof(s->ledger); return s->ledger;
     default: *bytes = 0; return NULL;
     }
 }

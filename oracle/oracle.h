@@ -21,7 +21,7 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity);
 void or_destroy(or_sim* s);
 /* Grid-unit tables (as sp_derive_grid_params makes them). */
 void or_set_gparams(or_sim* s, const sp_gparams* P, const sp_gmat* mats);
-void or_set_solids(or_sim* s, const float* sdf, const float* solid, const uint32_t* region);
+void or_set_solids(or_sim* s, const float* sdf, const float* solid, const uint32_t* region, const float* temp);
 /* Replaces the particles; clears the warm-start state and the step state (reset). */
 void or_reset(or_sim* s, const sp_particle* p, uint32_t n);
 /* Sets the batch words of StepBuf (record: frame_left, batch_steps, batch_done = 0). */

@@ -34,12 +34,13 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              PRHS=10, PRES=11, STEP=12, SOLVE=13, SPROP=14, AUX=15, GRHO=16, VEL0=17, MGZ=18,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
-             VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42)
+             VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
+             LEDGER=47)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18)
+             VISC_VCYCLE=18, HEAT=19)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -54,7 +55,9 @@ class Material(C.Structure):
                 ("tension", C.c_float), ("contact_angle", C.c_float), ("mix_rate", C.c_float),
                 ("expansion", C.c_float), ("melt_lo", C.c_float), ("melt_hi", C.c_float),
                 ("visc_exponent", C.c_float), ("load", C.c_float), ("spawn_temp", C.c_float),
-                ("latent", C.c_float), ("heat_capacity", C.c_float)]
+                ("latent", C.c_float), ("heat_capacity", C.c_float), ("conductivity", C.c_float),
+                ("emissivity", C.c_float), ("boil", C.c_float), ("latent_vap", C.c_float),
+                ("freeze_shift", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -74,6 +77,8 @@ class SimParams(C.Structure):
                 ("viscous_warm", C.c_uint32), ("viscosity_warm", C.c_uint32),
                 ("viscosity_precond", C.c_uint32), ("viscosity_mg_levels", C.c_uint32),
                 ("viscosity_mg_sweeps", C.c_uint32), ("mg_precision", C.c_uint32),
+                ("heat_rate", C.c_float), ("air_cooling", C.c_float), ("contact_heat", C.c_float),
+                ("eddy_heat", C.c_float), ("evaporation", C.c_uint32),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
@@ -117,7 +122,8 @@ class GMat(C.Structure):
                                           "temp_model", "residue")] + \
                [(n, C.c_float) for n in ("rho", "ln_nu", "sigma", "mix_rate", "cos_c", "sin_c",
                                           "grain", "rho_solid", "visc_exp", "expansion", "load",
-                                          "pad")]
+                                          "freeze_shift")] + \
+               [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4)]
 
 
 class GParams(C.Structure):
@@ -137,7 +143,9 @@ class GParams(C.Structure):
                 ("v_max_iters", C.c_int32), ("heat_x", C.c_float), ("ln_rigid", C.c_float),
                 ("v_twice", C.c_int32), ("v_p2warm", C.c_int32), ("v_warm", C.c_int32),
                 ("v_precond", C.c_int32), ("v_mg_levels", C.c_int32), ("v_mg_sweeps", C.c_int32),
-                ("mg_half", C.c_int32)]
+                ("mg_half", C.c_int32), ("heat_rate", C.c_float), ("film", C.c_float),
+                ("radiation", C.c_float), ("contact", C.c_float), ("eddy", C.c_float), ("evap", C.c_float),
+                ("chan_t", C.c_int32), ("pad_heat", C.c_int32)]
 
 
 class LevelsInfo(C.Structure):
@@ -191,7 +199,7 @@ SIGNATURES = {
     "sp_stats": ([_P, C.POINTER(StatsOut)], C.c_int),
     "sp_sort": ([_P], C.c_int),
     "sp_default_params": ([C.POINTER(SimParams)], None),
-    "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32,
+    "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32,
                                C.POINTER(GParams), _P], C.c_int),
     "sp_levels": ([C.c_int32, C.c_int32, C.POINTER(LevelsInfo)], C.c_int),
     "sp_debug_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
@@ -207,6 +215,7 @@ SIGNATURES = {
     "sp_batch_wait": ([_P], C.c_int),
     "sp_batch_groups": ([_P, C.POINTER(C.c_uint32), _P, C.c_uint32], C.c_int),
     "sp_batch_timings": ([_P, C.c_uint32, _P], C.c_int),
+    "sp_download_ledger": ([_P, _P], C.c_int),
     "sp_evict": ([_P], C.c_int),
     "sp_is_resident": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "sp_tank_bytes": ([C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
@@ -262,12 +271,12 @@ def default_params():
     return p
 
 
-def derive_grid_params(params, W, H, rows):
+def derive_grid_params(params, W, H, rows, thermal=False):
     """sp_derive_grid_params: the grid-unit block and material table for a tank of `rows`."""
     g = GParams()
     mats = (GMat * MAX_MATERIALS)()
     arr = (C.c_int32 * max(len(rows), 1))(*rows)
-    rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), C.byref(g), mats)
+    rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), int(thermal), C.byref(g), mats)
     if rc != OK:
         raise SproutError(rc, "derive_grid_params")
     return g, mats

@@ -47,6 +47,11 @@ struct Ctx {
     float4 *vma, *vmb;
     float2 *vmd, *vmr, *vmx, *vmz;
     double4* vpart;       // per-tile partial sums of the viscosity solve
+    // heat
+    float4 *therm, *dheat;
+    int2* hcell;          // per cell: heat capacity and heat sums of the particles in it
+    float* solid_t;       // temperature a solid is held at (SP_NO_TEMP: insulating)
+    uint32_t* ledger;     // SP_LEDGER_WORDS words: energy entries as 64-bit fixed point
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -115,6 +120,11 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.vtiles = (uint32_t*)take(sizeof(uint32_t) * nb0);
     c.part_sums = (double4*)take(sizeof(double4) * 2 * nb0);
     c.vpart = (double4*)take(sizeof(double4) * nb0);
+    c.therm = (float4*)take(sizeof(float4) * N);
+    c.dheat = (float4*)take(sizeof(float4) * N);
+    c.hcell = (int2*)take(sizeof(int2) * N);
+    c.solid_t = (float*)take(sizeof(float) * N);
+    c.ledger = (uint32_t*)take(sizeof(uint32_t) * SP_LEDGER_WORDS);
     if (bytes) *bytes = o;
     return c;
 }
@@ -152,6 +162,71 @@ SP_DEV void bspline3(float f, float w[3]) {
 }
 
 SP_DEV int floori(float x) { return (int)floorf(x); }
+
+// ---- heat (common.glsl:408-453, 485-530, 579-606; particles.comp:67-80) -----------------
+
+// ln of the viscosity factor of row r at T, deg C (temp_law, common.glsl:410-417).
+SP_DEV float temp_law(const Ctx& c, int r, float T) {
+    const sp_gmat& m = c.mats[r];
+    if (m.temp_model == 0) return 0.0f;
+    const float Tk = fmaxf_(T + 273.15f, 1.0f);
+    const float x = m.temp_model == 1 ? m.temp[0] * (1.0f / Tk - 1.0f / m.temp[2])
+                                      : m.temp[0] / fmaxf_(Tk - m.temp[1], 5.0f) - m.temp[0] / fmaxf_(m.temp[2] - m.temp[1], 5.0f);
+    return clampf_(x, -30.0f, 30.0f);
+}
+
+// Density factor of row r at concentration s and T (expansion, common.glsl:485-494).
+SP_DEV float expansion(const Ctx& c, int row, float s, float T) {
+    const sp_gmat& m = c.mats[row];
+    float b = m.expansion, tr = m.temp[2] - 273.15f, tb = m.th2[2];
+    if (m.diluent >= 0) {
+        const sp_gmat& d = c.mats[m.diluent];
+        b = mixf_(d.expansion, b, s);
+        tr = mixf_(d.temp[2] - 273.15f, tr, s);
+        tb = mixf_(d.th2[2], tb, s);
+    }
+    return clampf_(1.0f - b * (fminf_(T, tb) - tr), 0.7f, 1.3f);
+}
+
+// Solidus, liquidus, latent heat (K) and heat capacity of row at concentration s
+// (thermal_of, common.glsl:500-509).
+SP_DEV float4 thermal_of(const Ctx& c, int row, float s) {
+    const sp_gmat& m = c.mats[row];
+    const float ld = m.th2[0];
+    if (m.diluent < 0) return make_float4(m.th[2], m.th[3], ld, m.th[0]);
+    const sp_gmat& d = c.mats[m.diluent];
+    const float shift = m.freeze_shift * s;
+    const float C = mixf_(d.th[0], m.th[0], s);
+    return make_float4(d.th[2] + shift, d.th[3] + shift, mixf_(d.th[0] * d.th2[0], m.th[0] * ld, s) / C, C);
+}
+
+// Temperature and solid share of enthalpy temperature te (phase_of, common.glsl:515-521).
+SP_DEV float2 phase_of(float te, float4 m) {
+    if (te >= m.y) return make_float2(te, 0.0f);
+    const float lo = m.x - m.z;
+    if (te <= lo) return make_float2(te + m.z, 1.0f);
+    const float fr = (te - lo) / (m.y - lo);
+    return make_float2(m.x + fr * (m.y - m.x), 1.0f - fr);
+}
+
+// A particle's heat capacity and heat in fixed point (heat_quanta, particles.comp:70-73).
+SP_DEV int2 heat_quanta(float C, float T) {
+    const int ci = (int)floorf(C * SP_MASS_SCALE + 0.5f);
+    return make_int2(ci, (int)floorf((float)ci * (T + 273.15f) / SP_T_SCALE + 0.5f));
+}
+
+// The enthalpy temperature that keeps the heat content when the heat capacity changes
+// (keep_heat, particles.comp:78-80).
+SP_DEV float keep_heat(float t, float c0, float c1) { return c0 == c1 ? t : (t + 273.15f) * c0 / c1 - 273.15f; }
+
+// Boiling point of row at concentration s (particles.comp:368, 957).
+SP_DEV float boil_of(const Ctx& c, int row, float s) {
+    const sp_gmat& m = c.mats[row];
+    return m.diluent >= 0 ? mixf_(c.mats[m.diluent].th2[2], m.th2[2], s) : m.th2[2];
+}
+
+// Accumulator channel of phase group g's own temperature sum, -1 none (common.glsl:582-584).
+SP_DEV int group_heat_chan(const sp_gparams& P, int g) { return (int)((P.group_heat >> (8u * (uint32_t)g)) & 255u) - 1; }
 
 // Wave64 reductions.
 SP_DEV float wave_max(float v) {

@@ -71,6 +71,13 @@ __global__ __launch_bounds__(256) void k_clear_acc(const TankBatch tb) {
     for (size_t k = blockIdx.x * (size_t)256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256) p4[k] = make_uint4(0, 0, 0, 0);
     const size_t k = n4 * 4 + blockIdx.x * (size_t)256 + threadIdx.x;
     if (blockIdx.x == 0 && k < n) c.acc[k] = 0;
+    // and the particles' heat sums of each cell while the tank is thermal (mod.rs:2629-2631)
+    if (c.P->caps & SP_CAP_THERMAL) {
+        const size_t h2 = (size_t)c.N;  // int2 per cell: N / 2 uint4 and a possible tail
+        uint4* h4 = reinterpret_cast<uint4*>(c.hcell);
+        for (size_t q = blockIdx.x * (size_t)256 + threadIdx.x; q < h2 / 2; q += (size_t)gridDim.x * 256) h4[q] = make_uint4(0, 0, 0, 0);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && (h2 & 1)) c.hcell[h2 - 1] = make_int2(0, 0);
+    }
 }
 
 // Each tank's active viscosity tiles cleared before k_visc_setup marks them.
@@ -123,6 +130,8 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float rho = 0.0f, mu = 0.0f, rho_c = 0.0f, nu_c = 0.0f;
     int major = 0, top_row = -1;
     float top_share = 0.0f;
+    const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
+    float T = P.heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
     for (int q = 0; q < nmix; q++) c.mixb[(size_t)i * nmix + q] = -1.0f;
     if (cw > 0) {
         int others = 0;
@@ -131,6 +140,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
             if (ch >= 0) others += c.acc[k + ch];
         }
         float sh[SP_MAX_PRESENT], scn[SP_MAX_PRESENT];
+        float Csh = 0.0f, Cg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int j = 0; j < P.n_present; j++) {
             sh[j] = 0.0f;
             scn[j] = 1.0f;
@@ -141,6 +151,36 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
             if (w <= 0) continue;
             sh[j] = w >= cw ? 1.0f : clampf_((float)w / (float)cw, 0.0f, 1.0f);
             if (m.diluent >= 0 && m.chan_s >= 0) scn[j] = clampf_((float)c.acc[k + m.chan_s] / (float)w, 0.0f, 1.0f);
+            // with heat: the cell's heat capacity, conductivity and emissivity (grid_prep.comp:100-108)
+            if (th_on) {
+                const float cr = sh[j] * thermal_of(c, r, scn[j]).w;
+                Csh += cr;
+                Cg[m.group] += cr;
+                const float kp = m.th[1];
+                kappa += sh[j] * (m.diluent >= 0 ? mixf_(c.mats[m.diluent].th[1], kp, scn[j]) : kp);
+                const float em = m.th2[1];
+                emis += sh[j] * (m.diluent >= 0 ? mixf_(c.mats[m.diluent].th2[1], em, scn[j]) : em);
+            }
+        }
+        // the temperature of each phase group from its own sum; T leaves as the cell's mean
+        // (group_temperatures, common.glsl:598-622)
+        float Tg[4] = {T, T, T, T};
+        if (th_on && Csh > 0.0f) {
+            const int Htot = c.acc[k + P.chan_t];
+            T = (float)Htot * SP_T_SCALE / (Csh * (float)cw) - 273.15f;
+            for (int q = 0; q < 4; q++) Tg[q] = T;
+            int rest = Htot, first = -1;
+            for (int q = 0; q < 4; q++) {
+                const int ch = group_heat_chan(P, q);
+                if (ch >= 0) {
+                    const int h = c.acc[k + ch];
+                    rest -= h;
+                    if (Cg[q] > 0.0f) Tg[q] = (float)h * SP_T_SCALE / (Cg[q] * (float)cw) - 273.15f;
+                } else if (first < 0 && (P.group_mask & (1u << q)) != 0u) {
+                    first = q;
+                }
+            }
+            if (first >= 0 && Cg[first] > 0.0f) Tg[first] = (float)rest * SP_T_SCALE / (Cg[first] * (float)cw) - 273.15f;
         }
         for (int j = 0; j < P.n_present; j++) {
             float share = sh[j];
@@ -149,19 +189,33 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
             const sp_gmat& m = c.mats[r];
             float sc = scn[j];
             float rho_m = m.rho, lnu = m.ln_nu, sig = m.sigma;
+            const float Tm = Tg[m.group];
+            if (th_on) lnu += temp_law(c, r, Tm);
+            const float xp = th_on ? expansion(c, r, sc, Tm) : 1.0f;
             float csx = m.cos_c, csy = m.sin_c;
             if (m.diluent >= 0) {
                 const sp_gmat& d = c.mats[m.diluent];
                 rho_m = mixf_(d.rho, rho_m, sc);
                 float bs = m.visc_exp == 1.0f ? sc : powf(sc, m.visc_exp);
-                lnu = mixf_(d.ln_nu, lnu, bs);
+                lnu = mixf_(th_on ? d.ln_nu + temp_law(c, m.diluent, Tm) : d.ln_nu, lnu, bs);
                 sig = mixf_(d.sigma, sig, sc);
                 csx = mixf_(d.cos_c, csx, sc);
                 csy = mixf_(d.sin_c, csy, sc);
                 if (m.fam == 0) fam0 = sc;
                 else if (m.fam > 0) c.mixb[(size_t)i * nmix + m.fam - 1] = sc;
             }
-            rho_m *= 1.0f;
+            rho_m *= xp;
+            // frozen liquid: the frozen viscosity and the solid's density (grid_prep.comp:150-159)
+            if (th_on) {
+                const float4 th = thermal_of(c, r, sc);
+                const float fs = th.y > -1e8f ? clampf_((th.y - Tm) / fmaxf_(th.y - th.x, 1e-3f), 0.0f, 1.0f) : 0.0f;
+                if (fs > 0.0f) {
+                    lnu = mixf_(lnu, fmaxf_(lnu, P.ln_rigid), fs);
+                    const float rs = m.rho_solid;
+                    rho_m = mixf_(rho_m, m.diluent >= 0 ? mixf_(c.mats[m.diluent].rho_solid, rs, sc) : rs, fs);
+                    frozen += share * fs;
+                }
+            }
             if (share > top_share) {
                 top_share = share;
                 top_row = r;
@@ -187,6 +241,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
             rho_c = rg[major] / f[major];
             nu_c = expf(lg[major] / f[major]);
         }
+        Craw = Csh * (float)cw / SP_MASS_SCALE;
     }
     float phi_t = phi_f + sol.z;
     bool has = phi_f > 0.04f && sol.w > 0.02f;
@@ -208,6 +263,11 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
         c.grho[i] = make_float4(rg[0], rg[1], rg[2], rg[3]);
     }
     c.sprop[i] = make_float4(sigma, csx_, csy_, (float)major + (float)(top_row + 1) / 128.0f);
+    if (th_on) {
+        c.therm[i] = make_float4(T, Craw, kappa, frozen);
+        // the heat pass reads the emissivity here, then stores its result (grid_prep.comp:256-261)
+        c.dheat[i] = make_float4(P.heat_rate > 0.0f ? emis : 0.0f, 0.0f, 0.0f, 0.0f);
+    }
     if (level <= 0.0f) c.pres[i] = make_float2(0.0f, 0.0f);
 }
 

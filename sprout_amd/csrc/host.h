@@ -17,16 +17,23 @@ struct Tank {
     int families = 0;
     uint32_t group_mask = 0;
     int primary = -1;
+    bool thermal = false;                   // temperatures carried and conducted
+    int chan_t = -1;                        // channel of the temperature sum
+    std::array<int, 4> chan_g{-1, -1, -1, -1};  // per phase group after the first: its own sum
 };
 
 void default_params(sp_sim_params* p);
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present);
-bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t);
+bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal = false);
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
            std::string* why);
 void levels(int W, int H, sp_levels_info& L);
-int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows);
-float dt_limit(const sp_sim_params& p, const std::vector<int>& rows);
+int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
+float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
+// The range of temperatures a thermal tank of `rows` can be carried through without
+// freezing, boiling or a liquid turning rigid (the passes for those are not carried):
+// (lowest allowed, highest allowed), deg C.
+std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows);
 void settle_spawn(const sp_sim_params& p, sp_particle& q);
 
 }  // namespace sp

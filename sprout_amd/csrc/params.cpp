@@ -14,6 +14,9 @@ namespace sp {
 
 static const float WATER_DENSITY = 998.0f;  // params.rs:808
 static const float T_REF = 20.0f;           // material.rs:27
+static const float WATER_HEAT = WATER_DENSITY * 4182.0f;  // params.rs:1315
+static const float STEFAN_BOLTZMANN = 5.670374e-8f;       // params.rs:1317
+static const float NO_MELT = -1.0e9f;                      // mod.rs:76
 
 // MatProps::default (material.rs:344-420), the fields this build reads.
 static sp_material default_material() {
@@ -39,6 +42,11 @@ static sp_material default_material() {
     m.spawn_temp = T_REF;
     m.latent = 0.0f;
     m.heat_capacity = 4182.0f;
+    m.conductivity = 0.6f;  // material.rs:365-378
+    m.emissivity = 0.9f;
+    m.boil = 1.0e9f;
+    m.latent_vap = 0.0f;
+    m.freeze_shift = 0.0f;
     return m;
 }
 
@@ -84,6 +92,11 @@ void default_params(sp_sim_params* p) {
     p->viscosity_mg_levels = 1;
     p->viscosity_mg_sweeps = 2;
     p->mg_precision = 0;
+    p->heat_rate = 1.0f;  // params.rs:395-399
+    p->air_cooling = 10.0f;
+    p->contact_heat = 3000.0f;
+    p->eddy_heat = 0.17f;
+    p->evaporation = 1;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -96,6 +109,9 @@ void default_params(sp_sim_params* p) {
         m.melt_hi = 0.0f;
         m.latent = 334.0e3f;
         m.density_solid = 917.0f;
+        m.boil = 100.0f;
+        m.latent_vap = 2257.0e3f;
+        m.emissivity = 0.96f;
     }
     // presets.rs:61-89 honey
     {
@@ -108,7 +124,10 @@ void default_params(sp_sim_params* p) {
         m.tension = 0.076f;
         m.mix_rate = 1.5f;
         m.heat_capacity = 2300.0f;
+        m.conductivity = 0.35f;
         m.expansion = 5.0e-4f;
+        m.boil = 110.0f;
+        m.latent_vap = 2257.0e3f;
     }
     // presets.rs:92-111 olive oil
     {
@@ -120,6 +139,7 @@ void default_params(sp_sim_params* p) {
         m.temp_b = 3900.0f;
         m.tension = 0.032f;
         m.heat_capacity = 1970.0f;
+        m.conductivity = 0.17f;
         m.expansion = 7.0e-4f;
         m.melt_lo = -8.0f;
         m.melt_hi = -4.0f;
@@ -136,6 +156,10 @@ void default_params(sp_sim_params* p) {
         m.tension = 0.485f;
         m.contact_angle = 140.0f;
         m.heat_capacity = 139.5f;
+        m.conductivity = 8.3f;
+        m.emissivity = 0.1f;
+        m.boil = 356.7f;
+        m.latent_vap = 295.0e3f;
         m.expansion = 1.8e-4f;
         m.melt_lo = -39.0f;
         m.melt_hi = -38.8f;
@@ -180,15 +204,56 @@ static float dt_capillary(const sp_sim_params& p, const std::vector<int>& rows) 
     return 0.5f * sqrtf(1.0f / (2.0f * 3.14159265f * worst));
 }
 
-float dt_limit(const sp_sim_params& p, const std::vector<int>& rows) { return std::min(p.dt_max, dt_capillary(p, rows)); }
+// Volumetric heat capacity a particle of the row carries, J/(m^3 K) (params.rs:888-893).
+static float heat_vol(const sp_material& m) { return m.density * m.heat_capacity; }
 
-// FluidSim::viscosity_mode (mod.rs:2768-2792), isothermal.
-int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows) {
-    float dt = dt_limit(p, rows);
+// SimParams::dt_thermal (params.rs:914-923): a fifth of the fastest cell diffusion time.
+static float dt_thermal(const sp_sim_params& p, const std::vector<int>& rows) {
+    if (p.heat_rate <= 0.0f) return INFINITY;
+    float dx2 = p.cell_size * p.cell_size, dt = INFINITY;
+    for (int i : rows) dt = std::min(dt, 0.2f * dx2 * heat_vol(p.mat[i]) / std::max(p.mat[i].conductivity, 1e-6f) / p.heat_rate);
+    return dt;
+}
+
+// FluidSim::dt_limit (mod.rs:2331-2338), without the elastic limit (nothing elastic here).
+float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal) {
+    float dt = std::min(p.dt_max, dt_capillary(p, rows));
+    if (thermal) dt = std::min(dt, dt_thermal(p, rows));
+    return dt;
+}
+
+std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows) {
+    float lo = -273.15f, hi = INFINITY;
+    for (int r : rows) {
+        const sp_material& m = p.mat[r];
+        if (freezes(m)) lo = std::max(lo, m.melt_hi);
+        hi = std::min(hi, m.boil);
+        // the temperature where the liquid reaches rigid_viscosity (the law falls with T)
+        if (m.temp_model != 0 && viscosity_at(m, lo) >= p.rigid_viscosity) {
+            float a = lo, b = std::min(hi, 1000.0f);
+            for (int k = 0; k < 60; k++) {
+                float mid = 0.5f * (a + b);
+                if (viscosity_at(m, mid) >= p.rigid_viscosity) a = mid;
+                else b = mid;
+            }
+            lo = std::max(lo, b);
+        }
+    }
+    return {lo, hi};
+}
+
+// FluidSim::viscosity_mode (mod.rs:2768-2792). A thermal tank counts a liquid that can
+// freeze or has a temperature law as arbitrarily viscous.
+int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal) {
+    float dt = dt_limit(p, rows, thermal);
     float d = 0.0f;
     for (int r : rows) {
+        const sp_material& m = p.mat[r];
         float cells = 1.0f / (p.cell_size * p.cell_size);
-        float nu = viscosity_at(p.mat[r], p.ambient_temp) * cells;
+        float nu;
+        if (!thermal) nu = viscosity_at(m, p.ambient_temp) * cells;
+        else if (freezes(m) || m.temp_model != 0) nu = INFINITY;
+        else nu = m.viscosity * cells;
         d = std::max(d, nu * dt);
     }
     if (d <= 0.0f) return 0;
@@ -209,7 +274,7 @@ std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present
 }
 
 // Tank::layout (mod.rs:271-331), isothermal and without elastic channels.
-bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t) {
+bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal) {
     if ((int)rows.size() > SP_MAX_PRESENT) return false;
     t = Tank();
     t.rows = rows;
@@ -234,6 +299,17 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t) 
         }
         t.chan.push_back({c[0], c[1], c[2]});
     }
+    // the temperature sum, and one per phase group after the first (mod.rs:303-318)
+    t.thermal = thermal;
+    if (thermal) {
+        t.chan_t = next++;
+        bool first = true;
+        for (int g = 0; g < 4; g++)
+            if (group_mask & (1u << g)) {
+                if (!first) t.chan_g[g] = next++;
+                first = false;
+            }
+    }
     if (next > SP_ACC_MAX) return false;
     int stride = std::max(next, SP_ACC_MIN);
     stride = (stride + 3) / 4 * 4;
@@ -244,17 +320,21 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t) 
     return true;
 }
 
-// FluidSim::caps (mod.rs:2717-2766) for an isothermal tank; returns false for a tank that
-// needs a CAP-gated pass this build does not carry.
+// FluidSim::caps (mod.rs:2717-2766); returns false for a tank that needs a CAP-gated pass
+// this build does not carry. A thermal tank is carried without CAP_RIGID: the reference
+// sets it for every liquid that could freeze or has a temperature law, but its passes only
+// act on frozen or rigid liquid, which the temperature range a thermal tank is held to
+// (thermal_range) excludes.
 static bool tank_caps(const sp_sim_params& p, const Tank& t, uint32_t& caps) {
     caps = 0;
-    if (t.families > 0) caps |= 1u;  // CAP_FAMILIES
+    if (t.families > 0) caps |= SP_CAP_FAMILIES;
+    if (t.thermal) caps |= SP_CAP_THERMAL;
     bool rigid = false;
     for (int r : t.rows) {
         const sp_material& m = p.mat[r];
-        rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (freezes(m) && p.ambient_temp < m.melt_hi);
+        rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (!t.thermal && freezes(m) && p.ambient_temp < m.melt_hi);
     }
-    if (rigid) caps |= 4u;  // CAP_RIGID
+    if (rigid) caps |= SP_CAP_RIGID;
     return !rigid;
 }
 
@@ -277,13 +357,17 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         float solid = m.density_solid > 0.0f ? m.density_solid : m.density;
         float ln_nu = logf(std::max(m.viscosity / (dx * dx), 1e-8f));
         float rho = m.density;
-        ln_nu += temp_factor_ln(m, p.ambient_temp);
-        rho *= std::min(std::max(1.0f - m.expansion * (p.ambient_temp - m.temp_ref), 0.7f), 1.3f);
-        if (freezes(m) && m.diluent < 0 && p.ambient_temp < m.melt_hi) {
-            float fs = std::min(std::max((m.melt_hi - p.ambient_temp) / std::max(m.melt_hi - m.melt_lo, 1e-3f), 0.0f), 1.0f);
-            float frozen = std::max(logf(p.rigid_viscosity / (dx * dx)), ln_nu);
-            ln_nu += (frozen - ln_nu) * fs;
-            rho += (solid - rho) * fs;
+        // An isothermal tank: the temperature law and the frozen state are constants,
+        // folded in here (mod.rs:2194-2206); a thermal tank evaluates them per cell.
+        if (!t.thermal) {
+            ln_nu += temp_factor_ln(m, p.ambient_temp);
+            rho *= std::min(std::max(1.0f - m.expansion * (p.ambient_temp - m.temp_ref), 0.7f), 1.3f);
+            if (freezes(m) && m.diluent < 0 && p.ambient_temp < m.melt_hi) {
+                float fs = std::min(std::max((m.melt_hi - p.ambient_temp) / std::max(m.melt_hi - m.melt_lo, 1e-3f), 0.0f), 1.0f);
+                float frozen = std::max(logf(p.rigid_viscosity / (dx * dx)), ln_nu);
+                ln_nu += (frozen - ln_nu) * fs;
+                rho += (solid - rho) * fs;
+            }
         }
         sp_gmat& o = mats[r];
         o.group = std::min(m.group, 3);
@@ -305,6 +389,20 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.visc_exp = m.visc_exponent;
         o.expansion = m.expansion;
         o.load = m.load;
+        o.freeze_shift = m.freeze_shift;
+        // thermal rows (mod.rs:2213-2214)
+        o.th[0] = heat_vol(m) / WATER_HEAT;
+        o.th[1] = m.conductivity / (WATER_HEAT * dx * dx);
+        o.th[2] = std::max(m.melt_lo, NO_MELT);
+        o.th[3] = std::max(std::max(m.melt_hi, m.melt_lo + 0.25f), NO_MELT);
+        o.th2[0] = (freezes(m) || m.diluent >= 0) ? m.latent / m.heat_capacity : 0.0f;
+        o.th2[1] = m.emissivity;
+        o.th2[2] = m.boil;
+        o.th2[3] = m.latent_vap / m.heat_capacity;
+        o.temp[0] = m.temp_b;
+        o.temp[1] = m.temp_c;
+        o.temp[2] = m.temp_ref + 273.15f;
+        o.temp[3] = 0.0f;
     }
     uint32_t caps = 0;
     if (!tank_caps(p, t, caps)) {
@@ -333,7 +431,8 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.caps = caps;
     g.primary = t.rows.empty() ? -1 : t.primary;
     g.group_heat = 0;
-    g.visc_mode = viscosity_mode(p, t.rows);
+    for (int q = 0; q < 4; q++) g.group_heat += ((uint32_t)(t.chan_g[q] + 1) & 255u) << (8 * q);
+    g.visc_mode = viscosity_mode(p, t.rows, t.thermal);
     for (int k = 0; k < SP_MAX_PRESENT; k++) {
         g.present[k] = 0;
         g.fams[k][0] = -1;
@@ -350,7 +449,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.cfl = p.cfl;
     g.tool_crowd_max = p.tool_crowd_max;
     g.density_k = p.correction_max;
-    g.dt_max = dt_limit(p, t.rows);
+    g.dt_max = dt_limit(p, t.rows, t.thermal);
     g.crowd_jitter = p.crowd_jitter;
     g.ext_speed = 0.0f;
     g.wall_margin = 0.1f;
@@ -388,6 +487,14 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     }
     g.v_mg_sweeps = (int)std::min(std::max(p.viscosity_mg_sweeps, 1u), 2u);
     g.mg_half = p.mg_precision == 1u ? 1 : 0;
+    // heat (mod.rs:2299-2300, 2321)
+    g.heat_rate = p.heat_rate;
+    g.film = p.air_cooling / (WATER_HEAT * dx);
+    g.radiation = STEFAN_BOLTZMANN / (WATER_HEAT * dx);
+    g.contact = p.contact_heat / (WATER_HEAT * dx);
+    g.eddy = std::max(p.eddy_heat, 0.0f);
+    g.evap = p.evaporation != 0 ? 2.257e6f / 1005.0f : 0.0f;
+    g.chan_t = t.chan_t;
     return SP_OK;
 }
 

@@ -18,8 +18,8 @@ constexpr int LDS_INTS = 12288;   // scatter box budget: 48 KiB
 
 struct ScatterIn {
     bool live;
-    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc;
-    int cw, cs;
+    float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc, CT;
+    int cw, cs, ct, cg;
 };
 
 SP_DEV int cidx(const Ctx& c, int x, int y) { return y * c.W + x; }
@@ -166,6 +166,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     int row = (int)(p.kind & 255u) - 1;
     int Mgroup = c.mats[row].group, Mcw = c.mats[row].chan_w, Mcs = c.mats[row].chan_s;
     int fam = c.mats[row].fam;
+    const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C0 = p.c[0], C1 = p.c[1], C2 = p.c[2], C3 = p.c[3];
@@ -224,6 +225,25 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                         }
                     }
                 }
+            // Heat (particles.comp:353-373): the change of enthalpy temperature the heat
+            // pass worked out for the cells this particle scattered to, with the same
+            // weights, and the exchange among the particles of its own cell, against its
+            // own temperature as it entered the cell's sums.
+            if (th_on && (p.kind & SP_KIND_FRESH) == 0u) {
+                float dT = 0.0f;
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++) dT += wx[a] * wy[b] * c.dheat[cidx(c, bx + a, by + b)].x;
+                const int ox = clampi_(floori(p.pos[0]), 0, c.W - 1), oy = clampi_(floori(p.pos[1]), 0, c.H - 1);
+                const float4 own = c.dheat[cidx(c, ox, oy)];
+                if (own.y > 0.0f) {
+                    const float4 th = thermal_of(c, row, p.s);
+                    const int2 q = heat_quanta(th.w, fminf_(phase_of(p.t, th).x, boil_of(c, row, p.s)));
+                    dT += own.y * (own.z - (float)q.y / (float)q.x) * SP_T_SCALE;
+                }
+                p.t += dT;
+            }
             if (sw > 1e-4f) {
                 sgx /= sw;
                 sgy /= sw;
@@ -313,6 +333,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                         float s0 = diluent0 ? 0.0f : p.s;
                         int rr = diluent0 ? P.primary : row;
                         float s1 = clampf_(s0 + (sfs / sfw - s0) * (1.0f - expf(-c.mats[rr].mix_rate * dt)), 0.0f, 1.0f);
+                        const float c0 = th_on ? thermal_of(c, row, p.s).w : 0.0f;
                         if (!diluent0) {
                             p.s = s1;
                         } else if (s1 > 0.0f) {
@@ -323,6 +344,8 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                             p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
                             p.s = s1;
                         }
+                        // the same heat content at the new heat capacity (particles.comp:541)
+                        if (th_on) p.t = keep_heat(p.t, c0, thermal_of(c, row, p.s).w);
                     }
                 }
                 if (fam < 0 && (P.caps & 1u) && (int)(p.kind & 255u) - 1 == row && c.mats[row].fam < 0) {
@@ -431,10 +454,35 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     g.Csy = p.c[1] * dt;
     g.Csz = p.c[2] * dt;
     g.Csw = p.c[3] * dt;
-    g.rho = row_density(c, row, p.s);
+    float rho_p = row_density(c, row, p.s);
+    // Thermal state for the scatter (particles.comp:951-969): the temperature capped at the
+    // boiling point, the expansion and the solid's density where frozen, and the
+    // particle's quanta into its own cell's sums for the exchange inside the cell.
+    float CT = 0.0f;
+    if (th_on) {
+        const sp_gmat& Mr = c.mats[row];
+        const float4 th = thermal_of(c, row, p.s);
+        const float2 ph = phase_of(p.t, th);
+        const float tb = boil_of(c, row, p.s);
+        rho_p *= expansion(c, row, p.s, ph.x);
+        if (ph.y > 0.0f) {
+            const float rs = Mr.diluent >= 0 ? mixf_(c.mats[Mr.diluent].rho_solid, Mr.rho_solid, p.s) : Mr.rho_solid;
+            rho_p = mixf_(rho_p, rs, ph.y);
+        }
+        CT = th.w * (fminf_(ph.x, tb) + 273.15f) / SP_T_SCALE;
+        const int hx = clampi_(floori(p.pos[0]), 0, c.W - 1), hy = clampi_(floori(p.pos[1]), 0, c.H - 1);
+        const int2 q = heat_quanta(th.w, fminf_(ph.x, tb));
+        int2* hc = &c.hcell[cidx(c, hx, hy)];
+        atomicAdd(&hc->x, q.x);
+        atomicAdd(&hc->y, q.y);
+    }
+    g.rho = rho_p;
     g.sc = p.s;
     g.cw = Mcw;
     g.cs = Mcs;
+    g.CT = CT;
+    g.ct = th_on ? P.chan_t : -1;
+    g.cg = th_on ? group_heat_chan(P, c.mats[row].group) : -1;
 }
 
 // The scatter of one particle (particles.comp:997-1062) through `add(node x, node y,
@@ -488,6 +536,12 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
                 add(bx + a, by + b, 4, wi);
                 if (g.cw >= 0) add(bx + a, by + b, g.cw, wi);
                 if (g.cs >= 0 && g.sc > 0.0f) add(bx + a, by + b, g.cs, (int)floorf(wx[a] * wy[b] * g.sc * SP_MASS_SCALE + 0.5f));
+                // heat: C (T + 273.15) / T_SCALE into the sum and the group's own (particles.comp:1047-1051)
+                if (g.ct >= 0) {
+                    const int h = (int)floorf(wx[a] * wy[b] * g.CT * SP_MASS_SCALE + 0.5f);
+                    add(bx + a, by + b, g.ct, h);
+                    if (g.cg >= 0) add(bx + a, by + b, g.cg, h);
+                }
             }
     }
 }

@@ -235,6 +235,8 @@ void launch_grid_passes(const Run& r, const Recs& recs) {
     hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, r.stream, r.tb);
+    // heat conduction while the tank is thermal (record_substep, mod.rs:2663-2666)
+    if ((r.gp.caps & SP_CAP_THERMAL) && r.gp.heat_rate > 0.0f) hipLaunchKernelGGL(k_heat, g0, TILE, 0, r.stream, r.tb);
     launch_visc_before(r, recs);
     hipLaunchKernelGGL(k_rhs, g0, TILE, 0, r.stream, r.tb);
 }
@@ -257,7 +259,8 @@ void launch_particles(const Run& r, int gather) {
 // kernels read themselves stay out of the key).
 uint32_t schedule_key(const sp_gparams& g) {
     return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
-           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12;
+           (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12 |
+           (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13;
 }
 
 }  // namespace

@@ -22,6 +22,7 @@
 #include "particles.hip"
 #include "solve.hip"
 #include "visc.hip"
+#include "heat.hip"
 #include "schedule.hip"
 
 namespace {
@@ -100,6 +101,12 @@ struct sp_sim {
     Timing tm;
     bool in_batch = false;  // a batch drives the tank (sp_batch_*)
     bool served = false;    // a server drives the tank (sp_server_*)
+
+    // heat (FluidSim::thermal, solids_heated, mod.rs:959-962)
+    bool thermal = false;        // temperatures carried and conducted since the last reset
+    bool solids_heated = false;  // the solids hold temperatures
+    float t_lo = INFINITY, t_hi = -INFINITY;    // the temperatures the particles entering since the last reset held
+    float st_lo = INFINITY, st_hi = -INFINITY;  // the temperatures the solids hold (set_solids)
 };
 
 namespace {
@@ -136,6 +143,21 @@ void launch_sort(sp_sim* s, int tiled, hipStream_t stream) {
 // Uploads the grid-unit tables if the parameters or the tank changed.
 int upload_tables(sp_sim* s) {
     if (!s->tables_dirty) return SP_OK;
+    if (s->thermal) {
+        // A thermal tank is carried between its liquids' freezing and boiling points (and
+        // below where a temperature law would make one rigid): the rigid-body, steam and
+        // boiling passes that act outside that range are not carried. Conduction, air and
+        // contact exchange never leave the range of the temperatures put in.
+        const std::array<float, 2> r = sp::thermal_range(s->params, s->tank.rows);
+        const float lo = std::min(std::min(s->t_lo, s->st_lo), s->params.ambient_temp);
+        const float hi = std::max(std::max(s->t_hi, s->st_hi), s->params.ambient_temp);
+        if (!(lo > r[0] && hi < r[1])) {
+            set_err(s, "thermal tank holds temperatures %.2f .. %.2f C outside %.2f .. %.2f C, where its liquids would "
+                       "freeze, boil or turn rigid (rigid bodies, steam and boiling are not carried)",
+                    lo, hi, r[0], r[1]);
+            return SP_E_UNSUPPORTED;
+        }
+    }
     std::string why;
     int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why);
     if (rc != SP_OK) {
@@ -382,22 +404,38 @@ int guarded(sp_sim* s, F&& f, bool device = true) {
     }
 }
 
-// The rows the particles carry, with their diluents (note_materials, mod.rs:2051-2081).
+// The rows the particles carry, with their diluents, and whether the tank becomes thermal
+// (note_materials, mod.rs:2051-2081): a particle more than 0.05 K off the ambient
+// temperature makes it so.
 bool note_materials(sp_sim* s, const sp_particle* p, uint32_t n, bool reset) {
     std::vector<int> present = reset ? std::vector<int>() : s->tank.rows;
+    bool thermal = s->thermal;
     for (uint32_t k = 0; k < n; k++) {
         uint32_t low = p[k].kind & 255u;
         if (p[k].kind == SP_KIND_DEAD || low == 0u) continue;
         int r = (int)low - 1;
+        thermal |= fabsf(p[k].t - s->params.ambient_temp) > 0.05f;
         if (std::find(present.begin(), present.end(), r) == present.end()) present.push_back(r);
     }
     std::vector<int> rows = sp::closure(s->params, present);
-    if (rows == s->tank.rows) return true;
+    if (rows == s->tank.rows && thermal == s->thermal && !(reset && s->tank.rows.empty())) return true;
     sp::Tank t;
-    if (!sp::tank_layout(s->params, rows, t)) return false;
+    if (!sp::tank_layout(s->params, rows, t, thermal)) return false;
     s->tank = t;
+    s->thermal = thermal;
     s->tables_dirty = true;
     return true;
+}
+
+// The temperatures entering the tank (particles are enthalpy temperatures; above the
+// melting range, which a thermal tank is held to, they are the temperatures themselves).
+void note_temps(sp_sim* s, const sp_particle* p, uint32_t n) {
+    for (uint32_t k = 0; k < n; k++) {
+        if (p[k].kind == SP_KIND_DEAD || (p[k].kind & 255u) == 0u) continue;
+        if (p[k].t < s->t_lo || p[k].t > s->t_hi) s->tables_dirty = true;  // checked again at the next record
+        s->t_lo = std::min(s->t_lo, p[k].t);
+        s->t_hi = std::max(s->t_hi, p[k].t);
+    }
 }
 
 }  // namespace
@@ -421,12 +459,12 @@ int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out) {
 }
 
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h, const int32_t* rows, int32_t n_rows,
-                          sp_gparams* out, sp_gmat* mats) {
+                          int32_t thermal, sp_gparams* out, sp_gmat* mats) {
     if (!params || !out || !mats || (n_rows > 0 && !rows) || n_rows < 0) return SP_E_INVALID;
     std::vector<int> present(rows, rows + n_rows);
     std::vector<int> r = sp::closure(*params, present);
     sp::Tank t;
-    if (!sp::tank_layout(*params, r, t)) return SP_E_CAPACITY;
+    if (!sp::tank_layout(*params, r, t, thermal != 0)) return SP_E_CAPACITY;
     return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr);
 }
 
@@ -558,14 +596,35 @@ int sp_set_params(sp_sim* s, const sp_sim_params* p) {
 int sp_set_solids(sp_sim* s, const float* sdf, const float* solid, const uint32_t* region, const float* temp) {
     return guarded(s, [&]() -> int {
         if (!sdf || !solid || !region) return SP_E_INVALID;
-        if (temp) {
-            set_err(s, "solids held at a temperature make the tank thermal, which this build does not carry");
-            return SP_E_UNSUPPORTED;
-        }
         HIPCHK(hipMemcpyAsync((void*)s->c.sdf, sdf, sizeof(float) * (size_t)(s->W + 1) * (s->H + 1), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync((void*)s->c.solid, solid, sizeof(float4) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync((void*)s->c.region, region, sizeof(uint32_t) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
+        // solid temperatures (mod.rs:2442-2448): held ones make the tank thermal for good
+        std::vector<float> st((size_t)s->N, SP_NO_TEMP);
+        if (temp) st.assign(temp, temp + s->N);
+        HIPCHK(hipMemcpyAsync((void*)s->c.solid_t, st.data(), sizeof(float) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
+        s->solids_heated = temp != nullptr;
+        s->st_lo = INFINITY;
+        s->st_hi = -INFINITY;
+        for (float t : st)
+            if (t > 0.5f * SP_NO_TEMP) {
+                s->st_lo = std::min(s->st_lo, t);
+                s->st_hi = std::max(s->st_hi, t);
+            }
+        s->tables_dirty = true;  // the range guard sees the new solid temperatures at the next record
+        if (temp) {
+            if (!s->thermal) {  // enable_heat (mod.rs:2163-2177)
+                sp::Tank t;
+                if (!sp::tank_layout(s->params, s->tank.rows, t, true)) {
+                    set_err(s, "the accumulators have no room for the temperature sums");
+                    return SP_E_CAPACITY;
+                }
+                s->tank = t;
+                s->thermal = true;
+                s->tables_dirty = true;
+            }
+        }
         return SP_OK;
     });
 }
@@ -584,6 +643,12 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
             if (q.kind == SP_KIND_DEAD || low == 0u || (int)low - 1 >= s->params.mat_count) continue;
             ps.push_back(q);
         }
+        // a reset starts from an empty tank, thermal only if the solids are heated (mod.rs:1991-1992)
+        s->thermal = s->solids_heated;
+        s->tank = sp::Tank();
+        s->t_lo = INFINITY;
+        s->t_hi = -INFINITY;
+        note_temps(s, ps.data(), (uint32_t)ps.size());
         if (!note_materials(s, ps.data(), (uint32_t)ps.size(), true)) {
             set_err(s, "the particles hold more materials than the accumulators have channels for");
             return SP_E_CAPACITY;
@@ -597,6 +662,8 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         HIPCHK(hipMemsetAsync(s->c.p2keep, 0, sizeof(float) * (size_t)s->N, s->stream));
         HIPCHK(hipMemsetAsync(s->c.vd, 0, sizeof(float2) * (size_t)s->N, s->stream));
         HIPCHK(hipMemsetAsync(s->c.S, 0, sizeof(sp_stepbuf), s->stream));
+        HIPCHK(hipMemsetAsync(s->c.dheat, 0, sizeof(float4) * (size_t)s->N, s->stream));
+        HIPCHK(hipMemsetAsync(s->c.ledger, 0, sizeof(uint32_t) * SP_LEDGER_WORDS, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->pending_spawn.clear();
         s->time_base = s->sim_time;
@@ -631,6 +698,7 @@ int sp_spawn(sp_sim* s, const sp_particle* particles, uint32_t n, uint32_t* acce
             sp::settle_spawn(s->params, ps[k]);
         }
         if (!note_materials(s, ps.data(), n, false)) return SP_OK;
+        note_temps(s, ps.data(), n);
         for (auto& q : ps) {
             q.id = s->next_id++;
             q.kind |= SP_KIND_FRESH;
@@ -757,6 +825,19 @@ int sp_download_pressure(sp_sim* s, float* pres, float* dt) {
         HIPCHK(hipStreamSynchronize(s->stream));
         for (int i = 0; i < s->N; i++) pres[i] = p[i].x;
         if (dt) *dt = s->stats.dt;
+        return SP_OK;
+    });
+}
+
+int sp_download_ledger(sp_sim* s, double* energy8) {
+    return guarded(s, [&]() -> int {
+        if (!energy8) return SP_E_INVALID;
+        uint32_t w[SP_LEDGER_WORDS];
+        HIPCHK(hipMemcpyAsync(w, s->c.ledger, sizeof(w), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        // download_ledger (mod.rs:3396-3399): LEDGER_SCALE per K of one water-filled cell
+        const double unit = 998.0 * 4182.0 * (double)s->params.cell_size * (double)s->params.cell_size;
+        for (int e = 0; e < 8; e++) energy8[e] = (double)((uint64_t)w[2 * e] | (uint64_t)w[2 * e + 1] << 32) / 65536.0 * unit;
         return SP_OK;
     });
 }
@@ -910,6 +991,11 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_VMX: *bytes = 8 * T; return c.vmx;
     case SP_FIELD_VMZ: *bytes = 8 * T; return c.vmz;
     case SP_FIELD_P2KEEP: *bytes = 4 * N; return c.p2keep;
+    case SP_FIELD_THERM: *bytes = 16 * N; return c.therm;
+    case SP_FIELD_DHEAT: *bytes = 16 * N; return c.dheat;
+    case SP_FIELD_HCELL: *bytes = 8 * N; return c.hcell;
+    case SP_FIELD_SOLID_T: *bytes = 4 * N; return c.solid_t;
+    case SP_FIELD_LEDGER: *bytes = 4 * SP_LEDGER_WORDS; return c.ledger;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1007,6 +1093,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
             break;
         }
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_HEAT: hipLaunchKernelGGL(k_heat, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_PRESSURE: launch_pressure(r, full.p); break;
         case SP_STAGE_PROJECT: launch_project(r); break;
         case SP_STAGE_SUBSTEP:

@@ -76,8 +76,14 @@ class FluidSim:
         solid = np.ascontiguousarray(field.solid, np.float32)
         region = np.ascontiguousarray(field.region, np.uint32)
         assert sdf.size == (W + 1) * (H + 1) and solid.size == 4 * W * H and region.size == W * H
-        self._check(self._lib.sp_set_solids(self._h, sdf.ctypes.data, solid.ctypes.data, region.ctypes.data, None),
-                    "set_solids")
+        temp = getattr(field, "temp", None)
+        if temp is not None and len(temp):
+            temp = np.ascontiguousarray(temp, np.float32)
+            assert temp.size == W * H
+        else:
+            temp = None
+        self._check(self._lib.sp_set_solids(self._h, sdf.ctypes.data, solid.ctypes.data, region.ctypes.data,
+                                            None if temp is None else temp.ctypes.data), "set_solids")
 
     def reset(self, particles):
         """FluidSim::reset(Vec<Particle>) (mod.rs:1982-2001)."""

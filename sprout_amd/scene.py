@@ -51,13 +51,17 @@ class Box:
         return self.c, float(np.hypot(self.half[0], self.half[1]))
 
 
+NO_TEMP = -1.0e9    # scene.rs: NO_TEMP, an insulating solid
+
+
 class SolidField:
     """SolidField (scene.rs:110-122)."""
 
-    def __init__(self, sdf, solid, region):
+    def __init__(self, sdf, solid, region, temp=None):
         self.sdf = sdf        # float32[(H+1)*(W+1)]
         self.solid = solid    # float32[H*W, 4]
         self.region = region  # uint32[H*W]
+        self.temp = temp      # float32[H*W] deg C, NO_TEMP where insulating; None: no solid has one
 
 
 class Scene:
@@ -65,6 +69,12 @@ class Scene:
         self.grid = (int(grid[0]), int(grid[1]))
         self.wall = F32(wall)
         self.shapes = []
+        self.temps = []  # per shape: the temperature it is held at (deg C), or None
+
+    def add(self, shape, temp=None):
+        self.shapes.append(shape)
+        self.temps.append(temp)
+        return self
 
     def wall_sdf(self, px, py):
         W, H = F32(self.grid[0]), F32(self.grid[1])
@@ -122,7 +132,26 @@ class Scene:
                 occ = (occ + (K[a] * K[b]) * pad[b:b + h, a:a + w]).astype(F32)
         solid = np.stack([au, av, occ, open_], axis=-1).reshape(h * w, 4).astype(F32)
         region = _regions(solid, w, h)
-        return SolidField(np.ascontiguousarray(sdf.reshape(-1), F32), np.ascontiguousarray(solid), region)
+        # cells in or against a shape held at a temperature (scene.rs:295-313)
+        temp = None
+        temps = list(self.temps) + [None] * (len(self.shapes) - len(self.temps))
+        if any(t is not None for t in temps):
+            temp = np.full(w * h, NO_TEMP, F32)
+            for s, t in zip(self.shapes, temps):
+                if t is None:
+                    continue
+                c, r = s.bound()
+                r = r + 2.0
+                i0 = max(int(math.floor(c[0] - r)), 0)
+                i1 = min(max(int(math.ceil(c[0] + r)), 0), w)
+                j0 = max(int(math.floor(c[1] - r)), 0)
+                j1 = min(max(int(math.ceil(c[1] + r)), 0), h)
+                for j in range(j0, j1):
+                    xs = np.arange(i0, i1, dtype=F32) + F32(0.5)
+                    d = s.sdf(xs, np.full(len(xs), F32(j) + F32(0.5), F32))
+                    row = temp[j * w + i0:j * w + i1]
+                    row[d < F32(0.75)] = F32(t)
+        return SolidField(np.ascontiguousarray(sdf.reshape(-1), F32), np.ascontiguousarray(solid), region, temp)
 
 
 def _open_fraction(d0, d1):

@@ -78,3 +78,62 @@ def four_liquids(grid, seed=0, fill=(0.8, 0.8), cell_size=0.005, pressure_tol=1e
         parts.append(q)
     parts = settle(np.concatenate(parts), params)
     return params, scene.rasterize(), parts, sorted(FOUR_ROWS)
+
+
+def floor_strip(scene, x0, x1):
+    """A box filling the tank floor between x0 and x1 (cells), flush with its top."""
+    w = float(scene.wall)
+    return sc.Box(((x0 + x1) / 2.0, w / 2.0), ((x1 - x0) / 2.0, w / 2.0))
+
+
+def thermal_pool(grid, frac=(1.0, 0.6), temp=None, row=N.ROW_WATER, hot=None, cell_size=0.005, heat_rate=1.0,
+                 air_cooling=10.0, evaporation=1, emissivity=None, expansion=None, eddy_heat=0.17, pressure_tol=1e-6):
+    """(params, solid field, settled particles, rows) of a pool of one liquid, x < frac[0] W,
+    y < frac[1] H, in the tank frame, whose particles hold temperatures: `temp` is a
+    function of the particle positions (x, y arrays, cells) giving deg C, or None for the
+    ambient 20 deg C. `hot` = (x0, x1, deg C): a strip of the floor held at a temperature."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.heat_rate = heat_rate
+    params.air_cooling = air_cooling
+    params.evaporation = evaporation
+    params.eddy_heat = eddy_heat
+    if emissivity is not None:
+        params.mat[row].emissivity = emissivity
+    if expansion is not None:
+        params.mat[row].expansion = expansion
+    W, H = grid
+    scene = sc.Scene(grid)
+    if hot is not None:
+        x0, x1, t = hot
+        scene.add(floor_strip(scene, x0, x1), temp=t)
+    parts = sc.fill_rect(scene, (0.0, 0.0, frac[0] * W, frac[1] * H), row)
+    field = scene.rasterize()
+    parts = settle(parts, params)
+    if temp is not None:
+        parts["t"] = np.asarray(temp(parts["pos"][:, 0], parts["pos"][:, 1]), np.float32)
+    return params, field, parts, [row]
+
+
+def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), strip_temp=76.85):
+    """The two-fluid tank with heat: water below (density 1000, 1e-6 m^2/s), oil above
+    (900, 1e-4), a honey-like blob (1400, 0.09) near the top centre, everything at 20 deg C
+    (293 K), on a floor strip held at 350 K. (params, solid field, particles, rows)."""
+    W, H = grid
+    p = dam_params(cell_size=cell_size)
+    p.mat[N.ROW_WATER].density = 1000.0
+    p.mat[N.ROW_WATER].viscosity = 1e-6
+    p.mat[N.ROW_OIL].density = 900.0
+    p.mat[N.ROW_OIL].viscosity = 1e-4
+    p.mat[N.ROW_HONEY].density = 1400.0
+    p.mat[N.ROW_HONEY].viscosity = 0.09
+    scene = sc.Scene(grid)
+    scene.add(floor_strip(scene, W * strip[0], W * strip[1]), temp=strip_temp)
+    water = sc.fill_rect(scene, (0.0, 0.0, float(W), 0.35 * H), N.ROW_WATER)
+    oil = sc.fill_rect(scene, (0.0, 0.35 * H, float(W), 0.55 * H), N.ROW_OIL, len(water))
+    rng = np.random.default_rng(seed)
+    cx = W * (0.5 + 0.05 * (rng.random() - 0.5))
+    honey = sc.fill_rect(scene, (cx - 0.06 * W, 0.72 * H, cx + 0.06 * W, 0.86 * H), N.ROW_HONEY, len(water) + len(oil))
+    parts = np.concatenate([water, oil, honey])
+    field = scene.rasterize()
+    parts = settle(parts, p)
+    return p, field, parts, [N.ROW_WATER, N.ROW_HONEY, N.ROW_OIL]

@@ -65,7 +65,7 @@ class Pair:
                   "MGA": (self.L.total, 4), "MGD": (self.L.total,), "SPROP": (self.N, 4),
                   "VEL0": (self.N, 2), "PTILE": (self.L.tiles,), "AUX": (self.N, 4), "GRHO": (self.N, 4),
                   "SURF2": (self.N, 4), "GBASE": (self.N, 4), "VW": (self.N, 4), "VS": (self.N, 4),
-                  "P2KEEP": (self.N,)}
+                  "P2KEEP": (self.N,), "THERM": (self.N, 4), "DHEAT": (self.N, 4), "SOLID_T": (self.N,)}
         for v in ("VR", "VP", "VP2", "VQ", "VD", "VR2", "VQ2", "VAZ"):
             shapes[v] = (self.N, 2)
         for v in ("VMA", "VMB"):
@@ -82,6 +82,10 @@ class Pair:
                 self.gpu.put_field("ACC", self.ora.field("ACC", np.int32, self.N * self.stride))
             elif n == "PARTICLES":
                 self.gpu.put_field("PARTICLES", self.ora.field("PARTICLES", N.PARTICLE_DTYPE, self.ora.n_slots()))
+            elif n == "HCELL":
+                self.gpu.put_field("HCELL", self.ora.field("HCELL", np.int32, 2 * self.N))
+            elif n == "LEDGER":
+                self.gpu.put_field("LEDGER", self.ora.field("LEDGER", np.uint32, 16))
             elif n in ("PTILE", "VTILE"):
                 self.gpu.put_field(n, self.ora.field(n, np.uint32, self.L.tiles))
             elif n == "VTILES":
