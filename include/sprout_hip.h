@@ -170,7 +170,9 @@ int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* e
  * scene's results are those it gets stepped alone with the same requests (sp_record +
  * sp_wait per request). Requests of one scene run in arrival order.
  * While added to a server, a handle must not be used by the caller (its calls return
- * SP_E_BUSY); after sp_server_destroy it is the caller's again. */
+ * SP_E_BUSY); after sp_server_destroy it is the caller's again.
+ * In this build the server is switched off: sp_server_create returns SP_E_UNSUPPORTED.
+ * sp_pool_* below serves the same requests, driven by the caller's serving loop. */
 typedef struct sp_server sp_server;
 typedef struct sp_request_info {
     uint64_t ticket;
@@ -190,6 +192,7 @@ typedef struct sp_server_stats {
     double p50_ms, p99_ms, mean_ms, max_ms;   /* request latency, completed requests */
     double span_s;               /* first submit to last completion */
     double tanks_per_launch;     /* mean */
+    uint64_t rounds;             /* sp_pool rounds (0 for the server) */
 } sp_server_stats;
 int sp_server_create(int32_t device, uint64_t hbm_budget, uint32_t streams, sp_server** out);
 /* Runs every request already submitted, stops the worker, and gives the handles back. */
@@ -208,6 +211,49 @@ int sp_server_drain(sp_server* server);
 /* Counters and latency percentiles since the last sp_server_reset_stats. */
 int sp_server_stats_get(sp_server* server, sp_server_stats* out);
 int sp_server_reset_stats(sp_server* server);
+
+/* ---- the continuous-batching pool ------------------------------------------------------ */
+
+/* No counterpart in the reference: serving step requests for many resident scenes (tanks),
+ * for callers that run a serving loop. Requests are queued from any thread
+ * (sp_pool_submit, never blocking) and run in rounds by whichever thread calls sp_pool_step
+ * (one round), sp_pool_wait or sp_pool_drain (rounds until done). In a round every scene with
+ * an active request advances one substep: scenes of the same grid and solver schedule share
+ * each launch (up to 32 tanks, sp_batch's grouping), launches of different groups overlap on
+ * `streams` streams, and the round ends when they all have. A request of n substeps is active
+ * for n consecutive rounds; a request that arrives while a round runs starts at the next one
+ * (continuous batching at substep granularity); a scene's requests run in arrival order, one
+ * at a time. Scenes stay on the device while their evictable bytes (sp_tank_bytes) fit
+ * `hbm_budget`; over it, idle scenes are evicted to host memory, least recently used first,
+ * and brought back when a request of theirs starts, between rounds. Each scene's results are
+ * those it gets stepped alone with the same requests (sp_record + sp_wait per request).
+ * While added to a pool, a handle must not be used by the caller (its calls return
+ * SP_E_BUSY); after sp_pool_destroy it is the caller's again. Request records are kept for
+ * the last 2^20 done requests. */
+typedef struct sp_pool sp_pool;
+int sp_pool_create(int32_t device, uint64_t hbm_budget, uint32_t streams, sp_pool** out);
+/* Runs every request already submitted, then gives the handles back. */
+void sp_pool_destroy(sp_pool* pool);
+const char* sp_pool_last_error(sp_pool* pool);
+/* Adds a tank (set up: parameters, solids, particles) as scene *scene (0, 1, ...);
+ * SP_E_CAPACITY if its evictable bytes alone exceed the budget. */
+int sp_pool_add(sp_pool* pool, sp_sim* sim, uint32_t* scene);
+/* Queues a request of `substeps` substeps covering `target_s` seconds (as sp_record).
+ * Thread-safe; never waits for a round. */
+int sp_pool_submit(sp_pool* pool, uint32_t scene, uint32_t substeps, float target_s, uint64_t* ticket);
+/* Runs one round; with nothing to run, first waits up to `wait_ms` for a request.
+ * *advanced: the scenes the round advanced (0: none). */
+int sp_pool_step(sp_pool* pool, double wait_ms, uint32_t* advanced);
+/* The request's state: SP_E_BUSY while it is not done. */
+int sp_pool_poll(sp_pool* pool, uint64_t ticket, sp_request_info* out);
+/* Runs rounds (or waits for the thread running them) until the request is done. */
+int sp_pool_wait(sp_pool* pool, uint64_t ticket, sp_request_info* out);
+/* Runs rounds until every submitted request is done. */
+int sp_pool_drain(sp_pool* pool);
+/* Counters and latency percentiles since the last sp_pool_reset_stats (launch_ms in
+ * sp_request_info: the request's first round). */
+int sp_pool_stats_get(sp_pool* pool, sp_server_stats* out);
+int sp_pool_reset_stats(sp_pool* pool);
 
 /* ---- host-side derivations (no device needed) ---------------------------------------- */
 

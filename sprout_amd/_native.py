@@ -109,7 +109,7 @@ class ServerStats(C.Structure):
                 ("restores", C.c_uint64), ("resident_bytes", C.c_uint64), ("scenes", C.c_uint32),
                 ("resident", C.c_uint32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("span_s", C.c_double),
-                ("tanks_per_launch", C.c_double)]
+                ("tanks_per_launch", C.c_double), ("rounds", C.c_uint64)]
 
 
 class StatsOut(C.Structure):
@@ -229,6 +229,17 @@ SIGNATURES = {
     "sp_server_drain": ([_P], C.c_int),
     "sp_server_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
     "sp_server_reset_stats": ([_P], C.c_int),
+    "sp_pool_create": ([C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
+    "sp_pool_destroy": ([_P], None),
+    "sp_pool_last_error": ([_P], C.c_char_p),
+    "sp_pool_add": ([_P, _P, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_pool_submit": ([_P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_uint64)], C.c_int),
+    "sp_pool_step": ([_P, C.c_double, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_pool_poll": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
+    "sp_pool_wait": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
+    "sp_pool_drain": ([_P], C.c_int),
+    "sp_pool_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
+    "sp_pool_reset_stats": ([_P], C.c_int),
 }
 
 _lib = None

@@ -1114,3 +1114,4 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
 
 #include "batch.hip"
 #include "serve.hip"
+#include "pool.hip"

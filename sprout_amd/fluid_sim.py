@@ -276,6 +276,82 @@ class FluidServer:
         self._check(self._lib.sp_server_reset_stats(self._h), "server reset_stats")
 
 
+class FluidPool:
+    """Continuous batching of step requests over many resident scenes (sp_pool_*), run in
+    rounds by the thread that calls step(), wait() or drain(); submit() may be called from
+    any thread. The FluidSim handles added belong to the pool until close(); each scene's
+    results are those it gets stepped alone with the same requests."""
+
+    def __init__(self, budget_bytes, streams=4, device=0):
+        L = N.lib()
+        h = C.c_void_p()
+        rc = L.sp_pool_create(device, int(budget_bytes), streams, C.byref(h))
+        if rc != N.OK:
+            raise N.SproutError(rc, "sp_pool_create")
+        self._h = h
+        self._lib = L
+        self.sims = []
+
+    def close(self):
+        """Runs what was submitted and gives the handles back."""
+        if getattr(self, "_h", None):
+            self._lib.sp_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != N.OK:
+            raise N.SproutError(rc, f"{what}: {self._lib.sp_pool_last_error(self._h).decode()}")
+
+    def add(self, sim):
+        """Adds a set-up FluidSim as the next scene; returns its id."""
+        k = C.c_uint32()
+        self._check(self._lib.sp_pool_add(self._h, sim._h, C.byref(k)), "pool add")
+        self.sims.append(sim)
+        return k.value
+
+    def submit(self, scene, substeps, target):
+        t = C.c_uint64()
+        self._check(self._lib.sp_pool_submit(self._h, scene, substeps, float(target), C.byref(t)), "pool submit")
+        return t.value
+
+    def step(self, wait_ms=0.0):
+        """One round; returns the scenes it advanced."""
+        n = C.c_uint32()
+        self._check(self._lib.sp_pool_step(self._h, float(wait_ms), C.byref(n)), "pool step")
+        return n.value
+
+    def wait(self, ticket):
+        info = N.RequestInfo()
+        self._check(self._lib.sp_pool_wait(self._h, ticket, C.byref(info)), "pool wait")
+        return info
+
+    def poll(self, ticket):
+        """The request's info, or None while it is not done."""
+        info = N.RequestInfo()
+        rc = self._lib.sp_pool_poll(self._h, ticket, C.byref(info))
+        if rc == N.E_BUSY:
+            return None
+        self._check(rc, "pool poll")
+        return info
+
+    def drain(self):
+        self._check(self._lib.sp_pool_drain(self._h), "pool drain")
+
+    def stats(self):
+        st = N.ServerStats()
+        self._check(self._lib.sp_pool_stats_get(self._h, C.byref(st)), "pool stats")
+        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
+
+    def reset_stats(self):
+        self._check(self._lib.sp_pool_reset_stats(self._h), "pool reset_stats")
+
+
 class FluidBatch:
     """Many FluidSim handles advanced together (sp_batch_*): tanks of the same grid and
     solver schedule share each kernel launch of a substep, groups overlap on the device, and

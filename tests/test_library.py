@@ -96,3 +96,12 @@ def test_server_and_batch_refuse_bad_arguments():
     assert L.sp_is_resident(None, C.byref(r)) == N.E_INVALID
     # the request server is switched off until a stall it met on the device is understood
     assert L.sp_server_create(0, 1 << 30, 2, C.byref(h)) == N.E_UNSUPPORTED
+    # the pool checks its arguments before touching a device
+    assert L.sp_pool_create(0, 1 << 30, 0, C.byref(h)) == N.E_INVALID
+    assert L.sp_pool_create(0, 1 << 30, 17, C.byref(h)) == N.E_INVALID
+    assert L.sp_pool_create(0, 0, 2, C.byref(h)) == N.E_INVALID
+    assert L.sp_pool_submit(None, 0, 1, 0.1, None) == N.E_INVALID
+    assert L.sp_pool_step(None, 0.0, None) == N.E_INVALID
+    assert L.sp_pool_drain(None) == N.E_INVALID
+    assert L.sp_pool_wait(None, 1, None) == N.E_INVALID
+    assert L.sp_pool_last_error(None) == b"null pool"
