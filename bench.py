@@ -15,6 +15,15 @@ Workloads (config.workload):
   reductions.
 * dam_break_1024x576_fp32 (--workload dam): one block of water of the same size, explicit
   viscosity, one pressure solve.
+* mixed_64_tanks_bf16mg (--workload mixed): 64 independent dam-break tanks advanced as one
+  batch, grids drawn from 128x72, 256x144, 384x216 and 512x288 by seed, each block filling
+  30-60 % of its tank, water or oil, pressure multigrid levels in BF16/FP16; tanks of one
+  grid and schedule share every launch.
+
+`served` (--serve-rates): step requests of one substep for scenes drawn uniformly from 64
+resident mixed tanks, arriving as a Poisson process at each given rate, run by the
+continuous-batching pool (sp_pool_*) from this process's serving loop; p50/p99 request
+latency (submit to done) and the aggregate particle-steps/s of the served requests.
 
 Prints one JSON line. `roofline` is for the particle pass (k_particles), the largest single
 kernel of a substep: algorithmic bytes per launch = 96 B per live particle (its 48-byte
@@ -44,8 +53,8 @@ def pmc_traffic(kind, W, H):
     """HBM bytes per k_particles launch from the committed FETCH_SIZE + WRITE_SIZE summary
     of this workload (each counter scaled by what it reads for a plain copy of known size),
     or None where none was measured."""
-    path = PMC_SUMMARY[kind]
-    if (W, H) != (1024, 576) or not os.path.exists(path):
+    path = PMC_SUMMARY.get(kind)
+    if (W, H) != (1024, 576) or not path or not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
@@ -61,6 +70,84 @@ def workload(kind, grid, seed=0, mg_precision=0):
         out = cases.dam_break(grid, frac=(0.8, 0.8))
     out[0].mg_precision = mg_precision
     return out
+
+
+def served(rates, seconds, n_scenes=64, seed=0, streams=4):
+    """The served leg: for each rate, Poisson arrivals of one-substep requests for scenes
+    drawn uniformly from `n_scenes` resident mixed tanks over `seconds`, run by the pool
+    from this thread while a feeder thread submits; then drained."""
+    import threading
+    import cases
+    from sprout_amd import _native as N
+    from sprout_amd.fluid_sim import FluidPool, FluidSim
+
+    sims = []
+    for grid, params, field, parts, rows in cases.mixed_tanks(n_scenes, seed):
+        s = FluidSim(grid, grid[0] * grid[1] * 4, params=params)
+        s.set_solids(field)
+        s.reset(parts)
+        sims.append(s)
+    counts = [s.stats().live for s in sims]
+    budget = sum(N.tank_bytes(s.grid[0], s.grid[1], s.capacity)[0] for s in sims)
+    pool = FluidPool(budget, streams=streams)
+    for s in sims:
+        pool.add(s)
+    # warm-up: every scene's first substeps (no gather after a reset, first sorts)
+    for _ in range(3):
+        for k in range(n_scenes):
+            pool.submit(k, 1, 1 / 240)
+        pool.drain()
+    rng = np.random.default_rng(seed + 1)
+    out = []
+    for rate in rates:
+        n = max(int(rate * seconds), 1)
+        times = np.cumsum(rng.exponential(1.0 / rate, n))
+        scenes = rng.integers(0, n_scenes, n)
+        pool.reset_stats()
+        done = threading.Event()
+
+        def feed():
+            try:
+                t0 = time.perf_counter()
+                i = 0
+                while i < n:
+                    now = time.perf_counter() - t0
+                    while i < n and times[i] <= now:
+                        pool.submit(int(scenes[i]), 1, 1 / 240)
+                        i += 1
+                    if i < n:
+                        time.sleep(min(max(times[i] - now, 0.0), 0.001))
+            finally:
+                done.set()
+
+        th = threading.Thread(target=feed)
+        t0 = time.perf_counter()
+        th.start()
+        while not done.is_set():
+            pool.step(wait_ms=0.5)
+        th.join()
+        pool.drain()
+        wall = time.perf_counter() - t0
+        st = pool.stats()
+        print(f"served: {rate:.0f}/s offered, {st['completed']} requests, p99 {st['p99_ms']:.2f} ms", file=sys.stderr,
+              flush=True)
+        out.append({
+            "offered_rate_per_s": rate,
+            "requests": int(st["completed"]),
+            "failed": int(st["failed"]),
+            "served_rate_per_s": st["completed"] / wall,
+            "particle_steps_per_s": st["particle_steps"] / wall,
+            "latency_ms": {"p50": st["p50_ms"], "p99": st["p99_ms"], "mean": st["mean_ms"], "max": st["max_ms"]},
+            "rounds": int(st["rounds"]),
+            "scenes_per_round": st["completed"] / max(st["rounds"], 1),
+            "tanks_per_launch": st["tanks_per_launch"],
+            "evictions": int(st["evictions"]),
+        })
+    pool.close()
+    for s in sims:
+        s.close()
+    return {"scenes": n_scenes, "particles": int(sum(counts)), "seconds_per_rate": seconds, "streams": streams,
+            "request": "1 substep (1/240 s) of one scene", "budget": "every scene resident", "by_rate": out}
 
 
 def cpu_baseline(grid, params, field, parts, rows, substeps):
@@ -100,7 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--grid", default="1024x576")
-    ap.add_argument("--workload", choices=["four", "dam"], default="four")
+    ap.add_argument("--workload", choices=["four", "dam", "mixed"], default="four")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=8)
     ap.add_argument("--tanks", type=int, default=1,
@@ -111,6 +198,9 @@ def main():
                     help="pressure multigrid storage: FP32, or BF16 iterates / FP16 coarse matrices")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the pressure solve's comparison with an FP64 solve of the same system")
+    ap.add_argument("--serve-rates", default="",
+                    help="comma-separated request rates (per s) for the served leg over 64 mixed tanks; empty: none")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -122,12 +212,21 @@ def main():
     W, H = (int(x) for x in args.grid.split("x"))
     tanks = []
     mgp = 1 if args.mg_precision == "mixed" else 0
-    for seed in range(max(args.tanks, 1)):
-        params, field, parts, rows = workload(args.workload, (W, H), seed, mgp)
-        sim = FluidSim((W, H), W * H * 4, params=params)
-        sim.set_solids(field)
-        sim.reset(parts)
-        tanks.append((sim, params, field, parts, rows))
+    if args.workload == "mixed":
+        import cases
+        mgp = 1
+        for grid, params, field, parts, rows in cases.mixed_tanks(max(args.tanks, 1), 0):
+            sim = FluidSim(grid, grid[0] * grid[1] * 4, params=params)
+            sim.set_solids(field)
+            sim.reset(parts)
+            tanks.append((sim, params, field, parts, rows))
+    else:
+        for seed in range(max(args.tanks, 1)):
+            params, field, parts, rows = workload(args.workload, (W, H), seed, mgp)
+            sim = FluidSim((W, H), W * H * 4, params=params)
+            sim.set_solids(field)
+            sim.reset(parts)
+            tanks.append((sim, params, field, parts, rows))
     sim, params, field, parts, rows = tanks[0]
     n_p = sum(len(t[3]) for t in tanks)
     batch = FluidBatch([t[0] for t in tanks]) if len(tanks) > 1 else None
@@ -147,6 +246,7 @@ def main():
             step(k)
             done += k
 
+    print(f"bench: {len(tanks)} tank(s), {n_p} particles set up", file=sys.stderr, flush=True)
     run(args.warmup)
     ker = []
     phases = []
@@ -182,14 +282,25 @@ def main():
     ph = np.mean(np.array(phases), axis=0)
     # one particle-pass launch covers every tank of the (first) launch group
     group_tanks = batch.groups()[0] if batch is not None else 1
-    alg_bytes = group_tanks * (96.0 * len(parts) + 100.0 * W * H)
+    if args.workload == "mixed":
+        g0 = [t for t in tanks if t[0].grid == tanks[0][0].grid and t[4] == tanks[0][4]][:group_tanks]
+        alg_bytes = sum(96.0 * len(t[3]) + 100.0 * t[0].grid[0] * t[0].grid[1] for t in g0)
+    else:
+        alg_bytes = group_tanks * (96.0 * len(parts) + 100.0 * W * H)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     live = sum(t[0].stats().live for t in tanks)
-    name = {"four": "four_liquids", "dam": "dam_break"}[args.workload]
+    name = {"four": "four_liquids", "dam": "dam_break", "mixed": "mixed"}[args.workload]
+    if args.workload == "mixed":
+        wname = f"mixed_{len(tanks)}_tanks_bf16mg"
+        sizes = sorted({t[0].grid for t in tanks})
+        grid_desc = {f"{g[0]}x{g[1]}": sum(1 for t in tanks if t[0].grid == g) for g in sizes}
+    else:
+        wname = f"{name}_{W}x{H}_fp32" + (f"_batch{len(tanks)}" if len(tanks) > 1 else "")
+        grid_desc = [W, H]
     config = {
-        "workload": f"{name}_{W}x{H}_fp32" + (f"_batch{len(tanks)}" if len(tanks) > 1 else ""),
+        "workload": wname,
         "tanks": len(tanks),
-        "grid": [W, H],
+        "grid": grid_desc,
         "particles": n_p,
         "live_particles_end": int(live),
         "cell_size_m": 0.005,
@@ -259,11 +370,14 @@ def main():
             "iterations": [int(v) for v in K["iters"]],
         }
     if batch is not None:
+        out["launch_groups"] = batch.groups()
         batch.close()
     for t in tanks:
         t[0].close()
+    if args.serve_rates:
+        out["served"] = served([float(r) for r in args.serve_rates.split(",")], args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline((W, H), params, field, parts, rows, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline(tanks[0][0].grid, params, field, parts, rows, args.cpu_substeps)
     print(json.dumps(out))
 
 

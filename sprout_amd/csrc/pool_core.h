@@ -30,7 +30,8 @@
 //   void evict(uint32_t scene); void restore(uint32_t scene);
 //   uint64_t key(uint32_t scene);
 //   int begin(uint32_t scene, uint32_t substeps, float target);  // 0, or the request fails with it
-//   void launch(int stream, const std::vector<uint32_t>& scenes);  // one substep of each
+//   void launch(int stream, const std::vector<uint32_t>& scenes,    // one substep of each; `last`:
+//               const std::vector<char>& last);                      //   its request's final one
 //   void sync();                                                    // the round's launches ended
 //   uint64_t finish(uint32_t scene, uint32_t substeps);             // read back; particle-steps
 // A thrown std::exception fails every request not done, and the pool with it.
@@ -369,7 +370,11 @@ private:
             for (size_t a = 0; a < ks.size(); a += max_group_)
                 groups.emplace_back(ks.begin() + (long)a, ks.begin() + (long)std::min(ks.size(), a + max_group_));
         }
-        for (size_t gi = 0; gi < groups.size(); gi++) be_.launch((int)(gi % (size_t)streams_), groups[gi]);
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            std::vector<char> last(groups[gi].size());
+            for (size_t i = 0; i < last.size(); i++) last[i] = scenes_[groups[gi][i]].left == 1 ? 1 : 0;
+            be_.launch((int)(gi % (size_t)streams_), groups[gi], last);
+        }
         be_.sync();
         {
             std::lock_guard<std::mutex> g(mu_);

@@ -49,6 +49,30 @@ def dam_break(grid, frac=(0.4, 0.6), **kw):
     return params, field, parts, [N.ROW_WATER]
 
 
+MIXED_SIZES = ((128, 72), (256, 144), (384, 216), (512, 288))
+
+
+def mixed_tanks(n=64, seed=0, mg_precision=1, pressure_tol=1e-6):
+    """Independent dam-break tanks of uneven sizes: each tank's grid is drawn from
+    MIXED_SIZES, its block fills 30-60 % of the tank (area drawn, then the width fraction,
+    the height following), its liquid is water or oil (the built-in presets), cell 5 mm,
+    dt 1/240 s; pressure multigrid storage `mg_precision` (1: BF16 iterates, FP16 coarse
+    matrices). List of (grid, params, solid field, settled particles, rows)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        grid = MIXED_SIZES[int(rng.integers(len(MIXED_SIZES)))]
+        area = float(rng.uniform(0.3, 0.6))
+        fw = float(rng.uniform(max(area / 0.9, 0.35), 0.9))
+        row = (N.ROW_WATER, N.ROW_OIL)[int(rng.integers(2))]
+        params = dam_params(pressure_tol=pressure_tol)
+        params.mg_precision = mg_precision
+        scene, parts = sc.dam_break(grid, frac=(fw, area / fw), row=row)
+        field = scene.rasterize()
+        out.append((grid, params, field, settle(parts, params), [row]))
+    return out
+
+
 FOUR_ROWS = (N.ROW_OIL, N.ROW_WATER, N.ROW_HONEY, N.ROW_MERCURY)
 FOUR_DENSITY = {N.ROW_OIL: 800.0, N.ROW_WATER: 1000.0, N.ROW_HONEY: 1300.0, N.ROW_MERCURY: 13500.0}
 

@@ -40,6 +40,7 @@ struct MockBackend {
         uint32_t active = 0;  // substeps of the active request, 0: idle
         uint32_t ran = 0;     // its launches so far
         uint64_t last_round = 0;
+        bool last_seen = false;      // its latest launch was flagged as its request's last
         std::vector<uint32_t> done;  // substep counts of its finished requests, in order
         int begins = 0;
         int refuse_at = -1;  // this begin (0-based) is refused
@@ -93,12 +94,16 @@ struct MockBackend {
         s.ran = 0;
         return 0;
     }
-    void launch(int stream, const std::vector<uint32_t>& ks) {
+    void launch(int stream, const std::vector<uint32_t>& ks, const std::vector<char>& last) {
         enter();
         if (stream < 0 || stream >= streams) fail("stream out of range", stream);
         if (ks.empty() || ks.size() > max_group) fail("launch group size", (long)ks.size());
-        for (uint32_t k : ks) {
+        if (last.size() != ks.size()) fail("last flags per member");
+        for (size_t i = 0; i < ks.size(); i++) {
+            const uint32_t k = ks[i];
             Scene& s = sc[k];
+            if ((last[i] != 0) != (s.ran + 1 == s.active)) fail("last-substep flag wrong", k, (long)s.ran);
+            s.last_seen = last[i] != 0;
             if (!s.resident) fail("launch of an evicted scene", k);
             if (!s.active) fail("launch of an idle scene", k);
             if (s.key != sc[ks[0]].key) fail("launch mixes keys", k);
@@ -123,6 +128,7 @@ struct MockBackend {
         enter();
         Scene& s = sc[k];
         if (s.active != substeps || s.ran != substeps) fail("finish before every substep ran", k, (long)s.ran);
+        if (!s.last_seen) fail("finish without a launch flagged last", k);
         s.done.push_back(substeps);
         s.active = 0;
         leave();
