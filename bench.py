@@ -198,9 +198,9 @@ def main():
                     help="pressure multigrid storage: FP32, or BF16 iterates / FP16 coarse matrices")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the pressure solve's comparison with an FP64 solve of the same system")
-    ap.add_argument("--serve-rates", default="",
+    ap.add_argument("--serve-rates", default="1000,2000,4000",
                     help="comma-separated request rates (per s) for the served leg over 64 mixed tanks; empty: none")
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
