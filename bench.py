@@ -369,6 +369,16 @@ def main():
             "velocity_rel_l2_vs_fp64": vel_err,
             "iterations": [int(v) for v in K["iters"]],
         }
+    from sprout_amd import _native as N
+    shapes = N.autotune_shapes()
+    out["autotune"] = {
+        "what": "particle-pass launch shape per (grid, tanks per launch, accumulator stride), chosen by timing "
+                "each candidate on the launches themselves (every shape gives bit-identical results)",
+        "candidates": [{"threads": t, "box_ints": b} for t, b in shapes],
+        "chosen": [{"grid": [e["grid_w"], e["grid_h"]], "tanks": e["tanks"], "stride": e["stride"],
+                    "shape": shapes[e["best"]] if e["best"] >= 0 else None, "best_ms": e["best_ms"],
+                    "default_ms": e["default_ms"]} for e in N.autotune_report()],
+    }
     if batch is not None:
         out["launch_groups"] = batch.groups()
         batch.close()

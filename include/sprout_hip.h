@@ -255,6 +255,27 @@ int sp_pool_drain(sp_pool* pool);
 int sp_pool_stats_get(sp_pool* pool, sp_server_stats* out);
 int sp_pool_reset_stats(sp_pool* pool);
 
+/* ---- launch-shape autotuner ----------------------------------------------------------- */
+
+/* No counterpart in the reference (its work-group shapes are fixed in the shaders): the
+ * particle pass's launch shape (work-items per group, capacity of the LDS scatter box) is
+ * chosen per key (grid, tanks per launch, accumulator stride) by timing the candidates on the
+ * launches themselves, process-wide. Every shape gives bit-identical results, so tuning never
+ * changes a result. mode -1: tune (the default); -2: forget what was tuned and tune again;
+ * 0 .. n-1: use that shape everywhere (sp_autotune_shapes). samples > 0: timed launches per
+ * candidate before the fastest median is kept (default 3). No device needed. */
+int sp_autotune(int32_t mode, uint32_t samples);
+/* The candidates: threads[k], box_ints[k] for k < min(cap, *n). */
+int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, uint32_t cap, uint32_t* n);
+typedef struct sp_tune_entry {
+    int32_t grid_w, grid_h, tanks, stride;  /* the key */
+    int32_t best;                           /* chosen shape, -1 while tuning */
+    float best_ms, default_ms;              /* median particle-pass ms: chosen shape, shape 0 */
+    uint32_t samples;                       /* timings taken */
+} sp_tune_entry;
+/* The keys seen so far: out[k] for k < min(cap, *n). */
+int sp_autotune_report(sp_tune_entry* out, uint32_t cap, uint32_t* n);
+
 /* ---- host-side derivations (no device needed) ---------------------------------------- */
 
 /* Default SI parameters and the built-in materials this build carries (water, honey, oil,

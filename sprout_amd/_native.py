@@ -112,6 +112,11 @@ class ServerStats(C.Structure):
                 ("tanks_per_launch", C.c_double), ("rounds", C.c_uint64)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("grid_w", C.c_int32), ("grid_h", C.c_int32), ("tanks", C.c_int32), ("stride", C.c_int32),
+                ("best", C.c_int32), ("best_ms", C.c_float), ("default_ms", C.c_float), ("samples", C.c_uint32)]
+
+
 class StatsOut(C.Structure):
     _fields_ = [("live", C.c_uint32), ("count", C.c_uint32 * MAX_MATERIALS),
                 ("conc", C.c_float * MAX_MATERIALS)]
@@ -229,6 +234,9 @@ SIGNATURES = {
     "sp_server_drain": ([_P], C.c_int),
     "sp_server_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
     "sp_server_reset_stats": ([_P], C.c_int),
+    "sp_autotune": ([C.c_int32, C.c_uint32], C.c_int),
+    "sp_autotune_shapes": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_autotune_report": ([C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_pool_create": ([C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
     "sp_pool_destroy": ([_P], None),
     "sp_pool_last_error": ([_P], C.c_char_p),
@@ -268,6 +276,39 @@ def tank_bytes(W, H, capacity):
     if rc != OK:
         raise SproutError(rc, "tank_bytes")
     return ev.value, fx.value
+
+
+def autotune(mode=-1, samples=0):
+    """sp_autotune: -1 tune the particle pass's launch shape on the launches (default), -2
+    forget and tune again, k >= 0 use shape k everywhere; samples > 0: timings per shape."""
+    rc = lib().sp_autotune(int(mode), int(samples))
+    if rc != OK:
+        raise SproutError(rc, "autotune")
+
+
+def autotune_shapes():
+    """The candidate shapes: [(threads per group, LDS scatter-box ints), ...]."""
+    n = C.c_uint32()
+    t = np.zeros(16, np.int32)
+    b = np.zeros(16, np.int32)
+    rc = lib().sp_autotune_shapes(t.ctypes.data, b.ctypes.data, 16, C.byref(n))
+    if rc != OK:
+        raise SproutError(rc, "autotune_shapes")
+    return [(int(t[k]), int(b[k])) for k in range(min(n.value, 16))]
+
+
+def autotune_report():
+    """The tuned keys: dicts of grid, tanks, stride, best shape (-1 while tuning), the
+    median particle-pass ms of the best shape and of shape 0, and the timings taken."""
+    n = C.c_uint32()
+    rc = lib().sp_autotune_report(None, 0, C.byref(n))
+    if rc != OK:
+        raise SproutError(rc, "autotune_report")
+    arr = (TuneEntry * max(n.value, 1))()
+    rc = lib().sp_autotune_report(arr, n.value, C.byref(n))
+    if rc != OK:
+        raise SproutError(rc, "autotune_report")
+    return [{name: getattr(arr[k], name) for name, _ in TuneEntry._fields_} for k in range(n.value)]
 
 
 class SproutError(RuntimeError):

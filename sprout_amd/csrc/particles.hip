@@ -13,8 +13,6 @@
 
 namespace {
 
-constexpr int PT = 256;           // work-items per group
-constexpr int LDS_INTS = 12288;   // scatter box budget: 48 KiB
 
 struct ScatterIn {
     bool live;
@@ -548,21 +546,25 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
 
 }  // namespace
 
-// The particle pass (particles.comp:1064-1101).
+// The particle pass (particles.comp:1064-1101), NT work-items per group, a scatter box of
+// BOX ints in LDS, registers for at least MINB groups per compute unit. Every (NT, BOX) gives the same result (integer sums; the speed maximum is
+// a maximum), so the launch shape is free to tune.
 // gather < 0: each tank decides from its own step state (the first substep after a reset,
 // which k_step has just counted, has no grid velocities to gather yet).
-__global__ __launch_bounds__(PT) void k_particles(const TankBatch tb, int gather) {
+template <int NT, int BOX, int MINB>
+__global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int gather) {
+    static_assert(NT % 64 == 0 && NT <= 1024 && BOX * 4 <= 64 * 1024, "launch shape");
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     if (gather < 0) gather = c.S->substeps > 1u ? 1 : 0;
-    __shared__ int sacc[LDS_INTS];
+    __shared__ int sacc[BOX];
     __shared__ int sbox[4];
     const int t = threadIdx.x;
-    const uint32_t id = blockIdx.x * PT + t;
+    const uint32_t id = blockIdx.x * NT + t;
     ScatterIn g;
     float umax = 0.0f;
-    __shared__ float s_max[PT / 64];
+    __shared__ float s_max[NT / 64];
     advance(c, id, gather, g, umax);
-    group_max_to<PT>(&c.S->umax_part, umax, s_max);
+    group_max_to<NT>(&c.S->umax_part, umax, s_max);
 
     if (t == 0) {
         sbox[0] = 0x7fffffff;
@@ -587,14 +589,14 @@ __global__ __launch_bounds__(PT) void k_particles(const TankBatch tb, int gather
     const int bw = sbox[2] + 2 - b0x + 1;
     const int bh = sbox[3] + 2 - b0y + 1;
     const int n = bw * bh * stride;
-    const bool shared = n <= LDS_INTS;
+    const bool shared = n <= BOX;
     if (shared) {
-        for (int k = t; k < n; k += PT) sacc[k] = 0;
+        for (int k = t; k < n; k += NT) sacc[k] = 0;
         __syncthreads();
         if (g.live)
             deposit(g, [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * stride + ch], v); });
         __syncthreads();
-        for (int k = t; k < n; k += PT) {
+        for (int k = t; k < n; k += NT) {
             int v = sacc[k];
             if (v == 0) continue;
             int cell = k / stride;

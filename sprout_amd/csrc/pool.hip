@@ -78,6 +78,7 @@ struct PoolBackend {
     std::vector<sp_sim*> sims;
     std::vector<hipStream_t> streams;
     std::vector<char> used;  // streams with launches in this round
+    std::vector<std::unique_ptr<Timing>> tms;  // per stream: timings for the launch-shape tuner
     struct Start {
         bool pending = false;  // written by the first launch of the request
         float target = 0.0f;
@@ -169,7 +170,8 @@ struct PoolBackend {
         const dim3 per_tank(1, 1, (unsigned)r.T);
         if (ps.mask) hipLaunchKernelGGL(k_pool_begin, per_tank, dim3(64), 0, stream, r.tb, ps);
         for (uint32_t m : members) maybe_sort(sims[m], stream);
-        launch_substep(r, nullptr, nullptr, -1, rec);
+        Timing* tm = tuner().sampling(particle_key(r)) ? tms[(size_t)si].get() : nullptr;
+        launch_substep(r, nullptr, tm, -1, rec);
         if (last_mask) {
             Mailbox& b = box(n_launch);
             hipLaunchKernelGGL(k_pool_finish, per_tank, dim3(64), 0, stream, r.tb, last_mask, b.dev);
@@ -187,6 +189,7 @@ struct PoolBackend {
         for (size_t i = 0; i < streams.size(); i++)
             if (used[i]) {
                 HIPCHK(hipStreamSynchronize(streams[i]));
+                tms[i]->collect();
                 used[i] = 0;
             }
         n_launch = 0;
@@ -226,6 +229,7 @@ void free_pool_streams(PoolBackend& be) {
     for (hipStream_t st : be.streams)
         if (st) (void)hipStreamDestroy(st);
     be.streams.clear();
+    be.tms.clear();
     for (auto& b : be.boxes) {
         (void)hipFree(b.dev);
         (void)hipHostFree(b.host);
@@ -249,6 +253,10 @@ int sp_pool_create(int32_t device, uint64_t hbm_budget, uint32_t streams, sp_poo
         be.streams.assign(streams, nullptr);
         be.used.assign(streams, 0);
         for (auto& st : be.streams) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (uint32_t i = 0; i < streams; i++) {
+            be.tms.emplace_back(new Timing());
+            be.tms.back()->create();
+        }
     } catch (const HipError&) {
         free_pool_streams(be);
         delete v;

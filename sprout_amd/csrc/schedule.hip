@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 struct HipError {
@@ -250,9 +252,129 @@ void launch_clear_acc(const Run& r) {
     hipLaunchKernelGGL(k_clear_acc, dim3(std::max(blocks, 1), 1, r.T), dim3(256), 0, r.stream, r.tb);
 }
 
-void launch_particles(const Run& r, int gather) {
-    const uint32_t blocks = std::max<uint32_t>((r.slots + 255) / 256, 1);
-    hipLaunchKernelGGL(k_particles, dim3(blocks, 1, r.T), dim3(256), 0, r.stream, r.tb, gather);
+}  // namespace
+
+// ---- the particle pass's launch shape, tuned on the launches themselves -----------------
+//
+// Candidates: work-items per group and the capacity of the LDS scatter box. Every shape
+// gives bit-identical results, so each launch of a key may use any of them: while a key is
+// tuned, its launches take the candidates in turn (launch by launch, since timings arrive
+// only after the launches' stream is synchronised), the timed ones (Timing's events around
+// the particle pass) report their durations, and once every candidate has `samples` timings
+// the fastest median is kept for the key. Shape 0 is the untuned default.
+struct PShape {
+    int threads, box, min_groups;
+};
+constexpr int N_PSHAPES = 9;
+constexpr PShape P_SHAPES[N_PSHAPES] = {{256, 12288, 1}, {256, 8192, 1}, {256, 4096, 1}, {128, 2048, 1}, {128, 4096, 1},
+                                        {512, 12288, 1}, {512, 8192, 1}, {256, 4096, 5},  {256, 6144, 1}};
+using PKernel = void (*)(TankBatch, int);
+const PKernel P_KERNELS[N_PSHAPES] = {k_particles<256, 12288, 1>, k_particles<256, 8192, 1>, k_particles<256, 4096, 1>,
+                                      k_particles<128, 2048, 1>,  k_particles<128, 4096, 1>, k_particles<512, 12288, 1>,
+                                      k_particles<512, 8192, 1>,  k_particles<256, 4096, 5>, k_particles<256, 6144, 1>};
+
+// What a shape is tuned for: the grid, the tanks per launch and the accumulator stride.
+uint64_t particle_key(const Run& r) {
+    return (uint64_t)(uint32_t)r.W | (uint64_t)(uint32_t)r.H << 14 | (uint64_t)(uint32_t)r.T << 28 |
+           (uint64_t)(uint32_t)r.gp.acc_stride << 34;
+}
+
+class Tuner {
+public:
+    struct Report {
+        uint64_t key;
+        int best;
+        float best_ms, default_ms;
+        uint32_t samples;
+    };
+
+    // The shape of the next launch of `key`.
+    int pick(uint64_t key) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (forced_ >= 0) return forced_;
+        if (samples_ == 0) return 0;
+        Entry& e = map_[key];
+        if (e.best >= 0) return e.best;
+        return (int)(e.turn++ % (uint32_t)N_PSHAPES);
+    }
+    // Whether launches of `key` still gather timings.
+    bool sampling(uint64_t key) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (forced_ >= 0 || samples_ == 0) return false;
+        auto it = map_.find(key);
+        return it == map_.end() || it->second.best < 0;
+    }
+    void add(uint64_t key, int v, float ms) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (forced_ >= 0 || samples_ == 0 || v < 0 || v >= N_PSHAPES) return;
+        Entry& e = map_[key];
+        if (e.best >= 0) return;
+        e.ms[v].push_back(ms);
+        for (int k = 0; k < N_PSHAPES; k++)
+            if (e.ms[k].size() < samples_) return;
+        float best = 0.0f;
+        for (int k = 0; k < N_PSHAPES; k++) {
+            const float m = median(e.ms[k]);
+            if (e.best < 0 || m < best) {
+                e.best = k;
+                best = m;
+            }
+        }
+    }
+    // mode -1: tune; -2: forget what was tuned, then tune; 0 .. N_PSHAPES-1: that shape
+    // everywhere. samples > 0: timings per candidate before choosing.
+    bool set(int mode, uint32_t samples) {
+        if (mode < -2 || mode >= N_PSHAPES) return false;
+        std::lock_guard<std::mutex> g(mu_);
+        if (mode == -2) map_.clear();
+        forced_ = mode >= 0 ? mode : -1;
+        if (samples > 0) samples_ = samples;
+        return true;
+    }
+    std::vector<Report> report() {
+        std::lock_guard<std::mutex> g(mu_);
+        std::vector<Report> out;
+        for (auto& kv : map_) {
+            const Entry& e = kv.second;
+            uint32_t n = 0;
+            for (int k = 0; k < N_PSHAPES; k++) n += (uint32_t)e.ms[k].size();
+            out.push_back({kv.first, e.best, e.best >= 0 ? median(e.ms[e.best]) : 0.0f,
+                           e.ms[0].empty() ? 0.0f : median(e.ms[0]), n});
+        }
+        return out;
+    }
+
+private:
+    struct Entry {
+        int best = -1;
+        uint32_t turn = 0;  // launches picked while tuning
+        std::vector<float> ms[N_PSHAPES];
+    };
+    static float median(std::vector<float> v) {
+        if (v.empty()) return 0.0f;
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    }
+    std::mutex mu_;
+    int forced_ = -1;
+    uint32_t samples_ = 3;
+    std::map<uint64_t, Entry> map_;
+};
+
+Tuner& tuner() {
+    static Tuner t;
+    return t;
+}
+
+namespace {
+
+// Launches the particle pass in the shape the tuner picks; returns the shape.
+int launch_particles(const Run& r, int gather) {
+    const int v = tuner().pick(particle_key(r));
+    const PShape sh = P_SHAPES[v];
+    const uint32_t blocks = std::max<uint32_t>((r.slots + (uint32_t)sh.threads - 1) / (uint32_t)sh.threads, 1);
+    hipLaunchKernelGGL(P_KERNELS[v], dim3(blocks, 1, r.T), dim3((unsigned)sh.threads), 0, r.stream, r.tb, gather);
+    return v;
 }
 
 // What the host decides about a substep's launches from the tables (the parameters the
@@ -302,6 +424,8 @@ struct Timing {
     static constexpr int EV_PER = 9;
     bool on = false;
     std::vector<hipEvent_t> ev;
+    uint64_t pkey[SUBSTEPS] = {};  // the particle pass's tuning key and shape per timed substep
+    int pshape[SUBSTEPS] = {};
     uint32_t timed = 0, timed_last = 0;
     float phase[7] = {};
     float particle_ms = 0.0f;
@@ -316,6 +440,10 @@ struct Timing {
             if (e) (void)hipEventDestroy(e);
     }
     hipEvent_t* next() { return on && timed < (uint32_t)SUBSTEPS ? &ev[(size_t)timed++ * EV_PER] : nullptr; }
+    void note_shape(uint64_t key, int shape) {
+        pkey[timed - 1] = key;
+        pshape[timed - 1] = shape;
+    }
     // Averages over the substeps recorded since the last call (after the stream finished).
     void collect() {
         if (!on || timed == 0) return;
@@ -331,6 +459,7 @@ struct Timing {
             }
             HIPCHK(hipEventElapsedTime(&t, e[5], e[6]));
             kp += t;
+            tuner().add(pkey[k], pshape[k], t);
         }
         for (int q = 0; q < 7; q++) phase[q] = (float)(acc[q] / timed);
         particle_ms = (float)(kp / timed);
@@ -380,8 +509,11 @@ void launch_substep(const Run& r, GraphSet* gs, Timing* tm, int gather, const Re
     hipLaunchKernelGGL(k_step, per_tank(r), dim3(64), 0, r.stream, r.tb);
     launch_clear_acc(r);
     if (e) HIPCHK(hipEventRecord(e[5], r.stream));
-    launch_particles(r, gather);
-    if (e) HIPCHK(hipEventRecord(e[6], r.stream));
+    const int shape = launch_particles(r, gather);
+    if (e) {
+        tm->note_shape(particle_key(r), shape);
+        HIPCHK(hipEventRecord(e[6], r.stream));
+    }
     if (e) HIPCHK(hipEventRecord(e[1], r.stream));
     if (g) HIPCHK(hipGraphLaunch(g->exec[0], r.stream));
     else launch_grid_passes(r, recs);

@@ -935,6 +935,37 @@ int sp_timings(sp_sim* s, float* out10) {
     }, false);
 }
 
+int sp_autotune(int32_t mode, uint32_t samples) { return tuner().set(mode, samples) ? SP_OK : SP_E_INVALID; }
+
+int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, uint32_t cap, uint32_t* n) {
+    if (!n) return SP_E_INVALID;
+    *n = N_PSHAPES;
+    for (uint32_t k = 0; k < cap && k < (uint32_t)N_PSHAPES; k++) {
+        if (threads) threads[k] = P_SHAPES[k].threads;
+        if (box_ints) box_ints[k] = P_SHAPES[k].box;
+    }
+    return SP_OK;
+}
+
+int sp_autotune_report(sp_tune_entry* out, uint32_t cap, uint32_t* n) {
+    if (!n || (cap > 0 && !out)) return SP_E_INVALID;
+    const std::vector<Tuner::Report> r = tuner().report();
+    *n = (uint32_t)r.size();
+    for (uint32_t k = 0; k < cap && k < r.size(); k++) {
+        const uint64_t key = r[k].key;
+        sp_tune_entry& e = out[k];
+        e.grid_w = (int32_t)(key & 0x3fff);
+        e.grid_h = (int32_t)(key >> 14 & 0x3fff);
+        e.tanks = (int32_t)(key >> 28 & 0x3f);
+        e.stride = (int32_t)(key >> 34 & 0xff);
+        e.best = r[k].best;
+        e.best_ms = r[k].best_ms;
+        e.default_ms = r[k].default_ms;
+        e.samples = r[k].samples;
+    }
+    return SP_OK;
+}
+
 int sp_kernel_time(sp_sim* s, int32_t kernel, float* avg_ms, uint32_t* launches) {
     return guarded(s, [&]() -> int {
         if (!avg_ms || !launches || kernel != 0) return SP_E_INVALID;

@@ -105,3 +105,15 @@ def test_server_and_batch_refuse_bad_arguments():
     assert L.sp_pool_drain(None) == N.E_INVALID
     assert L.sp_pool_wait(None, 1, None) == N.E_INVALID
     assert L.sp_pool_last_error(None) == b"null pool"
+
+
+def test_autotune_modes_and_shapes_without_device():
+    shapes = N.autotune_shapes()
+    assert len(shapes) >= 2 and shapes[0] == (256, 12288)
+    assert all(t % 64 == 0 and 0 < b * 4 <= 64 * 1024 for t, b in shapes)
+    L = N.lib()
+    assert L.sp_autotune(len(shapes), 0) == N.E_INVALID
+    assert L.sp_autotune(-3, 0) == N.E_INVALID
+    N.autotune(1)
+    N.autotune(-1)
+    assert isinstance(N.autotune_report(), list)
