@@ -374,7 +374,7 @@ def main():
     out["autotune"] = {
         "what": "particle-pass launch shape per (grid, tanks per launch, accumulator stride), chosen by timing "
                 "each candidate on the launches themselves (every shape gives bit-identical results)",
-        "candidates": [{"threads": t, "box_ints": b} for t, b in shapes],
+        "candidates": [{"threads": t, "box_ints": b, "min_groups_per_cu": m} for t, b, m in shapes],
         "chosen": [{"grid": [e["grid_w"], e["grid_h"]], "tanks": e["tanks"], "stride": e["stride"],
                     "shape": shapes[e["best"]] if e["best"] >= 0 else None, "best_ms": e["best_ms"],
                     "default_ms": e["default_ms"]} for e in N.autotune_report()],

@@ -265,8 +265,10 @@ int sp_pool_reset_stats(sp_pool* pool);
  * 0 .. n-1: use that shape everywhere (sp_autotune_shapes). samples > 0: timed launches per
  * candidate before the fastest median is kept (default 3). No device needed. */
 int sp_autotune(int32_t mode, uint32_t samples);
-/* The candidates: threads[k], box_ints[k] for k < min(cap, *n). */
-int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, uint32_t cap, uint32_t* n);
+/* The candidates for k < min(cap, *n): threads[k] work-items per group, box_ints[k] ints of
+ * LDS scatter box, min_groups[k] groups per compute unit the registers are limited for (1:
+ * no limit). Any output may be NULL. */
+int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, int32_t* min_groups, uint32_t cap, uint32_t* n);
 typedef struct sp_tune_entry {
     int32_t grid_w, grid_h, tanks, stride;  /* the key */
     int32_t best;                           /* chosen shape, -1 while tuning */

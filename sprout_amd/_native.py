@@ -235,7 +235,7 @@ SIGNATURES = {
     "sp_server_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
     "sp_server_reset_stats": ([_P], C.c_int),
     "sp_autotune": ([C.c_int32, C.c_uint32], C.c_int),
-    "sp_autotune_shapes": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_autotune_shapes": ([_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_autotune_report": ([C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_pool_create": ([C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
     "sp_pool_destroy": ([_P], None),
@@ -287,14 +287,16 @@ def autotune(mode=-1, samples=0):
 
 
 def autotune_shapes():
-    """The candidate shapes: [(threads per group, LDS scatter-box ints), ...]."""
+    """The candidate shapes: [(threads per group, LDS scatter-box ints, groups per compute
+    unit the registers are limited for), ...]."""
     n = C.c_uint32()
     t = np.zeros(16, np.int32)
     b = np.zeros(16, np.int32)
-    rc = lib().sp_autotune_shapes(t.ctypes.data, b.ctypes.data, 16, C.byref(n))
+    m = np.zeros(16, np.int32)
+    rc = lib().sp_autotune_shapes(t.ctypes.data, b.ctypes.data, m.ctypes.data, 16, C.byref(n))
     if rc != OK:
         raise SproutError(rc, "autotune_shapes")
-    return [(int(t[k]), int(b[k])) for k in range(min(n.value, 16))]
+    return [(int(t[k]), int(b[k]), int(m[k])) for k in range(min(n.value, 16))]
 
 
 def autotune_report():

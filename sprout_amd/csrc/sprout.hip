@@ -937,12 +937,13 @@ int sp_timings(sp_sim* s, float* out10) {
 
 int sp_autotune(int32_t mode, uint32_t samples) { return tuner().set(mode, samples) ? SP_OK : SP_E_INVALID; }
 
-int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, uint32_t cap, uint32_t* n) {
+int sp_autotune_shapes(int32_t* threads, int32_t* box_ints, int32_t* min_groups, uint32_t cap, uint32_t* n) {
     if (!n) return SP_E_INVALID;
     *n = N_PSHAPES;
     for (uint32_t k = 0; k < cap && k < (uint32_t)N_PSHAPES; k++) {
         if (threads) threads[k] = P_SHAPES[k].threads;
         if (box_ints) box_ints[k] = P_SHAPES[k].box;
+        if (min_groups) min_groups[k] = P_SHAPES[k].min_groups;
     }
     return SP_OK;
 }

@@ -109,8 +109,9 @@ def test_server_and_batch_refuse_bad_arguments():
 
 def test_autotune_modes_and_shapes_without_device():
     shapes = N.autotune_shapes()
-    assert len(shapes) >= 2 and shapes[0] == (256, 12288)
-    assert all(t % 64 == 0 and 0 < b * 4 <= 64 * 1024 for t, b in shapes)
+    assert len(shapes) >= 2 and shapes[0] == (256, 12288, 1)
+    assert len(set(shapes)) == len(shapes)
+    assert all(t % 64 == 0 and 0 < b * 4 <= 64 * 1024 and m >= 1 for t, b, m in shapes)
     L = N.lib()
     assert L.sp_autotune(len(shapes), 0) == N.E_INVALID
     assert L.sp_autotune(-3, 0) == N.E_INVALID
