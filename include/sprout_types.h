@@ -132,6 +132,10 @@ typedef struct sp_sim_params {
     float contact_heat;
     float eddy_heat;
     uint32_t evaporation;
+    /* Rigid bodies of frozen or too viscous liquid (params.rs:198-208): 1 = a body resting on
+     * solids is supported by its contacts, with this Coulomb friction. */
+    uint32_t rigid_contact;
+    float rigid_friction;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -298,6 +302,12 @@ typedef struct sp_levels_info {
 #define SP_LEDGER_SCALE 65536.0f
 #define SP_T_SCALE 4096.0f
 #define SP_LEDGER_WORDS 16
+/* Rigid bodies (rigid.comp, common.glsl:263-275; mod.rs:77-78) */
+#define SP_RIGID_BODIES 4096
+#define SP_BODY_WORDS 32
+#define SP_RIGID_WORDS (16 + SP_BODY_WORDS * SP_RIGID_BODIES)
+#define SP_RIGID_FIX 65536.0f
+#define SP_RIGID_SHARE 0.5f
 
 /* Internal fields (sp_debug_download / sp_debug_upload). Sizes: cells N = W*H. */
 enum {
@@ -349,6 +359,9 @@ enum {
     SP_FIELD_HCELL = 45,     /* int2[N]    heat capacity and heat sums of the particles in the cell */
     SP_FIELD_SOLID_T = 46,   /* float[N]   temperature a solid is held at (-1e9: insulating) */
     SP_FIELD_LEDGER = 47,    /* uint32[16] energy entries 0-7, 64-bit fixed point */
+    SP_FIELD_RL = 48,        /* int2[N]    rigid body: root cell (-1: not rigid), at a root its slot */
+    SP_FIELD_RBODY = 49,     /* uint32[16 + 32*4096] bodies claimed, then per slot 64-bit sums */
+    SP_FIELD_STIFF = 50,     /* float[N]   1 where the liquid is too viscous to flow (rh.w) */
     SP_FIELD_COUNT
 };
 
@@ -374,6 +387,7 @@ enum {
     SP_STAGE_PROJECT_VISC = 17, /* project_visc.comp */
     SP_STAGE_VISC_VCYCLE = 18,  /* one viscosity V-cycle: vr -> vmz level 0 (and vaz) */
     SP_STAGE_HEAT = 19,         /* heat.comp */
+    SP_STAGE_RIGID = 20,        /* rigid.comp; arg = mode 0..4 */
     SP_STAGE_COUNT
 };
 

@@ -3,15 +3,17 @@
  * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
  * times the CPU baseline; nothing in the product calls it.
  *
- * Scope: the isothermal substep without the CAP-gated passes (rigid, rheology, magnets,
- * elasticity, settling, steam, reactions), tools, currents or film spatter:
+ * Scope: the substep with heat and rigid bodies, without the other CAP-gated passes
+ * (rheology, magnets, elasticity, settling, steam, reactions), tools, currents or film
+ * spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
  *   (visc_setup/split/diag/strain/apply/iter/update.comp, vmg_build/down/up.comp,
  *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
- *   pressure solve + project_visc.comp, sort.comp, count.comp, stats.comp.
+ *   pressure solve + project_visc.comp, heat.comp, rigid.comp, sort.comp, count.comp,
+ *   stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
  * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
@@ -61,6 +63,10 @@ struct or_sim {
     int32_t* hcell;   /* 2 per cell */
     float* solid_t;
     uint32_t ledger[SP_LEDGER_WORDS];
+    /* rigid bodies */
+    int32_t* rl;      /* 2 per cell: root cell (-1: not rigid); at a root, its slot */
+    uint32_t* rbody;  /* SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot */
+    float* stiff;     /* 1 where the liquid is too viscous to flow */
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -216,6 +222,81 @@ static float sample_sdf(const or_sim* s, float px, float py, float* gx, float* g
     *gx = mixf(d10 - d00, d11 - d01, fy);
     *gy = mixf(d01 - d00, d11 - d10, fx);
     return mixf(mixf(d00, d10, fx), mixf(d01, d11, fx), fy);
+}
+
+/* ---- rigid bodies (common.glsl:263-340) ----------------------------------------------- */
+
+/* row_lnu (common.glsl:631-636) */
+static float row_lnu(const or_sim* s, int row, float sc, float T, int th) {
+    const sp_gmat* m = &s->mats[row];
+    float l = m->ln_nu + (th ? temp_law(s, row, T) : 0.0f);
+    int d = m->diluent;
+    if (d >= 0) {
+        float bs = m->visc_exp == 1.0f ? sc : powf(sc, m->visc_exp);
+        l = mixf(s->mats[d].ln_nu + (th ? temp_law(s, d, T) : 0.0f), l, bs);
+    }
+    return l;
+}
+
+/* get64 (common.glsl:277-279) */
+static float get64(const or_sim* s, uint32_t w) {
+    return ((float)(int32_t)s->rbody[w + 1] * 4294967296.0f + (float)s->rbody[w]) / SP_RIGID_FIX;
+}
+
+/* body_contact (common.glsl:290-324) */
+static void body_contact(const or_sim* s, uint32_t o, v2 cm, v2* U, float* w) {
+    const float mu = s->P.rigid_friction;
+    const float nc = get64(s, o + 14u);
+    if (mu < 0.0f || nc < 0.5f) return;
+    const float nsx = get64(s, o + 16u), nsy = get64(s, o + 18u);
+    const float nsl = len2(nsx, nsy);
+    if (nsl < 0.25f * nc) return;
+    const float nx = nsx / nsl, ny = nsy / nsl;
+    const float tx = ny, ty = -nx;
+    const float ccx = get64(s, o + 20u) / nc, ccy = get64(s, o + 22u) / nc;
+    const float sqx = get64(s, o + 24u) / nc - ccx * ccx;
+    const float sqy = get64(s, o + 26u) / nc - ccy * ccy;
+    const float sqz = get64(s, o + 28u) / nc - ccx * ccy;
+    const float ext = sqrtf(3.0f * fmaxf_(tx * tx * sqx + ty * ty * sqy + 2.0f * tx * ty * sqz, 0.0f)) + 0.5f;
+    const float rcx = ccx - cm.x, rcy = ccy - cm.y;
+    float ucx = U->x + *w * -rcy, ucy = U->y + *w * rcx;
+    const float vn = ucx * nx + ucy * ny;
+    const float sp = (cm.x - ccx) * tx + (cm.y - ccy) * ty;
+    float vn1 = vn, w1 = *w;
+    if (fabsf(sp) <= ext) {
+        vn1 = fmaxf_(vn, 0.0f);
+        w1 = clampf(*w, -vn1 / ext, vn1 / ext);
+    } else {
+        const float edge = signf(sp) * ext;
+        vn1 = fmaxf_(vn + *w * edge, 0.0f) - *w * edge;
+    }
+    if (vn1 == vn && w1 == *w) return;
+    const float vt = ucx * tx + ucy * ty;
+    const float vt1 = signf(vt) * fmaxf_(fabsf(vt) - mu * (vn1 - vn), 0.0f);
+    ucx += (vn1 - vn) * nx + (vt1 - vt) * tx;
+    ucy += (vn1 - vn) * ny + (vt1 - vt) * ty;
+    *w = w1;
+    U->x = ucx - *w * -rcy;
+    U->y = ucy - *w * rcx;
+}
+
+/* body_motion (common.glsl:326-340): (u, v, angular velocity) of slot b (root r) at (px, py) */
+static int body_motion(const or_sim* s, uint32_t b, int r, float px, float py, float u[3]) {
+    const uint32_t o = 16u + (uint32_t)SP_BODY_WORDS * b;
+    const float M = get64(s, o);
+    if (M <= 1e-3f) return 0;
+    const v2 cm = {get64(s, o + 2u) / M, get64(s, o + 4u) / M};
+    v2 U = {get64(s, o + 6u) / M, get64(s, o + 8u) / M};
+    const float L = get64(s, o + 10u) - M * (cm.x * U.y - cm.y * U.x);
+    const float I = get64(s, o + 12u) - M * (cm.x * cm.x + cm.y * cm.y);
+    float w = I > 1e-3f ? L / I : 0.0f;
+    body_contact(s, o, cm, &U, &w);
+    const float dx = px - ((float)(r % s->W) + 0.5f) - cm.x;
+    const float dy = py - ((float)(r / s->W) + 0.5f) - cm.y;
+    u[0] = U.x + w * -dy;
+    u[1] = U.y + w * dx;
+    u[2] = w;
+    return 1;
 }
 
 /* sample_vel (air.glsl:41-78) */
@@ -429,13 +510,43 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
     int Mgroup = M->group, Mdil = M->diluent, Mcw = M->chan_w, Mcs = M->chan_s;
     int fam = M->fam;
     const int th_on = thermal(s);
+    const int rig = (P->caps & SP_CAP_RIGID) != 0u;
+    /* frozen, or too viscous to flow: part of a rigid body (particles.comp:145-151) */
+    int frozen = 0;
+    if (rig) {
+        v2 ph = {P->heat_x, 0.0f};
+        if (th_on) ph = phase_of(p.t, thermal_of(s, row, p.s));
+        frozen = ph.y > 0.5f || row_lnu(s, row, p.s, ph.x, th_on) >= P->ln_rigid;
+    }
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C[4] = {p.c[0], p.c[1], p.c[2], p.c[3]};
         float corrx = 0.0f, corry = 0.0f, slip = 0.0f, crowd = 0.0f;
         gather_face(s, p.pos[0], p.pos[1], 0, &vx, &corrx, &C[0], &C[1]);
         gather_face(s, p.pos[0], p.pos[1], 1, &vy, &corry, &C[2], &C[3]);
+        /* held by the rigid body around it (particles.comp:223-248) */
         int held = 0;
+        if (rig) {
+            const int px1 = clampi((int)floorf(p.pos[0]), 1, s->W - 2), py1 = clampi((int)floorf(p.pos[1]), 1, s->H - 2);
+            const int in_body = s->rl[2 * cidx(s, px1, py1)] >= 0;
+            if (frozen || in_body) {
+                static const int ox[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1}, oy[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
+                for (int k = 0; k < 9 && !held; k++) {
+                    const int root = s->rl[2 * cidx(s, px1 + ox[k], py1 + oy[k])];
+                    if (root < 0 || s->rl[2 * root + 1] < 0) continue;
+                    float u[3];
+                    if (body_motion(s, (uint32_t)s->rl[2 * root + 1], root, p.pos[0], p.pos[1], u)) {
+                        vx = u[0];
+                        vy = u[1];
+                        C[0] = 0.0f;
+                        C[1] = -u[2];
+                        C[2] = u[2];
+                        C[3] = 0.0f;
+                        held = 1;
+                    }
+                }
+            }
+        }
         /* Stabilisation (particles.comp:281-285). */
         if (P->apic_clamp > 0.0f) {
             float lim = P->apic_clamp / dt;
@@ -639,6 +750,11 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
             }
         }
         (void)Mdil;
+        /* held: the density correction only where the body holds a clump (particles.comp:573-576) */
+        if (held && crowd <= SP_REST_PHI + fmaxf_(P->corr[1], 0.05f)) {
+            corrx = 0.0f;
+            corry = 0.0f;
+        }
         /* Speed cap (particles.comp:831-835). */
         float vcap = P->ext[0];
         {
@@ -937,6 +1053,12 @@ static void stage_grid_prep(or_sim* s) {
                 else if (P->gravity > 0.0f && phi_t > 0.25f && packed(s, i - W)) level = 0.05f;
             }
             if (sol.w > 0.0f && level <= 0.0f) s->region_air |= 1u << (s->region[i] & SP_REGION_MASK);
+            /* stiff where the viscosity would exceed the rigid cap (grid_prep.comp:230-237) */
+            if (P->caps & SP_CAP_RIGID) {
+                const float capv = rho * expf(P->ln_rigid);
+                s->stiff[i] = (mu >= capv && level > 0.0f) ? 1.0f : 0.0f;
+                mu = fminf_(mu, capv);
+            }
             s->cell[i] = (v4){phi_f, rho, mu, level};
             s->comp[i] = (v4){rho, f[1], fam0, phi_t};
             if (any_sorts) s->gbase[i] = (v4){bg[0], bg[1], bg[2], 0.0f};
@@ -2692,6 +2814,141 @@ static void stage_visc_before(or_sim* s) {
     }
 }
 
+/* ---- rigid bodies (rigid.comp) --------------------------------------------------------- */
+
+/* rigid (rigid.comp:34-47, without the rheology pass's unyielded and jammed cells) */
+static int rigid_cell(const or_sim* s, int i) {
+    if (s->cell[i].w <= 0.0f) return 0;
+    if (thermal(s) && s->therm[i].w > SP_RIGID_SHARE) return 1;
+    return s->stiff[i] > 0.5f;
+}
+
+static int rigid_root(const or_sim* s, int i) {
+    while (s->rl[2 * i] != i) i = s->rl[2 * i];
+    return i;
+}
+
+/* round(x) into the 64-bit sum at words w, w + 1 (add64, rigid.comp:92-99) */
+static void rigid_add64(or_sim* s, uint32_t w, float x) {
+    const float r = floorf(x + 0.5f);
+    uint64_t v;
+    memcpy(&v, s->rbody + w, 8);
+    v += (uint64_t)(int64_t)r;
+    memcpy(s->rbody + w, &v, 8);
+}
+
+/* rigid.comp modes 0-4 (rigid.comp:101-196). The union joins 8-connected rigid cells with the
+ * smallest cell index as each body's root (what the reference's atomicMin union-find leaves);
+ * bodies claim slots in the order of their roots (the reference's order of arrival, which
+ * the sums and motions do not depend on). */
+static void stage_rigid(or_sim* s, int mode) {
+    const int W = s->W, H = s->H, N = s->N;
+    if (mode == 0) {
+        for (int i = 0; i < N; i++) {
+            s->rl[2 * i] = rigid_cell(s, i) ? i : -1;
+            s->rl[2 * i + 1] = -1;
+        }
+        s->rbody[0] = 0u;
+        return;
+    }
+    if (mode == 1) {
+        for (int cy = 0; cy < H; cy++)
+            for (int cx = 0; cx < W; cx++) {
+                const int i = cy * W + cx;
+                if (s->rl[2 * i] < 0) continue;
+                const int nb[4] = {cx > 0 ? i - 1 : -1, cy > 0 ? i - W : -1, cy > 0 && cx > 0 ? i - W - 1 : -1,
+                                   cy > 0 && cx < W - 1 ? i - W + 1 : -1};
+                for (int k = 0; k < 4; k++) {
+                    if (nb[k] < 0 || s->rl[2 * nb[k]] < 0) continue;
+                    int a = rigid_root(s, i), b = rigid_root(s, nb[k]);
+                    if (a < b) s->rl[2 * b] = a;
+                    else if (b < a) s->rl[2 * a] = b;
+                }
+            }
+        return;
+    }
+    if (mode == 2) {
+        for (int i = 0; i < N; i++)
+            if (s->rl[2 * i] >= 0) s->rl[2 * i] = rigid_root(s, i);
+        for (int i = 0; i < N; i++) {
+            if (s->rl[2 * i] != i) continue;
+            const uint32_t b = s->rbody[0]++;
+            if (b < (uint32_t)SP_RIGID_BODIES) {
+                memset(s->rbody + 16 + SP_BODY_WORDS * b, 0, 4 * SP_BODY_WORDS);
+                s->rl[2 * i + 1] = (int)b;
+            }
+        }
+        return;
+    }
+    if (mode == 3) {
+        for (int cy = 0; cy < H; cy++)
+            for (int cx = 0; cx < W; cx++) {
+                const int i = cy * W + cx;
+                const int l = s->rl[2 * i];
+                if (l < 0) continue;
+                const int b = s->rl[2 * l + 1];
+                if (b < 0) continue;
+                const v4 me = s->cell[i];
+                const float m = me.y * fminf_(me.x, 1.0f) * SP_RIGID_FIX;
+                const float dx = (float)(cx - l % W), dy = (float)(cy - l / W);
+                const int last = N - 1;
+                const v4 q = s->vel[i];
+                const float ux = 0.5f * (q.x + s->vel[i + 1 < last ? i + 1 : last].x);
+                const float uy = 0.5f * (q.y + s->vel[i + W < last ? i + W : last].y);
+                const uint32_t o = 16u + (uint32_t)SP_BODY_WORDS * (uint32_t)b;
+                rigid_add64(s, o, m);
+                rigid_add64(s, o + 2u, m * dx);
+                rigid_add64(s, o + 4u, m * dy);
+                rigid_add64(s, o + 6u, m * ux);
+                rigid_add64(s, o + 8u, m * uy);
+                rigid_add64(s, o + 10u, m * (dx * uy - dy * ux));
+                rigid_add64(s, o + 12u, m * (dx * dx + dy * dy + 1.0f / 6.0f));
+                if (s->P.rigid_friction >= 0.0f) {
+                    float gx, gy;
+                    const float dist = sample_sdf(s, (float)cx + 0.5f, (float)cy + 0.5f, &gx, &gy);
+                    if (dist < 1.0f && gx * gx + gy * gy > 1e-8f) {
+                        const float gl = len2(gx, gy);
+                        const float nx = gx / gl, ny = gy / gl;
+                        const float F = SP_RIGID_FIX;
+                        rigid_add64(s, o + 14u, F);
+                        rigid_add64(s, o + 16u, F * nx);
+                        rigid_add64(s, o + 18u, F * ny);
+                        rigid_add64(s, o + 20u, F * dx);
+                        rigid_add64(s, o + 22u, F * dy);
+                        rigid_add64(s, o + 24u, F * dx * dx);
+                        rigid_add64(s, o + 26u, F * dy * dy);
+                        rigid_add64(s, o + 28u, F * dx * dy);
+                    }
+                }
+            }
+        return;
+    }
+    /* mode 4: faces between two cells of one body move with it */
+    const float clump = SP_REST_PHI + fmaxf_(s->P.corr[1], 0.05f);
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            const int i = cy * W + cx;
+            const int l = s->rl[2 * i];
+            v4 q = s->vel[i];
+            int changed = 0;
+            float u[3];
+            const int cl = s->cell[i].x > clump;
+            if (l >= 0 && cx > 0 && s->rl[2 * (i - 1)] == l && face_live(q.z) && s->rl[2 * l + 1] >= 0 &&
+                body_motion(s, (uint32_t)s->rl[2 * l + 1], l, (float)cx, (float)cy + 0.5f, u)) {
+                q.x = u[0];
+                if (!cl && s->cell[i - 1].x <= clump) q.z = 0.0f;
+                changed = 1;
+            }
+            if (l >= 0 && cy > 0 && s->rl[2 * (i - W)] == l && face_live(q.w) && s->rl[2 * l + 1] >= 0 &&
+                body_motion(s, (uint32_t)s->rl[2 * l + 1], l, (float)cx + 0.5f, (float)cy, u)) {
+                q.y = u[1];
+                if (!cl && s->cell[i - W].x <= clump) q.w = 0.0f;
+                changed = 1;
+            }
+            if (changed) s->vel[i] = q;
+        }
+}
+
 /* ---- public entry points -------------------------------------------------------------- */
 
 void or_stage(or_sim* s, int stage, int arg) {
@@ -2720,6 +2977,9 @@ void or_stage(or_sim* s, int stage, int arg) {
         break;
     }
     case SP_STAGE_HEAT: stage_heat(s); break;
+    case SP_STAGE_RIGID:
+        if (arg >= 0 && arg <= 4) stage_rigid(s, arg);
+        break;
     case SP_STAGE_RHS: stage_rhs(s); break;
     case SP_STAGE_PRESSURE: stage_pressure(s); break;
     case SP_STAGE_PROJECT: stage_project(s); break;
@@ -2754,6 +3014,8 @@ void or_substep(or_sim* s, int gather) {
         pressure_iterate(s, 1);
         stage_project_visc(s);
     }
+    if (s->P.caps & SP_CAP_RIGID)
+        for (int mode = 0; mode < 5; mode++) stage_rigid(s, mode);
 }
 
 or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
@@ -2796,6 +3058,9 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->dheat = (v4*)calloc(N, sizeof(v4));
     s->hcell = (int32_t*)calloc(2 * N, sizeof(int32_t));
     s->solid_t = (float*)calloc(N, sizeof(float));
+    s->rl = (int32_t*)calloc(2 * (size_t)N, sizeof(int32_t));
+    s->rbody = (uint32_t*)calloc(SP_RIGID_WORDS, sizeof(uint32_t));
+    s->stiff = (float*)calloc(N, sizeof(float));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -2831,6 +3096,9 @@ void or_destroy(or_sim* s) {
     free(s->dheat);
     free(s->hcell);
     free(s->solid_t);
+    free(s->rl);
+    free(s->rbody);
+    free(s->stiff);
     free(s);
 }
 
@@ -2921,6 +3189,9 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_HCELL: *bytes = sizeof(int32_t) * 2 * N; return s->hcell;
     case SP_FIELD_SOLID_T: *bytes = sizeof(float) * N; return s->solid_t;
     case SP_FIELD_LEDGER: *bytes = sizeof(s->ledger); return s->ledger;
+    case SP_FIELD_RL: *bytes = 8 * (size_t)N; return s->rl;
+    case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return s->rbody;
+    case SP_FIELD_STIFF: *bytes = 4 * (size_t)N; return s->stiff;
     default: *bytes = 0; return NULL;
     }
 }

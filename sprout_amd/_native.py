@@ -28,6 +28,8 @@ ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
 
 # Status codes
 OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
+RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
+RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
 
 # Field ids (SP_FIELD_*)
 FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7, MGA=8, MGD=9,
@@ -35,12 +37,12 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47)
+             LEDGER=47, RL=48, RBODY=49, STIFF=50)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18, HEAT=19)
+             VISC_VCYCLE=18, HEAT=19, RIGID=20)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -79,6 +81,7 @@ class SimParams(C.Structure):
                 ("viscosity_mg_sweeps", C.c_uint32), ("mg_precision", C.c_uint32),
                 ("heat_rate", C.c_float), ("air_cooling", C.c_float), ("contact_heat", C.c_float),
                 ("eddy_heat", C.c_float), ("evaporation", C.c_uint32),
+                ("rigid_contact", C.c_uint32), ("rigid_friction", C.c_float),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 

@@ -52,6 +52,10 @@ struct Ctx {
     int2* hcell;          // per cell: heat capacity and heat sums of the particles in it
     float* solid_t;       // temperature a solid is held at (SP_NO_TEMP: insulating)
     uint32_t* ledger;     // SP_LEDGER_WORDS words: energy entries as 64-bit fixed point
+    // rigid bodies (rigid.comp)
+    int2* rl;             // per cell: root cell of its body (-1: not rigid); at a root, its slot
+    uint32_t* rbody;      // SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot
+    float* stiff;         // per cell: 1 where the liquid is too viscous to flow (rh.w)
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -125,6 +129,9 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.hcell = (int2*)take(sizeof(int2) * N);
     c.solid_t = (float*)take(sizeof(float) * N);
     c.ledger = (uint32_t*)take(sizeof(uint32_t) * SP_LEDGER_WORDS);
+    c.rl = (int2*)take(sizeof(int2) * N);
+    c.rbody = (uint32_t*)take(sizeof(uint32_t) * SP_RIGID_WORDS);
+    c.stiff = (float*)take(sizeof(float) * N);
     if (bytes) *bytes = o;
     return c;
 }
@@ -139,6 +146,86 @@ SP_DEV float signf_(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f
 SP_DEV bool face_live(float s) { return fabsf(s) < 1e29f; }
 SP_DEV bool face_closed(float s) { return s < -1e29f; }
 SP_DEV float f4c(const float4& a, int k) { return k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w; }
+
+// Bilinear SDF and its gradient (air.glsl:13-22; rigid.comp:50-61).
+SP_DEV float sample_sdf(const Ctx& c, float px, float py, float& gx, float& gy) {
+    float x = clampf_(px, 0.0f, (float)c.W - 0.001f), y = clampf_(py, 0.0f, (float)c.H - 0.001f);
+    int ix = (int)x, iy = (int)y;
+    float fx = x - (float)ix, fy = y - (float)iy;
+    int w1 = c.W + 1;
+    int k = iy * w1 + ix;
+    float d00 = c.sdf[k], d10 = c.sdf[k + 1], d01 = c.sdf[k + w1], d11 = c.sdf[k + w1 + 1];
+    gx = mixf_(d10 - d00, d11 - d01, fy);
+    gy = mixf_(d01 - d00, d11 - d10, fx);
+    return mixf_(mixf_(d00, d10, fx), mixf_(d01, d11, fx), fy);
+}
+
+// ---- rigid bodies (common.glsl:263-340) ---------------------------------------------------
+
+
+// The 64-bit fixed-point sum at words w (low), w + 1 (high) of the body table, as a float
+// (get64, common.glsl:277-279).
+SP_DEV float get64(const Ctx& c, uint32_t w) {
+    return ((float)(int32_t)c.rbody[w + 1] * 4294967296.0f + (float)c.rbody[w]) / SP_RIGID_FIX;
+}
+
+// The body's support on the solids it touches (body_contact, common.glsl:290-324): motion into
+// the support is stopped, turning limited to what keeps its ends out of the solid (or tipping
+// about its edge where the centre of mass overhangs), and Coulomb friction along it bounded
+// by the normal speed taken away. Positions relative to the root cell.
+SP_DEV void body_contact(const Ctx& c, uint32_t o, float2 cm, float2& U, float& w) {
+    const float mu = c.P->rigid_friction;
+    const float nc = get64(c, o + 14u);
+    if (mu < 0.0f || nc < 0.5f) return;
+    const float nsx = get64(c, o + 16u), nsy = get64(c, o + 18u);
+    const float nsl = len2_(nsx, nsy);
+    if (nsl < 0.25f * nc) return;  // contacts on opposite sides: wedged, left to the solve
+    const float nx = nsx / nsl, ny = nsy / nsl;
+    const float tx = ny, ty = -nx;
+    const float ccx = get64(c, o + 20u) / nc, ccy = get64(c, o + 22u) / nc;
+    const float sqx = get64(c, o + 24u) / nc - ccx * ccx;
+    const float sqy = get64(c, o + 26u) / nc - ccy * ccy;
+    const float sqz = get64(c, o + 28u) / nc - ccx * ccy;
+    const float ext = sqrtf(3.0f * fmaxf_(tx * tx * sqx + ty * ty * sqy + 2.0f * tx * ty * sqz, 0.0f)) + 0.5f;
+    const float rcx = ccx - cm.x, rcy = ccy - cm.y;
+    float ucx = U.x + w * -rcy, ucy = U.y + w * rcx;
+    const float vn = ucx * nx + ucy * ny;
+    const float s = (cm.x - ccx) * tx + (cm.y - ccy) * ty;
+    float vn1 = vn, w1 = w;
+    if (fabsf(s) <= ext) {
+        vn1 = fmaxf_(vn, 0.0f);
+        w1 = clampf_(w, -vn1 / ext, vn1 / ext);
+    } else {
+        const float edge = signf_(s) * ext;
+        vn1 = fmaxf_(vn + w * edge, 0.0f) - w * edge;
+    }
+    if (vn1 == vn && w1 == w) return;
+    const float vt = ucx * tx + ucy * ty;
+    const float vt1 = signf_(vt) * fmaxf_(fabsf(vt) - mu * (vn1 - vn), 0.0f);
+    ucx += (vn1 - vn) * nx + (vt1 - vt) * tx;
+    ucy += (vn1 - vn) * ny + (vt1 - vt) * ty;
+    w = w1;
+    U.x = ucx - w * -rcy;
+    U.y = ucy - w * rcx;
+}
+
+// Rigid motion of body slot b (root cell r) at x (cells): (u, v, angular velocity). False if
+// the body is too light to have one (body_motion, common.glsl:326-340).
+SP_DEV bool body_motion(const Ctx& c, uint32_t b, int r, float px, float py, float3& u) {
+    const uint32_t o = 16u + (uint32_t)SP_BODY_WORDS * b;
+    const float M = get64(c, o);
+    if (M <= 1e-3f) return false;
+    const float2 cm = make_float2(get64(c, o + 2u) / M, get64(c, o + 4u) / M);
+    float2 U = make_float2(get64(c, o + 6u) / M, get64(c, o + 8u) / M);
+    const float L = get64(c, o + 10u) - M * (cm.x * U.y - cm.y * U.x);
+    const float I = get64(c, o + 12u) - M * (cm.x * cm.x + cm.y * cm.y);
+    float w = I > 1e-3f ? L / I : 0.0f;
+    body_contact(c, o, cm, U, w);
+    const float dx = px - ((float)(r % c.W) + 0.5f) - cm.x;
+    const float dy = py - ((float)(r / c.W) + 0.5f) - cm.y;
+    u = make_float3(U.x + w * -dy, U.y + w * dx, w);
+    return true;
+}
 SP_DEV float smoothstep_(float e0, float e1, float x) {
     float t = clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
@@ -207,6 +294,18 @@ SP_DEV float2 phase_of(float te, float4 m) {
     if (te <= lo) return make_float2(te + m.z, 1.0f);
     const float fr = (te - lo) / (m.y - lo);
     return make_float2(m.x + fr * (m.y - m.x), 1.0f - fr);
+}
+// ln kinematic viscosity (cells^2/s) of material `row` at concentration s and temperature T
+// (row_lnu, common.glsl:631-636; an isothermal tank's table has the temperature folded in).
+SP_DEV float row_lnu(const Ctx& c, int row, float s, float T, bool thermal) {
+    const sp_gmat& m = c.mats[row];
+    float l = m.ln_nu + (thermal ? temp_law(c, row, T) : 0.0f);
+    const int d = m.diluent;
+    if (d >= 0) {
+        const float bs = m.visc_exp == 1.0f ? s : powf(s, m.visc_exp);
+        l = mixf_(c.mats[d].ln_nu + (thermal ? temp_law(c, d, T) : 0.0f), l, bs);
+    }
+    return l;
 }
 
 // A particle's heat capacity and heat in fixed point (heat_quanta, particles.comp:70-73).

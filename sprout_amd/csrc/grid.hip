@@ -254,6 +254,13 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
         uint32_t bit = 1u << (c.region[i] & SP_REGION_MASK);
         if ((*c.region_air & bit) == 0u) atomicOr(c.region_air, bit);
     }
+    // the viscosity solve stops at rigid_viscosity: a cell that would need more is stiff and
+    // moves rigidly (grid_prep.comp:230-237)
+    if (P.caps & SP_CAP_RIGID) {
+        const float cap = rho * expf(P.ln_rigid);
+        c.stiff[i] = (mu >= cap && level > 0.0f) ? 1.0f : 0.0f;
+        mu = fminf_(mu, cap);
+    }
     c.cell[i] = make_float4(phi_f, rho, mu, level);
     c.comp[i] = make_float4(rho, f[1], fam0, phi_t);
     uint32_t gm = P.group_mask;

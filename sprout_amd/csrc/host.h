@@ -30,9 +30,9 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
-// The range of temperatures a thermal tank of `rows` can be carried through without
-// freezing, boiling or a liquid turning rigid (the passes for those are not carried):
-// (lowest allowed, highest allowed), deg C.
+// The range of temperatures a thermal tank of `rows` can be carried through: below the
+// lowest boiling point (the steam and boiling passes are not carried); freezing and liquid
+// turning rigid are (rigid bodies). (lowest allowed, highest allowed), deg C.
 std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows);
 void settle_spawn(const sp_sim_params& p, sp_particle& q);
 

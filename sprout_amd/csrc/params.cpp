@@ -97,6 +97,8 @@ void default_params(sp_sim_params* p) {
     p->contact_heat = 3000.0f;
     p->eddy_heat = 0.17f;
     p->evaporation = 1;
+    p->rigid_contact = 1;  // params.rs:400-402
+    p->rigid_friction = 0.6f;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -223,23 +225,9 @@ float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool therma
 }
 
 std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows) {
-    float lo = -273.15f, hi = INFINITY;
-    for (int r : rows) {
-        const sp_material& m = p.mat[r];
-        if (freezes(m)) lo = std::max(lo, m.melt_hi);
-        hi = std::min(hi, m.boil);
-        // the temperature where the liquid reaches rigid_viscosity (the law falls with T)
-        if (m.temp_model != 0 && viscosity_at(m, lo) >= p.rigid_viscosity) {
-            float a = lo, b = std::min(hi, 1000.0f);
-            for (int k = 0; k < 60; k++) {
-                float mid = 0.5f * (a + b);
-                if (viscosity_at(m, mid) >= p.rigid_viscosity) a = mid;
-                else b = mid;
-            }
-            lo = std::max(lo, b);
-        }
-    }
-    return {lo, hi};
+    float hi = INFINITY;
+    for (int r : rows) hi = std::min(hi, p.mat[r].boil);
+    return {-273.15f, hi};
 }
 
 // FluidSim::viscosity_mode (mod.rs:2768-2792). A thermal tank counts a liquid that can
@@ -320,22 +308,24 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
     return true;
 }
 
-// FluidSim::caps (mod.rs:2717-2766); returns false for a tank that needs a CAP-gated pass
-// this build does not carry. A thermal tank is carried without CAP_RIGID: the reference
-// sets it for every liquid that could freeze or has a temperature law, but its passes only
-// act on frozen or rigid liquid, which the temperature range a thermal tank is held to
-// (thermal_range) excludes.
-static bool tank_caps(const sp_sim_params& p, const Tank& t, uint32_t& caps) {
-    caps = 0;
+// FluidSim::caps (mod.rs:2717-2766) for the CAP-gated passes this build carries: families,
+// heat and rigid bodies. A thermal tank turns rigid bodies on for any liquid that can freeze
+// or has a temperature law; any tank for a liquid already as viscous as rigid_viscosity, or
+// frozen at the ambient temperature.
+static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
+    uint32_t caps = 0;
     if (t.families > 0) caps |= SP_CAP_FAMILIES;
     if (t.thermal) caps |= SP_CAP_THERMAL;
     bool rigid = false;
     for (int r : t.rows) {
         const sp_material& m = p.mat[r];
+        const sp_material* dil = m.diluent >= 0 ? &p.mat[m.diluent] : nullptr;
+        if (t.thermal)
+            rigid |= freezes(m) || (dil && freezes(*dil)) || m.temp_model != 0 || (dil && dil->temp_model != 0);
         rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (!t.thermal && freezes(m) && p.ambient_temp < m.melt_hi);
     }
     if (rigid) caps |= SP_CAP_RIGID;
-    return !rigid;
+    return caps;
 }
 
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why) {
@@ -404,11 +394,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.temp[2] = m.temp_ref + 273.15f;
         o.temp[3] = 0.0f;
     }
-    uint32_t caps = 0;
-    if (!tank_caps(p, t, caps)) {
-        if (why) *why = "the tank needs rigid bodies (a liquid that freezes or exceeds rigid_viscosity), which this build does not carry";
-        return SP_E_UNSUPPORTED;
-    }
+    const uint32_t caps = tank_caps(p, t);
     // gpu_params (mod.rs:2238-2330).
     g.W = W;
     g.H = H;
@@ -423,7 +409,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.kappa_max = p.curvature_max;
     g.sealed_vent = (float)p.sealed_vent;
     g.cell_size = p.cell_size;
-    g.rigid_friction = -1.0f;
+    g.rigid_friction = p.rigid_contact != 0 ? p.rigid_friction : -1.0f;  // misc.w (mod.rs:2280)
     g.n_present = (int)t.rows.size();
     g.acc_stride = std::max(t.stride, SP_ACC_MIN);
     g.n_fam = t.families;

@@ -35,19 +35,6 @@ SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d
     return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
 }
 
-// Bilinear SDF and its gradient (air.glsl:13-22).
-SP_DEV float sample_sdf(const Ctx& c, float px, float py, float& gx, float& gy) {
-    float x = clampf_(px, 0.0f, (float)c.W - 0.001f), y = clampf_(py, 0.0f, (float)c.H - 0.001f);
-    int ix = (int)x, iy = (int)y;
-    float fx = x - (float)ix, fy = y - (float)iy;
-    int w1 = c.W + 1;
-    int k = iy * w1 + ix;
-    float d00 = c.sdf[k], d10 = c.sdf[k + 1], d01 = c.sdf[k + w1], d11 = c.sdf[k + w1 + 1];
-    gx = mixf_(d10 - d00, d11 - d01, fy);
-    gy = mixf_(d01 - d00, d11 - d10, fx);
-    return mixf_(mixf_(d00, d10, fx), mixf_(d01, d11, fx), fy);
-}
-
 // Bilinear velocity over live faces, `fb` where no face is live (air.glsl:41-78).
 SP_DEV float2 sample_vel(const Ctx& c, float px, float py, float2 fb) {
     float2 o = fb;
@@ -165,12 +152,45 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     int Mgroup = c.mats[row].group, Mcw = c.mats[row].chan_w, Mcs = c.mats[row].chan_s;
     int fam = c.mats[row].fam;
     const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
+    const bool rig = (P.caps & SP_CAP_RIGID) != 0u;
+    // frozen particles, and those too viscous to flow, belong to a rigid body
+    // (particles.comp:145-151)
+    bool frozen = false;
+    if (rig) {
+        const float2 ph = th_on ? phase_of(p.t, thermal_of(c, row, p.s)) : make_float2(P.heat_x, 0.0f);
+        frozen = ph.y > 0.5f || row_lnu(c, row, p.s, ph.x, th_on) >= P.ln_rigid;
+    }
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C0 = p.c[0], C1 = p.c[1], C2 = p.c[2], C3 = p.c[3];
         float corrx = 0.0f, corry = 0.0f, slip = 0.0f, crowd = 0.0f;
         gather_face(c, p.pos[0], p.pos[1], 0, vx, corrx, C0, C1);
         gather_face(c, p.pos[0], p.pos[1], 1, vy, corry, C2, C3);
+        // a frozen particle, or any particle in a cell of a body, moves exactly with the rigid
+        // body around it: the first body in the 3 x 3 cells around its own, own cell first
+        // (particles.comp:223-248; the yield-stress plug of the rheology pass is not carried)
+        bool held = false;
+        if (rig) {
+            const int px1 = clampi_(floori(p.pos[0]), 1, c.W - 2), py1 = clampi_(floori(p.pos[1]), 1, c.H - 2);
+            const bool in_body = c.rl[cidx(c, px1, py1)].x >= 0;
+            if (frozen || in_body) {
+                const int ox[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1}, oy[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
+                for (int k = 0; k < 9 && !held; k++) {
+                    const int root = c.rl[cidx(c, px1 + ox[k], py1 + oy[k])].x;
+                    if (root < 0 || c.rl[root].y < 0) continue;
+                    float3 u;
+                    if (body_motion(c, (uint32_t)c.rl[root].y, root, p.pos[0], p.pos[1], u)) {
+                        vx = u.x;
+                        vy = u.y;
+                        C0 = 0.0f;
+                        C1 = -u.z;
+                        C2 = u.z;
+                        C3 = 0.0f;
+                        held = true;
+                    }
+                }
+            }
+        }
         if (P.apic_clamp > 0.0f) {
             float lim = P.apic_clamp / dt;
             C0 = clampf_(C0, -lim, lim);
@@ -250,7 +270,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                 int gg = Mgroup;
                 float fg = gg == 0 ? 1.0f - sgx - sgy - sgz : sgv[gg - 1];
                 float others = 1.0f - fg;
-                if (others > 1e-4f && groups) {
+                if (others > 1e-4f && groups && !held) {
                     float sgr[4] = {0, 0, 0, 0}, snu[4] = {0, 0, 0, 0}, snw[4] = {0, 0, 0, 0};
                     for (int b = 0; b < 3; b++)
                         for (int a = 0; a < 3; a++) {
@@ -312,7 +332,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                         slip *= 1.0f - hold * resolved;
                     }
                 }
-                if (sorts) {
+                if (sorts && !held) {
                     float fgi = groups ? sgv[Mgroup - 1] : 1.0f;
                     if (fgi > 1e-4f) {
                         float rmean = sbg / sw / fgi;
@@ -370,6 +390,12 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                 }
             }
         }
+        // a held particle takes the density correction only where its body holds a clump
+        // (particles.comp:573-576)
+        if (held && crowd <= SP_REST_PHI + fmaxf_(P.corr[1], 0.05f)) {
+            corrx = 0.0f;
+            corry = 0.0f;
+        }
         float vcap = P.ext[0];
         {
             float sp = len2_(vx, vy);
@@ -380,7 +406,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
             }
         }
         float vax = vx, vay = vy;
-        if (P.ext[1] >= 2.0f) {
+        if (P.ext[1] >= 2.0f && !held) {
             float h = 0.5f * dt;
             float ex = (vx + corrx) + 0.0f, ey = (vy + corry) + slip;
             float xmx = clampf_(p.pos[0] + h * ex, 2.01f, (float)c.W - 2.01f);

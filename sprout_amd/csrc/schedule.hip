@@ -220,13 +220,22 @@ void launch_visc_after(const Run& r, const Recs& recs) {
     if (twice_projected(r.gp)) launch_visc_solve(r, recs.v, true);
 }
 
-// ... and the second pressure solve with its projection (mod.rs:2692-2697).
+// The rigid bodies' five passes after the projections (mod.rs:2701-2707).
+void launch_rigid(const Run& r) {
+    if (!(r.gp.caps & SP_CAP_RIGID)) return;
+    for (int mode = 0; mode < 5; mode++) hipLaunchKernelGGL(k_rigid, grid0(r), TILE, 0, r.stream, r.tb, mode);
+}
+
+// ... and the second pressure solve with its projection (mod.rs:2692-2697), then the rigid
+// bodies.
 void launch_second_projection(const Run& r, const Recs& recs) {
-    if (!twice_projected(r.gp)) return;
-    const dim3 g0 = grid0(r);
-    hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, r.stream, r.tb);
-    launch_pressure(r, recs.p2, true);
-    hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, r.stream, r.tb);
+    if (twice_projected(r.gp)) {
+        const dim3 g0 = grid0(r);
+        hipLaunchKernelGGL(k_rhs_visc, g0, TILE, 0, r.stream, r.tb);
+        launch_pressure(r, recs.p2, true);
+        hipLaunchKernelGGL(k_project_visc, g0, TILE, 0, r.stream, r.tb);
+    }
+    launch_rigid(r);
 }
 
 // ---- the other passes ------------------------------------------------------------------
@@ -382,7 +391,7 @@ int launch_particles(const Run& r, int gather) {
 uint32_t schedule_key(const sp_gparams& g) {
     return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
            (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12 |
-           (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13;
+           (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14;
 }
 
 }  // namespace

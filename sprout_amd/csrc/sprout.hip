@@ -23,6 +23,7 @@
 #include "solve.hip"
 #include "visc.hip"
 #include "heat.hip"
+#include "rigid.hip"
 #include "schedule.hip"
 
 namespace {
@@ -144,16 +145,15 @@ void launch_sort(sp_sim* s, int tiled, hipStream_t stream) {
 int upload_tables(sp_sim* s) {
     if (!s->tables_dirty) return SP_OK;
     if (s->thermal) {
-        // A thermal tank is carried between its liquids' freezing and boiling points (and
-        // below where a temperature law would make one rigid): the rigid-body, steam and
-        // boiling passes that act outside that range are not carried. Conduction, air and
-        // contact exchange never leave the range of the temperatures put in.
+        // A thermal tank is carried below its liquids' lowest boiling point: the steam and
+        // boiling passes are not carried. Conduction, air and contact exchange never leave
+        // the range of the temperatures put in.
         const std::array<float, 2> r = sp::thermal_range(s->params, s->tank.rows);
         const float lo = std::min(std::min(s->t_lo, s->st_lo), s->params.ambient_temp);
         const float hi = std::max(std::max(s->t_hi, s->st_hi), s->params.ambient_temp);
         if (!(lo > r[0] && hi < r[1])) {
-            set_err(s, "thermal tank holds temperatures %.2f .. %.2f C outside %.2f .. %.2f C, where its liquids would "
-                       "freeze, boil or turn rigid (rigid bodies, steam and boiling are not carried)",
+            set_err(s, "thermal tank holds temperatures %.2f .. %.2f C outside %.2f .. %.2f C: its liquids would boil "
+                       "(steam and boiling are not carried)",
                     lo, hi, r[0], r[1]);
             return SP_E_UNSUPPORTED;
         }
@@ -1028,6 +1028,9 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_HCELL: *bytes = 8 * N; return c.hcell;
     case SP_FIELD_SOLID_T: *bytes = 4 * N; return c.solid_t;
     case SP_FIELD_LEDGER: *bytes = 4 * SP_LEDGER_WORDS; return c.ledger;
+    case SP_FIELD_RL: *bytes = 8 * N; return c.rl;
+    case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return c.rbody;
+    case SP_FIELD_STIFF: *bytes = 4 * N; return c.stiff;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1126,6 +1129,10 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         }
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_HEAT: hipLaunchKernelGGL(k_heat, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_RIGID:
+            if (arg < 0 || arg > 4) return SP_E_INVALID;
+            hipLaunchKernelGGL(k_rigid, g0, TILE, 0, s->stream, cs, arg);
+            break;
         case SP_STAGE_PRESSURE: launch_pressure(r, full.p); break;
         case SP_STAGE_PROJECT: launch_project(r); break;
         case SP_STAGE_SUBSTEP:

@@ -138,6 +138,40 @@ def thermal_pool(grid, frac=(1.0, 0.6), temp=None, row=N.ROW_WATER, hot=None, ce
     return params, field, parts, [row]
 
 
+def ice_block(grid, block=(0.35, 0.65, 0.35, 0.55), ice_temp=-10.0, water_temp=20.0, pool=0.5, cell_size=0.005,
+              heat_rate=1.0, pressure_tol=1e-6):
+    """A block of ice floating in water (a thermal tank; water can freeze, so it has rigid
+    bodies): water at `water_temp` up to `pool` x H, and a block of frozen water at `ice_temp`
+    (fractions of W and H: x0, x1, y0, y1) partly above it. An ice particle carries the
+    enthalpy temperature T - L / c (phase_of). (params, solid field, particles, rows)."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.heat_rate = heat_rate
+    W, H = grid
+    m = params.mat[N.ROW_WATER]
+    scene = sc.Scene(grid)
+    x0, x1, y0, y1 = block[0] * W, block[1] * W, block[2] * H, block[3] * H
+    water = sc.fill_rect(scene, (0.0, 0.0, float(W), pool * H), N.ROW_WATER)
+    inside = (water["pos"][:, 0] >= x0) & (water["pos"][:, 0] < x1) & (water["pos"][:, 1] >= y0)
+    water = water[~inside]
+    ice = sc.fill_rect(scene, (x0, y0, x1, y1), N.ROW_WATER, len(water))
+    water["t"] = np.float32(water_temp)
+    ice["t"] = np.float32(ice_temp - m.latent / m.heat_capacity)
+    parts = settle(np.concatenate([water, ice]), params)
+    return params, scene.rasterize(), parts, [N.ROW_WATER]
+
+
+def stiff_block(grid, block=(0.3, 0.6, 0.4, 0.7), viscosity=5.0, cell_size=0.005, pressure_tol=1e-6):
+    """A block of a liquid more viscous than rigid_viscosity (honey at `viscosity` m^2/s, an
+    isothermal tank) falling in an empty tank: every cell it fills is stiff and the block
+    moves as one rigid body. (params, solid field, particles, rows)."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.mat[N.ROW_HONEY].viscosity = viscosity
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (block[0] * W, block[2] * H, block[1] * W, block[3] * H), N.ROW_HONEY)
+    return params, scene.rasterize(), settle(parts, params), [N.ROW_HONEY]
+
+
 def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), strip_temp=76.85):
     """The two-fluid tank with heat: water below (density 1000, 1e-6 m^2/s), oil above
     (900, 1e-4), a honey-like blob (1400, 0.09) near the top centre, everything at 20 deg C

@@ -86,6 +86,12 @@ class Pair:
                 self.gpu.put_field("HCELL", self.ora.field("HCELL", np.int32, 2 * self.N))
             elif n == "LEDGER":
                 self.gpu.put_field("LEDGER", self.ora.field("LEDGER", np.uint32, 16))
+            elif n == "RL":
+                self.gpu.put_field("RL", self.ora.field("RL", np.int32, 2 * self.N))
+            elif n == "RBODY":
+                self.gpu.put_field("RBODY", self.ora.field("RBODY", np.uint32, N.RIGID_WORDS))
+            elif n == "STIFF":
+                self.gpu.put_field("STIFF", self.ora.field("STIFF", np.float32, self.N))
             elif n in ("PTILE", "VTILE"):
                 self.gpu.put_field(n, self.ora.field(n, np.uint32, self.L.tiles))
             elif n == "VTILES":
