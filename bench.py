@@ -15,6 +15,10 @@ Workloads (config.workload):
   reductions.
 * dam_break_1024x576_fp32 (--workload dam): one block of water of the same size, explicit
   viscosity, one pressure solve.
+* two_fluid_heat_512x288_fp32 (--workload two_fluid, --grid 512x288): water below, oil above,
+  a honey-like blob dropped from the top centre, 1 cm cells, everything at 20 deg C on a floor
+  strip held at 76.85 deg C (350 K): heat conduction, the implicit viscosity with the second
+  pressure solve, and the rigid-body passes a thermal water tank runs.
 * mixed_64_tanks_bf16mg (--workload mixed): 64 independent dam-break tanks advanced as one
   batch, grids drawn from 128x72, 256x144, 384x216 and 512x288 by seed, each block filling
   30-60 % of its tank, water or oil, pressure multigrid levels in BF16/FP16; tanks of one
@@ -66,6 +70,8 @@ def workload(kind, grid, seed=0, mg_precision=0):
     import cases
     if kind == "four":
         out = cases.four_liquids(grid, seed=seed, viscosity_tol=1e-5)
+    elif kind == "two_fluid":
+        out = cases.two_fluid_heat(grid, seed=seed)
     else:
         out = cases.dam_break(grid, frac=(0.8, 0.8))
     out[0].mg_precision = mg_precision
@@ -150,14 +156,14 @@ def served(rates, seconds, n_scenes=64, seed=0, streams=4):
             "request": "1 substep (1/240 s) of one scene", "budget": "every scene resident", "by_rate": out}
 
 
-def cpu_baseline(grid, params, field, parts, rows, substeps):
+def cpu_baseline(grid, tables, field, parts, substeps):
     """The oracle on one host core: `substeps` substeps of the same tank from the same state,
-    after one untimed substep (the first after a reset, without a gather)."""
-    from sprout_amd import _native as N
+    after one untimed substep (the first after a reset, without a gather). `tables`: the
+    grid-unit tables the device tank derived (sp_sim grid_params)."""
     from oracle.binding import OracleSim
 
     W, H = grid
-    g, mats = N.derive_grid_params(params, W, H, rows)
+    g, mats = tables
     g.n_slots = len(parts)
     o = OracleSim(grid, W * H * 4)
     o.set_gparams(g, mats)
@@ -165,6 +171,12 @@ def cpu_baseline(grid, params, field, parts, rows, substeps):
     o.reset(parts)
     o.begin_batch(1 / 240, 1)
     o.substep(0)
+    if substeps <= 0:
+        # about ten seconds of work: one substep measured first
+        o.begin_batch(1 / 240, 1)
+        t0 = time.perf_counter()
+        o.substep(1)
+        substeps = int(min(max(round(10.0 / max(time.perf_counter() - t0, 1e-3)), 4), 400))
     o.begin_batch(substeps / 240, substeps)
     t0 = time.perf_counter()
     for _ in range(substeps):
@@ -175,8 +187,8 @@ def cpu_baseline(grid, params, field, parts, rows, substeps):
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{substeps} substeps of the same {W}x{H} tank ({len(parts)} particles) after one warm-up "
-                  f"substep, single-threaded C oracle, {dt:.1f} s",
+        "sample": f"{substeps} substeps of the same {W}x{H} tank ({len(parts)} particles) after the warm-up "
+                  f"substeps, single-threaded C oracle, {dt:.1f} s",
     }
 
 
@@ -187,9 +199,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--grid", default="1024x576")
-    ap.add_argument("--workload", choices=["four", "dam", "mixed"], default="four")
+    ap.add_argument("--workload", choices=["four", "dam", "mixed", "two_fluid"], default="four")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=8)
+    ap.add_argument("--cpu-substeps", type=int, default=0,
+                    help="substeps of the CPU baseline sample; 0: as many as take about ten seconds")
     ap.add_argument("--tanks", type=int, default=1,
                     help="tanks advanced together in one batch (seeds 0 .. tanks-1; value is their aggregate)")
     ap.add_argument("--latency-steps", type=int, default=64,
@@ -289,7 +302,7 @@ def main():
         alg_bytes = group_tanks * (96.0 * len(parts) + 100.0 * W * H)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     live = sum(t[0].stats().live for t in tanks)
-    name = {"four": "four_liquids", "dam": "dam_break", "mixed": "mixed"}[args.workload]
+    name = {"four": "four_liquids", "dam": "dam_break", "mixed": "mixed", "two_fluid": "two_fluid_heat"}[args.workload]
     if args.workload == "mixed":
         wname = f"mixed_{len(tanks)}_tanks_bf16mg"
         sizes = sorted({t[0].grid for t in tanks})
@@ -303,7 +316,7 @@ def main():
         "grid": grid_desc,
         "particles": n_p,
         "live_particles_end": int(live),
-        "cell_size_m": 0.005,
+        "cell_size_m": float(params.cell_size),
         "dt": "1/240 s fixed",
         "pressure_tol": float(params.pressure_tol),
         "substeps_per_batch": args.batch,
@@ -379,6 +392,7 @@ def main():
                     "shape": shapes[e["best"]] if e["best"] >= 0 else None, "best_ms": e["best_ms"],
                     "default_ms": e["default_ms"]} for e in N.autotune_report()],
     }
+    tables0 = tanks[0][0].grid_params()
     if batch is not None:
         out["launch_groups"] = batch.groups()
         batch.close()
@@ -387,7 +401,7 @@ def main():
     if args.serve_rates:
         out["served"] = served([float(r) for r in args.serve_rates.split(",")], args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(tanks[0][0].grid, params, field, parts, rows, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline(tanks[0][0].grid, tables0, field, parts, args.cpu_substeps)
     print(json.dumps(out))
 
 
