@@ -79,6 +79,13 @@ typedef struct sp_material {
     float boil;            /* deg C (1e9: never boils) */
     float latent_vap;      /* J/kg */
     float freeze_shift;    /* K per unit concentration (a concentrate freezes like its diluent, shifted) */
+    /* Rheology (material.rs:146-182, 239-240, 311-314): model (0 Newtonian, 1 Carreau,
+     * 2 Herschel-Bulkley, 3 shear thickening), yield stress (Pa), flow index n, shear time
+     * lambda (s), viscosity ratio (Carreau: infinite-shear / zero-shear; thickening: the
+     * cap); a surface skin (N/m) and the internal friction of a granular liquid. */
+    uint32_t rheology;
+    float yield_stress, flow_index, shear_time, visc_ratio;
+    float skin, friction;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -179,7 +186,9 @@ typedef struct sp_gmat {
     float visc_exp, expansion, load, freeze_shift;  /* misc */
     float th[4];     /* volumetric heat capacity / water's; conductivity (cells^2/s as water's); solidus, liquidus (deg C) */
     float th2[4];    /* latent heat as degrees (L / c); emissivity; boiling point (deg C); latent heat of vaporisation as degrees */
-    float temp[4];   /* temperature law: B, C (K); reference temperature (K); 0 */
+    float temp[4];   /* temperature law: B, C (K); reference temperature (K); skin (cells^3/s^2) */
+    float rheo_p[4]; /* yield stress (cells^2/s^2), flow index, shear time (s), viscosity ratio */
+    float gr[4];     /* internal friction, 0, 0, 0 */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -297,6 +306,7 @@ typedef struct sp_levels_info {
 #define SP_CAP_FAMILIES 1u
 #define SP_CAP_THERMAL 2u
 #define SP_CAP_RIGID 4u
+#define SP_CAP_RHEO 8u
 /* Heat (common.glsl:431-436) */
 #define SP_NO_TEMP (-1e9f)
 #define SP_LEDGER_SCALE 65536.0f
@@ -361,7 +371,7 @@ enum {
     SP_FIELD_LEDGER = 47,    /* uint32[16] energy entries 0-7, 64-bit fixed point */
     SP_FIELD_RL = 48,        /* int2[N]    rigid body: root cell (-1: not rigid), at a root its slot */
     SP_FIELD_RBODY = 49,     /* uint32[16 + 32*4096] bodies claimed, then per slot 64-bit sums */
-    SP_FIELD_STIFF = 50,     /* float[N]   1 where the liquid is too viscous to flow (rh.w) */
+    SP_FIELD_RH = 50,        /* float4[N]  non-Newtonian share, its row, its concentration; 1 where stiff */
     SP_FIELD_COUNT
 };
 
@@ -388,6 +398,7 @@ enum {
     SP_STAGE_VISC_VCYCLE = 18,  /* one viscosity V-cycle: vr -> vmz level 0 (and vaz) */
     SP_STAGE_HEAT = 19,         /* heat.comp */
     SP_STAGE_RIGID = 20,        /* rigid.comp; arg = mode 0..4 */
+    SP_STAGE_RHEO = 21,         /* rheo.comp */
     SP_STAGE_COUNT
 };
 
@@ -403,7 +414,7 @@ enum {
 #endif
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
-SP_STATIC_ASSERT(sizeof(sp_gmat) == 128, "one grid-unit material row is 32 words");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 160, "one grid-unit material row is 40 words");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 
 #endif

@@ -3,17 +3,16 @@
  * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
  * times the CPU baseline; nothing in the product calls it.
  *
- * Scope: the substep with heat and rigid bodies, without the other CAP-gated passes
- * (rheology, magnets, elasticity, settling, steam, reactions), tools, currents or film
- * spatter:
+ * Scope: the substep with heat, rigid bodies and rheology, without the other CAP-gated
+ * passes (magnets, elasticity, settling, steam, reactions), tools, currents or film spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
  *   (visc_setup/split/diag/strain/apply/iter/update.comp, vmg_build/down/up.comp,
  *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
- *   pressure solve + project_visc.comp, heat.comp, rigid.comp, sort.comp, count.comp,
- *   stats.comp.
+ *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, sort.comp,
+ *   count.comp, stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
  * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
@@ -66,7 +65,7 @@ struct or_sim {
     /* rigid bodies */
     int32_t* rl;      /* 2 per cell: root cell (-1: not rigid); at a root, its slot */
     uint32_t* rbody;  /* SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot */
-    float* stiff;     /* 1 where the liquid is too viscous to flow */
+    v4* rh;           /* non-Newtonian share, its row, its concentration; w = 1 where stiff */
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -529,7 +528,9 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
         if (rig) {
             const int px1 = clampi((int)floorf(p.pos[0]), 1, s->W - 2), py1 = clampi((int)floorf(p.pos[1]), 1, s->H - 2);
             const int in_body = s->rl[2 * cidx(s, px1, py1)] >= 0;
-            if (frozen || in_body) {
+            const int px0 = clampi((int)floorf(p.pos[0]), 0, s->W - 1), py0 = clampi((int)floorf(p.pos[1]), 0, s->H - 1);
+            const int plug = (s->mats[row].rheo_p[0] > 0.0f || s->mats[row].temp[3] > 0.0f) && s->cell[cidx(s, px0, py0)].w <= 0.0f;
+            if (frozen || plug || in_body) {
                 static const int ox[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1}, oy[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
                 for (int k = 0; k < 9 && !held; k++) {
                     const int root = s->rl[2 * cidx(s, px1 + ox[k], py1 + oy[k])];
@@ -933,6 +934,7 @@ static void stage_grid_prep(or_sim* s) {
             int major = 0, top_row = -1;
             float top_share = 0.0f;
             float T = P->heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
+            v4 nn = {0.0f, -1.0f, 1.0f, 0.0f};  /* rh: the cell's non-Newtonian material */
             for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
             if (cw > 0) {
                 int others = 0;
@@ -1020,6 +1022,9 @@ static void stage_grid_prep(or_sim* s) {
                             frozen += share * fs;
                         }
                     }
+                    /* the non-Newtonian material with the largest share (grid_prep.comp:162) */
+                    if ((m->rheo != 0 || m->temp[3] > 0.0f || m->gr[0] > 0.0f) && share > nn.x)
+                        nn = (v4){share, (float)r, sc, 0.0f};
                     if (share > top_share) {
                         top_share = share;
                         top_row = r;
@@ -1050,14 +1055,15 @@ static void stage_grid_prep(or_sim* s) {
             float level = has ? phi_t - SP_LIQUID_THR : -SP_LIQUID_THR;
             if (has && level <= 0.0f && cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
                 if ((packed(s, i - 1) && packed(s, i + 1)) || (packed(s, i - W) && packed(s, i + W))) level = 0.05f;
-                else if (P->gravity > 0.0f && phi_t > 0.25f && packed(s, i - W)) level = 0.05f;
+                else if (P->gravity > 0.0f && nn.x <= 0.0f && phi_t > 0.25f && packed(s, i - W)) level = 0.05f;
             }
             if (sol.w > 0.0f && level <= 0.0f) s->region_air |= 1u << (s->region[i] & SP_REGION_MASK);
             /* stiff where the viscosity would exceed the rigid cap (grid_prep.comp:230-237) */
-            if (P->caps & SP_CAP_RIGID) {
+            if (P->caps & (SP_CAP_RIGID | SP_CAP_RHEO)) {
                 const float capv = rho * expf(P->ln_rigid);
-                s->stiff[i] = (mu >= capv && level > 0.0f) ? 1.0f : 0.0f;
+                if (mu >= capv && level > 0.0f) nn.w = 1.0f;
                 mu = fminf_(mu, capv);
+                s->rh[i] = nn;
             }
             s->cell[i] = (v4){phi_f, rho, mu, level};
             s->comp[i] = (v4){rho, f[1], fam0, phi_t};
@@ -2816,11 +2822,83 @@ static void stage_visc_before(or_sim* s) {
 
 /* ---- rigid bodies (rigid.comp) --------------------------------------------------------- */
 
-/* rigid (rigid.comp:34-47, without the rheology pass's unyielded and jammed cells) */
-static int rigid_cell(const or_sim* s, int i) {
-    if (s->cell[i].w <= 0.0f) return 0;
+/* shear_rate (common.glsl:527-540) */
+static float uf_(const or_sim* s, int k) { return face_live(s->vel[k].z) ? s->vel[k].x : 0.0f; }
+static float vf_(const or_sim* s, int k, float gv) { return face_live(s->vel[k].w) ? s->vel[k].y + gv : 0.0f; }
+static float shear_rate(const or_sim* s, int i, int pre) {
+    const int W = s->W;
+    const float gv = pre ? s->P.gravity * s->S.dt : 0.0f;
+    const float dxx = uf_(s, i + 1) - uf_(s, i);
+    const float dyy = vf_(s, i + W, gv) - vf_(s, i, gv);
+    const float duy = 0.25f * (uf_(s, i + W) + uf_(s, i + 1 + W) - uf_(s, i - W) - uf_(s, i + 1 - W));
+    const float dvx = 0.25f * (vf_(s, i + 1, gv) + vf_(s, i + 1 + W, gv) - vf_(s, i - 1, gv) - vf_(s, i - 1 + W, gv));
+    const float dxy = 0.5f * (duy + dvx);
+    return sqrtf(2.0f * (dxx * dxx + dyy * dyy + 2.0f * dxy * dxy));
+}
+
+/* nn_eval (common.glsl:546-571): the factor on the Newtonian viscosity and the yield stress
+ * of cell i's share of its non-Newtonian material at shear rate g */
+static void nn_eval(const or_sim* s, int i, v4 q, float g, float* F, float* tau) {
+    const sp_gmat* m = &s->mats[(int)q.y];
+    const float n = m->rheo_p[1], lam = m->rheo_p[2], ratio = m->rheo_p[3];
+    *F = 1.0f;
+    *tau = 0.0f;
+    if (m->rheo == 1) {
+        *F = ratio + (1.0f - ratio) * powf(1.0f + (lam * g) * (lam * g), 0.5f * (n - 1.0f));
+    } else if (m->rheo == 2) {
+        *F = powf(fmaxf_(g, 1e-6f), n - 1.0f);
+        *tau = m->rheo_p[0] * q.z;
+    } else if (m->rheo == 3) {
+        *F = fminf_(powf(1.0f + (lam * g) * (lam * g), 0.5f * (n - 1.0f)), ratio);
+    }
+    const float fr = m->gr[0];
+    if (fr > 0.0f) *tau += fr * q.z * q.z * q.z * fmaxf_(s->pres[i].x, 0.0f) / fmaxf_(s->S.dt, 1e-6f);
+    const int W = s->W;
+    const float skin = m->temp[3];
+    if (skin > 0.0f && (s->cell[i - 1].w <= 0.0f || s->cell[i + 1].w <= 0.0f || s->cell[i - W].w <= 0.0f || s->cell[i + W].w <= 0.0f))
+        *tau += skin;
+    *F = 1.0f + q.x * (*F - 1.0f);
+    *tau *= q.x;
+}
+
+/* rheo.comp (rheo.comp:31-55) */
+static void stage_rheo(or_sim* s) {
+    const int W = s->W, H = s->H;
+    for (int cy = 1; cy < H - 1; cy++)
+        for (int cx = 1; cx < W - 1; cx++) {
+            const int i = cy * W + cx;
+            v4 me = s->cell[i];
+            v4 q = s->rh[i];
+            if (me.w <= 0.0f || q.x <= 0.0f) continue;
+            const float g = shear_rate(s, i, 1);
+            float F, tau;
+            nn_eval(s, i, q, g, &F, &tau);
+            const float eff = me.z * F + tau / fmaxf_(g, 1e-6f);
+            const float capv = me.y * expf(s->P.ln_rigid);
+            const float skin = q.x * s->mats[(int)q.y].temp[3];
+            if (skin > 0.0f && me.y * g * g < skin &&
+                (s->cell[i - 1].w <= 0.0f || s->cell[i + 1].w <= 0.0f || s->cell[i - W].w <= 0.0f || s->cell[i + W].w <= 0.0f)) {
+                q.w = 1.0f;
+                s->rh[i] = q;
+            }
+            me.z = fminf_(eff, capv);
+            s->cell[i] = me;
+        }
+}
+
+/* rigid (rigid.comp:34-47) */
+static int rigid_cell(const or_sim* s, int cx, int cy, int i) {
+    const v4 me = s->cell[i];
+    if (me.w <= 0.0f) return 0;
     if (thermal(s) && s->therm[i].w > SP_RIGID_SHARE) return 1;
-    return s->stiff[i] > 0.5f;
+    const v4 q = s->rh[i];
+    if (q.w > 0.5f) return 1;
+    if (!(s->P.caps & SP_CAP_RHEO) || q.x <= 0.0f || cx < 1 || cy < 1 || cx >= s->W - 1 || cy >= s->H - 1) return 0;
+    const float g = shear_rate(s, i, 0);
+    float F, tau;
+    nn_eval(s, i, q, g, &F, &tau);
+    if (tau > 0.0f && me.z * g < tau) return 1;
+    return s->mats[(int)q.y].rheo == 3 && me.z * F >= me.y * expf(s->P.ln_rigid);
 }
 
 static int rigid_root(const or_sim* s, int i) {
@@ -2845,7 +2923,7 @@ static void stage_rigid(or_sim* s, int mode) {
     const int W = s->W, H = s->H, N = s->N;
     if (mode == 0) {
         for (int i = 0; i < N; i++) {
-            s->rl[2 * i] = rigid_cell(s, i) ? i : -1;
+            s->rl[2 * i] = rigid_cell(s, i % W, i / W, i) ? i : -1;
             s->rl[2 * i + 1] = -1;
         }
         s->rbody[0] = 0u;
@@ -2977,6 +3055,7 @@ void or_stage(or_sim* s, int stage, int arg) {
         break;
     }
     case SP_STAGE_HEAT: stage_heat(s); break;
+    case SP_STAGE_RHEO: stage_rheo(s); break;
     case SP_STAGE_RIGID:
         if (arg >= 0 && arg <= 4) stage_rigid(s, arg);
         break;
@@ -2999,6 +3078,7 @@ void or_substep(or_sim* s, int gather) {
     stage_clear_acc(s);
     stage_particles(s, gather);
     stage_grid_prep(s);
+    if (s->P.caps & SP_CAP_RHEO) stage_rheo(s);
     stage_coef(s);
     stage_surf_smooth(s);
     stage_surf_curv(s);
@@ -3060,7 +3140,7 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->solid_t = (float*)calloc(N, sizeof(float));
     s->rl = (int32_t*)calloc(2 * (size_t)N, sizeof(int32_t));
     s->rbody = (uint32_t*)calloc(SP_RIGID_WORDS, sizeof(uint32_t));
-    s->stiff = (float*)calloc(N, sizeof(float));
+    s->rh = (v4*)calloc(N, sizeof(v4));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -3098,7 +3178,7 @@ void or_destroy(or_sim* s) {
     free(s->solid_t);
     free(s->rl);
     free(s->rbody);
-    free(s->stiff);
+    free(s->rh);
     free(s);
 }
 
@@ -3191,7 +3271,7 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_LEDGER: *bytes = sizeof(s->ledger); return s->ledger;
     case SP_FIELD_RL: *bytes = 8 * (size_t)N; return s->rl;
     case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return s->rbody;
-    case SP_FIELD_STIFF: *bytes = 4 * (size_t)N; return s->stiff;
+    case SP_FIELD_RH: *bytes = sizeof(v4) * (size_t)N; return s->rh;
     default: *bytes = 0; return NULL;
     }
 }

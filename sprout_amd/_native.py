@@ -37,12 +37,12 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, STIFF=50)
+             LEDGER=47, RL=48, RBODY=49, RH=50)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18, HEAT=19, RIGID=20)
+             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -59,7 +59,9 @@ class Material(C.Structure):
                 ("visc_exponent", C.c_float), ("load", C.c_float), ("spawn_temp", C.c_float),
                 ("latent", C.c_float), ("heat_capacity", C.c_float), ("conductivity", C.c_float),
                 ("emissivity", C.c_float), ("boil", C.c_float), ("latent_vap", C.c_float),
-                ("freeze_shift", C.c_float)]
+                ("freeze_shift", C.c_float), ("rheology", C.c_uint32), ("yield_stress", C.c_float),
+                ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
+                ("skin", C.c_float), ("friction", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -131,7 +133,8 @@ class GMat(C.Structure):
                [(n, C.c_float) for n in ("rho", "ln_nu", "sigma", "mix_rate", "cos_c", "sin_c",
                                           "grain", "rho_solid", "visc_exp", "expansion", "load",
                                           "freeze_shift")] + \
-               [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4)]
+               [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4),
+                ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4)]
 
 
 class GParams(C.Structure):

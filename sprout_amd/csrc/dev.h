@@ -55,7 +55,7 @@ struct Ctx {
     // rigid bodies (rigid.comp)
     int2* rl;             // per cell: root cell of its body (-1: not rigid); at a root, its slot
     uint32_t* rbody;      // SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot
-    float* stiff;         // per cell: 1 where the liquid is too viscous to flow (rh.w)
+    float4* rh;           // per cell: non-Newtonian share, its row, its concentration; w = 1 where stiff
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -131,7 +131,7 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.ledger = (uint32_t*)take(sizeof(uint32_t) * SP_LEDGER_WORDS);
     c.rl = (int2*)take(sizeof(int2) * N);
     c.rbody = (uint32_t*)take(sizeof(uint32_t) * SP_RIGID_WORDS);
-    c.stiff = (float*)take(sizeof(float) * N);
+    c.rh = (float4*)take(sizeof(float4) * N);
     if (bytes) *bytes = o;
     return c;
 }
@@ -158,6 +158,51 @@ SP_DEV float sample_sdf(const Ctx& c, float px, float py, float& gx, float& gy) 
     gx = mixf_(d10 - d00, d11 - d01, fy);
     gy = mixf_(d01 - d00, d11 - d10, fx);
     return mixf_(mixf_(d00, d10, fx), mixf_(d01, d11, fx), fy);
+}
+
+// ---- rheology (common.glsl:527-571) --------------------------------------------------------
+
+// Shear rate sqrt(2 D:D) of cell i from the live faces around it; `pre`: from the velocity
+// before the solve, which has not had gravity's step yet on v faces (shear_rate,
+// common.glsl:527-540).
+SP_DEV float shear_rate(const Ctx& c, int i, bool pre) {
+    const int W = c.W;
+    const float gv = pre ? c.P->gravity * c.S->dt : 0.0f;
+    auto UF = [&](int k) { const float4 q = c.vel[k]; return face_live(q.z) ? q.x : 0.0f; };
+    auto VF = [&](int k) { const float4 q = c.vel[k]; return face_live(q.w) ? q.y + gv : 0.0f; };
+    const float dxx = UF(i + 1) - UF(i);
+    const float dyy = VF(i + W) - VF(i);
+    const float duy = 0.25f * (UF(i + W) + UF(i + 1 + W) - UF(i - W) - UF(i + 1 - W));
+    const float dvx = 0.25f * (VF(i + 1) + VF(i + 1 + W) - VF(i - 1) - VF(i - 1 + W));
+    const float dxy = 0.5f * (duy + dvx);
+    return sqrtf(2.0f * (dxx * dxx + dyy * dyy + 2.0f * dxy * dxy));
+}
+
+// Non-Newtonian response of cell i (q = rh[i]) at shear rate g: the factor F on the
+// Newtonian viscosity and the yield stress, both for the cell's share of the material
+// (nn_eval, common.glsl:546-571).
+SP_DEV void nn_eval(const Ctx& c, int i, float4 q, float g, float& F, float& tau) {
+    const int r = (int)q.y;
+    const sp_gmat& m = c.mats[r];
+    const float n = m.rheo_p[1], lam = m.rheo_p[2], ratio = m.rheo_p[3];
+    F = 1.0f;
+    tau = 0.0f;
+    if (m.rheo == 1) {
+        F = ratio + (1.0f - ratio) * powf(1.0f + (lam * g) * (lam * g), 0.5f * (n - 1.0f));
+    } else if (m.rheo == 2) {
+        F = powf(fmaxf_(g, 1e-6f), n - 1.0f);
+        tau = m.rheo_p[0] * q.z;
+    } else if (m.rheo == 3) {
+        F = fminf_(powf(1.0f + (lam * g) * (lam * g), 0.5f * (n - 1.0f)), ratio);
+    }
+    const float fr = m.gr[0];
+    if (fr > 0.0f) tau += fr * q.z * q.z * q.z * fmaxf_(c.pres[i].x, 0.0f) / fmaxf_(c.S->dt, 1e-6f);
+    const int W = c.W;
+    const float skin = m.temp[3];
+    if (skin > 0.0f && (c.cell[i - 1].w <= 0.0f || c.cell[i + 1].w <= 0.0f || c.cell[i - W].w <= 0.0f || c.cell[i + W].w <= 0.0f))
+        tau += skin;
+    F = 1.0f + q.x * (F - 1.0f);
+    tau *= q.x;
 }
 
 // ---- rigid bodies (common.glsl:263-340) ---------------------------------------------------

@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float top_share = 0.0f;
     const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
     float T = P.heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
+    float4 nn = make_float4(0.0f, -1.0f, 1.0f, 0.0f);  // rh: the cell's non-Newtonian material
     for (int q = 0; q < nmix; q++) c.mixb[(size_t)i * nmix + q] = -1.0f;
     if (cw > 0) {
         int others = 0;
@@ -216,6 +217,8 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
                     frozen += share * fs;
                 }
             }
+            // the cell's non-Newtonian material: the one with the largest share (grid_prep.comp:162)
+            if ((m.rheo != 0 || m.temp[3] > 0.0f || m.gr[0] > 0.0f) && share > nn.x) nn = make_float4(share, (float)r, sc, 0.0f);
             if (share > top_share) {
                 top_share = share;
                 top_row = r;
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float level = has ? phi_t - SP_LIQUID_THR : -SP_LIQUID_THR;
     if (has && level <= 0.0f && cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
         if ((packed(c, i - 1) && packed(c, i + 1)) || (packed(c, i - W) && packed(c, i + W))) level = 0.05f;
-        else if (P.gravity > 0.0f && phi_t > 0.25f && packed(c, i - W)) level = 0.05f;
+        else if (P.gravity > 0.0f && nn.x <= 0.0f && phi_t > 0.25f && packed(c, i - W)) level = 0.05f;
     }
     if (sol.w > 0.0f && level <= 0.0f) {
         uint32_t bit = 1u << (c.region[i] & SP_REGION_MASK);
@@ -256,10 +259,11 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     }
     // the viscosity solve stops at rigid_viscosity: a cell that would need more is stiff and
     // moves rigidly (grid_prep.comp:230-237)
-    if (P.caps & SP_CAP_RIGID) {
+    if (P.caps & (SP_CAP_RIGID | SP_CAP_RHEO)) {
         const float cap = rho * expf(P.ln_rigid);
-        c.stiff[i] = (mu >= cap && level > 0.0f) ? 1.0f : 0.0f;
+        if (mu >= cap && level > 0.0f) nn.w = 1.0f;
         mu = fminf_(mu, cap);
+        c.rh[i] = nn;
     }
     c.cell[i] = make_float4(phi_f, rho, mu, level);
     c.comp[i] = make_float4(rho, f[1], fam0, phi_t);

@@ -47,6 +47,13 @@ static sp_material default_material() {
     m.boil = 1.0e9f;
     m.latent_vap = 0.0f;
     m.freeze_shift = 0.0f;
+    m.rheology = 0;  // material.rs:351-355, 384, 405
+    m.yield_stress = 0.0f;
+    m.flow_index = 1.0f;
+    m.shear_time = 1.0f;
+    m.visc_ratio = 1.0f;
+    m.skin = 0.0f;
+    m.friction = 0.0f;
     return m;
 }
 
@@ -309,9 +316,11 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
 }
 
 // FluidSim::caps (mod.rs:2717-2766) for the CAP-gated passes this build carries: families,
-// heat and rigid bodies. A thermal tank turns rigid bodies on for any liquid that can freeze
-// or has a temperature law; any tank for a liquid already as viscous as rigid_viscosity, or
-// frozen at the ambient temperature.
+// heat, rigid bodies and rheology. A non-Newtonian liquid, or one with a skin or internal
+// friction, turns rheology on, and rigid bodies for a yield stress, shear thickening, a skin
+// or friction. A thermal tank turns rigid bodies on for any liquid that can freeze or has a
+// temperature law; any tank for a liquid already as viscous as rigid_viscosity, or frozen at
+// the ambient temperature.
 static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
     uint32_t caps = 0;
     if (t.families > 0) caps |= SP_CAP_FAMILIES;
@@ -320,6 +329,10 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
     for (int r : t.rows) {
         const sp_material& m = p.mat[r];
         const sp_material* dil = m.diluent >= 0 ? &p.mat[m.diluent] : nullptr;
+        if (m.rheology != 0 || m.skin > 0.0f || m.friction > 0.0f) {
+            caps |= SP_CAP_RHEO;
+            rigid |= m.rheology == 2 || m.rheology == 3 || m.skin > 0.0f || m.friction > 0.0f;
+        }
         if (t.thermal)
             rigid |= freezes(m) || (dil && freezes(*dil)) || m.temp_model != 0 || (dil && dil->temp_model != 0);
         rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (!t.thermal && freezes(m) && p.ambient_temp < m.melt_hi);
@@ -365,7 +378,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.chan_w = chan[0];
         o.chan_s = chan[1];
         o.fam = chan[2];
-        o.rheo = 0;
+        o.rheo = (int32_t)m.rheology;
         o.temp_model = m.temp_model;
         o.residue = -1;
         o.rho = rho / WATER_DENSITY;
@@ -392,7 +405,13 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.temp[0] = m.temp_b;
         o.temp[1] = m.temp_c;
         o.temp[2] = m.temp_ref + 273.15f;
-        o.temp[3] = 0.0f;
+        o.temp[3] = tension_cells(p, m.skin);
+        // rheology (mod.rs:2184, 2211): the yield stress in cells^2/s^2 per unit density
+        o.rheo_p[0] = m.yield_stress / (WATER_DENSITY * dx * dx);
+        o.rheo_p[1] = m.flow_index;
+        o.rheo_p[2] = m.shear_time;
+        o.rheo_p[3] = m.visc_ratio;
+        o.gr[0] = m.friction;
     }
     const uint32_t caps = tank_caps(p, t);
     // gpu_params (mod.rs:2238-2330).

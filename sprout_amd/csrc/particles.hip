@@ -166,14 +166,18 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
         float corrx = 0.0f, corry = 0.0f, slip = 0.0f, crowd = 0.0f;
         gather_face(c, p.pos[0], p.pos[1], 0, vx, corrx, C0, C1);
         gather_face(c, p.pos[0], p.pos[1], 1, vy, corry, C2, C3);
-        // a frozen particle, or any particle in a cell of a body, moves exactly with the rigid
-        // body around it: the first body in the 3 x 3 cells around its own, own cell first
-        // (particles.comp:223-248; the yield-stress plug of the rheology pass is not carried)
+        // a frozen particle, a yield-stress particle on a plug, or any particle in a cell of a
+        // body moves exactly with the rigid body around it: the first body in the 3 x 3 cells
+        // around its own, own cell first (particles.comp:223-248)
         bool held = false;
         if (rig) {
             const int px1 = clampi_(floori(p.pos[0]), 1, c.W - 2), py1 = clampi_(floori(p.pos[1]), 1, c.H - 2);
             const bool in_body = c.rl[cidx(c, px1, py1)].x >= 0;
-            if (frozen || in_body) {
+            // a yield-stress liquid's particle in a cell too sparse to be liquid (the surface
+            // of a resting pile), next to a plug
+            const int px0 = clampi_(floori(p.pos[0]), 0, c.W - 1), py0 = clampi_(floori(p.pos[1]), 0, c.H - 1);
+            const bool plug = (c.mats[row].rheo_p[0] > 0.0f || c.mats[row].temp[3] > 0.0f) && c.cell[cidx(c, px0, py0)].w <= 0.0f;
+            if (frozen || plug || in_body) {
                 const int ox[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1}, oy[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
                 for (int k = 0; k < 9 && !held; k++) {
                     const int root = c.rl[cidx(c, px1 + ox[k], py1 + oy[k])].x;

@@ -16,12 +16,22 @@
 
 namespace {
 
-// Rigid: liquid that is mostly frozen, or too viscous to flow (rigid.comp:34-47; the
-// non-Newtonian cases belong to the rheology pass, which is not carried).
-SP_DEV bool rigid_cell(const Ctx& c, int i) {
-    if (c.cell[i].w <= 0.0f) return false;
+// Rigid: liquid that is mostly frozen, too viscous to flow, or, after the solve, a
+// non-Newtonian cell that does not flow: its stress mu g stays below its yield stress (an
+// unyielded plug), or its viscosity at its shear rate reaches the cap (jammed shear-thickening
+// liquid) (rigid.comp:34-47).
+SP_DEV bool rigid_cell(const Ctx& c, int cx, int cy, int i) {
+    const float4 me = c.cell[i];
+    if (me.w <= 0.0f) return false;
     if ((c.P->caps & SP_CAP_THERMAL) && c.therm[i].w > SP_RIGID_SHARE) return true;
-    return c.stiff[i] > 0.5f;
+    const float4 q = c.rh[i];
+    if (q.w > 0.5f) return true;
+    if (!(c.P->caps & SP_CAP_RHEO) || q.x <= 0.0f || cx < 1 || cy < 1 || cx >= c.W - 1 || cy >= c.H - 1) return false;
+    const float g = shear_rate(c, i, false);
+    float F, tau;
+    nn_eval(c, i, q, g, F, tau);
+    if (tau > 0.0f && me.z * g < tau) return true;
+    return c.mats[(int)q.y].rheo == 3 && me.z * F >= me.y * expf(c.P->ln_rigid);
 }
 
 // The label of cell i as memory holds it now (read past the non-coherent cache: other groups
@@ -67,6 +77,33 @@ SP_DEV void add64(const Ctx& c, uint32_t w, float x) {
 
 }  // namespace
 
+// rheo.comp: the non-Newtonian viscosity of each liquid cell from the shear rate of the
+// velocity before the solve (lagged by one solve), mu_eff = mu F + tau / g up to the rigid
+// cap; a skin cell holds (stiff) until the motion across it breaks it (rheo.comp:31-55).
+__global__ __launch_bounds__(256) void k_rheo(const TankBatch tb) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const int cx = blockIdx.x * 16 + threadIdx.x, cy = blockIdx.y * 16 + threadIdx.y;
+    if (cx < 1 || cy < 1 || cx >= c.W - 1 || cy >= c.H - 1) return;
+    const int W = c.W;
+    const int i = cy * W + cx;
+    float4 me = c.cell[i];
+    float4 q = c.rh[i];
+    if (me.w <= 0.0f || q.x <= 0.0f) return;
+    const float g = shear_rate(c, i, true);
+    float F, tau;
+    nn_eval(c, i, q, g, F, tau);
+    const float eff = me.z * F + tau / fmaxf_(g, 1e-6f);
+    const float cap = me.y * expf(c.P->ln_rigid);
+    const float skin = q.x * c.mats[(int)q.y].temp[3];
+    if (skin > 0.0f && me.y * g * g < skin &&
+        (c.cell[i - 1].w <= 0.0f || c.cell[i + 1].w <= 0.0f || c.cell[i - W].w <= 0.0f || c.cell[i + W].w <= 0.0f)) {
+        q.w = 1.0f;
+        c.rh[i] = q;
+    }
+    me.z = fminf_(eff, cap);
+    c.cell[i] = me;
+}
+
 // rigid.comp:101-196.
 __global__ __launch_bounds__(256) void k_rigid(const TankBatch tb, int mode) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
@@ -75,7 +112,7 @@ __global__ __launch_bounds__(256) void k_rigid(const TankBatch tb, int mode) {
     const int W = c.W;
     const int i = cy * W + cx;
     if (mode == 0) {
-        c.rl[i] = make_int2(rigid_cell(c, i) ? i : -1, -1);
+        c.rl[i] = make_int2(rigid_cell(c, cx, cy, i) ? i : -1, -1);
         if (i == 0) c.rbody[0] = 0u;
         return;
     }

@@ -243,6 +243,8 @@ void launch_second_projection(const Run& r, const Recs& recs) {
 void launch_grid_passes(const Run& r, const Recs& recs) {
     const dim3 g0 = grid0(r);
     hipLaunchKernelGGL(k_grid_prep, g0, TILE, 0, r.stream, r.tb);
+    // non-Newtonian viscosity (record_substep, mod.rs:2639-2642)
+    if (r.gp.caps & SP_CAP_RHEO) hipLaunchKernelGGL(k_rheo, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, r.stream, r.tb);
@@ -391,7 +393,8 @@ int launch_particles(const Run& r, int gather) {
 uint32_t schedule_key(const sp_gparams& g) {
     return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
            (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12 |
-           (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14;
+           (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14 |
+           (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15;
 }
 
 }  // namespace

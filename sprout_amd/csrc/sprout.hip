@@ -1030,7 +1030,7 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_LEDGER: *bytes = 4 * SP_LEDGER_WORDS; return c.ledger;
     case SP_FIELD_RL: *bytes = 8 * N; return c.rl;
     case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return c.rbody;
-    case SP_FIELD_STIFF: *bytes = 4 * N; return c.stiff;
+    case SP_FIELD_RH: *bytes = 16 * N; return c.rh;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1129,6 +1129,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         }
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_HEAT: hipLaunchKernelGGL(k_heat, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_RHEO: hipLaunchKernelGGL(k_rheo, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_RIGID:
             if (arg < 0 || arg > 4) return SP_E_INVALID;
             hipLaunchKernelGGL(k_rigid, g0, TILE, 0, s->stream, cs, arg);

@@ -172,6 +172,28 @@ def stiff_block(grid, block=(0.3, 0.6, 0.4, 0.7), viscosity=5.0, cell_size=0.005
     return params, scene.rasterize(), settle(parts, params), [N.ROW_HONEY]
 
 
+def rheo_block(grid, rheology=2, yield_stress=300.0, flow_index=1.0, shear_time=1.0, visc_ratio=1.0,
+               viscosity=0.05, skin=0.0, friction=0.0, block=(0.35, 0.65, 0.05, 0.55), cell_size=0.005,
+               pressure_tol=1e-6):
+    """A block of a non-Newtonian liquid (the honey row with a rheology: 1 Carreau, 2
+    Herschel-Bulkley, 3 shear thickening, 0 Newtonian) slumping on the floor of an empty tank,
+    isothermal. `block` in fractions of W and H. (params, solid field, particles, rows)."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    m = params.mat[N.ROW_HONEY]
+    m.viscosity = viscosity
+    m.rheology = rheology
+    m.yield_stress = yield_stress
+    m.flow_index = flow_index
+    m.shear_time = shear_time
+    m.visc_ratio = visc_ratio
+    m.skin = skin
+    m.friction = friction
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (block[0] * W, block[2] * H, block[1] * W, block[3] * H), N.ROW_HONEY)
+    return params, scene.rasterize(), settle(parts, params), [N.ROW_HONEY]
+
+
 def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), strip_temp=76.85):
     """The two-fluid tank with heat: water below (density 1000, 1e-6 m^2/s), oil above
     (900, 1e-4), a honey-like blob (1400, 0.09) near the top centre, everything at 20 deg C

@@ -90,8 +90,8 @@ class Pair:
                 self.gpu.put_field("RL", self.ora.field("RL", np.int32, 2 * self.N))
             elif n == "RBODY":
                 self.gpu.put_field("RBODY", self.ora.field("RBODY", np.uint32, N.RIGID_WORDS))
-            elif n == "STIFF":
-                self.gpu.put_field("STIFF", self.ora.field("STIFF", np.float32, self.N))
+            elif n == "RH":
+                self.gpu.put_field("RH", self.ora.field("RH", np.float32, (self.N, 4)))
             elif n in ("PTILE", "VTILE"):
                 self.gpu.put_field(n, self.ora.field(n, np.uint32, self.L.tiles))
             elif n == "VTILES":

@@ -30,12 +30,12 @@ def up_to_projection(p):
     p.both("CLEAR_ACC")
     p.both("PARTICLES", 0)
     p.both("GRID_PREP")
-    a, b = p.f("STIFF", np.float32, p.N)
+    a, b = p.f("RH", np.float32, (p.N, 4))
     assert np.array_equal(a, b), "stiff flags"
     for st in ("COEF", "SURF", "VISC", "RHS", "PRESSURE", "PROJECT"):
         p.ora.stage(st)
     assert np.all(np.isfinite(p.ora.field("VEL", np.float32, (p.N, 4))))
-    p.sync_to_gpu("VEL", "CELL", "STIFF", "THERM")
+    p.sync_to_gpu("VEL", "CELL", "RH", "THERM")
 
 
 def sums_by_root(rl, words):

@@ -53,7 +53,11 @@ def put_body(o, grid, mask, rho=1.0, phi=1.0, vel=None):
     cell[:, 1] = np.where(m, rho, 0.0)
     cell[:, 3] = np.where(m, 0.5, -0.5)
     o.put_field("CELL", cell)
-    o.put_field("STIFF", m.astype(np.float32))
+    rh = np.zeros((n, 4), np.float32)
+    rh[:, 1] = -1.0
+    rh[:, 2] = 1.0
+    rh[:, 3] = m.astype(np.float32)
+    o.put_field("RH", rh)
     v = np.zeros((n, 4), np.float32) if vel is None else vel.reshape(n, 4).astype(np.float32)
     o.put_field("VEL", v)
 
@@ -246,8 +250,8 @@ def test_stiff_block_falls_rigidly():
     b = p["pos"][by_id].astype(np.float64)
     drop = (a[:, 1] - b[:, 1]).mean()
     assert drop > 0.5  # it fell
-    stiff = o.field("STIFF", np.float32, grid[0] * grid[1])
-    assert stiff.sum() > 10
+    rh = o.field("RH", np.float32, (grid[0] * grid[1], 4))
+    assert rh[:, 3].sum() > 10
     sel = np.random.default_rng(1).choice(len(a), min(300, len(a)), replace=False)
     da, db = pairwise(a[sel]), pairwise(b[sel])
     assert np.abs(da - db).max() < 2e-3 * max(1.0, drop)
