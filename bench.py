@@ -6,13 +6,18 @@ substeps as the reference's blocking helpers record them (advance_blocking,
 mod.rs:3284-3312). Inputs are resident in device memory when the timed region starts.
 
 Workloads (config.workload):
-* four_liquids_1024x576_fp32 (default): oil, water, honey and mercury in four seeded
+* four_liquids_1024x576_mixed_mg (default): oil, water, honey and mercury in four seeded
   rectangles tiling 80 % x 80 % of a 1024 x 576 tank (about 1.49 M particles at 4 per cell),
   densities 800 / 1000 / 1300 / 13500 kg/m^3, viscosities log-uniform over 1e-6 .. 1e-1 m^2/s
   and tensions uniform over 0.02 .. 0.07 N/m from seed 0; cell 5 mm, fixed dt 1/240 s;
   pressure tolerance 1e-6 and viscosity tolerance 1e-5 (the tightest the reference
-  accepts); the implicit viscosity with the second pressure solve; all FP32 with FP64
-  reductions.
+  accepts); the implicit viscosity with the second pressure solve; the pressure multigrid's
+  coarse levels in BF16 (iterates, right-hand sides) and FP16 (matrices), FP32 level 0,
+  smoothing and CG vectors, FP64 reductions (--mg-precision fp32: FP32 levels).
+
+Besides the single tank the line carries `fp32_single` (the same tank with FP32 multigrid
+levels, the plain FP32 path) and `batched` (--batch-tanks tanks of the workload, seeds 0 ..,
+advanced as one batch) with its ratio to the plain FP32 tank.
 * dam_break_1024x576_fp32 (--workload dam): one block of water of the same size, explicit
   viscosity, one pressure solve.
 * two_fluid_heat_512x288_fp32 (--workload two_fluid, --grid 512x288): water below, oil above,
@@ -75,6 +80,59 @@ def workload(kind, grid, seed=0, mg_precision=0):
     else:
         out = cases.dam_break(grid, frac=(0.8, 0.8))
     out[0].mg_precision = mg_precision
+    return out
+
+
+def throughput_leg(kind, grid, n_tanks, mgp, steps, warmup, latency_steps, chunk=16):
+    """A leg of the same workload: `n_tanks` tanks (seeds 0 ..) advanced as one batch (or the
+    tank alone), multigrid storage `mgp`; aggregate particle-steps/s over `steps` substeps
+    after `warmup`, and the latency of one substep of all of them, record + wait."""
+    from sprout_amd.fluid_sim import FluidBatch, FluidSim
+
+    W, H = grid
+    sims = []
+    for seed in range(n_tanks):
+        params, field, parts, rows = workload(kind, grid, seed, mgp)
+        sim = FluidSim(grid, W * H * 4, params=params)
+        sim.set_solids(field)
+        sim.reset(parts)
+        sims.append(sim)
+    n_p = sum(s.stats().live for s in sims)
+    batch = FluidBatch(sims) if n_tanks > 1 else None
+
+    def step(k):
+        if batch is not None:
+            batch.record(k, k / 240.0)
+            batch.wait()
+        else:
+            sims[0].record(k, k / 240.0)
+            sims[0].wait()
+
+    for k0 in range(0, warmup, chunk):
+        step(min(chunk, warmup - k0))
+    t0 = time.perf_counter()
+    for k0 in range(0, steps, chunk):
+        step(min(chunk, steps - k0))
+    wall = time.perf_counter() - t0
+    lat = []
+    for _ in range(latency_steps):
+        t1 = time.perf_counter()
+        step(1)
+        lat.append(1e3 * (time.perf_counter() - t1))
+    lat = np.sort(np.array(lat)) if lat else np.zeros(1)
+
+    def pct(q):
+        return float(lat[min(max(int(np.ceil(q / 100 * len(lat))), 1), len(lat)) - 1])
+    out = {"tanks": n_tanks, "particles": int(n_p), "steps": steps, "warmup": warmup,
+           "mg_storage": "fp32" if mgp == 0 else "bf16/fp16 coarse levels",
+           "particle_steps_per_s": n_p * steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "step_latency_ms": {"p50": pct(50), "p99": pct(99), "requests": latency_steps,
+                               "request": f"1 substep of all {n_tanks} tank(s), record + wait"}}
+    if batch is not None:
+        out["launch_groups"] = batch.groups()
+        batch.close()
+    for s in sims:
+        s.close()
     return out
 
 
@@ -207,13 +265,17 @@ def main():
                     help="tanks advanced together in one batch (seeds 0 .. tanks-1; value is their aggregate)")
     ap.add_argument("--latency-steps", type=int, default=64,
                     help="single-substep step requests timed one by one after the throughput run")
-    ap.add_argument("--mg-precision", choices=["fp32", "mixed"], default="fp32",
+    ap.add_argument("--mg-precision", choices=["fp32", "mixed"], default="mixed",
                     help="pressure multigrid storage: FP32, or BF16 iterates / FP16 coarse matrices")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the pressure solve's comparison with an FP64 solve of the same system")
     ap.add_argument("--serve-rates", default="1000,2000,4000",
                     help="comma-separated request rates (per s) for the served leg over 64 mixed tanks; empty: none")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--batch-tanks", type=int, default=8,
+                    help="tanks of the batched leg (the same workload, seeds 0 ..); 0: no batched leg")
+    ap.add_argument("--no-fp32-leg", action="store_true",
+                    help="skip the plain FP32 single-tank leg the batched leg is compared with")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -308,7 +370,7 @@ def main():
         sizes = sorted({t[0].grid for t in tanks})
         grid_desc = {f"{g[0]}x{g[1]}": sum(1 for t in tanks if t[0].grid == g) for g in sizes}
     else:
-        wname = f"{name}_{W}x{H}_fp32" + (f"_batch{len(tanks)}" if len(tanks) > 1 else "")
+        wname = f"{name}_{W}x{H}_" + ("fp32" if mgp == 0 else "mixed_mg") + (f"_batch{len(tanks)}" if len(tanks) > 1 else "")
         grid_desc = [W, H]
     config = {
         "workload": wname,
@@ -398,6 +460,14 @@ def main():
         batch.close()
     for t in tanks:
         t[0].close()
+    if args.workload != "mixed":
+        if not args.no_fp32_leg:
+            out["fp32_single"] = throughput_leg(args.workload, (W, H), 1, 0, 96, 32, 0)
+        if args.batch_tanks > 1:
+            out["batched"] = throughput_leg(args.workload, (W, H), args.batch_tanks, mgp, 64, 32, 16)
+            if "fp32_single" in out:
+                out["batched"]["vs_plain_fp32_single"] = (out["batched"]["particle_steps_per_s"] /
+                                                          out["fp32_single"]["particle_steps_per_s"])
     if args.serve_rates:
         out["served"] = served([float(r) for r in args.serve_rates.split(",")], args.serve_seconds)
     if not args.no_cpu:
