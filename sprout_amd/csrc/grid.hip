@@ -304,6 +304,35 @@ SP_DEV void face_coef(float4 a, float4 b, float open, float& g, float& rho) {
 // aux and solid, which grid_prep has finished).
 __global__ __launch_bounds__(256) void k_coef_smooth(const TankBatch tb) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    // the tile and a ring of two cells, each cell's smoothing inputs worked out once:
+    // its open-weighted fill phi (-1 where it takes no part: outside the grid or closed),
+    // phi times its group-1 share, and its group-2 and group-3 shares
+    constexpr int S = 20;
+    __shared__ float s_phi[S * S], s_py[S * S], s_az[S * S], s_aw[S * S];
+    const bool more = (c.P->group_mask & 12u) != 0u;
+    const int x0 = (int)blockIdx.x * 16 - 2, y0 = (int)blockIdx.y * 16 - 2;
+    for (int k = (int)(threadIdx.y * 16 + threadIdx.x); k < S * S; k += 256) {
+        const int nx = x0 + k % S, ny = y0 + k / S;
+        float phi = -1.0f, py = 0.0f, az = 0.0f, aw = 0.0f;
+        if (nx >= 0 && ny >= 0 && nx < c.W && ny < c.H) {
+            const int j = ny * c.W + nx;
+            const float open = c.solid[j].w;
+            if (open > 0.02f) {
+                phi = clampf_(c.cell[j].x / (SP_REST_PHI * open), 0.0f, 1.0f);
+                py = phi * c.comp[j].y;
+                if (more) {
+                    const float4 ax = c.aux[j];
+                    az = ax.z;
+                    aw = ax.w;
+                }
+            }
+        }
+        s_phi[k] = phi;
+        s_py[k] = py;
+        s_az[k] = az;
+        s_aw[k] = aw;
+    }
+    __syncthreads();
     int cx, cy, i;
     if (!cell_of_thread(c, cx, cy, i)) return;
     float4 o = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
@@ -315,24 +344,22 @@ __global__ __launch_bounds__(256) void k_coef_smooth(const TankBatch tb) {
     }
     c.coef[i] = o;
     const float B5[5] = {1.0f, 4.0f, 6.0f, 4.0f, 1.0f};
-    const bool more = (c.P->group_mask & 12u) != 0u;
     float sx = 0, sy = 0, s2x = 0, s2y = 0, wsum = 0;
+    const int lx = (int)threadIdx.x + 2, ly = (int)threadIdx.y + 2;
+#pragma unroll
     for (int b = -2; b <= 2; b++)
+#pragma unroll
         for (int a = -2; a <= 2; a++) {
-            int nx = cx + a, ny = cy + b;
-            if (nx < 0 || ny < 0 || nx >= c.W || ny >= c.H) continue;
-            int j = ny * c.W + nx;
-            float open = c.solid[j].w;
-            if (open <= 0.02f) continue;
-            float phi = clampf_(c.cell[j].x / (SP_REST_PHI * open), 0.0f, 1.0f);
+            const int k = (ly + b) * S + lx + a;
+            const float phi = s_phi[k];
+            if (phi < 0.0f) continue;
             float w = B5[a + 2] * B5[b + 2];
             sx += w * phi;
-            sy += w * (phi * c.comp[j].y);
+            sy += w * s_py[k];
             if (more) {
                 float wp = w * phi;
-                float4 ax = c.aux[j];
-                s2x += wp * ax.z;
-                s2y += wp * ax.w;
+                s2x += wp * s_az[k];
+                s2y += wp * s_aw[k];
             }
             wsum += w;
         }
