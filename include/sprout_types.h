@@ -86,6 +86,9 @@ typedef struct sp_material {
     uint32_t rheology;
     float yield_stress, flow_index, shear_time, visc_ratio;
     float skin, friction;
+    /* Viscoelasticity (material.rs:233-235): the shear modulus G (Pa; 0 = not elastic) and
+     * the relaxation time lambda (s) of an upper-convected Maxwell liquid. */
+    float shear_modulus, relax_time;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -143,6 +146,10 @@ typedef struct sp_sim_params {
      * solids is supported by its contacts, with this Coulomb friction. */
     uint32_t rigid_contact;
     float rigid_friction;
+    /* Viscoelastic stabilisation (params.rs:366-371): the largest stress, times G (0: no
+     * cap), and the largest deformation per substep that enters the stress (0: no limit). */
+    float elastic_stress_max;
+    float elastic_step_max;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -188,7 +195,7 @@ typedef struct sp_gmat {
     float th2[4];    /* latent heat as degrees (L / c); emissivity; boiling point (deg C); latent heat of vaporisation as degrees */
     float temp[4];   /* temperature law: B, C (K); reference temperature (K); skin (cells^3/s^2) */
     float rheo_p[4]; /* yield stress (cells^2/s^2), flow index, shear time (s), viscosity ratio */
-    float gr[4];     /* internal friction, 0, 0, 0 */
+    float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); 0 */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -231,7 +238,11 @@ typedef struct sp_gparams {
     float eddy;           /* heat3.x: Smagorinsky constant of the sub-grid mixing (0 off) */
     float evap;           /* heat3.y: L_vap / c_p,air (K; 0: no evaporation) */
     int32_t chan_t;       /* heat2.z: accumulator channel of the temperature sum (-1: isothermal) */
-    int32_t pad_heat;
+    /* Viscoelasticity (GpuParams elastic, mod.rs:2316-2321), used where caps has CAP_ELASTIC: */
+    int32_t chan_e;       /* elastic.x: first of the three accumulator channels of the stress (-1: none) */
+    float el_g;           /* elastic.y: the largest shear modulus in the tank (cells^2/s^2, at least 1e-3 Pa) */
+    float el_stress_max;  /* elastic.z: most stress, times G (0: no cap) */
+    float el_step_max;    /* elastic.w: most deformation per substep (0: no limit) */
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -307,6 +318,10 @@ typedef struct sp_levels_info {
 #define SP_CAP_THERMAL 2u
 #define SP_CAP_RIGID 4u
 #define SP_CAP_RHEO 8u
+#define SP_CAP_ELASTIC 32u
+/* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
+ * accumulators */
+#define SP_STRESS_FIX 65536.0f
 /* Heat (common.glsl:431-436) */
 #define SP_NO_TEMP (-1e9f)
 #define SP_LEDGER_SCALE 65536.0f
@@ -372,6 +387,8 @@ enum {
     SP_FIELD_RL = 48,        /* int2[N]    rigid body: root cell (-1: not rigid), at a root its slot */
     SP_FIELD_RBODY = 49,     /* uint32[16 + 32*4096] bodies claimed, then per slot 64-bit sums */
     SP_FIELD_RH = 50,        /* float4[N]  non-Newtonian share, its row, its concentration; 1 where stiff */
+    SP_FIELD_ESTR = 51,      /* float4[N]  viscoelastic stress per cell: xx, yy, xy, 0 (cells^2/s^2) */
+    SP_FIELD_PSTRESS = 52,   /* float4[n_slots] viscoelastic stress per particle slot: xx, yy, xy, 0 */
     SP_FIELD_COUNT
 };
 
@@ -399,6 +416,7 @@ enum {
     SP_STAGE_HEAT = 19,         /* heat.comp */
     SP_STAGE_RIGID = 20,        /* rigid.comp; arg = mode 0..4 */
     SP_STAGE_RHEO = 21,         /* rheo.comp */
+    SP_STAGE_ELASTIC = 22,      /* elastic.comp */
     SP_STAGE_COUNT
 };
 

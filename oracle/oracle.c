@@ -3,15 +3,16 @@
  * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
  * times the CPU baseline; nothing in the product calls it.
  *
- * Scope: the substep with heat, rigid bodies and rheology, without the other CAP-gated
- * passes (magnets, elasticity, settling, steam, reactions), tools, currents or film spatter:
+ * Scope: the substep with heat, rigid bodies, rheology and viscoelasticity, without the
+ * other CAP-gated passes (magnets, settling, steam, reactions), tools, currents or film
+ * spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
  *   (visc_setup/split/diag/strain/apply/iter/update.comp, vmg_build/down/up.comp,
  *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
- *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, sort.comp,
+ *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp, sort.comp,
  *   count.comp, stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
@@ -66,6 +67,9 @@ struct or_sim {
     int32_t* rl;      /* 2 per cell: root cell (-1: not rigid); at a root, its slot */
     uint32_t* rbody;  /* SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot */
     v4* rh;           /* non-Newtonian share, its row, its concentration; w = 1 where stiff */
+    /* viscoelasticity */
+    v4* estr;         /* per cell: stress xx, yy, xy (cells^2/s^2) */
+    v4* pstress;      /* per particle slot: its stress xx, yy, xy */
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -398,6 +402,8 @@ typedef struct {
     int live;
     float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc, CT;
     int cw, cs, ct, cg;
+    int ce;               /* first stress channel, -1: no stress to scatter */
+    float te[3];          /* the stress over el_g, in STRESS_FIX units */
 } scatter_in;
 
 static void acc_add(or_sim* s, int x, int y, int ch, int v) {
@@ -454,8 +460,53 @@ static void deposit(or_sim* s, const scatter_in* g) {
                     acc_add(s, bx + a, by + b, g->ct, h);
                     if (g->cg >= 0) acc_add(s, bx + a, by + b, g->cg, h);
                 }
+                /* viscoelastic stress, fixed point (particles.comp:1053-1058) */
+                if (g->ce >= 0)
+                    for (int q = 0; q < 3; q++) acc_add(s, bx + a, by + b, g->ce + q, (int)floorf(w[0][a] * w[1][b] * g->te[q] + 0.5f));
             }
     }
+}
+
+/* The upper-convected Maxwell stress of one particle over a substep (particles.comp:249-279),
+ * in the shader's matrix form: L the velocity gradient [[du/dx, du/dy], [dv/dx, dv/dy]] (the
+ * APIC matrix) scaled to at most el_step_max deformation per substep, T the stress,
+ * D = L T + T L^T + G (L + L^T), T' = (T + dt D) / (1 + dt / lambda); the norm capped at
+ * el_stress_max G; no stress without a cell's worth of liquid in the 3 x 3 cells around. */
+static v4 stress_update(const or_sim* s, float px, float py, const float C[4], v4 tau, float G, float lam, float dt) {
+    const sp_gparams* P = &s->P;
+    float L[2][2] = {{C[0], C[1]}, {C[2], C[3]}};
+    float lnorm = sqrtf(((C[0] * C[0] + C[1] * C[1]) + C[2] * C[2]) + C[3] * C[3]) * dt;
+    if (P->el_step_max > 0.0f && lnorm > P->el_step_max) {
+        float k = P->el_step_max / lnorm;
+        for (int r = 0; r < 2; r++)
+            for (int c = 0; c < 2; c++) L[r][c] *= k;
+    }
+    float T[2][2] = {{tau.x, tau.z}, {tau.z, tau.y}};
+    float D[2][2], Tn[2][2];
+    for (int r = 0; r < 2; r++)
+        for (int c = 0; c < 2; c++) {
+            float lt = L[r][0] * T[0][c] + L[r][1] * T[1][c];
+            float tl = T[r][0] * L[c][0] + T[r][1] * L[c][1];
+            D[r][c] = (lt + tl) + G * (L[r][c] + L[c][r]);
+        }
+    float k = 1.0f + dt / fmaxf_(lam, 1e-4f);
+    for (int r = 0; r < 2; r++)
+        for (int c = 0; c < 2; c++) Tn[r][c] = (T[r][c] + dt * D[r][c]) / k;
+    v4 t = {Tn[0][0], Tn[1][1], 0.5f * (Tn[0][1] + Tn[1][0]), 0.0f};
+    float tn = sqrtf((t.x * t.x + t.y * t.y) + t.z * t.z);
+    if (P->el_stress_max > 0.0f && tn > P->el_stress_max * G) {
+        float f = P->el_stress_max * G / tn;
+        t.x *= f;
+        t.y *= f;
+        t.z *= f;
+        t.w *= f;
+    }
+    int x1 = clampi((int)px, 1, s->W - 2), y1 = clampi((int)py, 1, s->H - 2);
+    float around = 0.0f;
+    for (int b = -1; b <= 1; b++)
+        for (int a = -1; a <= 1; a++) around += fminf_(s->cell[cidx(s, x1 + a, y1 + b)].x, 1.0f);
+    if (around < 1.0f) t = (v4){0.0f, 0.0f, 0.0f, 0.0f};
+    return t;
 }
 
 /* G2P of one face component (particles.comp:165-222). comp = 0: u faces, 1: v faces. */
@@ -517,6 +568,11 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
         if (th_on) ph = phase_of(p.t, thermal_of(s, row, p.s));
         frozen = ph.y > 0.5f || row_lnu(s, row, p.s, ph.x, th_on) >= P->ln_rigid;
     }
+    /* viscoelastic stress: a particle that has not run a substep starts unstressed
+     * (particles.comp:152-156) */
+    const int el_on = (P->caps & SP_CAP_ELASTIC) != 0u;
+    v4 tau = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (el_on && (p.kind & SP_KIND_FRESH) == 0u && s->mats[row].gr[1] > 0.0f) tau = s->pstress[id];
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C[4] = {p.c[0], p.c[1], p.c[2], p.c[3]};
@@ -547,6 +603,13 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                     }
                 }
             }
+        }
+        /* the stress over the substep, from the velocity gradient before the clamp
+         * (particles.comp:249-279) */
+        if (el_on) {
+            float G = s->mats[row].gr[1];
+            if (G > 0.0f) tau = stress_update(s, p.pos[0], p.pos[1], C, tau, G, s->mats[row].gr[2], dt);
+            s->pstress[id] = tau;
         }
         /* Stabilisation (particles.comp:281-285). */
         if (P->apic_clamp > 0.0f) {
@@ -869,6 +932,11 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
     g->CT = CT;
     g->ct = th_on ? s->P.chan_t : -1;
     g->cg = th_on ? group_heat_chan(s, s->mats[row].group) : -1;
+    /* the stress to scatter (particles.comp:983-984) */
+    g->ce = el_on && (tau.x != 0.0f || tau.y != 0.0f || tau.z != 0.0f || tau.w != 0.0f) ? s->P.chan_e : -1;
+    g->te[0] = tau.x / s->P.el_g * SP_STRESS_FIX;
+    g->te[1] = tau.y / s->P.el_g * SP_STRESS_FIX;
+    g->te[2] = tau.z / s->P.el_g * SP_STRESS_FIX;
 }
 
 static void stage_particles(or_sim* s, int gather) {
@@ -1064,6 +1132,12 @@ static void stage_grid_prep(or_sim* s) {
                 if (mu >= capv && level > 0.0f) nn.w = 1.0f;
                 mu = fminf_(mu, capv);
                 s->rh[i] = nn;
+            }
+            /* the cell's viscoelastic stress, by volume (grid_prep.comp:241-244) */
+            if (P->caps & SP_CAP_ELASTIC) {
+                float sc = P->el_g * P->inv_ppc / SP_STRESS_FIX;
+                s->estr[i] = (v4){(float)s->acc[k + P->chan_e] * sc, (float)s->acc[k + P->chan_e + 1] * sc,
+                                  (float)s->acc[k + P->chan_e + 2] * sc, 0.0f};
             }
             s->cell[i] = (v4){phi_f, rho, mu, level};
             s->comp[i] = (v4){rho, f[1], fam0, phi_t};
@@ -3029,6 +3103,39 @@ static void stage_rigid(or_sim* s, int mode) {
 
 /* ---- public entry points -------------------------------------------------------------- */
 
+/* ---- elastic.comp --------------------------------------------------------------------- */
+
+/* tau_xy at the lower-left corner of cell i (elastic.comp:15-18) */
+static float txy_node(const or_sim* s, int i) {
+    int W = s->W;
+    return 0.25f * (((s->estr[i].z + s->estr[i - 1].z) + s->estr[i - W].z) + s->estr[i - W - 1].z);
+}
+
+/* The divergence of the stress on the live faces, explicitly, cells 2 .. grid - 2
+ * (elastic.comp:20-38). Each cell writes only its own faces from cell and estr, which the
+ * pass does not change: the order of the cells does not matter. */
+static void stage_elastic(or_sim* s) {
+    int W = s->W, H = s->H;
+    float dt = s->S.dt;
+    for (int cy = 2; cy < H - 1; cy++)
+        for (int cx = 2; cx < W - 1; cx++) {
+            int i = cidx(s, cx, cy);
+            v4 q = s->vel[i];
+            v4 a = s->cell[i], l = s->cell[i - 1], b = s->cell[i - W];
+            if (face_live(q.z)) {
+                float rho = (a.x * a.y + l.x * l.y) / fmaxf_(a.x + l.x, 1e-6f);
+                float f = (s->estr[i].x - s->estr[i - 1].x) + (txy_node(s, i + W) - txy_node(s, i));
+                q.x += dt * f / fmaxf_(rho, 0.05f);
+            }
+            if (face_live(q.w)) {
+                float rho = (a.x * a.y + b.x * b.y) / fmaxf_(a.x + b.x, 1e-6f);
+                float f = (s->estr[i].y - s->estr[i - W].y) + (txy_node(s, i + 1) - txy_node(s, i));
+                q.y += dt * f / fmaxf_(rho, 0.05f);
+            }
+            s->vel[i] = q;
+        }
+}
+
 void or_stage(or_sim* s, int stage, int arg) {
     switch (stage) {
     case SP_STAGE_STEP: stage_step(s); break;
@@ -3056,6 +3163,7 @@ void or_stage(or_sim* s, int stage, int arg) {
     }
     case SP_STAGE_HEAT: stage_heat(s); break;
     case SP_STAGE_RHEO: stage_rheo(s); break;
+    case SP_STAGE_ELASTIC: stage_elastic(s); break;
     case SP_STAGE_RIGID:
         if (arg >= 0 && arg <= 4) stage_rigid(s, arg);
         break;
@@ -3079,6 +3187,7 @@ void or_substep(or_sim* s, int gather) {
     stage_particles(s, gather);
     stage_grid_prep(s);
     if (s->P.caps & SP_CAP_RHEO) stage_rheo(s);
+    if (s->P.caps & SP_CAP_ELASTIC) stage_elastic(s);
     stage_coef(s);
     stage_surf_smooth(s);
     stage_surf_curv(s);
@@ -3141,6 +3250,8 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->rl = (int32_t*)calloc(2 * (size_t)N, sizeof(int32_t));
     s->rbody = (uint32_t*)calloc(SP_RIGID_WORDS, sizeof(uint32_t));
     s->rh = (v4*)calloc(N, sizeof(v4));
+    s->estr = (v4*)calloc(N, sizeof(v4));
+    s->pstress = (v4*)calloc(capacity ? capacity : 1, sizeof(v4));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -3179,6 +3290,8 @@ void or_destroy(or_sim* s) {
     free(s->rl);
     free(s->rbody);
     free(s->rh);
+    free(s->estr);
+    free(s->pstress);
     free(s);
 }
 
@@ -3207,6 +3320,7 @@ void or_reset(or_sim* s, const sp_particle* p, uint32_t n) {
     memset(&s->S, 0, sizeof(s->S));
     memset(s->dheat, 0, sizeof(v4) * (size_t)s->N);
     memset(s->ledger, 0, sizeof(s->ledger));
+    memset(s->pstress, 0, sizeof(v4) * (s->cap ? s->cap : 1));
 }
 
 void or_begin_batch(or_sim* s, float target, uint32_t substeps) {
@@ -3272,6 +3386,8 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_RL: *bytes = 8 * (size_t)N; return s->rl;
     case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return s->rbody;
     case SP_FIELD_RH: *bytes = sizeof(v4) * (size_t)N; return s->rh;
+    case SP_FIELD_ESTR: *bytes = sizeof(v4) * (size_t)N; return s->estr;
+    case SP_FIELD_PSTRESS: *bytes = sizeof(v4) * (size_t)s->n_slots; return s->pstress;
     default: *bytes = 0; return NULL;
     }
 }
@@ -3312,15 +3428,25 @@ void or_sort(or_sim* s) {
     }
     for (size_t c = 0; c < N; c++) count[c + 1] += count[c];
     sp_particle* out = (sp_particle*)calloc(s->cap ? s->cap : 1, sizeof(sp_particle));
+    v4* sout = (v4*)calloc(s->cap ? s->cap : 1, sizeof(v4));
     uint32_t live = count[N];
+    /* the stress moves with its particle (sort.comp:66) */
     for (uint32_t id = 0; id < s->n_slots; id++)
-        if (cellof[id] >= 0) out[count[cellof[id]]++] = s->part[id];
+        if (cellof[id] >= 0) {
+            sout[count[cellof[id]]] = s->pstress[id];
+            out[count[cellof[id]]++] = s->part[id];
+        }
     /* dead slots keep their (dead) contents at the end */
     uint32_t k = live;
     for (uint32_t id = 0; id < s->n_slots; id++)
-        if (cellof[id] < 0) out[k++] = s->part[id];
+        if (cellof[id] < 0) {
+            sout[k] = s->pstress[id];
+            out[k++] = s->part[id];
+        }
     memcpy(s->part, out, sizeof(sp_particle) * s->n_slots);
+    if (s->P.caps & SP_CAP_ELASTIC) memcpy(s->pstress, sout, sizeof(v4) * s->n_slots);
     free(out);
+    free(sout);
     free(cellof);
     free(count);
 }

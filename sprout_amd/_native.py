@@ -30,6 +30,8 @@ ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
 OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
 RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
+CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_ELASTIC = 1, 2, 4, 8, 32
+STRESS_FIX = 65536.0
 
 # Field ids (SP_FIELD_*)
 FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7, MGA=8, MGD=9,
@@ -37,12 +39,12 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21)
+             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21, ELASTIC=22)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -61,7 +63,8 @@ class Material(C.Structure):
                 ("emissivity", C.c_float), ("boil", C.c_float), ("latent_vap", C.c_float),
                 ("freeze_shift", C.c_float), ("rheology", C.c_uint32), ("yield_stress", C.c_float),
                 ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
-                ("skin", C.c_float), ("friction", C.c_float)]
+                ("skin", C.c_float), ("friction", C.c_float), ("shear_modulus", C.c_float),
+                ("relax_time", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -84,6 +87,7 @@ class SimParams(C.Structure):
                 ("heat_rate", C.c_float), ("air_cooling", C.c_float), ("contact_heat", C.c_float),
                 ("eddy_heat", C.c_float), ("evaporation", C.c_uint32),
                 ("rigid_contact", C.c_uint32), ("rigid_friction", C.c_float),
+                ("elastic_stress_max", C.c_float), ("elastic_step_max", C.c_float),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
@@ -156,7 +160,8 @@ class GParams(C.Structure):
                 ("v_precond", C.c_int32), ("v_mg_levels", C.c_int32), ("v_mg_sweeps", C.c_int32),
                 ("mg_half", C.c_int32), ("heat_rate", C.c_float), ("film", C.c_float),
                 ("radiation", C.c_float), ("contact", C.c_float), ("eddy", C.c_float), ("evap", C.c_float),
-                ("chan_t", C.c_int32), ("pad_heat", C.c_int32)]
+                ("chan_t", C.c_int32), ("chan_e", C.c_int32), ("el_g", C.c_float),
+                ("el_stress_max", C.c_float), ("el_step_max", C.c_float)]
 
 
 class LevelsInfo(C.Structure):

@@ -56,6 +56,9 @@ struct Ctx {
     int2* rl;             // per cell: root cell of its body (-1: not rigid); at a root, its slot
     uint32_t* rbody;      // SP_RIGID_WORDS: bodies claimed, then 64-bit sums per slot
     float4* rh;           // per cell: non-Newtonian share, its row, its concentration; w = 1 where stiff
+    // viscoelasticity (elastic.comp)
+    float4* estr;         // per cell: stress xx, yy, xy (cells^2/s^2), scattered by volume
+    float4* pstress;      // per particle slot: its stress xx, yy, xy; moved with the particle by the sort
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -132,6 +135,8 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.rl = (int2*)take(sizeof(int2) * N);
     c.rbody = (uint32_t*)take(sizeof(uint32_t) * SP_RIGID_WORDS);
     c.rh = (float4*)take(sizeof(float4) * N);
+    c.estr = (float4*)take(sizeof(float4) * N);
+    c.pstress = (float4*)take(sizeof(float4) * (r.cap ? r.cap : 1));
     if (bytes) *bytes = o;
     return c;
 }

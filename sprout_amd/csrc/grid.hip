@@ -95,7 +95,8 @@ SP_DEV bool packed(const Ctx& c, int i) {
 }
 }  // namespace
 
-// grid_prep.comp:20-263, isothermal, without the CAP-gated fields.
+// grid_prep.comp:20-263, with the CAP-gated fields this build carries (heat, rigid bodies,
+// rheology, viscoelasticity).
 __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     int cx, cy, i;
@@ -264,6 +265,13 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
         if (mu >= cap && level > 0.0f) nn.w = 1.0f;
         mu = fminf_(mu, cap);
         c.rh[i] = nn;
+    }
+    // the viscoelastic stress of the cell: its particles' stress sums, by volume
+    // (grid_prep.comp:241-244)
+    if (P.caps & SP_CAP_ELASTIC) {
+        const float sc = P.el_g * P.inv_ppc / SP_STRESS_FIX;
+        c.estr[i] = make_float4((float)c.acc[k + P.chan_e] * sc, (float)c.acc[k + P.chan_e + 1] * sc,
+                                (float)c.acc[k + P.chan_e + 2] * sc, 0.0f);
     }
     c.cell[i] = make_float4(phi_f, rho, mu, level);
     c.comp[i] = make_float4(rho, f[1], fam0, phi_t);

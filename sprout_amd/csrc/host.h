@@ -20,6 +20,7 @@ struct Tank {
     bool thermal = false;                   // temperatures carried and conducted
     int chan_t = -1;                        // channel of the temperature sum
     std::array<int, 4> chan_g{-1, -1, -1, -1};  // per phase group after the first: its own sum
+    int chan_e = -1;                        // first of the three stress channels (-1: nothing elastic)
 };
 
 void default_params(sp_sim_params* p);

@@ -54,6 +54,8 @@ static sp_material default_material() {
     m.visc_ratio = 1.0f;
     m.skin = 0.0f;
     m.friction = 0.0f;
+    m.shear_modulus = 0.0f;  // material.rs:380-381
+    m.relax_time = 0.0f;
     return m;
 }
 
@@ -106,6 +108,8 @@ void default_params(sp_sim_params* p) {
     p->evaporation = 1;
     p->rigid_contact = 1;  // params.rs:400-402
     p->rigid_friction = 0.6f;
+    p->elastic_stress_max = 10.0f;  // params.rs:447-448
+    p->elastic_step_max = 0.25f;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -224,11 +228,24 @@ static float dt_thermal(const sp_sim_params& p, const std::vector<int>& rows) {
     return dt;
 }
 
-// FluidSim::dt_limit (mod.rs:2331-2338), without the elastic limit (nothing elastic here).
+// SimParams::dt_elastic (params.rs:951-960): elastic waves, up to sqrt(1 + cap) times the
+// shear-wave speed sqrt(G / rho), cross at most half a cell per substep (with no cap, as if
+// it were 50 G).
+static float dt_elastic(const sp_sim_params& p, const std::vector<int>& rows) {
+    const float stiff = 1.0f + (p.elastic_stress_max > 0.0f ? p.elastic_stress_max : 50.0f);
+    float dt = INFINITY;
+    for (int i : rows) {
+        const sp_material& m = p.mat[i];
+        if (m.shear_modulus > 0.0f) dt = std::min(dt, 0.5f * p.cell_size / sqrtf(stiff * m.shear_modulus / m.density));
+    }
+    return dt;
+}
+
+// FluidSim::dt_limit (mod.rs:2331-2338).
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal) {
     float dt = std::min(p.dt_max, dt_capillary(p, rows));
     if (thermal) dt = std::min(dt, dt_thermal(p, rows));
-    return dt;
+    return std::min(dt, dt_elastic(p, rows));
 }
 
 std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows) {
@@ -268,7 +285,8 @@ std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present
     return rows;
 }
 
-// Tank::layout (mod.rs:271-331), isothermal and without elastic channels.
+// Tank::layout (mod.rs:271-331): the materials' channels, then the temperature sums of a
+// thermal tank, then the three stress channels while an elastic material is present.
 bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal) {
     if ((int)rows.size() > SP_MAX_PRESENT) return false;
     t = Tank();
@@ -305,6 +323,12 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
                 first = false;
             }
     }
+    bool elastic = false;
+    for (int r : rows) elastic |= p.mat[r].shear_modulus > 0.0f;
+    if (elastic) {
+        t.chan_e = next;
+        next += 3;
+    }
     if (next > SP_ACC_MAX) return false;
     int stride = std::max(next, SP_ACC_MIN);
     stride = (stride + 3) / 4 * 4;
@@ -316,7 +340,7 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
 }
 
 // FluidSim::caps (mod.rs:2717-2766) for the CAP-gated passes this build carries: families,
-// heat, rigid bodies and rheology. A non-Newtonian liquid, or one with a skin or internal
+// heat, rigid bodies, rheology and viscoelasticity (stress channels in the layout). A non-Newtonian liquid, or one with a skin or internal
 // friction, turns rheology on, and rigid bodies for a yield stress, shear thickening, a skin
 // or friction. A thermal tank turns rigid bodies on for any liquid that can freeze or has a
 // temperature law; any tank for a liquid already as viscous as rigid_viscosity, or frozen at
@@ -338,6 +362,7 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
         rigid |= viscosity_at(m, p.ambient_temp) >= p.rigid_viscosity || (!t.thermal && freezes(m) && p.ambient_temp < m.melt_hi);
     }
     if (rigid) caps |= SP_CAP_RIGID;
+    if (t.chan_e >= 0) caps |= SP_CAP_ELASTIC;
     return caps;
 }
 
@@ -412,6 +437,9 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.rheo_p[2] = m.shear_time;
         o.rheo_p[3] = m.visc_ratio;
         o.gr[0] = m.friction;
+        // viscoelasticity (mod.rs:2215): the shear modulus as a stress per unit density
+        o.gr[1] = m.shear_modulus / (WATER_DENSITY * dx * dx);
+        o.gr[2] = m.relax_time;
     }
     const uint32_t caps = tank_caps(p, t);
     // gpu_params (mod.rs:2238-2330).
@@ -500,6 +528,13 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.eddy = std::max(p.eddy_heat, 0.0f);
     g.evap = p.evaporation != 0 ? 2.257e6f / 1005.0f : 0.0f;
     g.chan_t = t.chan_t;
+    // viscoelasticity (mod.rs:2316-2321)
+    float gmax = 0.0f;
+    for (int r : t.rows) gmax = std::max(gmax, p.mat[r].shear_modulus);
+    g.chan_e = t.chan_e;
+    g.el_g = std::max(gmax, 1e-3f) / (WATER_DENSITY * dx * dx);
+    g.el_stress_max = p.elastic_stress_max;
+    g.el_step_max = p.elastic_step_max;
     return SP_OK;
 }
 

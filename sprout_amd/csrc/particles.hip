@@ -18,7 +18,18 @@ struct ScatterIn {
     bool live;
     float px, py, vsx, vsy, Csx, Csy, Csz, Csw, rho, sc, CT;
     int cw, cs, ct, cg;
+    int ce;  // first stress channel, -1: no stress to scatter (the stress is then in the slot)
 };
+
+// The stress a particle scatters, over el_g in STRESS_FIX units (particles.comp:984). With
+// g.ce >= 0 the particle's slot holds it: the gather wrote it, or, on a substep without a
+// gather, the particle read it from there.
+SP_DEV float3 scatter_stress(const Ctx& c, const ScatterIn& g, uint32_t id) {
+    if (g.ce < 0) return make_float3(0.0f, 0.0f, 0.0f);
+    const float4 t = c.pstress[id];
+    const float e = c.P->el_g;
+    return make_float3(t.x / e * SP_STRESS_FIX, t.y / e * SP_STRESS_FIX, t.z / e * SP_STRESS_FIX);
+}
 
 SP_DEV int cidx(const Ctx& c, int x, int y) { return y * c.W + x; }
 
@@ -101,6 +112,52 @@ SP_DEV bool collide(const Ctx& c, float& x, float& y, float& vx, float& vy, bool
     return false;
 }
 
+// The upper-convected Maxwell stress of one particle over a substep (particles.comp:249-279):
+// dtau/dt = L tau + tau L^T - tau / lambda + G (L + L^T), L the velocity gradient (the APIC
+// matrix C = du/dx, du/dy, dv/dx, dv/dy) with at most el_step_max deformation per substep,
+// relaxation implicit, the norm capped at el_stress_max G; no stress without a cell's worth
+// of liquid in the 3 x 3 cells around the particle. tau = (xx, yy, xy, 0).
+SP_DEV float4 stress_update(const Ctx& c, float px, float py, float C0, float C1, float C2, float C3, float4 tau, float G,
+                            float lam, float dt) {
+    const sp_gparams& P = *c.P;
+    float a = C0, b = C1, cc = C2, d = C3;
+    const float lnorm = sqrtf(((C0 * C0 + C1 * C1) + C2 * C2) + C3 * C3) * dt;
+    if (P.el_step_max > 0.0f && lnorm > P.el_step_max) {
+        const float k = P.el_step_max / lnorm;
+        a *= k;
+        b *= k;
+        cc *= k;
+        d *= k;
+    }
+    const float X = tau.x, Y = tau.y, Z = tau.z;
+    // L T; T L^T is its transpose (T is symmetric)
+    const float lt00 = a * X + b * Z, lt01 = a * Z + b * Y, lt10 = cc * X + d * Z, lt11 = cc * Z + d * Y;
+    const float D00 = (lt00 + lt00) + G * (a + a);
+    const float D11 = (lt11 + lt11) + G * (d + d);
+    const float D01 = (lt01 + lt10) + G * (b + cc);
+    const float D10 = (lt10 + lt01) + G * (cc + b);
+    const float k = 1.0f + dt / fmaxf_(lam, 1e-4f);
+    const float T00 = (X + dt * D00) / k, T11 = (Y + dt * D11) / k;
+    const float T01 = (Z + dt * D01) / k, T10 = (Z + dt * D10) / k;
+    float4 t = make_float4(T00, T11, 0.5f * (T10 + T01), 0.0f);
+    const float tn = sqrtf((t.x * t.x + t.y * t.y) + t.z * t.z);
+    if (P.el_stress_max > 0.0f && tn > P.el_stress_max * G) {
+        const float s = P.el_stress_max * G / tn;
+        t.x *= s;
+        t.y *= s;
+        t.z *= s;
+        t.w *= s;
+    }
+    const int x1 = clampi_((int)px, 1, c.W - 2), y1 = clampi_((int)py, 1, c.H - 2);
+    float around = 0.0f;
+#pragma unroll
+    for (int bb = -1; bb <= 1; bb++)
+#pragma unroll
+        for (int aa = -1; aa <= 1; aa++) around += fminf_(c.cell[cidx(c, x1 + aa, y1 + bb)].x, 1.0f);
+    if (around < 1.0f) t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return t;
+}
+
 // APIC gather of one face component over its 3 x 3 live faces (particles.comp:165-222).
 SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, float& cr, float& Ca, float& Cb) {
     float gx = comp == 0 ? px : px - 0.5f, gy = comp == 0 ? py - 0.5f : py;
@@ -139,7 +196,9 @@ SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, fl
 }
 
 // Gather, drift, exchange, advect and collide one particle (particles.comp:98-985, the
-// branches this build carries); fills the scatter inputs.
+// branches this build carries); fills the scatter inputs. EL: the tank may be elastic (the
+// stress code is compiled only into the kernels elastic tanks run).
+template <bool EL>
 SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& umax) {
     const sp_gparams& P = *c.P;
     g.live = false;
@@ -160,6 +219,11 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
         const float2 ph = th_on ? phase_of(p.t, thermal_of(c, row, p.s)) : make_float2(P.heat_x, 0.0f);
         frozen = ph.y > 0.5f || row_lnu(c, row, p.s, ph.x, th_on) >= P.ln_rigid;
     }
+    // viscoelastic stress: a particle that has not run a substep yet starts unstressed (its
+    // slot may hold an old stress) (particles.comp:152-156)
+    const bool el_on = EL && (P.caps & SP_CAP_ELASTIC) != 0u;
+    float4 tau = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (el_on && (p.kind & SP_KIND_FRESH) == 0u && c.mats[row].gr[1] > 0.0f) tau = c.pstress[id];
     if (gather) {
         float vx = p.vel[0], vy = p.vel[1];
         float C0 = p.c[0], C1 = p.c[1], C2 = p.c[2], C3 = p.c[3];
@@ -194,6 +258,11 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                     }
                 }
             }
+        }
+        if (el_on) {
+            const float G = c.mats[row].gr[1];
+            if (G > 0.0f) tau = stress_update(c, p.pos[0], p.pos[1], C0, C1, C2, C3, tau, G, c.mats[row].gr[2], dt);
+            c.pstress[id] = tau;
         }
         if (P.apic_clamp > 0.0f) {
             float lim = P.apic_clamp / dt;
@@ -511,12 +580,14 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     g.CT = CT;
     g.ct = th_on ? P.chan_t : -1;
     g.cg = th_on ? group_heat_chan(P, c.mats[row].group) : -1;
+    // whether there is stress to scatter (particles.comp:983)
+    g.ce = el_on && (tau.x != 0.0f || tau.y != 0.0f || tau.z != 0.0f || tau.w != 0.0f) ? P.chan_e : -1;
 }
 
 // The scatter of one particle (particles.comp:997-1062) through `add(node x, node y,
 // channel, value)`.
 template <class Add>
-SP_DEV void deposit(const ScatterIn& g, Add add) {
+SP_DEV void deposit(const ScatterIn& g, float3 te, Add add) {
     float wx[3], wy[3];
     {
         float gx = g.px, gy = g.py - 0.5f;
@@ -570,6 +641,13 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
                     add(bx + a, by + b, g.ct, h);
                     if (g.cg >= 0) add(bx + a, by + b, g.cg, h);
                 }
+                // viscoelastic stress over el_g, fixed point (particles.comp:1053-1058)
+                if (g.ce >= 0) {
+                    const float wab = wx[a] * wy[b];
+                    add(bx + a, by + b, g.ce, (int)floorf(wab * te.x + 0.5f));
+                    add(bx + a, by + b, g.ce + 1, (int)floorf(wab * te.y + 0.5f));
+                    add(bx + a, by + b, g.ce + 2, (int)floorf(wab * te.z + 0.5f));
+                }
             }
     }
 }
@@ -577,11 +655,11 @@ SP_DEV void deposit(const ScatterIn& g, Add add) {
 }  // namespace
 
 // The particle pass (particles.comp:1064-1101), NT work-items per group, a scatter box of
-// BOX ints in LDS, registers for at least MINB groups per compute unit. Every (NT, BOX) gives the same result (integer sums; the speed maximum is
+// BOX ints in LDS, registers for at least MINB groups per compute unit; EL for elastic tanks. Every (NT, BOX) gives the same result (integer sums; the speed maximum is
 // a maximum), so the launch shape is free to tune.
 // gather < 0: each tank decides from its own step state (the first substep after a reset,
 // which k_step has just counted, has no grid velocities to gather yet).
-template <int NT, int BOX, int MINB>
+template <int NT, int BOX, int MINB, bool EL>
 __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int gather) {
     static_assert(NT % 64 == 0 && NT <= 1024 && BOX * 4 <= 64 * 1024, "launch shape");
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
@@ -593,7 +671,7 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
     ScatterIn g;
     float umax = 0.0f;
     __shared__ float s_max[NT / 64];
-    advance(c, id, gather, g, umax);
+    advance<EL>(c, id, gather, g, umax);
     group_max_to<NT>(&c.S->umax_part, umax, s_max);
 
     if (t == 0) {
@@ -624,7 +702,8 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
         for (int k = t; k < n; k += NT) sacc[k] = 0;
         __syncthreads();
         if (g.live)
-            deposit(g, [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * stride + ch], v); });
+            deposit(g, scatter_stress(c, g, id),
+                    [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * stride + ch], v); });
         __syncthreads();
         for (int k = t; k < n; k += NT) {
             int v = sacc[k];
@@ -634,7 +713,7 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
             atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + (k - cell * stride)], v);
         }
     } else if (g.live) {
-        deposit(g, [&](int x, int y, int ch, int v) { atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + ch], v); });
+        deposit(g, scatter_stress(c, g, id), [&](int x, int y, int ch, int v) { atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + ch], v); });
     }
 }
 
@@ -665,11 +744,15 @@ __global__ __launch_bounds__(256) void k_sort_keys(const TankBatch tb, uint32_t*
     vals[id] = id;
 }
 
-__global__ __launch_bounds__(256) void k_sort_gather(const TankBatch tb, const uint32_t* vals, sp_particle* out, uint32_t n) {
+// The particles gathered into their new order, and with `sout` their stress (sort.comp:66).
+__global__ __launch_bounds__(256) void k_sort_gather(const TankBatch tb, const uint32_t* vals, sp_particle* out, float4* sout,
+                                                     uint32_t n) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     uint32_t j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
-    out[j] = c.part[vals[j]];
+    const uint32_t from = vals[j];
+    out[j] = c.part[from];
+    if (sout) sout[j] = c.pstress[from];
 }
 
 // Live particles per cell in a window (count.comp:17-23).

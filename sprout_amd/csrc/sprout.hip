@@ -24,6 +24,7 @@
 #include "visc.hip"
 #include "heat.hip"
 #include "rigid.hip"
+#include "elastic.hip"
 #include "schedule.hip"
 
 namespace {
@@ -72,6 +73,7 @@ struct sp_sim {
     Ctx c{};                  // its buffers (host copy of what every kernel rebuilds)
     Run run;                  // the launch group of this tank alone
     sp_particle* d_part_tmp = nullptr;
+    float4* d_stress_tmp = nullptr;  // the sort's target for the particles' stress
     sp_particle* d_spawn = nullptr;
     uint32_t* d_counter = nullptr;  // stats / claim counters
     uint32_t* d_count_out = nullptr;
@@ -136,9 +138,13 @@ void launch_sort(sp_sim* s, int tiled, hipStream_t stream) {
     size_t bytes = s->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(s->d_sort_tmp, bytes, s->d_keys, s->d_keys2, s->d_vals, s->d_vals2,
                                               (int)s->n_slots, 0, end_bit, stream));
+    // the stress moves with its particle while the tank is elastic (record_sort, mod.rs:3238-3240)
+    const bool elastic = (s->gp.caps & SP_CAP_ELASTIC) != 0u;
     hipLaunchKernelGGL(k_sort_gather, dim3(blocks), dim3(256), 0, stream, s->run.tb, (const uint32_t*)s->d_vals2,
-                       s->d_part_tmp, s->n_slots);
+                       s->d_part_tmp, elastic ? s->d_stress_tmp : nullptr, s->n_slots);
     HIPCHK(hipMemcpyAsync(s->c.part, s->d_part_tmp, sizeof(sp_particle) * s->n_slots, hipMemcpyDeviceToDevice, stream));
+    if (elastic)
+        HIPCHK(hipMemcpyAsync(s->c.pstress, s->d_stress_tmp, sizeof(float4) * s->n_slots, hipMemcpyDeviceToDevice, stream));
 }
 
 // Uploads the grid-unit tables if the parameters or the tank changed.
@@ -181,6 +187,7 @@ void alloc_big(sp_sim* s) {
         TankRef ref = s->ref;
         ref.base = dalloc<char>(s->arena_bytes, got);
         s->d_part_tmp = dalloc<sp_particle>(s->cap, got);
+        s->d_stress_tmp = dalloc<float4>(s->cap, got);
         s->d_keys = dalloc<uint32_t>(s->cap, got);
         s->d_vals = dalloc<uint32_t>(s->cap, got);
         s->d_keys2 = dalloc<uint32_t>(s->cap, got);
@@ -205,7 +212,8 @@ void free_big(sp_sim* s) {
 
 // Bytes of device memory the tank gives back when evicted.
 size_t big_bytes(const sp_sim* s) {
-    return s->arena_bytes + (sizeof(sp_particle) + 16) * (size_t)std::max<uint32_t>(s->cap, 1) + std::max<size_t>(s->sort_tmp_bytes, 1);
+    return s->arena_bytes + (sizeof(sp_particle) + sizeof(float4) + 16) * (size_t)std::max<uint32_t>(s->cap, 1) +
+           std::max<size_t>(s->sort_tmp_bytes, 1);
 }
 
 // Evicts the tank: its arena goes to pinned host memory and its evictable device memory is
@@ -404,6 +412,14 @@ int guarded(sp_sim* s, F&& f, bool device = true) {
     }
 }
 
+// Takes a new layout. A tank that turns elastic gives its particles a zeroed stress
+// (set_elastic, mod.rs:2082-2100): particles already in it start unstressed.
+void adopt_tank(sp_sim* s, const sp::Tank& t) {
+    if (t.chan_e >= 0 && s->tank.chan_e < 0 && s->ref.base)
+        HIPCHK(hipMemsetAsync(s->c.pstress, 0, sizeof(float4) * (size_t)std::max<uint32_t>(s->cap, 1), s->stream));
+    s->tank = t;
+}
+
 // The rows the particles carry, with their diluents, and whether the tank becomes thermal
 // (note_materials, mod.rs:2051-2081): a particle more than 0.05 K off the ambient
 // temperature makes it so.
@@ -421,7 +437,7 @@ bool note_materials(sp_sim* s, const sp_particle* p, uint32_t n, bool reset) {
     if (rows == s->tank.rows && thermal == s->thermal && !(reset && s->tank.rows.empty())) return true;
     sp::Tank t;
     if (!sp::tank_layout(s->params, rows, t, thermal)) return false;
-    s->tank = t;
+    adopt_tank(s, t);
     s->thermal = thermal;
     s->tables_dirty = true;
     return true;
@@ -588,7 +604,7 @@ int sp_set_params(sp_sim* s, const sp_sim_params* p) {
         // the tank's layout follows the table (diluents may have changed)
         std::vector<int> rows = sp::closure(s->params, s->tank.rows);
         sp::Tank t;
-        if (sp::tank_layout(s->params, rows, t)) s->tank = t;
+        if (sp::tank_layout(s->params, rows, t, s->thermal)) adopt_tank(s, t);
         return SP_OK;
     });
 }
@@ -620,7 +636,7 @@ int sp_set_solids(sp_sim* s, const float* sdf, const float* solid, const uint32_
                     set_err(s, "the accumulators have no room for the temperature sums");
                     return SP_E_CAPACITY;
                 }
-                s->tank = t;
+                adopt_tank(s, t);
                 s->thermal = true;
                 s->tables_dirty = true;
             }
@@ -664,6 +680,7 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         HIPCHK(hipMemsetAsync(s->c.S, 0, sizeof(sp_stepbuf), s->stream));
         HIPCHK(hipMemsetAsync(s->c.dheat, 0, sizeof(float4) * (size_t)s->N, s->stream));
         HIPCHK(hipMemsetAsync(s->c.ledger, 0, sizeof(uint32_t) * SP_LEDGER_WORDS, s->stream));
+        HIPCHK(hipMemsetAsync(s->c.pstress, 0, sizeof(float4) * (size_t)std::max<uint32_t>(s->cap, 1), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->pending_spawn.clear();
         s->time_base = s->sim_time;
@@ -918,7 +935,7 @@ int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* e
         (void)hipGetLastError();
         sort_tmp = 8 * cap + (1u << 20);
     }
-    *evictable = arena + (sizeof(sp_particle) + 16) * cap + std::max<size_t>(sort_tmp, 1);
+    *evictable = arena + (sizeof(sp_particle) + sizeof(float4) + 16) * cap + std::max<size_t>(sort_tmp, 1);
     *fixed = sizeof(sp_particle) * SP_SPAWN_MAX + sizeof(uint32_t) * (8 + 2 * SP_MAX_MATERIALS) +
              sizeof(uint32_t) * (size_t)grid_w * grid_h;
     return SP_OK;
@@ -1031,6 +1048,8 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_RL: *bytes = 8 * N; return c.rl;
     case SP_FIELD_RBODY: *bytes = 4 * (size_t)SP_RIGID_WORDS; return c.rbody;
     case SP_FIELD_RH: *bytes = 16 * N; return c.rh;
+    case SP_FIELD_ESTR: *bytes = 16 * N; return c.estr;
+    case SP_FIELD_PSTRESS: *bytes = 16 * (size_t)s->n_slots; return c.pstress;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1130,6 +1149,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         case SP_STAGE_RHS: hipLaunchKernelGGL(k_rhs, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_HEAT: hipLaunchKernelGGL(k_heat, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_RHEO: hipLaunchKernelGGL(k_rheo, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_ELASTIC: hipLaunchKernelGGL(k_elastic, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_RIGID:
             if (arg < 0 || arg > 4) return SP_E_INVALID;
             hipLaunchKernelGGL(k_rigid, g0, TILE, 0, s->stream, cs, arg);

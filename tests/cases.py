@@ -217,3 +217,48 @@ def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), st
     field = scene.rasterize()
     parts = settle(parts, p)
     return p, field, parts, [N.ROW_WATER, N.ROW_HONEY, N.ROW_OIL]
+
+
+ROW_SLIME = 3  # a built-in row with the default material, made viscoelastic by slime_params
+
+
+def slime_params(params, row=ROW_SLIME, shear_modulus=100.0, relax_time=1.0, viscosity=1.0e-2, density=1030.0,
+                 tension=0.06):
+    """Makes `row` the reference's slime (presets.rs:428-445): a Maxwell liquid, G 100 Pa,
+    relaxation 1 s, over a 0.01 m^2/s solvent, 1030 kg/m^3 (isothermal fields only)."""
+    m = params.mat[row]
+    m.shear_modulus = shear_modulus
+    m.relax_time = relax_time
+    m.viscosity = viscosity
+    m.density = density
+    m.tension = tension
+    return params
+
+
+def slime_drop(grid, block=(0.35, 0.65, 0.35, 0.75), cell_size=0.005, pressure_tol=1e-6, **mat):
+    """A block of slime falling onto the floor of an empty tank (the reference's slime drop),
+    isothermal. `block` in fractions of W and H. (params, solid field, particles, rows)."""
+    params = slime_params(dam_params(cell_size=cell_size, pressure_tol=pressure_tol), **mat)
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (block[0] * W, block[2] * H, block[1] * W, block[3] * H), ROW_SLIME)
+    return params, scene.rasterize(), settle(parts, params), [ROW_SLIME]
+
+
+def shear_layer(grid=(96, 48), y0=16.0, h=16.0, x=(8.0, 88.0), amp=2.0, cell_size=0.01, **mat):
+    """The fundamental shear mode of a free slime layer in zero gravity (the reference's
+    viscoelastic check, checks.rs:456-503): a layer y0 <= y < y0 + h of the tank, x[0] <= x <
+    x[1], whose particles move as u = amp cos(k (y - y0)), k = pi / h, v = 0 (cells / s), with
+    the matching APIC gradient du/dy. No surface tension; adaptive substeps (no fixed rate).
+    (params, solid field, particles, rows, k)."""
+    params = slime_params(dam_params(cell_size=cell_size, fixed_hz=0.0), tension=0.0, **mat)
+    params.gravity = 0.0
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (x[0], y0, x[1], y0 + h), ROW_SLIME)
+    k = np.pi / h
+    yy = parts["pos"][:, 1].astype(np.float64) - y0
+    parts["vel"][:, 0] = (amp * np.cos(k * yy)).astype(np.float32)
+    parts["vel"][:, 1] = 0.0
+    parts["c"][:] = 0.0
+    parts["c"][:, 1] = (-amp * k * np.sin(k * yy)).astype(np.float32)
+    return params, scene.rasterize(), settle(parts, params), [ROW_SLIME], k

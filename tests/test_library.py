@@ -38,6 +38,30 @@ def test_struct_layouts():
     assert N.STEP_DTYPE.itemsize == 48      # 12 words (mod.rs:123 STEP_WORDS)
 
 
+def test_python_structs_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives the header's
+    struct (a field added on one side only shifts everything after it)."""
+    import subprocess
+    pairs = {"sp_material": N.Material, "sp_sim_params": N.SimParams, "sp_gmat": N.GMat,
+             "sp_gparams": N.GParams, "sp_solver_stats": N.SolverStats, "sp_levels_info": N.LevelsInfo,
+             "sp_server_stats": N.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sprout_hip.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, py in pairs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
+
+
 def test_errors_cross_the_abi_as_codes():
     L = N.lib()
     h = C.c_void_p()
