@@ -119,6 +119,54 @@ int sp_stats(sp_sim* sim, sp_stats_out* out);
  * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
 int sp_sort(sp_sim* sim);
 
+/* ---- render-facing outputs ------------------------------------------------------------- */
+
+/* The buffers a renderer reads (FluidSim::particle_buffer, sdf_buffer, cell_buffer,
+ * composition_buffer, surface_buffer, material_buffer; src/sim/mod.rs:1608-1940, read by
+ * render.rs:639, 817-819 and record_cell_optics). */
+enum {
+    SP_RENDER_PARTICLES = 0,    /* sp_particle[n_slots] (particle_buffer; dead slots included) */
+    SP_RENDER_SDF = 1,          /* float[(W+1)(H+1)], node SDF of the solids (sdf_buffer) */
+    SP_RENDER_CELL = 2,         /* float4[W*H]: occupancy, density, viscosity, level (cell_buffer) */
+    SP_RENDER_COMPOSITION = 3,  /* float4[W*H]: density, oil share, family-0 concentration, fill (composition_buffer) */
+    SP_RENDER_SURFACE = 4,      /* float4[W*H]: tension, contact normal, group + (row + 1) / 128 (surface_buffer) */
+    SP_RENDER_MATERIALS = 5,    /* sp_gmat[SP_MAX_MATERIALS] (material_buffer, this library's row layout) */
+    SP_RENDER_VELOCITY = 6,     /* float4[W*H]: u, v and the faces' states (the grid's velocity) */
+    SP_RENDER_COUNT
+};
+
+/* Where a render buffer lives: a device address on `device`, valid until the tank's memory
+ * moves. `generation` changes whenever it does (eviction and the restore on the next call,
+ * destruction), so a renderer that keeps the address checks it once per frame. */
+typedef struct sp_device_view {
+    void* ptr;
+    uint64_t bytes;
+    uint32_t count;      /* elements */
+    uint32_t stride;     /* bytes per element */
+    int32_t device;
+    uint32_t pad;
+    uint64_t generation;
+} sp_device_view;
+
+/* The device view of render buffer `which` (SP_RENDER_*), restoring an evicted tank first.
+ * The contents are what the last finished substep left: read after sp_wait / a completed
+ * sp_poll, or on the tank's own stream (sp_stream). Replaces the Subbuffer accessors of
+ * src/sim/mod.rs:1608-1940 for a renderer on the same device (HIP, or Vulkan through
+ * external memory). */
+int sp_render_view(sp_sim* sim, int32_t which, sp_device_view* out);
+/* A host copy of render buffer `which`, for a renderer elsewhere: bytes must be at least
+ * the view's size. Waits for the tank's work first. */
+int sp_render_download(sp_sim* sim, int32_t which, void* out, size_t bytes);
+/* The hipStream_t the tank's work runs on, as a void pointer (for ordering a renderer's
+ * reads after the step without a host wait). */
+int sp_stream(sp_sim* sim, void** out);
+/* FluidSim::resets (src/sim/mod.rs:1941-1943): how many times the tank was emptied and
+ * refilled; what a renderer remembers from frame to frame is void when it changes. */
+int sp_resets(sp_sim* sim, uint64_t* out);
+/* FluidSim::shares_a_group (src/sim/mod.rs:1872-1880): whether two liquids of one phase
+ * group are in the tank (the renderer then tints from the particles). */
+int sp_shares_a_group(sp_sim* sim, int32_t* out);
+
 /* ---- batches of tanks ------------------------------------------------------------------ */
 
 /* No counterpart in the reference, which steps one tank per FluidSim: the batched workload.

@@ -164,6 +164,14 @@ class GParams(C.Structure):
                 ("el_stress_max", C.c_float), ("el_step_max", C.c_float)]
 
 
+class DeviceView(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("bytes", C.c_uint64), ("count", C.c_uint32), ("stride", C.c_uint32),
+                ("device", C.c_int32), ("pad", C.c_uint32), ("generation", C.c_uint64)]
+
+
+RENDER = dict(PARTICLES=0, SDF=1, CELL=2, COMPOSITION=3, SURFACE=4, MATERIALS=5, VELOCITY=6)
+
+
 class LevelsInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("tail", C.c_int32), ("total", C.c_int32), ("tiles", C.c_int32),
                 ("dims", (C.c_int32 * 2) * MAX_LEVELS), ("offs", C.c_int32 * MAX_LEVELS),
@@ -214,6 +222,11 @@ SIGNATURES = {
     "sp_cell_counts": ([_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P], C.c_int),
     "sp_stats": ([_P, C.POINTER(StatsOut)], C.c_int),
     "sp_sort": ([_P], C.c_int),
+    "sp_render_view": ([_P, C.c_int32, _P], C.c_int),
+    "sp_render_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
+    "sp_stream": ([_P, C.POINTER(C.c_void_p)], C.c_int),
+    "sp_resets": ([_P, C.POINTER(C.c_uint64)], C.c_int),
+    "sp_shares_a_group": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "sp_default_params": ([C.POINTER(SimParams)], None),
     "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32,
                                C.POINTER(GParams), _P], C.c_int),

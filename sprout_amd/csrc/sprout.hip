@@ -57,6 +57,7 @@ struct sp_sim {
     void* host_arena = nullptr;
     size_t arena_bytes = 0;
     uint64_t generation = 0;  // bumped when the arena moves
+    uint64_t resets = 0;      // sp_reset calls (FluidSim::resets)
 
     sp_sim_params params{};
     bool params_set = false;
@@ -683,6 +684,7 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         HIPCHK(hipMemsetAsync(s->c.pstress, 0, sizeof(float4) * (size_t)std::max<uint32_t>(s->cap, 1), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->pending_spawn.clear();
+        s->resets++;
         s->time_base = s->sim_time;
         s->substeps_base = s->stats.substeps;
         s->grid_valid = false;
@@ -902,6 +904,93 @@ int sp_sort(sp_sim* s) {
         HIPCHK(hipStreamSynchronize(s->stream));
         return SP_OK;
     });
+}
+
+namespace {
+
+// The render buffer `which` of a resident tank: address, element count and size.
+int render_buffer(sp_sim* s, int32_t which, void** ptr, uint32_t* count, uint32_t* stride) {
+    const Ctx& c = s->c;
+    switch (which) {
+    case SP_RENDER_PARTICLES: *ptr = c.part; *count = s->n_slots; *stride = sizeof(sp_particle); break;
+    case SP_RENDER_SDF: *ptr = (void*)c.sdf; *count = (uint32_t)((s->W + 1) * (s->H + 1)); *stride = sizeof(float); break;
+    case SP_RENDER_CELL: *ptr = c.cell; *count = (uint32_t)s->N; *stride = sizeof(float4); break;
+    case SP_RENDER_COMPOSITION: *ptr = c.comp; *count = (uint32_t)s->N; *stride = sizeof(float4); break;
+    case SP_RENDER_SURFACE: *ptr = c.sprop; *count = (uint32_t)s->N; *stride = sizeof(float4); break;
+    case SP_RENDER_MATERIALS: {
+        // the table the kernels read, current with the parameters
+        const int rc = upload_tables(s);
+        if (rc != SP_OK) return rc;
+        *ptr = (void*)c.mats;
+        *count = SP_MAX_MATERIALS;
+        *stride = sizeof(sp_gmat);
+        break;
+    }
+    case SP_RENDER_VELOCITY: *ptr = c.vel; *count = (uint32_t)s->N; *stride = sizeof(float4); break;
+    default: set_err(s, "unknown render buffer %d", which); return SP_E_INVALID;
+    }
+    return SP_OK;
+}
+
+}  // namespace
+
+int sp_render_view(sp_sim* s, int32_t which, sp_device_view* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        void* p = nullptr;
+        uint32_t count = 0, stride = 0;
+        const int rc = render_buffer(s, which, &p, &count, &stride);
+        if (rc != SP_OK) return rc;
+        *out = sp_device_view{p, (uint64_t)count * stride, count, stride, s->device, 0u, s->generation};
+        return SP_OK;
+    });
+}
+
+int sp_render_download(sp_sim* s, int32_t which, void* out, size_t bytes) {
+    return guarded(s, [&]() -> int {
+        void* p = nullptr;
+        uint32_t count = 0, stride = 0;
+        const int rc = render_buffer(s, which, &p, &count, &stride);
+        if (rc != SP_OK) return rc;
+        const size_t n = (size_t)count * stride;
+        if (!out || bytes < n) {
+            set_err(s, "render buffer %d needs %zu bytes (%zu given)", which, n, bytes);
+            return SP_E_INVALID;
+        }
+        if (n) HIPCHK(hipMemcpyAsync(out, p, n, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return SP_OK;
+    });
+}
+
+int sp_stream(sp_sim* s, void** out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        *out = (void*)s->stream;
+        return SP_OK;
+    }, false);
+}
+
+int sp_resets(sp_sim* s, uint64_t* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        *out = s->resets;
+        return SP_OK;
+    }, false);
+}
+
+// FluidSim::shares_a_group (mod.rs:1872-1880); no steam row is carried here.
+int sp_shares_a_group(sp_sim* s, int32_t* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        int seen[8] = {0};
+        *out = 0;
+        for (int r : s->tank.rows) {
+            const int g = std::min(std::max(s->params.mat[r].group, 0), 7);
+            if (++seen[g] >= 2) *out = 1;
+        }
+        return SP_OK;
+    }, false);
 }
 
 int sp_evict(sp_sim* s) {

@@ -146,6 +146,37 @@ class FluidSim:
     def sort(self):
         self._check(self._lib.sp_sort(self._h), "sort")
 
+    # -- render-facing outputs (mod.rs:1608-1943) ---------------------------------------
+    def render_view(self, which):
+        """The device view (address, bytes, count, stride, device, generation) of a render
+        buffer: "PARTICLES", "SDF", "CELL", "COMPOSITION", "SURFACE", "MATERIALS", "VELOCITY"."""
+        v = N.DeviceView()
+        self._check(self._lib.sp_render_view(self._h, N.RENDER[which], C.byref(v)), f"render_view {which}")
+        return v
+
+    def render_download(self, which):
+        """A host copy of a render buffer, as bytes-shaped numpy array of its elements."""
+        v = self.render_view(which)
+        out = np.zeros(v.bytes, np.uint8)
+        self._check(self._lib.sp_render_download(self._h, N.RENDER[which], out.ctypes.data if v.bytes else None, out.nbytes),
+                    f"render_download {which}")
+        return out
+
+    def stream(self):
+        h = C.c_void_p()
+        self._check(self._lib.sp_stream(self._h, C.byref(h)), "stream")
+        return h.value
+
+    def resets(self):
+        n = C.c_uint64()
+        self._check(self._lib.sp_resets(self._h, C.byref(n)), "resets")
+        return n.value
+
+    def shares_a_group(self):
+        out = C.c_int32()
+        self._check(self._lib.sp_shares_a_group(self._h, C.byref(out)), "shares_a_group")
+        return bool(out.value)
+
     # -- outputs -----------------------------------------------------------------------
     def download_particles(self):
         """FluidSim::download_particles (mod.rs:3421-3432): live particles in slot order."""
