@@ -89,6 +89,9 @@ typedef struct sp_material {
     /* Viscoelasticity (material.rs:233-235): the shear modulus G (Pa; 0 = not elastic) and
      * the relaxation time lambda (s) of an upper-convected Maxwell liquid. */
     float shear_modulus, relax_time;
+    /* Suspensions (material.rs:198-200): the diameter of the grains of a concentrate (m), which
+     * settle (or cream) through the diluent at their terminal velocity; 0 = dissolved. */
+    float grain;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -189,7 +192,7 @@ typedef struct sp_gmat {
     int32_t group, diluent, chan_w, chan_s;   /* ia */
     int32_t fam, rheo, temp_model, residue;   /* ib */
     float rho, ln_nu, sigma, mix_rate;        /* a: relative density, ln nu (cells^2/s), tension/rho_w, 1/s */
-    float cos_c, sin_c, grain, rho_solid;     /* b */
+    float cos_c, sin_c, grain, rho_solid;     /* b: grain = its diameter in cells */
     float visc_exp, expansion, load, freeze_shift;  /* misc */
     float th[4];     /* volumetric heat capacity / water's; conductivity (cells^2/s as water's); solidus, liquidus (deg C) */
     float th2[4];    /* latent heat as degrees (L / c); emissivity; boiling point (deg C); latent heat of vaporisation as degrees */
@@ -319,6 +322,7 @@ typedef struct sp_levels_info {
 #define SP_CAP_RIGID 4u
 #define SP_CAP_RHEO 8u
 #define SP_CAP_ELASTIC 32u
+#define SP_CAP_SETTLE 64u
 /* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
  * accumulators */
 #define SP_STRESS_FIX 65536.0f
@@ -389,6 +393,8 @@ enum {
     SP_FIELD_RH = 50,        /* float4[N]  non-Newtonian share, its row, its concentration; 1 where stiff */
     SP_FIELD_ESTR = 51,      /* float4[N]  viscoelastic stress per cell: xx, yy, xy, 0 (cells^2/s^2) */
     SP_FIELD_PSTRESS = 52,   /* float4[n_slots] viscoelastic stress per particle slot: xx, yy, xy, 0 */
+    SP_FIELD_FAMOCC = 53,    /* float[N*n_fam] occupancy of each concentration family per cell */
+    SP_FIELD_DSETTLE = 54,   /* float2[N*n_fam] settling's gain and loss factors per cell and family */
     SP_FIELD_COUNT
 };
 
@@ -417,6 +423,7 @@ enum {
     SP_STAGE_RIGID = 20,        /* rigid.comp; arg = mode 0..4 */
     SP_STAGE_RHEO = 21,         /* rheo.comp */
     SP_STAGE_ELASTIC = 22,      /* elastic.comp */
+    SP_STAGE_SETTLE = 23,       /* settle.comp */
     SP_STAGE_COUNT
 };
 

@@ -3,8 +3,8 @@
  * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
  * times the CPU baseline; nothing in the product calls it.
  *
- * Scope: the substep with heat, rigid bodies, rheology and viscoelasticity, without the
- * other CAP-gated passes (magnets, settling, steam, reactions), tools, currents or film
+ * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity and settling,
+ * without the other CAP-gated passes (magnets, steam, reactions), tools, currents or film
  * spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
@@ -12,8 +12,8 @@
  *   (visc_setup/split/diag/strain/apply/iter/update.comp, vmg_build/down/up.comp,
  *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
- *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp, sort.comp,
- *   count.comp, stats.comp.
+ *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp,
+ *   settle.comp, sort.comp, count.comp, stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
  * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
@@ -70,6 +70,9 @@ struct or_sim {
     /* viscoelasticity */
     v4* estr;         /* per cell: stress xx, yy, xy (cells^2/s^2) */
     v4* pstress;      /* per particle slot: its stress xx, yy, xy */
+    /* settling, per cell and family (cell * n_fam + f) */
+    float* famocc;    /* the family's occupancy of the cell */
+    v2* dsettle;      /* gain and loss factors of the family's concentration this substep */
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -659,6 +662,38 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                         }
                     }
                 }
+            /* Settling (particles.comp:332-351): the cells' gain and loss factors of the
+             * particle's family with its scatter weights; water that is the primary family's
+             * diluent takes part at concentration 0 and becomes the concentrate if it gains. */
+            if (P->caps & SP_CAP_SETTLE) {
+                int dil0 = fam < 0 && P->primary >= 0 && P->fams[0][1] == row;
+                int sf = dil0 ? 0 : fam;
+                if (sf >= 0 && (p.kind & SP_KIND_FRESH) == 0u && s->mats[dil0 ? P->primary : row].grain > 0.0f) {
+                    float dsx = 0.0f, dsy = 0.0f;
+                    for (int b = 0; b < 3; b++)
+                        for (int a = 0; a < 3; a++) {
+                            float wt = wx[a] * wy[b];
+                            v2 d = s->dsettle[(size_t)cidx(s, bx + a, by + b) * P->n_fam + sf];
+                            dsx += wt * d.x;
+                            dsy += wt * d.y;
+                        }
+                    float s0 = dil0 ? 0.0f : p.s;
+                    float s1 = clampf(s0 + (1.0f - s0) * dsx - s0 * dsy, 0.0f, 1.0f);
+                    float c0 = th_on ? thermal_of(s, row, p.s).w : 0.0f;
+                    if (dil0 && s1 > 0.0f) {
+                        row = P->primary;
+                        M = &s->mats[row];
+                        Mgroup = M->group;
+                        Mdil = M->diluent;
+                        Mcw = M->chan_w;
+                        Mcs = M->chan_s;
+                        fam = 0;
+                        p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                    }
+                    if (!dil0 || s1 > 0.0f) p.s = s1;
+                    if (th_on) p.t = keep_heat(p.t, c0, thermal_of(s, row, p.s).w);
+                }
+            }
             /* Heat (particles.comp:353-373): the change of enthalpy temperature the heat pass
              * worked out for the cells this particle scattered to, with the same weights, and
              * the exchange among the particles of its own cell against its own temperature
@@ -1004,6 +1039,10 @@ static void stage_grid_prep(or_sim* s) {
             float T = P->heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
             v4 nn = {0.0f, -1.0f, 1.0f, 0.0f};  /* rh: the cell's non-Newtonian material */
             for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
+            /* each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147) */
+            int settle = (P->caps & SP_CAP_SETTLE) != 0u;
+            if (settle)
+                for (int q = 0; q < P->n_fam; q++) s->famocc[(size_t)i * P->n_fam + q] = 0.0f;
             if (cw > 0) {
                 int others = 0;
                 for (int j = 0; j < P->n_present; j++) {
@@ -1076,6 +1115,7 @@ static void stage_grid_prep(or_sim* s) {
                         int fm = m->fam;
                         if (fm == 0) fam0 = sc;
                         else if (fm > 0) s->mixb[(size_t)i * nmix + fm - 1] = sc;
+                        if (settle && fm >= 0) s->famocc[(size_t)i * P->n_fam + fm] = share * phi_f;
                     }
                     rho_m *= xp;
                     /* frozen liquid: the frozen viscosity and the solid's density
@@ -3103,6 +3143,54 @@ static void stage_rigid(or_sim* s, int mode) {
 
 /* ---- public entry points -------------------------------------------------------------- */
 
+/* ---- settle.comp ---------------------------------------------------------------------- */
+
+static float fam_conc(const or_sim* s, int i, int f) {
+    return f == 0 ? s->comp[i].z : s->mixb[(size_t)i * (s->P.n_fam - 1) + f - 1];
+}
+
+/* Flux (cell volumes / s) of family f's concentrate from cell up into cell dn below it: the
+ * hindered flux v_t s_up (1 - s_dn) min(occupancies), limited to what up holds and what dn
+ * has room for over the substep (settle.comp:21-30). */
+static float settle_flux(const or_sim* s, int up, int dn, int f, float vt, float dt) {
+    int nf = s->P.n_fam;
+    float ou = s->famocc[(size_t)up * nf + f], od = s->famocc[(size_t)dn * nf + f];
+    if (ou <= 1e-3f || od <= 1e-3f) return 0.0f;
+    float su = fam_conc(s, up, f), sd = fam_conc(s, dn, f);
+    if (su <= 0.0f || sd < 0.0f) return 0.0f;
+    float F = vt * su * (1.0f - sd) * fminf_(ou, od);
+    return fminf_(F, fminf_(su * ou, (1.0f - sd) * od) / dt);
+}
+
+/* Each cell's gain over its room and loss over its holding, per family (settle.comp:32-53).
+ * Reads only comp, mixb, famocc and therm, which the pass does not write. */
+static void stage_settle(or_sim* s) {
+    const sp_gparams* P = &s->P;
+    int W = s->W, H = s->H, nf = P->n_fam;
+    float dt = s->S.dt;
+    int th_on = thermal(s);
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            for (int f = 0; f < nf; f++) {
+                int row = P->fams[f][0], dl = P->fams[f][1];
+                float d = s->mats[row].grain;
+                float occ = s->famocc[(size_t)i * nf + f];
+                v2 ds = {0.0f, 0.0f};
+                if (d > 0.0f && occ > 1e-3f && cy > 0 && cy < H - 1) {
+                    float drho = s->mats[row].rho - s->mats[dl].rho;
+                    float T = th_on ? s->therm[i].x : P->heat_x;
+                    float vt = terminal(s, drho, s->mats[dl].rho, expf(row_lnu(s, dl, 1.0f, T, th_on)), d);
+                    float gain = drho > 0.0f ? settle_flux(s, i + W, i, f, vt, dt) : settle_flux(s, i - W, i, f, vt, dt);
+                    float loss = drho > 0.0f ? settle_flux(s, i, i - W, f, vt, dt) : settle_flux(s, i, i + W, f, vt, dt);
+                    float sc = fam_conc(s, i, f);
+                    ds = (v2){dt * (gain / fmaxf_(occ * (1.0f - sc), 1e-6f)), dt * (loss / fmaxf_(occ * sc, 1e-6f))};
+                }
+                s->dsettle[(size_t)i * nf + f] = ds;
+            }
+        }
+}
+
 /* ---- elastic.comp --------------------------------------------------------------------- */
 
 /* tau_xy at the lower-left corner of cell i (elastic.comp:15-18) */
@@ -3164,6 +3252,7 @@ void or_stage(or_sim* s, int stage, int arg) {
     case SP_STAGE_HEAT: stage_heat(s); break;
     case SP_STAGE_RHEO: stage_rheo(s); break;
     case SP_STAGE_ELASTIC: stage_elastic(s); break;
+    case SP_STAGE_SETTLE: stage_settle(s); break;
     case SP_STAGE_RIGID:
         if (arg >= 0 && arg <= 4) stage_rigid(s, arg);
         break;
@@ -3188,6 +3277,7 @@ void or_substep(or_sim* s, int gather) {
     stage_grid_prep(s);
     if (s->P.caps & SP_CAP_RHEO) stage_rheo(s);
     if (s->P.caps & SP_CAP_ELASTIC) stage_elastic(s);
+    if (s->P.caps & SP_CAP_SETTLE) stage_settle(s);
     stage_coef(s);
     stage_surf_smooth(s);
     stage_surf_curv(s);
@@ -3252,6 +3342,8 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->rh = (v4*)calloc(N, sizeof(v4));
     s->estr = (v4*)calloc(N, sizeof(v4));
     s->pstress = (v4*)calloc(capacity ? capacity : 1, sizeof(v4));
+    s->famocc = (float*)calloc(N * SP_MAX_PRESENT, sizeof(float));
+    s->dsettle = (v2*)calloc(N * SP_MAX_PRESENT, sizeof(v2));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -3292,6 +3384,8 @@ void or_destroy(or_sim* s) {
     free(s->rh);
     free(s->estr);
     free(s->pstress);
+    free(s->famocc);
+    free(s->dsettle);
     free(s);
 }
 
@@ -3388,6 +3482,8 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_RH: *bytes = sizeof(v4) * (size_t)N; return s->rh;
     case SP_FIELD_ESTR: *bytes = sizeof(v4) * (size_t)N; return s->estr;
     case SP_FIELD_PSTRESS: *bytes = sizeof(v4) * (size_t)s->n_slots; return s->pstress;
+    case SP_FIELD_FAMOCC: *bytes = sizeof(float) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->famocc;
+    case SP_FIELD_DSETTLE: *bytes = sizeof(v2) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->dsettle;
     default: *bytes = 0; return NULL;
     }
 }

@@ -30,7 +30,7 @@ ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
 OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
 RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
-CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_ELASTIC = 1, 2, 4, 8, 32
+CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 32, 64
 STRESS_FIX = 65536.0
 
 # Field ids (SP_FIELD_*)
@@ -39,12 +39,12 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21, ELASTIC=22)
+             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21, ELASTIC=22, SETTLE=23)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -64,7 +64,7 @@ class Material(C.Structure):
                 ("freeze_shift", C.c_float), ("rheology", C.c_uint32), ("yield_stress", C.c_float),
                 ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
                 ("skin", C.c_float), ("friction", C.c_float), ("shear_modulus", C.c_float),
-                ("relax_time", C.c_float)]
+                ("relax_time", C.c_float), ("grain", C.c_float)]
 
 
 class SimParams(C.Structure):

@@ -59,6 +59,9 @@ struct Ctx {
     // viscoelasticity (elastic.comp)
     float4* estr;         // per cell: stress xx, yy, xy (cells^2/s^2), scattered by volume
     float4* pstress;      // per particle slot: its stress xx, yy, xy; moved with the particle by the sort
+    // settling of suspensions (settle.comp), per cell and concentration family (cell * n_fam + f)
+    float* famocc;        // the family's occupancy of the cell
+    float2* dsettle;      // this substep's gain and loss factors of the family's concentration
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -137,6 +140,8 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.rh = (float4*)take(sizeof(float4) * N);
     c.estr = (float4*)take(sizeof(float4) * N);
     c.pstress = (float4*)take(sizeof(float4) * (r.cap ? r.cap : 1));
+    c.famocc = (float*)take(sizeof(float) * N * SP_MAX_PRESENT);
+    c.dsettle = (float2*)take(sizeof(float2) * N * SP_MAX_PRESENT);
     if (bytes) *bytes = o;
     return c;
 }
@@ -356,6 +361,13 @@ SP_DEV float row_lnu(const Ctx& c, int row, float s, float T, bool thermal) {
         l = mixf_(c.mats[d].ln_nu + (thermal ? temp_law(c, d, T) : 0.0f), l, bs);
     }
     return l;
+}
+
+// Terminal speed of a sphere of diameter d (cells) and density difference drho through a
+// liquid of density rho_c and viscosity nu_c, Stokes to Newton drag (common.glsl:370-374).
+SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d) {
+    float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * c.P->gravity * d;
+    return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
 }
 
 // A particle's heat capacity and heat in fixed point (heat_quanta, particles.comp:70-73).

@@ -135,6 +135,10 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float T = P.heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
     float4 nn = make_float4(0.0f, -1.0f, 1.0f, 0.0f);  // rh: the cell's non-Newtonian material
     for (int q = 0; q < nmix; q++) c.mixb[(size_t)i * nmix + q] = -1.0f;
+    // each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147)
+    const bool settle = (P.caps & SP_CAP_SETTLE) != 0u;
+    if (settle)
+        for (int q = 0; q < P.n_fam; q++) c.famocc[(size_t)i * P.n_fam + q] = 0.0f;
     if (cw > 0) {
         int others = 0;
         for (int j = 0; j < P.n_present; j++) {
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
                 csy = mixf_(d.sin_c, csy, sc);
                 if (m.fam == 0) fam0 = sc;
                 else if (m.fam > 0) c.mixb[(size_t)i * nmix + m.fam - 1] = sc;
+                if (settle && m.fam >= 0) c.famocc[(size_t)i * P.n_fam + m.fam] = share * phi_f;
             }
             rho_m *= xp;
             // frozen liquid: the frozen viscosity and the solid's density (grid_prep.comp:150-159)

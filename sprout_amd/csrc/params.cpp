@@ -56,6 +56,7 @@ static sp_material default_material() {
     m.friction = 0.0f;
     m.shear_modulus = 0.0f;  // material.rs:380-381
     m.relax_time = 0.0f;
+    m.grain = 0.0f;  // material.rs:363
     return m;
 }
 
@@ -340,7 +341,7 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
 }
 
 // FluidSim::caps (mod.rs:2717-2766) for the CAP-gated passes this build carries: families,
-// heat, rigid bodies, rheology and viscoelasticity (stress channels in the layout). A non-Newtonian liquid, or one with a skin or internal
+// heat, rigid bodies, rheology, viscoelasticity (stress channels in the layout) and settling. A non-Newtonian liquid, or one with a skin or internal
 // friction, turns rheology on, and rigid bodies for a yield stress, shear thickening, a skin
 // or friction. A thermal tank turns rigid bodies on for any liquid that can freeze or has a
 // temperature law; any tank for a liquid already as viscous as rigid_viscosity, or frozen at
@@ -363,6 +364,10 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
     }
     if (rigid) caps |= SP_CAP_RIGID;
     if (t.chan_e >= 0) caps |= SP_CAP_ELASTIC;
+    // FluidSim::settles (mod.rs:2139-2143): a family whose concentrate has grains, under gravity
+    bool settles = false;
+    for (int r : t.rows) settles |= p.mat[r].diluent >= 0 && p.mat[r].grain > 0.0f;
+    if (t.families > 0 && p.gravity > 0.0f && settles) caps |= SP_CAP_SETTLE;
     return caps;
 }
 
@@ -412,7 +417,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.mix_rate = m.mix_rate;
         o.cos_c = cs;
         o.sin_c = sn;
-        o.grain = 0.0f;
+        o.grain = m.grain / dx;  // mod.rs:2210
         o.rho_solid = solid / WATER_DENSITY;
         o.visc_exp = m.visc_exponent;
         o.expansion = m.expansion;

@@ -40,12 +40,6 @@ SP_DEV float row_density(const Ctx& c, int row, float s) {
     return d >= 0 ? mixf_(c.mats[d].rho, rho, s) : rho;
 }
 
-// Terminal drift speed (common.glsl:370-374).
-SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d) {
-    float qa = 0.44f * rho_c, qb = 24.0f * nu_c * rho_c / d, qc = 4.0f / 3.0f * fabsf(drho) * c.P->gravity * d;
-    return qc > 0.0f ? 2.0f * qc / (qb + sqrtf(qb * qb + 4.0f * qa * qc)) : 0.0f;
-}
-
 // Bilinear velocity over live faces, `fb` where no face is live (air.glsl:41-78).
 SP_DEV float2 sample_vel(const Ctx& c, float px, float py, float2 fb) {
     float2 o = fb;
@@ -196,9 +190,9 @@ SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, fl
 }
 
 // Gather, drift, exchange, advect and collide one particle (particles.comp:98-985, the
-// branches this build carries); fills the scatter inputs. EL: the tank may be elastic (the
-// stress code is compiled only into the kernels elastic tanks run).
-template <bool EL>
+// branches this build carries); fills the scatter inputs. X: the tank may be elastic or
+// settle (that code is compiled only into the kernels such tanks run).
+template <bool X>
 SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& umax) {
     const sp_gparams& P = *c.P;
     g.live = false;
@@ -221,7 +215,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     }
     // viscoelastic stress: a particle that has not run a substep yet starts unstressed (its
     // slot may hold an old stress) (particles.comp:152-156)
-    const bool el_on = EL && (P.caps & SP_CAP_ELASTIC) != 0u;
+    const bool el_on = X && (P.caps & SP_CAP_ELASTIC) != 0u;
     float4 tau = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (el_on && (p.kind & SP_KIND_FRESH) == 0u && c.mats[row].gr[1] > 0.0f) tau = c.pstress[id];
     if (gather) {
@@ -316,6 +310,36 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                         }
                     }
                 }
+            // Settling (particles.comp:332-351): the cells' gain and loss factors of the
+            // particle's family, with its scatter weights; water that is the primary family's
+            // diluent takes part at concentration 0 and becomes the concentrate when it gains.
+            if (X && (P.caps & SP_CAP_SETTLE)) {
+                const bool dil0 = fam < 0 && P.primary >= 0 && P.fams[0][1] == row;
+                const int sf = dil0 ? 0 : fam;
+                if (sf >= 0 && (p.kind & SP_KIND_FRESH) == 0u && c.mats[dil0 ? P.primary : row].grain > 0.0f) {
+                    float dsx = 0.0f, dsy = 0.0f;
+                    for (int b = 0; b < 3; b++)
+                        for (int a = 0; a < 3; a++) {
+                            const float wt = wx[a] * wy[b];
+                            const float2 d = c.dsettle[(size_t)cidx(c, bx + a, by + b) * P.n_fam + sf];
+                            dsx += wt * d.x;
+                            dsy += wt * d.y;
+                        }
+                    const float s0 = dil0 ? 0.0f : p.s;
+                    const float s1 = clampf_(s0 + (1.0f - s0) * dsx - s0 * dsy, 0.0f, 1.0f);
+                    const float c0 = th_on ? thermal_of(c, row, p.s).w : 0.0f;
+                    if (dil0 && s1 > 0.0f) {
+                        row = P.primary;
+                        Mgroup = c.mats[row].group;
+                        Mcw = c.mats[row].chan_w;
+                        Mcs = c.mats[row].chan_s;
+                        fam = 0;
+                        p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                    }
+                    if (!dil0 || s1 > 0.0f) p.s = s1;
+                    if (th_on) p.t = keep_heat(p.t, c0, thermal_of(c, row, p.s).w);
+                }
+            }
             // Heat (particles.comp:353-373): the change of enthalpy temperature the heat
             // pass worked out for the cells this particle scattered to, with the same
             // weights, and the exchange among the particles of its own cell, against its
@@ -655,11 +679,12 @@ SP_DEV void deposit(const ScatterIn& g, float3 te, Add add) {
 }  // namespace
 
 // The particle pass (particles.comp:1064-1101), NT work-items per group, a scatter box of
-// BOX ints in LDS, registers for at least MINB groups per compute unit; EL for elastic tanks. Every (NT, BOX) gives the same result (integer sums; the speed maximum is
+// BOX ints in LDS, registers for at least MINB groups per compute unit; X for tanks that are
+// elastic or settle. Every (NT, BOX) gives the same result (integer sums; the speed maximum is
 // a maximum), so the launch shape is free to tune.
 // gather < 0: each tank decides from its own step state (the first substep after a reset,
 // which k_step has just counted, has no grid velocities to gather yet).
-template <int NT, int BOX, int MINB, bool EL>
+template <int NT, int BOX, int MINB, bool X>
 __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int gather) {
     static_assert(NT % 64 == 0 && NT <= 1024 && BOX * 4 <= 64 * 1024, "launch shape");
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
@@ -671,7 +696,7 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
     ScatterIn g;
     float umax = 0.0f;
     __shared__ float s_max[NT / 64];
-    advance<EL>(c, id, gather, g, umax);
+    advance<X>(c, id, gather, g, umax);
     group_max_to<NT>(&c.S->umax_part, umax, s_max);
 
     if (t == 0) {

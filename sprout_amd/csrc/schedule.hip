@@ -247,6 +247,8 @@ void launch_grid_passes(const Run& r, const Recs& recs) {
     if (r.gp.caps & SP_CAP_RHEO) hipLaunchKernelGGL(k_rheo, g0, TILE, 0, r.stream, r.tb);
     // the viscoelastic stress's force on the faces (mod.rs:2643-2646)
     if (r.gp.caps & SP_CAP_ELASTIC) hipLaunchKernelGGL(k_elastic, g0, TILE, 0, r.stream, r.tb);
+    // settling of suspensions (mod.rs:2647-2650)
+    if (r.gp.caps & SP_CAP_SETTLE) hipLaunchKernelGGL(k_settle, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, r.stream, r.tb);
@@ -282,21 +284,22 @@ constexpr int N_PSHAPES = 9;
 constexpr PShape P_SHAPES[N_PSHAPES] = {{256, 12288, 1}, {256, 8192, 1}, {256, 4096, 1}, {128, 2048, 1}, {128, 4096, 1},
                                         {512, 12288, 1}, {512, 8192, 1}, {256, 4096, 5},  {256, 6144, 1}};
 using PKernel = void (*)(TankBatch, int);
-// [elastic][shape]: the stress update and its scatter only in the kernels of elastic tanks
-#define SP_PK(EL)                                                                                                     \
-    {k_particles<256, 12288, 1, EL>, k_particles<256, 8192, 1, EL>, k_particles<256, 4096, 1, EL>,                     \
-     k_particles<128, 2048, 1, EL>,  k_particles<128, 4096, 1, EL>, k_particles<512, 12288, 1, EL>,                    \
-     k_particles<512, 8192, 1, EL>,  k_particles<256, 4096, 5, EL>, k_particles<256, 6144, 1, EL>}
+// [exotic][shape]: the stress update and its scatter, and settling, only in the kernels of
+// tanks that are elastic or settle
+#define SP_PK(X)                                                                                                      \
+    {k_particles<256, 12288, 1, X>, k_particles<256, 8192, 1, X>, k_particles<256, 4096, 1, X>,                        \
+     k_particles<128, 2048, 1, X>,  k_particles<128, 4096, 1, X>, k_particles<512, 12288, 1, X>,                       \
+     k_particles<512, 8192, 1, X>,  k_particles<256, 4096, 5, X>, k_particles<256, 6144, 1, X>}
 const PKernel P_KERNELS[2][N_PSHAPES] = {SP_PK(false), SP_PK(true)};
 #undef SP_PK
 
-bool elastic_run(const Run& r) { return (r.gp.caps & SP_CAP_ELASTIC) != 0u; }
+bool exotic_run(const Run& r) { return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE)) != 0u; }
 
 // What a shape is tuned for: the grid, the tanks per launch, the accumulator stride and
-// whether the tanks are elastic.
+// whether the tanks run the elastic or settling code.
 uint64_t particle_key(const Run& r) {
     return (uint64_t)(uint32_t)r.W | (uint64_t)(uint32_t)r.H << 14 | (uint64_t)(uint32_t)r.T << 28 |
-           (uint64_t)(uint32_t)r.gp.acc_stride << 34 | (uint64_t)elastic_run(r) << 41;
+           (uint64_t)(uint32_t)r.gp.acc_stride << 34 | (uint64_t)exotic_run(r) << 41;
 }
 
 class Tuner {
@@ -393,7 +396,7 @@ int launch_particles(const Run& r, int gather) {
     const int v = tuner().pick(particle_key(r));
     const PShape sh = P_SHAPES[v];
     const uint32_t blocks = std::max<uint32_t>((r.slots + (uint32_t)sh.threads - 1) / (uint32_t)sh.threads, 1);
-    hipLaunchKernelGGL(P_KERNELS[elastic_run(r) ? 1 : 0][v], dim3(blocks, 1, r.T), dim3((unsigned)sh.threads), 0, r.stream,
+    hipLaunchKernelGGL(P_KERNELS[exotic_run(r) ? 1 : 0][v], dim3(blocks, 1, r.T), dim3((unsigned)sh.threads), 0, r.stream,
                        r.tb, gather);
     return v;
 }
@@ -404,7 +407,8 @@ uint32_t schedule_key(const sp_gparams& g) {
     return (uint32_t)g.visc_mode | (uint32_t)g.v_twice << 2 | (uint32_t)g.v_precond << 3 | (uint32_t)g.v_warm << 4 |
            (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12 |
            (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14 |
-           (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15 | (uint32_t)((g.caps & SP_CAP_ELASTIC) != 0u) << 16;
+           (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15 | (uint32_t)((g.caps & SP_CAP_ELASTIC) != 0u) << 16 |
+           (uint32_t)((g.caps & SP_CAP_SETTLE) != 0u) << 17;
 }
 
 }  // namespace

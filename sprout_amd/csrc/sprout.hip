@@ -25,6 +25,7 @@
 #include "heat.hip"
 #include "rigid.hip"
 #include "elastic.hip"
+#include "settle.hip"
 #include "schedule.hip"
 
 namespace {
@@ -1139,6 +1140,8 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_RH: *bytes = 16 * N; return c.rh;
     case SP_FIELD_ESTR: *bytes = 16 * N; return c.estr;
     case SP_FIELD_PSTRESS: *bytes = 16 * (size_t)s->n_slots; return c.pstress;
+    case SP_FIELD_FAMOCC: *bytes = 4 * N * (size_t)std::max(s->gp.n_fam, 1); return c.famocc;
+    case SP_FIELD_DSETTLE: *bytes = 8 * N * (size_t)std::max(s->gp.n_fam, 1); return c.dsettle;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1239,6 +1242,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         case SP_STAGE_HEAT: hipLaunchKernelGGL(k_heat, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_RHEO: hipLaunchKernelGGL(k_rheo, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_ELASTIC: hipLaunchKernelGGL(k_elastic, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_SETTLE: hipLaunchKernelGGL(k_settle, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_RIGID:
             if (arg < 0 || arg > 4) return SP_E_INVALID;
             hipLaunchKernelGGL(k_rigid, g0, TILE, 0, s->stream, cs, arg);

@@ -262,3 +262,32 @@ def shear_layer(grid=(96, 48), y0=16.0, h=16.0, x=(8.0, 88.0), amp=2.0, cell_siz
     parts["c"][:] = 0.0
     parts["c"][:, 1] = (-amp * k * np.sin(k * yy)).astype(np.float32)
     return params, scene.rasterize(), settle(parts, params), [ROW_SLIME], k
+
+
+def mud_params(params, grain=100e-6, density=2000.0, load=0.5, viscosity=0.05):
+    """Makes the honey row (a concentrate of water) the reference's mud (presets.rs:119-140):
+    silt grains of `grain` m at 2000 kg/m^3 suspended in water, emitted at concentration
+    `load`; Newtonian here (the preset's Carreau thinning is left out)."""
+    m = params.mat[N.ROW_HONEY]
+    m.density = density
+    m.viscosity = viscosity
+    m.grain = grain
+    m.load = load
+    m.mix_rate = 0.0
+    return params
+
+
+def mud_column(grid, block=(0.3, 0.7, 0.0, 0.6), cell_size=0.005, pressure_tol=1e-6, water_above=0.0, **mat):
+    """A column of mud (a suspension of grains in water) standing on the floor of a tank,
+    optionally under a layer of clear water `water_above` (fraction of H) deep, isothermal.
+    (params, solid field, particles, rows)."""
+    params = mud_params(dam_params(cell_size=cell_size, pressure_tol=pressure_tol), **mat)
+    W, H = grid
+    scene = sc.Scene(grid)
+    x0, x1, y0, y1 = block[0] * W, block[1] * W, block[2] * H, block[3] * H
+    mud = sc.fill_rect(scene, (x0, y0, x1, y1), N.ROW_HONEY)
+    parts = [mud]
+    if water_above > 0.0:
+        parts.append(sc.fill_rect(scene, (x0, y1, x1, y1 + water_above * H), N.ROW_WATER, len(mud)))
+    parts = np.concatenate(parts)
+    return params, scene.rasterize(), settle(parts, params), [N.ROW_WATER, N.ROW_HONEY]
