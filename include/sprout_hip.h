@@ -114,6 +114,11 @@ int sp_cell_counts(sp_sim* sim, int32_t x0, int32_t y0, int32_t w, int32_t h, ui
  * live particles, and per table row the count and the concentration sum. */
 int sp_stats(sp_sim* sim, sp_stats_out* out);
 
+/* `FluidSim::set_magnets` (src/sim/mod.rs:2018-2022): replaces the magnets placed in the
+ * world (the first SP_MAX_MAGNETS are kept); they survive a reset. A magnet and a liquid
+ * with a susceptibility turn the magnetic pass on (magnet.comp). */
+int sp_set_magnets(sp_sim* sim, const sp_magnet* magnets, uint32_t n);
+
 /* The periodic counting sort of the particles by cell (record_sort, src/sim/mod.rs:
  * 3223-3242, sort.comp). Runs on the handle's stream; the order within a cell is stable
  * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
@@ -334,11 +339,11 @@ void sp_default_params(sp_sim_params* out);
 
 /* Tank::layout (src/sim/mod.rs:271-331) + gpu_params / gpu_materials (src/sim/mod.rs:
  * 2179-2330) for a tank holding the table rows `rows[0..n_rows)` (ascending), isothermal
- * or (thermal = 1) carrying temperatures: fills the grid-unit parameter block and the
- * material table (SP_MAX_MATERIALS rows). */
+ * or (thermal = 1) carrying temperatures, with magnets[0..n_magnets) placed: fills the
+ * grid-unit parameter block and the material table (SP_MAX_MATERIALS rows). */
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h,
-                          const int32_t* rows, int32_t n_rows, int32_t thermal, sp_gparams* out,
-                          sp_gmat* mats);
+                          const int32_t* rows, int32_t n_rows, int32_t thermal, const sp_magnet* magnets,
+                          uint32_t n_magnets, sp_gparams* out, sp_gmat* mats);
 
 /* Levels::new (src/sim/mod.rs:727-753): the multigrid hierarchy of a W x H grid. */
 int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out);

@@ -22,6 +22,7 @@ extern "C" {
 #define SP_PPC 4             /* params.rs:11 PPC */
 #define SP_ACC_MIN 8         /* mod.rs:68 ACC_MIN */
 #define SP_ACC_MAX 32        /* mod.rs:69 ACC_MAX */
+#define SP_MAX_MAGNETS 8     /* params.rs:1071 MAX_MAGNETS */
 #define SP_ACC_FIRST 5       /* mod.rs:71 ACC_FIRST */
 #define SP_REGION_MASK 31u   /* common.glsl:220 */
 #define SP_REGION_VENT 256u  /* common.glsl:221 */
@@ -92,6 +93,9 @@ typedef struct sp_material {
     /* Suspensions (material.rs:198-200): the diameter of the grains of a concentrate (m), which
      * settle (or cream) through the diluent at their terminal velocity; 0 = dissolved. */
     float grain;
+    /* Magnetism (material.rs:236-238): the volume susceptibility (SI; 0 = not magnetic) and
+     * the saturation magnetisation as mu0 Ms (T). */
+    float susceptibility, saturation;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -153,9 +157,23 @@ typedef struct sp_sim_params {
      * cap), and the largest deformation per substep that enters the stress (0: no limit). */
     float elastic_stress_max;
     float elastic_step_max;
+    /* Magnetism (params.rs:218): how much of a local shape demagnetisation is blended in
+     * (0 = a sphere's, N = 1/3, everywhere). */
+    float magnet_demag;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
+
+/* `Magnet` (params.rs:1082-1094): a permanent magnet, the field of a point dipole at `pos`
+ * (cells) in the plane of the tank, `strength` tesla on its axis at distance `radius` (cells),
+ * north pole towards `dir` (unit vector; Magnet::new(pos, radius, degrees, strength) gives
+ * (sin, cos) of the angle clockwise from straight up). */
+typedef struct sp_magnet {
+    float pos[2];
+    float radius;
+    float strength;
+    float dir[2];
+} sp_magnet;
 
 /* `ToolState` (params.rs:1141-1150). */
 typedef struct sp_tool_state {
@@ -198,7 +216,7 @@ typedef struct sp_gmat {
     float th2[4];    /* latent heat as degrees (L / c); emissivity; boiling point (deg C); latent heat of vaporisation as degrees */
     float temp[4];   /* temperature law: B, C (K); reference temperature (K); skin (cells^3/s^2) */
     float rheo_p[4]; /* yield stress (cells^2/s^2), flow index, shear time (s), viscosity ratio */
-    float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); 0 */
+    float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); susceptibility (el.z) */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -246,6 +264,12 @@ typedef struct sp_gparams {
     float el_g;           /* elastic.y: the largest shear modulus in the tank (cells^2/s^2, at least 1e-3 Pa) */
     float el_stress_max;  /* elastic.z: most stress, times G (0: no cap) */
     float el_step_max;    /* elastic.w: most deformation per substep (0: no limit) */
+    /* Magnetism (GpuParams mag, magnet, magnet_dir, heat3.w; mod.rs:2258-2264, 2313), used
+     * where caps has CAP_MAGNET: */
+    float mag[4];         /* magnets; largest susceptibility in the tank (at least 1e-6); its saturation mu0 Ms (T); 1 / (rho_w dx^2) */
+    float magnet[SP_MAX_MAGNETS][4];      /* position (cells), radius (cells, at least 0.5), strength (T at the rim) */
+    float magnet_dir[SP_MAX_MAGNETS][2];  /* north pole direction */
+    float demag;          /* heat3.w: magnet_demag in [0, 1] */
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -323,6 +347,8 @@ typedef struct sp_levels_info {
 #define SP_CAP_RHEO 8u
 #define SP_CAP_ELASTIC 32u
 #define SP_CAP_SETTLE 64u
+#define SP_CAP_MAGNET 16u
+#define SP_MU0 1.25663706e-6f  /* magnet.comp MU0 */
 /* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
  * accumulators */
 #define SP_STRESS_FIX 65536.0f
@@ -395,6 +421,7 @@ enum {
     SP_FIELD_PSTRESS = 52,   /* float4[n_slots] viscoelastic stress per particle slot: xx, yy, xy, 0 */
     SP_FIELD_FAMOCC = 53,    /* float[N*n_fam] occupancy of each concentration family per cell */
     SP_FIELD_DSETTLE = 54,   /* float2[N*n_fam] settling's gain and loss factors per cell and family */
+    SP_FIELD_MAGC = 55,      /* float2[N] susceptibility of the cell's liquid, magnetic energy density (grid units) */
     SP_FIELD_COUNT
 };
 
@@ -424,6 +451,7 @@ enum {
     SP_STAGE_RHEO = 21,         /* rheo.comp */
     SP_STAGE_ELASTIC = 22,      /* elastic.comp */
     SP_STAGE_SETTLE = 23,       /* settle.comp */
+    SP_STAGE_MAGNET = 24,       /* magnet.comp; arg = mode 0 (energy) or 1 (force) */
     SP_STAGE_COUNT
 };
 

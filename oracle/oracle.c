@@ -3,9 +3,8 @@
  * shaders, run through Vulkano). TEST INFRASTRUCTURE ONLY: it checks the HIP path and
  * times the CPU baseline; nothing in the product calls it.
  *
- * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity and settling,
- * without the other CAP-gated passes (magnets, steam, reactions), tools, currents or film
- * spatter:
+ * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity, settling and
+ * magnetism, without steam, boiling and reactions, tools, currents or film spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
@@ -13,7 +12,7 @@
  *   visc_mgu/mgz.comp), rhs.comp, mg_build/down/up/tail.comp, cg_scale/init/apply/update.comp
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
  *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp,
- *   settle.comp, sort.comp, count.comp, stats.comp.
+ *   settle.comp, magnet.comp, sort.comp, count.comp, stats.comp.
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
  * scalar reductions are FP64 (the build's precision plan); the reference sums FP32 in a
@@ -73,6 +72,8 @@ struct or_sim {
     /* settling, per cell and family (cell * n_fam + f) */
     float* famocc;    /* the family's occupancy of the cell */
     v2* dsettle;      /* gain and loss factors of the family's concentration this substep */
+    /* magnetism: per cell the susceptibility of its liquid and the magnetic energy density */
+    v2* magc;
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -1038,6 +1039,7 @@ static void stage_grid_prep(or_sim* s) {
             float top_share = 0.0f;
             float T = P->heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
             v4 nn = {0.0f, -1.0f, 1.0f, 0.0f};  /* rh: the cell's non-Newtonian material */
+            float chi = 0.0f;
             for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
             /* each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147) */
             int settle = (P->caps & SP_CAP_SETTLE) != 0u;
@@ -1137,6 +1139,7 @@ static void stage_grid_prep(or_sim* s) {
                         top_share = share;
                         top_row = r;
                     }
+                    chi += share * m->gr[3]; /* the cell's susceptibility (grid_prep.comp:168) */
                     int g = m->group;
                     f[g] += share;
                     rg[g] += share * rho_m;
@@ -1164,6 +1167,15 @@ static void stage_grid_prep(or_sim* s) {
             if (has && level <= 0.0f && cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
                 if ((packed(s, i - 1) && packed(s, i + 1)) || (packed(s, i - W) && packed(s, i + W))) level = 0.05f;
                 else if (P->gravity > 0.0f && nn.x <= 0.0f && phi_t > 0.25f && packed(s, i - W)) level = 0.05f;
+                /* with a magnet, "lying on" is towards gravity and the last substep's magnetic
+                 * pull together (grid_prep.comp:211-221) */
+                else if ((P->caps & SP_CAP_MAGNET) && nn.x <= 0.0f && phi_t > 0.25f && chi > 0.0f) {
+                    float km = chi / P->mag[1] / fmaxf_(rho, 0.05f);
+                    float px = -km * 0.5f * (s->magc[i + 1].y - s->magc[i - 1].y) - 0.0f;
+                    float py = -km * 0.5f * (s->magc[i + W].y - s->magc[i - W].y) - P->gravity;
+                    int toward = fabsf(px) > fabsf(py) ? (px > 0.0f ? i + 1 : i - 1) : (py > 0.0f ? i + W : i - W);
+                    if (packed(s, toward)) level = 0.05f;
+                }
             }
             if (sol.w > 0.0f && level <= 0.0f) s->region_air |= 1u << (s->region[i] & SP_REGION_MASK);
             /* stiff where the viscosity would exceed the rigid cap (grid_prep.comp:230-237) */
@@ -1179,6 +1191,7 @@ static void stage_grid_prep(or_sim* s) {
                 s->estr[i] = (v4){(float)s->acc[k + P->chan_e] * sc, (float)s->acc[k + P->chan_e + 1] * sc,
                                   (float)s->acc[k + P->chan_e + 2] * sc, 0.0f};
             }
+            if (P->caps & SP_CAP_MAGNET) s->magc[i].x = chi; /* grid_prep.comp:240 */
             s->cell[i] = (v4){phi_f, rho, mu, level};
             s->comp[i] = (v4){rho, f[1], fam0, phi_t};
             if (any_sorts) s->gbase[i] = (v4){bg[0], bg[1], bg[2], 0.0f};
@@ -3191,6 +3204,86 @@ static void stage_settle(or_sim* s) {
         }
 }
 
+/* ---- magnet.comp ---------------------------------------------------------------------- */
+
+/* The field of the placed magnets at x (cells), each a point dipole in the plane, B0 on the
+ * axis at the rim and constant inside it (common.glsl:390-401). */
+static v2 magnet_field(const or_sim* s, float x, float y) {
+    v2 B = {0.0f, 0.0f};
+    for (int k = 0; k < (int)s->P.mag[0]; k++) {
+        const float* mg = s->P.magnet[k];
+        float mx = s->P.magnet_dir[k][0], my = s->P.magnet_dir[k][1];
+        float dx = x - mg[0], dy = y - mg[1];
+        float dl = len2(dx, dy);
+        float r = fmaxf_(dl, mg[2]);
+        float ex = mx, ey = my;
+        if (dl > 1e-4f) {
+            ex = dx / dl;
+            ey = dy / dl;
+        }
+        float k3 = mg[3] * powf(mg[2] / r, 3.0f) * 0.5f;
+        float me = 3.0f * (mx * ex + my * ey);
+        B.x += k3 * (me * ex - mx);
+        B.y += k3 * (me * ey - my);
+    }
+    return B;
+}
+
+/* The velocity change of the face between cell i and the cell n behind it: the magnetisable
+ * share of the face (the mean of the two cells' susceptibilities where both are liquid, the
+ * liquid cell's where one is air, over the tank's largest) times the difference of the
+ * energy density over the face density (magnet.comp:63-67). */
+static float mag_face_force(const or_sim* s, int i, int n, float rho) {
+    int li = s->cell[i].w > 0.0f, ln = s->cell[n].w > 0.0f;
+    float sh = (li == ln ? 0.5f * (s->magc[i].x + s->magc[n].x) : li ? s->magc[i].x : s->magc[n].x) / s->P.mag[1];
+    return -s->S.dt * sh * (s->magc[i].y - s->magc[n].y) / fmaxf_(rho, 0.05f);
+}
+
+/* Mode 0: the magnetic energy density of a linear liquid that saturates, demagnetised as a
+ * sphere (blended towards the shape's N by `demag`), per cell; mode 1: its force on the live
+ * faces (magnet.comp:71-105). Mode 1 reads psi of neighbours and writes only velocities. */
+static void stage_magnet(or_sim* s, int mode) {
+    const sp_gparams* P = &s->P;
+    int W = s->W, H = s->H;
+    for (int cy = 0; cy < H; cy++)
+        for (int cx = 0; cx < W; cx++) {
+            int i = cidx(s, cx, cy);
+            if (mode == 1) {
+                if (cx < 1 || cy < 1) continue;
+                v4 me = s->vel[i], cf = s->coef[i];
+                if (face_live(me.z) && cf.x != 0.0f) me.x += mag_face_force(s, i, i - 1, cf.z);
+                if (face_live(me.w) && cf.y != 0.0f) me.y += mag_face_force(s, i, i - W, cf.w);
+                s->vel[i] = me;
+                continue;
+            }
+            v2 B = magnet_field(s, (float)cx + 0.5f, (float)cy + 0.5f);
+            float Bl = len2(B.x, B.y);
+            float Hf = Bl / SP_MU0;
+            float psi = 0.0f;
+            float chi = P->mag[1];
+            if (Hf > 0.0f && chi > 0.0f) {
+                float N = 1.0f / 3.0f;
+                if (P->demag > 0.0f) {
+                    float gx = 0.0f, gy = 0.0f;
+                    if (cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
+                        gx = 0.5f * (s->surf[i + 1].x - s->surf[i - 1].x);
+                        gy = 0.5f * (s->surf[i + W].x - s->surf[i - W].x);
+                    }
+                    float gl = len2(gx, gy);
+                    float nh = gl > 1e-6f ? (gx / gl) * (B.x / Bl) + (gy / gl) * (B.y / Bl) : 0.0f;
+                    N = mixf(1.0f / 3.0f, nh * nh, clampf(gl / 0.25f, 0.0f, 1.0f) * P->demag);
+                }
+                float ce = chi / (1.0f + N * chi);
+                float Ms = P->mag[2] / SP_MU0;
+                float xs = ce * Hf / Ms;
+                /* x - ln(1 + x), by its series where float cancellation would eat it */
+                float f = xs < 0.02f ? xs * xs * (0.5f - xs * (1.0f / 3.0f - 0.25f * xs)) : xs - logf(1.0f + xs);
+                psi = -SP_MU0 * Ms * Ms / ce * f * P->mag[3];
+            }
+            s->magc[i].y = psi;
+        }
+}
+
 /* ---- elastic.comp --------------------------------------------------------------------- */
 
 /* tau_xy at the lower-left corner of cell i (elastic.comp:15-18) */
@@ -3253,6 +3346,9 @@ void or_stage(or_sim* s, int stage, int arg) {
     case SP_STAGE_RHEO: stage_rheo(s); break;
     case SP_STAGE_ELASTIC: stage_elastic(s); break;
     case SP_STAGE_SETTLE: stage_settle(s); break;
+    case SP_STAGE_MAGNET:
+        if (arg == 0 || arg == 1) stage_magnet(s, arg);
+        break;
     case SP_STAGE_RIGID:
         if (arg >= 0 && arg <= 4) stage_rigid(s, arg);
         break;
@@ -3282,6 +3378,10 @@ void or_substep(or_sim* s, int gather) {
     stage_surf_smooth(s);
     stage_surf_curv(s);
     stage_surf_apply(s);
+    if (s->P.caps & SP_CAP_MAGNET) {
+        stage_magnet(s, 0);
+        stage_magnet(s, 1);
+    }
     if (thermal(s) && s->P.heat_rate > 0.0f) stage_heat(s);
     stage_visc_before(s);
     stage_rhs(s);
@@ -3344,6 +3444,7 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->pstress = (v4*)calloc(capacity ? capacity : 1, sizeof(v4));
     s->famocc = (float*)calloc(N * SP_MAX_PRESENT, sizeof(float));
     s->dsettle = (v2*)calloc(N * SP_MAX_PRESENT, sizeof(v2));
+    s->magc = (v2*)calloc(N, sizeof(v2));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -3386,6 +3487,7 @@ void or_destroy(or_sim* s) {
     free(s->pstress);
     free(s->famocc);
     free(s->dsettle);
+    free(s->magc);
     free(s);
 }
 
@@ -3483,6 +3585,7 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_ESTR: *bytes = sizeof(v4) * (size_t)N; return s->estr;
     case SP_FIELD_PSTRESS: *bytes = sizeof(v4) * (size_t)s->n_slots; return s->pstress;
     case SP_FIELD_FAMOCC: *bytes = sizeof(float) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->famocc;
+    case SP_FIELD_MAGC: *bytes = sizeof(v2) * N; return s->magc;
     case SP_FIELD_DSETTLE: *bytes = sizeof(v2) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->dsettle;
     default: *bytes = 0; return NULL;
     }

@@ -30,7 +30,8 @@ ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
 OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
 RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
 RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
-CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 32, 64
+CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_MAGNET, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 16, 32, 64
+MAX_MAGNETS = 8
 STRESS_FIX = 65536.0
 
 # Field ids (SP_FIELD_*)
@@ -39,12 +40,12 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17,
-             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21, ELASTIC=22, SETTLE=23)
+             VISC_VCYCLE=18, HEAT=19, RIGID=20, RHEO=21, ELASTIC=22, SETTLE=23, MAGNET=24)
 
 # `struct Particle` (common.glsl:11-19): 48 bytes.
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", (2,)), ("vel", "<f4", (2,)), ("c", "<f4", (4,)),
@@ -64,7 +65,8 @@ class Material(C.Structure):
                 ("freeze_shift", C.c_float), ("rheology", C.c_uint32), ("yield_stress", C.c_float),
                 ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
                 ("skin", C.c_float), ("friction", C.c_float), ("shear_modulus", C.c_float),
-                ("relax_time", C.c_float), ("grain", C.c_float)]
+                ("relax_time", C.c_float), ("grain", C.c_float), ("susceptibility", C.c_float),
+                ("saturation", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -88,8 +90,21 @@ class SimParams(C.Structure):
                 ("eddy_heat", C.c_float), ("evaporation", C.c_uint32),
                 ("rigid_contact", C.c_uint32), ("rigid_friction", C.c_float),
                 ("elastic_stress_max", C.c_float), ("elastic_step_max", C.c_float),
+                ("magnet_demag", C.c_float),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
+
+
+class Magnet(C.Structure):
+    """`Magnet` (params.rs:1082-1094): position and radius in cells, strength in tesla on the
+    axis at the rim, north pole direction."""
+    _fields_ = [("pos", C.c_float * 2), ("radius", C.c_float), ("strength", C.c_float), ("dir", C.c_float * 2)]
+
+    @staticmethod
+    def new(pos, radius, degrees, strength):
+        """Magnet::new: north pole `degrees` clockwise from straight up."""
+        a = np.radians(np.float32(degrees))
+        return Magnet((C.c_float * 2)(*pos), radius, strength, (C.c_float * 2)(float(np.sin(a)), float(np.cos(a))))
 
 
 class ToolState(C.Structure):
@@ -161,7 +176,9 @@ class GParams(C.Structure):
                 ("mg_half", C.c_int32), ("heat_rate", C.c_float), ("film", C.c_float),
                 ("radiation", C.c_float), ("contact", C.c_float), ("eddy", C.c_float), ("evap", C.c_float),
                 ("chan_t", C.c_int32), ("chan_e", C.c_int32), ("el_g", C.c_float),
-                ("el_stress_max", C.c_float), ("el_step_max", C.c_float)]
+                ("el_stress_max", C.c_float), ("el_step_max", C.c_float), ("mag", C.c_float * 4),
+                ("magnet", (C.c_float * 4) * MAX_MAGNETS), ("magnet_dir", (C.c_float * 2) * MAX_MAGNETS),
+                ("demag", C.c_float)]
 
 
 class DeviceView(C.Structure):
@@ -229,7 +246,8 @@ SIGNATURES = {
     "sp_shares_a_group": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "sp_default_params": ([C.POINTER(SimParams)], None),
     "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32,
-                               C.POINTER(GParams), _P], C.c_int),
+                               _P, C.c_uint32, C.POINTER(GParams), _P], C.c_int),
+    "sp_set_magnets": ([_P, _P, C.c_uint32], C.c_int),
     "sp_levels": ([C.c_int32, C.c_int32, C.POINTER(LevelsInfo)], C.c_int),
     "sp_debug_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "sp_debug_upload": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
@@ -349,12 +367,15 @@ def default_params():
     return p
 
 
-def derive_grid_params(params, W, H, rows, thermal=False):
-    """sp_derive_grid_params: the grid-unit block and material table for a tank of `rows`."""
+def derive_grid_params(params, W, H, rows, thermal=False, magnets=()):
+    """sp_derive_grid_params: the grid-unit block and material table for a tank of `rows`
+    with `magnets` (Magnet records) placed."""
     g = GParams()
     mats = (GMat * MAX_MATERIALS)()
     arr = (C.c_int32 * max(len(rows), 1))(*rows)
-    rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), int(thermal), C.byref(g), mats)
+    mg = (Magnet * max(len(magnets), 1))(*magnets)
+    rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), int(thermal), mg, len(magnets),
+                                     C.byref(g), mats)
     if rc != OK:
         raise SproutError(rc, "derive_grid_params")
     return g, mats

@@ -62,6 +62,9 @@ struct Ctx {
     // settling of suspensions (settle.comp), per cell and concentration family (cell * n_fam + f)
     float* famocc;        // the family's occupancy of the cell
     float2* dsettle;      // this substep's gain and loss factors of the family's concentration
+    // magnetism (magnet.comp): per cell the susceptibility of its liquid (grid_prep) and the
+    // magnetic energy density (the magnet pass, read by the next substep's grid_prep too)
+    float2* magc;
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -142,6 +145,7 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.pstress = (float4*)take(sizeof(float4) * (r.cap ? r.cap : 1));
     c.famocc = (float*)take(sizeof(float) * N * SP_MAX_PRESENT);
     c.dsettle = (float2*)take(sizeof(float2) * N * SP_MAX_PRESENT);
+    c.magc = (float2*)take(sizeof(float2) * N);
     if (bytes) *bytes = o;
     return c;
 }

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
     float T = P.heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
     float4 nn = make_float4(0.0f, -1.0f, 1.0f, 0.0f);  // rh: the cell's non-Newtonian material
+    float chi = 0.0f;
     for (int q = 0; q < nmix; q++) c.mixb[(size_t)i * nmix + q] = -1.0f;
     // each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147)
     const bool settle = (P.caps & SP_CAP_SETTLE) != 0u;
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
                 top_share = share;
                 top_row = r;
             }
+            chi += share * m.gr[3];  // the cell's susceptibility (grid_prep.comp:168)
             int g = m.group;
             f[g] += share;
             rg[g] += share * rho_m;
@@ -258,6 +260,15 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     if (has && level <= 0.0f && cx > 0 && cy > 0 && cx < W - 1 && cy < H - 1) {
         if ((packed(c, i - 1) && packed(c, i + 1)) || (packed(c, i - W) && packed(c, i + W))) level = 0.05f;
         else if (P.gravity > 0.0f && nn.x <= 0.0f && phi_t > 0.25f && packed(c, i - W)) level = 0.05f;
+        // with a magnet, "lying on" is towards gravity and the last substep's magnetic pull
+        // together (grid_prep.comp:211-221)
+        else if ((P.caps & SP_CAP_MAGNET) && nn.x <= 0.0f && phi_t > 0.25f && chi > 0.0f) {
+            const float km = chi / P.mag[1] / fmaxf_(rho, 0.05f);
+            const float px = -km * 0.5f * (c.magc[i + 1].y - c.magc[i - 1].y) - 0.0f;
+            const float py = -km * 0.5f * (c.magc[i + W].y - c.magc[i - W].y) - P.gravity;
+            const int toward = fabsf(px) > fabsf(py) ? (px > 0.0f ? i + 1 : i - 1) : (py > 0.0f ? i + W : i - W);
+            if (packed(c, toward)) level = 0.05f;
+        }
     }
     if (sol.w > 0.0f && level <= 0.0f) {
         uint32_t bit = 1u << (c.region[i] & SP_REGION_MASK);
@@ -278,6 +289,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
         c.estr[i] = make_float4((float)c.acc[k + P.chan_e] * sc, (float)c.acc[k + P.chan_e + 1] * sc,
                                 (float)c.acc[k + P.chan_e + 2] * sc, 0.0f);
     }
+    if (P.caps & SP_CAP_MAGNET) c.magc[i].x = chi;  // grid_prep.comp:240
     c.cell[i] = make_float4(phi_f, rho, mu, level);
     c.comp[i] = make_float4(rho, f[1], fam0, phi_t);
     uint32_t gm = P.group_mask;

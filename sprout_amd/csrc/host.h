@@ -27,7 +27,7 @@ void default_params(sp_sim_params* p);
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present);
 bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal = false);
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
-           std::string* why);
+           std::string* why, const std::vector<sp_magnet>& magnets = {});
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);

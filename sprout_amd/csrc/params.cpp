@@ -57,6 +57,8 @@ static sp_material default_material() {
     m.shear_modulus = 0.0f;  // material.rs:380-381
     m.relax_time = 0.0f;
     m.grain = 0.0f;  // material.rs:363
+    m.susceptibility = 0.0f;  // material.rs:382-383
+    m.saturation = 0.05f;
     return m;
 }
 
@@ -111,6 +113,7 @@ void default_params(sp_sim_params* p) {
     p->rigid_friction = 0.6f;
     p->elastic_stress_max = 10.0f;  // params.rs:447-448
     p->elastic_step_max = 0.25f;
+    p->magnet_demag = 0.0f;  // params.rs:404
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -346,7 +349,7 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
 // or friction. A thermal tank turns rigid bodies on for any liquid that can freeze or has a
 // temperature law; any tank for a liquid already as viscous as rigid_viscosity, or frozen at
 // the ambient temperature.
-static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
+static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnets) {
     uint32_t caps = 0;
     if (t.families > 0) caps |= SP_CAP_FAMILIES;
     if (t.thermal) caps |= SP_CAP_THERMAL;
@@ -368,10 +371,15 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t) {
     bool settles = false;
     for (int r : t.rows) settles |= p.mat[r].diluent >= 0 && p.mat[r].grain > 0.0f;
     if (t.families > 0 && p.gravity > 0.0f && settles) caps |= SP_CAP_SETTLE;
+    // a magnet and a liquid with a susceptibility (mod.rs:2749-2751)
+    bool magnetic = false;
+    for (int r : t.rows) magnetic |= p.mat[r].susceptibility > 0.0f;
+    if (n_magnets > 0 && magnetic) caps |= SP_CAP_MAGNET;
     return caps;
 }
 
-int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why) {
+int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why,
+           const std::vector<sp_magnet>& magnets) {
     memset(&g, 0, sizeof(g));
     memset(mats, 0, sizeof(sp_gmat) * SP_MAX_MATERIALS);
     const float dx = p.cell_size;
@@ -445,8 +453,9 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         // viscoelasticity (mod.rs:2215): the shear modulus as a stress per unit density
         o.gr[1] = m.shear_modulus / (WATER_DENSITY * dx * dx);
         o.gr[2] = m.relax_time;
+        o.gr[3] = m.susceptibility;  // mod.rs:2215 el.z
     }
-    const uint32_t caps = tank_caps(p, t);
+    const uint32_t caps = tank_caps(p, t, magnets.size());
     // gpu_params (mod.rs:2238-2330).
     g.W = W;
     g.H = H;
@@ -540,6 +549,29 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.el_g = std::max(gmax, 1e-3f) / (WATER_DENSITY * dx * dx);
     g.el_stress_max = p.elastic_stress_max;
     g.el_step_max = p.elastic_step_max;
+    // magnetism (mod.rs:2258-2264, 2313, 2321): the most susceptible liquid's susceptibility
+    // and saturation (the first of equals; 0.05 T with none)
+    float chi_max = 0.0f, sat = 0.05f;
+    for (int r : t.rows)
+        if (p.mat[r].susceptibility > chi_max) {
+            chi_max = p.mat[r].susceptibility;
+            sat = p.mat[r].saturation;
+        }
+    const size_t nm = std::min<size_t>(magnets.size(), SP_MAX_MAGNETS);
+    g.mag[0] = (float)nm;
+    g.mag[1] = std::max(chi_max, 1e-6f);
+    g.mag[2] = sat;
+    g.mag[3] = 1.0f / (WATER_DENSITY * dx * dx);
+    for (size_t k = 0; k < nm; k++) {
+        const sp_magnet& mg = magnets[k];
+        g.magnet[k][0] = mg.pos[0];
+        g.magnet[k][1] = mg.pos[1];
+        g.magnet[k][2] = std::max(mg.radius, 0.5f);
+        g.magnet[k][3] = mg.strength;
+        g.magnet_dir[k][0] = mg.dir[0];
+        g.magnet_dir[k][1] = mg.dir[1];
+    }
+    g.demag = std::min(std::max(p.magnet_demag, 0.0f), 1.0f);
     return SP_OK;
 }
 

@@ -252,6 +252,9 @@ void launch_grid_passes(const Run& r, const Recs& recs) {
     hipLaunchKernelGGL(k_coef_smooth, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_curv, g0, TILE, 0, r.stream, r.tb);
     hipLaunchKernelGGL(k_surf_apply, g0, TILE, 0, r.stream, r.tb);
+    // the magnetic energy, then its force on the faces (mod.rs:2655-2662)
+    if (r.gp.caps & SP_CAP_MAGNET)
+        for (int mode = 0; mode < 2; mode++) hipLaunchKernelGGL(k_magnet, g0, TILE, 0, r.stream, r.tb, mode);
     // heat conduction while the tank is thermal (record_substep, mod.rs:2663-2666)
     if ((r.gp.caps & SP_CAP_THERMAL) && r.gp.heat_rate > 0.0f) hipLaunchKernelGGL(k_heat, g0, TILE, 0, r.stream, r.tb);
     launch_visc_before(r, recs);
@@ -408,7 +411,7 @@ uint32_t schedule_key(const sp_gparams& g) {
            (uint32_t)g.v_mg_levels << 5 | (uint32_t)g.v_mg_sweeps << 10 | (uint32_t)g.mg_half << 12 |
            (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14 |
            (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15 | (uint32_t)((g.caps & SP_CAP_ELASTIC) != 0u) << 16 |
-           (uint32_t)((g.caps & SP_CAP_SETTLE) != 0u) << 17;
+           (uint32_t)((g.caps & SP_CAP_SETTLE) != 0u) << 17 | (uint32_t)((g.caps & SP_CAP_MAGNET) != 0u) << 18;
 }
 
 }  // namespace

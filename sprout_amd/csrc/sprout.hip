@@ -26,6 +26,7 @@
 #include "rigid.hip"
 #include "elastic.hip"
 #include "settle.hip"
+#include "magnet.hip"
 #include "schedule.hip"
 
 namespace {
@@ -59,6 +60,7 @@ struct sp_sim {
     size_t arena_bytes = 0;
     uint64_t generation = 0;  // bumped when the arena moves
     uint64_t resets = 0;      // sp_reset calls (FluidSim::resets)
+    std::vector<sp_magnet> magnets;  // placed in the world (FluidSim::magnets); survive a reset
 
     sp_sim_params params{};
     bool params_set = false;
@@ -167,7 +169,7 @@ int upload_tables(sp_sim* s) {
         }
     }
     std::string why;
-    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why);
+    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets);
     if (rc != SP_OK) {
         set_err(s, "%s", why.c_str());
         return rc;
@@ -477,13 +479,14 @@ int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out) {
 }
 
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h, const int32_t* rows, int32_t n_rows,
-                          int32_t thermal, sp_gparams* out, sp_gmat* mats) {
-    if (!params || !out || !mats || (n_rows > 0 && !rows) || n_rows < 0) return SP_E_INVALID;
+                          int32_t thermal, const sp_magnet* magnets, uint32_t n_magnets, sp_gparams* out, sp_gmat* mats) {
+    if (!params || !out || !mats || (n_rows > 0 && !rows) || n_rows < 0 || (n_magnets > 0 && !magnets)) return SP_E_INVALID;
     std::vector<int> present(rows, rows + n_rows);
     std::vector<int> r = sp::closure(*params, present);
     sp::Tank t;
     if (!sp::tank_layout(*params, r, t, thermal != 0)) return SP_E_CAPACITY;
-    return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr);
+    std::vector<sp_magnet> mg(magnets, magnets + std::min<uint32_t>(n_magnets, SP_MAX_MAGNETS));
+    return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr, mg);
 }
 
 int sp_create(const sp_config* cfg, sp_sim** out) {
@@ -897,6 +900,15 @@ int sp_stats(sp_sim* s, sp_stats_out* out) {
     });
 }
 
+int sp_set_magnets(sp_sim* s, const sp_magnet* m, uint32_t n) {
+    return guarded(s, [&]() -> int {
+        if (n > 0 && !m) return SP_E_INVALID;
+        s->magnets.assign(m, m + std::min<uint32_t>(n, SP_MAX_MAGNETS));
+        s->tables_dirty = true;
+        return SP_OK;
+    }, false);
+}
+
 int sp_sort(sp_sim* s) {
     return guarded(s, [&]() -> int {
         int rc = upload_tables(s);
@@ -1142,6 +1154,7 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_PSTRESS: *bytes = 16 * (size_t)s->n_slots; return c.pstress;
     case SP_FIELD_FAMOCC: *bytes = 4 * N * (size_t)std::max(s->gp.n_fam, 1); return c.famocc;
     case SP_FIELD_DSETTLE: *bytes = 8 * N * (size_t)std::max(s->gp.n_fam, 1); return c.dsettle;
+    case SP_FIELD_MAGC: *bytes = 8 * N; return c.magc;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1243,6 +1256,10 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
         case SP_STAGE_RHEO: hipLaunchKernelGGL(k_rheo, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_ELASTIC: hipLaunchKernelGGL(k_elastic, g0, TILE, 0, s->stream, cs); break;
         case SP_STAGE_SETTLE: hipLaunchKernelGGL(k_settle, g0, TILE, 0, s->stream, cs); break;
+        case SP_STAGE_MAGNET:
+            if (arg < 0 || arg > 1) return SP_E_INVALID;
+            hipLaunchKernelGGL(k_magnet, g0, TILE, 0, s->stream, cs, arg);
+            break;
         case SP_STAGE_RIGID:
             if (arg < 0 || arg > 4) return SP_E_INVALID;
             hipLaunchKernelGGL(k_rigid, g0, TILE, 0, s->stream, cs, arg);

@@ -146,6 +146,11 @@ class FluidSim:
     def sort(self):
         self._check(self._lib.sp_sort(self._h), "sort")
 
+    def set_magnets(self, magnets):
+        """FluidSim::set_magnets (mod.rs:2018-2022): a list of N.Magnet."""
+        arr = (N.Magnet * max(len(magnets), 1))(*magnets)
+        self._check(self._lib.sp_set_magnets(self._h, arr, len(magnets)), "set_magnets")
+
     # -- render-facing outputs (mod.rs:1608-1943) ---------------------------------------
     def render_view(self, which):
         """The device view (address, bytes, count, stride, device, generation) of a render

@@ -291,3 +291,30 @@ def mud_column(grid, block=(0.3, 0.7, 0.0, 0.6), cell_size=0.005, pressure_tol=1
         parts.append(sc.fill_rect(scene, (x0, y1, x1, y1 + water_above * H), N.ROW_WATER, len(mud)))
     parts = np.concatenate(parts)
     return params, scene.rasterize(), settle(parts, params), [N.ROW_WATER, N.ROW_HONEY]
+
+
+ROW_FERRO = 4  # a built-in row with the default material, made magnetisable by ferro_params
+
+
+def ferro_params(params, row=ROW_FERRO, susceptibility=2.0, saturation=0.05, density=1200.0, viscosity=5e-3):
+    """Makes `row` a ferrofluid (magnetisable liquid): volume susceptibility `susceptibility`,
+    saturation mu0 Ms `saturation` T."""
+    m = params.mat[row]
+    m.susceptibility = susceptibility
+    m.saturation = saturation
+    m.density = density
+    m.viscosity = viscosity
+    return params
+
+
+def ferro_pool(grid, depth=0.3, magnet_gap=6.0, strength=0.3, radius=4.0, cell_size=0.005, pressure_tol=1e-6, **mat):
+    """A pool of ferrofluid `depth` (fraction of H) deep on the floor of a tank, and a magnet
+    `magnet_gap` cells above its surface at the middle, north pole up. (params, solid field,
+    particles, rows, magnets)."""
+    params = ferro_params(dam_params(cell_size=cell_size, pressure_tol=pressure_tol), **mat)
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (0.0, 0.0, float(W), depth * H), ROW_FERRO)
+    top = max(depth * H, float(sc.WALL))
+    magnets = [N.Magnet.new((0.5 * W, top + magnet_gap), radius, 0.0, strength)]
+    return params, scene.rasterize(), settle(parts, params), [ROW_FERRO], magnets

@@ -67,7 +67,8 @@ class Pair:
                   "SURF2": (self.N, 4), "GBASE": (self.N, 4), "VW": (self.N, 4), "VS": (self.N, 4),
                   "P2KEEP": (self.N,), "THERM": (self.N, 4), "DHEAT": (self.N, 4), "SOLID_T": (self.N,),
                   "ESTR": (self.N, 4), "PSTRESS": (self.ora.n_slots(), 4),
-                  "FAMOCC": (self.N * max(self.g.n_fam, 1),), "DSETTLE": (self.N * max(self.g.n_fam, 1), 2)}
+                  "FAMOCC": (self.N * max(self.g.n_fam, 1),), "DSETTLE": (self.N * max(self.g.n_fam, 1), 2),
+                  "MAGC": (self.N, 2)}
         for v in ("VR", "VP", "VP2", "VQ", "VD", "VR2", "VQ2", "VAZ"):
             shapes[v] = (self.N, 2)
         for v in ("VMA", "VMB"):

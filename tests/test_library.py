@@ -44,7 +44,7 @@ def test_python_structs_match_the_header(tmp_path):
     import subprocess
     pairs = {"sp_material": N.Material, "sp_sim_params": N.SimParams, "sp_gmat": N.GMat,
              "sp_gparams": N.GParams, "sp_solver_stats": N.SolverStats, "sp_levels_info": N.LevelsInfo,
-             "sp_server_stats": N.ServerStats}
+             "sp_server_stats": N.ServerStats, "sp_magnet": N.Magnet, "sp_device_view": N.DeviceView}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sprout_hip.h"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -72,10 +72,10 @@ def test_errors_cross_the_abi_as_codes():
     p = N.default_params()
     g = N.GParams()
     mats = (N.GMat * N.MAX_MATERIALS)()
-    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, -1, 0, C.byref(g), mats) == N.E_INVALID
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, -1, 0, None, 0, C.byref(g), mats) == N.E_INVALID
     # thirteen materials do not fit a tank (MAX_PRESENT 12, mod.rs:2051-2081)
     rows = (C.c_int32 * 13)(*range(13))
-    assert L.sp_derive_grid_params(C.byref(p), 64, 64, rows, 13, 0, C.byref(g), mats) == N.E_CAPACITY
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, rows, 13, 0, None, 0, C.byref(g), mats) == N.E_CAPACITY
 
 
 def test_viscosity_modes_follow_the_diffusion_number():
