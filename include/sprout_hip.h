@@ -107,6 +107,10 @@ int sp_download_pressure(sp_sim* sim, float* pres, float* dt);
  * gained from solids, [3] lost to solids, [4]-[7] heat tool, cool tool, boiled off, made
  * inside (0 here: those passes are not carried). */
 int sp_download_ledger(sp_sim* sim, double* energy8);
+/* The counts of `FluidSim::download_ledger` (src/sim/mod.rs:3400-3404): per table row the
+ * particles boiled or evaporated away, the residues left and the steam made since the last
+ * reset (SP_MAX_MATERIALS each; any pointer may be NULL). Only evaporation counts here. */
+int sp_download_ledger_counts(sp_sim* sim, uint32_t* boiled, uint32_t* residue, uint32_t* steam);
 /* `FluidSim::cell_counts(lo, hi)` (src/sim/mod.rs:3354-3386, count.comp): live particles
  * per cell in [x0, x0+w) x [y0, y0+h), row by row, into out[w*h]. */
 int sp_cell_counts(sp_sim* sim, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out);

@@ -96,6 +96,10 @@ typedef struct sp_material {
     /* Magnetism (material.rs:236-238): the volume susceptibility (SI; 0 = not magnetic) and
      * the saturation magnetisation as mu0 Ms (T). */
     float susceptibility, saturation;
+    /* Evaporation below the boiling point (material.rs:307-310, 326-327): the share of the
+     * liquid at an open surface that evaporates per second at 20 deg C (doubling every 10 K;
+     * 0 = none), and how much visible vapour it gives off (0 here: puffs are not carried). */
+    float volatility, fume;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -217,6 +221,7 @@ typedef struct sp_gmat {
     float temp[4];   /* temperature law: B, C (K); reference temperature (K); skin (cells^3/s^2) */
     float rheo_p[4]; /* yield stress (cells^2/s^2), flow index, shear time (s), viscosity ratio */
     float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); susceptibility (el.z) */
+    float rx[4];     /* GpuMat rx: reaction rate, heat (0 here: reactions are not carried); self-heating (0); volatility (1/s) */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -348,6 +353,9 @@ typedef struct sp_levels_info {
 #define SP_CAP_ELASTIC 32u
 #define SP_CAP_SETTLE 64u
 #define SP_CAP_MAGNET 16u
+/* Not a reference capability: set where a liquid in the tank evaporates (volatility > 0), so
+ * that the particle kernel with the evaporation code runs only there. */
+#define SP_CAP_VOLATILE 0x10000u
 #define SP_MU0 1.25663706e-6f  /* magnet.comp MU0 */
 /* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
  * accumulators */
@@ -356,7 +364,9 @@ typedef struct sp_levels_info {
 #define SP_NO_TEMP (-1e9f)
 #define SP_LEDGER_SCALE 65536.0f
 #define SP_T_SCALE 4096.0f
-#define SP_LEDGER_WORDS 16
+/* 8 energy entries as 64-bit fixed point, then per table row the particles boiled or
+ * evaporated away, the residues left, the steam made (mod.rs:82 LEDGER_WORDS) */
+#define SP_LEDGER_WORDS (16 + 3 * SP_MAX_MATERIALS)
 /* Rigid bodies (rigid.comp, common.glsl:263-275; mod.rs:77-78) */
 #define SP_RIGID_BODIES 4096
 #define SP_BODY_WORDS 32
@@ -467,7 +477,7 @@ enum {
 #endif
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
-SP_STATIC_ASSERT(sizeof(sp_gmat) == 160, "one grid-unit material row is 40 words");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 176, "one grid-unit material row is 44 words");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 
 #endif

@@ -4,7 +4,8 @@
  * times the CPU baseline; nothing in the product calls it.
  *
  * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity, settling and
- * magnetism, without steam, boiling and reactions, tools, currents or film spatter:
+ * magnetism and evaporation, without steam, boiling and reactions, tools, currents or film
+ * spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
@@ -856,6 +857,32 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
             corry = 0.0f;
         }
         /* Speed cap (particles.comp:831-835). */
+        /* Evaporation below the boiling point (particles.comp:771-807): a particle of a
+         * volatile liquid in a cell open to the air leaves with probability rate x dt, the
+         * rate doubling every 10 K, a concentrate's scaled by its concentration; it takes its
+         * heat with it (ledger entry 6) and is counted. */
+        if (s->mats[row].rx[3] > 0.0f) {
+            int qx = clampi((int)p.pos[0], 1, s->W - 2), qy = clampi((int)p.pos[1], 1, s->H - 2);
+            int qi = cidx(s, qx, qy);
+            int open_air = 0;
+            for (int q = 0; q < 4; q++) {
+                int n = qi + (q == 0 ? -1 : q == 1 ? 1 : q == 2 ? -s->W : s->W);
+                if (s->cell[n].w <= 0.0f && s->solid[n].w > 0.5f) open_air = 1;
+            }
+            if (open_air) {
+                v4 th = thermal_of(s, row, p.s);
+                float Tp = th_on ? phase_of(p.t, th).x : P->heat_x;
+                float rate = s->mats[row].rx[3] * (s->mats[row].diluent >= 0 ? p.s : 1.0f) *
+                             exp2f(clampf((Tp - 20.0f) / 10.0f, -6.0f, 5.0f));
+                float u = (float)(hash_u(p.id * 0x85ebca6bu ^ hash_u(s->S.substeps + 0x27d4eb2fu)) >> 8) / 16777216.0f;
+                if (u < rate * dt) {
+                    if (th_on) ledger_add(s, 6, fmaxf_(th.w * (p.t + 273.15f), 0.0f) * P->inv_ppc);
+                    s->ledger[16 + row] += 1u;
+                    s->part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+            }
+        }
         float vcap = P->ext[0];
         {
             float sp = len2(vx, vy);

@@ -32,6 +32,8 @@ RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
 RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
 CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_MAGNET, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 16, 32, 64
 MAX_MAGNETS = 8
+CAP_VOLATILE = 0x10000
+LEDGER_WORDS = 16 + 3 * 96
 STRESS_FIX = 65536.0
 
 # Field ids (SP_FIELD_*)
@@ -66,7 +68,7 @@ class Material(C.Structure):
                 ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
                 ("skin", C.c_float), ("friction", C.c_float), ("shear_modulus", C.c_float),
                 ("relax_time", C.c_float), ("grain", C.c_float), ("susceptibility", C.c_float),
-                ("saturation", C.c_float)]
+                ("saturation", C.c_float), ("volatility", C.c_float), ("fume", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -153,7 +155,7 @@ class GMat(C.Structure):
                                           "grain", "rho_solid", "visc_exp", "expansion", "load",
                                           "freeze_shift")] + \
                [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4),
-                ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4)]
+                ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4), ("rx", C.c_float * 4)]
 
 
 class GParams(C.Structure):
@@ -248,6 +250,7 @@ SIGNATURES = {
     "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32,
                                _P, C.c_uint32, C.POINTER(GParams), _P], C.c_int),
     "sp_set_magnets": ([_P, _P, C.c_uint32], C.c_int),
+    "sp_download_ledger_counts": ([_P, _P, _P, _P], C.c_int),
     "sp_levels": ([C.c_int32, C.c_int32, C.POINTER(LevelsInfo)], C.c_int),
     "sp_debug_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "sp_debug_upload": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),

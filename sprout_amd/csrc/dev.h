@@ -367,6 +367,13 @@ SP_DEV float row_lnu(const Ctx& c, int row, float s, float T, bool thermal) {
     return l;
 }
 
+// Adds v >= 0 to the ledger's 64-bit fixed-point energy entry e (ledger_add, common.glsl:
+// 446-451: the quantum floor(v LEDGER_SCALE + 0.5) with a carry into the high word).
+SP_DEV void ledger_add(const Ctx& c, int e, float v) {
+    const uint64_t q = (uint64_t)(uint32_t)(v * SP_LEDGER_SCALE + 0.5f);
+    atomicAdd(reinterpret_cast<unsigned long long*>(c.ledger + 2 * e), (unsigned long long)q);
+}
+
 // Terminal speed of a sphere of diameter d (cells) and density difference drho through a
 // liquid of density rho_c and viscosity nu_c, Stokes to Newton drag (common.glsl:370-374).
 SP_DEV float terminal(const Ctx& c, float drho, float rho_c, float nu_c, float d) {

@@ -59,6 +59,8 @@ static sp_material default_material() {
     m.grain = 0.0f;  // material.rs:363
     m.susceptibility = 0.0f;  // material.rs:382-383
     m.saturation = 0.05f;
+    m.volatility = 0.0f;  // material.rs:404, 412
+    m.fume = 0.0f;
     return m;
 }
 
@@ -375,6 +377,8 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
     bool magnetic = false;
     for (int r : t.rows) magnetic |= p.mat[r].susceptibility > 0.0f;
     if (n_magnets > 0 && magnetic) caps |= SP_CAP_MAGNET;
+    for (int r : t.rows)
+        if (p.mat[r].volatility > 0.0f) caps |= SP_CAP_VOLATILE;
     return caps;
 }
 
@@ -454,8 +458,16 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.gr[1] = m.shear_modulus / (WATER_DENSITY * dx * dx);
         o.gr[2] = m.relax_time;
         o.gr[3] = m.susceptibility;  // mod.rs:2215 el.z
+        o.rx[3] = m.volatility;      // mod.rs:2231
     }
     const uint32_t caps = tank_caps(p, t, magnets.size());
+    // a liquid that evaporates as visible vapour leaves puffs (particles.comp:788-794), which
+    // this build does not carry
+    for (int r : t.rows)
+        if (p.mat[r].volatility > 0.0f && p.mat[r].fume > 0.0f) {
+            if (why) *why = "a fuming liquid evaporates into puffs of vapour, which are not carried";
+            return SP_E_UNSUPPORTED;
+        }
     // gpu_params (mod.rs:2238-2330).
     g.W = W;
     g.H = H;

@@ -191,7 +191,8 @@ SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, fl
 
 // Gather, drift, exchange, advect and collide one particle (particles.comp:98-985, the
 // branches this build carries); fills the scatter inputs. X: the tank may be elastic or
-// settle (that code is compiled only into the kernels such tanks run).
+// settle, or hold a volatile liquid (that code is compiled only into the kernels such tanks
+// run).
 template <bool X>
 SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& umax) {
     const sp_gparams& P = *c.P;
@@ -492,6 +493,32 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
         if (held && crowd <= SP_REST_PHI + fmaxf_(P.corr[1], 0.05f)) {
             corrx = 0.0f;
             corry = 0.0f;
+        }
+        // Evaporation below the boiling point (particles.comp:771-807): a particle of a volatile
+        // liquid in a cell open to the air (not one that only touches a solid) leaves with
+        // probability rate x dt, the rate doubling every 10 K and a concentrate's scaled by its
+        // concentration; it takes its heat with it (ledger entry 6) and is counted.
+        if (X && (P.caps & SP_CAP_VOLATILE) && c.mats[row].rx[3] > 0.0f) {
+            const int qx = clampi_((int)p.pos[0], 1, c.W - 2), qy = clampi_((int)p.pos[1], 1, c.H - 2);
+            const int qi = cidx(c, qx, qy);
+            bool open_air = false;
+            for (int q = 0; q < 4; q++) {
+                const int n = qi + (q == 0 ? -1 : q == 1 ? 1 : q == 2 ? -c.W : c.W);
+                if (c.cell[n].w <= 0.0f && c.solid[n].w > 0.5f) open_air = true;
+            }
+            if (open_air) {
+                const float4 th = thermal_of(c, row, p.s);
+                const float Tp = th_on ? phase_of(p.t, th).x : P.heat_x;
+                const float rate = c.mats[row].rx[3] * (c.mats[row].diluent >= 0 ? p.s : 1.0f) *
+                                   exp2f(clampf_((Tp - 20.0f) / 10.0f, -6.0f, 5.0f));
+                const float u = (float)(hash_u(p.id * 0x85ebca6bu ^ hash_u(c.S->substeps + 0x27d4eb2fu)) >> 8) / 16777216.0f;
+                if (u < rate * dt) {
+                    if (th_on) ledger_add(c, 6, fmaxf_(th.w * (p.t + 273.15f), 0.0f) * P.inv_ppc);
+                    atomicAdd(&c.ledger[16 + row], 1u);
+                    c.part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+            }
         }
         float vcap = P.ext[0];
         {

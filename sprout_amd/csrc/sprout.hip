@@ -865,6 +865,18 @@ int sp_download_ledger(sp_sim* s, double* energy8) {
     });
 }
 
+int sp_download_ledger_counts(sp_sim* s, uint32_t* boiled, uint32_t* residue, uint32_t* steam) {
+    return guarded(s, [&]() -> int {
+        std::vector<uint32_t> w(SP_LEDGER_WORDS);
+        HIPCHK(hipMemcpyAsync(w.data(), s->c.ledger, sizeof(uint32_t) * SP_LEDGER_WORDS, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        uint32_t* out[3] = {boiled, residue, steam};
+        for (int k = 0; k < 3; k++)
+            if (out[k]) memcpy(out[k], w.data() + 16 + k * SP_MAX_MATERIALS, sizeof(uint32_t) * SP_MAX_MATERIALS);
+        return SP_OK;
+    });
+}
+
 int sp_cell_counts(sp_sim* s, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out) {
     return guarded(s, [&]() -> int {
         if (!out || w < 0 || h < 0 || x0 < 0 || y0 < 0 || x0 + w > s->W || y0 + h > s->H) return SP_E_INVALID;

@@ -146,6 +146,13 @@ class FluidSim:
     def sort(self):
         self._check(self._lib.sp_sort(self._h), "sort")
 
+    def download_ledger_counts(self):
+        """Per table row: (boiled or evaporated away, residues left, steam made) since the last
+        reset (FluidSim::download_ledger's counts, mod.rs:3400-3404)."""
+        out = [np.zeros(N.MAX_MATERIALS, np.uint32) for _ in range(3)]
+        self._check(self._lib.sp_download_ledger_counts(self._h, *(o.ctypes.data for o in out)), "download_ledger_counts")
+        return tuple(out)
+
     def set_magnets(self, magnets):
         """FluidSim::set_magnets (mod.rs:2018-2022): a list of N.Magnet."""
         arr = (N.Magnet * max(len(magnets), 1))(*magnets)

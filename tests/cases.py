@@ -318,3 +318,17 @@ def ferro_pool(grid, depth=0.3, magnet_gap=6.0, strength=0.3, radius=4.0, cell_s
     top = max(depth * H, float(sc.WALL))
     magnets = [N.Magnet.new((0.5 * W, top + magnet_gap), radius, 0.0, strength)]
     return params, scene.rasterize(), settle(parts, params), [ROW_FERRO], magnets
+
+
+def volatile_pool(grid, volatility=2.0, depth=0.3, cell_size=0.005, pressure_tol=1e-6):
+    """A pool of a volatile liquid (the default material's row 5 with `volatility` 1/s) on the
+    floor of a tank, isothermal at 20 deg C. (params, solid field, particles, rows)."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.mat[ROW_VOLATILE].volatility = volatility
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (0.0, 0.0, float(W), depth * H), ROW_VOLATILE)
+    return params, scene.rasterize(), settle(parts, params), [ROW_VOLATILE]
+
+
+ROW_VOLATILE = 5  # a built-in row with the default material, made volatile by volatile_pool

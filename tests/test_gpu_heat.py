@@ -36,8 +36,8 @@ def prefix(p, gather=0):
 
 
 def ledger_words(p):
-    a = p.gpu.field("LEDGER", np.uint32, 16).astype(np.uint64)
-    b = p.ora.field("LEDGER", np.uint32, 16).astype(np.uint64)
+    a = p.gpu.field("LEDGER", np.uint32, N.LEDGER_WORDS)[:16].astype(np.uint64)
+    b = p.ora.field("LEDGER", np.uint32, N.LEDGER_WORDS)[:16].astype(np.uint64)
     la = [int(a[2 * e]) | int(a[2 * e + 1]) << 32 for e in range(8)]
     lb = [int(b[2 * e]) | int(b[2 * e + 1]) << 32 for e in range(8)]
     return la, lb

@@ -88,7 +88,7 @@ class Pair:
             elif n == "HCELL":
                 self.gpu.put_field("HCELL", self.ora.field("HCELL", np.int32, 2 * self.N))
             elif n == "LEDGER":
-                self.gpu.put_field("LEDGER", self.ora.field("LEDGER", np.uint32, 16))
+                self.gpu.put_field("LEDGER", self.ora.field("LEDGER", np.uint32, N.LEDGER_WORDS))
             elif n == "RL":
                 self.gpu.put_field("RL", self.ora.field("RL", np.int32, 2 * self.N))
             elif n == "RBODY":

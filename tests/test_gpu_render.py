@@ -65,7 +65,7 @@ def test_views_hold_what_the_tank_reports():
             "VELOCITY": (vel.view(np.uint8).reshape(-1), W * H, 16),
             "COMPOSITION": (sim.field("COMP", np.float32, (W * H, 4)).view(np.uint8).reshape(-1), W * H, 16),
             "SURFACE": (sim.field("SPROP", np.float32, (W * H, 4)).view(np.uint8).reshape(-1), W * H, 16),
-            "MATERIALS": (np.frombuffer(bytes(mats), np.uint8), N.MAX_MATERIALS, 160),
+            "MATERIALS": (np.frombuffer(bytes(mats), np.uint8), N.MAX_MATERIALS, C.sizeof(N.GMat)),
             "SDF": (np.ascontiguousarray(field.sdf, np.float32).view(np.uint8).reshape(-1), (W + 1) * (H + 1), 4),
         }
         for which, (data, count, stride) in want.items():

@@ -48,7 +48,7 @@ def heat(o, mats):
 
 
 def ledger(o):
-    L = o.field("LEDGER", np.uint32, 16).astype(np.uint64)
+    L = o.field("LEDGER", np.uint32, N.LEDGER_WORDS)[:16].astype(np.uint64)
     return [float((int(L[2 * e]) | int(L[2 * e + 1]) << 32) / 65536.0) for e in range(8)]
 
 
