@@ -100,6 +100,11 @@ typedef struct sp_material {
      * liquid at an open surface that evaporates per second at 20 deg C (doubling every 10 K;
      * 0 = none), and how much visible vapour it gives off (0 here: puffs are not carried). */
     float volatility, fume;
+    /* Reactions and heat made inside (material.rs:305-320): in a cell that also holds row
+     * `react_with` (-1: none) the liquid turns into row `react_into` (-1: it is used up) at
+     * `react_rate` per second, giving off `react_heat` J/kg; `self_heat` W/kg of its own. */
+    int32_t react_with, react_into;
+    float react_rate, react_heat, self_heat;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -221,7 +226,8 @@ typedef struct sp_gmat {
     float temp[4];   /* temperature law: B, C (K); reference temperature (K); skin (cells^3/s^2) */
     float rheo_p[4]; /* yield stress (cells^2/s^2), flow index, shear time (s), viscosity ratio */
     float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); susceptibility (el.z) */
-    float rx[4];     /* GpuMat rx: reaction rate, heat (0 here: reactions are not carried); self-heating (0); volatility (1/s) */
+    float rx[4];     /* GpuMat rx: reaction rate (1/s), heat of reaction (K of the liquid), self-heating (K/s), volatility (1/s) */
+    int32_t ic[4];   /* GpuMat ic: the partner row while it is in the tank (-1: no reaction), the product row, the partner's index among the tank's rows, 0 */
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -356,6 +362,10 @@ typedef struct sp_levels_info {
 /* Not a reference capability: set where a liquid in the tank evaporates (volatility > 0), so
  * that the particle kernel with the evaporation code runs only there. */
 #define SP_CAP_VOLATILE 0x10000u
+/* Not a reference capability either: a thermal tank with a liquid that heats itself. */
+#define SP_CAP_SELF_HEAT 0x20000u
+#define SP_CAP_REACT 256u
+#define SP_REACT_MIN 0.02f  /* common.glsl:259 */
 #define SP_MU0 1.25663706e-6f  /* magnet.comp MU0 */
 /* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
  * accumulators */
@@ -432,6 +442,7 @@ enum {
     SP_FIELD_FAMOCC = 53,    /* float[N*n_fam] occupancy of each concentration family per cell */
     SP_FIELD_DSETTLE = 54,   /* float2[N*n_fam] settling's gain and loss factors per cell and family */
     SP_FIELD_MAGC = 55,      /* float2[N] susceptibility of the cell's liquid, magnetic energy density (grid units) */
+    SP_FIELD_RMASK = 56,     /* uint32[N] bit j: the tank's j-th row makes up more than REACT_MIN of the cell */
     SP_FIELD_COUNT
 };
 
@@ -477,7 +488,7 @@ enum {
 #endif
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
-SP_STATIC_ASSERT(sizeof(sp_gmat) == 176, "one grid-unit material row is 44 words");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 192, "one grid-unit material row is 48 words");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 
 #endif

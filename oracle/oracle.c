@@ -4,8 +4,8 @@
  * times the CPU baseline; nothing in the product calls it.
  *
  * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity, settling and
- * magnetism and evaporation, without steam, boiling and reactions, tools, currents or film
- * spatter:
+ * magnetism, evaporation, reactions and heat made inside, without steam, boiling and
+ * residues, tools, currents or film spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
@@ -75,6 +75,8 @@ struct or_sim {
     v2* dsettle;      /* gain and loss factors of the family's concentration this substep */
     /* magnetism: per cell the susceptibility of its liquid and the magnetic energy density */
     v2* magc;
+    /* reactions: per cell, bit j where the tank's j-th row makes up more than REACT_MIN */
+    uint32_t* rmask;
 };
 
 /* ---- small helpers with the shader's float semantics ---------------------------------- */
@@ -857,6 +859,47 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
             corry = 0.0f;
         }
         /* Speed cap (particles.comp:831-835). */
+        /* Heat made inside (particles.comp:733-739), counted in the ledger. */
+        if (th_on && s->mats[row].rx[2] > 0.0f) {
+            float e = s->mats[row].rx[2] * s->mats[row].th[0] * (s->mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P->heat_rate;
+            p.t += e / thermal_of(s, row, p.s).w;
+            ledger_add(s, 7, e * P->inv_ppc);
+        }
+        /* A reaction (particles.comp:740-770): where the particle's cell also holds its
+         * partner it turns into the product with probability rate x dt, warmed by its heat
+         * of reaction; with no product it is used up and counted. */
+        if ((P->caps & SP_CAP_REACT) && s->mats[row].ic[0] >= 0) {
+            int qx = clampi((int)p.pos[0], 0, s->W - 1), qy = clampi((int)p.pos[1], 0, s->H - 1);
+            if ((s->rmask[cidx(s, qx, qy)] & (1u << (uint32_t)s->mats[row].ic[2])) != 0u) {
+                float own = s->mats[row].diluent >= 0 ? p.s : 1.0f;
+                float u = (float)(hash_u(p.id * 0x9e3779b1u ^ hash_u(s->S.substeps + 0x51ed270bu)) >> 8) / 16777216.0f;
+                if (u < s->mats[row].rx[0] * dt && own > SP_REACT_MIN) {
+                    float c0 = thermal_of(s, row, p.s).w;
+                    float e = s->mats[row].rx[1] * s->mats[row].th[0] * own;
+                    if (th_on) {
+                        p.t += e / c0;
+                        ledger_add(s, 7, e * P->inv_ppc);
+                    }
+                    int into = s->mats[row].ic[1];
+                    if (into < 0) {
+                        if (th_on) ledger_add(s, 6, fmaxf_(c0 * (p.t + 273.15f), 0.0f) * P->inv_ppc);
+                        s->ledger[16 + row] += 1u;
+                        s->part[id].kind = SP_KIND_DEAD;
+                        return;
+                    }
+                    row = into;
+                    M = &s->mats[row];
+                    Mgroup = M->group;
+                    Mdil = M->diluent;
+                    Mcw = M->chan_w;
+                    Mcs = M->chan_s;
+                    fam = M->fam;
+                    p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                    if (M->diluent < 0) p.s = 1.0f;
+                    if (th_on) p.t = keep_heat(p.t, c0, thermal_of(s, row, p.s).w);
+                }
+            }
+        }
         /* Evaporation below the boiling point (particles.comp:771-807): a particle of a
          * volatile liquid in a cell open to the air leaves with probability rate x dt, the
          * rate doubling every 10 K, a concentrate's scaled by its concentration; it takes its
@@ -1068,6 +1111,8 @@ static void stage_grid_prep(or_sim* s) {
             v4 nn = {0.0f, -1.0f, 1.0f, 0.0f};  /* rh: the cell's non-Newtonian material */
             float chi = 0.0f;
             for (int q = 0; q < nmix; q++) s->mixb[(size_t)i * nmix + q] = -1.0f;
+            int react = (P->caps & SP_CAP_REACT) != 0u;
+            if (cw <= 0 && react) s->rmask[i] = 0u; /* grid_prep.comp:71 */
             /* each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147) */
             int settle = (P->caps & SP_CAP_SETTLE) != 0u;
             if (settle)
@@ -1101,6 +1146,13 @@ static void stage_grid_prep(or_sim* s) {
                         float em = m->th2[1];
                         emis += sh[j] * (m->diluent >= 0 ? mixf(s->mats[m->diluent].th2[1], em, scn[j]) : em);
                     }
+                }
+                /* which rows the cell holds, for the reactions (grid_prep.comp:110-118) */
+                if (react) {
+                    uint32_t mask = 0u;
+                    for (int j = 0; j < P->n_present; j++)
+                        if (sh[j] * (s->mats[P->present[j]].diluent >= 0 ? scn[j] : 1.0f) > SP_REACT_MIN) mask |= 1u << (uint32_t)j;
+                    s->rmask[i] = mask;
                 }
                 /* The temperature of each phase group from its own sum (group_temperatures,
                  * common.glsl:598-622); T leaves as the cell's mean. */
@@ -3472,6 +3524,7 @@ or_sim* or_create(int32_t W, int32_t H, uint32_t capacity) {
     s->famocc = (float*)calloc(N * SP_MAX_PRESENT, sizeof(float));
     s->dsettle = (v2*)calloc(N * SP_MAX_PRESENT, sizeof(v2));
     s->magc = (v2*)calloc(N, sizeof(v2));
+    s->rmask = (uint32_t*)calloc(N, sizeof(uint32_t));
     for (size_t i = 0; i < N; i++) s->solid_t[i] = SP_NO_TEMP;
     return s;
 }
@@ -3515,6 +3568,7 @@ void or_destroy(or_sim* s) {
     free(s->famocc);
     free(s->dsettle);
     free(s->magc);
+    free(s->rmask);
     free(s);
 }
 
@@ -3613,6 +3667,7 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_PSTRESS: *bytes = sizeof(v4) * (size_t)s->n_slots; return s->pstress;
     case SP_FIELD_FAMOCC: *bytes = sizeof(float) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->famocc;
     case SP_FIELD_MAGC: *bytes = sizeof(v2) * N; return s->magc;
+    case SP_FIELD_RMASK: *bytes = sizeof(uint32_t) * N; return s->rmask;
     case SP_FIELD_DSETTLE: *bytes = sizeof(v2) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->dsettle;
     default: *bytes = 0; return NULL;
     }

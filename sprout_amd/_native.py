@@ -33,6 +33,8 @@ RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
 CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_MAGNET, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 16, 32, 64
 MAX_MAGNETS = 8
 CAP_VOLATILE = 0x10000
+CAP_SELF_HEAT = 0x20000
+CAP_REACT = 256
 LEDGER_WORDS = 16 + 3 * 96
 STRESS_FIX = 65536.0
 
@@ -42,7 +44,7 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55, RMASK=56)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
@@ -68,7 +70,9 @@ class Material(C.Structure):
                 ("flow_index", C.c_float), ("shear_time", C.c_float), ("visc_ratio", C.c_float),
                 ("skin", C.c_float), ("friction", C.c_float), ("shear_modulus", C.c_float),
                 ("relax_time", C.c_float), ("grain", C.c_float), ("susceptibility", C.c_float),
-                ("saturation", C.c_float), ("volatility", C.c_float), ("fume", C.c_float)]
+                ("saturation", C.c_float), ("volatility", C.c_float), ("fume", C.c_float),
+                ("react_with", C.c_int32), ("react_into", C.c_int32), ("react_rate", C.c_float),
+                ("react_heat", C.c_float), ("self_heat", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -155,7 +159,8 @@ class GMat(C.Structure):
                                           "grain", "rho_solid", "visc_exp", "expansion", "load",
                                           "freeze_shift")] + \
                [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4),
-                ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4), ("rx", C.c_float * 4)]
+                ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4), ("rx", C.c_float * 4),
+                ("ic", C.c_int32 * 4)]
 
 
 class GParams(C.Structure):

@@ -65,6 +65,8 @@ struct Ctx {
     // magnetism (magnet.comp): per cell the susceptibility of its liquid (grid_prep) and the
     // magnetic energy density (the magnet pass, read by the next substep's grid_prep too)
     float2* magc;
+    // reactions: per cell, bit j where the tank's j-th row makes up more than REACT_MIN
+    uint32_t* rmask;
 };
 
 // A tank as a kernel argument: its arena (one allocation holding every buffer of the tank)
@@ -146,6 +148,7 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.famocc = (float*)take(sizeof(float) * N * SP_MAX_PRESENT);
     c.dsettle = (float2*)take(sizeof(float2) * N * SP_MAX_PRESENT);
     c.magc = (float2*)take(sizeof(float2) * N);
+    c.rmask = (uint32_t*)take(sizeof(uint32_t) * N);
     if (bytes) *bytes = o;
     return c;
 }

@@ -136,6 +136,8 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float4 nn = make_float4(0.0f, -1.0f, 1.0f, 0.0f);  // rh: the cell's non-Newtonian material
     float chi = 0.0f;
     for (int q = 0; q < nmix; q++) c.mixb[(size_t)i * nmix + q] = -1.0f;
+    const bool react = (P.caps & SP_CAP_REACT) != 0u;
+    if (cw <= 0 && react) c.rmask[i] = 0u;  // grid_prep.comp:71
     // each family's occupancy of the cell for settling (grid_prep.comp:69-70, 147)
     const bool settle = (P.caps & SP_CAP_SETTLE) != 0u;
     if (settle)
@@ -168,6 +170,13 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
                 const float em = m.th2[1];
                 emis += sh[j] * (m.diluent >= 0 ? mixf_(c.mats[m.diluent].th2[1], em, scn[j]) : em);
             }
+        }
+        // which rows the cell holds, for the reactions (grid_prep.comp:110-118)
+        if (react) {
+            uint32_t mask = 0u;
+            for (int j = 0; j < P.n_present; j++)
+                if (sh[j] * (c.mats[P.present[j]].diluent >= 0 ? scn[j] : 1.0f) > SP_REACT_MIN) mask |= 1u << (uint32_t)j;
+            c.rmask[i] = mask;
         }
         // the temperature of each phase group from its own sum; T leaves as the cell's mean
         // (group_temperatures, common.glsl:598-622)

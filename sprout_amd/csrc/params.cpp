@@ -61,6 +61,11 @@ static sp_material default_material() {
     m.saturation = 0.05f;
     m.volatility = 0.0f;  // material.rs:404, 412
     m.fume = 0.0f;
+    m.react_with = -1;  // material.rs:403-409
+    m.react_into = -1;
+    m.react_rate = 0.0f;
+    m.react_heat = 0.0f;
+    m.self_heat = 0.0f;
     return m;
 }
 
@@ -279,13 +284,17 @@ int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool th
     return 2;
 }
 
-// The rows a tank holding `present` needs: those and their diluents (SimParams::closure,
-// params.rs:927-950, without residues and reaction products, which this build does not carry).
+// The rows a tank holding `present` needs: those, their diluents, and the products of
+// reactions whose partner is present (SimParams::closure, params.rs:927-941, without
+// residues, which this build does not carry).
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present) {
     std::vector<int> rows = present;
     for (size_t k = 0; k < rows.size(); k++) {
-        int d = p.mat[rows[k]].diluent;
-        if (d >= 0 && std::find(rows.begin(), rows.end(), d) == rows.end()) rows.push_back(d);
+        const sp_material& m = p.mat[rows[k]];
+        const bool partner = m.react_with >= 0 && std::find(present.begin(), present.end(), m.react_with) != present.end();
+        for (int extra : {m.diluent, partner ? m.react_into : -1})
+            if (extra >= 0 && extra < SP_MAX_MATERIALS && std::find(rows.begin(), rows.end(), extra) == rows.end())
+                rows.push_back(extra);
     }
     std::sort(rows.begin(), rows.end());
     return rows;
@@ -377,8 +386,14 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
     bool magnetic = false;
     for (int r : t.rows) magnetic |= p.mat[r].susceptibility > 0.0f;
     if (n_magnets > 0 && magnetic) caps |= SP_CAP_MAGNET;
-    for (int r : t.rows)
-        if (p.mat[r].volatility > 0.0f) caps |= SP_CAP_VOLATILE;
+    for (int r : t.rows) {
+        const sp_material& m = p.mat[r];
+        if (m.volatility > 0.0f) caps |= SP_CAP_VOLATILE;
+        if (t.thermal && m.self_heat > 0.0f) caps |= SP_CAP_SELF_HEAT;
+        // two liquids that react with each other (mod.rs:2756-2758)
+        if (m.react_with >= 0 && m.react_rate > 0.0f && std::find(t.rows.begin(), t.rows.end(), m.react_with) != t.rows.end())
+            caps |= SP_CAP_REACT;
+    }
     return caps;
 }
 
@@ -459,6 +474,17 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.gr[2] = m.relax_time;
         o.gr[3] = m.susceptibility;  // mod.rs:2215 el.z
         o.rx[3] = m.volatility;      // mod.rs:2231
+        o.rx[0] = m.react_rate;
+        o.rx[1] = m.react_heat / m.heat_capacity;
+        o.rx[2] = m.self_heat / m.heat_capacity;
+        // the reaction while the partner is in the tank (mod.rs:2217-2230)
+        int at = -1;
+        for (size_t k = 0; k < t.rows.size(); k++)
+            if (m.react_with >= 0 && t.rows[k] == m.react_with) at = (int)k;
+        o.ic[0] = at >= 0 && m.react_rate > 0.0f ? m.react_with : -1;
+        o.ic[1] = m.react_into;
+        o.ic[2] = at >= 0 ? at : 0;
+        o.ic[3] = 0;
     }
     const uint32_t caps = tank_caps(p, t, magnets.size());
     // a liquid that evaporates as visible vapour leaves puffs (particles.comp:788-794), which

@@ -494,6 +494,45 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
             corrx = 0.0f;
             corry = 0.0f;
         }
+        // Heat made inside (particles.comp:733-739): the liquid's own K/s, counted in the ledger.
+        if (X && th_on && (P.caps & SP_CAP_SELF_HEAT) && c.mats[row].rx[2] > 0.0f) {
+            const float e = c.mats[row].rx[2] * c.mats[row].th[0] * (c.mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P.heat_rate;
+            p.t += e / thermal_of(c, row, p.s).w;
+            ledger_add(c, 7, e * P.inv_ppc);
+        }
+        // A reaction (particles.comp:740-770): where the particle's cell also holds its partner
+        // it turns into the product with probability rate x dt, its heat of reaction warming it;
+        // with no product it is used up (counted in the ledger).
+        if (X && (P.caps & SP_CAP_REACT) && c.mats[row].ic[0] >= 0) {
+            const int rx = clampi_((int)p.pos[0], 0, c.W - 1), ry = clampi_((int)p.pos[1], 0, c.H - 1);
+            if ((c.rmask[cidx(c, rx, ry)] & (1u << (uint32_t)c.mats[row].ic[2])) != 0u) {
+                const float own = c.mats[row].diluent >= 0 ? p.s : 1.0f;
+                const float u = (float)(hash_u(p.id * 0x9e3779b1u ^ hash_u(c.S->substeps + 0x51ed270bu)) >> 8) / 16777216.0f;
+                if (u < c.mats[row].rx[0] * dt && own > SP_REACT_MIN) {
+                    const float c0 = thermal_of(c, row, p.s).w;
+                    const float e = c.mats[row].rx[1] * c.mats[row].th[0] * own;
+                    if (th_on) {
+                        p.t += e / c0;
+                        ledger_add(c, 7, e * P.inv_ppc);
+                    }
+                    const int into = c.mats[row].ic[1];
+                    if (into < 0) {
+                        if (th_on) ledger_add(c, 6, fmaxf_(c0 * (p.t + 273.15f), 0.0f) * P.inv_ppc);
+                        atomicAdd(&c.ledger[16 + row], 1u);
+                        c.part[id].kind = SP_KIND_DEAD;
+                        return;
+                    }
+                    row = into;
+                    Mgroup = c.mats[row].group;
+                    Mcw = c.mats[row].chan_w;
+                    Mcs = c.mats[row].chan_s;
+                    fam = c.mats[row].fam;
+                    p.kind = (p.kind & ~255u) | (uint32_t)(row + 1);
+                    if (c.mats[row].diluent < 0) p.s = 1.0f;
+                    if (th_on) p.t = keep_heat(p.t, c0, thermal_of(c, row, p.s).w);
+                }
+            }
+        }
         // Evaporation below the boiling point (particles.comp:771-807): a particle of a volatile
         // liquid in a cell open to the air (not one that only touches a solid) leaves with
         // probability rate x dt, the rate doubling every 10 K and a concentrate's scaled by its

@@ -296,7 +296,9 @@ using PKernel = void (*)(TankBatch, int);
 const PKernel P_KERNELS[2][N_PSHAPES] = {SP_PK(false), SP_PK(true)};
 #undef SP_PK
 
-bool exotic_run(const Run& r) { return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE | SP_CAP_VOLATILE)) != 0u; }
+bool exotic_run(const Run& r) {
+    return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE | SP_CAP_VOLATILE | SP_CAP_SELF_HEAT | SP_CAP_REACT)) != 0u;
+}
 
 // What a shape is tuned for: the grid, the tanks per launch, the accumulator stride and
 // whether the tanks run the elastic or settling code.
@@ -412,7 +414,8 @@ uint32_t schedule_key(const sp_gparams& g) {
            (uint32_t)((g.caps & SP_CAP_THERMAL) && g.heat_rate > 0.0f) << 13 | (uint32_t)((g.caps & SP_CAP_RIGID) != 0u) << 14 |
            (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15 | (uint32_t)((g.caps & SP_CAP_ELASTIC) != 0u) << 16 |
            (uint32_t)((g.caps & SP_CAP_SETTLE) != 0u) << 17 | (uint32_t)((g.caps & SP_CAP_MAGNET) != 0u) << 18 |
-           (uint32_t)((g.caps & SP_CAP_VOLATILE) != 0u) << 19;
+           (uint32_t)((g.caps & SP_CAP_VOLATILE) != 0u) << 19 | (uint32_t)((g.caps & SP_CAP_SELF_HEAT) != 0u) << 20 |
+           (uint32_t)((g.caps & SP_CAP_REACT) != 0u) << 21;
 }
 
 }  // namespace

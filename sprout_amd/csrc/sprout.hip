@@ -1167,6 +1167,7 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_FAMOCC: *bytes = 4 * N * (size_t)std::max(s->gp.n_fam, 1); return c.famocc;
     case SP_FIELD_DSETTLE: *bytes = 8 * N * (size_t)std::max(s->gp.n_fam, 1); return c.dsettle;
     case SP_FIELD_MAGC: *bytes = 8 * N; return c.magc;
+    case SP_FIELD_RMASK: *bytes = 4 * N; return c.rmask;
     default: *bytes = 0; return nullptr;
     }
 }

@@ -332,3 +332,25 @@ def volatile_pool(grid, volatility=2.0, depth=0.3, cell_size=0.005, pressure_tol
 
 
 ROW_VOLATILE = 5  # a built-in row with the default material, made volatile by volatile_pool
+
+
+ROW_ACID, ROW_LYE, ROW_BRINE = 5, 6, 7  # built-in rows with the default material, made reactive by reacting_mix
+
+
+def reacting_mix(grid, rate=6.0, into=ROW_BRINE, partner_rate=0.0, block=(0.2, 0.8, 0.0, 0.4), seed=0, cell_size=0.005,
+                 pressure_tol=1e-6):
+    """Two liquids that react, mixed particle by particle in one block on the floor: each
+    particle of the block is acid or lye at random (seed). Acid turns into `into` (-1: it is
+    used up) at `rate` per second where lye is present; lye turns into brine at
+    `partner_rate`. Isothermal. (params, solid field, particles, rows)."""
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    a, b = params.mat[ROW_ACID], params.mat[ROW_LYE]
+    a.react_with, a.react_into, a.react_rate = ROW_LYE, into, rate
+    b.react_with, b.react_into, b.react_rate = ROW_ACID, ROW_BRINE, partner_rate
+    W, H = grid
+    scene = sc.Scene(grid)
+    parts = sc.fill_rect(scene, (block[0] * W, block[2] * H, block[1] * W, block[3] * H), ROW_ACID)
+    lye = np.random.default_rng(seed).random(len(parts)) < 0.5
+    parts["kind"][lye] = ROW_LYE + 1
+    rows = [ROW_ACID, ROW_LYE]
+    return params, scene.rasterize(), settle(parts, params), rows
