@@ -15,7 +15,7 @@
  *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp,
  *   settle.comp, magnet.comp, sort.comp, count.comp, stats.comp.
  * Parity unpinned (restated only, with no closed-form or golden check of their own): the
- * surface-tension jump, the explicit viscosity step, the drift-flux and
+ * surface-tension jump, the drift-flux and
  * family-exchange branches, collide and the crowd jitter, the heat of reaction and
  * self-heating, and the magnetic resting rule of grid_prep.
  *

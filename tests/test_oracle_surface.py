@@ -6,7 +6,9 @@ geometry:
     mean of curvature x distance to the centre is 1 (the curvature of a circle, 1 / r) within
     5 %, for two radii, positive for a convex blob;
   * a straight surface (a half-plane of liquid, at 0 and 30 degrees): no curvature in the band.
-A dropped half, a sign error or a stencil slip fails one of them.
+A dropped half, a sign error or a stencil slip fails one of them. And of the explicit
+viscosity step (visc_setup + visc_apply mode 2): a shear wave in the bulk decays by the
+discrete Laplacian's Fourier factor, on the u and on the v faces.
 """
 import numpy as np
 import pytest
@@ -60,3 +62,52 @@ def test_straight_surface_has_none(deg):
     inner = (np.abs(d) < 1.5) & (x > 8) & (x < W - 8) & (y > 8) & (y < H - 8)
     assert inner.sum() > 40
     assert np.abs(k[inner]).max() < 0.02
+
+
+def shear_wave_step(component, nu, rho, kw, dt=1.0 / 240.0):
+    """One explicit viscosity step (visc_setup + visc_apply mode 2) of a shear wave in a full
+    tank: u = sin(k y) on the u faces, or v = sin(k x) on the v faces. Returns (before,
+    after) of that component."""
+    W = H = 64
+    params, field, parts, rows = cases.dam_break((W, H), frac=(0.1, 0.1), viscosity=1e-6)
+    g, mats = N.derive_grid_params(params, W, H, rows)
+    assert g.visc_mode == 1  # explicit
+    o = OracleSim((W, H), W * H * 4)
+    o.set_gparams(g, mats)
+    o.set_solids(field)
+    o.reset(parts)
+    st = o.field("STEP", N.STEP_DTYPE, 1)
+    st["dt"] = dt
+    o.put_field("STEP", st)
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    cell = np.zeros((H, W, 4), np.float32)
+    cell[..., 0] = 1.0
+    cell[..., 1] = rho
+    cell[..., 2] = nu * rho  # dynamic viscosity over water's density
+    cell[..., 3] = 0.5
+    o.put_field("CELL", cell.reshape(-1, 4))
+    vel = np.zeros((H, W, 4), np.float32)
+    if component == 0:
+        vel[..., 0] = np.sin(kw * (y + 0.5))
+    else:
+        vel[..., 1] = np.sin(kw * (x + 0.5))
+    o.put_field("VEL", vel.reshape(-1, 4))
+    o.stage("COEF")
+    o.stage("VISC")
+    got = o.field("VEL", np.float32, (H, W, 4)).astype(np.float64)
+    return vel[..., component].astype(np.float64), got[..., component]
+
+
+@pytest.mark.parametrize("component,nu,rho,period", [(0, 0.5, 1.0, 16.0), (1, 0.3, 1.3, 10.0)])
+def test_explicit_viscosity_decays_a_shear_wave(component, nu, rho, period):
+    """Pin of the explicit viscosity step (visc_setup + visc_apply mode 2): in the bulk a
+    shear wave of wavenumber k loses dt nu (2 - 2 cos k) of itself per step, the decay of the
+    discrete Laplacian's Fourier mode (nu = mu / rho)."""
+    kw = 2.0 * np.pi / period
+    dt = 1.0 / 240.0
+    before, after = shear_wave_step(component, nu, rho, kw, dt)
+    inner = (slice(8, 56), slice(8, 56))
+    change = after[inner] - before[inner]
+    want = -dt * nu * (2.0 - 2.0 * np.cos(kw)) * before[inner]
+    assert np.abs(want).max() > 1e-4
+    assert np.allclose(change, want, rtol=0.0, atol=2e-3 * np.abs(want).max())
