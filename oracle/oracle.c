@@ -16,7 +16,7 @@
  *   settle.comp, magnet.comp, sort.comp, count.comp, stats.comp.
  * Parity unpinned (restated only, with no closed-form or golden check of their own): the
  * surface-tension jump, the drift-flux and
- * family-exchange branches, collide and the crowd jitter, the heat of reaction and
+ * family-exchange branches, the crowd jitter, the heat of reaction and
  * self-heating, and the magnetic resting rule of grid_prep.
  *
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so

@@ -111,3 +111,41 @@ def test_explicit_viscosity_decays_a_shear_wave(component, nu, rho, period):
     want = -dt * nu * (2.0 - 2.0 * np.cos(kw)) * before[inner]
     assert np.abs(want).max() > 1e-4
     assert np.allclose(change, want, rtol=0.0, atol=2e-3 * np.abs(want).max())
+
+
+def test_collide_keeps_particles_out_of_the_floor():
+    """Pin of `collide` (particles.comp:51-65 and the advect loop): particles driven into the
+    floor end at least the wall margin above its surface (the solid border is WALL cells
+    thick), with no velocity into it; those that never reach it move freely."""
+    W = H = 48
+    params, field, parts, rows = cases.dam_break((W, H), frac=(0.5, 0.5))
+    g, mats = N.derive_grid_params(params, W, H, rows)
+    o = OracleSim((W, H), W * H * 4)
+    o.set_gparams(g, mats)
+    o.set_solids(field)
+    o.reset(parts)
+    st = o.field("STEP", N.STEP_DTYPE, 1)
+    st["dt"] = 1.0 / 240.0
+    o.put_field("STEP", st)
+    # a grid of empty faces: the gather finds no live face and the particles keep their
+    # own velocity
+    vel = np.zeros((W * H, 4), np.float32)
+    vel[:, 2:] = 1e30
+    o.put_field("VEL", vel)
+    p = o.field("PARTICLES", N.PARTICLE_DTYPE, o.n_slots())
+    p["vel"][:, 0] = 0.0
+    p["vel"][:, 1] = -400.0  # 1.67 cells per substep downwards, below the speed cap
+    p["c"][:] = 0.0
+    o.put_field("PARTICLES", p)
+    o.stage("CLEAR_ACC")
+    o.stage("PARTICLES", 1)
+    q = o.field("PARTICLES", N.PARTICLE_DTYPE, o.n_slots())
+    floor = float(cases.sc.WALL)
+    step = 400.0 / 240.0
+    assert 400.0 < g.ext[0]
+    hit = p["pos"][:, 1] - step < floor + g.wall_margin
+    assert hit.sum() > 50 and (~hit).sum() > 50
+    assert np.all(q["pos"][:, 1] >= floor + g.wall_margin - 1e-4)
+    free = p["pos"][:, 1] - step > floor + 0.5  # ends well clear of the floor's margin
+    assert np.allclose(q["pos"][free, 1], p["pos"][free, 1] - step, atol=1e-4)
+    assert np.all(q["vel"][hit, 1] >= 0.0)  # no velocity into the floor
