@@ -14,10 +14,10 @@
  *   with the reduce_tail.glsl scalar steps, project.comp, rhs_visc.comp + the second
  *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp,
  *   settle.comp, magnet.comp, sort.comp, count.comp, stats.comp.
+ *
  * Parity unpinned (restated only, with no closed-form or golden check of their own): the
- * surface-tension jump, the drift-flux and
- * family-exchange branches, the crowd jitter, the heat of reaction and
- * self-heating, and the magnetic resting rule of grid_prep.
+ * surface-tension jump, the drift-flux and family-exchange branches, the crowd jitter, the
+ * heat of reaction and self-heating, and the magnetic resting rule of grid_prep.
  *
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
