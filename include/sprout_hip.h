@@ -31,7 +31,7 @@ extern "C" {
 #define SP_E_NOMEM (-3)        /* device or host allocation failed */
 #define SP_E_UNSUPPORTED (-4)  /* a feature of the reference this build does not carry */
 #define SP_E_CAPACITY (-5)     /* more particles / materials than the handle holds */
-#define SP_E_BUSY (-6)         /* the handle is driven by a server; or a request is not done yet */
+#define SP_E_BUSY (-6)         /* the handle is driven by a pool; or a request is not done yet */
 
 typedef struct sp_sim sp_sim;
 
@@ -128,6 +128,19 @@ int sp_set_magnets(sp_sim* sim, const sp_magnet* magnets, uint32_t n);
  * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
 int sp_sort(sp_sim* sim);
 
+/* The same counting sort (sort.comp:26-68, with the order defined as above) over caller
+ * device memory, for particles that no handle owns: particles[n] in, sorted[n] out (they
+ * must not overlap), by cell_of(pos) of a grid_w x grid_h grid (tiled = 1: by 8 x 8 tile,
+ * then by cell within it), dead slots last. perm[n] (device memory, may be NULL) receives
+ * the input slot of each output position, for moving per-particle data alongside. Runs on
+ * `stream` (a hipStream_t; NULL: the null stream) on the current device, with work space
+ * the library allocates for the call, and returns once the sort has finished. Errors:
+ * SP_E_INVALID (arguments), SP_E_NOMEM, SP_E_DEVICE; sp_error_text() tells which. */
+int sp_bin_particles(const sp_particle* particles, uint32_t n, int32_t grid_w, int32_t grid_h, int32_t tiled,
+                     sp_particle* sorted, uint32_t* perm, void* stream);
+/* Text of the last error of a handle-free entry point on this thread ("" if none). */
+const char* sp_error_text(void);
+
 /* ---- render-facing outputs ------------------------------------------------------------- */
 
 /* The buffers a renderer reads (FluidSim::particle_buffer, sdf_buffer, cell_buffer,
@@ -213,33 +226,17 @@ int sp_is_resident(const sp_sim* sim, int32_t* out);
  * sp_evict) and `fixed` (held for the handle's life). No device needed. */
 int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* evictable, uint64_t* fixed);
 
-/* ---- the request server ------------------------------------------------------------------ */
+/* ---- request records of the pool ---------------------------------------------------- */
 
-/* No counterpart in the reference: serving many resident scenes. A server owns a worker
- * thread that takes step requests for the scenes (tanks) added to it, as they arrive, and
- * advances them with continuous batching: every pending request whose scene is idle is
- * launched at once, requests of same-size, same-schedule scenes for the same substep count
- * sharing launches (up to 32 tanks each, sp_batch's grouping), each launch on one of
- * `streams` streams so launches overlap on the device; a request that arrives while others
- * run goes out as soon as a stream and its scene are free. Scenes stay on the device while
- * their evictable bytes fit `hbm_budget` (sp_tank_bytes); over it, the least recently used
- * idle scenes are evicted to host memory and brought back when a request needs them. Each
- * scene's results are those it gets stepped alone with the same requests (sp_record +
- * sp_wait per request). Requests of one scene run in arrival order.
- * While added to a server, a handle must not be used by the caller (its calls return
- * SP_E_BUSY); after sp_server_destroy it is the caller's again.
- * In this build the server is switched off: sp_server_create returns SP_E_UNSUPPORTED.
- * sp_pool_* below serves the same requests, driven by the caller's serving loop. */
-typedef struct sp_server sp_server;
 typedef struct sp_request_info {
     uint64_t ticket;
     uint32_t scene, substeps;
     int32_t status;          /* SP_OK, or the error that failed it */
     uint32_t tanks;          /* tanks in the launch that ran it */
-    double submit_ms, launch_ms, done_ms;   /* server clock */
+    double submit_ms, launch_ms, done_ms;   /* pool clock */
     double latency_ms;       /* done - submit */
 } sp_request_info;
-typedef struct sp_server_stats {
+typedef struct sp_pool_stats {
     uint64_t submitted, completed, failed;
     uint64_t particle_steps;     /* particle slots x substeps of the completed requests */
     uint64_t launches;           /* group launches */
@@ -249,25 +246,8 @@ typedef struct sp_server_stats {
     double p50_ms, p99_ms, mean_ms, max_ms;   /* request latency, completed requests */
     double span_s;               /* first submit to last completion */
     double tanks_per_launch;     /* mean */
-    uint64_t rounds;             /* sp_pool rounds (0 for the server) */
-} sp_server_stats;
-int sp_server_create(int32_t device, uint64_t hbm_budget, uint32_t streams, sp_server** out);
-/* Runs every request already submitted, stops the worker, and gives the handles back. */
-void sp_server_destroy(sp_server* server);
-const char* sp_server_last_error(sp_server* server);
-/* Adds a tank (set up: parameters, solids, particles) as scene *scene (0, 1, ...). */
-int sp_server_add(sp_server* server, sp_sim* sim, uint32_t* scene);
-/* Queues a request of `substeps` substeps covering `target_s` seconds (as sp_record);
- * never blocks. Thread-safe. */
-int sp_server_submit(sp_server* server, uint32_t scene, uint32_t substeps, float target_s, uint64_t* ticket);
-/* Blocks until the request is done. sp_server_poll: SP_E_BUSY while it is not. */
-int sp_server_wait(sp_server* server, uint64_t ticket, sp_request_info* out);
-int sp_server_poll(sp_server* server, uint64_t ticket, sp_request_info* out);
-/* Blocks until every submitted request is done. */
-int sp_server_drain(sp_server* server);
-/* Counters and latency percentiles since the last sp_server_reset_stats. */
-int sp_server_stats_get(sp_server* server, sp_server_stats* out);
-int sp_server_reset_stats(sp_server* server);
+    uint64_t rounds;             /* rounds run */
+} sp_pool_stats;
 
 /* ---- the continuous-batching pool ------------------------------------------------------ */
 
@@ -309,7 +289,7 @@ int sp_pool_wait(sp_pool* pool, uint64_t ticket, sp_request_info* out);
 int sp_pool_drain(sp_pool* pool);
 /* Counters and latency percentiles since the last sp_pool_reset_stats (launch_ms in
  * sp_request_info: the request's first round). */
-int sp_pool_stats_get(sp_pool* pool, sp_server_stats* out);
+int sp_pool_stats_get(sp_pool* pool, sp_pool_stats* out);
 int sp_pool_reset_stats(sp_pool* pool);
 
 /* ---- launch-shape autotuner ----------------------------------------------------------- */

@@ -144,6 +144,9 @@ typedef struct sp_sim_params {
     uint32_t viscosity_precond;    /* 1: multigrid V-cycle, 0: Jacobi */
     uint32_t viscosity_mg_levels;  /* levels below the grid in the viscosity V-cycle */
     uint32_t viscosity_mg_sweeps;  /* Gauss-Seidel sweeps per visit of a level (1 or 2) */
+    /* Pressure solves record this share more iterations than the last batch needed
+     * (params.rs:284, range [0, 2], default 0.25; mod.rs:3098-3114). */
+    float iters_margin;
     /* Storage of the pressure multigrid (no counterpart in the reference, which keeps FP32):
      * 0 FP32; 1 mixed: the V-cycle's iterates and coarse right-hand sides in BF16, the
      * coarse levels' matrices in FP16, smoothing arithmetic in FP32; the level-0 matrix

@@ -50,6 +50,7 @@ def lib():
         L.or_n_slots.argtypes = [P]
         L.or_n_slots.restype = C.c_uint32
         L.or_sort.argtypes = [P]
+        L.or_spawn.argtypes = [P, P, C.c_uint32]
         L.or_cell_counts.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P]
         L.or_stats.argtypes = [P, P, P, P]
         L.or_last_iters.argtypes = [P]
@@ -115,6 +116,11 @@ class OracleSim:
 
     def sort(self):
         self._lib.or_sort(self._h)
+
+    def spawn(self, particles):
+        """The record's spawn step: fresh slots first, then dead slots in slot order."""
+        p = np.ascontiguousarray(particles)
+        self._lib.or_spawn(self._h, p.ctypes.data if len(p) else None, len(p))
 
     def cell_counts(self, x0, y0, w, h):
         out = np.zeros(w * h, np.uint32)

@@ -3737,6 +3737,22 @@ void or_sort(or_sim* s) {
     free(count);
 }
 
+/* The spawn step of record (mod.rs:2485-2503) and claim.comp with a defined order: pending
+ * particles fill fresh slots first; the rest go to dead slots, the k-th dead slot in slot
+ * order taking the k-th remaining particle, while there are any (claim.comp:10-17; the
+ * reference's order is that of its atomics). Particles beyond the dead slots are dropped. */
+void or_spawn(or_sim* s, const sp_particle* spawn, uint32_t n) {
+    uint32_t fresh = s->cap - s->n_slots;
+    if (fresh > n) fresh = n;
+    for (uint32_t k = 0; k < fresh; k++) s->part[s->n_slots + k] = spawn[k];
+    uint32_t old = s->n_slots;
+    s->n_slots += fresh;
+    s->P.n_slots = s->n_slots;
+    uint32_t k = fresh;
+    for (uint32_t id = 0; id < old && k < n; id++)
+        if (s->part[id].kind == SP_KIND_DEAD) s->part[id] = spawn[k++];
+}
+
 void or_cell_counts(or_sim* s, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out) {
     memset(out, 0, sizeof(uint32_t) * (size_t)(w > 0 && h > 0 ? w * h : 0));
     for (uint32_t id = 0; id < s->n_slots; id++) {

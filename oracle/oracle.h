@@ -36,6 +36,8 @@ long or_put(or_sim* s, int field, const void* in, size_t bytes);
 uint32_t or_n_slots(or_sim* s);
 /* Canonical counting sort by cell, stable by slot (sort.comp with a defined order). */
 void or_sort(or_sim* s);
+/* The spawn step of record: fresh slots first, then dead slots in slot order (claim.comp). */
+void or_spawn(or_sim* s, const sp_particle* spawn, uint32_t n);
 /* Live particles per cell in a window (count.comp). */
 void or_cell_counts(or_sim* s, int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t* out);
 /* Live count and per-row counts and concentration sums (stats.comp). */

@@ -91,7 +91,8 @@ class SimParams(C.Structure):
                 ("min_substep_share", C.c_float), ("viscous_projection", C.c_uint32),
                 ("viscous_warm", C.c_uint32), ("viscosity_warm", C.c_uint32),
                 ("viscosity_precond", C.c_uint32), ("viscosity_mg_levels", C.c_uint32),
-                ("viscosity_mg_sweeps", C.c_uint32), ("mg_precision", C.c_uint32),
+                ("viscosity_mg_sweeps", C.c_uint32), ("iters_margin", C.c_float),
+                ("mg_precision", C.c_uint32),
                 ("heat_rate", C.c_float), ("air_cooling", C.c_float), ("contact_heat", C.c_float),
                 ("eddy_heat", C.c_float), ("evaporation", C.c_uint32),
                 ("rigid_contact", C.c_uint32), ("rigid_friction", C.c_float),
@@ -133,7 +134,7 @@ class RequestInfo(C.Structure):
                 ("launch_ms", C.c_double), ("done_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
-class ServerStats(C.Structure):
+class PoolStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("particle_steps", C.c_uint64), ("launches", C.c_uint64), ("evictions", C.c_uint64),
                 ("restores", C.c_uint64), ("resident_bytes", C.c_uint64), ("scenes", C.c_uint32),
@@ -246,6 +247,8 @@ SIGNATURES = {
     "sp_cell_counts": ([_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P], C.c_int),
     "sp_stats": ([_P, C.POINTER(StatsOut)], C.c_int),
     "sp_sort": ([_P], C.c_int),
+    "sp_bin_particles": ([_P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P], C.c_int),
+    "sp_error_text": ([], C.c_char_p),
     "sp_render_view": ([_P, C.c_int32, _P], C.c_int),
     "sp_render_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
     "sp_stream": ([_P, C.POINTER(C.c_void_p)], C.c_int),
@@ -274,16 +277,6 @@ SIGNATURES = {
     "sp_evict": ([_P], C.c_int),
     "sp_is_resident": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "sp_tank_bytes": ([C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
-    "sp_server_create": ([C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)], C.c_int),
-    "sp_server_destroy": ([_P], None),
-    "sp_server_last_error": ([_P], C.c_char_p),
-    "sp_server_add": ([_P, _P, C.POINTER(C.c_uint32)], C.c_int),
-    "sp_server_submit": ([_P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(C.c_uint64)], C.c_int),
-    "sp_server_wait": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
-    "sp_server_poll": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
-    "sp_server_drain": ([_P], C.c_int),
-    "sp_server_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
-    "sp_server_reset_stats": ([_P], C.c_int),
     "sp_autotune": ([C.c_int32, C.c_uint32], C.c_int),
     "sp_autotune_shapes": ([_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_autotune_report": ([C.POINTER(TuneEntry), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
@@ -296,7 +289,7 @@ SIGNATURES = {
     "sp_pool_poll": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
     "sp_pool_wait": ([_P, C.c_uint64, C.POINTER(RequestInfo)], C.c_int),
     "sp_pool_drain": ([_P], C.c_int),
-    "sp_pool_stats_get": ([_P, C.POINTER(ServerStats)], C.c_int),
+    "sp_pool_stats_get": ([_P, C.POINTER(PoolStats)], C.c_int),
     "sp_pool_reset_stats": ([_P], C.c_int),
 }
 
@@ -317,6 +310,81 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime libsprout_hip.so links (the same loaded library, by its soname), for
+    device buffers handed to the handle-free entry points. Another copy of the runtime in
+    the process (a framework's bundled one) would see no devices once this one holds them."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        H.hipDeviceSynchronize.argtypes = []
+        for f in (H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipMemset, H.hipDeviceSynchronize):
+            f.restype = C.c_int
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """`nbytes` of device memory on the current device (hipMalloc), with host copies in and
+    out; freed on close()."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        rc = hip().hipMalloc(C.byref(p), max(self.nbytes, 1))
+        if rc != 0:
+            raise SproutError(E_NOMEM, f"hipMalloc({self.nbytes}) failed with HIP error {rc}")
+        self.ptr = p.value
+        hip().hipMemset(self.ptr, 0, max(self.nbytes, 1))
+
+    @classmethod
+    def of(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if arr.nbytes and hip().hipMemcpy(self.ptr, arr.ctypes.data, arr.nbytes, 1) != 0:
+            raise SproutError(E_DEVICE, "hipMemcpy to the device failed")
+
+    def download(self, dtype, count):
+        out = np.zeros(count, dtype)
+        assert out.nbytes <= self.nbytes
+        if out.nbytes and hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:
+            raise SproutError(E_DEVICE, "hipMemcpy from the device failed")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bin_particles(dev_in, n, W, H, tiled, dev_out, dev_perm=None, stream=None):
+    """sp_bin_particles over device buffers: the stable counting sort of sort.comp."""
+    L = lib()
+    rc = L.sp_bin_particles(dev_in, n, W, H, tiled, dev_out, dev_perm, stream)
+    if rc != OK:
+        raise SproutError(rc, f"bin_particles: {L.sp_error_text().decode()}")
 
 
 def tank_bytes(W, H, capacity):

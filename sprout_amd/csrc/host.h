@@ -26,8 +26,10 @@ struct Tank {
 void default_params(sp_sim_params* p);
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present);
 bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal = false);
+// ext_speed: the fastest speed a pour imposes this record, cells / s (the queued spawns'
+// fastest |vel|; the tool's is 0 for the idle tool this build carries).
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
-           std::string* why, const std::vector<sp_magnet>& magnets = {});
+           std::string* why, const std::vector<sp_magnet>& magnets = {}, float ext_speed = 0.0f);
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);

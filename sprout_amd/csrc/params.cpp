@@ -110,6 +110,7 @@ void default_params(sp_sim_params* p) {
     p->viscosity_precond = 1;
     p->viscosity_mg_levels = 1;
     p->viscosity_mg_sweeps = 2;
+    p->iters_margin = 0.25f;
     p->mg_precision = 0;
     p->heat_rate = 1.0f;  // params.rs:395-399
     p->air_cooling = 10.0f;
@@ -398,7 +399,7 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
 }
 
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why,
-           const std::vector<sp_magnet>& magnets) {
+           const std::vector<sp_magnet>& magnets, float ext_speed) {
     memset(&g, 0, sizeof(g));
     memset(mats, 0, sizeof(sp_gmat) * SP_MAX_MATERIALS);
     const float dx = p.cell_size;
@@ -536,7 +537,8 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.density_k = p.correction_max;
     g.dt_max = dt_limit(p, t.rows, t.thermal);
     g.crowd_jitter = p.crowd_jitter;
-    g.ext_speed = 0.0f;
+    // the tool's speed or the fastest queued spawn's, whichever is larger (mod.rs:2300)
+    g.ext_speed = std::max(ext_speed, 0.0f);
     g.wall_margin = 0.1f;
     g.tol[0] = p.pressure_tol;
     g.tol[1] = p.correction_tol;

@@ -1,5 +1,5 @@
 // particles.hip — the per-particle pass of a substep on gfx950, and the particle
-// bookkeeping kernels (sort, count, stats, claim).
+// bookkeeping kernels (count, stats); the sort and the claim are in bin.hip.
 //
 // One work-item per particle slot, 256 per work-group (four wave64s). Each work-item runs
 // the gather (grid -> particle, APIC), the advection with collision against the solid SDF,
@@ -808,44 +808,6 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
     }
 }
 
-// ---- counting sort by cell (sort.comp), stable by slot ---------------------------------
-//
-// The key of a slot is its cell (cell_of, sort.comp:28-30), or N for a dead slot, so dead
-// slots collect at the end. The (key, slot) pairs go through a stable LSD radix sort, and
-// the particles are gathered into the new order: within a cell the earlier slot stays
-// first, which makes the permutation a defined one (the reference's is atomic-ordered,
-// sort.comp:64).
-
-// tiled = 0: key = cell (the reference's order); tiled = 1: key = 8 x 8 tile, then cell within
-// the tile. The substep loop keeps the particles in tiled order (neighbours in memory are
-// neighbours in both x and y, so a work-group's scatter box stays near 11 x 11 cells);
-// sp_sort gives the reference's order. Either order leaves every result unchanged.
-__global__ __launch_bounds__(256) void k_sort_keys(const TankBatch tb, uint32_t* keys, uint32_t* vals, int tiled, uint32_t dead) {
-    const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= c.P->n_slots) return;
-    sp_particle p = c.part[id];
-    uint32_t key = dead;
-    if (p.kind != SP_KIND_DEAD) {
-        int x = clampi_((int)p.pos[0], 0, c.W - 1), y = clampi_((int)p.pos[1], 0, c.H - 1);
-        if (tiled) key = (uint32_t)((((y >> 3) * ((c.W + 7) >> 3) + (x >> 3)) << 6) | ((y & 7) << 3) | (x & 7));
-        else key = (uint32_t)(y * c.W + x);
-    }
-    keys[id] = key;
-    vals[id] = id;
-}
-
-// The particles gathered into their new order, and with `sout` their stress (sort.comp:66).
-__global__ __launch_bounds__(256) void k_sort_gather(const TankBatch tb, const uint32_t* vals, sp_particle* out, float4* sout,
-                                                     uint32_t n) {
-    const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t from = vals[j];
-    out[j] = c.part[from];
-    if (sout) sout[j] = c.pstress[from];
-}
-
 // Live particles per cell in a window (count.comp:17-23).
 __global__ __launch_bounds__(256) void k_count(const TankBatch tb, int x0, int y0, int w, int h, uint32_t* out) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
@@ -889,15 +851,3 @@ __global__ __launch_bounds__(64) void k_stats(const TankBatch tb, uint32_t* coun
     }
     if (live > 0u) atomicAdd(&counter[4], live);
 }
-
-// Dead slots claim pending spawns (claim.comp:10-17).
-__global__ __launch_bounds__(256) void k_claim(const TankBatch tb, const sp_particle* spawn, uint32_t spawn_count, uint32_t* counter) {
-    const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    uint32_t id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= c.P->n_slots) return;
-    if (c.part[id].kind != SP_KIND_DEAD) return;
-    if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= spawn_count) return;
-    uint32_t k = atomicAdd(counter, 1u);
-    if (k < spawn_count) c.part[id] = spawn[k];
-}
-

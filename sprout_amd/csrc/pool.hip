@@ -271,7 +271,7 @@ void sp_pool_destroy(sp_pool* v) {
     if (!v) return;
     (void)v->core->drain();
     const uint32_t n = v->core->scenes_added();
-    for (uint32_t k = 0; k < n; k++) v->be.sims[k]->served = false;
+    for (uint32_t k = 0; k < n; k++) v->be.sims[k]->pooled = false;
     (void)hipSetDevice(v->be.device);
     free_pool_streams(v->be);
     delete v;
@@ -285,7 +285,7 @@ const char* sp_pool_last_error(sp_pool* v) {
 
 int sp_pool_add(sp_pool* v, sp_sim* s, uint32_t* scene) {
     if (!v || !s || !scene) return SP_E_INVALID;
-    if (s->device != v->be.device || s->served || s->in_batch || !s->params_set) return SP_E_INVALID;
+    if (s->device != v->be.device || s->pooled || s->in_batch || !s->params_set) return SP_E_INVALID;
     // the tables the launch groups are keyed on, while the handle is still the caller's
     int rc = guarded(s, [&]() -> int { return upload_tables(s); });
     if (rc != SP_OK) return rc;
@@ -295,7 +295,7 @@ int sp_pool_add(sp_pool* v, sp_sim* s, uint32_t* scene) {
         v->be.starts.emplace_back();
         v->be.where.emplace_back(0u, 0u);
         (void)id;
-        s->served = true;
+        s->pooled = true;
         return true;
     });
     return ok ? SP_OK : SP_E_DEVICE;
@@ -335,10 +335,10 @@ int sp_pool_drain(sp_pool* v) {
     return v->core->drain() ? SP_OK : SP_E_DEVICE;
 }
 
-int sp_pool_stats_get(sp_pool* v, sp_server_stats* out) {
+int sp_pool_stats_get(sp_pool* v, sp_pool_stats* out) {
     if (!v || !out) return SP_E_INVALID;
     const pool::Stats st = v->core->stats();
-    *out = sp_server_stats{};
+    *out = sp_pool_stats{};
     out->submitted = st.submitted;
     out->completed = st.completed;
     out->failed = st.failed;

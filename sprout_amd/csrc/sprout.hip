@@ -5,7 +5,6 @@
 //
 // One translation unit: the kernel files are included here so that the kernels and the
 // launch code see each other's inline helpers.
-#include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -17,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "bin.hip"
 #include "grid.hip"
 #include "host.h"
 #include "particles.hip"
@@ -32,6 +32,7 @@
 namespace {
 
 thread_local std::string g_create_error;
+thread_local std::string g_call_error;  // the last error of a handle-free entry point
 
 template <class T>
 T* dalloc(size_t n, std::vector<void*>& owned) {
@@ -81,12 +82,7 @@ struct sp_sim {
     sp_particle* d_spawn = nullptr;
     uint32_t* d_counter = nullptr;  // stats / claim counters
     uint32_t* d_count_out = nullptr;
-    uint32_t* d_keys = nullptr;
-    uint32_t* d_vals = nullptr;
-    uint32_t* d_keys2 = nullptr;
-    uint32_t* d_vals2 = nullptr;
-    void* d_sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    void* d_bin = nullptr;  // the sort's and the claim's work space (bin_space_bytes(cap))
 
     // host state mirroring FluidSim
     uint32_t n_slots = 0;
@@ -95,6 +91,7 @@ struct sp_sim {
     uint32_t since_sort = 0;
     uint32_t sort_interval = 8;
     std::vector<sp_particle> pending_spawn;
+    float spawn_speed = 0.0f;  // fastest |vel| of the spawns queued for the next record (cells / s)
     double time_base = 0.0;
     double sim_time = 0.0;
     float dt_estimate = 1.0f / 240.0f;
@@ -107,7 +104,7 @@ struct sp_sim {
     GraphSet gs;
     Timing tm;
     bool in_batch = false;  // a batch drives the tank (sp_batch_*)
-    bool served = false;    // a server drives the tank (sp_server_*)
+    bool pooled = false;    // a pool drives the tank (sp_pool_*)
 
     // heat (FluidSim::thermal, solids_heated, mod.rs:959-962)
     bool thermal = false;        // temperatures carried and conducted since the last reset
@@ -129,26 +126,35 @@ void set_err(sp_sim* s, const char* fmt, ...) {
 
 // The counting sort of the particles by cell (record_sort, mod.rs:3223-3242), on `stream`:
 // keys are the cell (or the 8 x 8 tile, then the cell, with `tiled`), dead slots last;
-// stable by the previous slot.
+// stable by the previous slot (bin.hip).
 void launch_sort(sp_sim* s, int tiled, hipStream_t stream) {
     if (s->n_slots == 0) return;
-    uint32_t blocks = (s->n_slots + 255) / 256;
-    // keys below 2^end_bit; dead slots take the largest, so they collect at the end
-    uint32_t live_max = tiled ? (uint32_t)(((s->W + 7) / 8) * ((s->H + 7) / 8) * 64) : (uint32_t)s->N;
-    int end_bit = 1;
-    while (end_bit < 32 && (1u << end_bit) <= live_max) end_bit++;
-    uint32_t dead = end_bit >= 32 ? 0xffffffffu : (1u << end_bit) - 1u;
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, stream, s->run.tb, s->d_keys, s->d_vals, tiled, dead);
-    size_t bytes = s->sort_tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(s->d_sort_tmp, bytes, s->d_keys, s->d_keys2, s->d_vals, s->d_vals2,
-                                              (int)s->n_slots, 0, end_bit, stream));
+    const uint32_t blocks = (s->n_slots + 255) / 256;
+    const int bits = bin_bits(s->W, s->H, tiled);
+    BinSpace sp = bin_space_at(s->d_bin, s->cap);
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, stream, s->run.tb, sp.keys, sp.vals, s->n_slots, tiled,
+                       bin_dead_key(bits));
+    const uint32_t* order = launch_bin_passes(sp, s->n_slots, bits, stream);
     // the stress moves with its particle while the tank is elastic (record_sort, mod.rs:3238-3240)
     const bool elastic = (s->gp.caps & SP_CAP_ELASTIC) != 0u;
-    hipLaunchKernelGGL(k_sort_gather, dim3(blocks), dim3(256), 0, stream, s->run.tb, (const uint32_t*)s->d_vals2,
-                       s->d_part_tmp, elastic ? s->d_stress_tmp : nullptr, s->n_slots);
+    hipLaunchKernelGGL(k_sort_gather, dim3(blocks), dim3(256), 0, stream, s->run.tb, order, s->d_part_tmp,
+                       elastic ? s->d_stress_tmp : nullptr, s->n_slots);
     HIPCHK(hipMemcpyAsync(s->c.part, s->d_part_tmp, sizeof(sp_particle) * s->n_slots, hipMemcpyDeviceToDevice, stream));
     if (elastic)
         HIPCHK(hipMemcpyAsync(s->c.pstress, s->d_stress_tmp, sizeof(float4) * s->n_slots, hipMemcpyDeviceToDevice, stream));
+}
+
+// Dead slots claim the `claim` pending spawns in s->d_spawn, in slot order (claim.comp:10-17).
+void launch_claim(sp_sim* s, uint32_t claim, hipStream_t stream) {
+    if (claim == 0 || s->n_slots == 0) return;
+    BinSpace sp = bin_space_at(s->d_bin, s->cap);
+    const uint32_t groups = bin_groups(s->n_slots);
+    const uint32_t nch = (groups + bin::SCAN_CHUNK - 1) / bin::SCAN_CHUNK;
+    hipLaunchKernelGGL(k_claim_count, dim3(groups), dim3(bin::NT), 0, stream, s->run.tb, s->n_slots, sp.hist);
+    hipLaunchKernelGGL(k_bin_scan_chunks, dim3(nch), dim3(bin::NT), 0, stream, sp.hist, groups, sp.chunk);
+    hipLaunchKernelGGL(k_bin_scan_top, dim3(1), dim3(bin::TOP_NT), 0, stream, sp.chunk, nch);
+    hipLaunchKernelGGL(k_claim, dim3(groups), dim3(bin::NT), 0, stream, s->run.tb, s->n_slots, (const sp_particle*)s->d_spawn, claim,
+                       (const uint32_t*)sp.hist, (const uint32_t*)sp.chunk);
 }
 
 // Uploads the grid-unit tables if the parameters or the tank changed.
@@ -169,7 +175,7 @@ int upload_tables(sp_sim* s) {
         }
     }
     std::string why;
-    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets);
+    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets, s->spawn_speed);
     if (rc != SP_OK) {
         set_err(s, "%s", why.c_str());
         return rc;
@@ -192,11 +198,7 @@ void alloc_big(sp_sim* s) {
         ref.base = dalloc<char>(s->arena_bytes, got);
         s->d_part_tmp = dalloc<sp_particle>(s->cap, got);
         s->d_stress_tmp = dalloc<float4>(s->cap, got);
-        s->d_keys = dalloc<uint32_t>(s->cap, got);
-        s->d_vals = dalloc<uint32_t>(s->cap, got);
-        s->d_keys2 = dalloc<uint32_t>(s->cap, got);
-        s->d_vals2 = dalloc<uint32_t>(s->cap, got);
-        s->d_sort_tmp = dalloc<uint8_t>(s->sort_tmp_bytes, got);
+        s->d_bin = dalloc<uint8_t>(bin_space_bytes(s->cap), got);
         s->ref = ref;
     } catch (...) {
         for (void* p : got) (void)hipFree(p);
@@ -214,11 +216,11 @@ void free_big(sp_sim* s) {
     s->ref.base = nullptr;
 }
 
-// Bytes of device memory the tank gives back when evicted.
-size_t big_bytes(const sp_sim* s) {
-    return s->arena_bytes + (sizeof(sp_particle) + sizeof(float4) + 16) * (size_t)std::max<uint32_t>(s->cap, 1) +
-           std::max<size_t>(s->sort_tmp_bytes, 1);
+// Bytes of device memory a tank of this arena and capacity gives back when evicted.
+size_t big_bytes(size_t arena_bytes, uint32_t cap) {
+    return arena_bytes + (sizeof(sp_particle) + sizeof(float4)) * (size_t)std::max<uint32_t>(cap, 1) + bin_space_bytes(cap);
 }
+size_t big_bytes(const sp_sim* s) { return big_bytes(s->arena_bytes, s->cap); }
 
 // Evicts the tank: its arena goes to pinned host memory and its evictable device memory is
 // freed. Nothing else of the tank changes, so a restored tank continues bit for bit.
@@ -281,10 +283,11 @@ void readback_apply(sp_sim* s, const sp_stepbuf& S, const sp_solve& K) {
         if (unconverged > 0) rec = std::min(rec + rec / 2 + 2, maxi);
         else rec = std::min(std::max(used + (uint32_t)((float)used * margin) + 2, 2u), maxi);
     };
-    adapt(s->rec.p, K.it_max, p_un, s->params.pressure_max_iters, 0.25f);
-    if (s->gp.visc_mode == 2 && s->gp.v_twice) adapt(s->rec.p2, K.p2_it_max, p2_un, s->params.pressure_max_iters, 0.25f);
-    else s->rec.p2 = s->rec.p;
-    if (s->gp.visc_mode == 2) adapt(s->rec.v, K.v_it_max, v_un, s->params.viscosity_max_iters, 0.5f);
+    const float margin = std::max(s->params.iters_margin, 0.0f);
+    adapt(s->rec.p, K.it_max, p_un, s->params.pressure_max_iters, margin);
+    if (s->gp.visc_mode == 2 && s->gp.v_twice) adapt(s->rec.p2, K.p2_it_max, p2_un, s->params.pressure_max_iters, margin);
+    else s->rec.p2 = s->rec.p;  // not running: ready for a viscous liquid with what the first solve needs
+    adapt(s->rec.v, K.v_it_max, v_un, std::max(s->params.viscosity_max_iters, 1u), 0.5f);
 }
 
 // The per-batch iteration maxima start again (after they were read), on `stream`;
@@ -343,11 +346,12 @@ int record_prepare(sp_sim* s, uint32_t substeps, float target) {
     }
     int rc = upload_tables(s);
     if (rc != SP_OK) return rc;
-    if (claim > 0) {
-        HIPCHK(hipMemsetAsync(s->d_counter, 0, sizeof(uint32_t) * 4, s->stream));
-        hipLaunchKernelGGL(k_claim, dim3((s->n_slots + 255) / 256), dim3(256), 0, s->stream, s->run.tb,
-                           (const sp_particle*)s->d_spawn, claim, s->d_counter);
+    // the spawns' speed bounds this record's steps only (record, mod.rs:2520)
+    if (s->spawn_speed != 0.0f) {
+        s->spawn_speed = 0.0f;
+        s->tables_dirty = true;
     }
+    launch_claim(s, claim, s->stream);
     if (substeps > 0) {
         // StepBuf words 0, 2, 3: target time, recorded substeps, done (record, mod.rs:2521-2526)
         uint32_t words[4];
@@ -394,8 +398,8 @@ int wait(sp_sim* s) {
 template <class F>
 int guarded(sp_sim* s, F&& f, bool device = true) {
     if (!s) return SP_E_INVALID;
-    if (s->served) {
-        set_err(s, "the tank is driven by a server");
+    if (s->pooled) {
+        set_err(s, "the tank is driven by a pool");
         return SP_E_BUSY;
     }
     try {
@@ -499,6 +503,10 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         g_create_error = "grid must be between 8 x 8 and 8192 x 8192 cells";
         return SP_E_INVALID;
     }
+    if (bin_chunks(cfg->capacity) > (uint32_t)bin::TOP_MAX) {
+        g_create_error = "capacity beyond what one sort covers";
+        return SP_E_INVALID;
+    }
     sp_sim* s = new (std::nothrow) sp_sim();
     if (!s) {
         g_create_error = "host allocation failed";
@@ -532,12 +540,6 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         TankRef& ref = s->ref;
         ref = TankRef{nullptr, s->W, s->H, s->cap, s->lv.total, s->lv.tiles, s->lv.n, s->lv.tail};
         (void)ctx_of(ref, &s->arena_bytes);
-        {
-            size_t bytes = 0;
-            HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                      (uint32_t*)nullptr, (int)std::max<uint32_t>(s->cap, 1), 0, 32, s->stream));
-            s->sort_tmp_bytes = bytes;
-        }
         alloc_big(s);
         Ctx& c = s->c;
         s->d_spawn = dalloc<sp_particle>(SP_SPAWN_MAX, o);
@@ -688,6 +690,7 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         HIPCHK(hipMemsetAsync(s->c.pstress, 0, sizeof(float4) * (size_t)std::max<uint32_t>(s->cap, 1), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->pending_spawn.clear();
+        s->spawn_speed = 0.0f;
         s->resets++;
         s->time_base = s->sim_time;
         s->substeps_base = s->stats.substeps;
@@ -722,6 +725,14 @@ int sp_spawn(sp_sim* s, const sp_particle* particles, uint32_t n, uint32_t* acce
         }
         if (!note_materials(s, ps.data(), n, false)) return SP_OK;
         note_temps(s, ps.data(), n);
+        // the pour's speed bounds the next record's time step (spawn, mod.rs:2039-2041)
+        for (const auto& q : ps) {
+            const float v = hypotf(q.vel[0], q.vel[1]);
+            if (v > s->spawn_speed) {
+                s->spawn_speed = v;
+                s->tables_dirty = true;
+            }
+        }
         for (auto& q : ps) {
             q.id = s->next_id++;
             q.kind |= SP_KIND_FRESH;
@@ -931,6 +942,71 @@ int sp_sort(sp_sim* s) {
     });
 }
 
+const char* sp_error_text(void) { return g_call_error.c_str(); }
+
+namespace {
+
+// Checks the arguments of the handle-free binning; SP_OK or SP_E_INVALID with the reason.
+int bin_args(const sp_particle* particles, uint32_t n, int32_t W, int32_t H, int32_t tiled, const sp_particle* sorted) {
+    g_call_error.clear();
+    if (W < 1 || H < 1 || W > 8192 || H > 8192 || (tiled != 0 && tiled != 1)) {
+        g_call_error = "grid must be between 1 x 1 and 8192 x 8192 cells, tiled 0 or 1";
+        return SP_E_INVALID;
+    }
+    if (n > 0 && (!particles || !sorted)) {
+        g_call_error = "null particle array";
+        return SP_E_INVALID;
+    }
+    const uintptr_t a = (uintptr_t)particles, b = (uintptr_t)sorted, bytes = (uintptr_t)n * sizeof(sp_particle);
+    if (n > 0 && a < b + bytes && b < a + bytes) {
+        g_call_error = "particles and sorted overlap";
+        return SP_E_INVALID;
+    }
+    if (bin_chunks(n) > (uint32_t)bin::TOP_MAX) {
+        g_call_error = "too many particles for one sort";
+        return SP_E_INVALID;
+    }
+    return SP_OK;
+}
+
+// The binning of caller particles on `stream`, in the work space `ws` (bin_space_bytes(n)).
+void launch_bin_particles(const sp_particle* particles, uint32_t n, int32_t W, int32_t H, int32_t tiled, sp_particle* sorted,
+                          uint32_t* perm, void* ws, hipStream_t stream) {
+    BinSpace sp = bin_space_at(ws, n);
+    const int bits = bin_bits(W, H, tiled);
+    const uint32_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bin_keys, dim3(blocks), dim3(256), 0, stream, particles, n, W, H, tiled, bin_dead_key(bits), sp.keys,
+                       sp.vals);
+    const uint32_t* order = launch_bin_passes(sp, n, bits, stream);
+    hipLaunchKernelGGL(k_bin_gather, dim3(blocks), dim3(256), 0, stream, particles, order, sorted, perm, n);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+int sp_bin_particles(const sp_particle* particles, uint32_t n, int32_t grid_w, int32_t grid_h, int32_t tiled,
+                     sp_particle* sorted, uint32_t* perm, void* stream) {
+    int rc = bin_args(particles, n, grid_w, grid_h, tiled, sorted);
+    if (rc != SP_OK || n == 0) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    void* ws = nullptr;
+    hipError_t e = hipMalloc(&ws, bin_space_bytes(n));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_call_error = std::string("hipMalloc (binning work space): ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? SP_E_NOMEM : SP_E_DEVICE;
+    }
+    try {
+        launch_bin_particles(particles, n, grid_w, grid_h, tiled, sorted, perm, ws, st);
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (const HipError& err) {
+        g_call_error = std::string(err.what) + " failed: " + hipGetErrorString(err.e);
+        rc = SP_E_DEVICE;
+    }
+    (void)hipFree(ws);
+    return rc;
+}
+
 namespace {
 
 // The render buffer `which` of a resident tank: address, element count and size.
@@ -1040,16 +1116,7 @@ int sp_tank_bytes(int32_t grid_w, int32_t grid_h, uint32_t capacity, uint64_t* e
     TankRef ref{nullptr, grid_w, grid_h, capacity, lv.total, lv.tiles, lv.n, lv.tail};
     size_t arena = 0;
     (void)ctx_of(ref, &arena);
-    const size_t cap = std::max<uint32_t>(capacity, 1);
-    // the radix sort's work space as the library asks for it; with no device to ask, its
-    // bound: a second key and value array and 1 MiB of histograms
-    size_t sort_tmp = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (uint32_t*)nullptr, (int)cap, 0, 32) != hipSuccess) {
-        (void)hipGetLastError();
-        sort_tmp = 8 * cap + (1u << 20);
-    }
-    *evictable = arena + (sizeof(sp_particle) + sizeof(float4) + 16) * cap + std::max<size_t>(sort_tmp, 1);
+    *evictable = big_bytes(arena, capacity);
     *fixed = sizeof(sp_particle) * SP_SPAWN_MAX + sizeof(uint32_t) * (8 + 2 * SP_MAX_MATERIALS) +
              sizeof(uint32_t) * (size_t)grid_w * grid_h;
     return SP_OK;
@@ -1296,5 +1363,4 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
 }  // extern "C"
 
 #include "batch.hip"
-#include "serve.hip"
 #include "pool.hip"

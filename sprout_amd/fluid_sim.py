@@ -250,75 +250,6 @@ class FluidSim:
         return g, mats
 
 
-class FluidServer:
-    """The request server (sp_server_*): resident scenes stepped on request with continuous
-    batching and an HBM budget. The FluidSim handles added belong to the server until close();
-    each scene's results are those it gets stepped alone with the same requests."""
-
-    def __init__(self, budget_bytes, streams=4, device=0):
-        L = N.lib()
-        h = C.c_void_p()
-        rc = L.sp_server_create(device, int(budget_bytes), streams, C.byref(h))
-        if rc != N.OK:
-            raise N.SproutError(rc, "sp_server_create")
-        self._h = h
-        self._lib = L
-        self.sims = []
-
-    def close(self):
-        """Runs what was submitted, stops the worker and gives the handles back."""
-        if getattr(self, "_h", None):
-            self._lib.sp_server_destroy(self._h)
-            self._h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
-    def _check(self, rc, what):
-        if rc != N.OK:
-            raise N.SproutError(rc, f"{what}: {self._lib.sp_server_last_error(self._h).decode()}")
-
-    def add(self, sim):
-        """Adds a set-up FluidSim as the next scene; returns its id."""
-        k = C.c_uint32()
-        self._check(self._lib.sp_server_add(self._h, sim._h, C.byref(k)), "server add")
-        self.sims.append(sim)
-        return k.value
-
-    def submit(self, scene, substeps, target):
-        t = C.c_uint64()
-        self._check(self._lib.sp_server_submit(self._h, scene, substeps, float(target), C.byref(t)), "server submit")
-        return t.value
-
-    def wait(self, ticket):
-        info = N.RequestInfo()
-        self._check(self._lib.sp_server_wait(self._h, ticket, C.byref(info)), "server wait")
-        return info
-
-    def poll(self, ticket):
-        """The request's info, or None while it is not done."""
-        info = N.RequestInfo()
-        rc = self._lib.sp_server_poll(self._h, ticket, C.byref(info))
-        if rc == N.E_BUSY:
-            return None
-        self._check(rc, "server poll")
-        return info
-
-    def drain(self):
-        self._check(self._lib.sp_server_drain(self._h), "server drain")
-
-    def stats(self):
-        st = N.ServerStats()
-        self._check(self._lib.sp_server_stats_get(self._h, C.byref(st)), "server stats")
-        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
-
-    def reset_stats(self):
-        self._check(self._lib.sp_server_reset_stats(self._h), "server reset_stats")
-
-
 class FluidPool:
     """Continuous batching of step requests over many resident scenes (sp_pool_*), run in
     rounds by the thread that calls step(), wait() or drain(); submit() may be called from
@@ -387,9 +318,9 @@ class FluidPool:
         self._check(self._lib.sp_pool_drain(self._h), "pool drain")
 
     def stats(self):
-        st = N.ServerStats()
+        st = N.PoolStats()
         self._check(self._lib.sp_pool_stats_get(self._h, C.byref(st)), "pool stats")
-        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
+        return {name: getattr(st, name) for name, _ in N.PoolStats._fields_}
 
     def reset_stats(self):
         self._check(self._lib.sp_pool_reset_stats(self._h), "pool reset_stats")

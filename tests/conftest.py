@@ -10,11 +10,3 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libsprout_hip.so on the device)")
-
-
-@pytest.fixture(scope="session")
-def gpu_available():
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    return True

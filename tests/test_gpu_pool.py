@@ -13,7 +13,7 @@ import pytest
 
 from sprout_amd import _native as N
 from sprout_amd.fluid_sim import FluidPool
-from test_gpu_serve import SPECS, assert_same, build
+from test_gpu_residency import SPECS, assert_same, build
 
 pytestmark = pytest.mark.gpu
 

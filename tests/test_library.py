@@ -44,7 +44,7 @@ def test_python_structs_match_the_header(tmp_path):
     import subprocess
     pairs = {"sp_material": N.Material, "sp_sim_params": N.SimParams, "sp_gmat": N.GMat,
              "sp_gparams": N.GParams, "sp_solver_stats": N.SolverStats, "sp_levels_info": N.LevelsInfo,
-             "sp_server_stats": N.ServerStats, "sp_magnet": N.Magnet, "sp_device_view": N.DeviceView}
+             "sp_pool_stats": N.PoolStats, "sp_magnet": N.Magnet, "sp_device_view": N.DeviceView}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sprout_hip.h"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -108,18 +108,12 @@ def test_tank_bytes_scale_with_the_tank():
     assert L.sp_tank_bytes(4, 4, 16, C.byref(ev), C.byref(fx)) == N.E_INVALID
 
 
-def test_server_and_batch_refuse_bad_arguments():
+def test_pool_and_batch_refuse_bad_arguments():
     L = N.lib()
     h = C.c_void_p()
-    assert L.sp_server_create(0, 1 << 30, 0, C.byref(h)) == N.E_INVALID
-    assert L.sp_server_create(0, 1 << 30, 17, C.byref(h)) == N.E_INVALID
-    assert L.sp_server_submit(None, 0, 1, 0.1, None) == N.E_INVALID
-    assert L.sp_server_drain(None) == N.E_INVALID
     assert L.sp_batch_create(None, 0, C.byref(h)) == N.E_INVALID
     r = C.c_int32()
     assert L.sp_is_resident(None, C.byref(r)) == N.E_INVALID
-    # the request server is switched off until a stall it met on the device is understood
-    assert L.sp_server_create(0, 1 << 30, 2, C.byref(h)) == N.E_UNSUPPORTED
     # the pool checks its arguments before touching a device
     assert L.sp_pool_create(0, 1 << 30, 0, C.byref(h)) == N.E_INVALID
     assert L.sp_pool_create(0, 1 << 30, 17, C.byref(h)) == N.E_INVALID

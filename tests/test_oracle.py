@@ -341,6 +341,36 @@ def test_sort_is_stable_by_cell():
     assert np.array_equal(after["id"], before["id"][order])
 
 
+def test_spawn_fills_fresh_then_dead_slots_in_order():
+    """The record's spawn step (mod.rs:2485-2503 + claim.comp:10-17): fresh slots take the
+    first pending particles, then every dead slot, in slot order, takes the next one; no live
+    particle is overwritten and what finds no slot is dropped."""
+    n0, cap = 200, 230
+    o = OracleSim((32, 24), cap)
+    rng = np.random.default_rng(5)
+    slots = np.zeros(n0, N.PARTICLE_DTYPE)
+    slots["kind"] = 1
+    slots["id"] = np.arange(n0)
+    dead = np.sort(rng.choice(n0, 40, replace=False))
+    slots["kind"][dead] = N.KIND_DEAD
+    o.put_field("PARTICLES", slots)
+    for m in (10, 30 + 25, 30 + 40 + 9):
+        o.put_field("PARTICLES", slots)
+        sp = np.zeros(m, N.PARTICLE_DTYPE)
+        sp["kind"] = 2
+        sp["id"] = 1000 + np.arange(m)
+        o.spawn(sp)
+        got = o.field("PARTICLES", N.PARTICLE_DTYPE, o.n_slots())
+        fresh = min(m, cap - n0)
+        assert o.n_slots() == n0 + fresh
+        assert np.array_equal(got["id"][n0:], 1000 + np.arange(fresh))
+        live = np.setdiff1d(np.arange(n0), dead)
+        assert np.array_equal(got["id"][live], live)  # live particles untouched
+        claimed = min(m - fresh, len(dead))
+        assert np.array_equal(got["id"][dead[:claimed]], 1000 + fresh + np.arange(claimed))
+        assert np.all(got["kind"][dead[claimed:]] == N.KIND_DEAD)
+
+
 def test_cell_counts_and_stats():
     o, g, mats, parts, field = make((64, 36))
     got = o.cell_counts(3, 3, 30, 20).reshape(20, 30)
