@@ -305,15 +305,17 @@ struct BinSpace {
     uint32_t *keys, *vals, *keys2, *vals2, *hist, *chunk;
 };
 
-uint32_t bin_groups(uint32_t n) { return std::max<uint32_t>((n + bin::TILE - 1) / bin::TILE, 1u); }
-uint32_t bin_chunks(uint32_t n) {
-    return (uint32_t)(((size_t)bin::RADIX * bin_groups(n) + bin::SCAN_CHUNK - 1) / bin::SCAN_CHUNK);
+constexpr uint32_t bin_groups(uint32_t n) {
+    return n == 0u ? 1u : (uint32_t)(((uint64_t)n + bin::TILE - 1) / bin::TILE);
 }
-size_t bin_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr uint32_t bin_chunks(uint32_t n) {
+    return (uint32_t)(((uint64_t)bin::RADIX * bin_groups(n) + bin::SCAN_CHUNK - 1) / bin::SCAN_CHUNK);
+}
+constexpr size_t bin_align(size_t b) { return (b + 255) & ~(size_t)255; }
 
-size_t bin_space_bytes(uint32_t n) {
-    const size_t m = std::max<uint32_t>(n, 1);
-    return 4 * bin_align(4 * m) + bin_align(4 * (size_t)bin::RADIX * bin_groups(n)) + bin_align(4 * (size_t)bin_chunks(n));
+constexpr size_t bin_space_bytes(uint32_t n) {
+    return 4 * bin_align(4 * (size_t)(n == 0u ? 1u : n)) + bin_align(4 * (size_t)bin::RADIX * bin_groups(n)) +
+           bin_align(4 * (size_t)bin_chunks(n));
 }
 
 BinSpace bin_space_at(void* base, uint32_t n) {
