@@ -138,6 +138,17 @@ int sp_sort(sp_sim* sim);
  * SP_E_INVALID (arguments), SP_E_NOMEM, SP_E_DEVICE; sp_error_text() tells which. */
 int sp_bin_particles(const sp_particle* particles, uint32_t n, int32_t grid_w, int32_t grid_h, int32_t tiled,
                      sp_particle* sorted, uint32_t* perm, void* stream);
+/* No counterpart in the reference. sp_bin_particles for callers that cannot allocate
+ * device memory (work recorded under HIP graph capture, serving loops that forbid
+ * hipMalloc): the work space is a fixed array of the library in device memory, one per
+ * device (about 17 MB, held until the process ends), instead of memory allocated for the
+ * call, so at most SP_BIN_NOALLOC_MAX particles (more: SP_E_CAPACITY). The call enqueues
+ * the sort on `stream` and returns without waiting. Every call on one device uses that
+ * device's one array: calls whose sorts overlap on a device (other streams, other threads)
+ * race and give wrong results; order them (one stream, or events). */
+#define SP_BIN_NOALLOC_MAX 1048576u
+int sp_bin_particles_noalloc(const sp_particle* particles, uint32_t n, int32_t grid_w, int32_t grid_h, int32_t tiled,
+                             sp_particle* sorted, uint32_t* perm, void* stream);
 /* Text of the last error of a handle-free entry point on this thread ("" if none). */
 const char* sp_error_text(void);
 

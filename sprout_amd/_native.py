@@ -17,6 +17,7 @@ MAX_MATERIALS = 96
 MAX_LEVELS = 16
 PPC = 4
 SPAWN_MAX = 32768
+BIN_NOALLOC_MAX = 1048576  # SP_BIN_NOALLOC_MAX
 
 KIND_DEAD = 0
 KIND_FRESH = 0x10000
@@ -248,6 +249,7 @@ SIGNATURES = {
     "sp_stats": ([_P, C.POINTER(StatsOut)], C.c_int),
     "sp_sort": ([_P], C.c_int),
     "sp_bin_particles": ([_P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P], C.c_int),
+    "sp_bin_particles_noalloc": ([_P, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P], C.c_int),
     "sp_error_text": ([], C.c_char_p),
     "sp_render_view": ([_P, C.c_int32, _P], C.c_int),
     "sp_render_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),

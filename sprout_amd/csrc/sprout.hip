@@ -1007,6 +1007,67 @@ int sp_bin_particles(const sp_particle* particles, uint32_t n, int32_t grid_w, i
     return rc;
 }
 
+// ---- the binning without allocation ------------------------------------------------------
+//
+// The work space of sp_bin_particles_noalloc: a fixed array in device memory, one per device
+// (each device that runs the library holds its own copy until the process ends), sized for
+// SP_BIN_NOALLOC_MAX particles. Host code never passes `scratch` itself to a kernel (on the
+// host its name is only a stand-in for the device symbol): it takes the array's device
+// address from hipGetSymbolAddress once per device and uses that address everywhere.
+constexpr size_t BIN_NOALLOC_BYTES = bin_space_bytes(SP_BIN_NOALLOC_MAX);
+static __device__ __align__(256) unsigned char scratch[BIN_NOALLOC_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up on the first call on
+// that device (with it current) and kept.
+hipError_t scratch_of_current_device(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+int sp_bin_particles_noalloc(const sp_particle* particles, uint32_t n, int32_t grid_w, int32_t grid_h, int32_t tiled,
+                             sp_particle* sorted, uint32_t* perm, void* stream) {
+    int rc = bin_args(particles, n, grid_w, grid_h, tiled, sorted);
+    if (rc != SP_OK || n == 0) return rc;
+    if (n > SP_BIN_NOALLOC_MAX) {
+        g_call_error = "sp_bin_particles_noalloc sorts at most SP_BIN_NOALLOC_MAX (" + std::to_string(SP_BIN_NOALLOC_MAX) +
+                       ") particles; " + std::to_string(n) + " given";
+        return SP_E_CAPACITY;
+    }
+    void* ws = nullptr;
+    hipError_t e = scratch_of_current_device(&ws);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_call_error = std::string("hipGetSymbolAddress (binning work space): ") + hipGetErrorString(e);
+        return SP_E_DEVICE;
+    }
+    try {
+        launch_bin_particles(particles, n, grid_w, grid_h, tiled, sorted, perm, ws, (hipStream_t)stream);
+    } catch (const HipError& err) {
+        g_call_error = std::string(err.what) + " failed: " + hipGetErrorString(err.e);
+        return SP_E_DEVICE;
+    }
+    return SP_OK;
+}
+
 namespace {
 
 // The render buffer `which` of a resident tank: address, element count and size.
