@@ -302,6 +302,16 @@ typedef struct sp_stepbuf {
     uint32_t pad0;       /* float bits: Kahan compensation of `advanced` */
 } sp_stepbuf;
 
+/* The scalar state of a pressure CG in one iteration (reduce_tail.glsl:59-116's rz, rr, bb,
+ * alpha, beta, done and counts), FP64 scalars. The solve keeps two, by iteration parity:
+ * iteration k's state is worked out from iteration k - 1's into the other slot, so that
+ * nothing reads a slot while it is written. */
+typedef struct sp_cg_state {
+    double rz[2], rr[2], bb[2], beta[2], alpha[2];
+    uint32_t done[2], iters[2];
+    uint32_t sweeps, halt, cap, pad;
+} sp_cg_state;
+
 /* Krylov state of the pressure solve (`SolveBuf`, common.glsl:138-149), with the scalar
  * reductions in FP64. */
 typedef struct sp_solve {
@@ -322,6 +332,7 @@ typedef struct sp_solve {
      * tanks that record more iterations stops where it would alone. */
     uint32_t cap_p, cap_p2, cap_v, cap_now;
     uint32_t sweeps, halt, v_sweeps, v_halt;
+    sp_cg_state st[2];   /* the pressure CG's state entering iterations of each parity */
 } sp_solve;
 
 #define SP_MAX_LEVELS 16
@@ -446,6 +457,7 @@ enum {
     SP_FIELD_DSETTLE = 54,   /* float2[N*n_fam] settling's gain and loss factors per cell and family */
     SP_FIELD_MAGC = 55,      /* float2[N] susceptibility of the cell's liquid, magnetic energy density (grid units) */
     SP_FIELD_RMASK = 56,     /* uint32[N] bit j: the tank's j-th row makes up more than REACT_MIN of the cell */
+    SP_FIELD_MGX = 57,       /* float2[total]: the V-cycle's pre-smoothed iterate per level */
     SP_FIELD_COUNT
 };
 
@@ -493,5 +505,7 @@ SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:1
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
 SP_STATIC_ASSERT(sizeof(sp_gmat) == 192, "one grid-unit material row is 48 words");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
+SP_STATIC_ASSERT(sizeof(sp_cg_state) == 112, "CG state: 8 doubles, 12 words, no padding");
+SP_STATIC_ASSERT(sizeof(sp_solve) == 240 + 2 * 112, "solve record layout (mirrored by the Python binding)");
 
 #endif

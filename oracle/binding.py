@@ -14,7 +14,7 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55, RMASK=56)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55, RMASK=56, MGX=57)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
              VISC_SOLVE=14, RHS_VISC=15, PRESSURE2=16, PROJECT_VISC=17, VISC_VCYCLE=18, HEAT=19,

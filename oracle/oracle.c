@@ -3673,6 +3673,7 @@ static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     case SP_FIELD_FAMOCC: *bytes = sizeof(float) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->famocc;
     case SP_FIELD_MAGC: *bytes = sizeof(v2) * N; return s->magc;
     case SP_FIELD_RMASK: *bytes = sizeof(uint32_t) * N; return s->rmask;
+    case SP_FIELD_MGX: *bytes = sizeof(v2) * T; return s->mgx;
     case SP_FIELD_DSETTLE: *bytes = sizeof(v2) * N * (size_t)(s->P.n_fam > 1 ? s->P.n_fam : 1); return s->dsettle;
     default: *bytes = 0; return NULL;
     }

@@ -45,7 +45,7 @@ FIELD = dict(PARTICLES=0, ACC=1, VEL=2, CELL=3, COMP=4, COEF=5, SURF=6, PJUMP=7,
              MGR=19, PTILE=20, REGION_AIR=21, SURF2=22, VW=23, VS=24, GBASE=25, VR=26, VP=27,
              VP2=28, VQ=29, VD=30, VR2=31, VQ2=32, VAZ=33, VTILES=34, VTILE=35, VMA=36, VMB=37,
              VMD=38, VMR=39, VMX=40, VMZ=41, P2KEEP=42, THERM=43, DHEAT=44, HCELL=45, SOLID_T=46,
-             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55, RMASK=56)
+             LEDGER=47, RL=48, RBODY=49, RH=50, ESTR=51, PSTRESS=52, FAMOCC=53, DSETTLE=54, MAGC=55, RMASK=56, MGX=57)
 # Stage ids (SP_STAGE_*)
 STAGE = dict(STEP=0, CLEAR_ACC=1, PARTICLES=2, GRID_PREP=3, COEF=4, SURF=5, VISC=6, RHS=7,
              PRESSURE=8, PROJECT=9, SUBSTEP=10, MG_BUILD=11, VCYCLE=12, VISC_SPLIT=13,
@@ -213,6 +213,11 @@ STEP_DTYPE = np.dtype([("frame_left", "<f4"), ("advanced", "<f4"), ("batch_steps
                        ("batch_done", "<u4"), ("dt", "<f4"), ("dt_prev", "<f4"),
                        ("umax_grid", "<u4"), ("umax_part", "<u4"), ("dt_cfl", "<f4"),
                        ("substeps", "<u4"), ("umax_seen", "<f4"), ("pad0", "<u4")])
+# sp_cg_state: the pressure CG's scalars entering an iteration (two, by parity).
+CG_STATE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("rr", "<f8", (2,)), ("bb", "<f8", (2,)), ("beta", "<f8", (2,)),
+                           ("alpha", "<f8", (2,)), ("done", "<u4", (2,)), ("iters", "<u4", (2,)), ("sweeps", "<u4"),
+                           ("halt", "<u4"), ("cap", "<u4"), ("pad", "<u4")])
+assert CG_STATE_DTYPE.itemsize == 112
 SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<f8", (2,)),
                         ("bb", "<f8", (2,)), ("rr", "<f8", (2,)), ("done", "<u4", (2,)),
                         ("iters", "<u4", (2,)), ("it_max", "<u4"), ("unconv", "<u4"),
@@ -223,7 +228,9 @@ SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<
                         ("v_it_max", "<u4"), ("v_unconv", "<u4"), ("v_res_max", "<f4"),
                         ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("pad", "<u4"),
                         ("cap_p", "<u4"), ("cap_p2", "<u4"), ("cap_v", "<u4"), ("cap_now", "<u4"),
-                        ("sweeps", "<u4"), ("halt", "<u4"), ("v_sweeps", "<u4"), ("v_halt", "<u4")])
+                        ("sweeps", "<u4"), ("halt", "<u4"), ("v_sweeps", "<u4"), ("v_halt", "<u4"),
+                        ("st", CG_STATE_DTYPE, (2,))])
+assert SOLVE_DTYPE.itemsize == 464
 
 # Exported symbols of include/sprout_hip.h, with their argument types.
 _P = C.c_void_p

@@ -37,7 +37,7 @@ struct Ctx {
     float2 *mgx, *mgz, *mgr, *cgp, *cgp2, *cgq;
     float2* r2;           // second level-0 residual buffer of the CG (alternates with mgr)
     uint32_t* ptile;
-    double4* part_sums;   // per-block partial sums of a reduction
+    double4* part_sums;   // per-block partial sums of the pressure CG's reductions (three kinds)
     const int4* lev;      // per level: offset, width, height, tile offset
     int n_levels, tail;
     // implicit viscosity (per cell of the grid; the multigrid per cell of every level)
@@ -133,7 +133,7 @@ __host__ __device__ __forceinline__ Ctx ctx_of(const TankRef& r, size_t* bytes =
     c.ptile = (uint32_t*)take(sizeof(uint32_t) * (size_t)r.tiles);
     c.vtile = (uint32_t*)take(sizeof(uint32_t) * (size_t)r.tiles);
     c.vtiles = (uint32_t*)take(sizeof(uint32_t) * nb0);
-    c.part_sums = (double4*)take(sizeof(double4) * 2 * nb0);
+    c.part_sums = (double4*)take(sizeof(double4) * 3 * nb0);
     c.vpart = (double4*)take(sizeof(double4) * nb0);
     c.therm = (float4*)take(sizeof(float4) * N);
     c.dheat = (float4*)take(sizeof(float4) * N);

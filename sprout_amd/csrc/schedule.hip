@@ -36,6 +36,7 @@ struct Run {
     sp_gparams gp{};    // host copy of the tables: the schedule fields are equal across the group
     hipStream_t stream = nullptr;
     size_t tail_lds = 0;  // dynamic LDS of k_mg_tail
+    bool fuse = true;     // coarse multigrid levels two per launch (SPROUT_MG_FUSE=0: one)
     uint32_t slots = 0;   // most particle slots of any tank of the group
 };
 
@@ -55,24 +56,55 @@ dim3 grid0(const Run& r) { return groups_of(r, r.W, r.H); }
 
 // ---- pressure (record_pressure / record_cg / record_vcycle, mod.rs:2929-3031) -------------
 
-// One V-cycle on the level-0 residual (mgr, or r2 with rsel); with `skip_down0` the way down
-// on level 0 has already run (fused into the CG update).
-// H: the multigrid's storage (mg_half, MG<H> in solve.hip).
+Lv lv_of(const sp_levels_info& L, int l) { return Lv{L.offs[l], L.dims[l][0], L.dims[l][1], L.tile_offs[l]}; }
+dim3 grid_t0(const Run& r) { return dim3((r.W + T0 - 1) / T0, (r.H + T0 - 1) / T0, r.T); }
+int parts_t0(const Run& r) { return ((r.W + T0 - 1) / T0) * ((r.H + T0 - 1) / T0); }
+
+// The coarse levels 1 .. tail - 1 of a V-cycle, on the way down and on the way up: two
+// levels per launch, pairing from the coarsest (level 1, the largest, alone when their
+// number is odd), or one per launch without `fuse`. H: the multigrid's storage.
 template <bool H>
-void launch_vcycle_t(const Run& r, int gated, int reduce, int rsel, bool skip_down0) {
+void launch_coarse_down(const Run& r, int gated, int par) {
     const sp_levels_info& L = r.lv;
-    for (int l = skip_down0 ? 1 : 0; l < L.tail; l++)
-        hipLaunchKernelGGL(k_mg_down<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated, rsel);
-    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, gated);
-    for (int l = L.tail - 1; l >= 0; l--)
-        hipLaunchKernelGGL(k_mg_up<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, l, gated,
-                           l == 0 ? reduce : -1, r.nb0, rsel);
-    if (reduce >= 0) hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, reduce, r.nb0, gated);
+    int l = 1;
+    if (!r.fuse || (L.tail - 1) % 2 == 1) {
+        for (; l < L.tail && (l == 1 || !r.fuse); l++)
+            hipLaunchKernelGGL(k_mg_down<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, lv_of(L, l),
+                               lv_of(L, l + 1), l, gated, par, 0);
+    }
+    for (; l + 1 < L.tail; l += 2)
+        hipLaunchKernelGGL(k_mg_down2<H>, groups_of(r, L.dims[l + 1][0], L.dims[l + 1][1]), dim3(32, 32), DOWN2_LDS, r.stream, r.tb,
+                           lv_of(L, l), lv_of(L, l + 1), lv_of(L, l + 2), l, gated, par);
 }
 
-void launch_vcycle(const Run& r, int gated, int reduce, int rsel, bool skip_down0 = false) {
-    if (r.gp.mg_half) launch_vcycle_t<true>(r, gated, reduce, rsel, skip_down0);
-    else launch_vcycle_t<false>(r, gated, reduce, rsel, skip_down0);
+template <bool H>
+void launch_coarse_up(const Run& r, int gated, int par) {
+    const sp_levels_info& L = r.lv;
+    const int single = !r.fuse ? L.tail - 1 : (L.tail - 1) % 2;  // levels 1 .. single run one per launch
+    for (int l = L.tail - 2; l > single; l -= 2)
+        hipLaunchKernelGGL(k_mg_up2<H>, groups_of(r, L.dims[l + 1][0], L.dims[l + 1][1]), dim3(32, 32), UP2_LDS, r.stream, r.tb,
+                           lv_of(L, l), lv_of(L, l + 1), lv_of(L, l + 2), l, gated, par);
+    for (int l = single; l >= 1; l--)
+        hipLaunchKernelGGL(k_mg_up<H>, groups_of(r, L.dims[l][0], L.dims[l][1]), TILE, 0, r.stream, r.tb, lv_of(L, l),
+                           lv_of(L, l + 1), l, gated, par);
+}
+
+// One V-cycle from the level-0 residual (mgr, or r2 with rsel), ungated: the first of a solve
+// (reduce: with the partial sums r.z, r.r) and the debug stage.
+template <bool H>
+void launch_vcycle_t(const Run& r, int reduce, int rsel) {
+    const sp_levels_info& L = r.lv;
+    hipLaunchKernelGGL(k_mg_down<H>, grid0(r), TILE, 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), 0, 0, 0, rsel);
+    launch_coarse_down<H>(r, 0, 0);
+    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 0, 0);
+    launch_coarse_up<H>(r, 0, 0);
+    hipLaunchKernelGGL(k_up0<H>, grid_t0(r), dim3(T0, T0), 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), parts_t0(r), 0, 0, reduce,
+                       rsel);
+}
+
+void launch_vcycle(const Run& r, int reduce, int rsel) {
+    if (r.gp.mg_half) launch_vcycle_t<true>(r, reduce, rsel);
+    else launch_vcycle_t<false>(r, reduce, rsel);
 }
 
 template <bool H>
@@ -89,26 +121,34 @@ void launch_mg_build(const Run& r) {
 }
 
 // The first solve builds the hierarchy and rescales the warm start; the second (after the
-// viscosity, `second`) reuses both and starts from what rhs_visc left. The level-0 residual
-// alternates between mgr and r2, the direction between cgp and cgp2, with the iteration's
-// parity (k_update_down0, k_cg_apply).
+// viscosity, `second`) reuses both and starts from what rhs_visc left. Iteration k runs the
+// scalar step that ended iteration k - 1, cg_apply, alpha, the update with the way down on
+// level 0, the coarse levels, the tail and the way up; its parity k & 1 picks the
+// solve-state slot, the residual buffer (mgr / r2) and the direction (cgp / cgp2).
 template <bool H>
 void launch_pressure_t(const Run& r, uint32_t iters, bool second) {
-    const dim3 g0 = grid0(r);
+    const sp_levels_info& L = r.lv;
+    const Lv L0 = lv_of(L, 0), L1 = lv_of(L, 1);
+    const int nbp = parts_t0(r);
+    const dim3 gt0 = grid_t0(r), bt0(T0, T0);
     if (!second) {
         launch_mg_build_t<H>(r);
-        hipLaunchKernelGGL(k_cg_scale, g0, TILE, 0, r.stream, r.tb);
+        hipLaunchKernelGGL(k_cg_scale, grid0(r), TILE, 0, r.stream, r.tb);
     }
-    hipLaunchKernelGGL(k_cg_init, g0, TILE, 0, r.stream, r.tb, r.nb0);
-    launch_vcycle_t<H>(r, 0, second ? (int)RED_START2 : (int)RED_START, 0, false);
+    hipLaunchKernelGGL(k_cg_init, gt0, bt0, 0, r.stream, r.tb, L0, nbp);
+    launch_vcycle_t<H>(r, 1, 0);
     for (uint32_t k = 0; k < iters; k++) {
         const int par = (int)(k & 1);
-        hipLaunchKernelGGL(k_cg_apply<H>, g0, TILE, 0, r.stream, r.tb, r.nb0, par);
-        hipLaunchKernelGGL(k_cg_reduce, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, (int)RED_ALPHA, r.nb0, 1);
-        hipLaunchKernelGGL(k_update_down0<H>, g0, TILE, 0, r.stream, r.tb, par);
-        launch_vcycle_t<H>(r, 1, RED_BETA, par ^ 1, true);
+        hipLaunchKernelGGL(k_cg_step, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, nbp, par, k == 0 ? 1 : 0, second ? 1 : 0);
+        hipLaunchKernelGGL(k_cg_apply<H>, gt0, bt0, 0, r.stream, r.tb, L0, nbp, par);
+        hipLaunchKernelGGL(k_cg_alpha, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, nbp, par);
+        hipLaunchKernelGGL(k_update_down0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, par);
+        launch_coarse_down<H>(r, 1, par);
+        hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 1, par);
+        launch_coarse_up<H>(r, 1, par);
+        hipLaunchKernelGGL(k_up0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, nbp, 1, par, 1, par ^ 1);
     }
-    hipLaunchKernelGGL(k_cg_end, per_tank(r), dim3(64), 0, r.stream, r.tb, second ? 1 : 0);
+    hipLaunchKernelGGL(k_cg_end, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, nbp, (int)iters, second ? 1 : 0);
 }
 
 void launch_pressure(const Run& r, uint32_t iters, bool second = false) {

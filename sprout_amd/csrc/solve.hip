@@ -4,17 +4,22 @@
 //
 // Layout: every level of the hierarchy back to back in mga (float4 couplings L, R, B, T),
 // mgd (diagonal), mgx / mgz / mgr (float2 per cell: pre-smoothed iterate, finished
-// iterate, right-hand side), as the reference lays them out (mg.glsl:1-37). Level kernels
-// run one work-item per cell on 16 x 16 tiles and skip tiles without unknowns (ptile).
+// iterate, right-hand side), as the reference lays them out (mg.glsl:1-37). Kernels skip the
+// 16 x 16 tiles without unknowns (ptile).
+//
+// Launches per CG iteration: the level-0 kernels (cg_apply with the scalar step that ends
+// the previous iteration; the update fused with the way down; the way up), the coarse levels
+// two per launch, and the tail of the hierarchy in one work-group: eight at 1024 x 576
+// instead of the reference's thirteen dispatches and two one-group reductions.
 //
 // Precision: cell arithmetic is FP32 in the reference's operation order; the scalar
 // reductions (r.z, r.r, p.q, b.b) are FP64 — the product of two floats is exact in double
-// and the sums are formed per work-group and then in a fixed order by a one-group kernel,
-// so the scalars do not depend on scheduling. With mg_half the V-cycle's storage is
-// narrower (MG<true> below); its arithmetic stays FP32.
+// and the sums are formed per work-group and then in a fixed order, so the scalars do not
+// depend on scheduling. With mg_half the V-cycle's storage is narrower (MG<true> below);
+// its arithmetic stays FP32.
 //
 // Convergence without host round trips: the scalar state lives in device memory (sp_solve);
-// every kernel of an iteration reads the two `done` flags at entry and returns at once when
+// every kernel of an iteration reads its `done` flags at entry and returns at once when
 // both potentials have converged, so a batch can carry a fixed number of recorded iterations.
 #include "dev.h"
 
@@ -69,6 +74,9 @@ struct MG {
     // right-hand sides of the coarse levels (i >= N)
     SP_DEV static float2 R(const Ctx& c, int i) { return c.mgr[i]; }
     SP_DEV static void set_R(const Ctx& c, int i, float2 v) { c.mgr[i] = v; }
+    // a coarse right-hand side or iterate as it reads back after storing
+    SP_DEV static float2 stored_R(float2 v) { return v; }
+    SP_DEV static float2 stored_Z(float2 v) { return v; }
 };
 
 template <>
@@ -112,13 +120,27 @@ struct MG<true> {
         return unpack_bf2(reinterpret_cast<const uint32_t*>(c.mgr + c.N)[i - c.N]);
     }
     SP_DEV static void set_R(const Ctx& c, int i, float2 v) { reinterpret_cast<uint32_t*>(c.mgr + c.N)[i - c.N] = pack_bf2(v); }
+    SP_DEV static float2 stored_R(float2 v) { return unpack_bf2(pack_bf2(v)); }
+    SP_DEV static float2 stored_Z(float2 v) { return unpack_bf2(pack_bf2(v)); }
 };
-// A solve is over for this tank when both potentials converged or when it has done the
-// iterations this tank records (its cap), whichever comes first.
-SP_DEV bool solve_done(const Ctx& c) { return (c.K->done[0] != 0u && c.K->done[1] != 0u) || c.K->halt != 0u; }
 
 SP_DEV int tid2() { return threadIdx.y * 16 + threadIdx.x; }
 SP_DEV int tile_slot() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// The tile (of a grid of them, gridDim.x x gridDim.y) a work-group of the solve works on.
+// Work-groups go to the eight XCDs in turn, and each XCD has an L2 of its own, so
+// consecutive work-groups would sit on different L2s and fetch their neighbours' rings
+// again; here each XCD is handed a contiguous run of tiles (whole rows of them), so that the
+// rings mostly come from its own L2. A bijection of the tiles: only the speed depends on how
+// work-groups are dispatched.
+SP_DEV int2 tile_xy() {
+    const int nx = (int)gridDim.x, n = nx * (int)gridDim.y;
+    const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int q = n >> 3, r = n & 7, xcd = b & 7, k = b >> 3;
+    const int t = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
+    return make_int2(t % nx, t / nx);
+}
+SP_DEV int slot_of(int2 tq) { return tq.y * (int)gridDim.x + tq.x; }
 
 SP_DEV void add4(double4& a, const double4& b) {
     a.x += b.x;
@@ -144,7 +166,9 @@ SP_DEV double4 group_sum4(double4 v, double4* lds) {
     if ((t & 63) == 0) lds[t >> 6] = v;
     __syncthreads();
     double4 s = lds[0];
-#pragma unroll
+    // in order, a few waves' sums in flight at a time (all of them at once would hold
+    // 8 registers per wave of the group)
+#pragma unroll 3
     for (int w = 1; w < NT / 64; w++) add4(s, lds[w]);
     __syncthreads();
     return s;
@@ -152,85 +176,28 @@ SP_DEV double4 group_sum4(double4 v, double4* lds) {
 
 SP_DEV void store_part(double4* p, double4 v) { *p = v; }
 
-// Sum of parts[base .. base + n) by one group of NT work-items in a fixed order; each
-// work-item issues its loads four at a time before adding, so their latencies overlap.
+// Sum of parts[base .. base + n) by one group of NT work-items in a fixed order (work-item
+// t adds entries t, t + NT, t + 2 NT ... in turn); each work-item issues its loads eight at a
+// time before adding, so their latencies overlap (2304 partial sums at 1024 x 576: two
+// rounds for 256 work-items).
 template <int NT>
 SP_DEV double4 total_of(const double4* parts, int base, int n, double4* lds) {
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
     double4 s = make_double4(0, 0, 0, 0);
-    for (int k0 = 0; k0 < n; k0 += 4 * NT) {
-        double4 v[4];
+    for (int k0 = 0; k0 < n; k0 += 8 * NT) {
+        double4 v[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < 8; j++) {
             const int k = k0 + j * NT + t;
             v[j] = k < n ? parts[base + k] : make_double4(0, 0, 0, 0);
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) add4(s, v[j]);
+        for (int j = 0; j < 8; j++) add4(s, v[j]);
     }
     return group_sum4<NT>(s, lds);
 }
 constexpr int REDUCE_NT = 256;  // work-items of the one-group reduction kernels
 SP_DEV double4 total(const Ctx& c, int base, int n, double4* lds) { return total_of<REDUCE_NT>(c.part_sums, base, n, lds); }
-
-// Modes 0, 1, 2 of reduce_tail.glsl, and 8: the start of the second solve (after the
-// viscosity), where the density-correction potential counts as solved.
-enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2, RED_START2 = 8 };
-
-// The scalar step of the pressure CG (reduce_tail.glsl:59-116 modes 0, 1, 2), FP64.
-// parts [0, n): r.z (xy), r.r (zw) of the last V-cycle; [n, 2n): b.b (xy) or p.q (xy).
-SP_DEV void reduce_scalars(const Ctx& c, int mode, int n, double4* lds) {
-    sp_solve* K = c.K;
-    const sp_gparams& P = *c.P;
-    const double tx = (double)P.tol[0] * (double)P.tol[0], ty = (double)P.tol[1] * (double)P.tol[1];
-    if (mode == RED_START || mode == RED_START2) {
-        double4 a = total(c, 0, n, lds);
-        double4 b = total(c, n, n, lds);
-        if (tid2() == 0) {
-            K->bb[0] = b.x;
-            K->bb[1] = b.y;
-            K->rz[0] = a.x;
-            K->rz[1] = a.y;
-            K->rr[0] = a.z;
-            K->rr[1] = a.w;
-            K->beta[0] = K->beta[1] = 0.0;
-            K->alpha[0] = K->alpha[1] = 0.0;
-            K->iters[0] = K->iters[1] = 0u;
-            K->sweeps = 0u;
-            K->halt = 0u;
-            K->cap_now = mode == RED_START2 ? K->cap_p2 : K->cap_p;
-            K->done[0] = a.z <= tx * b.x ? 1u : 0u;
-            K->done[1] = (mode == RED_START2 || a.w <= ty * b.y) ? 1u : 0u;
-            K->solves += 1u;
-        }
-    } else if (mode == RED_ALPHA) {
-        double4 b = total(c, n, n, lds);
-        if (tid2() == 0) {
-            K->alpha[0] = (K->done[0] == 0u && b.x > 0.0) ? K->rz[0] / b.x : 0.0;
-            K->alpha[1] = (K->done[1] == 0u && b.y > 0.0) ? K->rz[1] / b.y : 0.0;
-        }
-    } else {
-        double4 a = total(c, 0, n, lds);
-        if (tid2() == 0) {
-            uint32_t was0 = K->done[0], was1 = K->done[1];
-            K->iters[0] += 1u - was0;
-            K->iters[1] += 1u - was1;
-            uint32_t d0 = was0 | (a.z <= tx * K->bb[0] ? 1u : 0u);
-            uint32_t d1 = was1 | (a.w <= ty * K->bb[1] ? 1u : 0u);
-            K->beta[0] = (d0 == 0u && K->rz[0] > 0.0) ? a.x / K->rz[0] : 0.0;
-            K->beta[1] = (d1 == 0u && K->rz[1] > 0.0) ? a.y / K->rz[1] : 0.0;
-            K->rz[0] = a.x;
-            K->rz[1] = a.y;
-            if (!was0) K->rr[0] = a.z;
-            if (!was1) K->rr[1] = a.w;
-            K->done[0] = d0;
-            K->done[1] = d1;
-            // the last kernel of an iteration: stop here if this tank records no more
-            K->sweeps += 1u;
-            if (K->cap_now != 0u && K->sweeps >= K->cap_now && (d0 == 0u || d1 == 0u)) K->halt = 1u;
-        }
-    }
-}
 
 }  // namespace
 
@@ -329,35 +296,282 @@ __global__ __launch_bounds__(256) void k_cg_scale(const TankBatch tb) {
     }
 }
 
-// cg_init.comp: r = b - A x and the partial sums of b.b (into the second half of parts).
-__global__ __launch_bounds__(256) void k_cg_init(const TankBatch tb, int nblocks) {
+// ---- the CG's scalar steps -------------------------------------------------------------
+//
+// The reference folds each scalar step into the last work-group of the kernel that made the
+// partial sums (reduce_tail.glsl:28-57, behind an atomic arrival and a device-wide fence).
+// Here a one-work-group kernel per step sums the partials in a fixed order after the kernel
+// boundary has ordered them: on a part with eight L2 caches a device-scope release in every
+// producing work-group costs more than a launch. (Letting every work-group of the consuming
+// kernel sum all 2304 partials at 1024 x 576 instead moves about 20 MB through each L2 per
+// step, which costs more again.) The scalar state of iteration k is kept in slot k & 1 of the
+// solve record (sp_cg_state): the step that starts iteration k reads slot (k - 1) & 1.
+//
+// Partial sums, per level-0 block (nbp of them): [0, nbp) r.z (xy), r.r (zw) of the last
+// level-0 up-sweep; [nbp, 2 nbp) b.b (xy) of cg_init; [2 nbp, 3 nbp) p.q (xy) of cg_apply.
+
+namespace {
+
+constexpr int T0 = 16;          // level-0 block of the Krylov kernels: TT x TT tiles of 16 x 16
+constexpr int TT = T0 / 16;
+constexpr int NT0 = T0 * T0;    // its work-items
+constexpr int PART_RZ = 0, PART_BB = 1, PART_PQ = 2;
+
+SP_DEV int tid0() { return threadIdx.y * T0 + threadIdx.x; }
+SP_DEV bool any_on(const uint32_t* s_on) {
+    uint32_t on = 0u;
+    for (int k = 0; k < TT * TT; k++) on |= s_on[k];
+    return on != 0u;
+}
+SP_DEV int slot0() { return slot_of(tile_xy()); }
+
+SP_DEV bool st_done(const sp_cg_state& s) { return (s.done[0] != 0u && s.done[1] != 0u) || s.halt != 0u; }
+// A solve is over for this tank, in the iteration of parity `par`, when both potentials
+// converged or when it has done the iterations this tank records (its cap).
+SP_DEV bool slot_done(const Ctx& c, int par) {
+    const sp_cg_state& s = c.K->st[par];
+    return (s.done[0] != 0u && s.done[1] != 0u) || s.halt != 0u;
+}
+
+// Whether 16 x 16 tile (tx, ty) of level L has unknowns (ptile); tiles outside do not.
+SP_DEV bool tile_on(const Ctx& c, const Lv& L, int tx, int ty) {
+    const int tw = (L.w + 15) >> 4, th = (L.h + 15) >> 4;
+    return tx >= 0 && ty >= 0 && tx < tw && ty < th && c.ptile[L.toff + ty * tw + tx] != 0u;
+}
+
+// The start of a solve (reduce_tail.glsl:59-75, mode 0; mode 8 for the second solve after
+// the viscosity, where the density-correction potential counts as solved): a = r.z (xy),
+// r.r (zw) of the first V-cycle, b = b.b (xy).
+SP_DEV sp_cg_state cg_start(const Ctx& c, const double4& a, const double4& b, bool second) {
+    const sp_gparams& P = *c.P;
+    const double tx = (double)P.tol[0] * (double)P.tol[0], ty = (double)P.tol[1] * (double)P.tol[1];
+    sp_cg_state s = {};
+    s.bb[0] = b.x;
+    s.bb[1] = b.y;
+    s.rz[0] = a.x;
+    s.rz[1] = a.y;
+    s.rr[0] = a.z;
+    s.rr[1] = a.w;
+    s.done[0] = a.z <= tx * b.x ? 1u : 0u;
+    s.done[1] = (second || a.w <= ty * b.y) ? 1u : 0u;
+    s.cap = second ? c.K->cap_p2 : c.K->cap_p;
+    return s;
+}
+
+// The end of an iteration (reduce_tail.glsl:77-104, mode 2): from the state e entering it and
+// a = r.z (xy), r.r (zw) of its up-sweep, the state entering the next one: the stop test
+// r.r <= tol^2 b.b per potential, beta, and the halt at this tank's recorded iterations.
+SP_DEV sp_cg_state cg_next(const Ctx& c, const sp_cg_state& e, const double4& a) {
+    const sp_gparams& P = *c.P;
+    const double tx = (double)P.tol[0] * (double)P.tol[0], ty = (double)P.tol[1] * (double)P.tol[1];
+    sp_cg_state s = e;
+    const uint32_t was0 = e.done[0], was1 = e.done[1];
+    s.iters[0] += 1u - was0;
+    s.iters[1] += 1u - was1;
+    const uint32_t d0 = was0 | (a.z <= tx * e.bb[0] ? 1u : 0u);
+    const uint32_t d1 = was1 | (a.w <= ty * e.bb[1] ? 1u : 0u);
+    s.beta[0] = (d0 == 0u && e.rz[0] > 0.0) ? a.x / e.rz[0] : 0.0;
+    s.beta[1] = (d1 == 0u && e.rz[1] > 0.0) ? a.y / e.rz[1] : 0.0;
+    s.rz[0] = a.x;
+    s.rz[1] = a.y;
+    if (!was0) s.rr[0] = a.z;
+    if (!was1) s.rr[1] = a.w;
+    s.done[0] = d0;
+    s.done[1] = d1;
+    s.sweeps += 1u;
+    if (s.cap != 0u && s.sweeps >= s.cap && (d0 == 0u || d1 == 0u)) s.halt = 1u;
+    return s;
+}
+
+// Index of global cell (x, y) in a staging array of side S holding the region that starts r
+// cells before (x0, y0).
+SP_DEV int sx_(int x, int y, int x0, int y0, int r, int S) { return (y - y0 + r) * S + (x - x0 + r); }
+
+SP_DEV float2 direction_of(float2 z, float2 po, float bx, float by) {
+    if (bx == 0.0f && by == 0.0f) return z;
+    return make_float2(bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y);
+}
+
+// ---- one level of the V-cycle on a T x T block: the arithmetic of mg_down.comp:36-69 and
+// mg_up.comp:26-75, operands from LDS ------------------------------------------------------
+
+// Down, from zero (mg_down.comp:36-60): z on the block plus one ring from the right-hand side
+// r and diagonal D on two rings (S2 = T + 4) and the couplings A on one (S1 = T + 2).
+template <int NT>
+SP_DEV void down_z(int x0, int y0, int S1, int S2, const float* sd, const float2* sr, const float4* sa, float2* sz) {
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int k = t; k < S1 * S1; k += NT) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
+        const int s2 = sx_(gx, gy, x0, y0, 2, S2);
+        const float D = sd[s2];
+        float2 z = make_float2(0.0f, 0.0f);
+        if (D > 0.0f) {
+            const float2 r = sr[s2];
+            if (((gx + gy) & 1) == 0) {
+                z = make_float2(r.x / D, r.y / D);
+            } else {
+                const float4 w = sa[k];
+                float dj[4];
+                float2 rj[4];
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const int sn = sx_(gx + NB_X[d], gy + NB_Y[d], x0, y0, 2, S2);
+                    dj[d] = sd[sn];
+                    rj[d] = sr[sn];
+                }
+                float2 sum = r;
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const float wd = f4c(w, d);
+                    const float2 cand = make_float2(sum.x + wd * rj[d].x / dj[d], sum.y + wd * rj[d].y / dj[d]);
+                    sum = dj[d] > 0.0f ? cand : sum;
+                }
+                z = make_float2(sum.x / D, sum.y / D);
+            }
+        }
+        sz[k] = z;
+    }
+}
+
+// The restriction of the residual left by down_z to coarse cell (cx, cy) (mg_down.comp:
+// 61-69): its red children's neighbours' z weighted by the couplings. (qx0, qy0): the fine
+// cell at index 0 of the ring-1 arrays (sz, sa); of the ring-2 array sd: (qx0 - 1, qy0 - 1).
+SP_DEV float2 restrict_red(int cx, int cy, int qx0, int qy0, int S1, int S2, const float* sd, const float4* sa, const float2* sz) {
+    float2 sum = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int qx = 2 * cx + k, qy = 2 * cy + k;
+        const bool okq = sd[(qy - qy0 + 1) * S2 + (qx - qx0 + 1)] > 0.0f;
+        const float4 w = sa[(qy - qy0) * S1 + (qx - qx0)];
+        bool ok[4];
+        float2 zn[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int nx = qx + NB_X[d], ny = qy + NB_Y[d];
+            ok[d] = okq && sd[(ny - qy0 + 1) * S2 + (nx - qx0 + 1)] > 0.0f;
+            zn[d] = sz[(ny - qy0) * S1 + (nx - qx0)];
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const float wd = f4c(w, d);
+            const float2 cand = make_float2(sum.x + wd * zn[d].x, sum.y + wd * zn[d].y);
+            sum = ok[d] ? cand : sum;
+        }
+    }
+    return sum;
+}
+
+// Up (mg_up.comp:26-36): the black cells' new values from the corrected iterate x + w z_parent
+// of their red neighbours, for the cells of the block plus `ring` rings. Region of the
+// arrays: sd, sx from (x0 - rx, y0 - rx), side SX; sa, sr from (x0 - ring, y0 - ring), side SR
+// (= the output's region); the parents' z from coarse cell (pcx, pcy), side SP.
+template <int NT>
+SP_DEV void up_black(int x0, int y0, int ring, int rx, int SX, int SR, int pcx, int pcy, int SP, const float* sd, const float2* sx,
+                     const float4* sa, const float2* sr, const float2* szc, float2* sb) {
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int k = t; k < SR * SR; k += NT) {
+        const int gx = x0 - ring + k % SR, gy = y0 - ring + k / SR;
+        float2 z = make_float2(0.0f, 0.0f);
+        const float D = sd[sx_(gx, gy, x0, y0, rx, SX)];
+        if (D > 0.0f && ((gx + gy) & 1) != 0) {
+            const float4 w = sa[k];
+            bool ok[4];
+            float2 xm[4], zc[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int mx = gx + NB_X[d], my = gy + NB_Y[d];
+                const int sm = sx_(mx, my, x0, y0, rx, SX);
+                ok[d] = sd[sm] > 0.0f;
+                xm[d] = sx[sm];
+                zc[d] = szc[((my >> 1) - pcy) * SP + ((mx >> 1) - pcx)];
+            }
+            float2 sum = sr[k];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const float wd = f4c(w, d);
+                const float2 cand = make_float2(sum.x + wd * (xm[d].x + SP_MG_OMEGA * zc[d].x),
+                                                sum.y + wd * (xm[d].y + SP_MG_OMEGA * zc[d].y));
+                sum = ok[d] ? cand : sum;
+            }
+            z = make_float2(sum.x / D, sum.y / D);
+        }
+        sb[k] = z;
+    }
+}
+
+// Up (mg_up.comp:37-75): z of cell (x, y) after the red half-sweep over the black values sb
+// (region from (x0 - rb, y0 - rb), side SB); sd from (x0 - rx, y0 - rx), side SX; sa, sr at
+// index k of a region from (x0 - rb, y0 - rb) of side SB as well.
+SP_DEV float2 up_red(int x, int y, int x0, int y0, int rb, int SB, int rx, int SX, const float* sd, const float4* sa, const float2* sr,
+                     const float2* sb, float2* rv_out) {
+    const float D = sd[sx_(x, y, x0, y0, rx, SX)];
+    const int s1 = sx_(x, y, x0, y0, rb, SB);
+    float2 z = make_float2(0.0f, 0.0f);
+    *rv_out = make_float2(0.0f, 0.0f);
+    if (!(D > 0.0f)) return z;
+    const float2 rv = sr[s1];
+    *rv_out = rv;
+    if (((x + y) & 1) == 0) {
+        const float4 w = sa[s1];
+        bool ok[4];
+        float2 zn[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int nx = x + NB_X[d], ny = y + NB_Y[d];
+            ok[d] = sd[sx_(nx, ny, x0, y0, rx, SX)] > 0.0f;
+            zn[d] = sb[sx_(nx, ny, x0, y0, rb, SB)];
+        }
+        float2 sum = rv;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const float wd = f4c(w, d);
+            const float2 cand = make_float2(sum.x + wd * zn[d].x, sum.y + wd * zn[d].y);
+            sum = ok[d] ? cand : sum;
+        }
+        z = make_float2(sum.x / D, sum.y / D);
+    } else {
+        z = sb[s1];
+    }
+    return z;
+}
+
+}  // namespace
+
+// ---- Krylov kernels on level 0 -----------------------------------------------------------
+
+// cg_init.comp: r = b - A x and the partial sums of b.b.
+__global__ __launch_bounds__(NT0) void k_cg_init(const TankBatch tb, Lv F, int nbp) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    __shared__ double4 lds[4];
-    const int slot = tile_slot();
-    if (c.ptile[slot] == 0u) {
-        if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
+    __shared__ double4 lds[NT0 / 64];
+    __shared__ uint32_t s_on[TT * TT];
+    const int t = tid0();
+    const int2 tq = tile_xy();
+    const int x0 = tq.x * T0, y0 = tq.y * T0;
+    if (t < TT * TT) s_on[t] = tile_on(c, F, (x0 >> 4) + t % TT, (y0 >> 4) + t / TT) ? 1u : 0u;
+    __syncthreads();
+    if (!any_on(s_on)) {
+        if (t == 0) store_part(&c.part_sums[PART_BB * nbp + slot0()], make_double4(0, 0, 0, 0));
         return;
     }
-    const Lv F = level(c, 0);
-    int x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     double4 acc = make_double4(0, 0, 0, 0);
-    if (inside(F, x, y)) {
-        int i = at(F, x, y);
-        float D = c.mgd[i];
+    if (inside(F, x, y) && s_on[(threadIdx.y >> 4) * TT + (threadIdx.x >> 4)]) {
+        const int i = at(F, x, y);
+        const float D = c.mgd[i];
         if (D > 0.0f) {
-            float2 xv = c.pres[i];
-            float4 w = c.mga[i];
+            const float2 xv = c.pres[i];
+            const float4 w = c.mga[i];
             float2 ax = make_float2(D * xv.x, D * xv.y);
             for (int d = 0; d < 4; d++) {
-                int nx = x + NB_X[d], ny = y + NB_Y[d];
+                const int nx = x + NB_X[d], ny = y + NB_Y[d];
                 if (unknown(c, F, nx, ny)) {
-                    float wd = f4c(w, d);
-                    float2 pn = c.pres[at(F, nx, ny)];
+                    const float wd = f4c(w, d);
+                    const float2 pn = c.pres[at(F, nx, ny)];
                     ax.x -= wd * pn.x;
                     ax.y -= wd * pn.y;
                 }
             }
-            float4 b = c.prhs[i];
+            const float4 b = c.prhs[i];
             c.mgr[i] = make_float2(b.x - ax.x, b.y - ax.y);
             acc.x = (double)b.x * (double)b.x;
             acc.y = (double)b.y * (double)b.y;
@@ -365,64 +579,67 @@ __global__ __launch_bounds__(256) void k_cg_init(const TankBatch tb, int nblocks
             c.mgr[i] = make_float2(0.0f, 0.0f);
         }
     }
-    double4 s = group_sum4(acc, lds);
-    if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], s);
+    const double4 s = group_sum4<NT0>(acc, lds);
+    if (t == 0) store_part(&c.part_sums[PART_BB * nbp + slot0()], s);
 }
 
-// ---- LDS-staged level kernels ----------------------------------------------------------
-//
-// Each work-group owns a 16 x 16 tile of its level. It first loads what its stencils reach
-// (the tile plus a ring of one or two cells: diagonals, couplings, right-hand sides,
-// iterates) into LDS with every work-item issuing its loads at once, then computes from
-// LDS. The arithmetic per cell, and its order, is that of the reference's kernels (which
-// read neighbours from global memory and recompute values they do not hold); only where
-// the operands come from changes, so results are bit-identical.
-
-namespace {
-
-constexpr int R1 = 18;  // tile + 1-cell ring
-constexpr int R2 = 20;  // tile + 2-cell ring
-
-// Index of global cell (x, y) in a ring-r staging array of the tile at (x0, y0).
-SP_DEV int sidx(int x, int y, int x0, int y0, int r) { return (y - y0 + r) * (16 + 2 * r) + (x - x0 + r); }
-
-SP_DEV float2 direction_of(float2 z, float2 po, float bx, float by) {
-    if (bx == 0.0f && by == 0.0f) return z;
-    return make_float2(bx != 0.0f ? z.x + bx * po.x : z.x, by != 0.0f ? z.y + by * po.y : z.y);
-}
-
-}  // namespace
-
-// cg_apply.comp: p = z + beta p_old, q = A p, and the partial sums of p.q (second half of
-// the partial buffer).
-template <bool H>
-__global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblocks, int par) {
+// The scalar step that starts iteration k (par = k & 1), one work-group per tank: the
+// solve's start (reduce_tail.glsl:59-75, mode 0; mode 8 for the second solve) for k = 0 from
+// b.b and the first V-cycle's r.z, r.r, otherwise the end of iteration k - 1 (mode 2) from its
+// r.z, r.r partial sums; the state goes into slot par. A solve that is over passes its state on.
+__global__ __launch_bounds__(REDUCE_NT) void k_cg_step(const TankBatch tb, int nbp, int par, int first, int second) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    // the search direction alternates between cgp and cgp2 with the iteration's parity
-    const float2* __restrict__ p_old = par ? c.cgp2 : c.cgp;
-    float2* __restrict__ p_new = par ? c.cgp : c.cgp2;
-    __shared__ double4 lds[4];
-    __shared__ float sd[R1 * R1];
-    __shared__ float2 sp[R1 * R1];
-    if (solve_done(c)) return;
-    const int slot = tile_slot();
-    if (c.ptile[slot] == 0u) {
-        if (tid2() == 0) store_part(&c.part_sums[nblocks + slot], make_double4(0, 0, 0, 0));
+    __shared__ double4 lds[REDUCE_NT / 64];
+    const sp_cg_state* P = &c.K->st[par ^ 1];
+    if (!first && st_done(*P)) {
+        if (threadIdx.x == 0) c.K->st[par] = *P;
         return;
     }
-    const Lv F = level(c, 0);
-    const float bx = (float)c.K->beta[0], by = (float)c.K->beta[1];
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    const int t = tid2();
-    for (int k = t; k < R1 * R1; k += 256) {
-        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
+    const double4 a = total_of<REDUCE_NT>(c.part_sums, PART_RZ * nbp, nbp, lds);
+    double4 b = make_double4(0, 0, 0, 0);
+    if (first) b = total_of<REDUCE_NT>(c.part_sums, PART_BB * nbp, nbp, lds);
+    if (threadIdx.x != 0) return;
+    if (first) {
+        c.K->st[par] = cg_start(c, a, b, second != 0);
+        c.K->solves += 1u;
+    } else {
+        c.K->st[par] = cg_next(c, *P, a);
+    }
+}
+
+// cg_apply.comp:23-51: p = z + beta p_old, q = A p and the partial sums of p.q. par = k & 1;
+// the direction alternates between cgp and cgp2 with it.
+template <bool H>
+__global__ __launch_bounds__(NT0) void k_cg_apply(const TankBatch tb, Lv F, int nbp, int par) {
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ p_old = par ? c.cgp2 : c.cgp;
+    float2* __restrict__ p_new = par ? c.cgp : c.cgp2;
+    constexpr int S1 = T0 + 2;
+    __shared__ double4 lds[NT0 / 64];
+    __shared__ float sd[S1 * S1];
+    __shared__ float2 sp[S1 * S1];
+    __shared__ uint32_t s_on[TT * TT];
+    const int t = tid0();
+    const int2 tq = tile_xy();
+    const int x0 = tq.x * T0, y0 = tq.y * T0;
+    if (t < TT * TT) s_on[t] = tile_on(c, F, (x0 >> 4) + t % TT, (y0 >> 4) + t / TT) ? 1u : 0u;
+    const sp_cg_state* E = &c.K->st[par];
+    if (st_done(*E)) return;
+    __syncthreads();
+    if (!any_on(s_on)) {
+        if (t == 0) store_part(&c.part_sums[PART_PQ * nbp + slot0()], make_double4(0, 0, 0, 0));
+        return;
+    }
+    const float bx = (float)E->beta[0], by = (float)E->beta[1];
+    for (int k = t; k < S1 * S1; k += NT0) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
         float d = 0.0f;
         float2 pv = make_float2(0.0f, 0.0f);
         if (inside(F, gx, gy)) {
-            int i = at(F, gx, gy);
+            const int i = at(F, gx, gy);
             d = c.mgd[i];
             if (d > 0.0f) {
-                float2 po = (bx == 0.0f && by == 0.0f) ? make_float2(0.0f, 0.0f) : p_old[i];
+                const float2 po = (bx == 0.0f && by == 0.0f) ? make_float2(0.0f, 0.0f) : p_old[i];
                 pv = direction_of(MG<H>::Z(c, i), po, bx, by);
             }
         }
@@ -432,19 +649,19 @@ __global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblock
     __syncthreads();
     const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     double4 acc = make_double4(0, 0, 0, 0);
-    if (inside(F, x, y)) {
-        int i = at(F, x, y);
-        int si = sidx(x, y, x0, y0, 1);
-        float D = sd[si];
+    if (inside(F, x, y) && s_on[(threadIdx.y >> 4) * TT + (threadIdx.x >> 4)]) {
+        const int i = at(F, x, y);
+        const int si = sx_(x, y, x0, y0, 1, S1);
+        const float D = sd[si];
         if (D > 0.0f) {
-            float2 p = sp[si];
-            float4 w = c.mga[i];
+            const float2 p = sp[si];
+            const float4 w = c.mga[i];
             float2 q = make_float2(D * p.x, D * p.y);
             for (int d = 0; d < 4; d++) {
-                int sn = sidx(x + NB_X[d], y + NB_Y[d], x0, y0, 1);
+                const int sn = sx_(x + NB_X[d], y + NB_Y[d], x0, y0, 1, S1);
                 if (sd[sn] > 0.0f) {
-                    float wd = f4c(w, d);
-                    float2 pn = sp[sn];
+                    const float wd = f4c(w, d);
+                    const float2 pn = sp[sn];
                     q.x -= wd * pn.x;
                     q.y -= wd * pn.y;
                 }
@@ -457,263 +674,423 @@ __global__ __launch_bounds__(256) void k_cg_apply(const TankBatch tb, int nblock
             p_new[i] = make_float2(0.0f, 0.0f);
         }
     }
-    double4 s = group_sum4(acc, lds);
-    if (t == 0) store_part(&c.part_sums[nblocks + slot], s);
+    const double4 s = group_sum4<NT0>(acc, lds);
+    if (t == 0) store_part(&c.part_sums[PART_PQ * nbp + slot0()], s);
 }
 
-namespace {
-
-// The way down on one level (mg_down.comp:36-69) over a tile whose residual for the tile
-// plus two rings is in sr (ring 2) and diagonals in sd (ring 2): z of the tile and its first
-// ring, then the restriction of the red cells' residual to the coarse level.
-struct DownStage {
-    float sd[R2 * R2];
-    float2 sr[R2 * R2];
-    float4 sa[R1 * R1];
-    float2 sz[R1 * R1];
-};
-
-template <bool H>
-SP_DEV void down_compute(const Ctx& c, const Lv& F, const Lv& C, int x0, int y0, DownStage& S) {
-    const int t = tid2();
-    // z after the red half-sweep from zero: red z = r / D; black from its red neighbours.
-    for (int k = t; k < R1 * R1; k += 256) {
-        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
-        int s2 = sidx(gx, gy, x0, y0, 2);
-        float D = S.sd[s2];
-        float2 z = make_float2(0.0f, 0.0f);
-        if (D > 0.0f) {
-            float2 r = S.sr[s2];
-            if (((gx + gy) & 1) == 0) {
-                z = make_float2(r.x / D, r.y / D);
-            } else {
-                float4 w = S.sa[k];
-                float2 sum = r;
-                for (int d = 0; d < 4; d++) {
-                    int sn = sidx(gx + NB_X[d], gy + NB_Y[d], x0, y0, 2);
-                    float dj = S.sd[sn];
-                    if (!(dj > 0.0f)) continue;
-                    float wd = f4c(w, d);
-                    float2 rj = S.sr[sn];
-                    sum.x += wd * rj.x / dj;
-                    sum.y += wd * rj.y / dj;
-                }
-                z = make_float2(sum.x / D, sum.y / D);
-            }
-        }
-        S.sz[k] = z;
-    }
-    __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (inside(F, x, y)) MG<H>::set_X(c, at(F, x, y), S.sz[sidx(x, y, x0, y0, 1)]);
-    if (((threadIdx.x | threadIdx.y) & 1) != 0) return;
-    const int cx = x >> 1, cy = y >> 1;
-    if (!inside(C, cx, cy)) return;
-    float2 sum = make_float2(0.0f, 0.0f);
-    for (int k = 0; k < 2; k++) {
-        int qx = x + k, qy = y + k;
-        int sq = sidx(qx, qy, x0, y0, 1);
-        if (!(S.sd[sidx(qx, qy, x0, y0, 2)] > 0.0f)) continue;
-        float4 w = S.sa[sq];
-        for (int d = 0; d < 4; d++) {
-            int nx = qx + NB_X[d], ny = qy + NB_Y[d];
-            if (!(S.sd[sidx(nx, ny, x0, y0, 2)] > 0.0f)) continue;
-            float2 zn = S.sz[sidx(nx, ny, x0, y0, 1)];
-            float wd = f4c(w, d);
-            sum.x += wd * zn.x;
-            sum.y += wd * zn.y;
-        }
-    }
-    MG<H>::set_R(c, at(C, cx, cy), sum);
-}
-
-// Loads the diagonals (ring 2) and couplings (ring 1) of the tile of level l.
-template <bool H>
-SP_DEV void stage_matrix(const Ctx& c, const Lv& F, int l, int x0, int y0, DownStage& S) {
-    const int t = tid2();
-    for (int k = t; k < R2 * R2; k += 256) {
-        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
-        S.sd[k] = inside(F, gx, gy) ? MG<H>::D(c, l, at(F, gx, gy)) : 0.0f;
-    }
-    for (int k = t; k < R1 * R1; k += 256) {
-        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
-        S.sa[k] = inside(F, gx, gy) ? MG<H>::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
-    }
-}
-
-}  // namespace
-
-// mg_down.comp:36-69 on level l: pre-smoothing from zero and restriction of the residual.
-// r_in: the level's right-hand side (level 0: the CG residual buffer of this iteration).
-template <bool H>
-__global__ __launch_bounds__(256) void k_mg_down(const TankBatch tb, int l, int gated, int rsel) {
+// cg_apply's scalar step (reduce_tail.glsl mode 1), one work-group per tank: alpha = rz / p.q
+// for each potential still running, into the iteration's state.
+__global__ __launch_bounds__(REDUCE_NT) void k_cg_alpha(const TankBatch tb, int nbp, int par) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
-    __shared__ DownStage S;
-    if (gated && solve_done(c)) return;
-    const Lv F = level(c, l), C = level(c, l + 1);
-    if (c.ptile[F.toff + tile_slot()] == 0u) return;
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    stage_matrix<H>(c, F, l, x0, y0, S);
-    for (int k = tid2(); k < R2 * R2; k += 256) {
-        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
-        float2 rv = make_float2(0.0f, 0.0f);
-        if (inside(F, gx, gy)) rv = l == 0 ? r_in[at(F, gx, gy)] : MG<H>::R(c, at(F, gx, gy));
-        S.sr[k] = rv;
-    }
-    __syncthreads();
-    down_compute<H>(c, F, C, x0, y0, S);
+    __shared__ double4 lds[REDUCE_NT / 64];
+    sp_cg_state* E = &c.K->st[par];
+    if (st_done(*E)) return;
+    const double4 pq = total_of<REDUCE_NT>(c.part_sums, PART_PQ * nbp, nbp, lds);
+    if (threadIdx.x != 0) return;
+    E->alpha[0] = (E->done[0] == 0u && pq.x > 0.0) ? E->rz[0] / pq.x : 0.0;
+    E->alpha[1] = (E->done[1] == 0u && pq.y > 0.0) ? E->rz[1] / pq.y : 0.0;
 }
 
-// cg_update.comp fused with the way down on level 0: x += alpha p, r_out = r_in - alpha q
-// (residual buffers alternate between iterations, so a tile may read its neighbours' old
-// residual while they write their new one), then mg_down level 0 on r_out.
+// cg_update.comp fused with the way down on level 0: x += alpha p, r_out = r_in - alpha q (residual buffers alternate with
+// the iteration's parity, so a block may read its neighbours' old residual while they write
+// their new one), then mg_down on level 0 from r_out: the pre-smoothed iterate and the
+// level-1 right-hand side.
 template <bool H>
-__global__ __launch_bounds__(256) void k_update_down0(const TankBatch tb, int par) {
+__global__ __launch_bounds__(NT0) void k_update_down0(const TankBatch tb, Lv F, Lv C, int par) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    // residual (mgr / r2) and direction (cgp / cgp2) buffers of iteration parity `par`
     const float2* __restrict__ r_in = par ? c.r2 : c.mgr;
     float2* __restrict__ r_out = par ? c.mgr : c.r2;
     const float2* __restrict__ p_new = par ? c.cgp : c.cgp2;
-    __shared__ DownStage S;
-    if (solve_done(c)) return;
-    const Lv F = level(c, 0), C = level(c, 1);
-    if (c.ptile[tile_slot()] == 0u) return;
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    const float ax = (float)c.K->alpha[0], ay = (float)c.K->alpha[1];
-    stage_matrix<H>(c, F, 0, x0, y0, S);
-    for (int k = tid2(); k < R2 * R2; k += 256) {
-        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
+    constexpr int S1 = T0 + 2, S2 = T0 + 4;
+    __shared__ float sd[S2 * S2];
+    __shared__ float2 sr[S2 * S2];
+    __shared__ float4 sa[S1 * S1];
+    __shared__ float2 sz[S1 * S1];
+    __shared__ double4 lds[NT0 / 64];
+    __shared__ uint32_t s_on[TT * TT];
+    const int t = tid0();
+    const int2 tq = tile_xy();
+    const int x0 = tq.x * T0, y0 = tq.y * T0;
+    if (t < TT * TT) s_on[t] = tile_on(c, F, (x0 >> 4) + t % TT, (y0 >> 4) + t / TT) ? 1u : 0u;
+    const sp_cg_state* E = &c.K->st[par];
+    if (st_done(*E)) return;
+    __syncthreads();
+    if (!any_on(s_on)) return;
+    const float ax = (float)E->alpha[0], ay = (float)E->alpha[1];
+    for (int k = t; k < S2 * S2; k += NT0) {
+        const int gx = x0 - 2 + k % S2, gy = y0 - 2 + k / S2;
+        float d = 0.0f;
         float2 rv = make_float2(0.0f, 0.0f);
         if (inside(F, gx, gy)) {
-            int i = at(F, gx, gy);
+            const int i = at(F, gx, gy);
+            d = c.mgd[i];
             rv = r_in[i];
-            if (c.mgd[i] > 0.0f) {
-                float2 q = c.cgq[i];
+            if (d > 0.0f) {
+                const float2 q = c.cgq[i];
                 rv.x -= ax * q.x;
                 rv.y -= ay * q.y;
             }
         }
-        S.sr[k] = rv;
+        sd[k] = d;
+        sr[k] = rv;
     }
-    {
-        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-        if (inside(F, x, y)) {
-            int i = at(F, x, y);
-            if (c.mgd[i] > 0.0f) {
-                float2 p = p_new[i];
-                float2 x1 = c.pres[i];
-                x1.x += ax * p.x;
-                x1.y += ay * p.y;
-                c.pres[i] = x1;
-            }
+    for (int k = t; k < S1 * S1; k += NT0) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
+        sa[k] = inside(F, gx, gy) ? c.mga[at(F, gx, gy)] : make_float4(0, 0, 0, 0);
+    }
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    const bool mine = inside(F, x, y) && s_on[(threadIdx.y >> 4) * TT + (threadIdx.x >> 4)] != 0u;
+    if (mine) {
+        const int i = at(F, x, y);
+        if (c.mgd[i] > 0.0f) {
+            const float2 p = p_new[i];
+            float2 x1 = c.pres[i];
+            x1.x += ax * p.x;
+            x1.y += ay * p.y;
+            c.pres[i] = x1;
         }
     }
     __syncthreads();
-    {
-        const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-        if (inside(F, x, y)) r_out[at(F, x, y)] = S.sr[sidx(x, y, x0, y0, 2)];
-    }
-    down_compute<H>(c, F, C, x0, y0, S);
+    if (mine) r_out[at(F, x, y)] = sr[sx_(x, y, x0, y0, 2, S2)];
+    down_z<NT0>(x0, y0, S1, S2, sd, sr, sa, sz);
+    __syncthreads();
+    if (mine) MG<H>::set_X(c, at(F, x, y), sz[sx_(x, y, x0, y0, 1, S1)]);
+    if (!mine || ((threadIdx.x | threadIdx.y) & 1) != 0) return;
+    const int cx = x >> 1, cy = y >> 1;
+    if (!inside(C, cx, cy)) return;
+    MG<H>::set_R(c, at(C, cx, cy), restrict_red(cx, cy, x0 - 1, y0 - 1, S1, S2, sd, sa, sz));
 }
 
-// mg_up.comp:38-75 on level l: coarse-grid correction and the black-then-red
-// post-smoothing; on level 0 (reduce >= 0) also the partial sums r.z, r.r.
+// mg_up.comp:38-75 on level 0: the coarse-grid correction and the black-then-red
+// post-smoothing; with `reduce` also the partial sums r.z, r.r. r_in: the level-0 residual
+// of this iteration (mgr, or r2 with rsel).
 template <bool H>
-__global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int gated, int reduce, int nblocks, int rsel) {
+__global__ __launch_bounds__(NT0) void k_up0(const TankBatch tb, Lv F, Lv C, int nbp, int gated, int par, int reduce, int rsel) {
     using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
-    __shared__ double4 lds[4];
-    __shared__ float sd[R2 * R2];
-    __shared__ float2 sx[R2 * R2];
-    __shared__ float4 sa[R1 * R1];
-    __shared__ float2 sr[R1 * R1];
-    __shared__ float2 szc[10 * 10];
-    __shared__ float2 sb[R1 * R1];
-    if (gated && solve_done(c)) return;
-    const Lv F = level(c, l), C = level(c, l + 1);
-    const int slot = tile_slot();
-    if (c.ptile[F.toff + slot] == 0u) {
-        if (reduce >= 0 && tid2() == 0) store_part(&c.part_sums[slot], make_double4(0, 0, 0, 0));
+    constexpr int S1 = T0 + 2, S2 = T0 + 4, SP = T0 / 2 + 2;
+    __shared__ float sd[S2 * S2];
+    __shared__ float2 sx[S2 * S2];
+    __shared__ float4 sa[S1 * S1];
+    __shared__ float2 sr[S1 * S1];
+    __shared__ float2 szc[SP * SP];
+    __shared__ float2 sb[S1 * S1];
+    __shared__ double4 lds[NT0 / 64];
+    __shared__ uint32_t s_on[TT * TT];
+    const int t = tid0();
+    const int2 tq = tile_xy();
+    const int x0 = tq.x * T0, y0 = tq.y * T0;
+    if (t < TT * TT) s_on[t] = tile_on(c, F, (x0 >> 4) + t % TT, (y0 >> 4) + t / TT) ? 1u : 0u;
+    if (gated && slot_done(c, par)) return;
+    __syncthreads();
+    if (!any_on(s_on)) {
+        if (reduce && t == 0) store_part(&c.part_sums[PART_RZ * nbp + slot0()], make_double4(0, 0, 0, 0));
         return;
     }
-    const int t = tid2();
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    for (int k = t; k < R2 * R2; k += 256) {
-        int gx = x0 - 2 + k % R2, gy = y0 - 2 + k / R2;
-        bool in = inside(F, gx, gy);
-        sd[k] = in ? M::D(c, l, at(F, gx, gy)) : 0.0f;
+    for (int k = t; k < S2 * S2; k += NT0) {
+        const int gx = x0 - 2 + k % S2, gy = y0 - 2 + k / S2;
+        const bool in = inside(F, gx, gy);
+        sd[k] = in ? M::D(c, 0, at(F, gx, gy)) : 0.0f;
         sx[k] = in ? M::X(c, at(F, gx, gy)) : make_float2(0.0f, 0.0f);
     }
-    for (int k = t; k < R1 * R1; k += 256) {
-        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
-        bool in = inside(F, gx, gy);
-        sa[k] = in ? M::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
-        sr[k] = in ? (l == 0 ? r_in[at(F, gx, gy)] : M::R(c, at(F, gx, gy))) : make_float2(0.0f, 0.0f);
+    for (int k = t; k < S1 * S1; k += NT0) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
+        const bool in = inside(F, gx, gy);
+        sa[k] = in ? M::A(c, 0, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
+        sr[k] = in ? r_in[at(F, gx, gy)] : make_float2(0.0f, 0.0f);
     }
-    // parents of the ring-2 region: coarse cells x0/2 - 1 .. x0/2 + 8
+    // parents of the ring-2 region: coarse cells x0/2 - 1 .. x0/2 + T0/2
     const int cx0 = (x0 >> 1) - 1, cy0 = (y0 >> 1) - 1;
-    for (int k = t; k < 100; k += 256) {
-        int gx = cx0 + k % 10, gy = cy0 + k / 10;
+    for (int k = t; k < SP * SP; k += NT0) {
+        const int gx = cx0 + k % SP, gy = cy0 + k / SP;
         szc[k] = inside(C, gx, gy) ? M::Z(c, at(C, gx, gy)) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    // new values of the black cells of the tile and its first ring (mg_up.comp:26-36)
-    for (int k = t; k < R1 * R1; k += 256) {
-        int gx = x0 - 1 + k % R1, gy = y0 - 1 + k / R1;
-        float2 z = make_float2(0.0f, 0.0f);
-        float D = sd[sidx(gx, gy, x0, y0, 2)];
-        if (D > 0.0f && ((gx + gy) & 1) != 0) {
-            float4 w = sa[k];
-            float2 sum = sr[k];
-            for (int d = 0; d < 4; d++) {
-                int mx = gx + NB_X[d], my = gy + NB_Y[d];
-                int sm = sidx(mx, my, x0, y0, 2);
-                if (!(sd[sm] > 0.0f)) continue;
-                float wd = f4c(w, d);
-                float2 xm = sx[sm];
-                float2 zc = szc[((my >> 1) - cy0) * 10 + ((mx >> 1) - cx0)];
-                sum.x += wd * (xm.x + SP_MG_OMEGA * zc.x);
-                sum.y += wd * (xm.y + SP_MG_OMEGA * zc.y);
-            }
-            z = make_float2(sum.x / D, sum.y / D);
-        }
-        sb[k] = z;
-    }
+    up_black<NT0>(x0, y0, 1, 2, S2, S1, cx0, cy0, SP, sd, sx, sa, sr, szc, sb);
     __syncthreads();
     const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     double4 acc = make_double4(0, 0, 0, 0);
-    if (inside(F, x, y)) {
-        int i = at(F, x, y);
-        int s1 = sidx(x, y, x0, y0, 1);
-        float D = sd[sidx(x, y, x0, y0, 2)];
-        float2 z = make_float2(0.0f, 0.0f);
-        if (D > 0.0f) {
-            float2 rv = sr[s1];
-            if (((x + y) & 1) == 0) {
-                float4 w = sa[s1];
-                float2 sum = rv;
-                for (int d = 0; d < 4; d++) {
-                    int nx = x + NB_X[d], ny = y + NB_Y[d];
-                    if (!(sd[sidx(nx, ny, x0, y0, 2)] > 0.0f)) continue;
-                    float wd = f4c(w, d);
-                    float2 zn = sb[sidx(nx, ny, x0, y0, 1)];
-                    sum.x += wd * zn.x;
-                    sum.y += wd * zn.y;
-                }
-                z = make_float2(sum.x / D, sum.y / D);
-            } else {
-                z = sb[s1];
-            }
+    if (inside(F, x, y) && s_on[(threadIdx.y >> 4) * TT + (threadIdx.x >> 4)]) {
+        float2 rv;
+        const float2 z = up_red(x, y, x0, y0, 1, S1, 2, S2, sd, sa, sr, sb, &rv);
+        if (sd[sx_(x, y, x0, y0, 2, S2)] > 0.0f)
             acc = make_double4((double)rv.x * (double)z.x, (double)rv.y * (double)z.y, (double)rv.x * (double)rv.x,
                                (double)rv.y * (double)rv.y);
-        }
-        M::set_Z(c, i, z);
+        M::set_Z(c, at(F, x, y), z);
     }
-    if (reduce >= 0) {
-        double4 s2 = group_sum4(acc, lds);
-        if (t == 0) store_part(&c.part_sums[slot], s2);
+    if (reduce) {
+        const double4 s2 = group_sum4<NT0>(acc, lds);
+        if (t == 0) store_part(&c.part_sums[PART_RZ * nbp + slot0()], s2);
+    }
+}
+
+// ---- coarse levels, one per launch (16 x 16 tiles) --------------------------------------
+
+// mg_down.comp:36-69 on level l (F; its coarse level C): pre-smoothing from zero and the
+// restriction of the residual. Level 0 (the first V-cycle of a solve) reads the CG residual
+// (mgr, or r2 with rsel).
+template <bool H>
+__global__ __launch_bounds__(256) void k_mg_down(const TankBatch tb, Lv F, Lv C, int l, int gated, int par, int rsel) {
+    using M = MG<H>;
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    const float2* __restrict__ r_in = rsel ? c.r2 : c.mgr;
+    constexpr int S1 = 18, S2 = 20;
+    __shared__ float sd[S2 * S2];
+    __shared__ float2 sr[S2 * S2];
+    __shared__ float4 sa[S1 * S1];
+    __shared__ float2 sz[S1 * S1];
+    if (gated && slot_done(c, par)) return;
+    const int2 tq = tile_xy();
+    if (c.ptile[F.toff + slot_of(tq)] == 0u) return;
+    const int x0 = tq.x * 16, y0 = tq.y * 16;
+    const int t = tid2();
+    for (int k = t; k < S2 * S2; k += 256) {
+        const int gx = x0 - 2 + k % S2, gy = y0 - 2 + k / S2;
+        const bool in = inside(F, gx, gy);
+        sd[k] = in ? M::D(c, l, at(F, gx, gy)) : 0.0f;
+        sr[k] = in ? (l == 0 ? r_in[at(F, gx, gy)] : M::R(c, at(F, gx, gy))) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < S1 * S1; k += 256) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
+        sa[k] = inside(F, gx, gy) ? M::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    down_z<256>(x0, y0, S1, S2, sd, sr, sa, sz);
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (inside(F, x, y)) M::set_X(c, at(F, x, y), sz[sx_(x, y, x0, y0, 1, S1)]);
+    if (((threadIdx.x | threadIdx.y) & 1) != 0) return;
+    const int cx = x >> 1, cy = y >> 1;
+    if (!inside(C, cx, cy)) return;
+    M::set_R(c, at(C, cx, cy), restrict_red(cx, cy, x0 - 1, y0 - 1, S1, S2, sd, sa, sz));
+}
+
+// mg_up.comp:38-75 on coarse level l >= 1 (F; its coarse level C).
+template <bool H>
+__global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, Lv F, Lv C, int l, int gated, int par) {
+    using M = MG<H>;
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    constexpr int S1 = 18, S2 = 20, SP = 10;
+    __shared__ float sd[S2 * S2];
+    __shared__ float2 sx[S2 * S2];
+    __shared__ float4 sa[S1 * S1];
+    __shared__ float2 sr[S1 * S1];
+    __shared__ float2 szc[SP * SP];
+    __shared__ float2 sb[S1 * S1];
+    if (gated && slot_done(c, par)) return;
+    const int2 tq = tile_xy();
+    if (c.ptile[F.toff + slot_of(tq)] == 0u) return;
+    const int t = tid2();
+    const int x0 = tq.x * 16, y0 = tq.y * 16;
+    for (int k = t; k < S2 * S2; k += 256) {
+        const int gx = x0 - 2 + k % S2, gy = y0 - 2 + k / S2;
+        const bool in = inside(F, gx, gy);
+        sd[k] = in ? M::D(c, l, at(F, gx, gy)) : 0.0f;
+        sx[k] = in ? M::X(c, at(F, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < S1 * S1; k += 256) {
+        const int gx = x0 - 1 + k % S1, gy = y0 - 1 + k / S1;
+        const bool in = inside(F, gx, gy);
+        sa[k] = in ? M::A(c, l, at(F, gx, gy)) : make_float4(0, 0, 0, 0);
+        sr[k] = in ? M::R(c, at(F, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    const int cx0 = (x0 >> 1) - 1, cy0 = (y0 >> 1) - 1;
+    for (int k = t; k < SP * SP; k += 256) {
+        const int gx = cx0 + k % SP, gy = cy0 + k / SP;
+        szc[k] = inside(C, gx, gy) ? M::Z(c, at(C, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    up_black<256>(x0, y0, 1, 2, S2, S1, cx0, cy0, SP, sd, sx, sa, sr, szc, sb);
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (inside(F, x, y)) {
+        float2 rv;
+        M::set_Z(c, at(F, x, y), up_red(x, y, x0, y0, 1, S1, 2, S2, sd, sa, sr, sb, &rv));
+    }
+}
+
+// ---- coarse levels two per launch ------------------------------------------------------
+//
+// A pair of adjacent coarse levels a = l and b = l + 1 in one launch: each work-group owns a
+// 16 x 16 tile of level b and the 32 x 32 cells of level a under it, and works out the
+// rings of both levels its stencils reach itself (the cells its neighbours also work out),
+// so that no level waits for another launch. Every cell's arithmetic, and its order, is that
+// of one level per launch; where a level's values pass through storage between the two
+// launches (the right-hand side of b on the way down, z of b on the way up), they are
+// rounded here as storage rounds them. So results are bit-identical to one level per launch.
+
+namespace {
+
+constexpr int NTF = 1024;  // work-items of a two-level kernel (32 x 32)
+
+// Way down: level a's diagonal and right-hand side on 6 rings, couplings and z on 5; level
+// b's diagonal and right-hand side on 2, couplings and z on 1.
+constexpr int D2_AD = 44, D2_AA = 42, D2_BD = 20, D2_BA = 18;
+// Way up: level b's diagonal and iterate on 3 rings, couplings, right-hand side and black
+// values on 2, z on 1, its parents on 12 x 12; level a's diagonal and iterate on 2,
+// couplings, right-hand side and black values on 1.
+constexpr int U2_BX = 22, U2_BR = 20, U2_BZ = 18, U2_BP = 12, U2_AX = 36, U2_AR = 34;
+
+}  // namespace
+
+constexpr size_t DOWN2_LDS = sizeof(float4) * (D2_AA * D2_AA + D2_BA * D2_BA) +
+                             sizeof(float2) * (D2_AD * D2_AD + D2_AA * D2_AA + D2_BD * D2_BD + D2_BA * D2_BA) +
+                             sizeof(float) * (D2_AD * D2_AD + D2_BD * D2_BD);
+constexpr size_t UP2_LDS = sizeof(float4) * (U2_BR * U2_BR + U2_AR * U2_AR) +
+                           sizeof(float2) * (U2_BX * U2_BX + 2 * U2_BR * U2_BR + U2_BP * U2_BP + U2_BZ * U2_BZ +
+                                             U2_AX * U2_AX + 2 * U2_AR * U2_AR) +
+                           sizeof(float) * (U2_BX * U2_BX + U2_AX * U2_AX);
+
+// mg_down.comp on levels a = l and b = l + 1 (A, B; B's coarse level Cc).
+template <bool H>
+__global__ __launch_bounds__(NTF) void k_mg_down2(const TankBatch tb, Lv A, Lv B, Lv Cc, int la, int gated, int par) {
+    using M = MG<H>;
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    extern __shared__ float4 dyn_lds[];
+    float4* aA = dyn_lds;
+    float4* bA = aA + D2_AA * D2_AA;
+    float2* ar = reinterpret_cast<float2*>(bA + D2_BA * D2_BA);
+    float2* az = ar + D2_AD * D2_AD;
+    float2* bR = az + D2_AA * D2_AA;
+    float2* bz = bR + D2_BD * D2_BD;
+    float* aD = reinterpret_cast<float*>(bz + D2_BA * D2_BA);
+    float* bD = aD + D2_AD * D2_AD;
+    __shared__ uint32_t s_on[4];  // level a's four tiles under this block
+    const int lb = la + 1;
+    if (gated && slot_done(c, par)) return;
+    const int2 tq = tile_xy();
+    if (c.ptile[B.toff + slot_of(tq)] == 0u) return;
+    const int t = threadIdx.y * 32 + threadIdx.x;
+    const int X0 = tq.x * 16, Y0 = tq.y * 16, ax0 = 2 * X0, ay0 = 2 * Y0;
+    if (t < 4) s_on[t] = tile_on(c, A, (ax0 >> 4) + (t & 1), (ay0 >> 4) + (t >> 1)) ? 1u : 0u;
+    for (int k = t; k < D2_AD * D2_AD; k += NTF) {
+        const int gx = ax0 - 6 + k % D2_AD, gy = ay0 - 6 + k / D2_AD;
+        const bool in = inside(A, gx, gy);
+        aD[k] = in ? M::D(c, la, at(A, gx, gy)) : 0.0f;
+        ar[k] = in ? M::R(c, at(A, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < D2_AA * D2_AA; k += NTF) {
+        const int gx = ax0 - 5 + k % D2_AA, gy = ay0 - 5 + k / D2_AA;
+        aA[k] = inside(A, gx, gy) ? M::A(c, la, at(A, gx, gy)) : make_float4(0, 0, 0, 0);
+    }
+    for (int k = t; k < D2_BD * D2_BD; k += NTF) {
+        const int gx = X0 - 2 + k % D2_BD, gy = Y0 - 2 + k / D2_BD;
+        bD[k] = inside(B, gx, gy) ? M::D(c, lb, at(B, gx, gy)) : 0.0f;
+    }
+    for (int k = t; k < D2_BA * D2_BA; k += NTF) {
+        const int gx = X0 - 1 + k % D2_BA, gy = Y0 - 1 + k / D2_BA;
+        bA[k] = inside(B, gx, gy) ? M::A(c, lb, at(B, gx, gy)) : make_float4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    // level a: z on its region and five rings (the region of its four-ring arrays is ax0 - 5)
+    down_z<NTF>(ax0 - 4, ay0 - 4, D2_AA, D2_AD, aD, ar, aA, az);
+    __syncthreads();
+    for (int k = t; k < 32 * 32; k += NTF) {
+        const int lx = k & 31, ly = k >> 5, gx = ax0 + lx, gy = ay0 + ly;
+        if (inside(A, gx, gy) && s_on[(ly >> 4) * 2 + (lx >> 4)]) M::set_X(c, at(A, gx, gy), az[sx_(gx, gy, ax0, ay0, 5, D2_AA)]);
+    }
+    // level b's right-hand side on two rings, as level b reads it back from storage
+    for (int k = t; k < D2_BD * D2_BD; k += NTF) {
+        const int qx = X0 - 2 + k % D2_BD, qy = Y0 - 2 + k / D2_BD;
+        float2 sum = make_float2(0.0f, 0.0f);
+        if (inside(B, qx, qy)) sum = restrict_red(qx, qy, ax0 - 5, ay0 - 5, D2_AA, D2_AD, aD, aA, az);
+        bR[k] = M::stored_R(sum);
+        const int ox = qx - X0, oy = qy - Y0;
+        if (ox >= 0 && oy >= 0 && ox < 16 && oy < 16 && inside(B, qx, qy) && s_on[(oy >> 3) * 2 + (ox >> 3)])
+            M::set_R(c, at(B, qx, qy), sum);
+    }
+    __syncthreads();
+    // level b: z on its tile and one ring
+    down_z<NTF>(X0, Y0, D2_BA, D2_BD, bD, bR, bA, bz);
+    __syncthreads();
+    for (int k = t; k < 16 * 16; k += NTF) {
+        const int lx = k & 15, ly = k >> 4, x = X0 + lx, y = Y0 + ly;
+        if (inside(B, x, y)) M::set_X(c, at(B, x, y), bz[sx_(x, y, X0, Y0, 1, D2_BA)]);
+        if (((lx | ly) & 1) != 0) continue;
+        const int cx = x >> 1, cy = y >> 1;
+        if (!inside(Cc, cx, cy)) continue;
+        M::set_R(c, at(Cc, cx, cy), restrict_red(cx, cy, X0 - 1, Y0 - 1, D2_BA, D2_BD, bD, bA, bz));
+    }
+}
+
+// mg_up.comp on levels b = l + 1 and then a = l (A, B; B's coarse level Cc holds z already).
+template <bool H>
+__global__ __launch_bounds__(NTF) void k_mg_up2(const TankBatch tb, Lv A, Lv B, Lv Cc, int la, int gated, int par) {
+    using M = MG<H>;
+    const Ctx c = ctx_of(tb.t[blockIdx.z]);
+    extern __shared__ float4 dyn_lds[];
+    float4* bA = dyn_lds;
+    float4* aA = bA + U2_BR * U2_BR;
+    float2* bx = reinterpret_cast<float2*>(aA + U2_AR * U2_AR);
+    float2* br = bx + U2_BX * U2_BX;
+    float2* bsb = br + U2_BR * U2_BR;
+    float2* bzc = bsb + U2_BR * U2_BR;
+    float2* bz = bzc + U2_BP * U2_BP;
+    float2* ax = bz + U2_BZ * U2_BZ;
+    float2* ar = ax + U2_AX * U2_AX;
+    float2* asb = ar + U2_AR * U2_AR;
+    float* bD = reinterpret_cast<float*>(asb + U2_AR * U2_AR);
+    float* aD = bD + U2_BX * U2_BX;
+    __shared__ uint32_t s_on[4];
+    const int lb = la + 1;
+    if (gated && slot_done(c, par)) return;
+    const int2 tq = tile_xy();
+    if (c.ptile[B.toff + slot_of(tq)] == 0u) return;
+    const int t = threadIdx.y * 32 + threadIdx.x;
+    const int X0 = tq.x * 16, Y0 = tq.y * 16, ax0 = 2 * X0, ay0 = 2 * Y0;
+    if (t < 4) s_on[t] = tile_on(c, A, (ax0 >> 4) + (t & 1), (ay0 >> 4) + (t >> 1)) ? 1u : 0u;
+    for (int k = t; k < U2_BX * U2_BX; k += NTF) {
+        const int gx = X0 - 3 + k % U2_BX, gy = Y0 - 3 + k / U2_BX;
+        const bool in = inside(B, gx, gy);
+        bD[k] = in ? M::D(c, lb, at(B, gx, gy)) : 0.0f;
+        bx[k] = in ? M::X(c, at(B, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < U2_BR * U2_BR; k += NTF) {
+        const int gx = X0 - 2 + k % U2_BR, gy = Y0 - 2 + k / U2_BR;
+        const bool in = inside(B, gx, gy);
+        bA[k] = in ? M::A(c, lb, at(B, gx, gy)) : make_float4(0, 0, 0, 0);
+        br[k] = in ? M::R(c, at(B, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    // parents of level b's three-ring region: coarse cells X0/2 - 2 .. X0/2 + 9
+    const int pcx = (X0 >> 1) - 2, pcy = (Y0 >> 1) - 2;
+    for (int k = t; k < U2_BP * U2_BP; k += NTF) {
+        const int gx = pcx + k % U2_BP, gy = pcy + k / U2_BP;
+        bzc[k] = inside(Cc, gx, gy) ? M::Z(c, at(Cc, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < U2_AX * U2_AX; k += NTF) {
+        const int gx = ax0 - 2 + k % U2_AX, gy = ay0 - 2 + k / U2_AX;
+        const bool in = inside(A, gx, gy);
+        aD[k] = in ? M::D(c, la, at(A, gx, gy)) : 0.0f;
+        ax[k] = in ? M::X(c, at(A, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    for (int k = t; k < U2_AR * U2_AR; k += NTF) {
+        const int gx = ax0 - 1 + k % U2_AR, gy = ay0 - 1 + k / U2_AR;
+        const bool in = inside(A, gx, gy);
+        aA[k] = in ? M::A(c, la, at(A, gx, gy)) : make_float4(0, 0, 0, 0);
+        ar[k] = in ? M::R(c, at(A, gx, gy)) : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    // level b: black values on two rings, then z on one ring (its own tile written)
+    up_black<NTF>(X0, Y0, 2, 3, U2_BX, U2_BR, pcx, pcy, U2_BP, bD, bx, bA, br, bzc, bsb);
+    __syncthreads();
+    for (int k = t; k < U2_BZ * U2_BZ; k += NTF) {
+        const int gx = X0 - 1 + k % U2_BZ, gy = Y0 - 1 + k / U2_BZ;
+        float2 rv;
+        const float2 z = up_red(gx, gy, X0, Y0, 2, U2_BR, 3, U2_BX, bD, bA, br, bsb, &rv);
+        const int ox = gx - X0, oy = gy - Y0;
+        if (ox >= 0 && oy >= 0 && ox < 16 && oy < 16 && inside(B, gx, gy)) M::set_Z(c, at(B, gx, gy), z);
+        bz[k] = M::stored_Z(z);  // as level a reads it back from storage
+    }
+    __syncthreads();
+    // level a: black values on one ring from the parents' z, then z on its region
+    up_black<NTF>(ax0, ay0, 1, 2, U2_AX, U2_AR, X0 - 1, Y0 - 1, U2_BZ, aD, ax, aA, ar, bz, asb);
+    __syncthreads();
+    for (int k = t; k < 32 * 32; k += NTF) {
+        const int lx = k & 31, ly = k >> 5, gx = ax0 + lx, gy = ay0 + ly;
+        if (!inside(A, gx, gy) || !s_on[(ly >> 4) * 2 + (lx >> 4)]) continue;
+        float2 rv;
+        M::set_Z(c, at(A, gx, gy), up_red(gx, gy, ax0, ay0, 1, U2_AR, 2, U2_AX, aD, aA, ar, asb, &rv));
     }
 }
 
@@ -732,12 +1109,12 @@ __global__ __launch_bounds__(256) void k_mg_up(const TankBatch tb, int l, int ga
 // iterate and right-hand side (float2) and diagonal; the first level's right-hand side is
 // loaded once with the matrix, so that the sweeps never wait on global memory.
 template <bool H>
-__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated) {
+__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated, int par) {
     using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     extern __shared__ float4 tail_lds[];
     __shared__ int4 sl[SP_MAX_LEVELS];
-    if (gated && solve_done(c)) return;
+    if (gated && slot_done(c, par)) return;
     const int first = c.tail, last = c.n_levels - 1;
     const int4 Lf = c.lev[first], Le = c.lev[last];
     const int BASE = Lf.x;
@@ -760,29 +1137,43 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated)
     }
     __syncthreads();
 
-    auto rhs_at = [&](int s) -> float2 { return sr[s]; };
+    // The neighbours' operands are loaded without waiting on whether each is an unknown, and
+    // added in order where it is: the same sums as adding under a branch, without a chain of
+    // dependent LDS round trips per cell.
     auto sat = [&](int4 L, int x, int y) { return L.x - BASE + y * L.y + x; };
-    auto sunk = [&](int4 L, int x, int y) { return x >= 0 && y >= 0 && x < L.y && y < L.z && sd[sat(L, x, y)] > 0.0f; };
+    // slot of cell (x, y) of level L, clamped to the level's first cell outside it; whether it
+    // is an unknown
+    auto nbr = [&](int4 L, int x, int y, int& s) -> bool {
+        const bool in = x >= 0 && y >= 0 && x < L.y && y < L.z;
+        s = L.x - BASE + (in ? y * L.y + x : 0);
+        const float dn = sd[s];
+        return in & (dn > 0.0f);
+    };
     auto sweep = [&](int4 L, int colour, bool zero, int t0, int step) {
         for (int t = t0; t < L.y * L.z; t += step) {
-            int x = t % L.y, y = t / L.y;
-            int s = L.x - BASE + t;
-            float D = sd[s];
+            const int x = t % L.y, y = t / L.y;
+            const int s = L.x - BASE + t;
+            const float D = sd[s];
             if (D <= 0.0f || ((x + y) & 1) != colour) {
                 if (zero) sx[s] = make_float2(0.0f, 0.0f);
                 continue;
             }
-            float2 sum = rhs_at(s);
+            float2 sum = sr[s];
             if (!zero) {
-                float4 w = sa[s];
+                const float4 w = sa[s];
+                int sn[4];
+                bool ok[4];
+                float2 xn[4];
+#pragma unroll
                 for (int d = 0; d < 4; d++) {
-                    int nx = x + NB_X[d], ny = y + NB_Y[d];
-                    if (sunk(L, nx, ny)) {
-                        float wd = f4c(w, d);
-                        float2 xn = sx[sat(L, nx, ny)];
-                        sum.x += wd * xn.x;
-                        sum.y += wd * xn.y;
-                    }
+                    ok[d] = nbr(L, x + NB_X[d], y + NB_Y[d], sn[d]);
+                    xn[d] = sx[sn[d]];
+                }
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const float wd = f4c(w, d);
+                    const float2 cand = make_float2(sum.x + wd * xn[d].x, sum.y + wd * xn[d].y);
+                    sum = ok[d] ? cand : sum;
                 }
             }
             sx[s] = make_float2(sum.x / D, sum.y / D);
@@ -790,47 +1181,63 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated)
     };
     auto restrict_to = [&](int4 F, int4 C, int t0, int step) {
         for (int t = t0; t < C.y * C.z; t += step) {
-            int cx = t % C.y, cy = t / C.y;
+            const int cx = t % C.y, cy = t / C.y;
             float2 sum = make_float2(0.0f, 0.0f);
+#pragma unroll
             for (int k = 0; k < 4; k++) {
-                int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
-                if (!sunk(F, x, y)) continue;
-                int s = sat(F, x, y);
-                float4 w = sa[s];
-                float2 az = make_float2(sd[s] * sx[s].x, sd[s] * sx[s].y);
+                const int x = 2 * cx + (k & 1), y = 2 * cy + (k >> 1);
+                int s;
+                const bool okc = nbr(F, x, y, s);
+                const float4 w = sa[s];
+                const float Dc = sd[s];
+                const float2 xc = sx[s];
+                const float2 r = sr[s];
+                int sn[4];
+                bool ok[4];
+                float2 xn[4];
+#pragma unroll
                 for (int d = 0; d < 4; d++) {
-                    int nx = x + NB_X[d], ny = y + NB_Y[d];
-                    if (sunk(F, nx, ny)) {
-                        float wd = f4c(w, d);
-                        float2 xn = sx[sat(F, nx, ny)];
-                        az.x -= wd * xn.x;
-                        az.y -= wd * xn.y;
-                    }
+                    ok[d] = nbr(F, x + NB_X[d], y + NB_Y[d], sn[d]);
+                    xn[d] = sx[sn[d]];
                 }
-                float2 r = rhs_at(s);
-                sum.x += r.x - az.x;
-                sum.y += r.y - az.y;
+                float2 az = make_float2(Dc * xc.x, Dc * xc.y);
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const float wd = f4c(w, d);
+                    const float2 cand = make_float2(az.x - wd * xn[d].x, az.y - wd * xn[d].y);
+                    az = ok[d] ? cand : az;
+                }
+                const float2 cand = make_float2(sum.x + (r.x - az.x), sum.y + (r.y - az.y));
+                sum = okc ? cand : sum;
             }
             sr[C.x - BASE + t] = sum;
         }
     };
     auto post_black = [&](int4 F, int4 C, int t0, int step) {
         for (int t = t0; t < F.y * F.z; t += step) {
-            int x = t % F.y, y = t / F.y;
+            const int x = t % F.y, y = t / F.y;
             if (((x + y) & 1) == 0) continue;
-            int s = F.x - BASE + t;
-            float D = sd[s];
+            const int s = F.x - BASE + t;
+            const float D = sd[s];
             if (D <= 0.0f) continue;
-            float4 w = sa[s];
-            float2 sum = rhs_at(s);
+            const float4 w = sa[s];
+            float2 sum = sr[s];
+            bool ok[4];
+            float2 xn[4], zc[4];
+#pragma unroll
             for (int d = 0; d < 4; d++) {
-                int nx = x + NB_X[d], ny = y + NB_Y[d];
-                if (!sunk(F, nx, ny)) continue;
-                float wd = f4c(w, d);
-                float2 xn = sx[sat(F, nx, ny)];
-                float2 zc = sx[sat(C, nx >> 1, ny >> 1)];
-                sum.x += wd * (xn.x + SP_MG_OMEGA * zc.x);
-                sum.y += wd * (xn.y + SP_MG_OMEGA * zc.y);
+                const int nx = x + NB_X[d], ny = y + NB_Y[d];
+                int sn;
+                ok[d] = nbr(F, nx, ny, sn);
+                xn[d] = sx[sn];
+                zc[d] = sx[ok[d] ? sat(C, nx >> 1, ny >> 1) : C.x - BASE];
+            }
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const float wd = f4c(w, d);
+                const float2 cand = make_float2(sum.x + wd * (xn[d].x + SP_MG_OMEGA * zc[d].x),
+                                                sum.y + wd * (xn[d].y + SP_MG_OMEGA * zc[d].y));
+                sum = ok[d] ? cand : sum;
             }
             sx[s] = make_float2(sum.x / D, sum.y / D);
         }
@@ -879,34 +1286,48 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated)
     for (int t = bt; t < NF; t += 1024) M::set_Z(c, BASE + t, sx[t]);
 }
 
-// The scalar step of the CG (cg_reduce.comp / reduce_tail.glsl modes 0, 1, 2) over the
-// partial sums the previous kernel wrote: one 256 work-item group, fixed summation order.
-// The reference folds this into the last work-group of the producing kernel; here the
-// kernel boundary orders the partials instead, which on a multi-XCD part is far cheaper
-// than a device-scope release fence in every producing work-group.
-__global__ __launch_bounds__(REDUCE_NT) void k_cg_reduce(const TankBatch tb, int mode, int nblocks, int gated) {
+
+// The end of a solve (reduce_tail.glsl:105-116, modes 3 and 9): the scalar step of its last
+// iteration (or its start, when none ran), the final state into the record's first fields,
+// and the statistics; the second solve keeps its own iteration count.
+__global__ __launch_bounds__(REDUCE_NT) void k_cg_end(const TankBatch tb, int nbp, int iters, int second) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ double4 lds[REDUCE_NT / 64];
-    if (gated && solve_done(c)) return;
-    reduce_scalars(c, mode, nblocks, lds);
-}
-
-// Solve statistics at the end of a solve (reduce_tail.glsl:105-116, modes 3 and 9: the
-// second solve keeps its own iteration count).
-__global__ void k_cg_end(const TankBatch tb, int second) {
-    const Ctx c = ctx_of(tb.t[blockIdx.z]);
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double4 a = make_double4(0, 0, 0, 0), b = a;
+    const sp_cg_state* P = &c.K->st[(iters - 1) & 1];
+    const bool ended = iters > 0 && st_done(*P);
+    if (!ended) a = total_of<REDUCE_NT>(c.part_sums, PART_RZ * nbp, nbp, lds);
+    if (iters == 0) b = total_of<REDUCE_NT>(c.part_sums, PART_BB * nbp, nbp, lds);
+    if (threadIdx.x != 0) return;
+    sp_cg_state E;
+    if (iters == 0) {
+        E = cg_start(c, a, b, second != 0);
+        c.K->solves += 1u;
+    } else {
+        E = ended ? *P : cg_next(c, *P, a);
+    }
     sp_solve* K = c.K;
-    uint32_t it = K->iters[0] > K->iters[1] ? K->iters[0] : K->iters[1];
+    for (int i = 0; i < 2; i++) {
+        K->rz[i] = E.rz[i];
+        K->rr[i] = E.rr[i];
+        K->bb[i] = E.bb[i];
+        K->beta[i] = E.beta[i];
+        K->done[i] = E.done[i];
+        K->iters[i] = E.iters[i];
+    }
+    K->sweeps = E.sweeps;
+    K->halt = E.halt;
+    K->cap_now = E.cap;
+    const uint32_t it = E.iters[0] > E.iters[1] ? E.iters[0] : E.iters[1];
     if (second) {
-        if (K->iters[0] > K->p2_it_max) K->p2_it_max = K->iters[0];
-        if (K->done[0] == 0u) K->p2_unconv += 1u;
-        K->p2_iters_last = K->iters[0];
+        if (E.iters[0] > K->p2_it_max) K->p2_it_max = E.iters[0];
+        if (E.done[0] == 0u) K->p2_unconv += 1u;
+        K->p2_iters_last = E.iters[0];
     } else {
         if (it > K->it_max) K->it_max = it;
-        if (K->done[0] == 0u) K->unconv += 1u;
+        if (E.done[0] == 0u) K->unconv += 1u;
         K->iters_last = it;
     }
-    float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
+    const float rel = E.bb[0] > 0.0 ? (float)sqrt(E.rr[0] / E.bb[0]) : 0.0f;
     if (rel > K->res_max) K->res_max = rel;
 }

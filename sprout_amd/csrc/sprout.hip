@@ -564,6 +564,11 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         // captured graphs (profilers that cannot follow graph launches need it)
         const char* gv = getenv("SPROUT_GRAPHS");
         s->gs.on = !(gv && gv[0] == '0');
+        // SPROUT_MG_FUSE=0 runs the coarse multigrid levels one per launch (the same results)
+        const char* fv = getenv("SPROUT_MG_FUSE");
+        int lds_max = 0;
+        HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+        r.fuse = !(fv && fv[0] == '0') && (size_t)lds_max >= std::max(DOWN2_LDS, UP2_LDS);
     } catch (const HipError& e) {
         char buf[256];
         snprintf(buf, sizeof(buf), "%s failed: %s", e.what, hipGetErrorString(e.e));
@@ -1296,6 +1301,7 @@ static void* field_ptr(sp_sim* s, int32_t field, size_t* bytes) {
     case SP_FIELD_DSETTLE: *bytes = 8 * N * (size_t)std::max(s->gp.n_fam, 1); return c.dsettle;
     case SP_FIELD_MAGC: *bytes = 8 * N; return c.magc;
     case SP_FIELD_RMASK: *bytes = 4 * N; return c.rmask;
+    case SP_FIELD_MGX: *bytes = 8 * T; return c.mgx;
     default: *bytes = 0; return nullptr;
     }
 }
@@ -1412,7 +1418,7 @@ int sp_debug_stage(sp_sim* s, int32_t stage, int32_t arg) {
             s->grid_valid = true;
             break;
         case SP_STAGE_MG_BUILD: launch_mg_build(r); break;
-        case SP_STAGE_VCYCLE: launch_vcycle(r, 0, -1, 0); break;
+        case SP_STAGE_VCYCLE: launch_vcycle(r, 0, 0); break;
         default: set_err(s, "unknown stage %d", stage); return SP_E_INVALID;
         }
         HIPCHK(hipGetLastError());
