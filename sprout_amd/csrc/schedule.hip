@@ -142,7 +142,7 @@ void launch_pressure_t(const Run& r, uint32_t iters, bool second) {
         launch_mg_build_t<H>(r);
         hipLaunchKernelGGL(k_cg_scale, grid0(r), TILE, 0, r.stream, r.tb);
     }
-    hipLaunchKernelGGL(k_cg_init, gt0, bt0, 0, r.stream, r.tb, L0, nbp);
+    hipLaunchKernelGGL(k_cg_init<H>, gt0, bt0, 0, r.stream, r.tb, L0, nbp);
     launch_vcycle_t<H>(r, 1, 0);
     for (uint32_t k = 0; k < iters; k++) {
         const int par = (int)(k & 1);

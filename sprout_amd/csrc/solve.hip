@@ -7,16 +7,21 @@
 // iterate, right-hand side), as the reference lays them out (mg.glsl:1-37). Kernels skip the
 // 16 x 16 tiles without unknowns (ptile).
 //
-// Launches per CG iteration: the level-0 kernels (cg_apply with the scalar step that ends
-// the previous iteration; the update fused with the way down; the way up), the coarse levels
-// two per launch, and the tail of the hierarchy in one work-group: eight at 1024 x 576
-// instead of the reference's thirteen dispatches and two one-group reductions.
+// Launches per CG iteration: the scalar step that ends the previous iteration and the one
+// after cg_apply (one work-group each), cg_apply, the update fused with the way down on
+// level 0, the coarse levels two per launch on the way down and up, the tail of the
+// hierarchy in one work-group, and the way up on level 0: ten at 1024 x 576, where the
+// reference records thirteen dispatches and two one-group reductions.
 //
 // Precision: cell arithmetic is FP32 in the reference's operation order; the scalar
 // reductions (r.z, r.r, p.q, b.b) are FP64 — the product of two floats is exact in double
 // and the sums are formed per work-group and then in a fixed order, so the scalars do not
 // depend on scheduling. With mg_half the V-cycle's storage is narrower (MG<true> below);
-// its arithmetic stays FP32.
+// its arithmetic stays FP32, and the level-0 residual b - A x and the operator A p are
+// evaluated in compensated FP32 (Ff below): the FP32 sums of those five-term stencils
+// cancel (the pressure is near hydrostatic, |A| |x| ~ 1e4 |b|), and their rounding, not the
+// tolerance, is what bounds the FP32 solver's error against the exact solution (1e-5 at
+// 1024 x 576; 1e-7 compensated).
 //
 // Convergence without host round trips: the scalar state lives in device memory (sp_solve);
 // every kernel of an iteration reads its `done` flags at entry and returns at once when
@@ -39,6 +44,26 @@ SP_DEV Lv level(const Ctx& c, int l) {
 SP_DEV bool inside(const Lv& L, int x, int y) { return x >= 0 && y >= 0 && x < L.w && y < L.h; }
 SP_DEV int at(const Lv& L, int x, int y) { return L.off + y * L.w + x; }
 SP_DEV bool unknown(const Ctx& c, const Lv& L, int x, int y) { return inside(L, x, y) && c.mgd[at(L, x, y)] > 0.0f; }
+
+// A sum of products carried as an unevaluated pair s + c of floats: each product a b is
+// split exactly into its rounded value and error by an FMA (TwoProduct), each addition's
+// rounding error is recovered by Knuth's TwoSum, and the errors are added into c. The
+// result is as accurate as if the sum were formed in twice the working precision and then
+// rounded (Ogita, Rump and Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 26,
+// 2005, algorithm Dot2). Needs -ffp-contract=off, which the library is built with.
+struct Ff {
+    float s, c;
+};
+SP_DEV Ff ff_prod(float a, float b) {
+    const float p = a * b;
+    return Ff{p, __builtin_fmaf(a, b, -p)};
+}
+SP_DEV Ff ff_add_prod(Ff acc, float a, float b) {
+    const float p = a * b, e = __builtin_fmaf(a, b, -p);
+    const float s = acc.s + p, bb = s - acc.s, err = (acc.s - (s - bb)) + (p - bb);
+    return Ff{s, acc.c + (err + e)};
+}
+SP_DEV float ff_value(Ff a) { return a.s + a.c; }
 
 // Storage of the multigrid levels. MG<false>: FP32, the reference's layout. MG<true>
 // (mg_half): every coarse level (1 and below) narrow: iterates (mgx, mgz) and right-hand
@@ -539,7 +564,8 @@ SP_DEV float2 up_red(int x, int y, int x0, int y0, int rb, int SB, int rx, int S
 
 // ---- Krylov kernels on level 0 -----------------------------------------------------------
 
-// cg_init.comp: r = b - A x and the partial sums of b.b.
+// cg_init.comp: r = b - A x and the partial sums of b.b (H: r compensated).
+template <bool H>
 __global__ __launch_bounds__(NT0) void k_cg_init(const TankBatch tb, Lv F, int nbp) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     __shared__ double4 lds[NT0 / 64];
@@ -561,18 +587,32 @@ __global__ __launch_bounds__(NT0) void k_cg_init(const TankBatch tb, Lv F, int n
         if (D > 0.0f) {
             const float2 xv = c.pres[i];
             const float4 w = c.mga[i];
-            float2 ax = make_float2(D * xv.x, D * xv.y);
-            for (int d = 0; d < 4; d++) {
-                const int nx = x + NB_X[d], ny = y + NB_Y[d];
-                if (unknown(c, F, nx, ny)) {
-                    const float wd = f4c(w, d);
-                    const float2 pn = c.pres[at(F, nx, ny)];
-                    ax.x -= wd * pn.x;
-                    ax.y -= wd * pn.y;
-                }
-            }
             const float4 b = c.prhs[i];
-            c.mgr[i] = make_float2(b.x - ax.x, b.y - ax.y);
+            if (H) {
+                Ff rx = ff_add_prod(Ff{b.x, 0.0f}, -D, xv.x), ry = ff_add_prod(Ff{b.y, 0.0f}, -D, xv.y);
+                for (int d = 0; d < 4; d++) {
+                    const int nx = x + NB_X[d], ny = y + NB_Y[d];
+                    if (unknown(c, F, nx, ny)) {
+                        const float wd = f4c(w, d);
+                        const float2 pn = c.pres[at(F, nx, ny)];
+                        rx = ff_add_prod(rx, wd, pn.x);
+                        ry = ff_add_prod(ry, wd, pn.y);
+                    }
+                }
+                c.mgr[i] = make_float2(ff_value(rx), ff_value(ry));
+            } else {
+                float2 ax = make_float2(D * xv.x, D * xv.y);
+                for (int d = 0; d < 4; d++) {
+                    const int nx = x + NB_X[d], ny = y + NB_Y[d];
+                    if (unknown(c, F, nx, ny)) {
+                        const float wd = f4c(w, d);
+                        const float2 pn = c.pres[at(F, nx, ny)];
+                        ax.x -= wd * pn.x;
+                        ax.y -= wd * pn.y;
+                    }
+                }
+                c.mgr[i] = make_float2(b.x - ax.x, b.y - ax.y);
+            }
             acc.x = (double)b.x * (double)b.x;
             acc.y = (double)b.y * (double)b.y;
         } else {
@@ -607,8 +647,8 @@ __global__ __launch_bounds__(REDUCE_NT) void k_cg_step(const TankBatch tb, int n
     }
 }
 
-// cg_apply.comp:23-51: p = z + beta p_old, q = A p and the partial sums of p.q. par = k & 1;
-// the direction alternates between cgp and cgp2 with it.
+// cg_apply.comp:23-51: p = z + beta p_old, q = A p (H: compensated) and the partial sums of
+// p.q. par = k & 1; the direction alternates between cgp and cgp2 with it.
 template <bool H>
 __global__ __launch_bounds__(NT0) void k_cg_apply(const TankBatch tb, Lv F, int nbp, int par) {
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
@@ -656,14 +696,29 @@ __global__ __launch_bounds__(NT0) void k_cg_apply(const TankBatch tb, Lv F, int 
         if (D > 0.0f) {
             const float2 p = sp[si];
             const float4 w = c.mga[i];
-            float2 q = make_float2(D * p.x, D * p.y);
-            for (int d = 0; d < 4; d++) {
-                const int sn = sx_(x + NB_X[d], y + NB_Y[d], x0, y0, 1, S1);
-                if (sd[sn] > 0.0f) {
-                    const float wd = f4c(w, d);
-                    const float2 pn = sp[sn];
-                    q.x -= wd * pn.x;
-                    q.y -= wd * pn.y;
+            float2 q;
+            if (H) {
+                Ff qx = ff_prod(D, p.x), qy = ff_prod(D, p.y);
+                for (int d = 0; d < 4; d++) {
+                    const int sn = sx_(x + NB_X[d], y + NB_Y[d], x0, y0, 1, S1);
+                    if (sd[sn] > 0.0f) {
+                        const float wd = f4c(w, d);
+                        const float2 pn = sp[sn];
+                        qx = ff_add_prod(qx, -wd, pn.x);
+                        qy = ff_add_prod(qy, -wd, pn.y);
+                    }
+                }
+                q = make_float2(ff_value(qx), ff_value(qy));
+            } else {
+                q = make_float2(D * p.x, D * p.y);
+                for (int d = 0; d < 4; d++) {
+                    const int sn = sx_(x + NB_X[d], y + NB_Y[d], x0, y0, 1, S1);
+                    if (sd[sn] > 0.0f) {
+                        const float wd = f4c(w, d);
+                        const float2 pn = sp[sn];
+                        q.x -= wd * pn.x;
+                        q.y -= wd * pn.y;
+                    }
                 }
             }
             p_new[i] = p;

@@ -194,11 +194,13 @@ def rheo_block(grid, rheology=2, yield_stress=300.0, flow_index=1.0, shear_time=
     return params, scene.rasterize(), settle(parts, params), [N.ROW_HONEY]
 
 
-def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), strip_temp=76.85):
+def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip_cells=10, strip_temp=76.85):
     """The two-fluid tank with heat: water below (density 1000, 1e-6 m^2/s), oil above
     (900, 1e-4), a honey-like blob (1400, 0.09) near the top centre, everything at 20 deg C
-    (293 K), on a floor strip held at 350 K. (params, solid field, particles, rows)."""
+    (293 K), on a floor strip of `strip_cells` cells at the centre held at 350 K.
+    (params, solid field, particles, rows)."""
     W, H = grid
+    strip = (0.5 - 0.5 * strip_cells / W, 0.5 + 0.5 * strip_cells / W)
     p = dam_params(cell_size=cell_size)
     p.mat[N.ROW_WATER].density = 1000.0
     p.mat[N.ROW_WATER].viscosity = 1e-6
@@ -217,6 +219,39 @@ def two_fluid_heat(grid=(512, 288), seed=0, cell_size=0.01, strip=(0.4, 0.6), st
     field = scene.rasterize()
     parts = settle(parts, p)
     return p, field, parts, [N.ROW_WATER, N.ROW_HONEY, N.ROW_OIL]
+
+
+LAYER_ROWS = (N.ROW_WATER, N.ROW_OIL, N.ROW_HONEY, 3, 4, 5)
+
+
+def layered_liquids(grid, k, seed=0, cell_size=0.005, fill=(0.6, 0.8), pressure_tol=1e-6, viscosity_tol=1e-4):
+    """k (2 .. 6) liquids in horizontal layers of seeded thicknesses, stacked in a seeded
+    order (heavy over light where the draw puts it so), filling a seeded 60-80 % of the tank
+    height; densities uniform over 900 .. 1400 kg/m^3 and kinematic viscosities log-uniform
+    over 1e-6 .. 0.09 m^2/s (the two-fluid tank's ranges), isothermal.
+    (params, solid field, particles, rows)."""
+    assert 2 <= k <= len(LAYER_ROWS)
+    rng = np.random.default_rng(seed)
+    params = dam_params(cell_size=cell_size, pressure_tol=pressure_tol)
+    params.viscosity_tol = viscosity_tol
+    rows = list(LAYER_ROWS[:k])
+    for r in rows:
+        m = params.mat[r]
+        m.density = float(rng.uniform(900.0, 1400.0))
+        m.viscosity = float(10.0 ** rng.uniform(-6.0, np.log10(0.09)))
+    W, H = grid
+    scene = sc.Scene(grid)
+    top = float(rng.uniform(*fill)) * H
+    cuts = np.sort(rng.uniform(0.0, 1.0, k - 1))
+    edges = np.concatenate([[0.0], cuts, [1.0]]) * top
+    parts = []
+    n = 0
+    for row, y0, y1 in zip([rows[j] for j in rng.permutation(k)], edges[:-1], edges[1:]):
+        q = sc.fill_rect(scene, (0.0, float(y0), float(W), float(y1)), row, n_before=n)
+        n += len(q)
+        parts.append(q)
+    parts = settle(np.concatenate(parts), params)
+    return params, scene.rasterize(), parts, sorted(rows)
 
 
 ROW_SLIME = 3  # a built-in row with the default material, made viscoelastic by slime_params

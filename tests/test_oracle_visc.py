@@ -377,3 +377,39 @@ def test_resting_viscous_pool_stays_at_rest():
     rms1, max1, _ = pool_speeds(0)
     assert max2 < 0.05 * gdt, (max2, gdt)
     assert rms2 < 0.1 * rms1, (rms2, rms1)
+
+
+def test_fp64_matrix_free_viscosity_system_is_the_definition():
+    """The FP64 matrix-free viscosity system the GPU tests compare each device solve with
+    (fp64_ref.ViscSystem) is the definition's A = M + G^T W G on the unknown faces, and its
+    right-hand side from the device's strain field is -(G^T W G u*) there; the oracle's
+    solve agrees with its FP64 solution to the solver tolerance."""
+    import fp64_ref
+
+    W, H = GRID
+    o, g, _ = make_viscous(GRID)
+    to_split(o, random_flow(W, H, 7))
+    n = W * H
+    vel0, live, _ = faces(o)
+    sysm = fp64_ref.ViscSystem(vel0, o.field("VW", np.float32, (n, 4)), o.field("COEF", np.float32, (n, 4)),
+                               o.field("VTILES", np.uint32, ((W + 15) // 16) * ((H + 15) // 16)), W, H)
+    unk = unknown_faces(o)
+    assert np.array_equal(sysm.unk, unk)
+    M, K = definition_matrix(o)
+    A = (M + K).tocsr()[unk][:, unk]
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        x = rng.standard_normal(int(unk.sum()))
+        np.testing.assert_allclose(sysm.apply(x), A @ x, rtol=1e-12, atol=1e-9 * np.abs(A @ x).max())
+    np.testing.assert_allclose(sysm.diag, A.diagonal(), rtol=1e-6)
+    ustar = np.concatenate([np.where(live[:, 0], vel0[:, 0], 0.0), np.where(live[:, 1], vel0[:, 1], 0.0)])
+    b = sysm.rhs(o.field("VS", np.float32, (n, 4)))
+    want = -(K @ ustar)[unk]
+    assert np.linalg.norm(b - want) <= 1e-6 * np.linalg.norm(want)
+    o.stage("VISC_SOLVE", 0)
+    e = fp64_ref.viscosity_errors(sysm, o.field("VS", np.float32, (n, 4)), o.field("VEL", np.float32, (n, 4)),
+                                  float(g.tol[2]))
+    assert e["true_residual"] <= 3 * float(g.tol[2]) + 1e-6
+    # at least as close as an FP64 CG stopped at the same tolerance
+    assert e["err_vs_fp64"] <= max(e["fp64_cg_same_tol_err"], 1e-6), e
+    assert e["vel_err_vs_fp64"] <= max(e["fp64_cg_same_tol_vel_err"], 1e-6), e
