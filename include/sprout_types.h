@@ -325,7 +325,8 @@ typedef struct sp_solve {
     double v_rz, v_alpha, v_beta, v_r0, v_rr, v_pq;
     uint32_t v_done, v_iters, v_it_max, v_unconv;
     float v_res_max;
-    uint32_t v_iters_last, p2_iters_last, pad;
+    uint32_t v_iters_last, p2_iters_last;
+    uint32_t iters_sum;  /* iterations of every first pressure solve so far, wrapping at 2^32 */
     /* This tank's recorded iterations of the pressure, second pressure and viscosity solves
      * (written by the host before each batch; 0 = no cap), the iterations the running solves
      * have done, and whether they stopped at the cap. A tank whose launches are shared with

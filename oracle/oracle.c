@@ -2217,6 +2217,7 @@ static void pressure_iterate(or_sim* s, int second) {
         if (it > K->it_max) K->it_max = it;
         if (K->done[0] == 0) K->unconv += 1;
         K->iters_last = k;
+        K->iters_sum += k;
     }
     float rel = K->bb[0] > 0.0 ? (float)sqrt(K->rr[0] / K->bb[0]) : 0.0f;
     if (rel > K->res_max) K->res_max = rel;

@@ -226,7 +226,7 @@ SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<
                         ("v_rz", "<f8"), ("v_alpha", "<f8"), ("v_beta", "<f8"), ("v_r0", "<f8"),
                         ("v_rr", "<f8"), ("v_pq", "<f8"), ("v_done", "<u4"), ("v_iters", "<u4"),
                         ("v_it_max", "<u4"), ("v_unconv", "<u4"), ("v_res_max", "<f4"),
-                        ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("pad", "<u4"),
+                        ("v_iters_last", "<u4"), ("p2_iters_last", "<u4"), ("iters_sum", "<u4"),
                         ("cap_p", "<u4"), ("cap_p2", "<u4"), ("cap_v", "<u4"), ("cap_now", "<u4"),
                         ("sweeps", "<u4"), ("halt", "<u4"), ("v_sweeps", "<u4"), ("v_halt", "<u4"),
                         ("st", CG_STATE_DTYPE, (2,))])

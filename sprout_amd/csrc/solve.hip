@@ -1389,6 +1389,7 @@ __global__ __launch_bounds__(REDUCE_NT) void k_cg_end(const TankBatch tb, int nb
         if (it > K->it_max) K->it_max = it;
         if (E.done[0] == 0u) K->unconv += 1u;
         K->iters_last = it;
+        K->iters_sum += it;
     }
     const float rel = E.bb[0] > 0.0 ? (float)sqrt(E.rr[0] / E.bb[0]) : 0.0f;
     if (rel > K->res_max) K->res_max = rel;

@@ -322,17 +322,55 @@ class Rng:
         return np.float32(x >> 8) / np.float32(1 << 24)
 
 
+def _xorshift(x):
+    """One xorshift32 step (maps.rs:456-465) of every state in the uint32 array x."""
+    x = x ^ (x << np.uint32(13))
+    x = x ^ (x >> np.uint32(17))
+    return x ^ (x << np.uint32(5))
+
+
+_LANE = 4096
+_JUMP = None
+
+
+def _jump_columns():
+    """The xorshift32 step is linear over GF(2): the images of the 32 unit vectors under
+    _LANE steps are the columns of its _LANE-th power."""
+    global _JUMP
+    if _JUMP is None:
+        x = (np.uint32(1) << np.arange(32, dtype=np.uint32)).astype(np.uint32)
+        for _ in range(_LANE):
+            x = _xorshift(x)
+        _JUMP = [int(v) for v in x]
+    return _JUMP
+
+
 def _rng_stream(seed, n):
-    """n successive Rng(seed).next() values: (x >> 8) / 2^24 is exact in float32."""
+    """n successive Rng(seed).next() values: (x >> 8) / 2^24 is exact in float32. The
+    stream is cut into lanes of _LANE values; each lane's starting state is the seed moved
+    ahead by whole lanes (the step's _LANE-th power applied as a GF(2) matrix), and the
+    lanes then step side by side."""
     x = seed & 0xFFFFFFFF
-    out = [0] * n
-    M = 0xFFFFFFFF
-    for k in range(n):
-        x ^= (x << 13) & M
-        x ^= x >> 17
-        x ^= (x << 5) & M
-        out[k] = x >> 8
-    return (np.asarray(out, np.float64) * (1.0 / (1 << 24))).astype(np.float32)
+    lanes = max((n + _LANE - 1) // _LANE, 1)
+    cols = _jump_columns()
+    starts = np.empty(lanes, np.uint32)
+    for j in range(lanes):
+        starts[j] = x
+        y = 0
+        b = 0
+        while x:
+            if x & 1:
+                y ^= cols[b]
+            x >>= 1
+            b += 1
+        x = y
+    out = np.empty((_LANE, lanes), np.uint32)
+    v = starts
+    for k in range(_LANE):
+        v = _xorshift(v)
+        out[k] = v
+    vals = out.T.reshape(-1)[:n] >> np.uint32(8)
+    return (vals.astype(np.float64) * (1.0 / (1 << 24))).astype(np.float32)
 
 
 def new_particles(row, pos):

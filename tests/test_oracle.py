@@ -440,6 +440,21 @@ def test_fill_rect_count_and_walls():
     assert np.all((out["pos"][:, 0] > sc.WALL) & (out["pos"][:, 0] < 64 - sc.WALL) & (out["pos"][:, 1] > sc.WALL))
 
 
+def test_rng_stream_is_the_xorshift_sequence():
+    """The lane-parallel stream (jumps by a GF(2) matrix power) equals xorshift32 stepped one
+    value at a time (maps.rs:456-465), across lane boundaries, and maps.rs's first values
+    from the seed of the first fill."""
+    def scalar(seed, n):
+        r = sc.Rng(seed)
+        return np.array([r.next() for _ in range(n)], np.float32)
+
+    for seed, n in [(0x9E3779B9 ^ 1, 3), (12345, 4095), (0xDEADBEEF, 4097), (0x9E3779B9 ^ (7 * 747796405 + 1), 12289)]:
+        assert np.array_equal(sc._rng_stream(seed, n), scalar(seed, n)), (seed, n)
+    # xorshift32 from 1: 270369, 67634689, 2647435461 (Marsaglia 2003), then >> 8 / 2^24
+    want = np.array([270369 >> 8, 67634689 >> 8, 2647435461 >> 8], np.float64) / (1 << 24)
+    assert np.array_equal(sc._rng_stream(1, 3), want.astype(np.float32))
+
+
 def test_config1_dam_break_count():
     """384 x 216, block x < 0.4 W, y < 0.6 H with the 3-cell frame: 301 x 253 particles."""
     scene, parts = sc.dam_break((384, 216))

@@ -6,9 +6,12 @@ geometry:
     mean of curvature x distance to the centre is 1 (the curvature of a circle, 1 / r) within
     5 %, for two radii, positive for a convex blob;
   * a straight surface (a half-plane of liquid, at 0 and 30 degrees): no curvature in the band.
-A dropped half, a sign error or a stencil slip fails one of them. And of the explicit
-viscosity step (visc_setup + visc_apply mode 2): a shear wave in the bulk decays by the
-discrete Laplacian's Fourier factor, on the u and on the v faces.
+A dropped half, a sign error or a stencil slip fails one of them. Of the pressure jump
+(surf_apply.comp) the curvature drives, by Laplace's law at the reference's own bar
+(validate.rs:1270-1360): a resting water drop holds sigma / R inside at two radii, and an
+oil disc in water holds the oil-water pair tension over R. And of the explicit viscosity
+step (visc_setup + visc_apply mode 2): a shear wave in the bulk decays by the discrete
+Laplacian's Fourier factor, on the u and on the v faces.
 """
 import numpy as np
 import pytest
@@ -149,3 +152,86 @@ def test_collide_keeps_particles_out_of_the_floor():
     free = p["pos"][:, 1] - step > floor + 0.5  # ends well clear of the floor's margin
     assert np.allclose(q["pos"][free, 1], p["pos"][free, 1] - step, atol=1e-4)
     assert np.all(q["vel"][hit, 1] >= 0.0)  # no velocity into the floor
+
+
+# ---- the surface-tension pressure jump (surf_apply.comp), against Laplace's law -------------
+
+def capillary_case(kind, grid, rad):
+    """validate.rs:1270-1360 on a smaller grid: 1 mm cells, no gravity; a water drop of
+    radius `rad` at the centre of an empty tank, or an oil disc of that radius inside water
+    filling 260/288 of the tank."""
+    from sprout_amd import scene as sc
+    W, H = grid
+    params = N.default_params()
+    params.cell_size = 0.001
+    params.gravity = 0.0
+    c = (W / 2, H / 2)
+    scene = sc.Scene(grid)
+    if kind == "drop":
+        ps = sc.fill_rect(scene, (c[0] - rad, c[1] - rad, c[0] + rad, c[1] + rad), N.ROW_WATER)
+        ps = ps[np.hypot(ps["pos"][:, 0] - c[0], ps["pos"][:, 1] - c[1]) < rad]
+        rows = [N.ROW_WATER]
+    else:
+        ps = sc.fill_rect(scene, (0.0, 0.0, float(W), H * 260.0 / 288.0), N.ROW_WATER)
+        ps["kind"][np.hypot(ps["pos"][:, 0] - c[0], ps["pos"][:, 1] - c[1]) < rad] = N.ROW_OIL + 1
+        rows = [N.ROW_WATER, N.ROW_OIL]
+    ps = cases.settle(ps, params)
+    g, mats = N.derive_grid_params(params, W, H, rows)
+    g.n_slots = len(ps)
+    o = OracleSim(grid, W * H * 4)
+    o.set_gparams(g, mats)
+    o.set_solids(scene.rasterize())
+    o.reset(ps)
+    return o, g, params, c
+
+
+def run_for(o, g, seconds):
+    first = True
+    while True:
+        o.begin_batch(16 * g.dt_max, 16)
+        for _ in range(16):
+            o.substep(0 if first else 1)
+            first = False
+        S = o.field("STEP", N.STEP_DTYPE, 1)[0]
+        if S["advanced"] >= seconds:
+            return S
+
+
+def pressure_rings(o, S, c, W, H):
+    pres = o.field("PRES", np.float32, (W * H, 2))[:, 0].reshape(H, W).astype(np.float64) / float(S["dt"])
+    j, i = np.mgrid[0:H, 0:W]
+    return pres, np.hypot(i + 0.5 - c[0], j + 0.5 - c[1])
+
+
+@pytest.mark.parametrize("rad", [16.0, 24.0])
+def test_resting_drop_holds_the_laplace_pressure(rad):
+    """Inside a resting drop (the mean over r < R/2 after 0.1 s) the pressure is sigma / R
+    within 10 % (the reference's own bar, validate.rs:1301-1306), at two radii, so a jump
+    that does not scale as 1/R fails; the spurious currents stay under 3 % of
+    sqrt(sigma / (rho R))."""
+    W, H = 128, 96
+    o, g, params, c = capillary_case("drop", (W, H), rad)
+    S = run_for(o, g, 0.1)
+    pres, d = pressure_rings(o, S, c, W, H)
+    sigma = params.mat[N.ROW_WATER].tension / (1000.0 * params.cell_size ** 3)
+    ratio = pres[d < 0.5 * rad].mean() / (sigma / rad)
+    assert abs(ratio - 1.0) < 0.10, ratio
+    p = o.field("PARTICLES", N.PARTICLE_DTYPE, o.n_slots())
+    rms = np.sqrt(np.mean((p["vel"].astype(np.float64) ** 2).sum(1)))
+    assert rms / np.sqrt(sigma / rad) < 0.03
+
+
+def test_oil_disc_in_water_holds_the_pair_tension_jump():
+    """An oil disc inside water: the pressure inside (r < R/2) less that of a ring outside
+    (R + 8 <= r < R + 16) is the oil-water tension over R within 10 % (validate.rs:
+    1320-1355); the pair tension, not either liquid's own, sets it."""
+    W, H = 128, 96
+    rad = 16.0
+    o, g, params, c = capillary_case("oil", (W, H), rad)
+    S = run_for(o, g, 0.1)
+    pres, d = pressure_rings(o, S, c, W, H)
+    aq, oily = params.mat[N.ROW_WATER].group, params.mat[N.ROW_OIL].group
+    sigma = params.pair_tension[aq][oily] / (1000.0 * params.cell_size ** 3)
+    jump = pres[d < 0.5 * rad].mean() - pres[(d >= rad + 8) & (d < rad + 16)].mean()
+    assert abs(jump / (sigma / rad) - 1.0) < 0.10, jump / (sigma / rad)
+    assert abs(params.pair_tension[aq][oily] - params.mat[N.ROW_WATER].tension) > 0.01
