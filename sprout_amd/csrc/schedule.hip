@@ -59,13 +59,6 @@ dim3 grid0(const Run& r) { return groups_of(r, r.W, r.H); }
 Lv lv_of(const sp_levels_info& L, int l) { return Lv{L.offs[l], L.dims[l][0], L.dims[l][1], L.tile_offs[l]}; }
 dim3 grid_t0(const Run& r) { return dim3((r.W + T0 - 1) / T0, (r.H + T0 - 1) / T0, r.T); }
 int parts_t0(const Run& r) { return ((r.W + T0 - 1) / T0) * ((r.H + T0 - 1) / T0); }
-TailLevels tail_levels(const Run& r) {
-    TailLevels t{};
-    for (int l = 0; l < r.lv.n; l++) t.lev[l] = make_int4(r.lv.offs[l], r.lv.dims[l][0], r.lv.dims[l][1], r.lv.tile_offs[l]);
-    t.first = r.lv.tail;
-    t.last = r.lv.n - 1;
-    return t;
-}
 
 // The coarse levels 1 .. tail - 1 of a V-cycle, on the way down and on the way up: two
 // levels per launch, pairing from the coarsest (level 1, the largest, alone when their
@@ -103,7 +96,7 @@ void launch_vcycle_t(const Run& r, int reduce, int rsel) {
     const sp_levels_info& L = r.lv;
     hipLaunchKernelGGL(k_mg_down<H>, grid0(r), TILE, 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), 0, 0, 0, rsel);
     launch_coarse_down<H>(r, 0, 0);
-    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, tail_levels(r), 0, 0);
+    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 0, 0);
     launch_coarse_up<H>(r, 0, 0);
     hipLaunchKernelGGL(k_up0<H>, grid_t0(r), dim3(T0, T0), 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), parts_t0(r), 0, 0, reduce,
                        rsel);
@@ -151,7 +144,7 @@ void launch_pressure_t(const Run& r, uint32_t iters, bool second) {
         hipLaunchKernelGGL(k_cg_alpha, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, nbp, par);
         hipLaunchKernelGGL(k_update_down0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, par);
         launch_coarse_down<H>(r, 1, par);
-        hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, tail_levels(r), 1, par);
+        hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 1, par);
         launch_coarse_up<H>(r, 1, par);
         hipLaunchKernelGGL(k_up0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, nbp, 1, par, 1, par ^ 1);
     }

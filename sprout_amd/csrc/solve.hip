@@ -1160,25 +1160,18 @@ __global__ __launch_bounds__(NTF) void k_mg_up2(const TankBatch tb, Lv A, Lv B, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");   \
     } while (0)
 
-// The levels of the hierarchy (offset, width, height, tile offset) as a kernel argument, and
-// the first and last of the tail.
-struct TailLevels {
-    int4 lev[SP_MAX_LEVELS];
-    int first, last;
-};
-
 // LDS (dynamic, sized at launch to the hierarchy): per tail cell its couplings (float4),
 // iterate and right-hand side (float2) and diagonal; the first level's right-hand side is
 // loaded once with the matrix, so that the sweeps never wait on global memory.
 template <bool H>
-__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, const TailLevels tl, int gated, int par) {
+__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated, int par) {
     using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     extern __shared__ float4 tail_lds[];
     __shared__ int4 sl[SP_MAX_LEVELS];
     if (gated && slot_done(c, par)) return;
-    const int first = tl.first, last = tl.last;
-    const int4 Lf = tl.lev[first], Le = tl.lev[last];
+    const int first = c.tail, last = c.n_levels - 1;
+    const int4 Lf = c.lev[first], Le = c.lev[last];
     const int BASE = Lf.x;
     const int total = Le.x + Le.y * Le.z - BASE;
     const int NF = Lf.y * Lf.z;
@@ -1187,7 +1180,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, const Tail
     float2* sr = sx + total;
     float* sd = reinterpret_cast<float*>(sr + total);
     const int bt = threadIdx.x;
-    if (bt < SP_MAX_LEVELS) sl[bt] = tl.lev[bt];
+    if (bt <= last) sl[bt] = c.lev[bt];
     __syncthreads();
     for (int t = bt; t < total; t += 1024) {
         int l = first;  // the level of tail cell t (the storage of the coarse matrices is per level)
