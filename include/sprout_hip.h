@@ -91,6 +91,32 @@ int sp_advance_blocking(sp_sim* sim, double seconds);
 int sp_step_blocking(sp_sim* sim, uint32_t n);
 /* `FluidSim::sim_time` (src/sim/mod.rs:1956). */
 int sp_sim_time(sp_sim* sim, double* out);
+/* `FluidSim::frame_blocking(substeps, target, tool)` (src/sim/mod.rs:3342-3350): one
+ * interactive frame, recorded and waited for; *covered receives the simulated time it
+ * covered (s). covered may be NULL. */
+int sp_frame_blocking(sp_sim* sim, uint32_t substeps, float target_s, const sp_tool_state* tool, double* covered);
+/* `FluidSim::slots` (src/sim/mod.rs:1946): particle slots that may hold live particles. */
+int sp_slots(sp_sim* sim, uint32_t* out);
+/* `FluidSim::dt_estimate` (src/sim/mod.rs:1961): the substep length the next batch will
+ * most likely get (1 / fixed_hz when the rate is fixed), s. */
+int sp_dt_estimate(sp_sim* sim, float* out);
+/* `FluidSim::recorded_iters` (src/sim/mod.rs:1966): the iterations the next substeps
+ * record for the first pressure solve, the second and the viscosity solve. */
+int sp_recorded_iters(sp_sim* sim, uint32_t* out3);
+/* `FluidSim::substep_cost` (src/sim/mod.rs:1972, 3055-3059): device time of one substep,
+ * s, from events around each batch recorded by sp_record and waited for by sp_wait,
+ * smoothed as x += 0.1 (cost - x); 0 before the first measurement. */
+int sp_substep_cost(sp_sim* sim, float* out);
+/* `FluidSim::profile_passes` (src/sim/mod.rs:3120-3150): runs one substep (it advances the
+ * tank, as the reference's does) with device timestamps between its phases and returns
+ * each phase's time (ms) and the total: "step_clear", "particles", "grid", "pressure",
+ * "project", "viscosity", "pressure2_project", "total". At most cap entries go to out;
+ * *n receives the number of phases. Blocking. */
+int sp_profile_passes(sp_sim* sim, sp_pass_time* out, uint32_t cap, uint32_t* n);
+/* The material table in the reference's `GpuMat` layout (src/sim/material.rs:508-549),
+ * as `FluidSim::material_buffer` holds it after the latest record: at most cap rows of
+ * SP_MAX_MATERIALS. Host side; no device work. */
+int sp_material_table(sp_sim* sim, sp_gpumat* out, uint32_t cap);
 
 /* `FluidSim::download_particles` (src/sim/mod.rs:3421-3432): the live particles (dead
  * slots dropped), in slot order. out may be NULL to query the count; otherwise at most
@@ -100,6 +126,14 @@ int sp_download_particles(sp_sim* sim, sp_particle* out, uint32_t cap, uint32_t*
  * vel[4*W*H] (u, v, state u, state v) and cell[4*W*H] (occupancy, density, viscosity,
  * level). Either pointer may be NULL. */
 int sp_download_grid(sp_sim* sim, float* vel, float* cell);
+/* The rigid labels of `GridDump` (src/sim/mod.rs:3436-3450, `labels`): labels[2*W*H], per
+ * cell the root cell of its body (-1: not rigid) and, at a root, the body's slot. */
+int sp_download_labels(sp_sim* sim, int32_t* labels);
+/* `FluidSim::download_bodies` (src/sim/mod.rs:3451-3462): per body of the latest substep
+ * its mass, mass-weighted offset sums (x, y), momentum (x, y), angular momentum about the
+ * root and inertia sum (grid units), 7 doubles each. At most cap bodies are written to out
+ * (may be NULL with cap 0); *n receives the bodies the table holds. */
+int sp_download_bodies(sp_sim* sim, double* out, uint32_t cap, uint32_t* n);
 /* `FluidSim::download_pressure` (src/sim/mod.rs:3464-3475): p*dt per cell, and dt. */
 int sp_download_pressure(sp_sim* sim, float* pres, float* dt);
 /* `FluidSim::download_ledger` (src/sim/mod.rs:3389-3406): energy that crossed the boundary
@@ -165,6 +199,7 @@ enum {
     SP_RENDER_SURFACE = 4,      /* float4[W*H]: tension, contact normal, group + (row + 1) / 128 (surface_buffer) */
     SP_RENDER_MATERIALS = 5,    /* sp_gmat[SP_MAX_MATERIALS] (material_buffer, this library's row layout) */
     SP_RENDER_VELOCITY = 6,     /* float4[W*H]: u, v and the faces' states (the grid's velocity) */
+    SP_RENDER_GPUMAT = 7,       /* sp_gpumat[SP_MAX_MATERIALS] (material_buffer in the reference's GpuMat layout) */
     SP_RENDER_COUNT
 };
 

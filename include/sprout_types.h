@@ -216,6 +216,33 @@ typedef struct sp_stats_out {
     float conc[SP_MAX_MATERIALS];
 } sp_stats_out;
 
+/* One row of the reference's GPU material table in its own 208-byte layout (`GpuMat`,
+ * src/sim/material.rs:508-549, filled by gpu_materials, src/sim/mod.rs:2179-2236), for a
+ * renderer that reads `material_buffer`. The kernels of this library read `sp_gmat`; this
+ * table is the same numbers in the reference's order. Fields this build does not carry
+ * (residue share, fizz) are 0, the residue row -1. */
+typedef struct sp_gpumat {
+    int32_t ia[4];   /* phase group, diluent row, weight channel, concentration channel */
+    int32_t ib[4];   /* family, rheology model, temperature model, residue row */
+    float a[4];      /* relative density, ln nu (cells^2/s), tension / rho_w, mixing rate */
+    float b[4];      /* cos, sin of the contact angle, grain diameter (cells), relative density frozen */
+    float rheo[4];   /* yield stress / rho_w (cells^2/s^2), flow index, shear time, viscosity ratio */
+    float temp[4];   /* temperature law B, C (K), reference temperature (K), skin / rho_w */
+    float th[4];     /* heat capacity vs water, conductivity, solidus, liquidus */
+    float th2[4];    /* latent heat as degrees, emissivity, boiling point, vaporisation as degrees */
+    float el[4];     /* shear modulus / rho_w, relaxation time, susceptibility, residue share */
+    float misc[4];   /* freeze shift, load, viscosity blending exponent, expansion */
+    int32_t ic[4];   /* reaction partner row, product row, partner's index among the rows, 0 */
+    float rx[4];     /* reaction rate, its heat as degrees, self-heating (K/s), volatility */
+    float gr[4];     /* friction, fume, fizz, 0 */
+} sp_gpumat;
+
+/* One entry of sp_profile_passes: a phase of the substep and its device time. */
+typedef struct sp_pass_time {
+    char name[24];
+    double ms;
+} sp_pass_time;
+
 /* One material row in grid units (`GpuMat`, material.rs:508-547, filled at mod.rs:
  * 2179-2236), the fields the isothermal step reads. */
 typedef struct sp_gmat {
@@ -505,6 +532,7 @@ enum {
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
 SP_STATIC_ASSERT(sizeof(sp_gmat) == 192, "one grid-unit material row is 48 words");
+SP_STATIC_ASSERT(sizeof(sp_gpumat) == 208, "GpuMat is 208 bytes (material.rs:549)");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 SP_STATIC_ASSERT(sizeof(sp_cg_state) == 112, "CG state: 8 doubles, 12 words, no padding");
 SP_STATIC_ASSERT(sizeof(sp_solve) == 240 + 2 * 112, "solve record layout (mirrored by the Python binding)");

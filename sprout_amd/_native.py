@@ -195,7 +195,7 @@ class DeviceView(C.Structure):
                 ("device", C.c_int32), ("pad", C.c_uint32), ("generation", C.c_uint64)]
 
 
-RENDER = dict(PARTICLES=0, SDF=1, CELL=2, COMPOSITION=3, SURFACE=4, MATERIALS=5, VELOCITY=6)
+RENDER = dict(PARTICLES=0, SDF=1, CELL=2, COMPOSITION=3, SURFACE=4, MATERIALS=5, VELOCITY=6, GPUMAT=7)
 
 
 class LevelsInfo(C.Structure):
@@ -232,6 +232,18 @@ SOLVE_DTYPE = np.dtype([("rz", "<f8", (2,)), ("alpha", "<f8", (2,)), ("beta", "<
                         ("st", CG_STATE_DTYPE, (2,))])
 assert SOLVE_DTYPE.itemsize == 464
 
+# sp_gpumat: a row of the reference's GpuMat table (material.rs:508-549), 208 bytes.
+GPUMAT_DTYPE = np.dtype([("ia", "<i4", (4,)), ("ib", "<i4", (4,)), ("a", "<f4", (4,)), ("b", "<f4", (4,)),
+                         ("rheo", "<f4", (4,)), ("temp", "<f4", (4,)), ("th", "<f4", (4,)), ("th2", "<f4", (4,)),
+                         ("el", "<f4", (4,)), ("misc", "<f4", (4,)), ("ic", "<i4", (4,)), ("rx", "<f4", (4,)),
+                         ("gr", "<f4", (4,))])
+assert GPUMAT_DTYPE.itemsize == 208
+
+
+class PassTime(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("ms", C.c_double)]
+
+
 # Exported symbols of include/sprout_hip.h, with their argument types.
 _P = C.c_void_p
 SIGNATURES = {
@@ -249,6 +261,15 @@ SIGNATURES = {
     "sp_advance_blocking": ([_P, C.c_double], C.c_int),
     "sp_step_blocking": ([_P, C.c_uint32], C.c_int),
     "sp_sim_time": ([_P, C.POINTER(C.c_double)], C.c_int),
+    "sp_frame_blocking": ([_P, C.c_uint32, C.c_float, C.POINTER(ToolState), C.POINTER(C.c_double)], C.c_int),
+    "sp_slots": ([_P, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_dt_estimate": ([_P, C.POINTER(C.c_float)], C.c_int),
+    "sp_recorded_iters": ([_P, _P], C.c_int),
+    "sp_substep_cost": ([_P, C.POINTER(C.c_float)], C.c_int),
+    "sp_profile_passes": ([_P, C.POINTER(PassTime), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
+    "sp_material_table": ([_P, _P, C.c_uint32], C.c_int),
+    "sp_download_labels": ([_P, _P], C.c_int),
+    "sp_download_bodies": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_download_particles": ([_P, _P, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
     "sp_download_grid": ([_P, _P, _P], C.c_int),
     "sp_download_pressure": ([_P, _P, C.POINTER(C.c_float)], C.c_int),

@@ -103,6 +103,12 @@ struct sp_sim {
 
     GraphSet gs;
     Timing tm;
+    // substep_cost (mod.rs:3055-3059): events around the latest batch sp_record enqueued
+    hipEvent_t batch_ev[2] = {nullptr, nullptr};
+    uint32_t batch_substeps = 0;  // substeps between them; 0: nothing to measure
+    float substep_cost = 0.0f;    // s, smoothed
+    sp_gpumat* d_gpumat = nullptr;  // the material table in GpuMat layout (SP_RENDER_GPUMAT)
+    sp_gpumat gpumat[SP_MAX_MATERIALS]{};
     bool in_batch = false;  // a batch drives the tank (sp_batch_*)
     bool pooled = false;    // a pool drives the tank (sp_pool_*)
 
@@ -158,6 +164,34 @@ void launch_claim(sp_sim* s, uint32_t claim, hipStream_t stream) {
 }
 
 // Uploads the grid-unit tables if the parameters or the tank changed.
+// A row of this library's material table in the reference's GpuMat order
+// (gpu_materials, mod.rs:2202-2233). Residue share and fizz are not carried (0).
+sp_gpumat to_gpumat(const sp_gmat& g, const sp_material& m) {
+    sp_gpumat o;
+    memset(&o, 0, sizeof(o));
+    const int32_t ia[4] = {g.group, g.diluent, g.chan_w, g.chan_s}, ib[4] = {g.fam, g.rheo, g.temp_model, g.residue};
+    const float a[4] = {g.rho, g.ln_nu, g.sigma, g.mix_rate}, b[4] = {g.cos_c, g.sin_c, g.grain, g.rho_solid};
+    const float el[4] = {g.gr[1], g.gr[2], g.gr[3], 0.0f};
+    const float misc[4] = {g.freeze_shift, g.load, g.visc_exp, g.expansion};
+    const float gr[4] = {g.gr[0], m.fume, 0.0f, 0.0f};
+    for (int k = 0; k < 4; k++) {
+        o.ia[k] = ia[k];
+        o.ib[k] = ib[k];
+        o.a[k] = a[k];
+        o.b[k] = b[k];
+        o.rheo[k] = g.rheo_p[k];
+        o.temp[k] = g.temp[k];
+        o.th[k] = g.th[k];
+        o.th2[k] = g.th2[k];
+        o.el[k] = el[k];
+        o.misc[k] = misc[k];
+        o.ic[k] = g.ic[k];
+        o.rx[k] = g.rx[k];
+        o.gr[k] = gr[k];
+    }
+    return o;
+}
+
 int upload_tables(sp_sim* s) {
     if (!s->tables_dirty) return SP_OK;
     if (s->thermal) {
@@ -182,6 +216,9 @@ int upload_tables(sp_sim* s) {
     }
     HIPCHK(hipMemcpyAsync((void*)s->c.P, &s->gp, sizeof(sp_gparams), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemcpyAsync((void*)s->c.mats, s->gm, sizeof(s->gm), hipMemcpyHostToDevice, s->stream));
+    for (int r = 0; r < SP_MAX_MATERIALS; r++) s->gpumat[r] = to_gpumat(s->gm[r], s->params.mat[r]);
+    if (s->d_gpumat)
+        HIPCHK(hipMemcpyAsync(s->d_gpumat, s->gpumat, sizeof(s->gpumat), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     s->tables_dirty = false;
     s->run.gp = s->gp;
@@ -378,10 +415,15 @@ void maybe_sort(sp_sim* s, hipStream_t stream) {
 int record(sp_sim* s, uint32_t substeps, float target) {
     int rc = record_prepare(s, substeps, target);
     if (rc != SP_OK) return rc;
+    if (substeps > 0 && s->batch_ev[0]) HIPCHK(hipEventRecord(s->batch_ev[0], s->stream));
     for (uint32_t k = 0; k < substeps; k++) {
         maybe_sort(s, s->stream);
         launch_substep(s->run, &s->gs, &s->tm, s->grid_valid ? 1 : 0, s->rec);
         s->grid_valid = true;
+    }
+    if (substeps > 0 && s->batch_ev[1]) {
+        HIPCHK(hipEventRecord(s->batch_ev[1], s->stream));
+        s->batch_substeps = substeps;
     }
     return SP_OK;
 }
@@ -389,6 +431,14 @@ int record(sp_sim* s, uint32_t substeps, float target) {
 int wait(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
     s->tm.collect();
+    if (s->batch_substeps > 0) {
+        // the latest batch's device time per substep, smoothed over about ten batches
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, s->batch_ev[0], s->batch_ev[1]));
+        const float cost = ms * 1e-3f / (float)s->batch_substeps;
+        s->substep_cost = s->substep_cost > 0.0f ? s->substep_cost + 0.1f * (cost - s->substep_cost) : cost;
+        s->batch_substeps = 0;
+    }
     readback(s);
     return SP_OK;
 }
@@ -521,6 +571,8 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         HIPCHK(hipSetDevice(s->device));
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->tm.create();
+        HIPCHK(hipEventCreate(&s->batch_ev[0]));
+        HIPCHK(hipEventCreate(&s->batch_ev[1]));
         sp::levels(s->W, s->H, s->lv);
         s->nb0 = ((s->W + 15) / 16) * ((s->H + 15) / 16);
         {
@@ -548,6 +600,7 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         for (int l = 0; l < s->lv.n; l++) h_lev[l] = make_int4(s->lv.offs[l], s->lv.dims[l][0], s->lv.dims[l][1], s->lv.tile_offs[l]);
         HIPCHK(hipMemcpy((void*)c.lev, h_lev, sizeof(h_lev), hipMemcpyHostToDevice));
         s->d_count_out = dalloc<uint32_t>(N, o);
+        s->d_gpumat = dalloc<sp_gpumat>(SP_MAX_MATERIALS, o);
         // solid field: closed everywhere until set_solids
         HIPCHK(hipDeviceSynchronize());
         sp::default_params(&s->params);
@@ -585,6 +638,8 @@ void sp_destroy(sp_sim* s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->gs.clear();
+    for (hipEvent_t& e : s->batch_ev)
+        if (e) (void)hipEventDestroy(e);
     free_big(s);
     for (void* p : s->owned) (void)hipFree(p);
     if (s->host_arena) (void)hipHostFree(s->host_arena);
@@ -824,6 +879,129 @@ int sp_sim_time(sp_sim* s, double* out) {
         *out = s->sim_time;
         return SP_OK;
     }, false);
+}
+
+int sp_frame_blocking(sp_sim* s, uint32_t substeps, float target_s, const sp_tool_state* tool, double* covered) {
+    return guarded(s, [&]() -> int {
+        if (tool && tool->mode != 0) {
+            set_err(s, "tools are not carried by this build (mode %d)", tool->mode);
+            return SP_E_UNSUPPORTED;
+        }
+        const double start = s->sim_time;
+        int rc = record(s, substeps, target_s);
+        if (rc != SP_OK) return rc;
+        rc = wait(s);
+        if (rc != SP_OK) return rc;
+        if (covered) *covered = s->sim_time - start;
+        return SP_OK;
+    });
+}
+
+int sp_slots(sp_sim* s, uint32_t* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        *out = s->n_slots;
+        return SP_OK;
+    }, false);
+}
+
+int sp_dt_estimate(sp_sim* s, float* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        *out = s->params.fixed_hz > 0.0f ? 1.0f / s->params.fixed_hz : s->dt_estimate;
+        return SP_OK;
+    }, false);
+}
+
+int sp_recorded_iters(sp_sim* s, uint32_t* out3) {
+    return guarded(s, [&]() -> int {
+        if (!out3) return SP_E_INVALID;
+        out3[0] = s->rec.p;
+        out3[1] = s->rec.p2;
+        out3[2] = s->rec.v;
+        return SP_OK;
+    }, false);
+}
+
+int sp_substep_cost(sp_sim* s, float* out) {
+    return guarded(s, [&]() -> int {
+        if (!out) return SP_E_INVALID;
+        *out = s->substep_cost;
+        return SP_OK;
+    }, false);
+}
+
+int sp_profile_passes(sp_sim* s, sp_pass_time* out, uint32_t cap, uint32_t* n) {
+    return guarded(s, [&]() -> int {
+        if (!n || (cap > 0 && !out)) return SP_E_INVALID;
+        // what earlier batches timed is collected first, so the phases are this substep's
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->tm.collect();
+        const float dt = s->params.fixed_hz > 0.0f ? 1.0f / s->params.fixed_hz : s->dt_estimate;
+        int rc = record(s, 1, dt);
+        if (rc != SP_OK) return rc;
+        rc = wait(s);
+        if (rc != SP_OK) return rc;
+        const float* ph = s->tm.phase;
+        const float part = s->tm.particle_ms;
+        const char* names[8] = {"step_clear", "particles", "grid", "pressure", "project", "viscosity", "pressure2_project",
+                                "total"};
+        const double ms[8] = {std::max(ph[1] - part, 0.0f), part, ph[2], ph[3], ph[4], ph[5], ph[6], ph[0]};
+        *n = 8;
+        for (uint32_t k = 0; k < cap && k < 8; k++) {
+            memset(out[k].name, 0, sizeof(out[k].name));
+            strncpy(out[k].name, names[k], sizeof(out[k].name) - 1);
+            out[k].ms = ms[k];
+        }
+        return SP_OK;
+    });
+}
+
+int sp_material_table(sp_sim* s, sp_gpumat* out, uint32_t cap) {
+    return guarded(s, [&]() -> int {
+        if (cap > 0 && !out) return SP_E_INVALID;
+        if (s->tables_dirty) {
+            // the table as the next record derives it, without touching the device
+            sp_gparams g;
+            sp_gmat gm[SP_MAX_MATERIALS];
+            std::string why;
+            int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, g, gm, &why, s->magnets, s->spawn_speed);
+            if (rc != SP_OK) {
+                set_err(s, "%s", why.c_str());
+                return rc;
+            }
+            for (uint32_t r = 0; r < cap && r < (uint32_t)SP_MAX_MATERIALS; r++) out[r] = to_gpumat(gm[r], s->params.mat[r]);
+            return SP_OK;
+        }
+        for (uint32_t r = 0; r < cap && r < (uint32_t)SP_MAX_MATERIALS; r++) out[r] = s->gpumat[r];
+        return SP_OK;
+    }, false);
+}
+
+int sp_download_labels(sp_sim* s, int32_t* labels) {
+    return guarded(s, [&]() -> int {
+        if (!labels) return SP_E_INVALID;
+        HIPCHK(hipMemcpyAsync(labels, s->c.rl, sizeof(int2) * (size_t)s->N, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return SP_OK;
+    });
+}
+
+int sp_download_bodies(sp_sim* s, double* out, uint32_t cap, uint32_t* n) {
+    return guarded(s, [&]() -> int {
+        if (!n || (cap > 0 && !out)) return SP_E_INVALID;
+        std::vector<uint32_t> w(SP_RIGID_WORDS);
+        HIPCHK(hipMemcpyAsync(w.data(), s->c.rbody, sizeof(uint32_t) * SP_RIGID_WORDS, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        const uint32_t nb = std::min<uint32_t>(w[0], SP_RIGID_BODIES);
+        auto get = [&](size_t k) { return (double)(int64_t)((uint64_t)w[k] | (uint64_t)w[k + 1] << 32) / 65536.0; };
+        for (uint32_t b = 0; b < nb && b < cap; b++) {
+            const size_t o = 16 + (size_t)SP_BODY_WORDS * b;
+            for (int q = 0; q < 7; q++) out[7 * (size_t)b + q] = get(o + 2 * q);
+        }
+        *n = nb;
+        return SP_OK;
+    });
 }
 
 int sp_download_particles(sp_sim* s, sp_particle* out, uint32_t cap, uint32_t* n) {
@@ -1094,6 +1272,14 @@ int render_buffer(sp_sim* s, int32_t which, void** ptr, uint32_t* count, uint32_
         break;
     }
     case SP_RENDER_VELOCITY: *ptr = c.vel; *count = (uint32_t)s->N; *stride = sizeof(float4); break;
+    case SP_RENDER_GPUMAT: {
+        const int rc = upload_tables(s);
+        if (rc != SP_OK) return rc;
+        *ptr = (void*)s->d_gpumat;
+        *count = SP_MAX_MATERIALS;
+        *stride = sizeof(sp_gpumat);
+        break;
+    }
     default: set_err(s, "unknown render buffer %d", which); return SP_E_INVALID;
     }
     return SP_OK;

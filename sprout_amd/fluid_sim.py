@@ -161,7 +161,7 @@ class FluidSim:
     # -- render-facing outputs (mod.rs:1608-1943) ---------------------------------------
     def render_view(self, which):
         """The device view (address, bytes, count, stride, device, generation) of a render
-        buffer: "PARTICLES", "SDF", "CELL", "COMPOSITION", "SURFACE", "MATERIALS", "VELOCITY"."""
+        buffer: "PARTICLES", "SDF", "CELL", "COMPOSITION", "SURFACE", "MATERIALS", "VELOCITY", "GPUMAT"."""
         v = N.DeviceView()
         self._check(self._lib.sp_render_view(self._h, N.RENDER[which], C.byref(v)), f"render_view {which}")
         return v
@@ -207,6 +207,70 @@ class FluidSim:
         cell = np.zeros((H, W, 4), np.float32)
         self._check(self._lib.sp_download_grid(self._h, vel.ctypes.data, cell.ctypes.data), "download_grid")
         return vel, cell
+
+    def download_labels(self):
+        """The rigid `labels` of GridDump (mod.rs:3436-3450): [H,W,2] int32, per cell the root
+        cell of its body (-1: not rigid) and, at a root, the body's slot."""
+        W, H = self.grid
+        lab = np.zeros((H, W, 2), np.int32)
+        self._check(self._lib.sp_download_labels(self._h, lab.ctypes.data), "download_labels")
+        return lab
+
+    def download_bodies(self):
+        """FluidSim::download_bodies (mod.rs:3451-3462): [bodies, 7] float64 (mass, offset sums
+        x, y, momentum x, y, angular momentum about the root, inertia sum)."""
+        n = C.c_uint32()
+        self._check(self._lib.sp_download_bodies(self._h, None, 0, C.byref(n)), "download_bodies")
+        out = np.zeros((n.value, 7), np.float64)
+        if n.value:
+            self._check(self._lib.sp_download_bodies(self._h, out.ctypes.data, n.value, C.byref(n)), "download_bodies")
+        return out[:n.value]
+
+    def slots(self):
+        """FluidSim::slots (mod.rs:1946)."""
+        v = C.c_uint32()
+        self._check(self._lib.sp_slots(self._h, C.byref(v)), "slots")
+        return v.value
+
+    def dt_estimate(self):
+        """FluidSim::dt_estimate (mod.rs:1961)."""
+        v = C.c_float()
+        self._check(self._lib.sp_dt_estimate(self._h, C.byref(v)), "dt_estimate")
+        return v.value
+
+    def recorded_iters(self):
+        """FluidSim::recorded_iters (mod.rs:1966): [pressure, second pressure, viscosity]."""
+        v = (C.c_uint32 * 3)()
+        self._check(self._lib.sp_recorded_iters(self._h, v), "recorded_iters")
+        return [int(x) for x in v]
+
+    def substep_cost(self):
+        """FluidSim::substep_cost (mod.rs:1972): device s per substep, smoothed; 0 before the
+        first measured batch."""
+        v = C.c_float()
+        self._check(self._lib.sp_substep_cost(self._h, C.byref(v)), "substep_cost")
+        return v.value
+
+    def frame_blocking(self, substeps, target, tool=None):
+        """FluidSim::frame_blocking (mod.rs:3342-3350): the simulated time covered, s."""
+        v = C.c_double()
+        t = C.byref(tool) if tool is not None else None
+        self._check(self._lib.sp_frame_blocking(self._h, substeps, target, t, C.byref(v)), "frame_blocking")
+        return v.value
+
+    def profile_passes(self):
+        """FluidSim::profile_passes (mod.rs:3120-3150): one substep, [(phase, ms)], with the
+        total last."""
+        out = (N.PassTime * 16)()
+        n = C.c_uint32()
+        self._check(self._lib.sp_profile_passes(self._h, out, 16, C.byref(n)), "profile_passes")
+        return [(out[k].name.decode(), out[k].ms) for k in range(n.value)]
+
+    def material_table(self):
+        """The GpuMat rows of material_buffer (material.rs:508-549) as a structured array."""
+        out = np.zeros(N.MAX_MATERIALS, N.GPUMAT_DTYPE)
+        self._check(self._lib.sp_material_table(self._h, out.ctypes.data, N.MAX_MATERIALS), "material_table")
+        return out
 
     def download_pressure(self):
         """FluidSim::download_pressure (mod.rs:3464-3475): (p*dt per cell [H,W], dt)."""

@@ -44,12 +44,16 @@ def test_python_structs_match_the_header(tmp_path):
     import subprocess
     pairs = {"sp_material": N.Material, "sp_sim_params": N.SimParams, "sp_gmat": N.GMat,
              "sp_gparams": N.GParams, "sp_solver_stats": N.SolverStats, "sp_levels_info": N.LevelsInfo,
-             "sp_pool_stats": N.PoolStats, "sp_magnet": N.Magnet, "sp_device_view": N.DeviceView}
+             "sp_pool_stats": N.PoolStats, "sp_magnet": N.Magnet, "sp_device_view": N.DeviceView,
+             "sp_pass_time": N.PassTime}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sprout_hip.h"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for fname, _ in py._fields_:
             lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append('printf("sp_gpumat %zu\\n", sizeof(sp_gpumat));')
+    for fname in N.GPUMAT_DTYPE.names:
+        lines.append(f'printf("sp_gpumat.{fname} %zu\\n", offsetof(sp_gpumat, {fname}));')
     lines.append("return 0; }")
     src = tmp_path / "layout.c"
     src.write_text("\n".join(lines))
@@ -60,6 +64,9 @@ def test_python_structs_match_the_header(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
+    assert int(got["sp_gpumat"]) == N.GPUMAT_DTYPE.itemsize == 208
+    for fname in N.GPUMAT_DTYPE.names:
+        assert int(got[f"sp_gpumat.{fname}"]) == N.GPUMAT_DTYPE.fields[fname][1], f"sp_gpumat.{fname}"
 
 
 def test_errors_cross_the_abi_as_codes():
