@@ -15,9 +15,14 @@
  *   pressure solve + project_visc.comp, heat.comp, rheo.comp, rigid.comp, elastic.comp,
  *   settle.comp, magnet.comp, sort.comp, count.comp, stats.comp.
  *
- * Parity unpinned (restated only, with no closed-form or golden check of their own): the
- * surface-tension jump, the drift-flux and family-exchange branches, the crowd jitter, the
- * heat of reaction and self-heating, and the magnetic resting rule of grid_prep.
+ * Every branch above has a check of its own on the CPU (closed forms, invariants, brute
+ * force): tests/test_oracle*.py. The last ones pinned: the surface-tension jump by Laplace's
+ * law (test_oracle_surface.py), the drift flux between and within phase groups by the
+ * terminal velocity of the drag balance and the cancelling volume flux, the family exchange
+ * by its relaxation rate and conservation, the crowd jitter by its bounds and uniformity
+ * (test_oracle_drift.py), self-heating and the heat of reaction by the energy they add
+ * (test_oracle_react.py), and the magnetic resting rule by the direction of the pull
+ * (test_oracle_magnet.py).
  *
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG

@@ -106,12 +106,12 @@ def test_material_table_in_the_reference_layout():
             assert np.array_equal(t["th"][r], np.float32(list(m.th)))
             assert np.array_equal(t["rx"][r], np.float32(list(m.rx)))
         # gpu_materials (mod.rs:2190-2207): an isothermal tank's relative density is the density
-        # at the ambient temperature (the expansion folded in) over water's
+        # at the ambient temperature (the expansion folded in) over the reference water density
         p = sim.params
         for r in (N.ROW_WATER, N.ROW_MERCURY):
             m = p.mat[r]
             rho = m.density * min(max(1.0 - m.expansion * (p.ambient_temp - m.temp_ref), 0.7), 1.3)
-            assert t["a"][r][0] == pytest.approx(rho / 1000.0, rel=1e-6)
+            assert t["a"][r][0] == pytest.approx(rho / 998.0, rel=1e-6)  # SimParams::WATER_DENSITY
         # the device view holds the same table
         v = sim.render_view("GPUMAT")
         assert v.count == N.MAX_MATERIALS and v.stride == 208

@@ -468,3 +468,73 @@ def test_rasterize_frame():
     assert np.all(solid[5:25, 5:35, :2] == 1)
     region = f.region.reshape(30, 40)
     assert np.count_nonzero(region & sc.REGION_VENT) == 1
+
+
+def test_grid_tables_scale_with_their_units():
+    """The grid-unit tables (gpu_params / gpu_materials, mod.rs:2179-2330) by dimensional
+    analysis rather than re-derivation: with cells a quarter the size, gravity (cells/s^2)
+    grows 4x, kinematic viscosity (cells^2/s) 16x, tension over water's density (cells^3/s^2)
+    64x, a yield stress or shear modulus over density (cells^2/s^2) 16x, a grain diameter
+    (cells) 4x and a conductivity (cells^2/s) 16x, while relative densities and heat
+    capacities stay; a liquid 1.5x as dense has 1.5x the relative density and volumetric heat
+    capacity; a viscosity 10x as large adds ln 10 to ln nu; the tolerances and step factors pass
+    through unchanged."""
+    rows = [N.ROW_WATER, N.ROW_HONEY, N.ROW_OIL, N.ROW_MERCURY]
+
+    def tables(dx, edit=None):
+        p = N.default_params()
+        p.cell_size = dx
+        h = p.mat[N.ROW_HONEY]
+        h.yield_stress, h.shear_modulus, h.relax_time, h.grain = 3.0, 40.0, 0.5, 2e-4
+        for (row, field), v in (edit or {}).items():
+            setattr(p.mat[row], field, v)
+        return p, *N.derive_grid_params(p, 96, 64, rows)
+
+    p, g, m = tables(0.004)
+    _, g4, m4 = tables(0.001)
+    assert g4.gravity / g.gravity == pytest.approx(4.0, rel=1e-6)
+    assert g.gravity == pytest.approx(9.81 / 0.004, rel=1e-6)
+    for r in rows:
+        assert np.exp(m4[r].ln_nu - m[r].ln_nu) == pytest.approx(16.0, rel=1e-5)
+        assert m4[r].sigma / m[r].sigma == pytest.approx(64.0, rel=1e-5)
+        assert m4[r].th[1] / m[r].th[1] == pytest.approx(16.0, rel=1e-5)
+        assert m4[r].rho == m[r].rho and m4[r].th[0] == m[r].th[0]
+        assert (m4[r].cos_c, m4[r].sin_c) == (m[r].cos_c, m[r].sin_c)
+    h = N.ROW_HONEY
+    assert m4[h].rheo_p[0] / m[h].rheo_p[0] == pytest.approx(16.0, rel=1e-5)
+    assert m4[h].gr[1] / m[h].gr[1] == pytest.approx(16.0, rel=1e-5)
+    assert m4[h].grain / m[h].grain == pytest.approx(4.0, rel=1e-5)
+    assert m[h].grain == pytest.approx(2e-4 / 0.004, rel=1e-6)
+    # absolute values at 4 mm cells (SI over the cell): tension over water's density, yield
+    # stress and modulus over water's density
+    assert m[N.ROW_WATER].sigma == pytest.approx(p.mat[N.ROW_WATER].tension / (998.0 * 0.004 ** 3), rel=1e-5)
+    assert m[h].rheo_p[0] == pytest.approx(3.0 / (998.0 * 0.004 ** 2), rel=1e-5)
+    assert m[h].gr[1] == pytest.approx(40.0 / (998.0 * 0.004 ** 2), rel=1e-5)
+    o = N.ROW_OIL
+    _, _, md = tables(0.004, {(o, "density"): 1.5 * p.mat[o].density})
+    assert md[o].rho / m[o].rho == pytest.approx(1.5, rel=1e-6)
+    assert md[o].th[0] / m[o].th[0] == pytest.approx(1.5, rel=1e-6)
+    _, _, mv = tables(0.004, {(o, "viscosity"): 10.0 * p.mat[o].viscosity})
+    assert mv[o].ln_nu - m[o].ln_nu == pytest.approx(np.log(10.0), abs=2e-6)
+    assert (g.tol[0], g.tol[1], g.tol[2]) == (np.float32(p.pressure_tol), np.float32(p.correction_tol),
+                                               np.float32(p.viscosity_tol))
+    assert g.cfl == np.float32(p.cfl) and g.sep_tau == np.float32(p.droplet_cells)
+
+
+def test_capillary_step_limit_is_the_capillary_time():
+    """The longest substep a tank allows (dt_capillary, params.rs:962-982) is half the capillary
+    time sqrt(rho dx^3 / (2 pi sigma)) of its worst surface (largest sigma / rho, a pair's
+    interfacial tension over the lighter liquid), in SI, where that is below dt_max."""
+    p = N.default_params()
+    p.cell_size = 5e-4
+    g, _ = N.derive_grid_params(p, 64, 64, [N.ROW_WATER])
+    w = p.mat[N.ROW_WATER]
+    cap = 0.5 * np.sqrt(w.density * p.cell_size ** 3 / (2 * np.pi * w.tension))
+    assert cap < p.dt_max
+    assert g.dt_max == pytest.approx(cap, rel=1e-5)
+    # water and oil: the pair's tension over the oil's density may be the worst surface
+    g2, _ = N.derive_grid_params(p, 64, 64, [N.ROW_WATER, N.ROW_OIL])
+    o = p.mat[N.ROW_OIL]
+    worst = max(w.tension / w.density, o.tension / o.density,
+                p.pair_tension[0][o.group] / min(w.density, o.density))
+    assert g2.dt_max == pytest.approx(0.5 * np.sqrt(p.cell_size ** 3 / (2 * np.pi * worst)), rel=1e-5)

@@ -183,3 +183,50 @@ def test_pool_rises_towards_the_magnet():
     # under the magnet the surface rises; far from it, it does not
     assert p["pos"][mid, 1].max() > top0 + 1.0
     assert np.percentile(p["pos"][far, 1], 99) < top0 + 0.6
+
+
+def resting_level(pull_x, with_magnet=True):
+    """grid_prep's level of a cell a little under half full of ferrofluid beside a packed
+    column of it (cell weights put straight into the accumulator: the column at x = 20, the
+    cell at (21, 15), nothing below it), with the last substep's magnetic energy density
+    psi = A x, A chosen so that the horizontal pull -km dpsi/dx is `pull_x` times gravity
+    (positive: to the right)."""
+    grid = (48, 32)
+    W, H = grid
+    params, field, parts, rows, magnets = cases.ferro_pool(grid)
+    o, g, mats = make(grid, (params, field, parts, rows, magnets if with_magnet else []))
+    assert bool(g.caps & N.CAP_MAGNET) == with_magnet
+    o.begin_batch(1 / 240, 1)
+    o.stage("STEP")
+    acc = np.zeros((H, W, g.acc_stride), np.int32)
+    acc[10:21, 20, 4] = int(1.0 * N.PPC * 1048576)
+    phi = 0.4
+    acc[15, 21, 4] = int(phi * N.PPC * 1048576)
+    o.put_field("ACC", acc.reshape(-1))
+    row = cases.ROW_FERRO
+    chi = mats[row].gr[3]
+    rho = mats[row].rho
+    km = chi / g.mag[1] / max(rho, 0.05) if with_magnet else 1.0
+    A = -pull_x * g.gravity / km
+    x = np.arange(W, dtype=np.float64)
+    magc = np.zeros((H, W, 2), np.float32)
+    magc[..., 1] = (A * x)[None, :]
+    o.put_field("MAGC", magc.reshape(-1, 2))
+    o.stage("GRID_PREP")
+    cell = o.field("CELL", np.float32, (H, W, 4))
+    return float(cell[15, 21, 3]), float(cell[15, 21, 0]), float(cell[15, 20, 3])
+
+
+def test_magnetic_resting_rule():
+    """grid_prep.comp:189-222: a cell not full enough to count as liquid rests on a packed
+    neighbour when the pull on it points there; with a magnet the pull is gravity plus the
+    magnetic force -chi/rho grad psi of the last substep (211-221), along its larger component.
+    Here the cell's only packed neighbour is on its left: pulled left harder than gravity it
+    rests (level 0.05), pulled right, or left but weaker than gravity (so down, onto nothing),
+    or with no magnet in the tank, it keeps its own level phi - 1/2."""
+    lvl, phi, col = resting_level(-4.0)
+    assert col > 0.0 and 0.25 < phi <= 0.5
+    assert lvl == np.float32(0.05)
+    for pull, mag in ((+4.0, True), (-0.25, True), (-4.0, False)):
+        lvl, phi, _ = resting_level(pull, mag)
+        assert lvl == pytest.approx(phi - 0.5, abs=1e-6), (pull, mag)
