@@ -61,21 +61,30 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 CHUNK = 16
-PMC_SUMMARY = {"dam": os.path.join(ROOT, "profiles", "pressure_dam_break_1024x576_pmc.md")}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_1024x576_pmc.md")
 
 
-def pmc_traffic(kind, W, H, mgp):
-    """HBM bytes per pressure solve from the committed FETCH_SIZE + WRITE_SIZE summary of
-    this workload (each counter scaled by what it reads for a plain copy of known size), or
-    None where none was measured."""
-    path = PMC_SUMMARY.get(kind)
-    if (W, H) != (1024, 576) or not path or not os.path.exists(path):
-        return None
-    key = "traffic_bytes_per_solve_" + ("mixed" if mgp else "fp32") + ":"
-    for line in open(path):
-        if line.startswith(key):
-            return float(line.split(":")[1])
-    return None
+def pmc_traffic(W, H, mgp, active_cells, iters_mean):
+    """HBM bytes of one pressure solve from the committed FETCH_SIZE + WRITE_SIZE summary of
+    the same kernels at 1024 x 576 (each counter scaled by what it reads for a plain copy of
+    known size): the set-up's bytes plus the measured bytes per active cell and CG iteration
+    times this solve's active cells and mean iterations. None where none was measured."""
+    if (W, H) != (1024, 576) or not mgp or not os.path.exists(PMC_SUMMARY):
+        return None, None
+    vals = {}
+    for line in open(PMC_SUMMARY):
+        if line.startswith("traffic_") and ":" in line:
+            k, v = line.split(":", 1)
+            vals[k.strip()] = float(v)
+    try:
+        setup = vals["traffic_setup_bytes_per_solve_mixed"]
+        per = vals["traffic_bytes_per_active_cell_iteration_mixed"]
+    except KeyError:
+        return None, None
+    return setup + per * active_cells * iters_mean, (
+        "FETCH_SIZE + WRITE_SIZE of the pressure-solve kernels on the 1024x576 dam break "
+        "(profiles/hbm_traffic_dam_break_1024x576_pmc.md): set-up bytes per solve plus bytes per "
+        "active cell and iteration, scaled to this solve's active cells and mean iterations")
 
 
 def workload(kind, grid, seed=0, mg_precision=0):
@@ -505,7 +514,9 @@ def main():
         config["viscosity_tol"] = float(params.viscosity_tol)
     roof = pressure_roofline(sim, sim.grid[0], sim.grid[1], mgp, float(ph[3]), iters_mean)
     if T.batch is None:
-        roof["traffic"] = pmc_traffic(args.workload, W, H, mgp)
+        roof["traffic"], src = pmc_traffic(W, H, mgp, roof["active_cells"], iters_mean)
+        if src:
+            roof["traffic_source"] = src
     out = {
         "metric": "particle-steps/s & pressure-solve ms at 1024x576; p50/p99 step latency, batched",
         "value": value,

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsprout_hip.so")
+# SPROUT_HIP_LIB: another build of the library (a variant or an installed copy)
+LIB_PATH = os.environ.get("SPROUT_HIP_LIB") or os.path.join(HERE, "libsprout_hip.so")
 
 GROUPS = 4
 MAX_PRESENT = 12
