@@ -562,7 +562,8 @@ def main():
             acc["viscosity"] = dict(ve, iterations=int(K["v_iters"]))
             sim.record(1, 1 / 240)
             sim.wait()
-        errs, vel_err, K = fp64_ref.pressure_accuracy(sim, float(params.pressure_tol), gather=1)
+        errs, vel_err, K = fp64_ref.pressure_accuracy(sim, (float(params.pressure_tol), float(params.correction_tol)),
+                                                      gather=1)
         acc["pressure"] = errs[0]
         acc["correction"] = errs[1]
         acc["pressure_velocity_rel_l2_vs_fp64"] = vel_err

@@ -48,9 +48,11 @@ def rel(a, b):
 def solve_errors(A, unk, prhs, pres, tol):
     """Per potential (pressure*dt, density correction): the device solution's error against
     the exact FP64 solution, its true relative residual ||b - A x|| / ||b||, and the error
-    of an FP64 CG (Jacobi-preconditioned, from zero) stopped at relative residual `tol`.
-    Entries are None for a potential whose right-hand side is zero. Also returns the exact
-    solutions as an (N, 2) FP64 array."""
+    of an FP64 CG (Jacobi-preconditioned, from zero) stopped at the potential's own relative
+    residual tolerance (`tol`: one for both, or a pair: pressure_tol, correction_tol). Entries
+    are None for a potential whose right-hand side is zero. Also returns the exact solutions
+    as an (N, 2) FP64 array."""
+    tols = tuple(tol) if isinstance(tol, (tuple, list)) else (tol, tol)
     out = []
     M = sp.diags(1.0 / A.diagonal())
     Ac = A.tocsc()
@@ -63,11 +65,11 @@ def solve_errors(A, unk, prhs, pres, tol):
             continue
         xe = sla.spsolve(Ac, b)
         x64[unk, comp] = xe
-        xcg, _ = sla.cg(A, b, rtol=tol, M=M, maxiter=100000)
+        xcg, _ = sla.cg(A, b, rtol=tols[comp], M=M, maxiter=100000)
         # the FP32 floor of the true residual: eps32 || |A| |x| || / ||b||
         floor = float(np.finfo(np.float32).eps * np.linalg.norm(abs(A) @ np.abs(x)) / np.linalg.norm(b))
-        out.append({"err_vs_fp64": rel(x, xe), "true_residual": rel(A @ x, b), "fp32_floor": floor,
-                    "fp64_cg_same_tol_err": rel(xcg, xe)})
+        out.append({"tol": float(tols[comp]), "err_vs_fp64": rel(x, xe), "true_residual": rel(A @ x, b),
+                    "fp32_floor": floor, "fp64_cg_same_tol_err": rel(xcg, xe)})
     return out, x64
 
 
