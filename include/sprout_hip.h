@@ -157,6 +157,12 @@ int sp_stats(sp_sim* sim, sp_stats_out* out);
  * with a susceptibility turn the magnetic pass on (magnet.comp). */
 int sp_set_magnets(sp_sim* sim, const sp_magnet* magnets, uint32_t n);
 
+/* `FluidSim::set_currents` (src/sim/mod.rs:2008-2012): replaces the currents and drains (the
+ * first SP_MAX_CURRENTS are kept); they act from the next record on and survive a reset. A
+ * current raises the velocity of the liquid inside it along its direction towards its speed
+ * (never braking it), a drain removes the liquid inside it (particles.comp:578-592, 809-829). */
+int sp_set_currents(sp_sim* sim, const sp_current* currents, uint32_t n);
+
 /* The periodic counting sort of the particles by cell (record_sort, src/sim/mod.rs:
  * 3223-3242, sort.comp). Runs on the handle's stream; the order within a cell is stable
  * by the previous slot (the reference's is atomic-ordered, sort.comp:64). */
@@ -373,7 +379,8 @@ void sp_default_params(sp_sim_params* out);
  * grid-unit parameter block and the material table (SP_MAX_MATERIALS rows). */
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h,
                           const int32_t* rows, int32_t n_rows, int32_t thermal, const sp_magnet* magnets,
-                          uint32_t n_magnets, sp_gparams* out, sp_gmat* mats);
+                          uint32_t n_magnets, const sp_current* currents, uint32_t n_currents,
+                          sp_gparams* out, sp_gmat* mats);
 
 /* Levels::new (src/sim/mod.rs:727-753): the multigrid hierarchy of a W x H grid. */
 int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out);

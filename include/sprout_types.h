@@ -187,6 +187,17 @@ typedef struct sp_magnet {
     float dir[2];
 } sp_magnet;
 
+/* `Current` (src/sim/params.rs:1113-1121): a zone of the tank, position and radius in cells,
+ * that drives the liquid in it towards `vel` (cells/s) at `rate` (1/s); a negative rate makes
+ * it a drain, which removes the liquid inside it at minus that rate per second. */
+#define SP_MAX_CURRENTS 16  /* params.rs:1069 MAX_CURRENTS */
+typedef struct sp_current {
+    float pos[2];
+    float radius;
+    float vel[2];
+    float rate;
+} sp_current;
+
 /* `ToolState` (params.rs:1141-1150). */
 typedef struct sp_tool_state {
     float pos[2];
@@ -311,6 +322,10 @@ typedef struct sp_gparams {
     float magnet[SP_MAX_MAGNETS][4];      /* position (cells), radius (cells, at least 0.5), strength (T at the rim) */
     float magnet_dir[SP_MAX_MAGNETS][2];  /* north pole direction */
     float demag;          /* heat3.w: magnet_demag in [0, 1] */
+    /* currents and drains (mod.rs:2265-2270): position (cells), radius (at least 0.01 cells),
+     * rate (1/s; negative: a drain); the velocity driven towards (cells/s). ext[3] counts them. */
+    float current_zone[SP_MAX_CURRENTS][4];
+    float current_vel[SP_MAX_CURRENTS][4];
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -406,6 +421,8 @@ typedef struct sp_levels_info {
 #define SP_CAP_VOLATILE 0x10000u
 /* Not a reference capability either: a thermal tank with a liquid that heats itself. */
 #define SP_CAP_SELF_HEAT 0x20000u
+/* Nor this: the tank has currents or drains (the particle kernel with their code runs). */
+#define SP_CAP_CURRENTS 0x40000u
 #define SP_CAP_REACT 256u
 #define SP_REACT_MIN 0.02f  /* common.glsl:259 */
 #define SP_MU0 1.25663706e-6f  /* magnet.comp MU0 */

@@ -868,7 +868,23 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
             corrx = 0.0f;
             corry = 0.0f;
         }
-        /* Speed cap (particles.comp:831-835). */
+        /* Drains (particles.comp:578-592): each particle in a zone of negative rate leaves with
+         * probability -rate dt (1 - 0.75 r^2). */
+        if (P->ext[3] > 0.0f) {
+            for (int i = 0; i < (int)P->ext[3]; i++) {
+                const float* z = P->current_zone[i];
+                if (z[3] >= 0.0f) continue;
+                float dzx = p.pos[0] - z[0], dzy = p.pos[1] - z[1];
+                float r2 = (dzx * dzx + dzy * dzy) / (z[2] * z[2]);
+                if (r2 < 1.0f) {
+                    float u = (float)(hash_u(p.id * 2246822519u + s->S.substeps * 3266489917u + (uint32_t)i) >> 8) / 16777216.0f;
+                    if (u < -z[3] * (1.0f - 0.75f * r2) * dt) {
+                        s->part[id].kind = SP_KIND_DEAD;
+                        return;
+                    }
+                }
+            }
+        }
         /* Heat made inside (particles.comp:733-739), counted in the ledger. */
         if (th_on && s->mats[row].rx[2] > 0.0f) {
             float e = s->mats[row].rx[2] * s->mats[row].th[0] * (s->mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P->heat_rate;
@@ -936,6 +952,29 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                 }
             }
         }
+        /* Currents (particles.comp:809-829): v gains dir max(speed - v.dir, 0)
+         * (1 - exp(-rate (1 - r^2)^2 dt)) drive inside each zone of positive rate. */
+        if (P->ext[3] > 0.0f) {
+            float drive = P->tool_crowd_max > 0.0f ? clampf((P->tool_crowd_max - crowd) / (0.25f * P->tool_crowd_max), 0.0f, 1.0f) : 1.0f;
+            for (int i = 0; i < (int)P->ext[3] && drive > 0.0f; i++) {
+                const float* z = P->current_zone[i];
+                if (z[3] <= 0.0f) continue;
+                float dzx = p.pos[0] - z[0], dzy = p.pos[1] - z[1];
+                float r2 = (dzx * dzx + dzy * dzy) / (z[2] * z[2]);
+                if (r2 < 1.0f) {
+                    float tx = P->current_vel[i][0], ty = P->current_vel[i][1];
+                    float speed = len2(tx, ty);
+                    float m = fmaxf_(speed, 1e-6f);
+                    float dx = tx / m, dy = ty / m;
+                    float fall = (1.0f - r2) * (1.0f - r2);
+                    float gain = fmaxf_(speed - (vx * dx + vy * dy), 0.0f);
+                    float relax = 1.0f - expf(-z[3] * fall * dt);
+                    vx += dx * gain * relax * drive;
+                    vy += dy * gain * relax * drive;
+                }
+            }
+        }
+        /* Speed cap (particles.comp:831-835). */
         float vcap = P->ext[0];
         {
             float sp = len2(vx, vy);

@@ -34,6 +34,9 @@ RIGID_BODIES, BODY_WORDS, RIGID_FIX = 4096, 32, 65536.0
 RIGID_WORDS = 16 + BODY_WORDS * RIGID_BODIES
 CAP_FAMILIES, CAP_THERMAL, CAP_RIGID, CAP_RHEO, CAP_MAGNET, CAP_ELASTIC, CAP_SETTLE = 1, 2, 4, 8, 16, 32, 64
 MAX_MAGNETS = 8
+MAX_CURRENTS = 16  # params.rs:1069
+CURRENT_SPEED, CURRENT_RATE, DRAIN_RATE = 150.0, 20.0, 40.0  # params.rs:1065, 1099, 1101
+CAP_CURRENTS = 0x40000
 CAP_VOLATILE = 0x10000
 CAP_SELF_HEAT = 0x20000
 CAP_REACT = 256
@@ -116,6 +119,26 @@ class Magnet(C.Structure):
         return Magnet((C.c_float * 2)(*pos), radius, strength, (C.c_float * 2)(float(np.sin(a)), float(np.cos(a))))
 
 
+class Current(C.Structure):
+    """`Current` (params.rs:1113-1121): position and radius in cells, the velocity the liquid is
+    driven towards (cells/s), rate (1/s; negative: a drain)."""
+    _fields_ = [("pos", C.c_float * 2), ("radius", C.c_float), ("vel", C.c_float * 2), ("rate", C.c_float)]
+
+    @staticmethod
+    def new(pos, radius, degrees, strength):
+        """Current::new (params.rs:1126-1130): flowing `degrees` clockwise from straight down at
+        `strength` x CURRENT_SPEED, at CURRENT_RATE."""
+        a = np.radians(np.float32(degrees))
+        sp = np.float32(CURRENT_SPEED * strength)
+        return Current((C.c_float * 2)(*pos), radius, (C.c_float * 2)(float(np.sin(a) * sp), float(-np.cos(a) * sp)),
+                       CURRENT_RATE)
+
+    @staticmethod
+    def drain(pos, radius, strength):
+        """Current::drain (params.rs:1134-1136)."""
+        return Current((C.c_float * 2)(*pos), radius, (C.c_float * 2)(0.0, 0.0), -DRAIN_RATE * strength)
+
+
 class ToolState(C.Structure):
     _fields_ = [("pos", C.c_float * 2), ("vel", C.c_float * 2), ("radius", C.c_float),
                 ("mode", C.c_int32), ("strength", C.c_float)]
@@ -188,7 +211,8 @@ class GParams(C.Structure):
                 ("chan_t", C.c_int32), ("chan_e", C.c_int32), ("el_g", C.c_float),
                 ("el_stress_max", C.c_float), ("el_step_max", C.c_float), ("mag", C.c_float * 4),
                 ("magnet", (C.c_float * 4) * MAX_MAGNETS), ("magnet_dir", (C.c_float * 2) * MAX_MAGNETS),
-                ("demag", C.c_float)]
+                ("demag", C.c_float), ("current_zone", (C.c_float * 4) * MAX_CURRENTS),
+                ("current_vel", (C.c_float * 4) * MAX_CURRENTS)]
 
 
 class DeviceView(C.Structure):
@@ -287,8 +311,9 @@ SIGNATURES = {
     "sp_shares_a_group": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "sp_default_params": ([C.POINTER(SimParams)], None),
     "sp_derive_grid_params": ([C.POINTER(SimParams), C.c_int32, C.c_int32, _P, C.c_int32, C.c_int32,
-                               _P, C.c_uint32, C.POINTER(GParams), _P], C.c_int),
+                               _P, C.c_uint32, _P, C.c_uint32, C.POINTER(GParams), _P], C.c_int),
     "sp_set_magnets": ([_P, _P, C.c_uint32], C.c_int),
+    "sp_set_currents": ([_P, _P, C.c_uint32], C.c_int),
     "sp_download_ledger_counts": ([_P, _P, _P, _P], C.c_int),
     "sp_levels": ([C.c_int32, C.c_int32, C.POINTER(LevelsInfo)], C.c_int),
     "sp_debug_download": ([_P, C.c_int32, _P, C.c_size_t], C.c_int),
@@ -474,15 +499,16 @@ def default_params():
     return p
 
 
-def derive_grid_params(params, W, H, rows, thermal=False, magnets=()):
+def derive_grid_params(params, W, H, rows, thermal=False, magnets=(), currents=()):
     """sp_derive_grid_params: the grid-unit block and material table for a tank of `rows`
-    with `magnets` (Magnet records) placed."""
+    with `magnets` (Magnet records) placed and `currents` (Current records)."""
     g = GParams()
     mats = (GMat * MAX_MATERIALS)()
     arr = (C.c_int32 * max(len(rows), 1))(*rows)
     mg = (Magnet * max(len(magnets), 1))(*magnets)
+    cu = (Current * max(len(currents), 1))(*currents)
     rc = lib().sp_derive_grid_params(C.byref(params), W, H, arr, len(rows), int(thermal), mg, len(magnets),
-                                     C.byref(g), mats)
+                                     cu, len(currents), C.byref(g), mats)
     if rc != OK:
         raise SproutError(rc, "derive_grid_params")
     return g, mats

@@ -29,7 +29,8 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
 // ext_speed: the fastest speed a pour imposes this record, cells / s (the queued spawns'
 // fastest |vel|; the tool's is 0 for the idle tool this build carries).
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
-           std::string* why, const std::vector<sp_magnet>& magnets = {}, float ext_speed = 0.0f);
+           std::string* why, const std::vector<sp_magnet>& magnets = {}, float ext_speed = 0.0f,
+           const std::vector<sp_current>& currents = {});
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);

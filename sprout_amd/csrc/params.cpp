@@ -399,7 +399,7 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
 }
 
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why,
-           const std::vector<sp_magnet>& magnets, float ext_speed) {
+           const std::vector<sp_magnet>& magnets, float ext_speed, const std::vector<sp_current>& currents) {
     memset(&g, 0, sizeof(g));
     memset(mats, 0, sizeof(sp_gmat) * SP_MAX_MATERIALS);
     const float dx = p.cell_size;
@@ -550,7 +550,20 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     }
     g.ext[1] = (float)p.advection_order;
     g.ext[2] = p.spray_cfl;
-    g.ext[3] = 0.0f;
+    // currents and drains (mod.rs:2265-2270, 2309): zone (position, radius at least 0.01, rate)
+    // and velocity; their number in ext.w
+    const size_t nc = std::min<size_t>(currents.size(), SP_MAX_CURRENTS);
+    for (size_t k = 0; k < nc; k++) {
+        const sp_current& cu = currents[k];
+        g.current_zone[k][0] = cu.pos[0];
+        g.current_zone[k][1] = cu.pos[1];
+        g.current_zone[k][2] = std::max(cu.radius, 0.01f);
+        g.current_zone[k][3] = cu.rate;
+        g.current_vel[k][0] = cu.vel[0];
+        g.current_vel[k][1] = cu.vel[1];
+    }
+    g.ext[3] = (float)nc;
+    if (nc > 0) g.caps |= SP_CAP_CURRENTS;
     g.corr[0] = p.correction_step_max;
     g.corr[1] = p.correction_deadband;
     g.corr[2] = (float)p.slip_cap;

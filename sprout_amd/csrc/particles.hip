@@ -494,6 +494,26 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
             corrx = 0.0f;
             corry = 0.0f;
         }
+        // Drains (particles.comp:578-592): a zone with a negative rate removes the liquid in it,
+        // each particle with probability -rate x dt, falling off to a quarter at the rim.
+        if (X && (P.caps & SP_CAP_CURRENTS)) {
+            const int nz = (int)P.ext[3];
+            for (int i = 0; i < nz; i++) {
+                const float zw = P.current_zone[i][3];
+                if (zw >= 0.0f) continue;
+                const float dzx = p.pos[0] - P.current_zone[i][0], dzy = p.pos[1] - P.current_zone[i][1];
+                const float rz = P.current_zone[i][2];
+                const float r2 = (dzx * dzx + dzy * dzy) / (rz * rz);
+                if (r2 < 1.0f) {
+                    const float u =
+                        (float)(hash_u(p.id * 2246822519u + c.S->substeps * 3266489917u + (uint32_t)i) >> 8) / 16777216.0f;
+                    if (u < -zw * (1.0f - 0.75f * r2) * dt) {
+                        c.part[id].kind = SP_KIND_DEAD;
+                        return;
+                    }
+                }
+            }
+        }
         // Heat made inside (particles.comp:733-739): the liquid's own K/s, counted in the ledger.
         if (X && th_on && (P.caps & SP_CAP_SELF_HEAT) && c.mats[row].rx[2] > 0.0f) {
             const float e = c.mats[row].rx[2] * c.mats[row].th[0] * (c.mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P.heat_rate;
@@ -556,6 +576,32 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                     atomicAdd(&c.ledger[16 + row], 1u);
                     c.part[id].kind = SP_KIND_DEAD;
                     return;
+                }
+            }
+        }
+        // Currents (particles.comp:809-829): the velocity along each current is raised towards
+        // its speed, never braked, and the motion across it left alone; a particle already in a
+        // crowd is driven less, and not at all past tool_crowd_max.
+        if (X && (P.caps & SP_CAP_CURRENTS)) {
+            const float drive =
+                P.tool_crowd_max > 0.0f ? clampf_((P.tool_crowd_max - crowd) / (0.25f * P.tool_crowd_max), 0.0f, 1.0f) : 1.0f;
+            const int nz = (int)P.ext[3];
+            for (int i = 0; i < nz && drive > 0.0f; i++) {
+                const float zw = P.current_zone[i][3];
+                if (zw <= 0.0f) continue;
+                const float dzx = p.pos[0] - P.current_zone[i][0], dzy = p.pos[1] - P.current_zone[i][1];
+                const float rz = P.current_zone[i][2];
+                const float r2 = (dzx * dzx + dzy * dzy) / (rz * rz);
+                if (r2 < 1.0f) {
+                    const float tx = P.current_vel[i][0], ty = P.current_vel[i][1];
+                    const float speed = len2_(tx, ty);
+                    const float m = fmaxf_(speed, 1e-6f);
+                    const float dx_ = tx / m, dy_ = ty / m;
+                    const float fall = (1.0f - r2) * (1.0f - r2);
+                    const float gain = fmaxf_(speed - (vx * dx_ + vy * dy_), 0.0f);
+                    const float relax = 1.0f - expf(-zw * fall * dt);
+                    vx += dx_ * gain * relax * drive;
+                    vy += dy_ * gain * relax * drive;
                 }
             }
         }

@@ -337,7 +337,8 @@ const PKernel P_KERNELS[2][N_PSHAPES] = {SP_PK(false), SP_PK(true)};
 #undef SP_PK
 
 bool exotic_run(const Run& r) {
-    return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE | SP_CAP_VOLATILE | SP_CAP_SELF_HEAT | SP_CAP_REACT)) != 0u;
+    return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE | SP_CAP_VOLATILE | SP_CAP_SELF_HEAT | SP_CAP_REACT |
+                         SP_CAP_CURRENTS)) != 0u;
 }
 
 // What a shape is tuned for: the grid, the tanks per launch, the accumulator stride and
@@ -455,7 +456,7 @@ uint32_t schedule_key(const sp_gparams& g) {
            (uint32_t)((g.caps & SP_CAP_RHEO) != 0u) << 15 | (uint32_t)((g.caps & SP_CAP_ELASTIC) != 0u) << 16 |
            (uint32_t)((g.caps & SP_CAP_SETTLE) != 0u) << 17 | (uint32_t)((g.caps & SP_CAP_MAGNET) != 0u) << 18 |
            (uint32_t)((g.caps & SP_CAP_VOLATILE) != 0u) << 19 | (uint32_t)((g.caps & SP_CAP_SELF_HEAT) != 0u) << 20 |
-           (uint32_t)((g.caps & SP_CAP_REACT) != 0u) << 21;
+           (uint32_t)((g.caps & SP_CAP_REACT) != 0u) << 21 | (uint32_t)((g.caps & SP_CAP_CURRENTS) != 0u) << 22;
 }
 
 }  // namespace

@@ -74,6 +74,9 @@ SP_DEV float ff_value(Ff a) { return a.s + a.c; }
 // along (in BF16 that last one cost 1.8x the iterations at 1024 x 576). Narrow values sit at
 // the start of each array's coarse region, which begins at cell N (level 1). Loads widen to
 // FP32 exactly, so all arithmetic is FP32; stores round to nearest even.
+#ifndef SP_MG_NARROW_ITERATES
+#define SP_MG_NARROW_ITERATES 1
+#endif
 struct alignas(8) Half4 {
     _Float16 x, y, z, w;
 };
@@ -137,16 +140,27 @@ struct MG<true> {
         if (i < c.N) a[i] = v;
         else reinterpret_cast<uint32_t*>(a + c.N)[i - c.N] = pack_bf2(v);
     }
+#if SP_MG_NARROW_ITERATES
     SP_DEV static float2 X(const Ctx& c, int i) { return vec(c.mgx, c, i); }
     SP_DEV static void set_X(const Ctx& c, int i, float2 v) { set_vec(c.mgx, c, i, v); }
     SP_DEV static float2 Z(const Ctx& c, int i) { return vec(c.mgz, c, i); }
     SP_DEV static void set_Z(const Ctx& c, int i, float2 v) { set_vec(c.mgz, c, i, v); }
+#else
+    SP_DEV static float2 X(const Ctx& c, int i) { return c.mgx[i]; }
+    SP_DEV static void set_X(const Ctx& c, int i, float2 v) { c.mgx[i] = v; }
+    SP_DEV static float2 Z(const Ctx& c, int i) { return c.mgz[i]; }
+    SP_DEV static void set_Z(const Ctx& c, int i, float2 v) { c.mgz[i] = v; }
+#endif
     SP_DEV static float2 R(const Ctx& c, int i) {
         return unpack_bf2(reinterpret_cast<const uint32_t*>(c.mgr + c.N)[i - c.N]);
     }
     SP_DEV static void set_R(const Ctx& c, int i, float2 v) { reinterpret_cast<uint32_t*>(c.mgr + c.N)[i - c.N] = pack_bf2(v); }
     SP_DEV static float2 stored_R(float2 v) { return unpack_bf2(pack_bf2(v)); }
+#if SP_MG_NARROW_ITERATES
     SP_DEV static float2 stored_Z(float2 v) { return unpack_bf2(pack_bf2(v)); }
+#else
+    SP_DEV static float2 stored_Z(float2 v) { return v; }
+#endif
 };
 
 SP_DEV int tid2() { return threadIdx.y * 16 + threadIdx.x; }

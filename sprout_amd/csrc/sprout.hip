@@ -62,6 +62,7 @@ struct sp_sim {
     uint64_t generation = 0;  // bumped when the arena moves
     uint64_t resets = 0;      // sp_reset calls (FluidSim::resets)
     std::vector<sp_magnet> magnets;  // placed in the world (FluidSim::magnets); survive a reset
+    std::vector<sp_current> currents;  // currents and drains (FluidSim::currents); survive a reset
 
     sp_sim_params params{};
     bool params_set = false;
@@ -209,7 +210,8 @@ int upload_tables(sp_sim* s) {
         }
     }
     std::string why;
-    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets, s->spawn_speed);
+    int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets, s->spawn_speed,
+                        s->currents);
     if (rc != SP_OK) {
         set_err(s, "%s", why.c_str());
         return rc;
@@ -533,14 +535,18 @@ int sp_levels(int32_t grid_w, int32_t grid_h, sp_levels_info* out) {
 }
 
 int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t grid_h, const int32_t* rows, int32_t n_rows,
-                          int32_t thermal, const sp_magnet* magnets, uint32_t n_magnets, sp_gparams* out, sp_gmat* mats) {
-    if (!params || !out || !mats || (n_rows > 0 && !rows) || n_rows < 0 || (n_magnets > 0 && !magnets)) return SP_E_INVALID;
+                          int32_t thermal, const sp_magnet* magnets, uint32_t n_magnets, const sp_current* currents,
+                          uint32_t n_currents, sp_gparams* out, sp_gmat* mats) {
+    if (!params || !out || !mats || (n_rows > 0 && !rows) || n_rows < 0 || (n_magnets > 0 && !magnets) ||
+        (n_currents > 0 && !currents))
+        return SP_E_INVALID;
     std::vector<int> present(rows, rows + n_rows);
     std::vector<int> r = sp::closure(*params, present);
     sp::Tank t;
     if (!sp::tank_layout(*params, r, t, thermal != 0)) return SP_E_CAPACITY;
     std::vector<sp_magnet> mg(magnets, magnets + std::min<uint32_t>(n_magnets, SP_MAX_MAGNETS));
-    return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr, mg);
+    std::vector<sp_current> cu(currents, currents + std::min<uint32_t>(n_currents, SP_MAX_CURRENTS));
+    return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr, mg, 0.0f, cu);
 }
 
 int sp_create(const sp_config* cfg, sp_sim** out) {
@@ -965,7 +971,8 @@ int sp_material_table(sp_sim* s, sp_gpumat* out, uint32_t cap) {
             sp_gparams g;
             sp_gmat gm[SP_MAX_MATERIALS];
             std::string why;
-            int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, g, gm, &why, s->magnets, s->spawn_speed);
+            int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, g, gm, &why, s->magnets, s->spawn_speed,
+                                s->currents);
             if (rc != SP_OK) {
                 set_err(s, "%s", why.c_str());
                 return rc;
@@ -1110,6 +1117,15 @@ int sp_set_magnets(sp_sim* s, const sp_magnet* m, uint32_t n) {
     return guarded(s, [&]() -> int {
         if (n > 0 && !m) return SP_E_INVALID;
         s->magnets.assign(m, m + std::min<uint32_t>(n, SP_MAX_MAGNETS));
+        s->tables_dirty = true;
+        return SP_OK;
+    }, false);
+}
+
+int sp_set_currents(sp_sim* s, const sp_current* cu, uint32_t n) {
+    return guarded(s, [&]() -> int {
+        if (n > 0 && !cu) return SP_E_INVALID;
+        s->currents.assign(cu, cu + std::min<uint32_t>(n, SP_MAX_CURRENTS));
         s->tables_dirty = true;
         return SP_OK;
     }, false);

@@ -159,6 +159,11 @@ class FluidSim:
         self._check(self._lib.sp_set_magnets(self._h, arr, len(magnets)), "set_magnets")
 
     # -- render-facing outputs (mod.rs:1608-1943) ---------------------------------------
+    def set_currents(self, currents):
+        """FluidSim::set_currents (mod.rs:2008-2012): Current records (the first 16 kept)."""
+        arr = (N.Current * max(len(currents), 1))(*currents)
+        self._check(self._lib.sp_set_currents(self._h, arr, len(currents)), "set_currents")
+
     def render_view(self, which):
         """The device view (address, bytes, count, stride, device, generation) of a render
         buffer: "PARTICLES", "SDF", "CELL", "COMPOSITION", "SURFACE", "MATERIALS", "VELOCITY", "GPUMAT"."""

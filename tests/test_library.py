@@ -79,10 +79,11 @@ def test_errors_cross_the_abi_as_codes():
     p = N.default_params()
     g = N.GParams()
     mats = (N.GMat * N.MAX_MATERIALS)()
-    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, -1, 0, None, 0, C.byref(g), mats) == N.E_INVALID
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, -1, 0, None, 0, None, 0, C.byref(g), mats) == N.E_INVALID
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, None, 0, 0, None, 0, None, 2, C.byref(g), mats) == N.E_INVALID
     # thirteen materials do not fit a tank (MAX_PRESENT 12, mod.rs:2051-2081)
     rows = (C.c_int32 * 13)(*range(13))
-    assert L.sp_derive_grid_params(C.byref(p), 64, 64, rows, 13, 0, None, 0, C.byref(g), mats) == N.E_CAPACITY
+    assert L.sp_derive_grid_params(C.byref(p), 64, 64, rows, 13, 0, None, 0, None, 0, C.byref(g), mats) == N.E_CAPACITY
 
 
 def test_viscosity_modes_follow_the_diffusion_number():
