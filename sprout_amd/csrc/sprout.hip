@@ -58,6 +58,7 @@ struct sp_sim {
     // residency: an evicted tank keeps its arena in pinned host memory
     bool resident = true;
     void* host_arena = nullptr;
+    std::vector<std::pair<size_t, size_t>> host_ranges;  // (arena offset, bytes) of what host_arena holds
     size_t arena_bytes = 0;
     uint64_t generation = 0;  // bumped when the arena moves
     uint64_t resets = 0;      // sp_reset calls (FluidSim::resets)
@@ -261,31 +262,97 @@ size_t big_bytes(size_t arena_bytes, uint32_t cap) {
 }
 size_t big_bytes(const sp_sim* s) { return big_bytes(s->arena_bytes, s->cap); }
 
-// Evicts the tank: its arena goes to pinned host memory and its evictable device memory is
-// freed. Nothing else of the tank changes, so a restored tank continues bit for bit.
+// The byte ranges of the arena that carry a tank from one substep to the next: all of it but
+// the work space every substep writes before it reads it (the accumulator, cleared first; the
+// Krylov and multigrid iterates and residuals of both solves, their partial sums; the face
+// coefficients, the right-hand sides and the pre-projection velocities, which the grid passes
+// rewrite), the particle slots past the last live one, the particles' stress in a tank with
+// nothing elastic (a tank that turns elastic zeroes it), and the per-family arrays past the
+// families the tank has. Sorted, disjoint, merged.
+std::vector<std::pair<size_t, size_t>> kept_ranges(const sp_sim* s) {
+    TankRef r = s->ref;
+    r.base = nullptr;
+    const Ctx c = ctx_of(r);
+    const size_t N = (size_t)s->N, T = (size_t)s->lv.total, cap = std::max<uint32_t>(s->cap, 1);
+    const size_t nb0 = (size_t)s->nb0;
+    const int nf = std::max(s->gp.n_fam, 0), nmix = std::max(s->gp.n_fam - 1, 0);
+    auto off = [](const void* p) { return (size_t)(uintptr_t)p; };
+    std::vector<std::pair<size_t, size_t>> skip;  // (start, end)
+    auto drop = [&](const void* p, size_t from, size_t to) {
+        if (to > from) skip.push_back({off(p) + from, off(p) + to});
+    };
+    drop(c.acc, 0, sizeof(int32_t) * N * SP_ACC_MAX);
+    for (const void* f : {(const void*)c.coef, (const void*)c.prhs}) drop(f, 0, sizeof(float4) * N);
+    for (const void* f : {(const void*)c.vel0, (const void*)c.cgp, (const void*)c.cgp2, (const void*)c.cgq, (const void*)c.r2,
+                          (const void*)c.vr, (const void*)c.vp, (const void*)c.vp2, (const void*)c.vq, (const void*)c.vr2,
+                          (const void*)c.vq2, (const void*)c.vaz})
+        drop(f, 0, sizeof(float2) * N);
+    for (const void* f : {(const void*)c.mgx, (const void*)c.mgz, (const void*)c.mgr, (const void*)c.vmr, (const void*)c.vmx,
+                          (const void*)c.vmz})
+        drop(f, 0, sizeof(float2) * T);
+    drop(c.part_sums, 0, sizeof(double4) * 3 * nb0);
+    drop(c.vpart, 0, sizeof(double4) * nb0);
+    drop(c.part, sizeof(sp_particle) * s->n_slots, sizeof(sp_particle) * cap);
+    drop(c.pstress, s->tank.chan_e >= 0 ? sizeof(float4) * s->n_slots : 0, sizeof(float4) * cap);
+    drop(c.mixb, sizeof(float) * N * nmix, sizeof(float) * N * SP_MAX_PRESENT);
+    drop(c.famocc, sizeof(float) * N * nf, sizeof(float) * N * SP_MAX_PRESENT);
+    drop(c.dsettle, sizeof(float2) * N * nf, sizeof(float2) * N * SP_MAX_PRESENT);
+    std::sort(skip.begin(), skip.end());
+    std::vector<std::pair<size_t, size_t>> keep;  // (offset, bytes)
+    size_t at = 0;
+    for (const auto& k : skip) {
+        if (k.first > at) keep.push_back({at, k.first - at});
+        at = std::max(at, k.second);
+    }
+    if (s->arena_bytes > at) keep.push_back({at, s->arena_bytes - at});
+    return keep;
+}
+
+// Evicts the tank: the arena's state (kept_ranges) goes to pinned host memory and its
+// evictable device memory is freed. Nothing else of the tank changes, so a restored tank
+// continues bit for bit.
 void evict(sp_sim* s) {
     if (!s->resident) return;
     HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<std::pair<size_t, size_t>> keep = kept_ranges(s);
+    size_t total = 0;
+    for (const auto& k : keep) total += k.second;
     void* h = nullptr;
-    HIPCHK(hipHostMalloc(&h, s->arena_bytes, hipHostMallocDefault));
-    hipError_t e = hipMemcpy(h, s->ref.base, s->arena_bytes, hipMemcpyDeviceToHost);
+    HIPCHK(hipHostMalloc(&h, std::max<size_t>(total, 1), hipHostMallocDefault));
+    hipError_t e = hipSuccess;
+    size_t at = 0;
+    for (const auto& k : keep) {
+        if (e == hipSuccess)
+            e = hipMemcpyAsync((char*)h + at, s->ref.base + k.first, k.second, hipMemcpyDeviceToHost, s->stream);
+        at += k.second;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         (void)hipHostFree(h);
-        throw HipError{e, "hipMemcpy (evict)"};
+        throw HipError{e, "hipMemcpyAsync (evict)"};
     }
     s->gs.clear();  // captured graphs hold the old addresses
     free_big(s);
     s->host_arena = h;
+    s->host_ranges = std::move(keep);
     s->resident = false;
 }
 
-// Brings an evicted tank back: fresh device memory, the arena copied in.
+// Brings an evicted tank back: fresh device memory, the kept ranges copied in (the work space
+// is written before it is read, as in any substep).
 void restore(sp_sim* s) {
     if (s->resident) return;
     alloc_big(s);
-    HIPCHK(hipMemcpy(s->ref.base, s->host_arena, s->arena_bytes, hipMemcpyHostToDevice));
+    size_t at = 0;
+    for (const auto& k : s->host_ranges) {
+        HIPCHK(hipMemcpyAsync(s->ref.base + k.first, (const char*)s->host_arena + at, k.second, hipMemcpyHostToDevice,
+                              s->stream));
+        at += k.second;
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
     (void)hipHostFree(s->host_arena);
     s->host_arena = nullptr;
+    s->host_ranges.clear();
     s->resident = true;
 }
 
