@@ -96,7 +96,7 @@ void launch_vcycle_t(const Run& r, int reduce, int rsel) {
     const sp_levels_info& L = r.lv;
     hipLaunchKernelGGL(k_mg_down<H>, grid0(r), TILE, 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), 0, 0, 0, rsel);
     launch_coarse_down<H>(r, 0, 0);
-    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 0, 0);
+    hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(TAIL_NT), r.tail_lds, r.stream, r.tb, 0, 0);
     launch_coarse_up<H>(r, 0, 0);
     hipLaunchKernelGGL(k_up0<H>, grid_t0(r), dim3(T0, T0), 0, r.stream, r.tb, lv_of(L, 0), lv_of(L, 1), parts_t0(r), 0, 0, reduce,
                        rsel);
@@ -144,7 +144,7 @@ void launch_pressure_t(const Run& r, uint32_t iters, bool second) {
         hipLaunchKernelGGL(k_cg_alpha, per_tank(r), dim3(REDUCE_NT), 0, r.stream, r.tb, nbp, par);
         hipLaunchKernelGGL(k_update_down0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, par);
         launch_coarse_down<H>(r, 1, par);
-        hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(1024), r.tail_lds, r.stream, r.tb, 1, par);
+        hipLaunchKernelGGL(k_mg_tail<H>, per_tank(r), dim3(TAIL_NT), r.tail_lds, r.stream, r.tb, 1, par);
         launch_coarse_up<H>(r, 1, par);
         hipLaunchKernelGGL(k_up0<H>, gt0, bt0, 0, r.stream, r.tb, L0, L1, nbp, 1, par, 1, par ^ 1);
     }

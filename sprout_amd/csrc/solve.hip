@@ -1174,11 +1174,16 @@ __global__ __launch_bounds__(NTF) void k_mg_up2(const TankBatch tb, Lv A, Lv B, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");   \
     } while (0)
 
+#ifndef SP_TAIL_NT
+#define SP_TAIL_NT 1024
+#endif
+constexpr int TAIL_NT = SP_TAIL_NT;  // work-items of the tail's group
+
 // LDS (dynamic, sized at launch to the hierarchy): per tail cell its couplings (float4),
 // iterate and right-hand side (float2) and diagonal; the first level's right-hand side is
 // loaded once with the matrix, so that the sweeps never wait on global memory.
 template <bool H>
-__global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated, int par) {
+__global__ __launch_bounds__(TAIL_NT) void k_mg_tail(const TankBatch tb, int gated, int par) {
     using M = MG<H>;
     const Ctx c = ctx_of(tb.t[blockIdx.z]);
     extern __shared__ float4 tail_lds[];
@@ -1196,7 +1201,7 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated,
     const int bt = threadIdx.x;
     if (bt <= last) sl[bt] = c.lev[bt];
     __syncthreads();
-    for (int t = bt; t < total; t += 1024) {
+    for (int t = bt; t < total; t += TAIL_NT) {
         int l = first;  // the level of tail cell t (the storage of the coarse matrices is per level)
         if (H)
             while (l < last && BASE + t >= sl[l + 1].x) l++;
@@ -1315,11 +1320,11 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated,
     int ls = first;
     while (ls < last && sl[ls].y * sl[ls].z > SP_TAIL_WARP_CELLS) ls++;
     for (int l = first; l < ls; l++) {
-        sweep(sl[l], 0, true, bt, 1024);
+        sweep(sl[l], 0, true, bt, TAIL_NT);
         __syncthreads();
-        sweep(sl[l], 1, false, bt, 1024);
+        sweep(sl[l], 1, false, bt, TAIL_NT);
         __syncthreads();
-        restrict_to(sl[l], sl[l + 1], bt, 1024);
+        restrict_to(sl[l], sl[l + 1], bt, TAIL_NT);
         __syncthreads();
     }
     if (bt < 64) {
@@ -1347,12 +1352,12 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const TankBatch tb, int gated,
     }
     __syncthreads();
     for (int l = ls - 1; l >= first; l--) {
-        post_black(sl[l], sl[l + 1], bt, 1024);
+        post_black(sl[l], sl[l + 1], bt, TAIL_NT);
         __syncthreads();
-        sweep(sl[l], 0, false, bt, 1024);
+        sweep(sl[l], 0, false, bt, TAIL_NT);
         __syncthreads();
     }
-    for (int t = bt; t < NF; t += 1024) M::set_Z(c, BASE + t, sx[t]);
+    for (int t = bt; t < NF; t += TAIL_NT) M::set_Z(c, BASE + t, sx[t]);
 }
 
 
