@@ -5,7 +5,7 @@
 // a group's substep costs the same number of launches as one tank's. A single handle is the
 // group of one.
 //
-// Parts of a substep are captured once per (iteration counts, schedule) as HIP graphs and
+// Parts of a substep are captured once per (schedule, recorded iteration count) as HIP graphs and
 // replayed: a substep is several hundred dependent launches. The kernels read everything
 // that changes between substeps (dt, tables, solve state) from device memory, so a replay
 // needs no new arguments.
@@ -463,28 +463,32 @@ uint32_t schedule_key(const sp_gparams& g) {
 
 // ---- graphs and timing ------------------------------------------------------------------
 
-// Instantiated graphs of one (recorded iterations, schedule) key: grid passes (with the
-// viscosity before the first solve), pressure solve, projection, the viscosity solve after
-// it, and the second pressure solve with its projection.
+// Instantiated graphs of the parts of a substep: grid passes (with the viscosity before the
+// first solve), pressure solve, projection, the viscosity solve after it, and the second
+// pressure solve with its projection. Each part is keyed on the schedule and on the one
+// recorded iteration count it runs (none for the projection), so a new count recaptures only
+// the part that runs it, and a part holds up to GRAPHS_PER_PART graphs, the least recently
+// used one making room for a new one.
 constexpr int GRAPH_PARTS = 5;
-struct Graphs {
-    Recs recs;
-    uint32_t sched = 0xffffffffu;
-    hipGraphExec_t exec[GRAPH_PARTS] = {};
+constexpr size_t GRAPHS_PER_PART = 32;
+struct GraphEntry {
+    uint32_t sched = 0, count = 0;
+    uint64_t used = 0;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct GraphSet {
-    std::vector<Graphs> graphs;
+    std::vector<GraphEntry> part[GRAPH_PARTS];
+    uint64_t tick = 0;
+    uint64_t captures = 0;
     bool on = true;
 
     void clear() {
-        for (auto& g : graphs)
-            for (auto& e : g.exec)
-                if (e) {
-                    (void)hipGraphExecDestroy(e);
-                    e = nullptr;
-                }
-        graphs.clear();
+        for (auto& v : part) {
+            for (auto& e : v)
+                if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            v.clear();
+        }
     }
     ~GraphSet() { clear(); }
 };
@@ -561,24 +565,48 @@ hipGraphExec_t capture(const Run& r, int part, const Recs& recs) {
     return e;
 }
 
-Graphs& graphs_for(GraphSet& gs, const Run& r, const Recs& recs) {
-    const uint32_t key = schedule_key(r.gp);
-    for (auto& g : gs.graphs)
-        if (g.recs == recs && g.sched == key) return g;
-    if (gs.graphs.size() >= 12) gs.clear();
-    gs.graphs.emplace_back();
-    Graphs& g = gs.graphs.back();
-    g.recs = recs;
-    g.sched = key;
-    for (int k = 0; k < GRAPH_PARTS; k++) g.exec[k] = capture(r, k, recs);
-    return g;
+// The recorded iteration count part `part` of a substep runs (0: none).
+uint32_t part_count(const Run& r, int part, const Recs& recs) {
+    switch (part) {
+    case 0: return r.gp.visc_mode == 2 && !r.gp.v_twice ? recs.v : 0u;
+    case 1: return recs.p;
+    case 3: return twice_projected(r.gp) ? recs.v : 0u;
+    case 4: return twice_projected(r.gp) ? recs.p2 : 0u;
+    default: return 0u;
+    }
+}
+
+hipGraphExec_t graph_for(GraphSet& gs, const Run& r, int part, const Recs& recs) {
+    const uint32_t key = schedule_key(r.gp), n = part_count(r, part, recs);
+    std::vector<GraphEntry>& v = gs.part[part];
+    for (auto& e : v)
+        if (e.sched == key && e.count == n) {
+            e.used = ++gs.tick;
+            return e.exec;
+        }
+    if (v.size() >= GRAPHS_PER_PART) {
+        auto lru = std::min_element(v.begin(), v.end(), [](const GraphEntry& a, const GraphEntry& b) { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        v.erase(lru);
+    }
+    GraphEntry e;
+    e.sched = key;
+    e.count = n;
+    e.used = ++gs.tick;
+    e.exec = capture(r, part, recs);
+    gs.captures++;
+    v.push_back(e);
+    return e.exec;
 }
 
 // One whole substep of the group (record_substep, mod.rs:2615-2708). gather = 0 on the
 // first substep after a reset (no grid velocities to gather yet).
 void launch_substep(const Run& r, GraphSet* gs, Timing* tm, int gather, const Recs& recs) {
     hipEvent_t* e = tm ? tm->next() : nullptr;
-    Graphs* g = gs && gs->on ? &graphs_for(*gs, r, recs) : nullptr;
+    hipGraphExec_t gx[GRAPH_PARTS] = {};
+    const bool g = gs && gs->on;
+    if (g)
+        for (int k = 0; k < GRAPH_PARTS; k++) gx[k] = graph_for(*gs, r, k, recs);
     if (e) HIPCHK(hipEventRecord(e[0], r.stream));
     hipLaunchKernelGGL(k_step, per_tank(r), dim3(64), 0, r.stream, r.tb);
     launch_clear_acc(r);
@@ -589,19 +617,19 @@ void launch_substep(const Run& r, GraphSet* gs, Timing* tm, int gather, const Re
         HIPCHK(hipEventRecord(e[6], r.stream));
     }
     if (e) HIPCHK(hipEventRecord(e[1], r.stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[0], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(gx[0], r.stream));
     else launch_grid_passes(r, recs);
     if (e) HIPCHK(hipEventRecord(e[2], r.stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[1], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(gx[1], r.stream));
     else launch_pressure(r, recs.p);
     if (e) HIPCHK(hipEventRecord(e[3], r.stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[2], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(gx[2], r.stream));
     else launch_project(r);
     if (e) HIPCHK(hipEventRecord(e[4], r.stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[3], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(gx[3], r.stream));
     else launch_visc_after(r, recs);
     if (e) HIPCHK(hipEventRecord(e[7], r.stream));
-    if (g) HIPCHK(hipGraphLaunch(g->exec[4], r.stream));
+    if (g) HIPCHK(hipGraphLaunch(gx[4], r.stream));
     else launch_second_projection(r, recs);
     if (e) HIPCHK(hipEventRecord(e[8], r.stream));
     HIPCHK(hipGetLastError());
