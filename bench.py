@@ -2,9 +2,11 @@
 with p50/p99 step latency batched and served.
 
 One "step" is one substep of the fluid step (step, particle pass, grid passes, viscosity,
-the MGPCG pressure solves, projections) over the whole tank, recorded in batches of 16
-substeps as the reference's blocking helpers record them (advance_blocking,
-mod.rs:3284-3312). Inputs are resident in device memory when the timed region starts.
+the MGPCG pressure solves, projections) over the whole tank. A single tank is recorded as the
+reference's frame loop records it: one record per 1/60 s frame of 4 substeps, each adapting
+its recorded iterations from the read-backs finished so far (poll_readbacks,
+mod.rs:3030-3115), up to four frames in flight; batches of tanks record 16 substeps, then
+wait. Inputs are resident in device memory when the timed region starts.
 
 Headline (`value`, config.workload):
 * four_liquids_1024x576_mixed_mg (default): oil, water, honey and mercury in four seeded
@@ -60,7 +62,12 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-CHUNK = 16
+CHUNK = 16  # substeps per record of a batch of tanks (record, then wait)
+# A single tank is driven as the reference's frame loop drives it: a record per 1/60 s frame
+# (4 substeps at 240 Hz), each taking up the read-backs finished so far without blocking, a
+# few frames in flight, and a wait every SYNC substeps (the phase timings hold 64 substeps).
+FRAME = 4
+SYNC = 64
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_dam_break_1024x576_pmc.md")
 
 
@@ -116,18 +123,26 @@ class Tanks:
         self.sims = [make_sim(*sp) for sp in specs]
         self.batch = FluidBatch(self.sims) if len(self.sims) > 1 else None
 
+    def chunk(self):
+        return CHUNK if self.batch is not None else SYNC
+
     def step(self, k):
         if self.batch is not None:
             self.batch.record(k, k / 240.0)
             self.batch.wait()
         else:
-            self.sims[0].record(k, k / 240.0)
-            self.sims[0].wait()
+            s = self.sims[0]
+            done = 0
+            while done < k:
+                f = min(FRAME, k - done)
+                s.record(f, f / 240.0)
+                done += f
+            s.wait()
 
     def run(self, n):
         done = 0
         while done < n:
-            k = min(CHUNK, n - done)
+            k = min(self.chunk(), n - done)
             self.step(k)
             done += k
 
@@ -456,7 +471,7 @@ def main():
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
-        k = min(CHUNK, args.steps - done)
+        k = min(T.chunk(), args.steps - done)
         T.step(k)
         if T.batch is None:
             ker.append(sim.kernel_time(0))
@@ -503,7 +518,10 @@ def main():
         "cell_size_m": float(params.cell_size),
         "dt": "1/240 s fixed",
         "pressure_tol": float(params.pressure_tol),
-        "substeps_per_batch": CHUNK,
+        "substeps_per_record": FRAME if len(specs) == 1 else CHUNK,
+        "driving": ("a record per 4 substeps (a 1/60 s frame), up to 4 in flight, each adapting its recorded "
+                    f"iterations from the read-backs finished so far; a wait every {SYNC} substeps"
+                    if len(specs) == 1 else f"a batch record of {CHUNK} substeps, then a wait"),
         "mg_storage": "fp32" if mgp == 0 else "bf16 iterates and coarse rhs, fp16 coarse matrices (fp32 smoothing, "
                                               "compensated-fp32 level-0 residual)",
     }

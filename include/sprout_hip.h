@@ -76,13 +76,18 @@ int sp_reset(sp_sim* sim, const sp_particle* particles, uint32_t n);
 int sp_spawn(sp_sim* sim, const sp_particle* particles, uint32_t n, uint32_t* accepted);
 
 /* Replaces `FluidSim::record(builder, substeps, target, tool)` (src/sim/mod.rs:2400-2563):
- * enqueues `substeps` substeps meant to cover `target_s` seconds on the handle's stream and
- * returns without waiting. tool may be NULL (idle); only mode 0 is carried. */
+ * enqueues `substeps` substeps meant to cover `target_s` seconds on the handle's stream, with
+ * the read-back of the batch's records at its end, and returns without waiting (unless four
+ * batches are already in flight: then it waits for the oldest). The recorded iterations come
+ * from the read-backs finished when it is called, as in the reference's frame loop. tool may
+ * be NULL (idle); only mode 0 is carried. */
 int sp_record(sp_sim* sim, uint32_t substeps, float target_s, const sp_tool_state* tool);
 /* Waits for everything recorded on the handle (submit_and_wait, src/sim/mod.rs:3483-3491). */
 int sp_wait(sp_sim* sim);
 /* Replaces `FluidSim::poll_readbacks` + `solver` (src/sim/mod.rs:3033-3116, :334-363):
- * copies the solver statistics of the batches finished so far. Never blocks. */
+ * takes up the read-backs of the batches finished so far, oldest first (simulated time,
+ * statistics, the next batches' recorded iterations) and copies the statistics. Never
+ * blocks. */
 int sp_poll(sp_sim* sim, sp_solver_stats* out);
 
 /* `FluidSim::advance_blocking(seconds)` (src/sim/mod.rs:3284-3312). */
@@ -104,7 +109,7 @@ int sp_dt_estimate(sp_sim* sim, float* out);
  * record for the first pressure solve, the second and the viscosity solve. */
 int sp_recorded_iters(sp_sim* sim, uint32_t* out3);
 /* `FluidSim::substep_cost` (src/sim/mod.rs:1972, 3055-3059): device time of one substep,
- * s, from events around each batch recorded by sp_record and waited for by sp_wait,
+ * s, from events around each batch recorded by sp_record, taken up by sp_poll or sp_wait,
  * smoothed as x += 0.1 (cost - x); 0 before the first measurement. */
 int sp_substep_cost(sp_sim* sim, float* out);
 /* `FluidSim::profile_passes` (src/sim/mod.rs:3120-3150): runs one substep (it advances the

@@ -286,8 +286,13 @@ const char* sp_pool_last_error(sp_pool* v) {
 int sp_pool_add(sp_pool* v, sp_sim* s, uint32_t* scene) {
     if (!v || !s || !scene) return SP_E_INVALID;
     if (s->device != v->be.device || s->pooled || s->in_batch || !s->params_set) return SP_E_INVALID;
-    // the tables the launch groups are keyed on, while the handle is still the caller's
-    int rc = guarded(s, [&]() -> int { return upload_tables(s); });
+    // the tables the launch groups are keyed on, while the handle is still the caller's, and
+    // the read-backs of the batches it recorded itself
+    int rc = guarded(s, [&]() -> int {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        poll_readbacks(s, true);
+        return upload_tables(s);
+    });
     if (rc != SP_OK) return rc;
     if (big_bytes(s) > v->core->budget()) return SP_E_CAPACITY;
     bool ok = v->core->add(scene, [&](uint32_t id) {

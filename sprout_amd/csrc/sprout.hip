@@ -105,10 +105,21 @@ struct sp_sim {
 
     GraphSet gs;
     Timing tm;
-    // substep_cost (mod.rs:3055-3059): events around the latest batch sp_record enqueued
-    hipEvent_t batch_ev[2] = {nullptr, nullptr};
-    uint32_t batch_substeps = 0;  // substeps between them; 0: nothing to measure
-    float substep_cost = 0.0f;    // s, smoothed
+    // Read-backs of the batches sp_record enqueued (poll_readbacks, mod.rs:3030-3115): each
+    // batch ends with a copy of the step and solve records into a slot of a ring in pinned
+    // host memory and an event, with events around its substeps for substep_cost
+    // (mod.rs:3055-3059). Finished slots are taken up in order without blocking; only then
+    // does the tank adapt its recorded iterations, so a caller may keep batches in flight.
+    static constexpr uint32_t RB_RING = 4;
+    struct Readback {
+        sp_stepbuf S;
+        sp_solve K;
+    };
+    Readback* rb = nullptr;  // RB_RING slots, pinned
+    hipEvent_t rb_start[RB_RING] = {}, rb_end[RB_RING] = {};
+    uint32_t rb_substeps[RB_RING] = {};
+    uint32_t rb_next = 0, rb_busy = 0;  // oldest slot in flight, slots in flight
+    float substep_cost = 0.0f;          // s, smoothed
     sp_gpumat* d_gpumat = nullptr;  // the material table in GpuMat layout (SP_RENDER_GPUMAT)
     sp_gpumat gpumat[SP_MAX_MATERIALS]{};
     bool in_batch = false;  // a batch drives the tank (sp_batch_*)
@@ -404,7 +415,35 @@ void clear_maxima(sp_sim* s, hipStream_t stream, const uint32_t* zero) {
     HIPCHK(hipMemcpyAsync(&s->c.K->v_it_max, zero, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
 }
 
+// Takes up the finished read-backs of sp_record's batches, oldest first (poll_readbacks,
+// mod.rs:3030-3115); `block`: waits for every one in flight.
+void poll_readbacks(sp_sim* s, bool block) {
+    while (s->rb_busy > 0) {
+        const uint32_t k = s->rb_next;
+        if (block) {
+            HIPCHK(hipEventSynchronize(s->rb_end[k]));
+        } else {
+            const hipError_t e = hipEventQuery(s->rb_end[k]);
+            if (e == hipErrorNotReady) return;
+            HIPCHK(e);
+        }
+        if (s->rb_substeps[k] > 0) {
+            // the batch's device time per substep, smoothed over about ten batches
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, s->rb_start[k], s->rb_end[k]));
+            const float cost = ms * 1e-3f / (float)s->rb_substeps[k];
+            s->substep_cost = s->substep_cost > 0.0f ? s->substep_cost + 0.1f * (cost - s->substep_cost) : cost;
+        }
+        readback_apply(s, s->rb[k].S, s->rb[k].K);
+        s->rb_next = (k + 1) % sp_sim::RB_RING;
+        s->rb_busy--;
+    }
+}
+
+// A read-back outside the ring (batches read their tanks back themselves), after the
+// ring's.
 void readback(sp_sim* s) {
+    poll_readbacks(s, true);
     sp_stepbuf S;
     sp_solve K;
     HIPCHK(hipMemcpyAsync(&S, s->c.S, sizeof(S), hipMemcpyDeviceToHost, s->stream));
@@ -481,34 +520,40 @@ void maybe_sort(sp_sim* s, hipStream_t stream) {
     s->since_sort++;
 }
 
+// FluidSim::record (mod.rs:2400-2563): the batch on the tank's stream, with the recorded
+// iterations that the read-backs finished so far set, and its read-back at its end.
 int record(sp_sim* s, uint32_t substeps, float target) {
+    poll_readbacks(s, false);
+    if (s->rb_busy == sp_sim::RB_RING) {
+        // every slot in flight: the oldest batch has to finish first
+        HIPCHK(hipEventSynchronize(s->rb_end[s->rb_next]));
+        poll_readbacks(s, false);
+    }
     int rc = record_prepare(s, substeps, target);
     if (rc != SP_OK) return rc;
-    if (substeps > 0 && s->batch_ev[0]) HIPCHK(hipEventRecord(s->batch_ev[0], s->stream));
+    const uint32_t slot = (s->rb_next + s->rb_busy) % sp_sim::RB_RING;
+    HIPCHK(hipEventRecord(s->rb_start[slot], s->stream));
     for (uint32_t k = 0; k < substeps; k++) {
         maybe_sort(s, s->stream);
         launch_substep(s->run, &s->gs, &s->tm, s->grid_valid ? 1 : 0, s->rec);
         s->grid_valid = true;
     }
-    if (substeps > 0 && s->batch_ev[1]) {
-        HIPCHK(hipEventRecord(s->batch_ev[1], s->stream));
-        s->batch_substeps = substeps;
-    }
+    // the batch's records, then the per-batch iteration maxima start again
+    HIPCHK(hipMemcpyAsync(&s->rb[slot].S, s->c.S, sizeof(sp_stepbuf), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(&s->rb[slot].K, s->c.K, sizeof(sp_solve), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemsetAsync(&s->c.K->it_max, 0, sizeof(uint32_t), s->stream));
+    HIPCHK(hipMemsetAsync(&s->c.K->p2_it_max, 0, sizeof(uint32_t), s->stream));
+    HIPCHK(hipMemsetAsync(&s->c.K->v_it_max, 0, sizeof(uint32_t), s->stream));
+    HIPCHK(hipEventRecord(s->rb_end[slot], s->stream));
+    s->rb_substeps[slot] = substeps;
+    s->rb_busy++;
     return SP_OK;
 }
 
 int wait(sp_sim* s) {
     HIPCHK(hipStreamSynchronize(s->stream));
     s->tm.collect();
-    if (s->batch_substeps > 0) {
-        // the latest batch's device time per substep, smoothed over about ten batches
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, s->batch_ev[0], s->batch_ev[1]));
-        const float cost = ms * 1e-3f / (float)s->batch_substeps;
-        s->substep_cost = s->substep_cost > 0.0f ? s->substep_cost + 0.1f * (cost - s->substep_cost) : cost;
-        s->batch_substeps = 0;
-    }
-    readback(s);
+    poll_readbacks(s, true);
     return SP_OK;
 }
 
@@ -644,8 +689,11 @@ int sp_create(const sp_config* cfg, sp_sim** out) {
         HIPCHK(hipSetDevice(s->device));
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->tm.create();
-        HIPCHK(hipEventCreate(&s->batch_ev[0]));
-        HIPCHK(hipEventCreate(&s->batch_ev[1]));
+        for (uint32_t k = 0; k < sp_sim::RB_RING; k++) {
+            HIPCHK(hipEventCreate(&s->rb_start[k]));
+            HIPCHK(hipEventCreate(&s->rb_end[k]));
+        }
+        HIPCHK(hipHostMalloc((void**)&s->rb, sizeof(sp_sim::Readback) * sp_sim::RB_RING, hipHostMallocDefault));
         sp::levels(s->W, s->H, s->lv);
         s->nb0 = ((s->W + 15) / 16) * ((s->H + 15) / 16);
         {
@@ -711,8 +759,11 @@ void sp_destroy(sp_sim* s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->gs.clear();
-    for (hipEvent_t& e : s->batch_ev)
-        if (e) (void)hipEventDestroy(e);
+    for (uint32_t k = 0; k < sp_sim::RB_RING; k++) {
+        if (s->rb_start[k]) (void)hipEventDestroy(s->rb_start[k]);
+        if (s->rb_end[k]) (void)hipEventDestroy(s->rb_end[k]);
+    }
+    if (s->rb) (void)hipHostFree(s->rb);
     free_big(s);
     for (void* p : s->owned) (void)hipFree(p);
     if (s->host_arena) (void)hipHostFree(s->host_arena);
@@ -822,6 +873,7 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         HIPCHK(hipMemsetAsync(s->c.ledger, 0, sizeof(uint32_t) * SP_LEDGER_WORDS, s->stream));
         HIPCHK(hipMemsetAsync(s->c.pstress, 0, sizeof(float4) * (size_t)std::max<uint32_t>(s->cap, 1), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
+        poll_readbacks(s, true);  // what ran before the reset counts towards the old bases
         s->pending_spawn.clear();
         s->spawn_speed = 0.0f;
         s->resets++;
@@ -893,6 +945,7 @@ int sp_wait(sp_sim* s) {
 int sp_poll(sp_sim* s, sp_solver_stats* out) {
     return guarded(s, [&]() -> int {
         if (!out) return SP_E_INVALID;
+        poll_readbacks(s, false);
         *out = s->stats;
         return SP_OK;
     }, false);
