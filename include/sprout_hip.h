@@ -143,8 +143,8 @@ int sp_download_bodies(sp_sim* sim, double* out, uint32_t cap, uint32_t* n);
 int sp_download_pressure(sp_sim* sim, float* pres, float* dt);
 /* `FluidSim::download_ledger` (src/sim/mod.rs:3389-3406): energy that crossed the boundary
  * since the last reset, J per metre of depth: [0] gained from air, [1] lost to air, [2]
- * gained from solids, [3] lost to solids, [4]-[7] heat tool, cool tool, boiled off, made
- * inside (0 here: those passes are not carried). */
+ * gained from solids, [3] lost to solids, [4]-[7] heat tool, cool tool (0 here: the tools
+ * are not carried), boiled off (boiling, evaporation, what reactions use up), made inside. */
 int sp_download_ledger(sp_sim* sim, double* energy8);
 /* The counts of `FluidSim::download_ledger` (src/sim/mod.rs:3400-3404): per table row the
  * particles boiled or evaporated away, the residues left and the steam made since the last

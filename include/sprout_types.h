@@ -41,6 +41,9 @@ extern "C" {
 #define SP_ROW_HONEY 1
 #define SP_ROW_OIL 2
 #define SP_ROW_MERCURY 8
+#define SP_ROW_BRINE 14   /* material.rs: Material::Brine */
+#define SP_ROW_SALT 56    /* material.rs: Material::Salt, what boiling brine leaves */
+#define SP_ROW_STEAM 57   /* material.rs: Material::Steam */
 #define SP_BUILTIN_COUNT 59  /* presets.rs:1130-1135: REAL_COUNT + SIMPLE_COUNT + 3 */
 
 /* `struct Particle` (common.glsl:11-19, params.rs:37-54): 48 bytes, std430. */
@@ -98,13 +101,18 @@ typedef struct sp_material {
     float susceptibility, saturation;
     /* Evaporation below the boiling point (material.rs:307-310, 326-327): the share of the
      * liquid at an open surface that evaporates per second at 20 deg C (doubling every 10 K;
-     * 0 = none), and how much visible vapour it gives off (0 here: puffs are not carried). */
+     * 0 = none), and how much visible vapour it gives off where it evaporates or boils away (a
+ * puff of mist, kind SP_KIND_PUFF; 0 = none). */
     float volatility, fume;
     /* Reactions and heat made inside (material.rs:305-320): in a cell that also holds row
      * `react_with` (-1: none) the liquid turns into row `react_into` (-1: it is used up) at
      * `react_rate` per second, giving off `react_heat` J/kg; `self_heat` W/kg of its own. */
     int32_t react_with, react_into;
     float react_rate, react_heat, self_heat;
+    /* Boiling (material.rs:221-225): the table row a liquid leaves behind when it boils away
+     * (-1: none) and the volume share of its concentrate that becomes that residue. */
+    int32_t residue;
+    float residue_share;
 } sp_material;
 
 /* `SimParams` (src/sim/params.rs:149-376), the fields the isothermal step reads, SI units. */
@@ -172,6 +180,14 @@ typedef struct sp_sim_params {
     /* Magnetism (params.rs:218): how much of a local shape demagnetisation is blended in
      * (0 = a sphere's, N = 1/3, everywhere). */
     float magnet_demag;
+    /* Steam (params.rs:219-249): 1 = boiling aqueous liquid turns into bubbles of steam that
+     * rise, condense in colder liquid and escape at the surface as puffs (0: boiled liquid
+     * leaves at once); the vapour quality at which boiling water becomes steam and below which
+     * steam condenses; the exchange rate among the particles of a cell holding steam (1/s);
+     * the escaped puffs' rise speed (m/s), lifetime (s), and the speed (m/s) and size (m) of
+     * the air's eddies they ride. */
+    uint32_t steam;
+    float steam_boil, steam_condense, steam_mixing, steam_rise, steam_life, steam_swirl, steam_eddy;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -230,8 +246,8 @@ typedef struct sp_stats_out {
 /* One row of the reference's GPU material table in its own 208-byte layout (`GpuMat`,
  * src/sim/material.rs:508-549, filled by gpu_materials, src/sim/mod.rs:2179-2236), for a
  * renderer that reads `material_buffer`. The kernels of this library read `sp_gmat`; this
- * table is the same numbers in the reference's order. Fields this build does not carry
- * (residue share, fizz) are 0, the residue row -1. */
+ * table is the same numbers in the reference's order. The one field this build does not
+ * carry (fizz, the renderer's bubbles) is 0. */
 typedef struct sp_gpumat {
     int32_t ia[4];   /* phase group, diluent row, weight channel, concentration channel */
     int32_t ib[4];   /* family, rheology model, temperature model, residue row */
@@ -269,6 +285,9 @@ typedef struct sp_gmat {
     float gr[4];     /* internal friction; shear modulus (cells^2/s^2, GpuMat el.x); relaxation time (s, el.y); susceptibility (el.z) */
     float rx[4];     /* GpuMat rx: reaction rate (1/s), heat of reaction (K of the liquid), self-heating (K/s), volatility (1/s) */
     int32_t ic[4];   /* GpuMat ic: the partner row while it is in the tank (-1: no reaction), the product row, the partner's index among the tank's rows, 0 */
+    float residue_share;  /* GpuMat el.w: volume share of the concentrate left as residue */
+    float fume;           /* GpuMat gr.y: vapour of a puff it leaves (0: none) */
+    float pad_m[2];
 } sp_gmat;
 
 /* The grid-unit parameter block (`GpuParams`, params.rs:1215-1302, filled at mod.rs:
@@ -326,6 +345,12 @@ typedef struct sp_gparams {
      * rate (1/s; negative: a drain); the velocity driven towards (cells/s). ext[3] counts them. */
     float current_zone[SP_MAX_CURRENTS][4];
     float current_vel[SP_MAX_CURRENTS][4];
+    /* Steam (GpuParams steam, steam2; mod.rs:2322-2323): the quality below which steam
+     * condenses, the puffs' rise (cells/s), their lifetime (s), the quality at which boiling
+     * water becomes steam; the exchange rate in a cell holding steam (1/s), 1 if boiling makes
+     * steam, the air's eddy speed (cells/s) and size (cells). */
+    float steam[4];
+    float steam2[4];
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */
@@ -424,6 +449,12 @@ typedef struct sp_levels_info {
 /* Nor this: the tank has currents or drains (the particle kernel with their code runs). */
 #define SP_CAP_CURRENTS 0x40000u
 #define SP_CAP_REACT 256u
+#define SP_CAP_STEAM 128u  /* a thermal tank holding the steam row (mod.rs:2752-2756) */
+/* Not reference capabilities: a thermal tank with a liquid that can boil (the particle kernel
+ * with the boiling code runs), and a tank that may hold puffs of vapour (steam escaped at the
+ * surface, or a liquid that fumes as it evaporates or boils away). */
+#define SP_CAP_BOILS 0x80000u
+#define SP_CAP_PUFFS 0x100000u
 #define SP_REACT_MIN 0.02f  /* common.glsl:259 */
 #define SP_MU0 1.25663706e-6f  /* magnet.comp MU0 */
 /* Viscoelastic stress (common.glsl:376-382): fixed-point scale of stress / el_g in the
@@ -548,7 +579,7 @@ enum {
 #endif
 SP_STATIC_ASSERT(sizeof(sp_particle) == 48, "Particle is 48 bytes (common.glsl:11-19)");
 SP_STATIC_ASSERT(sizeof(sp_stepbuf) == 48, "StepBuf is 12 words (common.glsl:122-135)");
-SP_STATIC_ASSERT(sizeof(sp_gmat) == 192, "one grid-unit material row is 48 words");
+SP_STATIC_ASSERT(sizeof(sp_gmat) == 208, "one grid-unit material row is 52 words");
 SP_STATIC_ASSERT(sizeof(sp_gpumat) == 208, "GpuMat is 208 bytes (material.rs:549)");
 SP_STATIC_ASSERT(sizeof(sp_solve) % 8 == 0, "FP64 Krylov scalars stay 8-byte aligned");
 SP_STATIC_ASSERT(sizeof(sp_cg_state) == 112, "CG state: 8 doubles, 12 words, no padding");

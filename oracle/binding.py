@@ -55,6 +55,7 @@ def lib():
         L.or_stats.argtypes = [P, P, P, P]
         L.or_last_iters.argtypes = [P]
         L.or_last_iters.restype = C.c_uint32
+        L.or_air_velocity.argtypes = [P, C.c_float, C.c_float, C.c_float, P, P]
         _lib = L
     return _lib
 
@@ -113,6 +114,13 @@ class OracleSim:
 
     def n_slots(self):
         return self._lib.or_n_slots(self._h)
+
+    def air_velocity(self, x, y, lean):
+        """The air's velocity (cells / s) and the liquid occupancy at (x, y) (air.glsl)."""
+        v = (C.c_float * 2)()
+        occ = C.c_float()
+        self._lib.or_air_velocity(self._h, float(x), float(y), float(lean), C.addressof(v), C.addressof(occ))
+        return float(v[0]), float(v[1]), float(occ.value)
 
     def sort(self):
         self._lib.or_sort(self._h)

@@ -4,8 +4,9 @@
  * times the CPU baseline; nothing in the product calls it.
  *
  * Scope: the substep with heat, rigid bodies, rheology, viscoelasticity, settling and
- * magnetism, evaporation, reactions and heat made inside, without steam, boiling and
- * residues, tools, currents or film spatter:
+ * magnetism, evaporation, reactions and heat made inside, boiling, steam, residues and the
+ * puffs of vapour riding the air (air.glsl), currents and drains; without the mouse tools,
+ * the smoke field puffs feed (a renderer input) or film spatter:
  *   step.comp, particles.comp (gather, drift, family exchange, advect, collide, scatter),
  *   grid_prep.comp, grid_coef.comp, surf_smooth/curv/apply.comp, visc_setup.comp +
  *   visc_apply.comp mode 2 (explicit viscosity), the implicit viscosity solve
@@ -21,8 +22,15 @@
  * terminal velocity of the drag balance and the cancelling volume flux, the family exchange
  * by its relaxation rate and conservation, the crowd jitter by its bounds and uniformity
  * (test_oracle_drift.py), self-heating and the heat of reaction by the energy they add
- * (test_oracle_react.py), and the magnetic resting rule by the direction of the pull
- * (test_oracle_magnet.py).
+ * (test_oracle_react.py), the magnetic resting rule by the direction of the pull
+ * (test_oracle_magnet.py), currents and drains by their relaxation and removal rates
+ * (test_oracle_currents.py), and boiling, steam, residues and puffs
+ * (test_oracle_steam.py): the exact latent-heat threshold and the heat booked for what
+ * left, the residue share and its solid state at the boiling point, steam forming at its
+ * vapour quality, escaping at the surface with probability x and condensing into the
+ * liquid it boiled from with its heat content kept, the bubble's drift at the drag
+ * balance's terminal velocity, the exchange rate of a cell holding steam, a puff's
+ * exponential take-up of the air's rise and its lifetime, and divergence-free eddies.
  *
  * Every float operation follows the shader's order (compile with -ffp-contract=off), so
  * that the integer scatter sums and cell indices can be compared bit for bit. The CG
@@ -380,6 +388,138 @@ static int collide(const or_sim* s, float* x, float* y, float* vx, float* vy, in
     return 0;
 }
 
+/* ---- the air above the liquid (air.glsl:91-175): a procedural flow escaped vapour rides --- */
+
+#define PUFF_FOLLOW 0.08f /* air.glsl:102 */
+
+/* vnoise3 (air.glsl:108-128): value noise in [-0.5, 0.5) and its gradient. */
+static v4 vnoise3(float x, float y, float z) {
+    float ix = floorf(x), iy = floorf(y), iz = floorf(z);
+    float fx = x - ix, fy = y - iy, fz = z - iz;
+    float ux = fx * fx * fx * (fx * (fx * 6.0f - 15.0f) + 10.0f);
+    float uy = fy * fy * fy * (fy * (fy * 6.0f - 15.0f) + 10.0f);
+    float uz = fz * fz * fz * (fz * (fz * 6.0f - 15.0f) + 10.0f);
+    float dux = 30.0f * fx * fx * (fx * (fx - 2.0f) + 1.0f);
+    float duy = 30.0f * fy * fy * (fy * (fy - 2.0f) + 1.0f);
+    float duz = 30.0f * fz * fz * (fz * (fz - 2.0f) + 1.0f);
+    uint32_t cx = (uint32_t)((int)ix + 4096), cy = (uint32_t)((int)iy + 4096), cz = (uint32_t)((int)iz + 4096);
+    float r[8];
+    for (int k = 0; k < 8; k++) {
+        uint32_t qx = cx + (uint32_t)(k & 1), qy = cy + (uint32_t)((k >> 1) & 1), qz = cz + (uint32_t)((k >> 2) & 1);
+        r[k] = (float)(hash_u(qx * 73856093u ^ qy * 19349663u ^ qz * 83492791u) >> 8) / 16777216.0f;
+    }
+    float k0 = r[0], k1 = r[1] - r[0], k2 = r[2] - r[0], k3 = r[4] - r[0];
+    float k4 = r[0] - r[1] - r[2] + r[3], k5 = r[0] - r[2] - r[4] + r[6], k6 = r[0] - r[1] - r[4] + r[5];
+    float k7 = -r[0] + r[1] + r[2] - r[3] + r[4] - r[5] - r[6] + r[7];
+    float n = k0 + k1 * ux + k2 * uy + k3 * uz + k4 * ux * uy + k5 * uy * uz + k6 * uz * ux + k7 * ux * uy * uz;
+    return (v4){n - 0.5f, dux * (k1 + k4 * uy + k6 * uz + k7 * uy * uz), duy * (k2 + k4 * ux + k5 * uz + k7 * ux * uz),
+                duz * (k3 + k5 * uy + k6 * ux + k7 * ux * uy)};
+}
+
+/* air_psi (air.glsl:130-141): the eddies' stream function and its gradient. */
+static v4 air_psi(const sp_gparams* P, float x, float y, float t) {
+    float L = fmaxf_(P->steam2[3], 1.0f);
+    float turn = 0.7f * t * P->steam2[2] / L;
+    float ax = x / L, ay = (y - 0.5f * P->steam[1] * t) / L;
+    v4 n1 = vnoise3(ax, ay, turn);
+    v4 n2 = vnoise3(ax * 2.17f + 17.3f, ay * 2.17f + 17.3f, 2.2f * turn);
+    float amp = 2.2f * P->steam2[2] * L;
+    return (v4){amp * (n1.x + 0.45f * n2.x), amp * ((n1.y + 0.45f * 2.17f * n2.y) / L), amp * ((n1.z + 0.45f * 2.17f * n2.z) / L), 0.0f};
+}
+
+/* puff_lean (air.glsl:143-146) */
+static float puff_lean(uint32_t pid) { return ((float)(hash_u(pid ^ 0x5bd1e995u) >> 8) / 16777216.0f - 0.5f) * 0.3f; }
+
+/* sample_occ (air.glsl:81-88): liquid occupancy, bilinear over the cell centres. */
+static float sample_occ(const or_sim* s, float x, float y) {
+    float gx = clampf(x - 0.5f, 0.0f, (float)s->W - 1.001f), gy = clampf(y - 0.5f, 0.0f, (float)s->H - 1.001f);
+    int x0 = (int)gx, y0 = (int)gy;
+    float fx = gx - (float)x0, fy = gy - (float)y0;
+    int k = y0 * s->W + x0;
+    return mixf(mixf(s->cell[k].x, s->cell[k + 1].x, fx), mixf(s->cell[k + s->W].x, s->cell[k + s->W + 1].x, fx), fy);
+}
+
+/* air_velocity (air.glsl:148-175), the idle tool. */
+static v2 air_velocity(const or_sim* s, float x, float y, float lean, float* occ) {
+    const sp_gparams* P = &s->P;
+    float t = s->S.advanced;
+    float nx, ny;
+    float d = sample_sdf(s, x, y, &nx, &ny);
+    float nl = len2(nx, ny);
+    if (nl > 1e-5f) {
+        nx = nx / nl;
+        ny = ny / nl;
+    } else {
+        nx = 0.0f;
+        ny = 1.0f;
+    }
+    float Lb = fmaxf_(P->steam2[3], 1.0f);
+    float r = clampf(d / Lb, 0.0f, 1.0f);
+    float ramp = r * (1.875f - r * r * (1.25f - 0.375f * r * r));
+    float dramp = d < Lb && d > 0.0f ? (1.875f - r * r * (3.75f - 1.875f * r * r)) / Lb : 0.0f;
+    float vx = 0.0f, vy = 0.0f;
+    if (P->steam2[2] > 0.0f) {
+        v4 psi = air_psi(P, x, y, t);
+        vx = ramp * psi.z + psi.x * dramp * ny;
+        vy = ramp * -psi.y + psi.x * dramp * -nx;
+    }
+    float rx = lean * P->steam[1], ry = 1.0f * P->steam[1];
+    float into = rx * nx + ry * ny;
+    if (into < 0.0f) {
+        float tx = ny, ty = -nx;
+        float side = (rx * tx + ry * ty) + lean * 1e-3f >= 0.0f ? 1.0f : -1.0f;
+        rx -= (nx + side * tx * 0.7f) * into * (1.0f - ramp);
+        ry -= (ny + side * ty * 0.7f) * into * (1.0f - ramp);
+    }
+    vx += rx;
+    vy += ry;
+    *occ = sample_occ(s, x, y);
+    v2 vl = sample_vel(s, x, y, (v2){vx, vy - P->steam[1]});
+    float k = smoothstepf(0.0f, 0.4f, *occ);
+    return (v2){mixf(vx, vl.x, k), mixf(vy, vl.y + P->steam[1], k)};
+}
+
+/* A puff of vapour (particles.comp:103-134): a tracer in the air, gone after steam.z seconds,
+ * at the grid's edge, or swallowed by the liquid. (The smoke field it feeds is the
+ * renderer's and is not restated.) */
+static void advance_puff(or_sim* s, uint32_t id, sp_particle p) {
+    const sp_gparams* P = &s->P;
+    float dt = s->S.dt;
+    p.s += dt;
+    if (p.s >= P->steam[2] || p.pos[0] < 2.01f || p.pos[1] < 2.01f || p.pos[0] > (float)s->W - 2.01f || p.pos[1] > (float)s->H - 2.01f) {
+        s->part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    float occ;
+    v2 target = air_velocity(s, p.pos[0], p.pos[1], puff_lean(p.id), &occ);
+    if (occ > 0.6f && p.s > 0.1f) {
+        s->part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    float k = 1.0f - expf(-dt / PUFF_FOLLOW);
+    float vx = mixf(p.vel[0], target.x, k), vy = mixf(p.vel[1], target.y, k);
+    float x = p.pos[0] + vx * dt, y = p.pos[1] + vy * dt;
+    collide(s, &x, &y, &vx, &vy, 0);
+    p.pos[0] = x;
+    p.pos[1] = y;
+    p.vel[0] = vx;
+    p.vel[1] = vy;
+    s->part[id] = p;
+}
+
+/* The particle leaves as a puff of vapour `fume` (particles.comp:704-712, 796-800). */
+static void become_puff(or_sim* s, uint32_t id, sp_particle p, float vx, float vy, float fume) {
+    p.kind = SP_KIND_PUFF;
+    p.s = 0.0f;
+    p.vel[0] = vx;
+    p.vel[1] = vy;
+    p.c[0] = fume;
+    p.c[1] = 0.0f;
+    p.c[2] = 0.0f;
+    p.c[3] = 0.0f;
+    s->part[id] = p;
+}
+
 /* ---- step.comp (step.comp:22-50) ------------------------------------------------------ */
 
 static void stage_step(or_sim* s) {
@@ -570,7 +710,10 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
     if (id >= s->n_slots) return;
     sp_particle p = s->part[id];
     if (p.kind == SP_KIND_DEAD) return;
-    if ((p.kind & 255u) == 0u) return;  /* a puff: not carried (no steam without heat) */
+    if ((p.kind & 255u) == 0u) {
+        advance_puff(s, id, p);
+        return;
+    }
     float dt = s->S.dt;
     int row = (int)(p.kind & 255u) - 1;
     const sp_gmat* M = &s->mats[row];
@@ -600,7 +743,7 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
         int held = 0;
         if (rig) {
             const int px1 = clampi((int)floorf(p.pos[0]), 1, s->W - 2), py1 = clampi((int)floorf(p.pos[1]), 1, s->H - 2);
-            const int in_body = s->rl[2 * cidx(s, px1, py1)] >= 0;
+            const int in_body = row != SP_ROW_STEAM && s->rl[2 * cidx(s, px1, py1)] >= 0;
             const int px0 = clampi((int)floorf(p.pos[0]), 0, s->W - 1), py0 = clampi((int)floorf(p.pos[1]), 0, s->H - 1);
             const int plug = (s->mats[row].rheo_p[0] > 0.0f || s->mats[row].temp[3] > 0.0f) && s->cell[cidx(s, px0, py0)].w <= 0.0f;
             if (frozen || plug || in_body) {
@@ -654,6 +797,9 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
             int groups = (P->group_mask & (P->group_mask - 1u)) != 0u;
             int sorts = group_sorts(s, Mgroup);
             float sbg = 0.0f;
+            /* the share of steam around it (particles.comp:303-305) */
+            int steamy = (P->caps & SP_CAP_STEAM) && Mgroup == 0;
+            float sfv = 0.0f;
             int nmix = P->n_fam - 1;
             for (int b = 0; b < 3; b++)
                 for (int a = 0; a < 3; a++) {
@@ -669,6 +815,7 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                             sgz += wt * s->aux[ci].w;
                         }
                         if (sorts) sbg += wt * v4c(s->gbase[ci], Mgroup - 1);
+                        if (steamy) sfv += wt * s->gbase[ci].w;
                         float sb = fam > 0 ? s->mixb[(size_t)ci * nmix + fam - 1] : q.z;
                         if (sb >= 0.0f) {
                             sfw += wt;
@@ -811,6 +958,19 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                         slip -= fminf_(vt, sep_max) * signf(drho);
                     }
                 }
+                /* Steam rises through the water around it (particles.comp:505-517): a bubble
+                 * of the unresolved size at its terminal velocity in water, by the share of
+                 * water; the water drifts down by the share of steam. */
+                if (steamy && !held) {
+                    float fv = clampf(sfv / sw, 0.0f, 1.0f);
+                    if (fv > 1e-4f) {
+                        float rw = s->mats[SP_ROW_WATER].rho;
+                        float vt = terminal(s, s->mats[SP_ROW_STEAM].rho - rw, rw, expf(s->mats[SP_ROW_WATER].ln_nu), P->sep_tau);
+                        float sep_max = P->corr[2] > 0.0f ? P->cfl / dt : 1e30f;
+                        vt = fminf_(vt, sep_max);
+                        slip += row == SP_ROW_STEAM ? vt * (1.0f - fv) : -vt * fv;
+                    }
+                }
                 /* Miscible exchange within a family (particles.comp:527-543). */
                 int diluent0 = fam < 0 && P->primary >= 0 && P->fams[0][1] == row;
                 if (fam >= 0 || diluent0) {
@@ -885,6 +1045,92 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                 }
             }
         }
+        /* Boiling (particles.comp:633-731): past its boiling point liquid takes up latent
+         * heat, its share the vapour quality x. Aqueous liquid in a tank that makes steam turns
+         * into steam at x = steam.w (keeping mass, heat and the row it boiled from in KIND_SRC);
+         * steam condenses below steam.x, and at the free surface escapes as a puff with
+         * probability x, else gives up its latent heat there and stays as its liquid. Any other
+         * liquid leaves at x = 1, leaving its residue, one particle per residue share. */
+        if (th_on && (P->caps & SP_CAP_BOILS)) {
+            v4 th = thermal_of(s, row, p.s);
+            int dil = s->mats[row].diluent;
+            float bo_t = s->mats[row].th2[2], bo_l = s->mats[row].th2[3];
+            if (dil >= 0) {
+                bo_t = mixf(s->mats[dil].th2[2], bo_t, p.s);
+                bo_l = mixf(s->mats[dil].th2[3], bo_l, p.s);
+            }
+            int vapour = (P->caps & SP_CAP_STEAM) && Mgroup == 0 && (P->steam2[1] > 0.5f || row == SP_ROW_STEAM);
+            int leave = 0;
+            if (vapour && row == SP_ROW_STEAM) {
+                float x = (p.t - bo_t) / fmaxf_(bo_l, 1e-3f);
+                int pcx = clampi((int)p.pos[0], 0, s->W - 1), pcy = clampi((int)p.pos[1], 0, s->H - 1);
+                int surface = s->cell[cidx(s, pcx, pcy)].w <= 0.0f || s->cell[cidx(s, pcx, clampi(pcy + 1, 0, s->H - 1))].w <= 0.0f;
+                float u = (float)(hash_u(p.id ^ hash_u(s->S.substeps ^ 0x2545f491u)) >> 8) / 16777216.0f;
+                if (surface && u < x) {
+                    leave = 1;
+                } else if (x < P->steam[0] || surface) {
+                    if (surface) {
+                        ledger_add(s, 6, fmaxf_(th.w * (p.t - bo_t), 0.0f) * P->inv_ppc);
+                        p.t = bo_t;
+                    }
+                    uint32_t src = (p.kind & SP_KIND_SRC) >> 8;
+                    row = src > 0u ? (int)src - 1 : SP_ROW_WATER;
+                    M = &s->mats[row];
+                    Mgroup = M->group;
+                    Mdil = M->diluent;
+                    Mcw = M->chan_w;
+                    Mcs = M->chan_s;
+                    fam = M->fam;
+                    p.kind = (p.kind & ~(255u | SP_KIND_SRC)) | (uint32_t)(row + 1);
+                    p.t = keep_heat(p.t, th.w, thermal_of(s, row, p.s).w);
+                }
+            } else if (p.t >= bo_t + bo_l * (vapour ? P->steam[3] : 1.0f)) {
+                if (vapour) {
+                    float c1 = thermal_of(s, SP_ROW_STEAM, 1.0f).w;
+                    s->ledger[16 + 2 * SP_MAX_MATERIALS + row] += 1u;
+                    p.kind = (p.kind & ~(255u | SP_KIND_SRC)) | ((uint32_t)(row + 1) << 8) | (uint32_t)(SP_ROW_STEAM + 1);
+                    row = SP_ROW_STEAM;
+                    M = &s->mats[row];
+                    Mgroup = M->group;
+                    Mdil = M->diluent;
+                    Mcw = M->chan_w;
+                    Mcs = M->chan_s;
+                    fam = -1;
+                    p.t = keep_heat(p.t, th.w, c1);
+                } else {
+                    leave = 1;
+                }
+            }
+            if (leave) {
+                uint32_t src = row == SP_ROW_STEAM ? (p.kind & SP_KIND_SRC) >> 8 : (uint32_t)(row + 1);
+                int from = src > 0u ? (int)src - 1 : SP_ROW_WATER;
+                int res = s->mats[from].residue;
+                float share = s->mats[from].residue_share * (s->mats[from].diluent >= 0 ? p.s : 1.0f);
+                float e = th.w * (p.t + 273.15f);
+                s->ledger[16 + row] += 1u;
+                if (res >= 0 && (float)(hash_u(p.id ^ 0x9e3779b9u) >> 8) / 16777216.0f < share) {
+                    s->ledger[16 + SP_MAX_MATERIALS + res] += 1u;
+                    row = res;
+                    M = &s->mats[row];
+                    Mgroup = M->group;
+                    Mdil = M->diluent;
+                    Mcw = M->chan_w;
+                    Mcs = M->chan_s;
+                    fam = M->fam;
+                    p.kind = (uint32_t)(row + 1);
+                    p.s = 1.0f;
+                    p.t = bo_t - thermal_of(s, row, 1.0f).z; /* the residue is a solid */
+                    e -= thermal_of(s, row, 1.0f).w * (p.t + 273.15f);
+                    ledger_add(s, 6, fmaxf_(e, 0.0f) * P->inv_ppc);
+                } else {
+                    ledger_add(s, 6, fmaxf_(e, 0.0f) * P->inv_ppc);
+                    float fume = row == SP_ROW_STEAM ? 1.0f : s->mats[row].fume;
+                    if (fume > 0.0f) become_puff(s, id, p, vx, vy, fume);
+                    else s->part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+            }
+        }
         /* Heat made inside (particles.comp:733-739), counted in the ledger. */
         if (th_on && s->mats[row].rx[2] > 0.0f) {
             float e = s->mats[row].rx[2] * s->mats[row].th[0] * (s->mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P->heat_rate;
@@ -947,7 +1193,9 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                 if (u < rate * dt) {
                     if (th_on) ledger_add(s, 6, fmaxf_(th.w * (p.t + 273.15f), 0.0f) * P->inv_ppc);
                     s->ledger[16 + row] += 1u;
-                    s->part[id].kind = SP_KIND_DEAD;
+                    /* a liquid that fumes leaves a puff of mist */
+                    if (s->mats[row].fume > 0.0f) become_puff(s, id, p, vx, vy, s->mats[row].fume);
+                    else s->part[id].kind = SP_KIND_DEAD;
                     return;
                 }
             }
@@ -1155,7 +1403,7 @@ static void stage_grid_prep(or_sim* s) {
             float sigma = 0.0f, cx_ = 0.0f, cy_ = 0.0f, fam0 = -1.0f;
             float rho = 0.0f, mu = 0.0f, rho_c = 0.0f, nu_c = 0.0f;
             int major = 0, top_row = -1;
-            float top_share = 0.0f;
+            float top_share = 0.0f, fv = 0.0f; /* fv: the share of steam */
             float T = P->heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
             v4 nn = {0.0f, -1.0f, 1.0f, 0.0f};  /* rh: the cell's non-Newtonian material */
             float chi = 0.0f;
@@ -1263,7 +1511,10 @@ static void stage_grid_prep(or_sim* s) {
                     /* the non-Newtonian material with the largest share (grid_prep.comp:162) */
                     if ((m->rheo != 0 || m->temp[3] > 0.0f || m->gr[0] > 0.0f) && share > nn.x)
                         nn = (v4){share, (float)r, sc, 0.0f};
-                    if (share > top_share) {
+                    /* steam's share, which never makes it the top material (grid_prep.comp:163-167) */
+                    if (r == SP_ROW_STEAM) {
+                        fv = share;
+                    } else if (share > top_share) {
                         top_share = share;
                         top_row = r;
                     }
@@ -1322,7 +1573,7 @@ static void stage_grid_prep(or_sim* s) {
             if (P->caps & SP_CAP_MAGNET) s->magc[i].x = chi; /* grid_prep.comp:240 */
             s->cell[i] = (v4){phi_f, rho, mu, level};
             s->comp[i] = (v4){rho, f[1], fam0, phi_t};
-            if (any_sorts) s->gbase[i] = (v4){bg[0], bg[1], bg[2], 0.0f};
+            if (any_sorts || (P->caps & SP_CAP_STEAM)) s->gbase[i] = (v4){bg[0], bg[1], bg[2], fv};
             if (multi) {
                 s->aux[i] = (v4){nu_c, rho_c, f[2], f[3]};
                 s->grho[i] = (v4){rg[0], rg[1], rg[2], rg[3]};
@@ -1376,6 +1627,11 @@ static void stage_heat(or_sim* s) {
             float own_x = 0.0f, own_y = 0.0f;
             if (hx > 0 && me.y > 0.0f && phi > 0.0f) {
                 float rate = 5.78f * me.z * 3.14159265f / (P->inv_ppc * me.y * P->inv_ppc / phi);
+                /* faster where the cell holds steam and no other liquid (heat.comp:95-104) */
+                if ((P->caps & SP_CAP_STEAM) && s->gbase[i].w > 0.0f) {
+                    int others = (P->group_mask & ~1u) != 0u && s->comp[i].y + s->aux[i].z + s->aux[i].w > 1e-3f;
+                    if (!others) rate = fmaxf_(rate, P->steam2[0]);
+                }
                 own_x = 1.0f - expf(-k * rate);
                 own_y = (float)hy / (float)hx;
             }
@@ -3658,6 +3914,12 @@ void or_begin_batch(or_sim* s, float target, uint32_t substeps) {
 
 uint32_t or_n_slots(or_sim* s) { return s->n_slots; }
 uint32_t or_last_iters(or_sim* s) { return s->K.iters_last; }
+
+void or_air_velocity(or_sim* s, float x, float y, float lean, float* vel2, float* occ) {
+    v2 v = air_velocity(s, x, y, lean, occ);
+    vel2[0] = v.x;
+    vel2[1] = v.y;
+}
 
 static void* field_ptr(or_sim* s, int field, size_t* bytes) {
     size_t N = (size_t)s->N, T = (size_t)s->lv.total;

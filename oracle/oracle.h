@@ -46,6 +46,9 @@ void or_stats(or_sim* s, uint32_t* live, uint32_t* count, uint32_t* conc256);
 void or_levels(int32_t W, int32_t H, sp_levels_info* out);
 /* The pressure iterations of the last solve. */
 uint32_t or_last_iters(or_sim* s);
+/* The air's velocity at (x, y) for a puff leaning `lean` and the liquid occupancy there
+ * (air.glsl:148-175), at the step state's time. */
+void or_air_velocity(or_sim* s, float x, float y, float lean, float* vel2, float* occ);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,9 @@ SPAWN_TEMP = -1.0e4
 SPAWN_S = -1.0
 
 ROW_WATER, ROW_HONEY, ROW_OIL, ROW_MERCURY = 0, 1, 2, 8
+ROW_BRINE, ROW_SALT, ROW_STEAM = 14, 56, 57  # material.rs: Brine, Salt, Steam
+KIND_PUFF = 0x40000  # params.rs:28
+KIND_SRC = 0xFF00    # params.rs:31: of steam, the row + 1 of the liquid it boiled from
 
 # Status codes
 OK, E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_CAPACITY, E_BUSY = 0, -1, -2, -3, -4, -5, -6
@@ -40,6 +43,9 @@ CAP_CURRENTS = 0x40000
 CAP_VOLATILE = 0x10000
 CAP_SELF_HEAT = 0x20000
 CAP_REACT = 256
+CAP_STEAM = 128
+CAP_BOILS = 0x80000
+CAP_PUFFS = 0x100000
 LEDGER_WORDS = 16 + 3 * 96
 STRESS_FIX = 65536.0
 
@@ -77,7 +83,8 @@ class Material(C.Structure):
                 ("relax_time", C.c_float), ("grain", C.c_float), ("susceptibility", C.c_float),
                 ("saturation", C.c_float), ("volatility", C.c_float), ("fume", C.c_float),
                 ("react_with", C.c_int32), ("react_into", C.c_int32), ("react_rate", C.c_float),
-                ("react_heat", C.c_float), ("self_heat", C.c_float)]
+                ("react_heat", C.c_float), ("self_heat", C.c_float), ("residue", C.c_int32),
+                ("residue_share", C.c_float)]
 
 
 class SimParams(C.Structure):
@@ -103,6 +110,9 @@ class SimParams(C.Structure):
                 ("rigid_contact", C.c_uint32), ("rigid_friction", C.c_float),
                 ("elastic_stress_max", C.c_float), ("elastic_step_max", C.c_float),
                 ("magnet_demag", C.c_float),
+                ("steam", C.c_uint32), ("steam_boil", C.c_float), ("steam_condense", C.c_float),
+                ("steam_mixing", C.c_float), ("steam_rise", C.c_float), ("steam_life", C.c_float),
+                ("steam_swirl", C.c_float), ("steam_eddy", C.c_float),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
@@ -186,7 +196,8 @@ class GMat(C.Structure):
                                           "freeze_shift")] + \
                [("th", C.c_float * 4), ("th2", C.c_float * 4), ("temp", C.c_float * 4),
                 ("rheo_p", C.c_float * 4), ("gr", C.c_float * 4), ("rx", C.c_float * 4),
-                ("ic", C.c_int32 * 4)]
+                ("ic", C.c_int32 * 4), ("residue_share", C.c_float), ("fume", C.c_float),
+                ("pad_m", C.c_float * 2)]
 
 
 class GParams(C.Structure):
@@ -212,7 +223,8 @@ class GParams(C.Structure):
                 ("el_stress_max", C.c_float), ("el_step_max", C.c_float), ("mag", C.c_float * 4),
                 ("magnet", (C.c_float * 4) * MAX_MAGNETS), ("magnet_dir", (C.c_float * 2) * MAX_MAGNETS),
                 ("demag", C.c_float), ("current_zone", (C.c_float * 4) * MAX_CURRENTS),
-                ("current_vel", (C.c_float * 4) * MAX_CURRENTS)]
+                ("current_vel", (C.c_float * 4) * MAX_CURRENTS), ("steam", C.c_float * 4),
+                ("steam2", C.c_float * 4)]
 
 
 class DeviceView(C.Structure):

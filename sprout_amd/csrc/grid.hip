@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     float sigma = 0.0f, csx_ = 0.0f, csy_ = 0.0f, fam0 = -1.0f;
     float rho = 0.0f, mu = 0.0f, rho_c = 0.0f, nu_c = 0.0f;
     int major = 0, top_row = -1;
-    float top_share = 0.0f;
+    float top_share = 0.0f, fv = 0.0f;  // fv: the share of steam
     const bool th_on = (P.caps & SP_CAP_THERMAL) != 0u;
     float T = P.heat_x, kappa = 0.0f, emis = 0.0f, frozen = 0.0f, Craw = 0.0f;
     float4 nn = make_float4(0.0f, -1.0f, 1.0f, 0.0f);  // rh: the cell's non-Newtonian material
@@ -235,7 +235,10 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
             }
             // the cell's non-Newtonian material: the one with the largest share (grid_prep.comp:162)
             if ((m.rheo != 0 || m.temp[3] > 0.0f || m.gr[0] > 0.0f) && share > nn.x) nn = make_float4(share, (float)r, sc, 0.0f);
-            if (share > top_share) {
+            // steam's share, which never makes it the cell's top material (grid_prep.comp:163-167)
+            if (r == SP_ROW_STEAM) {
+                fv = share;
+            } else if (share > top_share) {
                 top_share = share;
                 top_row = r;
             }
@@ -302,7 +305,8 @@ __global__ __launch_bounds__(256) void k_grid_prep(const TankBatch tb) {
     c.cell[i] = make_float4(phi_f, rho, mu, level);
     c.comp[i] = make_float4(rho, f[1], fam0, phi_t);
     uint32_t gm = P.group_mask;
-    if (group_sorts(P, c.mats, 1) || group_sorts(P, c.mats, 2) || group_sorts(P, c.mats, 3)) c.gbase[i] = make_float4(bg0, bg1, bg2, 0.0f);
+    if (group_sorts(P, c.mats, 1) || group_sorts(P, c.mats, 2) || group_sorts(P, c.mats, 3) || (P.caps & SP_CAP_STEAM))
+        c.gbase[i] = make_float4(bg0, bg1, bg2, fv);
     if ((gm & (gm - 1u)) != 0u) {
         c.aux[i] = make_float4(nu_c, rho_c, f[2], f[3]);
         c.grho[i] = make_float4(rg[0], rg[1], rg[2], rg[3]);

@@ -74,7 +74,13 @@ __global__ __launch_bounds__(256) void k_heat(const TankBatch tb) {
         const int2 hsum = c.hcell[i];
         float own_x = 0.0f, own_y = 0.0f;
         if (hsum.x > 0 && me.y > 0.0f && phi > 0.0f) {
-            const float rate = 5.78f * me.z * 3.14159265f / (P.inv_ppc * me.y * P.inv_ppc / phi);
+            float rate = 5.78f * me.z * 3.14159265f / (P.inv_ppc * me.y * P.inv_ppc / phi);
+            // a cell of aqueous liquid that holds steam mixes far faster (its bubbles condense
+            // on the water around them), unless another liquid shares the cell (heat.comp:95-104)
+            if ((P.caps & SP_CAP_STEAM) && c.gbase[i].w > 0.0f) {
+                const bool others = (P.group_mask & ~1u) != 0u && c.comp[i].y + c.aux[i].z + c.aux[i].w > 1e-3f;
+                if (!others) rate = fmaxf_(rate, P.steam2[0]);
+            }
             own_x = 1.0f - expf(-k * rate);
             own_y = (float)hsum.y / (float)hsum.x;
         }

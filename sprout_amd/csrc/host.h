@@ -26,6 +26,8 @@ struct Tank {
 void default_params(sp_sim_params* p);
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present);
 bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal = false);
+// The layout with steam and water added where boiling makes steam (Tank::layout_steam).
+bool tank_layout_steam(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal);
 // ext_speed: the fastest speed a pour imposes this record, cells / s (the queued spawns'
 // fastest |vel|; the tool's is 0 for the idle tool this build carries).
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
@@ -34,10 +36,6 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
-// The range of temperatures a thermal tank of `rows` can be carried through: below the
-// lowest boiling point (the steam and boiling passes are not carried); freezing and liquid
-// turning rigid are (rigid bodies). (lowest allowed, highest allowed), deg C.
-std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows);
 void settle_spawn(const sp_sim_params& p, sp_particle& q);
 
 }  // namespace sp

@@ -66,6 +66,8 @@ static sp_material default_material() {
     m.react_rate = 0.0f;
     m.react_heat = 0.0f;
     m.self_heat = 0.0f;
+    m.residue = -1;  // material.rs:376-377
+    m.residue_share = 0.0f;
     return m;
 }
 
@@ -122,6 +124,14 @@ void default_params(sp_sim_params* p) {
     p->elastic_stress_max = 10.0f;  // params.rs:447-448
     p->elastic_step_max = 0.25f;
     p->magnet_demag = 0.0f;  // params.rs:404
+    p->steam = 1;  // params.rs:405-412
+    p->steam_boil = 0.02f;
+    p->steam_condense = 0.005f;
+    p->steam_mixing = 1.0f;
+    p->steam_rise = 0.4f;
+    p->steam_life = 2.0f;
+    p->steam_swirl = 0.1f;
+    p->steam_eddy = 0.2f;
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -190,6 +200,47 @@ void default_params(sp_sim_params* p) {
         m.melt_hi = -38.8f;
         m.latent = 11.4e3f;
     }
+    // presets.rs:491-519 brine: boils at 108.8 deg C and leaves salt, 14.4 % of its volume
+    {
+        sp_material& m = p->mat[SP_ROW_BRINE];
+        m.diluent = SP_ROW_WATER;
+        m.density = 1200.0f;
+        m.viscosity = 1.6e-6f;
+        m.mix_rate = 1.0f;
+        m.tension = 0.082f;
+        m.heat_capacity = 3300.0f;
+        m.conductivity = 0.57f;
+        m.latent = 250.0e3f;
+        m.freeze_shift = -21.0f;
+        m.boil = 108.8f;
+        m.latent_vap = 2257.0e3f;
+        m.residue = SP_ROW_SALT;
+        m.residue_share = 0.144f;
+    }
+    // presets.rs:1064-1087 salt: solid at every temperature the tank reaches
+    {
+        sp_material& m = p->mat[SP_ROW_SALT];
+        m.density = 2165.0f;
+        m.viscosity = 1.0e-3f;
+        m.heat_capacity = 880.0f;
+        m.conductivity = 6.5f;
+        m.expansion = 1.0e-4f;
+        m.melt_lo = 800.0f;
+        m.melt_hi = 801.0f;
+        m.latent = 482.0e3f;
+        m.tension = 0.07f;
+    }
+    // presets.rs:1089-1107 steam: the vapour's density and viscosity; it carries the mass and
+    // heat of the water it boiled from (particle_heat_vol, params.rs:889-892)
+    {
+        sp_material& m = p->mat[SP_ROW_STEAM];
+        m.density = 0.6f;
+        m.viscosity = 2.0e-5f;
+        m.expansion = 0.0f;
+        m.boil = 100.0f;
+        m.latent_vap = 2257.0e3f;
+        m.emissivity = 0.0f;
+    }
 }
 
 static bool freezes(const sp_material& m) { return m.melt_hi > -273.0f; }
@@ -212,7 +263,11 @@ static float tension_cells(const sp_sim_params& p, float sigma) {
 }
 
 // SimParams::dt_capillary (params.rs:962-984).
-static float dt_capillary(const sp_sim_params& p, const std::vector<int>& rows) {
+static float dt_capillary(const sp_sim_params& p, const std::vector<int>& rows_in) {
+    // steam's surface is the water around the bubble (params.rs:964-965)
+    std::vector<int> rows;
+    for (int i : rows_in)
+        if (i != SP_ROW_STEAM) rows.push_back(i);
     float worst = 0.0f;
     for (int i : rows) {
         const sp_material& m = p.mat[i];
@@ -229,14 +284,18 @@ static float dt_capillary(const sp_sim_params& p, const std::vector<int>& rows) 
     return 0.5f * sqrtf(1.0f / (2.0f * 3.14159265f * worst));
 }
 
-// Volumetric heat capacity a particle of the row carries, J/(m^3 K) (params.rs:888-893).
-static float heat_vol(const sp_material& m) { return m.density * m.heat_capacity; }
+// Volumetric heat capacity a particle of the row carries, J/(m^3 K): its own, except
+// steam, which carries the water it boiled from (particle_heat_vol, params.rs:888-893).
+static float heat_vol(const sp_sim_params& p, int row) {
+    const sp_material& m = p.mat[row == SP_ROW_STEAM ? SP_ROW_WATER : row];
+    return m.density * m.heat_capacity;
+}
 
 // SimParams::dt_thermal (params.rs:914-923): a fifth of the fastest cell diffusion time.
 static float dt_thermal(const sp_sim_params& p, const std::vector<int>& rows) {
     if (p.heat_rate <= 0.0f) return INFINITY;
     float dx2 = p.cell_size * p.cell_size, dt = INFINITY;
-    for (int i : rows) dt = std::min(dt, 0.2f * dx2 * heat_vol(p.mat[i]) / std::max(p.mat[i].conductivity, 1e-6f) / p.heat_rate);
+    for (int i : rows) dt = std::min(dt, 0.2f * dx2 * heat_vol(p, i) / std::max(p.mat[i].conductivity, 1e-6f) / p.heat_rate);
     return dt;
 }
 
@@ -260,11 +319,6 @@ float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool therma
     return std::min(dt, dt_elastic(p, rows));
 }
 
-std::array<float, 2> thermal_range(const sp_sim_params& p, const std::vector<int>& rows) {
-    float hi = INFINITY;
-    for (int r : rows) hi = std::min(hi, p.mat[r].boil);
-    return {-273.15f, hi};
-}
 
 // FluidSim::viscosity_mode (mod.rs:2768-2792). A thermal tank counts a liquid that can
 // freeze or has a temperature law as arbitrarily viscous.
@@ -285,15 +339,15 @@ int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool th
     return 2;
 }
 
-// The rows a tank holding `present` needs: those, their diluents, and the products of
-// reactions whose partner is present (SimParams::closure, params.rs:927-941, without
-// residues, which this build does not carry).
+// The rows a tank holding `present` needs: those, their diluents, what they leave behind
+// when they boil, and the products of reactions whose partner is present
+// (SimParams::closure, params.rs:927-941).
 std::vector<int> closure(const sp_sim_params& p, const std::vector<int>& present) {
     std::vector<int> rows = present;
     for (size_t k = 0; k < rows.size(); k++) {
         const sp_material& m = p.mat[rows[k]];
         const bool partner = m.react_with >= 0 && std::find(present.begin(), present.end(), m.react_with) != present.end();
-        for (int extra : {m.diluent, partner ? m.react_into : -1})
+        for (int extra : {m.diluent, m.residue, partner ? m.react_into : -1})
             if (extra >= 0 && extra < SP_MAX_MATERIALS && std::find(rows.begin(), rows.end(), extra) == rows.end())
                 rows.push_back(extra);
     }
@@ -355,6 +409,22 @@ bool tank_layout(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, 
     return true;
 }
 
+// Tank::layout_steam (mod.rs:247-264): with steam and the water it condenses into added to
+// the rows while the tank is thermal, boiling makes steam and an aqueous liquid in it can
+// boil; without them if they do not fit (boiled liquid then leaves at once).
+bool tank_layout_steam(const sp_sim_params& p, const std::vector<int>& rows, Tank& t, bool thermal) {
+    bool boils = false;
+    for (int r : rows) boils |= r != SP_ROW_STEAM && p.mat[r].group == 0 && p.mat[r].boil < 1.0e8f;
+    if (thermal && p.steam != 0 && boils && std::find(rows.begin(), rows.end(), SP_ROW_STEAM) == rows.end()) {
+        std::vector<int> more = rows;
+        for (int r : {SP_ROW_WATER, SP_ROW_STEAM})
+            if (std::find(more.begin(), more.end(), r) == more.end()) more.push_back(r);
+        std::sort(more.begin(), more.end());
+        if (tank_layout(p, more, t, thermal)) return true;
+    }
+    return tank_layout(p, rows, t, thermal);
+}
+
 // FluidSim::caps (mod.rs:2717-2766) for the CAP-gated passes this build carries: families,
 // heat, rigid bodies, rheology, viscoelasticity (stress channels in the layout) and settling. A non-Newtonian liquid, or one with a skin or internal
 // friction, turns rheology on, and rigid bodies for a yield stress, shear thickening, a skin
@@ -387,8 +457,16 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
     bool magnetic = false;
     for (int r : t.rows) magnetic |= p.mat[r].susceptibility > 0.0f;
     if (n_magnets > 0 && magnetic) caps |= SP_CAP_MAGNET;
+    // steam carried in a thermal tank (mod.rs:2752-2756); and, for this build's kernel choice,
+    // a liquid that can boil there, and puffs of vapour that escaped steam or a fuming liquid
+    // leaves in the air
+    const bool steam = t.thermal && std::find(t.rows.begin(), t.rows.end(), SP_ROW_STEAM) != t.rows.end();
+    if (steam) caps |= SP_CAP_STEAM | SP_CAP_PUFFS;
     for (int r : t.rows) {
         const sp_material& m = p.mat[r];
+        const bool boils = t.thermal && m.boil < 1.0e8f;
+        if (boils) caps |= SP_CAP_BOILS;
+        if (m.fume > 0.0f && (m.volatility > 0.0f || boils)) caps |= SP_CAP_PUFFS;
         if (m.volatility > 0.0f) caps |= SP_CAP_VOLATILE;
         if (t.thermal && m.self_heat > 0.0f) caps |= SP_CAP_SELF_HEAT;
         // two liquids that react with each other (mod.rs:2756-2758)
@@ -438,7 +516,9 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.fam = chan[2];
         o.rheo = (int32_t)m.rheology;
         o.temp_model = m.temp_model;
-        o.residue = -1;
+        o.residue = m.residue;
+        o.residue_share = m.residue_share;  // mod.rs:2215 el.w
+        o.fume = m.fume;                    // mod.rs:2232 gr.y
         o.rho = rho / WATER_DENSITY;
         o.ln_nu = ln_nu;
         o.sigma = tension_cells(p, m.tension);
@@ -452,7 +532,7 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.load = m.load;
         o.freeze_shift = m.freeze_shift;
         // thermal rows (mod.rs:2213-2214)
-        o.th[0] = heat_vol(m) / WATER_HEAT;
+        o.th[0] = heat_vol(p, r) / WATER_HEAT;
         o.th[1] = m.conductivity / (WATER_HEAT * dx * dx);
         o.th[2] = std::max(m.melt_lo, NO_MELT);
         o.th[3] = std::max(std::max(m.melt_hi, m.melt_lo + 0.25f), NO_MELT);
@@ -488,13 +568,6 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
         o.ic[3] = 0;
     }
     const uint32_t caps = tank_caps(p, t, magnets.size());
-    // a liquid that evaporates as visible vapour leaves puffs (particles.comp:788-794), which
-    // this build does not carry
-    for (int r : t.rows)
-        if (p.mat[r].volatility > 0.0f && p.mat[r].fume > 0.0f) {
-            if (why) *why = "a fuming liquid evaporates into puffs of vapour, which are not carried";
-            return SP_E_UNSUPPORTED;
-        }
     // gpu_params (mod.rs:2238-2330).
     g.W = W;
     g.H = H;
@@ -564,6 +637,15 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     }
     g.ext[3] = (float)nc;
     if (nc > 0) g.caps |= SP_CAP_CURRENTS;
+    // steam (mod.rs:2322-2323)
+    g.steam[0] = std::min(std::max(p.steam_condense, 0.0f), 1.0f);
+    g.steam[1] = p.steam_rise / dx;
+    g.steam[2] = p.steam_life;
+    g.steam[3] = std::min(std::max(p.steam_boil, g.steam[0]), 1.0f);
+    g.steam2[0] = p.steam_mixing;
+    g.steam2[1] = (float)p.steam;
+    g.steam2[2] = p.steam_swirl / dx;
+    g.steam2[3] = p.steam_eddy / dx;
     g.corr[0] = p.correction_step_max;
     g.corr[1] = p.correction_deadband;
     g.corr[2] = (float)p.slip_cap;

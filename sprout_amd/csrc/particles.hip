@@ -152,6 +152,152 @@ SP_DEV float4 stress_update(const Ctx& c, float px, float py, float C0, float C1
     return t;
 }
 
+// ---- the air above the liquid (air.glsl:91-175) ----------------------------------------
+// Air is not simulated: escaped vapour rides a procedural flow, a buoyant rise (steam.y) plus
+// the curl of a stream function of drifting value noise (steam2.z: its speed, steam2.w: its
+// size), ramped to zero towards solids so that it flows along them.
+
+constexpr float PUFF_FOLLOW = 0.08f;  // s: how quickly a puff takes up the air's velocity
+
+// Value noise in [-0.5, 0.5) at x and its gradient (air.glsl:108-128): a hash per lattice
+// corner, quintic fade.
+SP_DEV float4 vnoise3(float x, float y, float z) {
+    const float ix = floorf(x), iy = floorf(y), iz = floorf(z);
+    const float fx = x - ix, fy = y - iy, fz = z - iz;
+    const float ux = fx * fx * fx * (fx * (fx * 6.0f - 15.0f) + 10.0f);
+    const float uy = fy * fy * fy * (fy * (fy * 6.0f - 15.0f) + 10.0f);
+    const float uz = fz * fz * fz * (fz * (fz * 6.0f - 15.0f) + 10.0f);
+    const float dux = 30.0f * fx * fx * (fx * (fx - 2.0f) + 1.0f);
+    const float duy = 30.0f * fy * fy * (fy * (fy - 2.0f) + 1.0f);
+    const float duz = 30.0f * fz * fz * (fz * (fz - 2.0f) + 1.0f);
+    const uint32_t cx = (uint32_t)((int)ix + 4096), cy = (uint32_t)((int)iy + 4096), cz = (uint32_t)((int)iz + 4096);
+    float r[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t qx = cx + (uint32_t)(k & 1), qy = cy + (uint32_t)((k >> 1) & 1), qz = cz + (uint32_t)((k >> 2) & 1);
+        r[k] = (float)(hash_u(qx * 73856093u ^ qy * 19349663u ^ qz * 83492791u) >> 8) / 16777216.0f;
+    }
+    const float k0 = r[0], k1 = r[1] - r[0], k2 = r[2] - r[0], k3 = r[4] - r[0];
+    const float k4 = r[0] - r[1] - r[2] + r[3], k5 = r[0] - r[2] - r[4] + r[6], k6 = r[0] - r[1] - r[4] + r[5];
+    const float k7 = -r[0] + r[1] + r[2] - r[3] + r[4] - r[5] - r[6] + r[7];
+    const float n = k0 + k1 * ux + k2 * uy + k3 * uz + k4 * ux * uy + k5 * uy * uz + k6 * uz * ux + k7 * ux * uy * uz;
+    return make_float4(n - 0.5f, dux * (k1 + k4 * uy + k6 * uz + k7 * uy * uz), duy * (k2 + k4 * ux + k5 * uz + k7 * ux * uz),
+                       duz * (k3 + k5 * uy + k6 * ux + k7 * ux * uy));
+}
+
+// The eddies' stream function (cells^2 / s) and its gradient at (x, y), time t: two octaves
+// that evolve and drift up with the plume (air.glsl:130-141).
+SP_DEV float3 air_psi(const sp_gparams& P, float x, float y, float t) {
+    const float L = fmaxf_(P.steam2[3], 1.0f);
+    const float turn = 0.7f * t * P.steam2[2] / L;
+    const float ax = x / L, ay = (y - 0.5f * P.steam[1] * t) / L;
+    const float4 n1 = vnoise3(ax, ay, turn);
+    const float4 n2 = vnoise3(ax * 2.17f + 17.3f, ay * 2.17f + 17.3f, 2.2f * turn);
+    const float amp = 2.2f * P.steam2[2] * L;
+    return make_float3(amp * (n1.x + 0.45f * n2.x), amp * ((n1.y + 0.45f * 2.17f * n2.y) / L),
+                       amp * ((n1.z + 0.45f * 2.17f * n2.z) / L));
+}
+
+// Each puff leans a little to one side as it rises, -0.15 .. 0.15 (air.glsl:143-146).
+SP_DEV float puff_lean(uint32_t pid) { return ((float)(hash_u(pid ^ 0x5bd1e995u) >> 8) / 16777216.0f - 0.5f) * 0.3f; }
+
+// Liquid occupancy at (x, y), bilinear over the cell centres (air.glsl:81-88).
+SP_DEV float sample_occ(const Ctx& c, float x, float y) {
+    const float gx = clampf_(x - 0.5f, 0.0f, (float)c.W - 1.001f), gy = clampf_(y - 0.5f, 0.0f, (float)c.H - 1.001f);
+    const int x0 = (int)gx, y0 = (int)gy;
+    const float fx = gx - (float)x0, fy = gy - (float)y0;
+    const int k = y0 * c.W + x0;
+    return mixf_(mixf_(c.cell[k].x, c.cell[k + 1].x, fx), mixf_(c.cell[k + c.W].x, c.cell[k + c.W + 1].x, fx), fy);
+}
+
+// Bilinear velocity over live faces, `fb` where no face is live (air.glsl:41-78).
+SP_DEV float2 sample_vel(const Ctx& c, float px, float py, float2 fb);
+
+// The air's velocity at (x, y), cells / s, and the liquid occupancy there (air.glsl:148-175);
+// the mouse tools' stirring is not carried (only the idle tool is).
+SP_DEV float2 air_velocity(const Ctx& c, float x, float y, float lean, float& occ) {
+    const sp_gparams& P = *c.P;
+    const float t = c.S->advanced;
+    float nx, ny;
+    const float d = sample_sdf(c, x, y, nx, ny);
+    const float nl = len2_(nx, ny);
+    if (nl > 1e-5f) {
+        nx = nx / nl;
+        ny = ny / nl;
+    } else {
+        nx = 0.0f;
+        ny = 1.0f;
+    }
+    const float Lb = fmaxf_(P.steam2[3], 1.0f);
+    const float r = clampf_(d / Lb, 0.0f, 1.0f);
+    const float ramp = r * (1.875f - r * r * (1.25f - 0.375f * r * r));
+    const float dramp = d < Lb && d > 0.0f ? (1.875f - r * r * (3.75f - 1.875f * r * r)) / Lb : 0.0f;
+    float vx = 0.0f, vy = 0.0f;
+    if (P.steam2[2] > 0.0f) {
+        const float3 psi = air_psi(P, x, y, t);
+        vx = ramp * psi.z + psi.x * dramp * ny;
+        vy = ramp * -psi.y + psi.x * dramp * -nx;
+    }
+    float rx = lean * P.steam[1], ry = 1.0f * P.steam[1];
+    const float into = rx * nx + ry * ny;
+    if (into < 0.0f) {
+        const float tx = ny, ty = -nx;
+        const float side = (rx * tx + ry * ty) + lean * 1e-3f >= 0.0f ? 1.0f : -1.0f;
+        rx -= (nx + side * tx * 0.7f) * into * (1.0f - ramp);
+        ry -= (ny + side * ty * 0.7f) * into * (1.0f - ramp);
+    }
+    vx += rx;
+    vy += ry;
+    // near liquid the air moves with it, and the steam still rises through it
+    occ = sample_occ(c, x, y);
+    const float2 vl = sample_vel(c, x, y, make_float2(vx, vy - P.steam[1]));
+    const float k = smoothstep_(0.0f, 0.4f, occ);
+    return make_float2(mixf_(vx, vl.x, k), mixf_(vy, vl.y + P.steam[1], k));
+}
+
+// A puff of vapour that left the tank (particles.comp:103-134): a tracer in the air above the
+// liquid, gone after steam.z seconds, at the edge of the grid, or swallowed by the liquid. The
+// vapour it lays into the renderer's smoke field is not carried (no smoke field here).
+SP_DEV void advance_puff(const Ctx& c, uint32_t id, sp_particle p) {
+    const sp_gparams& P = *c.P;
+    const float dt = c.S->dt;
+    p.s += dt;
+    if (p.s >= P.steam[2] || p.pos[0] < 2.01f || p.pos[1] < 2.01f || p.pos[0] > (float)c.W - 2.01f ||
+        p.pos[1] > (float)c.H - 2.01f) {
+        c.part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    float occ;
+    const float2 target = air_velocity(c, p.pos[0], p.pos[1], puff_lean(p.id), occ);
+    if (occ > 0.6f && p.s > 0.1f) {
+        c.part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    const float k = 1.0f - expf(-dt / PUFF_FOLLOW);
+    float vx = mixf_(p.vel[0], target.x, k), vy = mixf_(p.vel[1], target.y, k);
+    float x = p.pos[0] + vx * dt, y = p.pos[1] + vy * dt;
+    collide(c, x, y, vx, vy, false);
+    p.pos[0] = x;
+    p.pos[1] = y;
+    p.vel[0] = vx;
+    p.vel[1] = vy;
+    c.part[id] = p;
+}
+
+// A particle that leaves the tank as a puff of vapour `fume` from where it is, with velocity
+// (vx, vy) (particles.comp:704-712, 796-800).
+SP_DEV void become_puff(const Ctx& c, uint32_t id, sp_particle p, float vx, float vy, float fume) {
+    p.kind = SP_KIND_PUFF;
+    p.s = 0.0f;
+    p.vel[0] = vx;
+    p.vel[1] = vy;
+    p.c[0] = fume;
+    p.c[1] = 0.0f;
+    p.c[2] = 0.0f;
+    p.c[3] = 0.0f;
+    c.part[id] = p;
+}
+
 // APIC gather of one face component over its 3 x 3 live faces (particles.comp:165-222).
 SP_DEV void gather_face(const Ctx& c, float px, float py, int comp, float& u, float& cr, float& Ca, float& Cb) {
     float gx = comp == 0 ? px : px - 0.5f, gy = comp == 0 ? py - 0.5f : py;
@@ -200,7 +346,11 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
     if (id >= P.n_slots) return;
     sp_particle p = c.part[id];
     if (p.kind == SP_KIND_DEAD) return;
-    if ((p.kind & 255u) == 0u) return;
+    if ((p.kind & 255u) == 0u) {
+        // a puff (only tanks that may hold them run the kernel with their code)
+        if (X && (P.caps & SP_CAP_PUFFS)) advance_puff(c, id, p);
+        return;
+    }
     float dt = c.S->dt;
     int row = (int)(p.kind & 255u) - 1;
     int Mgroup = c.mats[row].group, Mcw = c.mats[row].chan_w, Mcs = c.mats[row].chan_s;
@@ -231,7 +381,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
         bool held = false;
         if (rig) {
             const int px1 = clampi_(floori(p.pos[0]), 1, c.W - 2), py1 = clampi_(floori(p.pos[1]), 1, c.H - 2);
-            const bool in_body = c.rl[cidx(c, px1, py1)].x >= 0;
+            const bool in_body = row != SP_ROW_STEAM && c.rl[cidx(c, px1, py1)].x >= 0;
             // a yield-stress liquid's particle in a cell too sparse to be liquid (the surface
             // of a resting pile), next to a plug
             const int px0 = clampi_(floori(p.pos[0]), 0, c.W - 1), py0 = clampi_(floori(p.pos[1]), 0, c.H - 1);
@@ -286,6 +436,10 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
             bool groups = (P.group_mask & (P.group_mask - 1u)) != 0u;
             bool sorts = group_sorts(P, c.mats, Mgroup);
             float sbg = 0.0f;
+            // the rise of steam through the aqueous liquid around it: the share of steam
+            // (particles.comp:303-305)
+            const bool steamy = X && (P.caps & SP_CAP_STEAM) && Mgroup == 0;
+            float sfv = 0.0f;
             int nmix = P.n_fam - 1;
 #pragma unroll
             for (int b = 0; b < 3; b++)
@@ -304,6 +458,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                             sgz += wt * ax.w;
                         }
                         if (sorts) sbg += wt * f4c(c.gbase[ci], Mgroup - 1);
+                        if (steamy) sfv += wt * c.gbase[ci].w;
                         float sb = fam > 0 ? c.mixb[(size_t)ci * nmix + fam - 1] : q.z;
                         if (sb >= 0.0f) {
                             sfw += wt;
@@ -443,6 +598,19 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                         slip -= fminf_(vt, sep_max) * signf_(drho);
                     }
                 }
+                // Steam rises through the water around it (particles.comp:505-517): a bubble of
+                // the unresolved size at its terminal velocity in water, by the share of water
+                // around it; the water drifts down by the share of steam.
+                if (steamy && !held) {
+                    const float fv = clampf_(sfv / sw, 0.0f, 1.0f);
+                    if (fv > 1e-4f) {
+                        const float rw = c.mats[SP_ROW_WATER].rho;
+                        float vt = terminal(c, c.mats[SP_ROW_STEAM].rho - rw, rw, expf(c.mats[SP_ROW_WATER].ln_nu), P.sep_tau);
+                        const float sep_max = P.corr[2] > 0.0f ? P.cfl / dt : 1e30f;
+                        vt = fminf_(vt, sep_max);
+                        slip += row == SP_ROW_STEAM ? vt * (1.0f - fv) : -vt * fv;
+                    }
+                }
                 bool diluent0 = fam < 0 && P.primary >= 0 && P.fams[0][1] == row;
                 if (fam >= 0 || diluent0) {
                     if (sfw > 1e-4f) {
@@ -514,6 +682,90 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                 }
             }
         }
+        // Boiling (particles.comp:633-731). Liquid past its boiling point takes up latent heat
+        // as it goes on heating; the share it holds is its vapour quality x. An aqueous liquid
+        // in a tank that makes steam turns into steam at x = steam.w, keeping its mass and heat
+        // and the row it boiled from (KIND_SRC); steam condenses back below x = steam.x, and at
+        // the free surface escapes as a puff with probability x, else gives up its latent heat
+        // there and stays, as its liquid at the boiling point. Any other liquid leaves at x = 1.
+        // What leaves leaves its residue behind, one particle per residue share on average.
+        if (X && th_on && (P.caps & SP_CAP_BOILS)) {
+            const float4 th = thermal_of(c, row, p.s);
+            const int dil = c.mats[row].diluent;
+            float bo_t = c.mats[row].th2[2], bo_l = c.mats[row].th2[3];
+            if (dil >= 0) {
+                bo_t = mixf_(c.mats[dil].th2[2], bo_t, p.s);
+                bo_l = mixf_(c.mats[dil].th2[3], bo_l, p.s);
+            }
+            const bool vapour = (P.caps & SP_CAP_STEAM) && Mgroup == 0 && (P.steam2[1] > 0.5f || row == SP_ROW_STEAM);
+            bool leave = false;
+            if (vapour && row == SP_ROW_STEAM) {
+                const float x = (p.t - bo_t) / fmaxf_(bo_l, 1e-3f);
+                const int pcx = clampi_((int)p.pos[0], 0, c.W - 1), pcy = clampi_((int)p.pos[1], 0, c.H - 1);
+                const bool surface = c.cell[cidx(c, pcx, pcy)].w <= 0.0f || c.cell[cidx(c, pcx, clampi_(pcy + 1, 0, c.H - 1))].w <= 0.0f;
+                const float u = (float)(hash_u(p.id ^ hash_u(c.S->substeps ^ 0x2545f491u)) >> 8) / 16777216.0f;
+                if (surface && u < x) {
+                    leave = true;
+                } else if (x < P.steam[0] || surface) {
+                    // back into the liquid it boiled from: at the surface, its latent heat
+                    // given up to the air; below it, condensed
+                    if (surface) {
+                        ledger_add(c, 6, fmaxf_(th.w * (p.t - bo_t), 0.0f) * P.inv_ppc);
+                        p.t = bo_t;
+                    }
+                    const uint32_t src = (p.kind & SP_KIND_SRC) >> 8;
+                    row = src > 0u ? (int)src - 1 : SP_ROW_WATER;
+                    Mgroup = c.mats[row].group;
+                    Mcw = c.mats[row].chan_w;
+                    Mcs = c.mats[row].chan_s;
+                    fam = c.mats[row].fam;
+                    p.kind = (p.kind & ~(255u | SP_KIND_SRC)) | (uint32_t)(row + 1);
+                    p.t = keep_heat(p.t, th.w, thermal_of(c, row, p.s).w);
+                }
+            } else if (p.t >= bo_t + bo_l * (vapour ? P.steam[3] : 1.0f)) {
+                if (vapour) {
+                    const float c1 = thermal_of(c, SP_ROW_STEAM, 1.0f).w;
+                    atomicAdd(&c.ledger[16 + 2 * SP_MAX_MATERIALS + row], 1u);
+                    p.kind = (p.kind & ~(255u | SP_KIND_SRC)) | ((uint32_t)(row + 1) << 8) | (uint32_t)(SP_ROW_STEAM + 1);
+                    row = SP_ROW_STEAM;
+                    Mgroup = c.mats[row].group;
+                    Mcw = c.mats[row].chan_w;
+                    Mcs = c.mats[row].chan_s;
+                    fam = -1;
+                    p.t = keep_heat(p.t, th.w, c1);
+                } else {
+                    leave = true;
+                }
+            }
+            if (leave) {
+                const uint32_t src = row == SP_ROW_STEAM ? (p.kind & SP_KIND_SRC) >> 8 : (uint32_t)(row + 1);
+                const int from = src > 0u ? (int)src - 1 : SP_ROW_WATER;
+                const int res = c.mats[from].residue;
+                const float share = c.mats[from].residue_share * (c.mats[from].diluent >= 0 ? p.s : 1.0f);
+                // heat content counted from absolute zero, as the ledger's other entries
+                float e = th.w * (p.t + 273.15f);
+                atomicAdd(&c.ledger[16 + row], 1u);
+                if (res >= 0 && (float)(hash_u(p.id ^ 0x9e3779b9u) >> 8) / 16777216.0f < share) {
+                    atomicAdd(&c.ledger[16 + SP_MAX_MATERIALS + res], 1u);
+                    row = res;
+                    Mgroup = c.mats[row].group;
+                    Mcw = c.mats[row].chan_w;
+                    Mcs = c.mats[row].chan_s;
+                    fam = c.mats[row].fam;
+                    p.kind = (uint32_t)(row + 1);
+                    p.s = 1.0f;
+                    p.t = bo_t - thermal_of(c, row, 1.0f).z;  // the residue is a solid
+                    e -= thermal_of(c, row, 1.0f).w * (p.t + 273.15f);
+                    ledger_add(c, 6, fmaxf_(e, 0.0f) * P.inv_ppc);
+                } else {
+                    ledger_add(c, 6, fmaxf_(e, 0.0f) * P.inv_ppc);
+                    const float fume = row == SP_ROW_STEAM ? 1.0f : c.mats[row].fume;
+                    if (fume > 0.0f) become_puff(c, id, p, vx, vy, fume);
+                    else c.part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+            }
+        }
         // Heat made inside (particles.comp:733-739): the liquid's own K/s, counted in the ledger.
         if (X && th_on && (P.caps & SP_CAP_SELF_HEAT) && c.mats[row].rx[2] > 0.0f) {
             const float e = c.mats[row].rx[2] * c.mats[row].th[0] * (c.mats[row].diluent >= 0 ? p.s : 1.0f) * dt * P.heat_rate;
@@ -574,7 +826,10 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                 if (u < rate * dt) {
                     if (th_on) ledger_add(c, 6, fmaxf_(th.w * (p.t + 273.15f), 0.0f) * P.inv_ppc);
                     atomicAdd(&c.ledger[16 + row], 1u);
-                    c.part[id].kind = SP_KIND_DEAD;
+                    // a liquid that fumes leaves a puff of mist
+                    const float fume = c.mats[row].fume;
+                    if (fume > 0.0f) become_puff(c, id, p, vx, vy, fume);
+                    else c.part[id].kind = SP_KIND_DEAD;
                     return;
                 }
             }

@@ -128,8 +128,6 @@ struct sp_sim {
     // heat (FluidSim::thermal, solids_heated, mod.rs:959-962)
     bool thermal = false;        // temperatures carried and conducted since the last reset
     bool solids_heated = false;  // the solids hold temperatures
-    float t_lo = INFINITY, t_hi = -INFINITY;    // the temperatures the particles entering since the last reset held
-    float st_lo = INFINITY, st_hi = -INFINITY;  // the temperatures the solids hold (set_solids)
 };
 
 namespace {
@@ -178,15 +176,15 @@ void launch_claim(sp_sim* s, uint32_t claim, hipStream_t stream) {
 
 // Uploads the grid-unit tables if the parameters or the tank changed.
 // A row of this library's material table in the reference's GpuMat order
-// (gpu_materials, mod.rs:2202-2233). Residue share and fizz are not carried (0).
-sp_gpumat to_gpumat(const sp_gmat& g, const sp_material& m) {
+// (gpu_materials, mod.rs:2202-2233). Fizz, which only the renderer reads, is not carried (0).
+sp_gpumat to_gpumat(const sp_gmat& g) {
     sp_gpumat o;
     memset(&o, 0, sizeof(o));
     const int32_t ia[4] = {g.group, g.diluent, g.chan_w, g.chan_s}, ib[4] = {g.fam, g.rheo, g.temp_model, g.residue};
     const float a[4] = {g.rho, g.ln_nu, g.sigma, g.mix_rate}, b[4] = {g.cos_c, g.sin_c, g.grain, g.rho_solid};
-    const float el[4] = {g.gr[1], g.gr[2], g.gr[3], 0.0f};
+    const float el[4] = {g.gr[1], g.gr[2], g.gr[3], g.residue_share};
     const float misc[4] = {g.freeze_shift, g.load, g.visc_exp, g.expansion};
-    const float gr[4] = {g.gr[0], m.fume, 0.0f, 0.0f};
+    const float gr[4] = {g.gr[0], g.fume, 0.0f, 0.0f};
     for (int k = 0; k < 4; k++) {
         o.ia[k] = ia[k];
         o.ib[k] = ib[k];
@@ -207,20 +205,6 @@ sp_gpumat to_gpumat(const sp_gmat& g, const sp_material& m) {
 
 int upload_tables(sp_sim* s) {
     if (!s->tables_dirty) return SP_OK;
-    if (s->thermal) {
-        // A thermal tank is carried below its liquids' lowest boiling point: the steam and
-        // boiling passes are not carried. Conduction, air and contact exchange never leave
-        // the range of the temperatures put in.
-        const std::array<float, 2> r = sp::thermal_range(s->params, s->tank.rows);
-        const float lo = std::min(std::min(s->t_lo, s->st_lo), s->params.ambient_temp);
-        const float hi = std::max(std::max(s->t_hi, s->st_hi), s->params.ambient_temp);
-        if (!(lo > r[0] && hi < r[1])) {
-            set_err(s, "thermal tank holds temperatures %.2f .. %.2f C outside %.2f .. %.2f C: its liquids would boil "
-                       "(steam and boiling are not carried)",
-                    lo, hi, r[0], r[1]);
-            return SP_E_UNSUPPORTED;
-        }
-    }
     std::string why;
     int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets, s->spawn_speed,
                         s->currents);
@@ -230,7 +214,7 @@ int upload_tables(sp_sim* s) {
     }
     HIPCHK(hipMemcpyAsync((void*)s->c.P, &s->gp, sizeof(sp_gparams), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemcpyAsync((void*)s->c.mats, s->gm, sizeof(s->gm), hipMemcpyHostToDevice, s->stream));
-    for (int r = 0; r < SP_MAX_MATERIALS; r++) s->gpumat[r] = to_gpumat(s->gm[r], s->params.mat[r]);
+    for (int r = 0; r < SP_MAX_MATERIALS; r++) s->gpumat[r] = to_gpumat(s->gm[r]);
     if (s->d_gpumat)
         HIPCHK(hipMemcpyAsync(s->d_gpumat, s->gpumat, sizeof(s->gpumat), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -602,28 +586,19 @@ bool note_materials(sp_sim* s, const sp_particle* p, uint32_t n, bool reset) {
         uint32_t low = p[k].kind & 255u;
         if (p[k].kind == SP_KIND_DEAD || low == 0u) continue;
         int r = (int)low - 1;
-        thermal |= fabsf(p[k].t - s->params.ambient_temp) > 0.05f;
+        // a particle off the ambient temperature, or of a liquid that makes heat (material.rs:455-457)
+        const sp_material& m = s->params.mat[r];
+        thermal |= fabsf(p[k].t - s->params.ambient_temp) > 0.05f || m.self_heat > 0.0f || (m.react_with >= 0 && m.react_heat != 0.0f);
         if (std::find(present.begin(), present.end(), r) == present.end()) present.push_back(r);
     }
     std::vector<int> rows = sp::closure(s->params, present);
     if (rows == s->tank.rows && thermal == s->thermal && !(reset && s->tank.rows.empty())) return true;
     sp::Tank t;
-    if (!sp::tank_layout(s->params, rows, t, thermal)) return false;
+    if (!sp::tank_layout_steam(s->params, rows, t, thermal)) return false;
     adopt_tank(s, t);
     s->thermal = thermal;
     s->tables_dirty = true;
     return true;
-}
-
-// The temperatures entering the tank (particles are enthalpy temperatures; above the
-// melting range, which a thermal tank is held to, they are the temperatures themselves).
-void note_temps(sp_sim* s, const sp_particle* p, uint32_t n) {
-    for (uint32_t k = 0; k < n; k++) {
-        if (p[k].kind == SP_KIND_DEAD || (p[k].kind & 255u) == 0u) continue;
-        if (p[k].t < s->t_lo || p[k].t > s->t_hi) s->tables_dirty = true;  // checked again at the next record
-        s->t_lo = std::min(s->t_lo, p[k].t);
-        s->t_hi = std::max(s->t_hi, p[k].t);
-    }
 }
 
 }  // namespace
@@ -655,7 +630,7 @@ int sp_derive_grid_params(const sp_sim_params* params, int32_t grid_w, int32_t g
     std::vector<int> present(rows, rows + n_rows);
     std::vector<int> r = sp::closure(*params, present);
     sp::Tank t;
-    if (!sp::tank_layout(*params, r, t, thermal != 0)) return SP_E_CAPACITY;
+    if (!sp::tank_layout_steam(*params, r, t, thermal != 0)) return SP_E_CAPACITY;
     std::vector<sp_magnet> mg(magnets, magnets + std::min<uint32_t>(n_magnets, SP_MAX_MAGNETS));
     std::vector<sp_current> cu(currents, currents + std::min<uint32_t>(n_currents, SP_MAX_CURRENTS));
     return sp::derive(*params, grid_w, grid_h, t, 0, *out, mats, nullptr, mg, 0.0f, cu);
@@ -795,7 +770,7 @@ int sp_set_params(sp_sim* s, const sp_sim_params* p) {
         // the tank's layout follows the table (diluents may have changed)
         std::vector<int> rows = sp::closure(s->params, s->tank.rows);
         sp::Tank t;
-        if (sp::tank_layout(s->params, rows, t, s->thermal)) adopt_tank(s, t);
+        if (sp::tank_layout_steam(s->params, rows, t, s->thermal)) adopt_tank(s, t);
         return SP_OK;
     });
 }
@@ -812,18 +787,10 @@ int sp_set_solids(sp_sim* s, const float* sdf, const float* solid, const uint32_
         HIPCHK(hipMemcpyAsync((void*)s->c.solid_t, st.data(), sizeof(float) * (size_t)s->N, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         s->solids_heated = temp != nullptr;
-        s->st_lo = INFINITY;
-        s->st_hi = -INFINITY;
-        for (float t : st)
-            if (t > 0.5f * SP_NO_TEMP) {
-                s->st_lo = std::min(s->st_lo, t);
-                s->st_hi = std::max(s->st_hi, t);
-            }
-        s->tables_dirty = true;  // the range guard sees the new solid temperatures at the next record
         if (temp) {
             if (!s->thermal) {  // enable_heat (mod.rs:2163-2177)
                 sp::Tank t;
-                if (!sp::tank_layout(s->params, s->tank.rows, t, true)) {
+                if (!sp::tank_layout_steam(s->params, s->tank.rows, t, true)) {
                     set_err(s, "the accumulators have no room for the temperature sums");
                     return SP_E_CAPACITY;
                 }
@@ -853,9 +820,6 @@ int sp_reset(sp_sim* s, const sp_particle* particles, uint32_t n) {
         // a reset starts from an empty tank, thermal only if the solids are heated (mod.rs:1991-1992)
         s->thermal = s->solids_heated;
         s->tank = sp::Tank();
-        s->t_lo = INFINITY;
-        s->t_hi = -INFINITY;
-        note_temps(s, ps.data(), (uint32_t)ps.size());
         if (!note_materials(s, ps.data(), (uint32_t)ps.size(), true)) {
             set_err(s, "the particles hold more materials than the accumulators have channels for");
             return SP_E_CAPACITY;
@@ -909,7 +873,6 @@ int sp_spawn(sp_sim* s, const sp_particle* particles, uint32_t n, uint32_t* acce
             sp::settle_spawn(s->params, ps[k]);
         }
         if (!note_materials(s, ps.data(), n, false)) return SP_OK;
-        note_temps(s, ps.data(), n);
         // the pour's speed bounds the next record's time step (spawn, mod.rs:2039-2041)
         for (const auto& q : ps) {
             const float v = hypotf(q.vel[0], q.vel[1]);
@@ -1097,7 +1060,7 @@ int sp_material_table(sp_sim* s, sp_gpumat* out, uint32_t cap) {
                 set_err(s, "%s", why.c_str());
                 return rc;
             }
-            for (uint32_t r = 0; r < cap && r < (uint32_t)SP_MAX_MATERIALS; r++) out[r] = to_gpumat(gm[r], s->params.mat[r]);
+            for (uint32_t r = 0; r < cap && r < (uint32_t)SP_MAX_MATERIALS; r++) out[r] = to_gpumat(gm[r]);
             return SP_OK;
         }
         for (uint32_t r = 0; r < cap && r < (uint32_t)SP_MAX_MATERIALS; r++) out[r] = s->gpumat[r];
@@ -1468,13 +1431,14 @@ int sp_resets(sp_sim* s, uint64_t* out) {
     }, false);
 }
 
-// FluidSim::shares_a_group (mod.rs:1872-1880); no steam row is carried here.
+// FluidSim::shares_a_group (mod.rs:1872-1880), steam aside.
 int sp_shares_a_group(sp_sim* s, int32_t* out) {
     return guarded(s, [&]() -> int {
         if (!out) return SP_E_INVALID;
         int seen[8] = {0};
         *out = 0;
         for (int r : s->tank.rows) {
+            if (r == SP_ROW_STEAM) continue;
             const int g = std::min(std::max(s->params.mat[r].group, 0), 7);
             if (++seen[g] >= 2) *out = 1;
         }

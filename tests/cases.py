@@ -389,3 +389,30 @@ def reacting_mix(grid, rate=6.0, into=ROW_BRINE, partner_rate=0.0, block=(0.2, 0
     parts["kind"][lye] = ROW_LYE + 1
     rows = [ROW_ACID, ROW_LYE]
     return params, scene.rasterize(), settle(parts, params), rows
+
+
+def boiling_pot(grid, x_range=(0.0, 0.06), row=N.ROW_WATER, steam=1, hot=350.0, fume=None, boil=None, latent_vap=None,
+                depth=0.5, seed=0, cell_size=0.005, pressure_tol=1e-6):
+    """A pot of `row` on a floor held at `hot` deg C, its particles near their boiling point:
+    each at the vapour quality x (the share of its latent heat of vaporisation taken up)
+    drawn uniformly from x_range by seed, so that within a few substeps some turn into steam,
+    rise, condense, escape as puffs or boil away. `steam` 0: boiled liquid leaves at once.
+    `fume`: the liquid's fume (puffs where it boils away); `boil`, `latent_vap`: its boiling
+    point (deg C) and latent heat (J/kg) if given. Air cooling and radiation off.
+    (params, solid field, particles, rows)."""
+    W = grid[0]
+    params, field, parts, rows = thermal_pool(grid, frac=(1.0, depth), row=row, hot=(0.3 * W, 0.7 * W, hot),
+                                              cell_size=cell_size, air_cooling=0.0, evaporation=0, emissivity=0.0,
+                                              pressure_tol=pressure_tol)
+    params.steam = steam
+    m = params.mat[row]
+    if fume is not None:
+        m.fume = fume
+    if boil is not None:
+        m.boil = boil
+    if latent_vap is not None:
+        m.latent_vap = latent_vap
+    x = np.random.default_rng(seed).uniform(x_range[0], x_range[1], len(parts))
+    parts["t"] = (m.boil + x * m.latent_vap / m.heat_capacity).astype(np.float32)
+    return params, field, parts, rows
+

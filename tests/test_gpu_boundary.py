@@ -102,7 +102,8 @@ def test_material_table_in_the_reference_layout():
             assert np.array_equal(t["a"][r], np.float32([m.rho, m.ln_nu, m.sigma, m.mix_rate]))
             assert np.array_equal(t["b"][r], np.float32([m.cos_c, m.sin_c, m.grain, m.rho_solid]))
             assert np.array_equal(t["misc"][r], np.float32([m.freeze_shift, m.load, m.visc_exp, m.expansion]))
-            assert np.array_equal(t["el"][r], np.float32([m.gr[1], m.gr[2], m.gr[3], 0.0]))
+            assert np.array_equal(t["el"][r], np.float32([m.gr[1], m.gr[2], m.gr[3], m.residue_share]))
+            assert np.array_equal(t["gr"][r][:2], np.float32([m.gr[0], m.fume]))
             assert np.array_equal(t["th"][r], np.float32(list(m.th)))
             assert np.array_equal(t["rx"][r], np.float32(list(m.rx)))
         # gpu_materials (mod.rs:2190-2207): an isothermal tank's relative density is the density

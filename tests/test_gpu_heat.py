@@ -146,8 +146,9 @@ def test_whole_thermal_substeps_match(grid, case):
         p.close()
 
 
-def test_thermal_tank_outside_the_carried_range_is_refused():
-    """A floor held above the boiling point would boil the water: steam is not carried."""
+def test_floor_above_the_boiling_point_is_carried_with_steam():
+    """A floor held above the boiling point (once refused): the tank takes steam into its
+    rows and records like any other."""
     from sprout_amd.fluid_sim import FluidSim
     grid = (48, 32)
     params, field, parts, _ = cases.thermal_pool(grid, hot=(10.0, 30.0, 120.0))
@@ -155,8 +156,10 @@ def test_thermal_tank_outside_the_carried_range_is_refused():
     try:
         s.set_solids(field)
         s.reset(parts)
-        with pytest.raises(N.SproutError) as e:
-            s.record(1, 1 / 240)
-        assert e.value.code == N.E_UNSUPPORTED
+        s.record(4, 4 / 240)
+        s.wait()
+        g, _ = s.grid_params()
+        assert g.caps & N.CAP_STEAM and N.ROW_STEAM in list(g.present[:g.n_present])
+        assert s.solver.substeps == 4
     finally:
         s.close()

@@ -34,7 +34,7 @@ def test_bindings_cover_the_header():
 
 def test_struct_layouts():
     assert N.PARTICLE_DTYPE.itemsize == 48  # params.rs:1319
-    assert C.sizeof(N.GMat) == 192
+    assert C.sizeof(N.GMat) == 208
     assert N.STEP_DTYPE.itemsize == 48      # 12 words (mod.rs:123 STEP_WORDS)
 
 

@@ -40,10 +40,11 @@ def test_tables_caps_and_fume():
     assert g.caps & N.CAP_VOLATILE and m[ROW].rx[3] == 2.0
     g, _ = N.derive_grid_params(p, 64, 48, [N.ROW_WATER])
     assert not g.caps & N.CAP_VOLATILE
+    assert not g.caps & N.CAP_PUFFS
+    # a liquid that fumes leaves puffs of mist: the tank may hold puffs
     p.mat[ROW].fume = 0.5
-    with pytest.raises(N.SproutError) as e:
-        N.derive_grid_params(p, 64, 48, [ROW])
-    assert e.value.code == N.E_UNSUPPORTED
+    g, m = N.derive_grid_params(p, 64, 48, [ROW])
+    assert g.caps & N.CAP_PUFFS and m[ROW].fume == 0.5
 
 
 @pytest.mark.parametrize("vol", [3.0, 12.0])
