@@ -188,6 +188,9 @@ typedef struct sp_sim_params {
      * the air's eddies they ride. */
     uint32_t steam;
     float steam_boil, steam_condense, steam_mixing, steam_rise, steam_life, steam_swirl, steam_eddy;
+    /* The heat and cool tools' power (params.rs:210): kelvin per second for water at the
+     * brush's centre. */
+    float heat_power;
     int32_t mat_count;
     sp_material mat[SP_MAX_MATERIALS];
 } sp_sim_params;
@@ -351,6 +354,14 @@ typedef struct sp_gparams {
      * steam, the air's eddy speed (cells/s) and size (cells). */
     float steam[4];
     float steam2[4];
+    /* The mouse tool of the record (GpuParams tool_*, mod.rs:2300-2305): position (cells),
+     * velocity (cells/s), radius (cells, at least 0.01), mode (0 idle, 1 grab / stir, 2 erase,
+     * 3 push / pull, 4 stream nozzle, 5 heat, 6 cool, 7 whirl, 8 hold), strength (1/s); the
+     * heat and cool tools' power (P.heat2.y, K/s for water). */
+    float tool_pos[2], tool_vel[2];
+    float tool_radius;
+    int32_t tool_mode;
+    float tool_strength, heat_power;
 } sp_gparams;
 
 /* Step state (`StepBuf`, common.glsl:122-135): 12 words. */

@@ -388,6 +388,24 @@ static int collide(const or_sim* s, float* x, float* y, float* vx, float* vy, in
     return 0;
 }
 
+/* ---- the mouse tool (air.glsl:24-37) ----------------------------------------------------- */
+
+/* tool_target: the velocity the tool drives a particle towards, (dx, dy) to the cursor. */
+static v2 tool_target(const sp_gparams* P, float dx, float dy) {
+    float l = len2(dx, dy);
+    if (P->tool_mode == 3) {
+        float m = fmaxf_(l, 0.15f * P->tool_radius);
+        return (v2){-dx / m * P->tool_vel[0], -dy / m * P->tool_vel[0]};
+    }
+    if (P->tool_mode == 4) return (v2){P->tool_vel[0], P->tool_vel[1]};
+    if (P->tool_mode == 7) {
+        float m = fmaxf_(l, 0.4f * P->tool_radius);
+        return (v2){dy / m * P->tool_vel[0] + dx * 2.0f, -dx / m * P->tool_vel[0] + dy * 2.0f};
+    }
+    if (P->tool_mode == 8) return (v2){0.0f, 0.0f};
+    return (v2){P->tool_vel[0] + dx * 4.0f, P->tool_vel[1] + dy * 4.0f};
+}
+
 /* ---- the air above the liquid (air.glsl:91-175): a procedural flow escaped vapour rides --- */
 
 #define PUFF_FOLLOW 0.08f /* air.glsl:102 */
@@ -439,7 +457,7 @@ static float sample_occ(const or_sim* s, float x, float y) {
     return mixf(mixf(s->cell[k].x, s->cell[k + 1].x, fx), mixf(s->cell[k + s->W].x, s->cell[k + s->W + 1].x, fx), fy);
 }
 
-/* air_velocity (air.glsl:148-175), the idle tool. */
+/* air_velocity (air.glsl:148-175). */
 static v2 air_velocity(const or_sim* s, float x, float y, float lean, float* occ) {
     const sp_gparams* P = &s->P;
     float t = s->S.advanced;
@@ -476,7 +494,19 @@ static v2 air_velocity(const or_sim* s, float x, float y, float lean, float* occ
     *occ = sample_occ(s, x, y);
     v2 vl = sample_vel(s, x, y, (v2){vx, vy - P->steam[1]});
     float k = smoothstepf(0.0f, 0.4f, *occ);
-    return (v2){mixf(vx, vl.x, k), mixf(vy, vl.y + P->steam[1], k)};
+    vx = mixf(vx, vl.x, k);
+    vy = mixf(vy, vl.y + P->steam[1], k);
+    int tm = P->tool_mode;
+    if (tm == 1 || tm == 3 || tm == 4 || tm == 7) {
+        float tdx = P->tool_pos[0] - x, tdy = P->tool_pos[1] - y;
+        float r2 = (tdx * tdx + tdy * tdy) / (P->tool_radius * P->tool_radius);
+        if (r2 < 1.0f) {
+            v2 tt = tool_target(P, tdx, tdy);
+            vx = mixf(vx, tt.x, 1.0f - r2);
+            vy = mixf(vy, tt.y, 1.0f - r2);
+        }
+    }
+    return (v2){vx, vy};
 }
 
 /* A puff of vapour (particles.comp:103-134): a tracer in the air, gone after steam.z seconds,
@@ -493,6 +523,10 @@ static void advance_puff(or_sim* s, uint32_t id, sp_particle p) {
     float occ;
     v2 target = air_velocity(s, p.pos[0], p.pos[1], puff_lean(p.id), &occ);
     if (occ > 0.6f && p.s > 0.1f) {
+        s->part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    if (P->tool_mode == 2 && len2(P->tool_pos[0] - p.pos[0], P->tool_pos[1] - p.pos[1]) < P->tool_radius) {
         s->part[id].kind = SP_KIND_DEAD;
         return;
     }
@@ -1045,6 +1079,40 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
                 }
             }
         }
+        /* The mouse tool (particles.comp:594-631) inside its radius: Erase removes the
+         * particle, Heat and Cool add or take heat_power (1 - r^2) dt kelvin for water (not
+         * below -150 deg C), booked in the ledger; the others drive the velocity towards the
+         * tool's target by strength (1 - r^2) dt, less in a crowd (but the nozzle). */
+        int pushed = 0;
+        if (P->tool_mode != 0) {
+            float tdx = P->tool_pos[0] - p.pos[0], tdy = P->tool_pos[1] - p.pos[1];
+            float r2 = (tdx * tdx + tdy * tdy) / (P->tool_radius * P->tool_radius);
+            if (r2 < 1.0f) {
+                if (P->tool_mode == 2) {
+                    s->part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+                float fall = 1.0f - r2;
+                if (P->tool_mode == 5 || P->tool_mode == 6) {
+                    if (th_on) {
+                        float e = P->heat_power * fall * dt;
+                        v4 tth = thermal_of(s, row, p.s);
+                        float C = tth.w;
+                        if (P->tool_mode == 6) e = fminf_(e, fmaxf_(phase_of(p.t, tth).x + 150.0f, 0.0f) * C);
+                        p.t += (P->tool_mode == 5 ? e : -e) / C;
+                        ledger_add(s, P->tool_mode == 5 ? 4 : 5, e * P->inv_ppc);
+                    }
+                } else {
+                    float k = clampf(P->tool_strength * fall * dt, 0.0f, 1.0f);
+                    if (P->tool_crowd_max > 0.0f && P->tool_mode != 4)
+                        k *= clampf((P->tool_crowd_max - crowd) / (0.25f * P->tool_crowd_max), 0.0f, 1.0f);
+                    v2 tt = tool_target(P, tdx, tdy);
+                    vx = mixf(vx, tt.x, k);
+                    vy = mixf(vy, tt.y, k);
+                    pushed = 1;
+                }
+            }
+        }
         /* Boiling (particles.comp:633-731): past its boiling point liquid takes up latent
          * heat, its share the vapour quality x. Aqueous liquid in a tank that makes steam turns
          * into steam at x = steam.w (keeping mass, heat and the row it boiled from in KIND_SRC);
@@ -1234,7 +1302,7 @@ static void advance(or_sim* s, uint32_t id, int gather, scatter_in* g, float* um
         }
         /* Midpoint advection (particles.comp:847-865). */
         float vax = vx, vay = vy;
-        if (P->ext[1] >= 2.0f && !held) {
+        if (P->ext[1] >= 2.0f && !pushed && !held) {
             float h = 0.5f * dt;
             float ex = (vx + corrx) + 0.0f, ey = (vy + corry) + slip;
             float xmx = clampf(p.pos[0] + h * ex, 2.01f, (float)s->W - 2.01f);

@@ -112,7 +112,7 @@ class SimParams(C.Structure):
                 ("magnet_demag", C.c_float),
                 ("steam", C.c_uint32), ("steam_boil", C.c_float), ("steam_condense", C.c_float),
                 ("steam_mixing", C.c_float), ("steam_rise", C.c_float), ("steam_life", C.c_float),
-                ("steam_swirl", C.c_float), ("steam_eddy", C.c_float),
+                ("steam_swirl", C.c_float), ("steam_eddy", C.c_float), ("heat_power", C.c_float),
                 ("mat_count", C.c_int32),
                 ("mat", Material * MAX_MATERIALS)]
 
@@ -224,7 +224,9 @@ class GParams(C.Structure):
                 ("magnet", (C.c_float * 4) * MAX_MAGNETS), ("magnet_dir", (C.c_float * 2) * MAX_MAGNETS),
                 ("demag", C.c_float), ("current_zone", (C.c_float * 4) * MAX_CURRENTS),
                 ("current_vel", (C.c_float * 4) * MAX_CURRENTS), ("steam", C.c_float * 4),
-                ("steam2", C.c_float * 4)]
+                ("steam2", C.c_float * 4), ("tool_pos", C.c_float * 2), ("tool_vel", C.c_float * 2),
+                ("tool_radius", C.c_float), ("tool_mode", C.c_int32), ("tool_strength", C.c_float),
+                ("heat_power", C.c_float)]
 
 
 class DeviceView(C.Structure):

@@ -32,7 +32,7 @@ bool tank_layout_steam(const sp_sim_params& p, const std::vector<int>& rows, Tan
 // fastest |vel|; the tool's is 0 for the idle tool this build carries).
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats,
            std::string* why, const std::vector<sp_magnet>& magnets = {}, float ext_speed = 0.0f,
-           const std::vector<sp_current>& currents = {});
+           const std::vector<sp_current>& currents = {}, const sp_tool_state* tool = nullptr);
 void levels(int W, int H, sp_levels_info& L);
 int viscosity_mode(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);
 float dt_limit(const sp_sim_params& p, const std::vector<int>& rows, bool thermal = false);

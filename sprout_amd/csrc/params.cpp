@@ -132,6 +132,7 @@ void default_params(sp_sim_params* p) {
     p->steam_life = 2.0f;
     p->steam_swirl = 0.1f;
     p->steam_eddy = 0.2f;
+    p->heat_power = 300.0f;  // params.rs:403
     p->mat_count = SP_BUILTIN_COUNT;
     for (int r = 0; r < SP_MAX_MATERIALS; r++) p->mat[r] = default_material();
     // presets.rs:37-57 water
@@ -476,8 +477,20 @@ static uint32_t tank_caps(const sp_sim_params& p, const Tank& t, size_t n_magnet
     return caps;
 }
 
+// ToolState::speed (params.rs:1202-1210): the fastest velocity the tool imposes, cells / s.
+static float tool_speed(const sp_tool_state& t) {
+    switch (t.mode) {
+    case 1: return hypotf(t.vel[0], t.vel[1]) + 4.0f * t.radius;
+    case 3:
+    case 4: return hypotf(t.vel[0], t.vel[1]);
+    case 7: return fabsf(t.vel[0]) + 2.0f * t.radius;
+    default: return 0.0f;
+    }
+}
+
 int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots, sp_gparams& g, sp_gmat* mats, std::string* why,
-           const std::vector<sp_magnet>& magnets, float ext_speed, const std::vector<sp_current>& currents) {
+           const std::vector<sp_magnet>& magnets, float ext_speed, const std::vector<sp_current>& currents,
+           const sp_tool_state* tool) {
     memset(&g, 0, sizeof(g));
     memset(mats, 0, sizeof(sp_gmat) * SP_MAX_MATERIALS);
     const float dx = p.cell_size;
@@ -610,8 +623,6 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     g.density_k = p.correction_max;
     g.dt_max = dt_limit(p, t.rows, t.thermal);
     g.crowd_jitter = p.crowd_jitter;
-    // the tool's speed or the fastest queued spawn's, whichever is larger (mod.rs:2300)
-    g.ext_speed = std::max(ext_speed, 0.0f);
     g.wall_margin = 0.1f;
     g.tol[0] = p.pressure_tol;
     g.tol[1] = p.correction_tol;
@@ -637,6 +648,19 @@ int derive(const sp_sim_params& p, int W, int H, const Tank& t, uint32_t n_slots
     }
     g.ext[3] = (float)nc;
     if (nc > 0) g.caps |= SP_CAP_CURRENTS;
+    // the tool (mod.rs:2300-2305); the speed it imposes or the fastest queued spawn's,
+    // whichever is larger
+    const sp_tool_state idle{};
+    const sp_tool_state& tl = tool ? *tool : idle;
+    g.ext_speed = std::max(tool_speed(tl), std::max(ext_speed, 0.0f));
+    g.tool_pos[0] = tl.pos[0];
+    g.tool_pos[1] = tl.pos[1];
+    g.tool_vel[0] = tl.vel[0];
+    g.tool_vel[1] = tl.vel[1];
+    g.tool_radius = std::max(tl.radius, 0.01f);
+    g.tool_mode = tl.mode;
+    g.tool_strength = tl.strength;
+    g.heat_power = p.heat_power;
     // steam (mod.rs:2322-2323)
     g.steam[0] = std::min(std::max(p.steam_condense, 0.0f), 1.0f);
     g.steam[1] = p.steam_rise / dx;

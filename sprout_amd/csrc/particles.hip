@@ -152,6 +152,28 @@ SP_DEV float4 stress_update(const Ctx& c, float px, float py, float C0, float C1
     return t;
 }
 
+// ---- the mouse tool (air.glsl:24-37; particles.comp:594-631) -----------------------------
+
+// The velocity the tool drives a particle towards, (dx, dy) from the particle to the cursor:
+// grab (1) carries liquid with the cursor and pulls it in; push / pull (3) drives it radially
+// at tool_vel.x with a smooth centre; the nozzle (4) holds it at the launch velocity; whirl
+// (7) turns it round the cursor, rigidly inside 0.4 of the radius, with a gentle pull in;
+// hold (8) brings it to rest.
+SP_DEV float2 tool_target(const sp_gparams& P, float dx, float dy) {
+    const float l = len2_(dx, dy);
+    if (P.tool_mode == 3) {
+        const float m = fmaxf_(l, 0.15f * P.tool_radius);
+        return make_float2(-dx / m * P.tool_vel[0], -dy / m * P.tool_vel[0]);
+    }
+    if (P.tool_mode == 4) return make_float2(P.tool_vel[0], P.tool_vel[1]);
+    if (P.tool_mode == 7) {
+        const float m = fmaxf_(l, 0.4f * P.tool_radius);
+        return make_float2(dy / m * P.tool_vel[0] + dx * 2.0f, -dx / m * P.tool_vel[0] + dy * 2.0f);
+    }
+    if (P.tool_mode == 8) return make_float2(0.0f, 0.0f);
+    return make_float2(P.tool_vel[0] + dx * 4.0f, P.tool_vel[1] + dy * 4.0f);
+}
+
 // ---- the air above the liquid (air.glsl:91-175) ----------------------------------------
 // Air is not simulated: escaped vapour rides a procedural flow, a buoyant rise (steam.y) plus
 // the curl of a stream function of drifting value noise (steam2.z: its speed, steam2.w: its
@@ -213,8 +235,7 @@ SP_DEV float sample_occ(const Ctx& c, float x, float y) {
 // Bilinear velocity over live faces, `fb` where no face is live (air.glsl:41-78).
 SP_DEV float2 sample_vel(const Ctx& c, float px, float py, float2 fb);
 
-// The air's velocity at (x, y), cells / s, and the liquid occupancy there (air.glsl:148-175);
-// the mouse tools' stirring is not carried (only the idle tool is).
+// The air's velocity at (x, y), cells / s, and the liquid occupancy there (air.glsl:148-175).
 SP_DEV float2 air_velocity(const Ctx& c, float x, float y, float lean, float& occ) {
     const sp_gparams& P = *c.P;
     const float t = c.S->advanced;
@@ -252,7 +273,20 @@ SP_DEV float2 air_velocity(const Ctx& c, float x, float y, float lean, float& oc
     occ = sample_occ(c, x, y);
     const float2 vl = sample_vel(c, x, y, make_float2(vx, vy - P.steam[1]));
     const float k = smoothstep_(0.0f, 0.4f, occ);
-    return make_float2(mixf_(vx, vl.x, k), mixf_(vy, vl.y + P.steam[1], k));
+    vx = mixf_(vx, vl.x, k);
+    vy = mixf_(vy, vl.y + P.steam[1], k);
+    // the tool stirs, pushes and blows the air as it does the liquid (Erase: the caller)
+    const int tm = P.tool_mode;
+    if (tm == 1 || tm == 3 || tm == 4 || tm == 7) {
+        const float tdx = P.tool_pos[0] - x, tdy = P.tool_pos[1] - y;
+        const float r2 = (tdx * tdx + tdy * tdy) / (P.tool_radius * P.tool_radius);
+        if (r2 < 1.0f) {
+            const float2 tt = tool_target(P, tdx, tdy);
+            vx = mixf_(vx, tt.x, 1.0f - r2);
+            vy = mixf_(vy, tt.y, 1.0f - r2);
+        }
+    }
+    return make_float2(vx, vy);
 }
 
 // A puff of vapour that left the tank (particles.comp:103-134): a tracer in the air above the
@@ -270,6 +304,10 @@ SP_DEV void advance_puff(const Ctx& c, uint32_t id, sp_particle p) {
     float occ;
     const float2 target = air_velocity(c, p.pos[0], p.pos[1], puff_lean(p.id), occ);
     if (occ > 0.6f && p.s > 0.1f) {
+        c.part[id].kind = SP_KIND_DEAD;
+        return;
+    }
+    if (P.tool_mode == 2 && len2_(P.tool_pos[0] - p.pos[0], P.tool_pos[1] - p.pos[1]) < P.tool_radius) {
         c.part[id].kind = SP_KIND_DEAD;
         return;
     }
@@ -682,6 +720,40 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
                 }
             }
         }
+        // The mouse tool (particles.comp:594-631), inside its radius: Erase removes the
+        // particle; Heat and Cool add or take heat_power x (1 - r^2) x dt kelvin for water
+        // (not below -150 deg C), counted in the ledger; the others drive the velocity towards
+        // the tool's target by strength x (1 - r^2) x dt, less in a crowd (but the nozzle).
+        bool pushed = false;
+        if (X && P.tool_mode != 0) {
+            const float tdx = P.tool_pos[0] - p.pos[0], tdy = P.tool_pos[1] - p.pos[1];
+            const float r2 = (tdx * tdx + tdy * tdy) / (P.tool_radius * P.tool_radius);
+            if (r2 < 1.0f) {
+                if (P.tool_mode == 2) {
+                    c.part[id].kind = SP_KIND_DEAD;
+                    return;
+                }
+                const float fall = 1.0f - r2;
+                if (P.tool_mode == 5 || P.tool_mode == 6) {
+                    if (th_on) {
+                        float e = P.heat_power * fall * dt;
+                        const float4 tth = thermal_of(c, row, p.s);
+                        const float C = tth.w;
+                        if (P.tool_mode == 6) e = fminf_(e, fmaxf_(phase_of(p.t, tth).x + 150.0f, 0.0f) * C);
+                        p.t += (P.tool_mode == 5 ? e : -e) / C;
+                        ledger_add(c, P.tool_mode == 5 ? 4 : 5, e * P.inv_ppc);
+                    }
+                } else {
+                    float k = clampf_(P.tool_strength * fall * dt, 0.0f, 1.0f);
+                    if (P.tool_crowd_max > 0.0f && P.tool_mode != 4)
+                        k *= clampf_((P.tool_crowd_max - crowd) / (0.25f * P.tool_crowd_max), 0.0f, 1.0f);
+                    const float2 tt = tool_target(P, tdx, tdy);
+                    vx = mixf_(vx, tt.x, k);
+                    vy = mixf_(vy, tt.y, k);
+                    pushed = true;
+                }
+            }
+        }
         // Boiling (particles.comp:633-731). Liquid past its boiling point takes up latent heat
         // as it goes on heating; the share it holds is its vapour quality x. An aqueous liquid
         // in a tank that makes steam turns into steam at x = steam.w, keeping its mass and heat
@@ -870,7 +942,7 @@ SP_DEV void advance(const Ctx& c, uint32_t id, int gather, ScatterIn& g, float& 
             }
         }
         float vax = vx, vay = vy;
-        if (P.ext[1] >= 2.0f && !held) {
+        if (P.ext[1] >= 2.0f && !pushed && !held) {
             float h = 0.5f * dt;
             float ex = (vx + corrx) + 0.0f, ey = (vy + corry) + slip;
             float xmx = clampf_(p.pos[0] + h * ex, 2.01f, (float)c.W - 2.01f);

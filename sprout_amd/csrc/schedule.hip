@@ -338,7 +338,8 @@ const PKernel P_KERNELS[2][N_PSHAPES] = {SP_PK(false), SP_PK(true)};
 
 bool exotic_run(const Run& r) {
     return (r.gp.caps & (SP_CAP_ELASTIC | SP_CAP_SETTLE | SP_CAP_VOLATILE | SP_CAP_SELF_HEAT | SP_CAP_REACT |
-                         SP_CAP_CURRENTS | SP_CAP_BOILS | SP_CAP_PUFFS | SP_CAP_STEAM)) != 0u;
+                         SP_CAP_CURRENTS | SP_CAP_BOILS | SP_CAP_PUFFS | SP_CAP_STEAM)) != 0u ||
+           r.gp.tool_mode != 0;
 }
 
 // What a shape is tuned for: the grid, the tanks per launch, the accumulator stride and

@@ -94,6 +94,7 @@ struct sp_sim {
     uint32_t sort_interval = 8;
     std::vector<sp_particle> pending_spawn;
     float spawn_speed = 0.0f;  // fastest |vel| of the spawns queued for the next record (cells / s)
+    sp_tool_state tool{};      // the mouse tool of the latest record (idle: mode 0)
     double time_base = 0.0;
     double sim_time = 0.0;
     float dt_estimate = 1.0f / 240.0f;
@@ -207,7 +208,7 @@ int upload_tables(sp_sim* s) {
     if (!s->tables_dirty) return SP_OK;
     std::string why;
     int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, s->gp, s->gm, &why, s->magnets, s->spawn_speed,
-                        s->currents);
+                        s->currents, &s->tool);
     if (rc != SP_OK) {
         set_err(s, "%s", why.c_str());
         return rc;
@@ -449,10 +450,16 @@ void write_caps(sp_sim* s, const Recs& rec, hipStream_t stream) {
 // The per-tank part of FluidSim::record (mod.rs:2400-2563) before the substeps are
 // recorded: pending spawns into fresh or dead slots, the tables, and the batch words of the
 // step state. On the tank's own stream.
-int record_prepare(sp_sim* s, uint32_t substeps, float target) {
+int record_prepare(sp_sim* s, uint32_t substeps, float target, const sp_tool_state* tool = nullptr) {
     if (!s->params_set) {
         set_err(s, "sp_set_params has not been called");
         return SP_E_INVALID;
+    }
+    // the record's tool goes into the tables (gpu_params, mod.rs:2300-2305)
+    const sp_tool_state t = tool ? *tool : sp_tool_state{};
+    if (memcmp(&t, &s->tool, sizeof(t)) != 0) {
+        s->tool = t;
+        s->tables_dirty = true;
     }
     // pending spawns: fresh slots first, then dead slots (mod.rs:2485-2503, claim.comp)
     uint32_t claim = 0;
@@ -506,14 +513,14 @@ void maybe_sort(sp_sim* s, hipStream_t stream) {
 
 // FluidSim::record (mod.rs:2400-2563): the batch on the tank's stream, with the recorded
 // iterations that the read-backs finished so far set, and its read-back at its end.
-int record(sp_sim* s, uint32_t substeps, float target) {
+int record(sp_sim* s, uint32_t substeps, float target, const sp_tool_state* tool = nullptr) {
     poll_readbacks(s, false);
     if (s->rb_busy == sp_sim::RB_RING) {
         // every slot in flight: the oldest batch has to finish first
         HIPCHK(hipEventSynchronize(s->rb_end[s->rb_next]));
         poll_readbacks(s, false);
     }
-    int rc = record_prepare(s, substeps, target);
+    int rc = record_prepare(s, substeps, target, tool);
     if (rc != SP_OK) return rc;
     const uint32_t slot = (s->rb_next + s->rb_busy) % sp_sim::RB_RING;
     HIPCHK(hipEventRecord(s->rb_start[slot], s->stream));
@@ -893,11 +900,11 @@ int sp_spawn(sp_sim* s, const sp_particle* particles, uint32_t n, uint32_t* acce
 
 int sp_record(sp_sim* s, uint32_t substeps, float target_s, const sp_tool_state* tool) {
     return guarded(s, [&]() -> int {
-        if (tool && tool->mode != 0) {
-            set_err(s, "tools are not carried by this build (mode %d)", tool->mode);
-            return SP_E_UNSUPPORTED;
+        if (tool && (tool->mode < 0 || tool->mode > 8)) {
+            set_err(s, "unknown tool mode %d", tool->mode);
+            return SP_E_INVALID;
         }
-        return record(s, substeps, target_s);
+        return record(s, substeps, target_s, tool);
     });
 }
 
@@ -972,12 +979,12 @@ int sp_sim_time(sp_sim* s, double* out) {
 
 int sp_frame_blocking(sp_sim* s, uint32_t substeps, float target_s, const sp_tool_state* tool, double* covered) {
     return guarded(s, [&]() -> int {
-        if (tool && tool->mode != 0) {
-            set_err(s, "tools are not carried by this build (mode %d)", tool->mode);
-            return SP_E_UNSUPPORTED;
+        if (tool && (tool->mode < 0 || tool->mode > 8)) {
+            set_err(s, "unknown tool mode %d", tool->mode);
+            return SP_E_INVALID;
         }
         const double start = s->sim_time;
-        int rc = record(s, substeps, target_s);
+        int rc = record(s, substeps, target_s, tool);
         if (rc != SP_OK) return rc;
         rc = wait(s);
         if (rc != SP_OK) return rc;
@@ -1055,7 +1062,7 @@ int sp_material_table(sp_sim* s, sp_gpumat* out, uint32_t cap) {
             sp_gmat gm[SP_MAX_MATERIALS];
             std::string why;
             int rc = sp::derive(s->params, s->W, s->H, s->tank, s->n_slots, g, gm, &why, s->magnets, s->spawn_speed,
-                                s->currents);
+                                s->currents, &s->tool);
             if (rc != SP_OK) {
                 set_err(s, "%s", why.c_str());
                 return rc;

@@ -146,6 +146,14 @@ class FluidSim:
     def sort(self):
         self._check(self._lib.sp_sort(self._h), "sort")
 
+    def download_ledger(self):
+        """FluidSim::download_ledger (mod.rs:3389-3406): energy that crossed the boundary since
+        the last reset, J per metre of depth: gained from / lost to air, gained from / lost to
+        solids, heat tool, cool tool, boiled off, made inside."""
+        out = (C.c_double * 8)()
+        self._check(self._lib.sp_download_ledger(self._h, out), "download_ledger")
+        return [float(x) for x in out]
+
     def download_ledger_counts(self):
         """Per table row: (boiled or evaporated away, residues left, steam made) since the last
         reset (FluidSim::download_ledger's counts, mod.rs:3400-3404)."""

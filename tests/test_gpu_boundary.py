@@ -58,9 +58,10 @@ def test_substep_cost_and_frame_blocking():
         assert sim.sim_time() - t0 == pytest.approx(covered, rel=1e-9)
         c2 = sim.substep_cost()
         assert c2 > 0.0 and c2 != c1  # smoothed towards the new batch
-        with pytest.raises(N.SproutError):
-            tool = N.ToolState(mode=1)
+        with pytest.raises(N.SproutError) as e:
+            tool = N.ToolState(mode=9)  # no such tool
             sim.frame_blocking(1, 1 / 240, tool)
+        assert e.value.code == N.E_INVALID
     finally:
         sim.close()
 
