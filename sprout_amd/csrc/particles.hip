@@ -1157,24 +1157,28 @@ __global__ __launch_bounds__(NT, MINB) void k_particles(const TankBatch tb, int 
     __syncthreads();
     if (sbox[0] == 0x7fffffff) return;  // no live particle in the group
     const int stride = c.P->acc_stride;
+    // In LDS a cell's channels take an odd number of words (the accumulator's stride is a
+    // multiple of 4): the same channel of neighbouring cells, which a wave's lanes add to
+    // together, then falls into different banks. The extra word stays zero.
+    const int sstride = stride | 1;
     const int b0x = sbox[0] - 1, b0y = sbox[1] - 1;
     const int bw = sbox[2] + 2 - b0x + 1;
     const int bh = sbox[3] + 2 - b0y + 1;
-    const int n = bw * bh * stride;
+    const int n = bw * bh * sstride;
     const bool shared = n <= BOX;
     if (shared) {
         for (int k = t; k < n; k += NT) sacc[k] = 0;
         __syncthreads();
         if (g.live)
             deposit(g, scatter_stress(c, g, id),
-                    [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * stride + ch], v); });
+                    [&](int x, int y, int ch, int v) { atomicAdd(&sacc[((y - b0y) * bw + (x - b0x)) * sstride + ch], v); });
         __syncthreads();
         for (int k = t; k < n; k += NT) {
             int v = sacc[k];
             if (v == 0) continue;
-            int cell = k / stride;
+            int cell = k / sstride;
             int x = b0x + cell % bw, y = b0y + cell / bw;
-            atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + (k - cell * stride)], v);
+            atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + (k - cell * sstride)], v);
         }
     } else if (g.live) {
         deposit(g, scatter_stress(c, g, id), [&](int x, int y, int ch, int v) { atomicAdd(&c.acc[(size_t)(y * c.W + x) * stride + ch], v); });
