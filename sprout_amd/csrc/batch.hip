@@ -27,6 +27,7 @@ struct sp_batch {
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<uint64_t> keys;  // the grouping the groups were built for
     std::vector<uint64_t> gens;  // and the tanks' arena generations (an evicted and restored tank moved)
+    uint32_t recorded = 0;       // substeps of the latest sp_batch_record (none: nothing to read back)
     std::string err;
 };
 
@@ -138,6 +139,7 @@ const char* sp_batch_last_error(const sp_batch* b) { return b ? b->err.c_str() :
 int sp_batch_record(sp_batch* b, uint32_t substeps, float target_s) {
     return batch_guarded(b, [&]() -> int {
         HIPCHK(hipSetDevice(b->device));
+        b->recorded = substeps;
         for (size_t k = 0; k < b->sims.size(); k++) {
             sp_sim* s = b->sims[k];
             int rc = guarded(s, [&]() -> int { return record_prepare(s, substeps, target_s); });
@@ -181,7 +183,9 @@ int sp_batch_wait(sp_batch* b) {
         for (size_t k = 0; k < b->sims.size(); k++) {
             sp_sim* s = b->sims[k];
             int rc = guarded(s, [&]() -> int {
-                readback(s);
+                // a record of no substeps reads nothing back (record, mod.rs:2559-2561)
+                if (b->recorded > 0) readback(s);
+                else poll_readbacks(s, true);
                 return SP_OK;
             });
             if (rc != SP_OK) {

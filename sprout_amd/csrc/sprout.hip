@@ -522,6 +522,9 @@ int record(sp_sim* s, uint32_t substeps, float target, const sp_tool_state* tool
     }
     int rc = record_prepare(s, substeps, target, tool);
     if (rc != SP_OK) return rc;
+    // a record of no substeps (the uploads only) has nothing to read back: the next batches
+    // keep the iterations they record (record, mod.rs:2559-2561)
+    if (substeps == 0) return SP_OK;
     const uint32_t slot = (s->rb_next + s->rb_busy) % sp_sim::RB_RING;
     HIPCHK(hipEventRecord(s->rb_start[slot], s->stream));
     for (uint32_t k = 0; k < substeps; k++) {
