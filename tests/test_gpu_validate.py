@@ -17,6 +17,15 @@ reference measures them and held to the values the reference reports for its own
   speeds of 0.0001 and 0.00004 x sqrt(g h) (its checks: every particle stays, none outside
   the box, rms below 0.01, no unconverged solve).
 
+* hydrostatic pool (validate.rs:125-141): water 100 cells deep across the tank, 10 s: the
+  reference reports the surface level at -0.55 % and rms speed 0.00013 x sqrt(g h) (its checks:
+  particles conserved, level within 2 %, rms below 0.01);
+* viscosity, analytical (validate.rs:1120-1236; zero gravity, nu = 0.01 m^2/s): a free slab
+  with u = A cos(pi y / H) decays at nu (pi / H)^2 (reported -3.9 %; its check: within 5 %), a
+  slab on a no-slip floor with u = A sin(pi y / 2H) at nu (pi / 2H)^2 (reported -1.8 %; within
+  10 %), and a disc in rigid rotation keeps 0.975 of its angular momentum after 1 s (its check:
+  above 0.95, every viscosity solve converged).
+
 The reference's particles are sorted in atomic order and its arithmetic is whatever its
 Vulkan driver compiled, so the flow here is not the same particle for particle; the bars are
 the spread of the reference's own convergence study (4.417 / 4.501 / 4.508 at a = 30 / 60 / 120
@@ -148,6 +157,122 @@ def test_sealed_liquid_stays_at_rest(name):
         assert len(ps) == len(parts) and escaped == 0
         assert rms < 0.01
         assert unconv == 0 and subs > 100
+    finally:
+        sim.close()
+
+
+def viscous_params():
+    """validate.rs:1130-1139: zero gravity, nu = 0.01 m^2/s and no tension for the first rows."""
+    p = N.default_params()
+    p.gravity = 0.0
+    for r in range(3):
+        p.mat[r].viscosity = 0.01
+        p.mat[r].tension = 0.0
+    p.pair_tension[0][1] = 0.0
+    p.pair_tension[1][0] = 0.0
+    return p
+
+
+def particles_where(bbox, inside, vel, grad):
+    """validate.rs:1089-1106: rest-lattice water in bbox where `inside`, with a velocity field
+    and its gradient as the APIC matrix."""
+    scene = sc.Scene(GRID)
+    ps = sc.fill_rect(scene, bbox, N.ROW_WATER)
+    ps = ps[inside(ps["pos"])]
+    ps["vel"] = vel(ps["pos"])
+    ps["c"] = grad(ps["pos"])
+    return scene, ps
+
+
+def world_from(params, scene, ps):
+    sim = FluidSim(GRID, GRID[0] * GRID[1] * N.PPC, params=params)
+    sim.set_solids(scene.rasterize())
+    sim.reset(ps)
+    sim.step_blocking(0)
+    return sim
+
+
+def mode_amplitude(ps, pick, shape, comp):
+    f = shape(ps["pos"][pick]).astype(np.float64)
+    return float(np.sum(f * ps["vel"][pick, comp]) / max(np.sum(f * f), 1e-12))
+
+
+@pytest.mark.parametrize("case", ["free", "floor"])
+def test_shear_decays_at_the_viscous_rate(case):
+    params = viscous_params()
+    nu = params.mat[N.ROW_WATER].viscosity / params.cell_size ** 2
+    h, a = 40.0, 20.0
+    y0 = 100.0 if case == "free" else WALL
+    k = np.pi / h if case == "free" else np.pi / (2.0 * h)
+    if case == "free":
+        shape = lambda x: np.cos(k * (x[:, 1] - y0))
+        vel = lambda x: np.stack([a * np.cos(k * (x[:, 1] - y0)), 0.0 * x[:, 0]], 1)
+        grad = lambda x: np.stack([0.0 * x[:, 0], -a * k * np.sin(k * (x[:, 1] - y0)), 0.0 * x[:, 0], 0.0 * x[:, 0]], 1)
+        t, bar, reported = 1.0, 0.05, -3.9
+    else:
+        shape = lambda x: np.sin(k * (x[:, 1] - y0))
+        vel = lambda x: np.stack([a * np.sin(k * (x[:, 1] - y0)), 0.0 * x[:, 0]], 1)
+        grad = lambda x: np.stack([0.0 * x[:, 0], a * k * np.cos(k * (x[:, 1] - y0)), 0.0 * x[:, 0], 0.0 * x[:, 0]], 1)
+        t, bar, reported = 2.0, 0.10, -1.8
+    scene, ps = particles_where((80.0, y0, 432.0, y0 + h), lambda x: np.ones(len(x), bool), vel, grad)
+    sim = world_from(params, scene, ps)
+    try:
+        mid = lambda q: (q["pos"][:, 0] > 180.0) & (q["pos"][:, 0] < 332.0)
+        p0 = sim.download_particles()
+        a0 = mode_amplitude(p0, mid(p0), shape, 0)
+        sim.advance_blocking(t)
+        p1 = sim.download_particles()
+        a1 = mode_amplitude(p1, mid(p1), shape, 0)
+        rate = np.log(a0 / a1) / t
+        theory = nu * k * k
+        print(f"\nshear decay ({case}): {rate:.4f} /s against {theory:.4f} ({(rate / theory - 1) * 100:+.1f} %; "
+              f"reference {reported:+.1f} %)", flush=True)
+        assert abs(rate / theory - 1.0) < bar
+    finally:
+        sim.close()
+
+
+def test_rigid_rotation_keeps_its_angular_momentum():
+    params = viscous_params()
+    c, rad, omega = (256.0, 144.0), 50.0, 2.0
+    scene, ps = particles_where(
+        (c[0] - rad, c[1] - rad, c[0] + rad, c[1] + rad),
+        lambda x: np.hypot(x[:, 0] - c[0], x[:, 1] - c[1]) < rad,
+        lambda x: np.stack([-omega * (x[:, 1] - c[1]), omega * (x[:, 0] - c[0])], 1),
+        lambda x: np.tile(np.float32([0.0, -omega, omega, 0.0]), (len(x), 1)))
+    sim = world_from(params, scene, ps)
+    try:
+        def angular(q):
+            x, y = q["pos"][:, 0].astype(np.float64), q["pos"][:, 1].astype(np.float64)
+            return float(np.sum((x - x.mean()) * q["vel"][:, 1] - (y - y.mean()) * q["vel"][:, 0]))
+        l0 = angular(sim.download_particles())
+        v0 = sim.solver.viscosity_unconverged
+        sim.advance_blocking(1.0)
+        l1 = angular(sim.download_particles())
+        st = sim.solver
+        print(f"\nrigid rotation: {l1 / l0:.4f} of the angular momentum after 1 s (reference 0.975); "
+              f"{st.viscosity_unconverged - v0} unconverged viscosity solves", flush=True)
+        assert l1 / l0 > 0.95
+        assert st.viscosity_unconverged - v0 == 0
+    finally:
+        sim.close()
+
+
+def test_hydrostatic_pool_keeps_its_level_and_rest():
+    depth = 100.0 + WALL
+    sim, params = world((0.0, 0.0, float(GRID[0]), depth))
+    try:
+        n0 = len(sim.download_particles())
+        sim.advance_blocking(10.0)
+        ps = sim.download_particles()
+        top = percentile(ps["pos"][:, 1], 0.995) - WALL
+        v_ref = float(np.sqrt(params.gravity / params.cell_size * 100.0))
+        rms = rms_speed(ps) / v_ref
+        print(f"\nhydrostatic pool: {n0} -> {len(ps)} particles, surface {top:.2f} cells ({(top / 100 - 1) * 100:+.2f} %; "
+              f"reference -0.55 %), rms / sqrt(g h) {rms:.5f} (reference 0.00013)", flush=True)
+        assert len(ps) == n0
+        assert abs(top / 100.0 - 1.0) < 0.02
+        assert rms < 0.01
     finally:
         sim.close()
 
