@@ -20,6 +20,18 @@ reference measures them and held to the values the reference reports for its own
 * hydrostatic pool (validate.rs:125-141): water 100 cells deep across the tank, 10 s: the
   reference reports the surface level at -0.55 % and rms speed 0.00013 x sqrt(g h) (its checks:
   particles conserved, level within 2 %, rms below 0.01);
+* settling (validate.rs:1603-1655): water 250 cells deep across a 1024 x 576 tank, about a
+  million particles, 6 s sampled every 0.5 s: the reference reports a peak kinetic energy per
+  particle of 0.0005 cells^2/s^2, a mean height drift of +0.017 cells and no unconverged solve
+  of 486 substeps (its checks: peak below 0.05, rms below 0.001 sqrt(g h), drift below 0.05,
+  total energy rise between samples below 2e-4 of the pool's potential energy);
+* viscous gravity current (validate.rs:346-389): a honey column 50 wide and 60 high slumping
+  over the no-slip floor follows Huppert's (1982) x = 1.411 (g q^3 / 3 nu)^(1/5) t^(1/5); the
+  reference reports the front at 16 s as 1.02 of that and the spreading exponent between 4 and
+  16 s as 0.205 (its checks: within 0.2 and 0.05); the same column of water runs at least twice
+  as far in 1 s;
+* buoyancy (validate.rs:391-420): discs of oil at the bottom of a water pool and of honey near
+  its top, 12 s: the oil collects within 12 cells of the surface, the honey sinks below 60;
 * viscosity, analytical (validate.rs:1120-1236; zero gravity, nu = 0.01 m^2/s): a free slab
   with u = A cos(pi y / H) decays at nu (pi / H)^2 (reported -3.9 %; its check: within 5 %), a
   slab on a no-slip floor with u = A sin(pi y / 2H) at nu (pi / 2H)^2 (reported -1.8 %; within
@@ -184,8 +196,8 @@ def particles_where(bbox, inside, vel, grad):
     return scene, ps
 
 
-def world_from(params, scene, ps):
-    sim = FluidSim(GRID, GRID[0] * GRID[1] * N.PPC, params=params)
+def world_from(params, scene, ps, grid=GRID):
+    sim = FluidSim(grid, grid[0] * grid[1] * N.PPC, params=params)
     sim.set_solids(scene.rasterize())
     sim.reset(ps)
     sim.step_blocking(0)
@@ -273,6 +285,121 @@ def test_hydrostatic_pool_keeps_its_level_and_rest():
         assert len(ps) == n0
         assert abs(top / 100.0 - 1.0) < 0.02
         assert rms < 0.01
+    finally:
+        sim.close()
+
+
+def row_of(ps):
+    return (ps["kind"] & 255).astype(np.int64) - 1
+
+
+def mean_height(ps, w):
+    w = np.asarray(w, np.float64)
+    return float(np.sum(w * ps["pos"][:, 1]) / max(float(np.sum(w)), 1e-9))
+
+
+def test_large_pool_settles_without_energy():
+    grid = (1024, 576)
+    params = N.default_params()
+    scene = sc.Scene(grid)
+    ps = sc.fill_rect(scene, (0.0, 0.0, 1024.0, 250.0), N.ROW_WATER)
+    n0 = len(ps)
+    sim = world_from(params, scene, ps, grid)
+    try:
+        g = params.gravity / params.cell_size
+
+        def energy(ps):
+            v = ps["vel"].astype(np.float64)
+            ke = 0.5 * float(np.mean(np.sum(v * v, 1)))
+            return ke, ke + g * float(np.mean(ps["pos"][:, 1].astype(np.float64)))
+
+        p = sim.download_particles()
+        last = energy(p)[1]
+        y0 = mean_height(p, np.ones(len(p)))
+        worst_rise = ke_peak = 0.0
+        for _ in range(12):
+            sim.advance_blocking(0.5)
+            ke, tot = energy(sim.download_particles())
+            worst_rise = max(worst_rise, tot - last)
+            ke_peak = max(ke_peak, ke)
+            last = tot
+        p = sim.download_particles()
+        rms = rms_speed(p) / float(np.sqrt(g * 247.0))
+        drift = mean_height(p, np.ones(len(p))) - y0
+        rise = worst_rise / (g * y0)
+        st = sim.solver
+        print(f"\nsettling 1024 x 576: {n0} -> {len(p)} particles, peak KE {ke_peak:.4f} (reference 0.0005), "
+              f"rms / sqrt(g h) {rms:.6f}, drift {drift:+.3f} cells (reference +0.017), energy rise {rise:.1e}; "
+              f"{st.pressure_unconverged} unconverged of {st.substeps} substeps (reference 0 of 486)", flush=True)
+        assert len(p) == n0
+        assert ke_peak < 0.05
+        assert rms < 0.001
+        assert abs(drift) < 0.05
+        assert rise < 2e-4
+        assert st.pressure_unconverged == 0
+    finally:
+        sim.close()
+
+
+def front(sim):
+    return percentile(sim.download_particles()["pos"][:, 0], 0.998) - WALL
+
+
+def test_honey_slump_follows_huppert():
+    bw, bh = 50.0, 60.0
+    sim, params = world((0.0, 0.0, WALL + bw, WALL + bh), N.ROW_HONEY)
+    try:
+        q = len(sim.download_particles()) / N.PPC
+        g = params.gravity / params.cell_size
+        nu = params.mat[N.ROW_HONEY].viscosity / params.cell_size ** 2
+        k = 1.411 * (g * q ** 3 / (3.0 * nu)) ** 0.2
+        t, pts = 0.0, []
+        for target in (2.0, 4.0, 8.0, 16.0):
+            sim.advance_blocking(target - t)
+            t = target
+            pts.append((target, front(sim)))
+        exponent = np.log(pts[3][1] / pts[1][1]) / np.log(pts[3][0] / pts[1][0])
+        ratio = pts[3][1] / (k * pts[3][0] ** 0.2)
+        line = " ".join(f"t={a:.0f}s {b:.0f}/{k * a ** 0.2:.0f}" for a, b in pts)
+        print(f"\nhoney slump: {line}; front / theory at 16 s {ratio:.2f} (reference 1.02), exponent {exponent:.3f} "
+              f"(reference 0.205); {sim.solver.viscosity_unconverged} unconverged viscosity solves", flush=True)
+        assert abs(exponent - 0.2) < 0.05
+        assert abs(ratio - 1.0) < 0.2
+        assert ratio == pytest.approx(1.02, abs=0.05)
+    finally:
+        sim.close()
+    fronts = []
+    for row in (N.ROW_WATER, N.ROW_HONEY):
+        sim, _ = world((0.0, 0.0, WALL + bw, WALL + bh), row)
+        try:
+            sim.advance_blocking(1.0)
+            fronts.append(front(sim))
+        finally:
+            sim.close()
+    print(f"front after 1 s: water {fronts[0]:.0f}, honey {fronts[1]:.0f} cells", flush=True)
+    assert fronts[0] - bw > 2.0 * (fronts[1] - bw)
+
+
+def test_oil_rises_and_honey_sinks_in_a_water_pool():
+    params = N.default_params()
+    scene = sc.Scene(GRID)
+    ps = sc.fill_rect(scene, (0.0, 0.0, 512.0, 150.0), N.ROW_WATER)
+    oil_c, honey_c, rad = (150.0, 35.0), (360.0, 115.0), 22.0
+    for c, row in ((oil_c, N.ROW_OIL), (honey_c, N.ROW_HONEY)):
+        inside = np.hypot(ps["pos"][:, 0] - c[0], ps["pos"][:, 1] - c[1]) < rad
+        ps[inside] = sc.new_particles(row, ps["pos"][inside].copy())
+    sim = world_from(params, scene, ps)
+    try:
+        sim.advance_blocking(12.0)
+        p = sim.download_particles()
+        surface = percentile(p["pos"][:, 1], 0.995)
+        rows = row_of(p)
+        oil_y = mean_height(p, rows == N.ROW_OIL)
+        honey_y = mean_height(p, np.where(rows == N.ROW_HONEY, p["s"], 0.0))
+        print(f"\nbuoyancy: oil {oil_c[1]:.0f} -> {oil_y:.0f} cells (surface {surface:.0f}), "
+              f"honey {honey_c[1]:.0f} -> {honey_y:.0f} cells", flush=True)
+        assert surface - oil_y < 12.0
+        assert honey_y < 60.0
     finally:
         sim.close()
 
