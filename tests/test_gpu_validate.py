@@ -32,6 +32,13 @@ reference measures them and held to the values the reference reports for its own
   as far in 1 s;
 * buoyancy (validate.rs:391-420): discs of oil at the bottom of a water pool and of honey near
   its top, 12 s: the oil collects within 12 cells of the surface, the honey sinks below 60;
+* surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
+  radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
+  currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
+  sigma_oa / R (1.002); a water drop of radius 20 stretched by 8 % oscillates in its n = 2
+  mode at Rayleigh's period (+0.6 %; within 10 %); the same oil drop inside water at 2.220 s,
+  +6.3 % over the viscous normal mode's 2.087 s (within 8 %); a honey blob in water, miscible,
+  keeps its shape (ellipticity unchanged within 0.01; reported 0.000);
 * viscosity, analytical (validate.rs:1120-1236; zero gravity, nu = 0.01 m^2/s): a free slab
   with u = A cos(pi y / H) decays at nu (pi / H)^2 (reported -3.9 %; its check: within 5 %), a
   slab on a no-slip floor with u = A sin(pi y / 2H) at nu (pi / 2H)^2 (reported -1.8 %; within
@@ -400,6 +407,186 @@ def test_oil_rises_and_honey_sinks_in_a_water_pool():
               f"honey {honey_c[1]:.0f} -> {honey_y:.0f} cells", flush=True)
         assert surface - oil_y < 12.0
         assert honey_y < 60.0
+    finally:
+        sim.close()
+
+
+CENTRE = (256.0, 144.0)
+
+
+def capillary_params():
+    """validate.rs:1277-1281: 1 mm cells, zero gravity."""
+    params = N.default_params()
+    params.cell_size = 0.001
+    params.gravity = 0.0
+    return params
+
+
+def tension_cells(params, sigma):
+    """params.rs:811-814: cells^3/s^2 over water's density."""
+    return sigma / (1000.0 * params.cell_size ** 3)
+
+
+def disc(rad):
+    return lambda x: np.hypot(x[:, 0] - CENTRE[0], x[:, 1] - CENTRE[1]) < rad
+
+
+def ellipse(rad, eps):
+    ax, ay = rad * (1.0 + eps), rad / (1.0 + eps)
+    return (lambda x: ((x[:, 0] - CENTRE[0]) / ax) ** 2 + ((x[:, 1] - CENTRE[1]) / ay) ** 2 < 1.0), ax, ay
+
+
+def still(x):
+    return np.zeros((len(x), 2), np.float32)
+
+
+def still4(x):
+    return np.zeros((len(x), 4), np.float32)
+
+
+def pool_with(inside, row):
+    """Water over [0, 512] x [0, 260] with the particles `inside` swapped for `row`."""
+    scene, ps = particles_where((0.0, 0.0, 512.0, 260.0), lambda x: np.ones(len(x), bool), still, still4)
+    sel = inside(ps["pos"])
+    ps[sel] = sc.new_particles(row, ps["pos"][sel].copy())
+    return scene, ps
+
+
+def ring_pressure(sim, r0, r1):
+    pres, dt = sim.download_pressure()
+    H, W = pres.shape
+    j, i = np.mgrid[0:H, 0:W]
+    d = np.hypot(i + 0.5 - CENTRE[0], j + 0.5 - CENTRE[1])
+    sel = (d >= r0) & (d < r1)
+    return float(pres[sel].astype(np.float64).mean()) / dt
+
+
+def ellipticity(ps, pick):
+    """validate.rs:1240-1252: (Ixx - Iyy) / (Ixx + Iyy) about the centroid."""
+    x = ps["pos"][pick].astype(np.float64)
+    d = x - x.mean(0)
+    ixx, iyy = float(np.sum(d[:, 0] ** 2)), float(np.sum(d[:, 1] ** 2))
+    return (ixx - iyy) / max(ixx + iyy, 1e-12)
+
+
+def period(samples):
+    """validate.rs:1254-1268: twice the mean spacing of the zero crossings."""
+    cross = []
+    for (t0, e0), (t1, e1) in zip(samples, samples[1:]):
+        if np.sign(e0) != np.sign(e1) and e0 != 0.0:
+            cross.append(t0 + (t1 - t0) * e0 / (e0 - e1))
+    if len(cross) < 3:
+        return None
+    return 2.0 * (cross[-1] - cross[0]) / (len(cross) - 1)
+
+
+def test_resting_drop_holds_its_laplace_pressure():
+    params = capillary_params()
+    sigma = tension_cells(params, params.mat[N.ROW_WATER].tension)
+    rad = 30.0
+    c = CENTRE
+    scene, ps = particles_where((c[0] - rad, c[1] - rad, c[0] + rad, c[1] + rad), disc(rad), still, still4)
+    sim = world_from(params, scene, ps)
+    try:
+        sim.advance_blocking(0.3)
+        p_in = ring_pressure(sim, 0.0, 0.5 * rad) / (sigma / rad)
+        spurious = rms_speed(sim.download_particles()) / np.sqrt(sigma / rad)
+        print(f"\nresting drop: pressure / (sigma / R) {p_in:.3f} (reference 1.002), spurious currents "
+              f"{spurious * 100:.2f} % (reference 1.2 %)", flush=True)
+        assert abs(p_in - 1.0) < 0.10
+        assert spurious < 0.02
+        assert p_in == pytest.approx(1.002, abs=0.02)
+    finally:
+        sim.close()
+
+
+def test_oil_disc_in_water_holds_the_interfacial_jump():
+    params = capillary_params()
+    sigma = tension_cells(params, params.pair_tension[0][1])
+    rad = 30.0
+    scene, ps = pool_with(disc(rad), N.ROW_OIL)
+    sim = world_from(params, scene, ps)
+    try:
+        sim.advance_blocking(0.3)
+        jump = (ring_pressure(sim, 0.0, 0.5 * rad) - ring_pressure(sim, rad + 8.0, rad + 16.0)) / (sigma / rad)
+        print(f"\noil disc in water: jump / (sigma_oa / R) {jump:.3f} (reference 1.002)", flush=True)
+        assert abs(jump - 1.0) < 0.10
+        assert jump == pytest.approx(1.002, abs=0.02)
+    finally:
+        sim.close()
+
+
+def oscillation(sim, t_step, samples, pick):
+    out = []
+    for k in range(samples + 1):
+        ps = sim.download_particles()
+        out.append((k * t_step, ellipticity(ps, pick(ps))))
+        sim.advance_blocking(t_step)
+    return period(out)
+
+
+def test_water_drop_oscillates_at_rayleighs_period():
+    params = capillary_params()
+    sigma = tension_cells(params, params.mat[N.ROW_WATER].tension)
+    rad, eps = 20.0, 0.08
+    inside, ax, ay = ellipse(rad, eps)
+    c = CENTRE
+    scene, ps = particles_where((c[0] - ax, c[1] - ay, c[0] + ax, c[1] + ay), inside, still, still4)
+    sim = world_from(params, scene, ps)
+    try:
+        t_theory = 2.0 * np.pi / np.sqrt(6.0 * sigma / rad ** 3)
+        t = oscillation(sim, t_theory / 40.0, 100, lambda ps: np.ones(len(ps), bool))
+        print(f"\nwater drop n = 2: period {t:.3f} s against Rayleigh's {t_theory:.3f} s "
+              f"({(t / t_theory - 1) * 100:+.1f} %; reference +0.6 %)", flush=True)
+        assert t is not None and abs(t / t_theory - 1.0) < 0.10
+        assert t / t_theory - 1.0 == pytest.approx(0.006, abs=0.03)
+    finally:
+        sim.close()
+
+
+def test_oil_drop_in_water_oscillates_at_the_viscous_normal_mode():
+    params = capillary_params()
+    sigma = tension_cells(params, params.pair_tension[0][1])
+    rad, eps = 20.0, 0.08
+    inside, _, _ = ellipse(rad, eps)
+    scene, ps = pool_with(inside, N.ROW_OIL)
+    sim = world_from(params, scene, ps)
+    try:
+        rho_sum = 1.0 + params.mat[N.ROW_OIL].density / params.mat[N.ROW_WATER].density
+        t_inviscid = 2.0 * np.pi / np.sqrt(6.0 * sigma / (rho_sum * rad ** 3))
+        t_viscous = 2.0874  # validate.rs:1439: the linear viscous normal mode at these settings
+        t = oscillation(sim, t_inviscid / 40.0, 90, lambda ps: row_of(ps) == N.ROW_OIL)
+        print(f"\noil drop in water n = 2: period {t:.3f} s against the viscous mode's {t_viscous:.3f} s "
+              f"({(t / t_viscous - 1) * 100:+.1f} %; reference 2.220 s, +6.3 %); inviscid {t_inviscid:.3f} s; "
+              f"{sim.solver.pressure_unconverged} unconverged of {sim.solver.substeps}", flush=True)
+        assert t is not None and abs(t / t_viscous - 1.0) < 0.08
+        assert t == pytest.approx(2.220, rel=0.03)
+    finally:
+        sim.close()
+
+
+def test_honey_blob_in_water_keeps_its_shape():
+    params = capillary_params()
+    params.mat[N.ROW_HONEY].mix_rate = 0.0
+    sigma = tension_cells(params, params.pair_tension[0][1])
+    rad, eps = 20.0, 0.08
+    inside, _, _ = ellipse(rad, eps)
+    scene, ps = pool_with(inside, N.ROW_HONEY)
+    sim = world_from(params, scene, ps)
+    try:
+        rho_sum = 1.0 + params.mat[N.ROW_OIL].density / params.mat[N.ROW_WATER].density
+        t_oil = 2.0 * np.pi / np.sqrt(6.0 * sigma / (rho_sum * rad ** 3))
+
+        def honey(ps):
+            return (row_of(ps) == N.ROW_HONEY) & (ps["s"] > 0.5)
+
+        p0 = sim.download_particles()
+        e0 = ellipticity(p0, honey(p0))
+        sim.advance_blocking(t_oil / 2.0)
+        p1 = sim.download_particles()
+        e1 = ellipticity(p1, honey(p1))
+        print(f"\nhoney blob in water: ellipticity {e0:.3f} -> {e1:.3f} (reference: change 0.000)", flush=True)
+        assert abs(e1 - e0) < 0.01
     finally:
         sim.close()
 
