@@ -32,6 +32,13 @@ reference measures them and held to the values the reference reports for its own
   as far in 1 s;
 * buoyancy (validate.rs:391-420): discs of oil at the bottom of a water pool and of honey near
   its top, 12 s: the oil collects within 12 cells of the surface, the honey sinks below 60;
+* Rayleigh-Taylor overturn (validate.rs:241-344): oil under water under honey, both interfaces
+  carrying a one-cell sine bump of wavelength 64, 48 s: the reference reports mean heights of
+  149 / 83 / 45 cells, 0.46 % of the aqueous phase left above the oil layer's floor (strip by
+  strip) and the mechanical energy never rising by more than +0.04 % of the energy released
+  (its checks: sorted with oil 20 cells above water, under 3 % trapped, honey volume within
+  0.1 %, over half the aqueous particles a blend, oil conserved, area within 3 %, energy rise
+  and kinetic energy left under 1 % of the released);
 * surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
   radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
   currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
@@ -587,6 +594,109 @@ def test_honey_blob_in_water_keeps_its_shape():
         e1 = ellipticity(p1, honey(p1))
         print(f"\nhoney blob in water: ellipticity {e0:.3f} -> {e1:.3f} (reference: change 0.000)", flush=True)
         assert abs(e1 - e0) < 0.01
+    finally:
+        sim.close()
+
+
+def test_three_layers_overturn_and_sort():
+    params = N.default_params()
+    scene = sc.Scene(GRID)
+    ps = sc.fill_rect(scene, (0.0, 0.0, 512.0, 180.0), N.ROW_WATER)
+    bump = np.sin(ps["pos"][:, 0] * np.float32(2.0 * np.pi) / np.float32(64.0))
+    oil = ps["pos"][:, 1] < 60.0 + bump
+    honey = ~oil & (ps["pos"][:, 1] >= 120.0 - bump)
+    for sel, row in ((oil, N.ROW_OIL), (honey, N.ROW_HONEY)):
+        ps[sel] = sc.new_particles(row, ps["pos"][sel].copy())
+    sim = world_from(params, scene, ps)
+    g = params.gravity / params.cell_size
+    mats = params.mat
+
+    def honey_in(p):
+        return np.where(row_of(p) == N.ROW_HONEY, p["s"], 0.0).astype(np.float64)
+
+    def rho(p):
+        r = row_of(p)
+        out = np.zeros(len(p))
+        for row in np.unique(r):
+            own = mats[row]
+            sel = r == row
+            if own.diluent >= 0:
+                d = mats[own.diluent].density
+                out[sel] = d + (own.density - d) * p["s"][sel].astype(np.float64)
+            else:
+                out[sel] = own.density
+        return out / 1000.0
+
+    def energy(p):
+        v = p["vel"].astype(np.float64)
+        w = rho(p)
+        ke = float(np.mean(0.5 * w * np.sum(v * v, 1)))
+        pe = float(np.mean(w * g * p["pos"][:, 1].astype(np.float64)))
+        return ke, pe
+
+    def trapped(p):
+        strip = np.minimum((p["pos"][:, 0] / np.float32(GRID[0]) * np.float32(16.0)).astype(np.int64), 15)
+        is_oil = row_of(p) == N.ROW_OIL
+        above = 0
+        for k in range(16):
+            floor = percentile(p["pos"][is_oil & (strip == k), 1], 0.05)
+            above += int(np.sum(~is_oil & (strip == k) & (p["pos"][:, 1] > floor)))
+        return above / max(int(np.sum(~is_oil)), 1)
+
+    def area(p):
+        i = p["pos"][:, 0].astype(np.int64)
+        j = p["pos"][:, 1].astype(np.int64)
+        ok = (i >= 0) & (i < GRID[0]) & (j >= 0) & (j < GRID[1])
+        return len(np.unique(j[ok] * GRID[0] + i[ok]))
+
+    try:
+        before = sim.download_particles()
+        honey0 = float(np.sum(honey_in(before)))
+        area0 = area(before)
+        pe0 = energy(before)[1]
+        last, worst_rise, ke_peak = pe0, 0.0, 0.0
+        trend = []
+        p = before
+        for k in range(24):
+            sim.advance_blocking(2.0)
+            p = sim.download_particles()
+            if k % 4 == 3:
+                trend.append(trapped(p))
+            ke, pe = energy(p)
+            worst_rise = max(worst_rise, ke + pe - last)
+            ke_peak = max(ke_peak, ke)
+            last = ke + pe
+        ke1, pe1 = energy(p)
+        released = pe0 - pe1
+        r = row_of(p)
+        is_oil = r == N.ROW_OIL
+        h = honey_in(p)
+        oil_y = mean_height(p, is_oil)
+        water_y = mean_height(p, np.where(is_oil, 0.0, 1.0 - h))
+        honey_y = mean_height(p, h)
+        below = trapped(p)
+        aq = h[~is_oil]
+        mixed = float(np.mean((aq > 0.1) & (aq < 0.9)))
+        honey_drift = float(np.sum(h)) / honey0 - 1.0
+        area_drift = area(p) / area0 - 1.0
+        st = sim.solver
+        print(f"\noverturn, 48 s: mean heights {oil_y:.0f} / {water_y:.0f} / {honey_y:.0f} (reference 149 / 83 / 45); "
+              f"above the oil floor {' '.join(f'{t * 100:.2f}' for t in trend)} % (reference at 48 s 0.46 %); "
+              f"energy rise {worst_rise / released * 100:+.2f} % of released (reference +0.04 %), left "
+              f"{ke1 / released * 100:.2f} %; blended {mixed * 100:.0f} %; honey {honey_drift * 100:+.4f} %, area "
+              f"{area_drift * 100:+.2f} %; {st.pressure_unconverged} / {st.viscosity_unconverged} unconverged of "
+              f"{st.substeps}", flush=True)
+        assert oil_y > water_y + 20.0 and water_y > honey_y
+        assert below < 0.03
+        assert abs(honey_drift) < 1e-3
+        assert mixed > 0.5
+        assert int(np.sum(row_of(before) == N.ROW_OIL)) == int(np.sum(is_oil))
+        assert abs(area_drift) < 0.03
+        assert worst_rise < 0.01 * released
+        assert ke1 < 0.01 * released
+        # the reference's reported heights, to a few cells
+        for got, want in ((oil_y, 149.0), (water_y, 83.0), (honey_y, 45.0)):
+            assert abs(got - want) < 4.0
     finally:
         sim.close()
 
