@@ -92,8 +92,10 @@ int sp_poll(sp_sim* sim, sp_solver_stats* out);
 
 /* `FluidSim::advance_blocking(seconds)` (src/sim/mod.rs:3284-3312). */
 int sp_advance_blocking(sp_sim* sim, double seconds);
-/* `FluidSim::step_blocking(n)` (src/sim/mod.rs:3316-3338): n ticks of 1/240 s. */
-int sp_step_blocking(sp_sim* sim, uint32_t n);
+/* `FluidSim::step_blocking(n, tool)` (src/sim/mod.rs:3316-3338): n ticks of 1/240 s in
+ * records of at most 16 substeps, each under `tool` (NULL = the idle tool); n = 0 records no
+ * substep and uploads what changed. */
+int sp_step_blocking(sp_sim* sim, uint32_t n, const sp_tool_state* tool);
 /* `FluidSim::sim_time` (src/sim/mod.rs:1956). */
 int sp_sim_time(sp_sim* sim, double* out);
 /* `FluidSim::frame_blocking(substeps, target, tool)` (src/sim/mod.rs:3342-3350): one

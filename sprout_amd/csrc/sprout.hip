@@ -924,10 +924,14 @@ int sp_poll(sp_sim* s, sp_solver_stats* out) {
     }, false);
 }
 
-int sp_step_blocking(sp_sim* s, uint32_t n) {
+int sp_step_blocking(sp_sim* s, uint32_t n, const sp_tool_state* tool) {
     return guarded(s, [&]() -> int {
+        if (tool && (tool->mode < 0 || tool->mode > 8)) {
+            set_err(s, "unknown tool mode %d", tool->mode);
+            return SP_E_INVALID;
+        }
         if (n == 0) {
-            int rc = record(s, 0, 0.0f);
+            int rc = record(s, 0, 0.0f, tool);
             if (rc != SP_OK) return rc;
             return wait(s);
         }
@@ -941,7 +945,7 @@ int sp_step_blocking(sp_sim* s, uint32_t n) {
             if (left <= sliver * dt) break;
             uint32_t k = (uint32_t)std::min(std::max(ceil(left / dt), 1.0), 16.0);
             double target = s->params.fixed_hz > 0.0f ? k * dt : left;
-            int rc = record(s, k, (float)target);
+            int rc = record(s, k, (float)target, tool);
             if (rc != SP_OK) return rc;
             rc = wait(s);
             if (rc != SP_OK) return rc;

@@ -106,9 +106,10 @@ class FluidSim:
     def wait(self):
         self._check(self._lib.sp_wait(self._h), "wait")
 
-    def step_blocking(self, substeps):
-        """FluidSim::step_blocking (mod.rs:3316-3338): ticks of 1/240 s."""
-        self._check(self._lib.sp_step_blocking(self._h, substeps), "step_blocking")
+    def step_blocking(self, substeps, tool=None):
+        """FluidSim::step_blocking (mod.rs:3316-3338): ticks of 1/240 s under `tool`."""
+        t = C.byref(tool) if tool is not None else None
+        self._check(self._lib.sp_step_blocking(self._h, substeps, t), "step_blocking")
 
     def advance_blocking(self, seconds):
         """FluidSim::advance_blocking (mod.rs:3284-3312)."""

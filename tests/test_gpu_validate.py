@@ -39,6 +39,12 @@ reference measures them and held to the values the reference reports for its own
   (its checks: sorted with oil 20 cells above water, under 3 % trapped, honey volume within
   0.1 %, over half the aqueous particles a blend, oil conserved, area within 3 %, energy rise
   and kinetic energy left under 1 % of the released);
+* stir and settle (validate.rs:1655-1700, 1734-1780): a pool 60 cells deep stirred for 1 s by
+  Grab (radius 12, 200 cells/s) and left 10 s: the reference reports the energy above rest
+  per particle falling 2053 -> 1063 and the particle-scale jitter (rms of the velocity less its
+  4 x 4-cell mean) 8.4 -> 3.0 cells/s (its checks: the 2 s means fall, no rise beyond the
+  density correction's lift g x 0.016 cells, each jitter sample below the one 4 s before, the
+  last under half the first and under 2 % of the energy);
 * surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
   radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
   currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
@@ -697,6 +703,57 @@ def test_three_layers_overturn_and_sort():
         # the reference's reported heights, to a few cells
         for got, want in ((oil_y, 149.0), (water_y, 83.0), (honey_y, 45.0)):
             assert abs(got - want) < 4.0
+    finally:
+        sim.close()
+
+
+def test_stirred_pool_settles_and_its_jitter_decays():
+    sim, params = world((0.0, 0.0, 512.0, WALL + 60.0))
+    try:
+        g = params.gravity / params.cell_size
+        speed, radius = 200.0, 12.0
+        x = 150.0
+        for _ in range(60):
+            x += speed * 4.0 / 240.0
+            tool = N.ToolState()
+            tool.pos[0], tool.pos[1] = x, WALL + 30.0
+            tool.vel[0], tool.vel[1] = speed, 0.0
+            tool.radius, tool.mode, tool.strength = radius, 1, 40.0
+            sim.step_blocking(4, tool)
+
+        def jitter(p):
+            bw, bh = GRID[0] // 4 + 1, GRID[1] // 4 + 1
+            k = (p["pos"][:, 1].astype(np.int64) // 4) * bw + p["pos"][:, 0].astype(np.int64) // 4
+            v = p["vel"].astype(np.float64)
+            n = np.bincount(k, minlength=bw * bh)
+            mx = np.bincount(k, v[:, 0], bw * bh)[k] / n[k]
+            my = np.bincount(k, v[:, 1], bw * bh)[k] / n[k]
+            return float(np.sqrt(np.mean((v[:, 0] - mx) ** 2 + (v[:, 1] - my) ** 2)))
+
+        rest = g * (WALL + 30.0)
+        rows, series = [], []
+        for k in range(41):
+            p = sim.download_particles()
+            v = p["vel"].astype(np.float64)
+            e = 0.5 * float(np.mean(np.sum(v * v, 1))) + g * float(np.mean(p["pos"][:, 1].astype(np.float64))) - rest
+            series.append(e)
+            if k % 8 == 0:
+                rows.append((k * 0.25, e, jitter(p)))
+            if k < 40:
+                sim.step_blocking(60)
+        means = [sum(series[8 * k + 1:8 * k + 9]) / 8.0 for k in range(5)]
+        ke_first, ke_last = rows[0][1], rows[-1][1]
+        lift = g * 0.016
+        worst = max(b - a for a, b in zip(means, means[1:]))
+        j_last = rows[-1][2]
+        share = 0.5 * j_last ** 2 / max(ke_last, 1e-9)
+        print(f"\nstir and settle: energy above rest {ke_first:.0f} -> {ke_last:.0f} (reference 2053 -> 1063), 2 s means "
+              f"{' '.join(f'{m:.0f}' for m in means)}, worst change {worst:+.1f} (lift {lift:.1f}); jitter "
+              f"{' '.join(f'{r[2]:.2f}' for r in rows)} cells/s (reference 8.4 -> 3.0), {share * 100:.2f} % of the energy",
+              flush=True)
+        assert means[0] < ke_first and means[4] < means[0] and worst < lift and ke_last > 0.0
+        assert all(c[2] < a[2] for a, c in zip(rows, rows[2:]))
+        assert j_last < 0.5 * rows[0][2] and share < 0.02
     finally:
         sim.close()
 
