@@ -52,6 +52,13 @@ reference measures them and held to the values the reference reports for its own
   mode at Rayleigh's period (+0.6 %; within 10 %); the same oil drop inside water at 2.220 s,
   +6.3 % over the viscous normal mode's 2.087 s (within 8 %); a honey blob in water, miscible,
   keeps its shape (ellipticity unchanged within 0.01; reported 0.000);
+* beading (validate.rs:1564-1600): a 30-cell square of oil inside water, zero gravity, 0.5 s:
+  R_max / R_equal-area is reported as 1.00 with tension and 1.19 without (its check: below 1.2
+  and at least 0.1 under the tensionless one);
+* grid convergence (validate.rs:1782-1822): the dam break at a = 30, 60 and 120 cells in tanks
+  5a + 40 by 2a + 40 (plus the walls): the reference reports Z(3.33) = 4.417 / 4.501 / 4.508
+  (its checks: the change on refinement shrinks, the last under 2 % of Z; the finest front speed
+  dZ/dT between 1.14 and 1.74, the spread of published experiments);
 * viscosity, analytical (validate.rs:1120-1236; zero gravity, nu = 0.01 m^2/s): a free slab
   with u = A cos(pi y / H) decays at nu (pi / H)^2 (reported -3.9 %; its check: within 5 %), a
   slab on a no-slip floor with u = A sin(pi y / 2H) at nu (pi / 2H)^2 (reported -1.8 %; within
@@ -756,4 +763,62 @@ def test_stirred_pool_settles_and_its_jitter_decays():
         assert j_last < 0.5 * rows[0][2] and share < 0.02
     finally:
         sim.close()
+
+
+def test_oil_square_in_water_rounds_off():
+    def rounded(sigma_oil):
+        params = capillary_params()
+        params.pair_tension[0][1] = sigma_oil
+        params.pair_tension[1][0] = sigma_oil
+        sq = lambda x: (np.abs(x[:, 0] - 256.0) < 15.0) & (np.abs(x[:, 1] - 60.0) < 15.0)
+        scene, ps = particles_where((0.0, 0.0, 512.0, 200.0), lambda x: np.ones(len(x), bool), still, still4)
+        sel = sq(ps["pos"])
+        ps[sel] = sc.new_particles(N.ROW_OIL, ps["pos"][sel].copy())
+        sim = world_from(params, scene, ps)
+        try:
+            sim.advance_blocking(0.5)
+            p = sim.download_particles()
+            x = p["pos"][row_of(p) == N.ROW_OIL]
+            c = x.mean(0)
+            rmax = percentile(np.hypot(x[:, 0] - c[0], x[:, 1] - c[1]), 0.995)
+            return rmax / np.sqrt(len(x) / N.PPC / np.pi)
+        finally:
+            sim.close()
+
+    with_t = rounded(N.default_params().pair_tension[0][1])
+    without = rounded(0.0)
+    print(f"\noil square in water: R_max / R_equal-area {with_t:.3f} with tension, {without:.3f} without "
+          f"(reference 1.00, 1.19)", flush=True)
+    assert with_t < without - 0.1 and with_t < 1.2
+    assert with_t == pytest.approx(1.00, abs=0.03) and without == pytest.approx(1.19, abs=0.03)
+
+
+def test_dam_break_converges_on_refinement():
+    fronts = []
+    for a in (30.0, 60.0, 120.0):
+        grid = (int(WALL + 5.0 * a + 40.0), int(WALL + 2.0 * a + 40.0))
+        params = N.default_params()
+        scene = sc.Scene(grid)
+        ps = sc.fill_rect(scene, (0.0, 0.0, WALL + a, WALL + 2.0 * a), N.ROW_WATER)
+        sim = world_from(params, scene, ps, grid)
+        try:
+            g = params.gravity / params.cell_size
+            line, t = [], 0.0
+            for tt in (1.19, 1.98, 2.65, 3.33):
+                target = tt / np.sqrt(2.0 * g / a)
+                sim.advance_blocking(target - t)
+                t = target
+                line.append((percentile(sim.download_particles()["pos"][:, 0], 0.999) - WALL) / a)
+            fronts.append(line)
+        finally:
+            sim.close()
+    d1 = abs(fronts[1][3] - fronts[0][3])
+    d2 = abs(fronts[2][3] - fronts[1][3])
+    speed = (fronts[2][3] - fronts[2][1]) / (3.33 - 1.98)
+    print(f"\ndam break convergence: Z(3.33) {' / '.join(f'{f[3]:.3f}' for f in fronts)} (reference 4.417 / 4.501 / "
+          f"4.508), changes {d1:.3f}, {d2:.3f}; finest dZ/dT {speed:.2f}", flush=True)
+    assert d2 < max(d1, 0.02) and d2 < 0.02 * fronts[2][3]
+    assert 1.14 <= speed <= 1.74
+    for f, want in zip(fronts, (4.417, 4.501, 4.508)):
+        assert f[3] == pytest.approx(want, rel=0.02)
 
