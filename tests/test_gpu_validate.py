@@ -52,6 +52,10 @@ reference measures them and held to the values the reference reports for its own
   mode at Rayleigh's period (+0.6 %; within 10 %); the same oil drop inside water at 2.220 s,
   +6.3 % over the viscous normal mode's 2.087 s (within 8 %); a honey blob in water, miscible,
   keeps its shape (ellipticity unchanged within 0.01; reported 0.000);
+* wetting (validate.rs:1480-1562): a half disc of water (R = 30) on the floor, 4 s, with the
+  contact angle of water, honey and oil set to 60, 90 and 120 degrees; the angle read from the
+  cap's area and apex height is reported as 77 / 89 / 144 (its checks: 90 within 5 degrees,
+  the three rising at least 8 degrees apart);
 * beading (validate.rs:1564-1600): a 30-cell square of oil inside water, zero gravity, 0.5 s:
   R_max / R_equal-area is reported as 1.00 with tension and 1.19 without (its check: below 1.2
   and at least 0.1 under the tensionless one);
@@ -821,4 +825,41 @@ def test_dam_break_converges_on_refinement():
     assert 1.14 <= speed <= 1.74
     for f, want in zip(fronts, (4.417, 4.501, 4.508)):
         assert f[3] == pytest.approx(want, rel=0.02)
+
+
+def test_drop_on_the_floor_settles_at_its_contact_angle():
+    rad = 30.0
+    base = (256.0, WALL)
+    ci = int(base[0])
+    angles = []
+    for angle in (60.0, 90.0, 120.0):
+        params = capillary_params()
+        for row in (N.ROW_WATER, N.ROW_HONEY, N.ROW_OIL):
+            params.mat[row].contact_angle = angle
+        half = lambda x: np.hypot(x[:, 0] - base[0], x[:, 1] - base[1]) < rad
+        scene, ps = particles_where((base[0] - rad, WALL, base[0] + rad, WALL + rad), half, still, still4)
+        sim = world_from(params, scene, ps)
+        try:
+            sim.advance_blocking(4.0)
+            p = sim.download_particles()
+        finally:
+            sim.close()
+        x = p["pos"]
+        body = int(np.sum(np.hypot(x[:, 0] - base[0], x[:, 1] - base[1]) < 2.0 * rad))
+        area = body / N.PPC
+        col = x[:, 0].astype(np.int64)
+        apex = int(np.sum((col >= ci - 2) & (col <= ci + 2))) / (5.0 * N.PPC)
+        shape = lambda t: (t - np.sin(t) * np.cos(t)) / (1.0 - np.cos(t)) ** 2
+        a, b = 0.05, 3.1
+        for _ in range(60):
+            mid = 0.5 * (a + b)
+            if shape(mid) > area / (apex * apex):
+                a = mid
+            else:
+                b = mid
+        angles.append(float(np.degrees(0.5 * (a + b))))
+    print(f"\ncontact angles set 60 / 90 / 120: {' / '.join(f'{a:.1f}' for a in angles)} deg from cap area and apex "
+          f"(reference 77 / 89 / 144)", flush=True)
+    assert abs(angles[1] - 90.0) < 5.0
+    assert angles[1] - angles[0] > 8.0 and angles[2] - angles[1] > 8.0
 
