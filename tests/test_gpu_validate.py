@@ -25,6 +25,9 @@ reference measures them and held to the values the reference reports for its own
   particle of 0.0005 cells^2/s^2, a mean height drift of +0.017 cells and no unconverged solve
   of 486 substeps (its checks: peak below 0.05, rms below 0.001 sqrt(g h), drift below 0.05,
   total energy rise between samples below 2e-4 of the pool's potential energy);
+* shallow pool open to the air (validate.rs:1700-1725): water 34.6 cells deep wall to wall,
+  its level halfway up a row of cells, 24 s: the mean particle speed every 4 s stays under 0.15
+  cells/s (it grew to 1.7 when the surface row pumped);
 * viscous gravity current (validate.rs:346-389): a honey column 50 wide and 60 high slumping
   over the no-slip floor follows Huppert's (1982) x = 1.411 (g q^3 / 3 nu)^(1/5) t^(1/5); the
   reference reports the front at 16 s as 1.02 of that and the spreading exponent between 4 and
@@ -862,4 +865,18 @@ def test_drop_on_the_floor_settles_at_its_contact_angle():
           f"(reference 77 / 89 / 144)", flush=True)
     assert abs(angles[1] - 90.0) < 5.0
     assert angles[1] - angles[0] > 8.0 and angles[2] - angles[1] > 8.0
+
+
+def test_shallow_pool_open_to_the_air_stays_still():
+    sim, _ = world((0.0, 0.0, 512.0, 0.12 * 288.0))
+    try:
+        speeds = []
+        for _ in range(6):
+            sim.advance_blocking(4.0)
+            v = sim.download_particles()["vel"].astype(np.float32)
+            speeds.append(float(np.mean(np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]))))
+        print(f"\nshallow pool: mean particle speed every 4 s {' '.join(f'{v:.3f}' for v in speeds)} cells/s", flush=True)
+        assert max(speeds) < 0.15
+    finally:
+        sim.close()
 
