@@ -48,6 +48,10 @@ reference measures them and held to the values the reference reports for its own
   4 x 4-cell mean) 8.4 -> 3.0 cells/s (its checks: the 2 s means fall, no rise beyond the
   density correction's lift g x 0.016 cells, each jitter sample below the one 4 s before, the
   last under half the first and under 2 % of the energy);
+* a brush held against the floor (validate.rs:838-862): Push (220 cells/s, strength 2.5 x 40,
+  radius 13) circling 12 cells above the floor of a pool 50 cells deep for 120 frames of four
+  ticks: the fullest cell stays under 64 particles (rest 4; the reference reports 37 since
+  its crowd limit, 815 before) and, released for 2 s, under 16;
 * surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
   radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
   currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
@@ -877,6 +881,34 @@ def test_shallow_pool_open_to_the_air_stays_still():
             speeds.append(float(np.mean(np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]))))
         print(f"\nshallow pool: mean particle speed every 4 s {' '.join(f'{v:.3f}' for v in speeds)} cells/s", flush=True)
         assert max(speeds) < 0.15
+    finally:
+        sim.close()
+
+
+def fullest(ps):
+    i = ps["pos"][:, 0].astype(np.int64)
+    j = ps["pos"][:, 1].astype(np.int64)
+    return int(np.bincount(j * GRID[0] + i, minlength=GRID[0] * GRID[1]).max())
+
+
+def test_push_held_against_the_floor_does_not_pack_liquid():
+    sim, _ = world((0.0, 0.0, 512.0, WALL + 50.0))
+    try:
+        worst = 0
+        for k in range(120):
+            a = k * 0.2
+            tool = N.ToolState()
+            tool.pos[0], tool.pos[1] = 256.0 + 6.0 * np.cos(a), WALL + 12.0 + 4.0 * np.sin(a)
+            tool.vel[0], tool.vel[1] = 220.0, 0.0
+            tool.radius, tool.mode, tool.strength = 13.0, 3, 40.0 * 2.5
+            sim.step_blocking(4, tool)
+            if k % 20 == 19:
+                worst = max(worst, fullest(sim.download_particles()))
+        sim.advance_blocking(2.0)
+        after = fullest(sim.download_particles())
+        print(f"\npush against the floor: fullest cell up to {worst} (reference 37), {after} after 2 s released", flush=True)
+        assert worst < 64
+        assert after < 16
     finally:
         sim.close()
 
