@@ -52,6 +52,9 @@ reference measures them and held to the values the reference reports for its own
   radius 13) circling 12 cells above the floor of a pool 50 cells deep for 120 frames of four
   ticks: the fullest cell stays under 64 particles (rest 4; the reference reports 37 since
   its crowd limit, 815 before) and, released for 2 s, under 16;
+* a clump inside ice (validate.rs:864-893): a block of water at -30 deg C (frozen) with 400
+  more particles stacked on one spot inside it, 3 s: the clump spreads out (fullest cell under
+  32) and over 95 % of the block stays frozen;
 * surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
   radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
   currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
@@ -909,6 +912,32 @@ def test_push_held_against_the_floor_does_not_pack_liquid():
         print(f"\npush against the floor: fullest cell up to {worst} (reference 37), {after} after 2 s released", flush=True)
         assert worst < 64
         assert after < 16
+    finally:
+        sim.close()
+
+
+def test_clump_inside_ice_spreads_out():
+    params = N.default_params()
+    scene = sc.Scene(GRID)
+    block = sc.fill_rect(scene, (200.0, 0.0, 312.0, WALL + 40.0), N.ROW_WATER)
+    k = np.arange(400)
+    stack = np.stack([256.0 + 0.001 * (k % 20), WALL + 20.0 + 0.001 * (k // 20)], 1).astype(np.float32)
+    ps = np.concatenate([block, sc.new_particles(N.ROW_WATER, stack)])
+    m = params.mat[N.ROW_WATER]
+    lo, hi, lat = m.melt_lo, m.melt_hi, m.latent / m.heat_capacity
+    assert lo > -273.0 and -30.0 <= lo
+    ps["t"] = -30.0 - lat  # params.rs:844-854: below the solidus, less the latent heat as degrees
+    sim = world_from(params, scene, ps)
+    try:
+        n0 = fullest(sim.download_particles())
+        sim.advance_blocking(3.0)
+        p = sim.download_particles()
+        # the frozen share of a particle passes one half halfway through its melting enthalpy
+        frozen = float(np.mean(p["t"] < (lo - lat) + 0.5 * (hi - lo + lat)))
+        n1 = fullest(p)
+        print(f"\nclump in ice: fullest cell {n0} -> {n1} after 3 s, {frozen * 100:.0f} % frozen", flush=True)
+        assert n1 < 32
+        assert frozen > 0.95
     finally:
         sim.close()
 
