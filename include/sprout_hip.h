@@ -90,8 +90,10 @@ int sp_wait(sp_sim* sim);
  * blocks. */
 int sp_poll(sp_sim* sim, sp_solver_stats* out);
 
-/* `FluidSim::advance_blocking(seconds)` (src/sim/mod.rs:3284-3312). */
-int sp_advance_blocking(sp_sim* sim, double seconds);
+/* `FluidSim::advance_blocking(seconds, tool)` (src/sim/mod.rs:3284-3312): records of at most
+ * 16 substeps until `seconds` are covered; `tool` (NULL = the idle tool) acts in the first
+ * record, the later ones carry it idle. */
+int sp_advance_blocking(sp_sim* sim, double seconds, const sp_tool_state* tool);
 /* `FluidSim::step_blocking(n, tool)` (src/sim/mod.rs:3316-3338): n ticks of 1/240 s in
  * records of at most 16 substeps, each under `tool` (NULL = the idle tool); n = 0 records no
  * substep and uploads what changed. */

@@ -297,7 +297,7 @@ SIGNATURES = {
     "sp_record": ([_P, C.c_uint32, C.c_float, C.POINTER(ToolState)], C.c_int),
     "sp_wait": ([_P], C.c_int),
     "sp_poll": ([_P, C.POINTER(SolverStats)], C.c_int),
-    "sp_advance_blocking": ([_P, C.c_double], C.c_int),
+    "sp_advance_blocking": ([_P, C.c_double, C.POINTER(ToolState)], C.c_int),
     "sp_step_blocking": ([_P, C.c_uint32, C.POINTER(ToolState)], C.c_int),
     "sp_sim_time": ([_P, C.POINTER(C.c_double)], C.c_int),
     "sp_frame_blocking": ([_P, C.c_uint32, C.c_float, C.POINTER(ToolState), C.POINTER(C.c_double)], C.c_int),

@@ -954,9 +954,20 @@ int sp_step_blocking(sp_sim* s, uint32_t n, const sp_tool_state* tool) {
     });
 }
 
-int sp_advance_blocking(sp_sim* s, double seconds) {
+int sp_advance_blocking(sp_sim* s, double seconds, const sp_tool_state* tool) {
     return guarded(s, [&]() -> int {
-        // advance_blocking (mod.rs:3284-3312)
+        if (tool && (tool->mode < 0 || tool->mode > 8)) {
+            set_err(s, "unknown tool mode %d", tool->mode);
+            return SP_E_INVALID;
+        }
+        // advance_blocking (mod.rs:3284-3312): the tool acts in the first record only, the
+        // later ones carry it idle (mode 0)
+        sp_tool_state idle{};
+        if (tool) {
+            idle = *tool;
+            idle.mode = 0;
+        }
+        bool first = true;
         double goal = s->sim_time + seconds;
         double sliver = std::max((double)s->params.min_substep_share, 1e-6);
         for (int guard = 0; guard < 1000000; guard++) {
@@ -966,8 +977,9 @@ int sp_advance_blocking(sp_sim* s, double seconds) {
             double nn = s->params.fixed_hz > 0.0f ? std::max(round(left / dt), 1.0) : ceil(left / dt);
             uint32_t k = (uint32_t)std::min(std::max(nn, 1.0), 16.0);
             double target = s->params.fixed_hz > 0.0f ? k * dt : left;
-            int rc = record(s, k, (float)target);
+            int rc = record(s, k, (float)target, tool ? (first ? tool : &idle) : nullptr);
             if (rc != SP_OK) return rc;
+            first = false;
             rc = wait(s);
             if (rc != SP_OK) return rc;
             if (s->params.fixed_hz > 0.0f && (goal - s->sim_time) < 0.5 * dt) break;

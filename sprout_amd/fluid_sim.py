@@ -111,9 +111,10 @@ class FluidSim:
         t = C.byref(tool) if tool is not None else None
         self._check(self._lib.sp_step_blocking(self._h, substeps, t), "step_blocking")
 
-    def advance_blocking(self, seconds):
-        """FluidSim::advance_blocking (mod.rs:3284-3312)."""
-        self._check(self._lib.sp_advance_blocking(self._h, float(seconds)), "advance_blocking")
+    def advance_blocking(self, seconds, tool=None):
+        """FluidSim::advance_blocking (mod.rs:3284-3312): `tool` acts in the first record."""
+        t = C.byref(tool) if tool is not None else None
+        self._check(self._lib.sp_advance_blocking(self._h, float(seconds), t), "advance_blocking")
 
     def sim_time(self):
         t = C.c_double()

@@ -55,6 +55,10 @@ reference measures them and held to the values the reference reports for its own
 * a clump inside ice (validate.rs:864-893): a block of water at -30 deg C (frozen) with 400
   more particles stacked on one spot inside it, 3 s: the clump spreads out (fullest cell under
   32) and over 95 % of the block stays frozen;
+* a block of ice in water (checks.rs:112-160; 256 x 144, heat_rate 1): a 40 x 20 block at
+  -10 deg C dropped two cells above where it floats into water at 20 deg C keeps its shape
+  within 2 cells and floats with 1 - rho_ice / rho_water of its height above the water, within
+  4 % (reported 10.7 %); 2 s under the Heat brush melt all but a third of it;
 * surface tension (validate.rs:1270-1480; 1 mm cells, zero gravity): a resting water drop of
   radius 30 cells holds sigma / R inside (reported 1.002; its check within 10 %) with spurious
   currents of 1.2 % of sqrt(sigma / rho R) (below 2 %); an oil disc in water holds a jump of
@@ -938,6 +942,59 @@ def test_clump_inside_ice_spreads_out():
         print(f"\nclump in ice: fullest cell {n0} -> {n1} after 3 s, {frozen * 100:.0f} % frozen", flush=True)
         assert n1 < 32
         assert frozen > 0.95
+    finally:
+        sim.close()
+
+
+def test_ice_block_floats_keeps_its_shape_and_melts_under_heat():
+    grid = (256, 144)
+    params = N.default_params()
+    params.heat_rate = 1.0
+    scene = sc.Scene(grid)
+    ps = sc.fill_rect(scene, (0.0, 0.0, 256.0, 60.0), N.ROW_WATER)
+    y0 = 44.0
+    block = sc.fill_rect(scene, (108.0, y0, 148.0, y0 + 20.0), N.ROW_WATER)
+    x = ps["pos"]
+    ps = ps[~((x[:, 0] > 108.0) & (x[:, 0] < 148.0) & (x[:, 1] > y0))]
+    m = params.mat[N.ROW_WATER]
+    lo, hi, lat = m.melt_lo, m.melt_hi, m.latent / m.heat_capacity
+    block["t"] = -10.0 - lat  # params.rs:844-854: below the solidus, less the latent heat
+    n_block = len(block)
+    sim = world_from(params, scene, np.concatenate([ps, block]), grid)
+
+    def frozen(p):
+        return p["t"] < (lo - lat) + 0.5 * (hi - lo + lat)
+
+    try:
+        freeboard = 0.0
+        for k in range(60):
+            sim.advance_blocking(0.1)
+            if k >= 10:
+                p = sim.download_particles()
+                f = frozen(p)
+                top = percentile(p["pos"][f, 1], 0.99)
+                far = ~f & ((p["pos"][:, 0] < 70.0) | (p["pos"][:, 0] > 186.0))
+                freeboard += (top - percentile(p["pos"][far, 1], 0.99)) / 50.0
+        p = sim.download_particles()
+        f = frozen(p)
+        xs, ys = p["pos"][f, 0], p["pos"][f, 1]
+        wid = percentile(xs, 0.99) - percentile(xs, 0.01)
+        hei = percentile(ys, 0.99) - percentile(ys, 0.01)
+        above = freeboard / 20.0
+        expect = 1.0 - m.density_solid / m.density
+        held = int(f.sum())
+        tool = N.ToolState()
+        tool.pos[0], tool.pos[1] = 128.0, float(np.mean(ys.astype(np.float64)))
+        tool.radius, tool.mode = 30.0, 5
+        for _ in range(120):
+            sim.advance_blocking(1.0 / 60.0, tool)
+        left = int(frozen(sim.download_particles()).sum())
+        print(f"\nice block: {wid:.1f} x {hei:.1f} cells (was 40 x 20), {held} of {n_block} frozen; {above * 100:.1f} % above "
+              f"the water (1 - rho_ice / rho_water {expect * 100:.1f} %, reference 10.7 %); heat brush 2 s: {held} -> {left} "
+              f"frozen", flush=True)
+        assert abs(wid - 40.0) < 2.0 and abs(hei - 20.0) < 2.0
+        assert abs(above - expect) < 0.04
+        assert left * 3 < held
     finally:
         sim.close()
 
